@@ -1,0 +1,96 @@
+"""Writes the golden fixtures under tests/golden/ from the reference library.
+
+Run in the build container only (needs /root/reference and the reference
+build made by `make -f oracle/ref.mk`):
+
+    python tests/golden/make_golden.py
+
+It drives oracle/_ref/gen_golden (tests/golden/gen_golden.cpp linked against
+the reference e8qlib) on the seeded inputs of cases.py and keeps its outputs
+as .npy/.e8ax fixtures plus manifest.json.
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import cases  # noqa: E402
+
+GEN = os.path.join(HERE, "..", "..", "oracle", "_ref", "gen_golden")
+
+
+def run(*args):
+    subprocess.run([GEN] + [str(a) for a in args], check=True)
+
+
+def main():
+    if not os.path.exists(GEN):
+        sys.exit("build the reference first: make -f oracle/ref.mk")
+    out = HERE
+    manifest = {"index": {}, "rotate": [], "inputs": {}}
+    with tempfile.TemporaryDirectory() as tmp:
+        t = lambda n: os.path.join(tmp, n)  # noqa: E731
+        # Level tables, packed per rate: for b = 1..13, branch 0 then branch 1.
+        run("levels", t("lev.npy"), t("scale.npy"), t("wcls.npy"))
+        lev = np.load(t("lev.npy"))
+        flat = np.concatenate([lev[b - 1, br, : 1 << b] for b in range(1, 14) for br in (0, 1)])
+        np.save(os.path.join(out, "levels.npy"), flat)
+        np.save(os.path.join(out, "fixed_scale.npy"), np.load(t("scale.npy")))
+        np.save(os.path.join(out, "class_means.npy"), np.load(t("wcls.npy")))
+        for dim, seed in cases.ROTATE_CASES:
+            x = cases.rotate_input(dim)
+            np.save(t("x.npy"), x)
+            run("rotate", dim, seed, t("x.npy"), t("s.npy"), t("r.npy"))
+            np.savez(os.path.join(out, f"rotate_{dim}.npz"), signs=np.load(t("s.npy")), out=np.load(t("r.npy")))
+            manifest["rotate"].append([dim, seed])
+        u = cases.encode_input()
+        np.save(t("u.npy"), u)
+        enc = {}
+        for b in range(1, 14):
+            run("encode", b, cases.ENCODE_PADDED, t("u.npy"), t("d.npy"), t("b.npy"))
+            enc[f"digits_b{b}"] = np.load(t("d.npy"))
+            enc[f"branch_b{b}"] = np.load(t("b.npy"))
+        np.savez(os.path.join(out, "encode.npz"), **enc)
+        manifest["inputs"]["encode"] = cases.digest(u)
+        acc = {}
+        for blocks in cases.ACCUM_BLOCKS:
+            pay, lut = cases.accum_input(blocks)
+            np.save(t("p.npy"), pay)
+            np.save(t("l.npy"), lut)
+            run("accum", t("p.npy"), t("l.npy"), t("o1.npy"), t("o2.npy"))
+            acc[f"single_{blocks}"] = np.load(t("o1.npy"))
+            acc[f"pair_{blocks}"] = np.load(t("o2.npy"))
+        np.savez(os.path.join(out, "accum.npz"), **acc)
+        q, c = cases.route_input()
+        np.save(t("q.npy"), q)
+        np.save(t("c.npy"), c)
+        run("route", t("q.npy"), t("c.npy"), 20, t("rd.npy"), t("ri.npy"), t("rn.npy"))
+        np.savez(os.path.join(out, "route.npz"), dist=np.load(t("rd.npy")), ids=np.load(t("ri.npy")),
+                 cn2=np.load(t("rn.npy")))
+        for name, (n, dim, nlist, bits, seed, alpha, _e, _nq) in cases.INDEX_CASES.items():
+            data, cent, lab, queries = cases.mixture(name)
+            for fn, arr in (("data", data), ("cent", cent), ("lab", lab), ("q", queries)):
+                np.save(t(fn + ".npy"), arr)
+            idx_path = os.path.join(out, f"{name}.e8ax")
+            run("build", t("data.npy"), t("cent.npy"), t("lab.npy"), bits, seed, alpha, idx_path)
+            res = {}
+            for (k, nprobe, dis) in cases.search_grid(name):
+                pre = t("s")
+                run("search", idx_path, t("q.npy"), k, nprobe, dis, pre)
+                tag = f"k{k}_p{nprobe}_d{dis}"
+                for part in ("ids", "dists", "stats", "rotq", "lut", "consts", "probe", "pdist"):
+                    res[f"{tag}_{part}"] = np.load(f"{pre}_{part}.npy")
+            res["queries"] = queries
+            np.savez(os.path.join(out, f"{name}_search.npz"), **res)
+            manifest["index"][name] = cases.digest(data, cent, lab)
+    with open(os.path.join(out, "manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
