@@ -1,0 +1,478 @@
+// e8q_encode.hip -- the build (encode) path on the device.
+//
+// Reference: E8qIndex::Build (e8q_build.cpp:87-503) and the E8 codec
+// (e8q_codec.cpp:297-726).  Every decision the encoder makes is in fp64 and
+// every fit is the reference's sequential fp64 FMA chain, so digits,
+// branches, packed bytes, g1/h3 and the u8 banks come out bit-identical.
+//
+// Kernels:
+//   MeanKernel        fp64 column sums in row order (one chain per dim)
+//   MeanFinishKernel  mean -> float -> rotated mean
+//   CentroidKernel    rotate, center, fp64 ||c||^2
+//   EncodeRowsKernel  one 64-lane workgroup per row: rotate, residual norm,
+//                     unit, E8 nearest-lattice-point encode (one lane per
+//                     8-dim block), 7 fit chains (one lane each), pack
+//   BanksKernel       per 32-vector batch u8 banks (e8q_build.cpp:33-80)
+#include "e8q_device.h"
+#include "e8q_kernels.h"
+
+namespace e8qdev {
+
+// ---------------------------------------------------------------------------
+// Mean: fp64 sum per dimension in row order (e8q_build.cpp:190-204).  A
+// workgroup owns 64 dims; all 256 threads stage 64-row tiles through LDS so
+// the loads stay coalesced, and 64 threads run the dependent adds.
+__global__ __launch_bounds__(256) void MeanKernel(const float* __restrict__ base, uint64_t n, uint32_t dim,
+                                                  double* __restrict__ mu) {
+    __shared__ float tile[64][65];
+    const uint32_t d0 = blockIdx.x * 64;
+    double acc = 0.0;
+    for (uint64_t r0 = 0; r0 < n; r0 += 64) {
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int rr = e >> 6, dd = e & 63;
+            const uint64_t r = r0 + rr;
+            tile[rr][dd] = (r < n && d0 + dd < dim) ? base[r * dim + d0 + dd] : 0.0F;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const uint64_t rn = min<uint64_t>(64, n - r0);
+            for (uint64_t rr = 0; rr < rn; ++rr) acc += static_cast<double>(tile[rr][threadIdx.x]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 64 && d0 + threadIdx.x < dim) mu[d0 + threadIdx.x] = acc;
+}
+
+__global__ void MeanFinishKernel(const double* __restrict__ mu, uint64_t n, IndexView ix, float* __restrict__ mean_rot) {
+    extern __shared__ float buf[];
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x)
+        buf[i] = i < ix.dim ? static_cast<float>(mu[i] / static_cast<double>(n)) : 0.0F;
+    __syncthreads();
+    RotateInLds(buf, ix);
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) mean_rot[i] = buf[i];
+}
+
+// Rotated, centered centroids and their fp64 squared norms (e8q_build.cpp:249-272).
+__global__ void CentroidKernel(const float* __restrict__ cent, IndexView ix, float* __restrict__ crot,
+                               float* __restrict__ cn2) {
+    extern __shared__ float buf[];
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    for (uint64_t c = blockIdx.x; c < ix.nlist; c += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < ix.dim ? cent[c * ix.dim + i] : 0.0F;
+        __syncthreads();
+        RotateInLds(buf, ix);
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+            buf[i] = buf[i] - ix.mean_rot[i];
+            crot[c * P + i] = buf[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (uint32_t i = 0; i < P; ++i) {
+                const double v = static_cast<double>(buf[i]);
+                s = fma(v, v, s);
+            }
+            cn2[c] = static_cast<float>(s);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// E8 encoder, one 8-dim block.
+
+struct Sel {
+    uint16_t d[8];
+    double cost;
+};
+
+// Nearest level per (branch, parity) class from one insertion point into the
+// merged sorted table (e8q_codec.cpp:387-414).  r: 0 = br0 even, 1 = br1
+// even, 2 = br0 odd, 3 = br1 odd.
+__device__ __forceinline__ void NearestMerged(double v, const EncodeView& ev, uint16_t dig[4], double cost[4]) {
+    const double* lev = ev.mlev;
+    const long n = ev.mlen;
+    long lo = 0, len = n;   // lower_bound: first index with lev >= v
+    while (len > 0) {
+        const long half = len >> 1;
+        if (lev[lo + half] < v) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    const long p = lo, p4 = p & 3;
+    for (long r = 0; r < 4; ++r) {
+        const long ib = p - 1 - ((p4 + 3 - r) & 3);
+        const long ia = p + ((r + 4 - p4) & 3);
+        const double lb = ib >= 0 ? lev[ib] : -__builtin_inf();
+        const double la = ia < n ? lev[ia] : __builtin_inf();
+        const uint16_t db = ib >= 0 ? ev.mdig[ib] : 0;
+        const uint16_t da = ia < n ? ev.mdig[ia] : 0;
+        const double cb = (v - lb) * (v - lb);
+        const double ca = (la - v) * (la - v);
+        const bool take_b = (cb < ca) || (cb == ca && db < da);
+        dig[r] = take_b ? db : da;
+        cost[r] = take_b ? cb : ca;
+    }
+}
+
+// Single-class search (the reference's four-table path, used when the merged
+// interleave does not hold, i.e. b == 1): e8q_codec.cpp:233-256.
+__device__ __forceinline__ void NearestClass(double v, const double* lev, const uint16_t* dg, int n, uint16_t* dig,
+                                             double* cost) {
+    int lo = 0, len = n;
+    while (len > 0) {
+        const int half = len >> 1;
+        if (lev[lo + half] < v) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    const int pos = lo;
+    int best = pos < n ? pos : n - 1;
+    if (pos > 0) {
+        const double dc = fabs(v - lev[pos - 1]);
+        const double dbv = fabs(v - lev[best]);
+        if ((dc < dbv) || (dc == dbv && dg[pos - 1] < dg[best])) best = pos - 1;
+    }
+    *dig = dg[best];
+    const double dv = v - lev[best];
+    *cost = dv * dv;
+}
+
+__device__ __forceinline__ Sel Assemble(const uint16_t de[8], const double ce[8], const uint16_t dO[8],
+                                        const double co[8]) {
+    Sel s;
+    s.cost = 0.0;
+    int par = 0;
+    for (int i = 0; i < 8; ++i) {
+        const bool odd = co[i] < ce[i];
+        s.d[i] = odd ? dO[i] : de[i];
+        s.cost += odd ? co[i] : ce[i];
+        par ^= odd ? 1 : 0;
+    }
+    if (par != 0) {
+        int sel = 0;
+        double best = 1.7976931348623157e308;
+        for (int i = 0; i < 8; ++i) {
+            const bool was_odd = s.d[i] == dO[i];
+            const double extra = was_odd ? ce[i] - co[i] : co[i] - ce[i];
+            if (extra < best) {
+                best = extra;
+                sel = i;
+            }
+        }
+        s.d[sel] = (s.d[sel] == dO[sel]) ? de[sel] : dO[sel];
+        s.cost += fmax(best, 0.0);
+    }
+    return s;
+}
+
+// Encodes one block of 8 unit-vector coordinates (EncodeVectorAtScaleInto,
+// e8q_codec.cpp:671-726).  sr = s * sqrt(padded).
+__device__ void EncodeBlock(const float* u8, double sr, const EncodeView& ev, uint16_t out[8], int* branch) {
+    uint16_t de[2][8], dO[2][8];
+    double ce[2][8], co[2][8];
+    const int half = 1 << (ev.bits - 1);
+    for (int i = 0; i < 8; ++i) {
+        const double v = static_cast<double>(u8[i]) * sr;
+        if (ev.merged) {
+            uint16_t dg[4];
+            double cs[4];
+            NearestMerged(v, ev, dg, cs);
+            de[0][i] = dg[0]; ce[0][i] = cs[0];
+            de[1][i] = dg[1]; ce[1][i] = cs[1];
+            dO[0][i] = dg[2]; co[0][i] = cs[2];
+            dO[1][i] = dg[3]; co[1][i] = cs[3];
+        } else {
+            for (int br = 0; br < 2; ++br) {
+                NearestClass(v, ev.cls_lev + (br * 2 + 0) * half, ev.cls_dig + (br * 2 + 0) * half, half, &de[br][i],
+                             &ce[br][i]);
+                NearestClass(v, ev.cls_lev + (br * 2 + 1) * half, ev.cls_dig + (br * 2 + 1) * half, half, &dO[br][i],
+                             &co[br][i]);
+            }
+        }
+    }
+    const Sel e0 = Assemble(de[0], ce[0], dO[0], co[0]);
+    const Sel e1 = Assemble(de[1], ce[1], dO[1], co[1]);
+    const bool one = e1.cost < e0.cost;
+    for (int i = 0; i < 8; ++i) out[i] = one ? e1.d[i] : e0.d[i];
+    *branch = one ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Per-row encode.  Workgroup = one wave; rows in a grid-stride loop.
+constexpr int kEncThreads = 64;
+
+__global__ __launch_bounds__(kEncThreads) void EncodeRowsKernel(IndexView ix, EncodeView ev, EncodeArgs a) {
+    extern __shared__ float sm[];
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    float* xb = sm;                                            // rotated row, then unit
+    float* rc = sm + P;                                        // rotated centroid
+    uint16_t* dg = reinterpret_cast<uint16_t*>(sm + 2 * P);    // digits
+    uint8_t* brs = reinterpret_cast<uint8_t*>(dg + P);         // branches (blocks)
+    __shared__ double s_fit[8];
+    __shared__ double s_rn2;
+    const int tid = threadIdx.x;
+    const int bits = ix.bits;
+    const int cbytes = bits - 1;
+    const double sr = ev.scale * sqrt(static_cast<double>(P));
+    const double inv_ls = 1.0 / sqrt(static_cast<double>(P));
+    const uint32_t Q = 1u << bits;
+    for (uint64_t r = blockIdx.x; r < a.n; r += gridDim.x) {
+        const uint32_t c = a.cluster_ids[r];
+        const uint32_t slot = a.slot[r];
+        const uint64_t rank = slot - ix.cluster_begin[c];
+        const uint32_t lane = static_cast<uint32_t>(rank & 31);
+        uint8_t* rec = a.arena + ix.arena_begin[c] + (rank >> 5) * ix.batch_bytes;
+        uint8_t* rrec = a.ref_arena + ix.ref_arena_begin[c] + (rank >> 5) * ix.ref_batch_bytes;
+        for (uint32_t i = tid; i < P; i += kEncThreads) xb[i] = i < ix.dim ? a.base[r * ix.dim + i] : 0.0F;
+        __syncthreads();
+        RotateInLds(xb, ix);
+        for (uint32_t i = tid; i < P; i += kEncThreads) {
+            xb[i] = xb[i] - ix.mean_rot[i];
+            rc[i] = ix.centroid_rot[static_cast<uint64_t>(c) * P + i];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double rn2 = 0.0;
+            for (uint32_t d = 0; d < P; ++d) {
+                const double rv = static_cast<double>(xb[d]) - static_cast<double>(rc[d]);
+                rn2 = fma(rv, rv, rn2);
+            }
+            s_rn2 = rn2;
+            a.ids[slot] = static_cast<uint32_t>(r);
+        }
+        __syncthreads();
+        const double rn2 = s_rn2;
+        const double rn = sqrt(rn2);
+        if (rn < 1e-20) {   // point on its centroid: everything stays zero
+            __syncthreads();
+            continue;
+        }
+        const double inv_rn = 1.0 / rn;
+        for (uint32_t d = tid; d < P; d += kEncThreads)
+            xb[d] = static_cast<float>((static_cast<double>(xb[d]) - static_cast<double>(rc[d])) * inv_rn);
+        __syncthreads();
+        for (int blk = tid; blk < ix.blocks; blk += kEncThreads) {
+            uint16_t d8[8];
+            int br;
+            EncodeBlock(xb + blk * 8, sr, ev, d8, &br);
+            for (int i = 0; i < 8; ++i) dg[blk * 8 + i] = d8[i];
+            brs[blk] = static_cast<uint8_t>(br);
+            // rowA byte (e8q_scan.cpp:193-205)
+            uint32_t byte = 0;
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t top = (bits == 1 && i == 7) ? static_cast<uint32_t>(br & 1)
+                                                           : ((d8[i] >> (bits - 1)) & 1u);
+                byte |= top << (7 - i);
+            }
+            rec[kHdr + blk * 32 + lane] = static_cast<uint8_t>(byte);
+            if (bits >= 2) {   // refine field (e8q_codec.cpp:827-837)
+                const int cb = cbytes;
+                const uint64_t m = (uint64_t{1} << cb) - 1;
+                uint64_t lo = 0, hi = 0;
+                for (int i = 0; i < 8; ++i) {
+                    const uint64_t f = i < 7 ? (d8[i] & m) : (((d8[7] & ~1u) & m) | static_cast<uint64_t>(br & 1));
+                    const int pos = cb * i;
+                    if (pos < 64) {
+                        lo |= f << pos;
+                        if (pos + cb > 64) hi |= f >> (64 - pos);
+                    } else {
+                        hi |= f << (pos - 64);
+                    }
+                }
+                uint8_t* out = rrec + lane * ix.refine_stride + blk * cb;
+                for (int b = 0; b < cb; ++b) out[b] = static_cast<uint8_t>(b < 8 ? (lo >> (8 * b)) : (hi >> (8 * (b - 8))));
+            }
+        }
+        __syncthreads();
+        // Seven fit chains, one lane each, in coordinate order (e8q_build.cpp:339-376):
+        // 0 <u,vE> 1 <c,vE> 2 ||vE||^2 3 <vE,w> 4 <u,w> 5 ||w||^2 6 <c,w>.
+        if (tid < 7) {
+            double acc = 0.0;
+            for (int blk = 0; blk < ix.blocks; ++blk) {
+                const uint32_t br = brs[blk];
+                for (int kk = 0; kk < 8; ++kk) {
+                    const uint32_t i = blk * 8 + kk;
+                    const uint32_t digit = dg[i];
+                    const double ve = ev.levels[br * Q + digit] * inv_ls;
+                    const double wc = ix.gate_mid[2 * kk + (digit >> (bits - 1))];
+                    const double u = static_cast<double>(xb[i]);
+                    const double cc = static_cast<double>(rc[i]);
+                    double x, y;
+                    switch (tid) {
+                        case 0: x = u; y = ve; break;
+                        case 1: x = ve; y = cc; break;
+                        case 2: x = ve; y = ve; break;
+                        case 3: x = wc; y = ve; break;
+                        case 4: x = wc; y = u; break;
+                        case 5: x = wc; y = wc; break;
+                        default: x = wc; y = cc; break;
+                    }
+                    // ||w||^2 at the 8th coordinate of a block: separately rounded
+                    // square (the reference build hoists it out of the FMA).
+                    acc = (tid == 5 && kk == 7) ? acc + x * y : fma(x, y, acc);
+                }
+            }
+            s_fit[tid] = acc;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const double due = s_fit[0], dce = s_fit[1], n2e = s_fit[2], dew = s_fit[3], duw = s_fit[4];
+            const double n2w = s_fit[5], dcw = s_fit[6];
+            double g1 = 0.0, h3 = 0.0;
+            if (n2e > 1e-30 && due > 1e-12) {
+                g1 = rn / due;
+                h3 = fma(2.0 * g1, dce, rn2);
+            }
+            float* g1b = reinterpret_cast<float*>(rrec + ix.refine_stride * 32);
+            float* h3b = g1b + 32;
+            g1b[lane] = static_cast<float>(g1);
+            h3b[lane] = static_cast<float>(h3);
+            const double g1v = static_cast<double>(static_cast<float>(g1));
+            const double sqrt_dm1 = sqrt(static_cast<double>(ix.dim > 1 ? ix.dim - 1 : 1));
+            const double n2w_safe = fmax(n2w, 1e-30);
+            const double uw_eps = 1e-12 * sqrt(n2w_safe);
+            double w_use, perp;
+            if (fabs(duw) > uw_eps) {
+                const double wx = 2.0 * rn / duw;
+                const double g4 = g1v * 4.0;
+                const double rx2 = fma(-(wx * g4), dew, fma(g1v * g4, n2e, n2w * (wx * wx)));
+                w_use = wx;
+                perp = sqrt(fmax(0.0, rx2)) / sqrt_dm1;
+            } else {
+                const double gamma = dew / n2w_safe;
+                const double vp2 = fmax(fma(-n2w, gamma * gamma, n2e), 0.0);
+                w_use = 2.0 * g1v * gamma;
+                perp = 2.0 * g1v * sqrt(vp2) / sqrt_dm1;
+            }
+            a.ow[slot] = static_cast<float>(w_use);
+            a.ob[slot] = static_cast<float>(fma(w_use, dcw, rn2));
+            a.op[slot] = static_cast<float>(perp);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Banks: one 32-lane half-wave per batch; workgroup = 8 batches of one
+// cluster.  Grid: x = cluster, loop over its batches.
+__global__ __launch_bounds__(256) void BanksKernel(IndexView ix, const float* __restrict__ ob,
+                                                   const float* __restrict__ ow, const float* __restrict__ op,
+                                                   uint8_t* __restrict__ arena) {
+    const uint64_t c = blockIdx.x;
+    const uint64_t vb = ix.cluster_begin[c];
+    const uint32_t cnt = static_cast<uint32_t>(ix.cluster_begin[c + 1] - vb);
+    const uint32_t nb = (cnt + 31) / 32;
+    const uint32_t t = threadIdx.x & 31;
+    for (uint32_t b = threadIdx.x >> 5; b < nb; b += blockDim.x >> 5) {
+        const uint32_t lo = b * 32;
+        const uint32_t m = min(32u, cnt - lo);
+        const bool live = t < m;
+        const uint64_t s = vb + lo + t;
+        const float vb_ = live ? ob[s] : 0.0F, vw = live ? ow[s] : 0.0F, vp = live ? op[s] : 0.0F;
+        float bmin = live ? vb_ : __builtin_inff(), bmax = live ? vb_ : -__builtin_inff();
+        float wmin = live ? vw : __builtin_inff(), wmax = live ? vw : -__builtin_inff();
+        float pmin = live ? vp : __builtin_inff(), pmax = live ? vp : -__builtin_inff();
+        for (int o = 16; o > 0; o >>= 1) {
+            bmin = fminf(bmin, __shfl_xor(bmin, o, 32));
+            bmax = fmaxf(bmax, __shfl_xor(bmax, o, 32));
+            wmin = fminf(wmin, __shfl_xor(wmin, o, 32));
+            wmax = fmaxf(wmax, __shfl_xor(wmax, o, 32));
+            pmin = fminf(pmin, __shfl_xor(pmin, o, 32));
+            pmax = fmaxf(pmax, __shfl_xor(pmax, o, 32));
+        }
+        const float wamax = fmaxf(fabsf(wmin), fabsf(wmax));
+        const float bsc = bmax > bmin ? (bmax - bmin) / 255.0F : 1.0F;
+        const float wsc = wmax > wmin ? (wmax - wmin) / 255.0F : 1.0F;
+        const float psc = pmax > pmin ? (pmax - pmin) / 255.0F : 1.0F;
+        uint8_t* bp = arena + ix.arena_begin[c] + static_cast<uint64_t>(b) * ix.batch_bytes;
+        uint32_t bq = 0, wq = 0, pq = 0;
+        if (live) {
+            int v = static_cast<int>((vb_ - bmin) / bsc);
+            bq = static_cast<uint32_t>(v < 0 ? 0 : (v > 255 ? 255 : v));
+            v = static_cast<int>(roundf((vw - wmin) / wsc));
+            wq = static_cast<uint32_t>(v < 0 ? 0 : (v > 255 ? 255 : v));
+            v = static_cast<int>(ceilf((vp - pmin) / psc));
+            pq = static_cast<uint32_t>(v < 0 ? 0 : (v > 255 ? 255 : v));
+        }
+        bp[t] = static_cast<uint8_t>(bq);
+        bp[32 + t] = static_cast<uint8_t>(wq);
+        bp[64 + t] = static_cast<uint8_t>(pq);
+        float* sc = reinterpret_cast<float*>(bp + 96);
+        if (t < 8) {
+            const float vals[8] = {bmin, bsc, wmin, wsc, pmin, psc, wamax, 0.0F};
+            sc[t] = vals[t];
+        }
+    }
+}
+
+// Debug: encode caller-provided unit vectors (no fits, no packing).
+__global__ void EncodeUnitsKernel(EncodeView ev, const float* __restrict__ unit, uint64_t rows, uint32_t P, double sr,
+                                  uint16_t* __restrict__ digits, uint8_t* __restrict__ branch) {
+    const uint32_t blocks = P / 8;
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= rows * blocks) return;
+    const uint64_t r = g / blocks;
+    const uint32_t blk = static_cast<uint32_t>(g % blocks);
+    uint16_t d8[8];
+    int br;
+    EncodeBlock(unit + r * P + blk * 8, sr, ev, d8, &br);
+    for (int i = 0; i < 8; ++i) digits[r * P + blk * 8 + i] = d8[i];
+    branch[r * blocks + blk] = static_cast<uint8_t>(br);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+
+hipError_t LaunchMean(hipStream_t st, const float* base, uint64_t n, uint32_t dim, double* mu) {
+    hipLaunchKernelGGL(MeanKernel, dim3((dim + 63) / 64), dim3(256), 0, st, base, n, dim, mu);
+    return hipGetLastError();
+}
+
+hipError_t LaunchMeanFinish(hipStream_t st, const double* mu, uint64_t n, const IndexView& ix, float* mean_rot) {
+    hipLaunchKernelGGL(MeanFinishKernel, dim3(1), dim3(256), ix.padded * sizeof(float), st, mu, n, ix, mean_rot);
+    return hipGetLastError();
+}
+
+hipError_t LaunchCentroids(hipStream_t st, const float* centroids, const IndexView& ix, float* centroid_rot,
+                           float* centroid_n2) {
+    const uint32_t grid = static_cast<uint32_t>(ix.nlist < 65536 ? ix.nlist : 65536);
+    hipLaunchKernelGGL(CentroidKernel, dim3(grid), dim3(256), ix.padded * sizeof(float), st, centroids, ix,
+                       centroid_rot, centroid_n2);
+    return hipGetLastError();
+}
+
+hipError_t LaunchEncodeRows(hipStream_t st, const IndexView& ix, const EncodeView& ev, const EncodeArgs& a) {
+    if (a.n == 0) return hipSuccess;
+    const size_t lds = 2 * ix.padded * sizeof(float) + ix.padded * sizeof(uint16_t) + ix.blocks + 16;
+    const uint32_t grid = static_cast<uint32_t>(a.n < 32768 ? a.n : 32768);
+    hipLaunchKernelGGL(EncodeRowsKernel, dim3(grid), dim3(kEncThreads), lds, st, ix, ev, a);
+    return hipGetLastError();
+}
+
+hipError_t LaunchBanks(hipStream_t st, const IndexView& ix, const float* ob, const float* ow, const float* op,
+                       uint8_t* arena) {
+    if (ix.nlist == 0) return hipSuccess;
+    hipLaunchKernelGGL(BanksKernel, dim3(static_cast<uint32_t>(ix.nlist)), dim3(256), 0, st, ix, ob, ow, op, arena);
+    return hipGetLastError();
+}
+
+hipError_t LaunchEncodeUnits(hipStream_t st, const IndexView& ix, const EncodeView& ev, const float* unit,
+                             uint64_t rows, double scale, uint16_t* digits, uint8_t* branch) {
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    const uint64_t total = rows * (P / 8);
+    if (total == 0) return hipSuccess;
+    const double sr = scale * sqrt(static_cast<double>(P));
+    hipLaunchKernelGGL(EncodeUnitsKernel, dim3(static_cast<uint32_t>((total + 63) / 64)), dim3(64), 0, st, ev, unit,
+                       rows, P, sr, digits, branch);
+    return hipGetLastError();
+}
+
+}  // namespace e8qdev

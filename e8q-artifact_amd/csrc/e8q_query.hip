@@ -1,0 +1,279 @@
+// e8q_query.hip -- query-side kernels: FHT-Kac-4 rotation, query LUT build
+// and quantization, route distances, top-nprobe selection.
+//
+// Parity: rotation is add/sub/mul only (exact reproduction of the
+// reference's butterfly order, rotator.cpp:53-80); the LUT and route
+// arithmetic reproduce the reference's fused multiply-adds one for one
+// (e8q_search.cpp:20-73, kernels_avx2.cpp:8-103).
+#include "e8q_device.h"
+#include "e8q_kernels.h"
+
+namespace e8qdev {
+
+// ---------------------------------------------------------------------------
+// Rotation: one workgroup per row (RotateInLds, e8q_device.h).
+__global__ void RotateRowsKernel(const float* __restrict__ x, uint64_t rows, uint32_t dim, IndexView ix,
+                                 float* __restrict__ out, int center) {
+    extern __shared__ float buf[];
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    for (uint64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < dim ? x[row * dim + i] : 0.0F;
+        __syncthreads();
+        RotateInLds(buf, ix);
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x)
+            out[row * P + i] = center ? buf[i] - ix.mean_rot[i] : buf[i];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Query LUT (PrepareQuery): per block two 16-entry tables, A over dims 4..7
+// (low nibble of the rowA byte) and B over dims 0..3 (high nibble), each
+// entry a fp64 FMA chain over the four dims, stored as float.  Stored
+// compactly as [blk][A16 | B16]; the reference's duplicated halves hold the
+// same bytes.  Then the symmetric u8 quantization with the AVX2 quantizer.
+// consts[q*8 + {0..4}] = delta, base, lut_err, score_max, qn2.
+__global__ void QueryPrepKernel(const float* __restrict__ rotq, uint64_t nq, IndexView ix,
+                                uint8_t* __restrict__ lut, float* __restrict__ consts) {
+    extern __shared__ float cf[];
+    __shared__ float red[64];
+    const int B = ix.blocks;
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    const uint32_t E = static_cast<uint32_t>(B) * 32;
+    for (uint64_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+        const float* q = rotq + qi * P;
+        float hmax = 0.0F;
+        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+            const uint32_t blk = e >> 5, w = e & 31, v = w & 15;
+            const float* qb = q + blk * 8;
+            const int dim0 = w < 16 ? 4 : 0;   // A: dims 4..7, B: dims 0..3
+            double acc = 0.0;
+            for (int h = 0; h < 4; ++h) {
+                const int bit = (v >> (3 - h)) & 1;
+                acc = fma(static_cast<double>(qb[dim0 + h]), ix.gate_mid[2 * (dim0 + h) + bit], acc);
+            }
+            const float f = static_cast<float>(acc);
+            cf[e] = f;
+            hmax = fmaxf(hmax, fabsf(f));
+        }
+        // block max of |cf| (max is order independent)
+        for (int o = 32; o > 0; o >>= 1) hmax = fmaxf(hmax, __shfl_xor(hmax, o));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = hmax;
+        __syncthreads();
+        float hc = 0.0F;
+        for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) hc = fmaxf(hc, red[w]);
+        const float delta = hc > 0.0F ? 2.0F * hc / 255.0F : 1.0F;
+        const float qinv = 1.0F / delta;
+        const float qoff = hc / delta;
+        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+            const float xq = fmaf(cf[e], qinv, qoff);
+            const float r = rintf(xq);
+            int iv = (r >= -2147483648.0F && r < 2147483648.0F) ? static_cast<int>(r) : INT32_MIN;
+            iv = iv < 0 ? 0 : (iv > 255 ? 255 : iv);
+            lut[qi * E + e] = static_cast<uint8_t>(iv);
+        }
+        if (threadIdx.x < 8) {
+            // QueryNorm2Avx2: 8 FMA chains over d = l, l+8, ..., then Hsum.
+            float a = 0.0F;
+            for (uint32_t d = threadIdx.x; d < P; d += 8) a = fmaf(q[d], q[d], a);
+            const float qn2 = Hsum8Lanes(a);
+            if (threadIdx.x == 0) {
+                float* c = consts + qi * 8;
+                c[0] = delta;
+                c[1] = -hc * static_cast<float>(2 * B);
+                c[2] = delta * (sqrtf(static_cast<float>(B)) * 2.0F);
+                c[3] = static_cast<float>(B) * fmaf(delta, 255.0F, hc * 2.0F);
+                c[4] = qn2;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Route distances d2[q][c] = (qn2 + cn2) - 2*dot, dot in the AVX2
+// RouteGemv order: lane l of a 32-lane group accumulates the FMA chain over
+// d = l, l+32, ...; then ((a0+a1)+(a2+a3)) per 8-lane slice and the Hsum.
+// Grid: x = centroid tiles of 8 (one 32-lane group each), y = query tiles.
+constexpr int kRouteQT = 8;
+
+__global__ void RouteDistKernel(const float* __restrict__ rotq, const float* __restrict__ consts, uint64_t nq,
+                                IndexView ix, float* __restrict__ dist) {
+    extern __shared__ float qs[];   // kRouteQT * P
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * kRouteQT;
+    const int nqt = static_cast<int>(min<uint64_t>(kRouteQT, nq - q0));
+    for (uint32_t i = threadIdx.x; i < kRouteQT * P; i += blockDim.x) {
+        const uint32_t qi = i / P;
+        qs[i] = qi < static_cast<uint32_t>(nqt) ? rotq[(q0 + qi) * P + (i % P)] : 0.0F;
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 31;
+    const uint64_t c = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 32) + (threadIdx.x >> 5);
+    const bool valid = c < ix.nlist;
+    const float* cc = ix.centroid_rot + (valid ? c : 0) * P;
+    float acc[kRouteQT];
+#pragma unroll
+    for (int t = 0; t < kRouteQT; ++t) acc[t] = 0.0F;
+    for (uint32_t d = lane; d < P; d += 32) {
+        const float cv = cc[d];
+#pragma unroll
+        for (int t = 0; t < kRouteQT; ++t) acc[t] = fmaf(qs[t * P + d], cv, acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < kRouteQT; ++t) {
+        float v = acc[t];
+        v = v + __shfl_down(v, 8, 32);    // lanes 0..7: a0+a1, lanes 16..23: a2+a3
+        v = v + __shfl_down(v, 16, 32);   // lanes 0..7: (a0+a1)+(a2+a3)
+        v = v + __shfl_down(v, 4, 32);    // lanes 0..3: s[i]+s[i+4]
+        v = v + __shfl_down(v, 2, 32);    // lanes 0..1
+        v = v + __shfl_down(v, 1, 32);    // lane 0
+        if (lane == 0 && valid && t < nqt) {
+            const float qn2 = consts[(q0 + t) * 8 + 4];
+            float d2 = fmaf(-v, 2.0F, ix.centroid_n2[c] + qn2);
+            if (d2 < 0.0F) d2 = 0.0F;
+            dist[(q0 + t) * ix.nlist + c] = d2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Top-nprobe per query: the nprobe smallest keys (dist bits << 32 | id) in
+// ascending key order (ivf_route.cpp:9-27's partial_sort).  Radix select
+// over the 64-bit key, 8 bits per pass from the top, then a bitonic sort of
+// the selected keys.  One workgroup per query; dist row staged in LDS.
+constexpr int kSelThreads = 256;
+
+__device__ __forceinline__ uint64_t KeyOf(const uint32_t* bits, uint32_t c) {
+    return (static_cast<uint64_t>(bits[c]) << 32) | c;
+}
+
+__global__ void SelectProbesKernel(const float* __restrict__ dist, uint64_t nq, uint32_t nlist,
+                                   const uint32_t* __restrict__ nprobe_q, uint32_t nprobe_all, uint32_t maxp,
+                                   uint32_t* __restrict__ probe_ids, float* __restrict__ probe_d2) {
+    extern __shared__ uint32_t sm[];
+    uint32_t* bits = sm;                                   // nlist
+    uint64_t* sel = reinterpret_cast<uint64_t*>(sm + ((nlist + 1) & ~1u));   // pow2 >= maxp
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_pick[2];
+    __shared__ uint32_t s_cnt;
+    uint32_t sortn = 1;
+    while (sortn < maxp) sortn <<= 1;
+    for (uint64_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+        const uint32_t m = min(nlist, nprobe_q ? nprobe_q[qi] : nprobe_all);
+        for (uint32_t c = threadIdx.x; c < nlist; c += blockDim.x) bits[c] = FloatBits(dist[qi * nlist + c]);
+        __syncthreads();
+        // Radix select: find the m-th smallest key K (1-based).
+        uint64_t prefix = 0, pmask = 0;
+        uint32_t need = m;
+        for (int shift = 56; shift >= 0 && m > 0; shift -= 8) {
+            for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+            __syncthreads();
+            for (uint32_t c = threadIdx.x; c < nlist; c += blockDim.x) {
+                const uint64_t key = KeyOf(bits, c);
+                if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t acc = 0, b = 0;
+                for (; b < 256; ++b) {
+                    if (acc + hist[b] >= need) break;
+                    acc += hist[b];
+                }
+                s_pick[0] = b;
+                s_pick[1] = need - acc;
+            }
+            __syncthreads();
+            prefix |= static_cast<uint64_t>(s_pick[0]) << shift;
+            pmask |= static_cast<uint64_t>(255) << shift;
+            need = s_pick[1];
+            __syncthreads();
+        }
+        const uint64_t K = prefix;   // keys are unique: exactly m keys <= K
+        if (threadIdx.x == 0) s_cnt = 0;
+        for (uint32_t i = threadIdx.x; i < sortn; i += blockDim.x) sel[i] = ~0ull;
+        __syncthreads();
+        if (m > 0)
+            for (uint32_t c = threadIdx.x; c < nlist; c += blockDim.x) {
+                const uint64_t key = KeyOf(bits, c);
+                if (key <= K) sel[atomicAdd(&s_cnt, 1u)] = key;
+            }
+        __syncthreads();
+        // Bitonic sort of sortn keys (ascending).
+        for (uint32_t k2 = 2; k2 <= sortn; k2 <<= 1) {
+            for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < sortn; i += blockDim.x) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint64_t a = sel[i], b = sel[l];
+                        const bool up = (i & k2) == 0;
+                        if ((a > b) == up) {
+                            sel[i] = b;
+                            sel[l] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < maxp; i += blockDim.x) {
+            const bool ok = i < m;
+            probe_ids[qi * maxp + i] = ok ? static_cast<uint32_t>(sel[i]) : 0xFFFFFFFFu;
+            probe_d2[qi * maxp + i] = ok ? __uint_as_float(static_cast<uint32_t>(sel[i] >> 32)) : 0.0F;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+
+hipError_t LaunchRotate(hipStream_t st, const float* x, uint64_t rows, uint32_t dim, const IndexView& ix,
+                        float* out, bool center) {
+    if (rows == 0) return hipSuccess;
+    const uint32_t grid = static_cast<uint32_t>(rows < 65536 ? rows : 65536);
+    hipLaunchKernelGGL(RotateRowsKernel, dim3(grid), dim3(256), ix.padded * sizeof(float), st, x, rows, dim, ix,
+                       out, center ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t LaunchQueryPrep(hipStream_t st, const float* rotq, uint64_t nq, const IndexView& ix, uint8_t* lut,
+                           float* consts) {
+    if (nq == 0) return hipSuccess;
+    const uint32_t grid = static_cast<uint32_t>(nq < 65536 ? nq : 65536);
+    hipLaunchKernelGGL(QueryPrepKernel, dim3(grid), dim3(256), static_cast<size_t>(ix.blocks) * 32 * sizeof(float),
+                       st, rotq, nq, ix, lut, consts);
+    return hipGetLastError();
+}
+
+hipError_t LaunchRoute(hipStream_t st, const float* rotq, const float* consts, uint64_t nq, const IndexView& ix,
+                       float* dist) {
+    if (nq == 0) return hipSuccess;
+    const dim3 grid(static_cast<uint32_t>((ix.nlist + 7) / 8), static_cast<uint32_t>((nq + kRouteQT - 1) / kRouteQT));
+    hipLaunchKernelGGL(RouteDistKernel, grid, dim3(256), kRouteQT * ix.padded * sizeof(float), st, rotq, consts, nq,
+                       ix, dist);
+    return hipGetLastError();
+}
+
+size_t SelectProbesLds(uint32_t nlist, uint32_t maxp) {
+    uint32_t sortn = 1;
+    while (sortn < maxp) sortn <<= 1;
+    return ((nlist + 1) & ~1u) * sizeof(uint32_t) + static_cast<size_t>(sortn) * sizeof(uint64_t);
+}
+
+hipError_t LaunchSelectProbes(hipStream_t st, const float* dist, uint64_t nq, uint32_t nlist, const uint32_t* nprobe_q,
+                              uint32_t nprobe_all, uint32_t maxp, uint32_t* probe_ids, float* probe_d2) {
+    if (nq == 0) return hipSuccess;
+    const uint32_t grid = static_cast<uint32_t>(nq < 65536 ? nq : 65536);
+    const size_t lds = SelectProbesLds(nlist, maxp);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&SelectProbesKernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(SelectProbesKernel, dim3(grid), dim3(kSelThreads), lds, st, dist, nq,
+                       nlist, nprobe_q, nprobe_all, maxp, probe_ids, probe_d2);
+    return hipGetLastError();
+}
+
+}  // namespace e8qdev

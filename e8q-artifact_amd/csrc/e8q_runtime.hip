@@ -1,0 +1,616 @@
+// e8q_runtime.hip -- the C ABI of libe8q_gpu.so (include/e8q_gpu.h):
+// device context, device-resident index, workspace, and the host side of
+// every kernel launch.  No exception crosses the ABI: failures return an
+// e8q_status and leave a thread-local message for e8q_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/e8q_gpu.h"
+#include "e8q_device.h"
+#include "e8q_kernels.h"
+
+using namespace e8qdev;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail {
+    e8q_status st;
+    std::string msg;
+};
+
+[[noreturn]] void Throw(e8q_status st, const std::string& m) { throw Fail{st, m}; }
+
+void Check(hipError_t e, const char* what) {
+    if (e != hipSuccess) Throw(e == hipErrorOutOfMemory ? E8Q_ENOMEM : E8Q_ERUNTIME,
+                               std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+e8q_status Guard(F&& f) {
+    try {
+        f();
+        return E8Q_OK;
+    } catch (const Fail& x) {
+        g_err = x.msg;
+        return x.st;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return E8Q_ENOMEM;
+    } catch (const std::exception& x) {
+        g_err = x.what();
+        return E8Q_ERUNTIME;
+    }
+}
+
+// Owned device allocation.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        reset();
+        if (b == 0) b = 1;
+        Check(hipMalloc(&p, b), "hipMalloc");
+        bytes = b;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct e8q_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DBuf ws;   // per-call workspace, grown on demand
+
+    // Bump allocator over ws for one call.
+    struct Carve {
+        std::vector<std::pair<void**, size_t>> want;
+        template <class T>
+        void add(T** p, size_t count) { want.push_back({reinterpret_cast<void**>(p), count * sizeof(T)}); }
+    };
+    void Bind() { Check(hipSetDevice(device), "hipSetDevice"); }
+    void Take(Carve& c) {
+        size_t total = 0;
+        for (auto& w : c.want) total += (w.second + 255) & ~size_t{255};
+        if (ws.bytes < total) ws.alloc(total + total / 4);
+        char* base = ws.as<char>();
+        size_t off = 0;
+        for (auto& w : c.want) {
+            *w.first = base + off;
+            off += (w.second + 255) & ~size_t{255};
+        }
+    }
+};
+
+struct e8q_dindex {
+    e8q_ctx* ctx = nullptr;
+    IndexView view{};
+    EncodeView ev{};
+    float alpha = 1.0F;
+    std::vector<uint64_t> cluster_begin, arena_begin, ref_arena_begin;
+    DBuf signs, mean_rot, centroid_rot, centroid_n2, d_cluster_begin, d_arena_begin, d_ref_arena_begin, arena,
+        ref_arena, ids, refine_tab, gate_mid, levels, mlev, mdig, cls_lev, cls_dig;
+    uint64_t device_bytes() const {
+        return signs.bytes + mean_rot.bytes + centroid_rot.bytes + centroid_n2.bytes + d_cluster_begin.bytes +
+               d_arena_begin.bytes + d_ref_arena_begin.bytes + arena.bytes + ref_arena.bytes + ids.bytes +
+               refine_tab.bytes + gate_mid.bytes + levels.bytes + mlev.bytes + mdig.bytes + cls_lev.bytes +
+               cls_dig.bytes;
+    }
+};
+
+namespace {
+
+template <class T>
+void Upload(DBuf& b, const T* src, size_t count, hipStream_t st) {
+    b.alloc(count * sizeof(T));
+    if (count) Check(hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+}
+
+// Encoder tables from the per-branch level tables (e8q_codec.cpp:125-352):
+// the merged sorted table with its period-4 class interleave verified, and
+// the four (branch, parity) single-class tables sorted by (level, digit).
+void BuildEncoderTables(e8q_dindex* ix, const double* levels, int bits, hipStream_t st) {
+    const uint32_t q = 1u << bits;
+    struct E {
+        double v;
+        uint16_t d;
+        int br;
+    };
+    std::vector<E> m;
+    m.reserve(2 * q);
+    for (uint32_t d = 0; d < q; ++d) {
+        m.push_back({levels[d], static_cast<uint16_t>(d), 0});
+        m.push_back({levels[q + d], static_cast<uint16_t>(d), 1});
+    }
+    std::stable_sort(m.begin(), m.end(), [](const E& a, const E& b) { return a.v < b.v; });
+    bool alt = true;
+    for (size_t i = 0; i < m.size(); ++i) {
+        if (i > 0 && !(m[i].v > m[i - 1].v)) alt = false;
+        if (m[i].br != static_cast<int>(i & 1) || (m[i].d & 1u) != ((i >> 1) & 1u)) alt = false;
+    }
+    std::vector<double> mlev(m.size());
+    std::vector<uint16_t> mdig(m.size());
+    for (size_t i = 0; i < m.size(); ++i) {
+        mlev[i] = m[i].v;
+        mdig[i] = m[i].d;
+    }
+    const uint32_t half = q / 2;
+    std::vector<double> cl(4 * half);
+    std::vector<uint16_t> cd(4 * half);
+    for (int br = 0; br < 2; ++br)
+        for (int par = 0; par < 2; ++par) {
+            std::vector<std::pair<double, uint16_t>> e;
+            for (uint32_t d = par; d < q; d += 2) e.push_back({levels[br * q + d], static_cast<uint16_t>(d)});
+            std::sort(e.begin(), e.end());
+            for (uint32_t i = 0; i < half; ++i) {
+                cl[(br * 2 + par) * half + i] = e[i].first;
+                cd[(br * 2 + par) * half + i] = e[i].second;
+            }
+        }
+    Upload(ix->levels, levels, 2 * q, st);
+    Upload(ix->mlev, mlev.data(), mlev.size(), st);
+    Upload(ix->mdig, mdig.data(), mdig.size(), st);
+    Upload(ix->cls_lev, cl.data(), cl.size(), st);
+    Upload(ix->cls_dig, cd.data(), cd.size(), st);
+    Check(hipStreamSynchronize(st), "encoder tables");
+    ix->ev.bits = bits;
+    ix->ev.merged = alt ? 1 : 0;
+    ix->ev.mlen = 2 * q;
+    ix->ev.mlev = ix->mlev.as<double>();
+    ix->ev.mdig = ix->mdig.as<uint16_t>();
+    ix->ev.levels = ix->levels.as<double>();
+    ix->ev.cls_lev = ix->cls_lev.as<double>();
+    ix->ev.cls_dig = ix->cls_dig.as<uint16_t>();
+}
+
+void RefreshView(e8q_dindex* x) {
+    IndexView& v = x->view;
+    v.signs = x->signs.as<uint8_t>();
+    v.mean_rot = x->mean_rot.as<float>();
+    v.centroid_rot = x->centroid_rot.as<float>();
+    v.centroid_n2 = x->centroid_n2.as<float>();
+    v.cluster_begin = x->d_cluster_begin.as<uint64_t>();
+    v.arena_begin = x->d_arena_begin.as<uint64_t>();
+    v.ref_arena_begin = x->d_ref_arena_begin.as<uint64_t>();
+    v.arena = x->arena.as<uint8_t>();
+    v.ref_arena = x->ref_arena.as<uint8_t>();
+    v.ids = x->ids.as<uint32_t>();
+    v.refine_tab = x->refine_tab.as<float>();
+    v.gate_mid = x->gate_mid.as<double>();
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* e8q_last_error(void) { return g_err.c_str(); }
+
+e8q_status e8q_ctx_create(int device, e8q_ctx** out) {
+    return Guard([&] {
+        if (!out) Throw(E8Q_EINVAL, "out is null");
+        int n = 0;
+        Check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        if (device < 0 || device >= n) Throw(E8Q_EINVAL, "device index out of range");
+        auto* c = new e8q_ctx();
+        c->device = device;
+        try {
+            c->Bind();
+            Check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+e8q_status e8q_ctx_destroy(e8q_ctx* ctx) {
+    return Guard([&] {
+        if (!ctx) return;
+        ctx->Bind();
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->ws.reset();
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+e8q_status e8q_ctx_synchronize(e8q_ctx* ctx) {
+    return Guard([&] {
+        if (!ctx) Throw(E8Q_EINVAL, "ctx is null");
+        ctx->Bind();
+        Check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+e8q_status e8q_index_create(e8q_ctx* ctx, const e8q_index_desc* d, e8q_dindex** out) {
+    return Guard([&] {
+        if (!ctx || !d || !out) Throw(E8Q_EINVAL, "null argument");
+        if (d->bits < 1 || d->bits > kMaxBits) Throw(E8Q_EINVAL, "E8Q bits must be in [1, 13]");
+        if (d->dim == 0 || d->nlist == 0) Throw(E8Q_EINVAL, "dim and nlist must be positive");
+        if (d->padded != (d->dim + 63) / 64 * 64) Throw(E8Q_EINVAL, "padded must be dim rounded up to 64");
+        if (!d->signs || !d->refine_tab || !d->gate_mid || !d->levels || !d->cluster_begin || !d->arena_begin ||
+            !d->ref_arena_begin)
+            Throw(E8Q_EINVAL, "missing table");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        auto* x = new e8q_dindex();
+        try {
+            x->ctx = ctx;
+            x->alpha = d->alpha;
+            IndexView& v = x->view;
+            v.n = d->n;
+            v.dim = d->dim;
+            v.padded = d->padded;
+            v.nlist = d->nlist;
+            v.bits = d->bits;
+            v.blocks = static_cast<int>(d->padded / 8);
+            v.batch_bytes = static_cast<uint32_t>((kHdr + 32 * static_cast<uint64_t>(v.blocks) + 63) & ~63ull);
+            v.refine_stride = d->bits >= 2 ? static_cast<uint32_t>(v.blocks * (d->bits - 1)) : 0;
+            v.ref_batch_bytes = static_cast<uint32_t>((32ull * v.refine_stride + 256 + 63) & ~63ull);
+            v.tab_stride = std::max<uint32_t>(256, 1u << d->bits);
+            uint32_t lg = 0;
+            while ((uint64_t{1} << (lg + 1)) <= d->dim) ++lg;
+            v.trunc = 1u << lg;
+            v.fac = 1.0F / std::sqrt(static_cast<float>(v.trunc));
+            const uint64_t L = d->nlist;
+            x->cluster_begin.assign(d->cluster_begin, d->cluster_begin + L + 1);
+            x->arena_begin.assign(d->arena_begin, d->arena_begin + L + 1);
+            x->ref_arena_begin.assign(d->ref_arena_begin, d->ref_arena_begin + L + 1);
+            if (x->cluster_begin[0] != 0 || x->cluster_begin[L] != d->n) Throw(E8Q_EINVAL, "cluster offsets inconsistent");
+            uint64_t ab = 0, rb = 0;
+            for (uint64_t c = 0; c < L; ++c) {
+                if (x->cluster_begin[c + 1] < x->cluster_begin[c]) Throw(E8Q_EINVAL, "cluster offsets inconsistent");
+                const uint64_t nb = (x->cluster_begin[c + 1] - x->cluster_begin[c] + 31) / 32;
+                if (x->arena_begin[c] != ab || x->ref_arena_begin[c] != rb) Throw(E8Q_EINVAL, "arena offsets inconsistent");
+                ab += nb * v.batch_bytes;
+                rb += nb * v.ref_batch_bytes;
+            }
+            if (x->arena_begin[L] != ab || x->ref_arena_begin[L] != rb) Throw(E8Q_EINVAL, "arena offsets inconsistent");
+            Upload(x->signs, d->signs, 4 * d->padded / 8, st);
+            Upload(x->refine_tab, d->refine_tab, 2 * static_cast<size_t>(v.tab_stride), st);
+            Upload(x->gate_mid, d->gate_mid, 16, st);
+            Upload(x->d_cluster_begin, d->cluster_begin, L + 1, st);
+            Upload(x->d_arena_begin, d->arena_begin, L + 1, st);
+            Upload(x->d_ref_arena_begin, d->ref_arena_begin, L + 1, st);
+            x->mean_rot.alloc(d->padded * sizeof(float));
+            x->centroid_rot.alloc(L * d->padded * sizeof(float));
+            x->centroid_n2.alloc(L * sizeof(float));
+            x->arena.alloc(ab + 64);
+            x->ref_arena.alloc(rb + 64);
+            x->ids.alloc(std::max<uint64_t>(d->n, 1) * sizeof(uint32_t));
+            Check(hipMemsetAsync(x->arena.p, 0, x->arena.bytes, st), "memset");
+            Check(hipMemsetAsync(x->ref_arena.p, 0, x->ref_arena.bytes, st), "memset");
+            BuildEncoderTables(x, d->levels, d->bits, st);
+            x->ev.scale = d->scale;
+            RefreshView(x);
+        } catch (...) {
+            delete x;
+            throw;
+        }
+        *out = x;
+    });
+}
+
+e8q_status e8q_index_destroy(e8q_dindex* idx) {
+    return Guard([&] {
+        if (!idx) return;
+        idx->ctx->Bind();
+        (void)hipStreamSynchronize(idx->ctx->stream);
+        delete idx;
+    });
+}
+
+uint64_t e8q_index_device_bytes(const e8q_dindex* idx) { return idx ? idx->device_bytes() : 0; }
+
+e8q_status e8q_index_upload(e8q_ctx* ctx, e8q_dindex* x, const e8q_index_arrays* h) {
+    return Guard([&] {
+        if (!ctx || !x || !h || !h->ids || !h->mean_rot || !h->centroid_rot || !h->centroid_n2 || !h->arena ||
+            !h->ref_arena)
+            Throw(E8Q_EINVAL, "null argument");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const IndexView& v = x->view;
+        Check(hipMemcpyAsync(x->ids.p, h->ids, v.n * 4, hipMemcpyHostToDevice, st), "upload ids");
+        Check(hipMemcpyAsync(x->mean_rot.p, h->mean_rot, v.padded * 4, hipMemcpyHostToDevice, st), "upload mean");
+        Check(hipMemcpyAsync(x->centroid_rot.p, h->centroid_rot, v.nlist * v.padded * 4, hipMemcpyHostToDevice, st),
+              "upload centroids");
+        Check(hipMemcpyAsync(x->centroid_n2.p, h->centroid_n2, v.nlist * 4, hipMemcpyHostToDevice, st), "upload cn2");
+        Check(hipMemcpyAsync(x->arena.p, h->arena, x->arena.bytes, hipMemcpyHostToDevice, st), "upload arena");
+        Check(hipMemcpyAsync(x->ref_arena.p, h->ref_arena, x->ref_arena.bytes, hipMemcpyHostToDevice, st),
+              "upload refine arena");
+        Check(hipStreamSynchronize(st), "upload");
+    });
+}
+
+e8q_status e8q_index_download(e8q_ctx* ctx, const e8q_dindex* x, e8q_index_arrays* h) {
+    return Guard([&] {
+        if (!ctx || !x || !h) Throw(E8Q_EINVAL, "null argument");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const IndexView& v = x->view;
+        auto dl = [&](void* dst, const DBuf& src, size_t bytes, const char* what) {
+            if (dst) Check(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, st), what);
+        };
+        dl(h->ids, x->ids, v.n * 4, "download ids");
+        dl(h->mean_rot, x->mean_rot, v.padded * 4, "download mean");
+        dl(h->centroid_rot, x->centroid_rot, v.nlist * v.padded * 4, "download centroids");
+        dl(h->centroid_n2, x->centroid_n2, v.nlist * 4, "download cn2");
+        dl(h->arena, x->arena, x->arena.bytes, "download arena");
+        dl(h->ref_arena, x->ref_arena, x->ref_arena.bytes, "download refine arena");
+        Check(hipStreamSynchronize(st), "download");
+    });
+}
+
+e8q_status e8q_index_encode(e8q_ctx* ctx, e8q_dindex* x, const float* base, const float* centroids,
+                            const uint32_t* cluster_ids, const uint32_t* slot, int on_device) {
+    return Guard([&] {
+        if (!ctx || !x || !base || !centroids || !cluster_ids || !slot) Throw(E8Q_EINVAL, "null argument");
+        const IndexView& v = x->view;
+        if (v.n == 0) Throw(E8Q_EINVAL, "empty build set");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        DBuf d_base, d_cent, d_cid, d_slot, mu, ob, ow, op;
+        const float* pb = base;
+        const float* pc = centroids;
+        const uint32_t* pid = cluster_ids;
+        const uint32_t* ps = slot;
+        if (!on_device) {
+            for (uint64_t i = 0; i < v.n; ++i)
+                if (cluster_ids[i] >= v.nlist) Throw(E8Q_EINVAL, "cluster id out of range");
+            Upload(d_base, base, v.n * v.dim, st);
+            Upload(d_cent, centroids, v.nlist * v.dim, st);
+            Upload(d_cid, cluster_ids, v.n, st);
+            Upload(d_slot, slot, v.n, st);
+            pb = d_base.as<float>();
+            pc = d_cent.as<float>();
+            pid = d_cid.as<uint32_t>();
+            ps = d_slot.as<uint32_t>();
+        }
+        mu.alloc(v.dim * sizeof(double));
+        ob.alloc(v.n * sizeof(float));
+        ow.alloc(v.n * sizeof(float));
+        op.alloc(v.n * sizeof(float));
+        Check(hipMemsetAsync(ob.p, 0, ob.bytes, st), "memset");
+        Check(hipMemsetAsync(ow.p, 0, ow.bytes, st), "memset");
+        Check(hipMemsetAsync(op.p, 0, op.bytes, st), "memset");
+        Check(hipMemsetAsync(x->arena.p, 0, x->arena.bytes, st), "memset");
+        Check(hipMemsetAsync(x->ref_arena.p, 0, x->ref_arena.bytes, st), "memset");
+        Check(LaunchMean(st, pb, v.n, static_cast<uint32_t>(v.dim), mu.as<double>()), "mean");
+        Check(LaunchMeanFinish(st, mu.as<double>(), v.n, v, x->mean_rot.as<float>()), "mean finish");
+        Check(LaunchCentroids(st, pc, v, x->centroid_rot.as<float>(), x->centroid_n2.as<float>()), "centroids");
+        EncodeArgs a{};
+        a.base = pb;
+        a.cluster_ids = pid;
+        a.slot = ps;
+        a.n = v.n;
+        a.arena = x->arena.as<uint8_t>();
+        a.ref_arena = x->ref_arena.as<uint8_t>();
+        a.ids = x->ids.as<uint32_t>();
+        a.ob = ob.as<float>();
+        a.ow = ow.as<float>();
+        a.op = op.as<float>();
+        Check(LaunchEncodeRows(st, v, x->ev, a), "encode rows");
+        Check(LaunchBanks(st, v, ob.as<float>(), ow.as<float>(), op.as<float>(), x->arena.as<uint8_t>()), "banks");
+        Check(hipStreamSynchronize(st), "encode");
+    });
+}
+
+e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* a, e8q_search_stats* stats) {
+    return Guard([&] {
+        if (!ctx || !x || !a) Throw(E8Q_EINVAL, "null argument");
+        if (a->k == 0 || a->k > static_cast<uint64_t>(kMaxK))
+            Throw(E8Q_EINVAL, "k must be in [1, " + std::to_string(kMaxK) + "]");
+        if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
+        const int flush = a->flush_every > 0 ? a->flush_every : 32;
+        if (flush > 256) Throw(E8Q_EINVAL, "flush_every must be <= 256");
+        if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "unknown search mode");
+        const IndexView& v = x->view;
+        const uint64_t nq = a->nq;
+        if (stats) *stats = e8q_search_stats{0, 0};
+        if (nq == 0) return;
+        uint32_t maxp = 0;
+        if (a->nprobe_per_query) {
+            for (uint64_t i = 0; i < nq; ++i) {
+                if (a->nprobe_per_query[i] == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+                maxp = std::max<uint32_t>(maxp, a->nprobe_per_query[i]);
+            }
+        } else {
+            if (a->nprobe == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+            maxp = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
+        }
+        maxp = static_cast<uint32_t>(std::min<uint64_t>(maxp, v.nlist));
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const uint64_t P = v.padded, E = static_cast<uint64_t>(v.blocks) * 32;
+        float *dq = nullptr, *rotq = nullptr, *consts = nullptr, *dist = nullptr, *pd2 = nullptr, *odist = nullptr;
+        uint8_t* lut = nullptr;
+        uint32_t *pids = nullptr, *npq = nullptr, *oids = nullptr;
+        unsigned long long* dstats = nullptr;
+        e8q_ctx::Carve cv;
+        if (!a->queries_on_device) cv.add(&dq, nq * v.dim);
+        cv.add(&rotq, nq * P);
+        cv.add(&lut, nq * E);
+        cv.add(&consts, nq * 8);
+        cv.add(&dist, nq * v.nlist);
+        cv.add(&pids, nq * maxp);
+        cv.add(&pd2, nq * maxp);
+        if (a->nprobe_per_query) cv.add(&npq, nq);
+        if (!a->outputs_on_device) {
+            cv.add(&oids, nq * a->k);
+            cv.add(&odist, nq * a->k);
+        }
+        cv.add(&dstats, 2);
+        ctx->Take(cv);
+        const float* qsrc = a->queries;
+        if (!a->queries_on_device) {
+            Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+            qsrc = dq;
+        }
+        if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
+        if (a->outputs_on_device) {
+            oids = a->out_ids;
+            odist = a->out_dists;
+        }
+        Check(hipMemsetAsync(dstats, 0, 16, st), "memset");
+        Check(LaunchRotate(st, qsrc, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
+        Check(LaunchQueryPrep(st, rotq, nq, v, lut, consts), "query prep");
+        Check(LaunchRoute(st, rotq, consts, nq, v, dist), "route");
+        const size_t sel_lds = SelectProbesLds(static_cast<uint32_t>(v.nlist), maxp);
+        if (sel_lds > 160 * 1024) Throw(E8Q_EINVAL, "nlist too large for the probe selector");
+        Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq,
+                                 static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist)), maxp, pids, pd2),
+              "select probes");
+        PlainArgs pa{};
+        pa.nq = nq;
+        pa.rotq = rotq;
+        pa.lut = lut;
+        pa.consts = consts;
+        pa.probe_ids = pids;
+        pa.probe_d2 = pd2;
+        pa.maxp = maxp;
+        pa.nprobe_q = npq;
+        pa.nprobe_all = maxp;
+        pa.k = a->k;
+        pa.z = a->z;
+        pa.disable_prune = a->disable_prune ? 1 : 0;
+        pa.flush_every = flush;
+        pa.out_ids = oids;
+        pa.out_dists = odist;
+        pa.stats = dstats;
+        Check(LaunchPlainSearch(st, v, pa), "plain search");
+        if (!a->outputs_on_device) {
+            Check(hipMemcpyAsync(a->out_ids, oids, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
+            Check(hipMemcpyAsync(a->out_dists, odist, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");
+        }
+        unsigned long long hs[2] = {0, 0};
+        Check(hipMemcpyAsync(hs, dstats, 16, hipMemcpyDeviceToHost, st), "download stats");
+        Check(hipStreamSynchronize(st), "search");
+        if (stats) {
+            stats->candidates = hs[0];
+            stats->survivors = hs[1];
+        }
+    });
+}
+
+e8q_status e8q_debug_coarse_accumulate(e8q_ctx* ctx, const uint8_t* rowa, uint64_t nbatch, const uint8_t* lut,
+                                       uint64_t blocks, int32_t* out) {
+    return Guard([&] {
+        if (!ctx || !rowa || !lut || !out) Throw(E8Q_EINVAL, "null argument");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        DBuf dr, dl, dout;
+        Upload(dr, rowa, nbatch * blocks * 32, st);
+        Upload(dl, lut, blocks * 64, st);
+        dout.alloc(nbatch * 32 * 4);
+        Check(LaunchCoarseAccumulate(st, dr.as<uint8_t>(), nbatch, dl.as<uint8_t>(), static_cast<int>(blocks),
+                                     dout.as<int32_t>()),
+              "coarse accumulate");
+        Check(hipMemcpyAsync(out, dout.p, nbatch * 32 * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipStreamSynchronize(st), "coarse accumulate");
+    });
+}
+
+e8q_status e8q_debug_query_prep(e8q_ctx* ctx, const e8q_dindex* x, const float* queries, uint64_t nq, uint64_t nprobe,
+                                float* rot_q, uint8_t* lut, float* consts, uint32_t* probe, float* cdist) {
+    return Guard([&] {
+        if (!ctx || !x || !queries) Throw(E8Q_EINVAL, "null argument");
+        const IndexView& v = x->view;
+        if (nprobe == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const uint64_t P = v.padded, E = static_cast<uint64_t>(v.blocks) * 32;
+        const uint32_t maxp = static_cast<uint32_t>(std::min<uint64_t>(nprobe, v.nlist));
+        DBuf dq, drq, dlut, dc, dd, dp, dp2;
+        Upload(dq, queries, nq * v.dim, st);
+        drq.alloc(nq * P * 4);
+        dlut.alloc(nq * E);
+        dc.alloc(nq * 8 * 4);
+        dd.alloc(nq * v.nlist * 4);
+        dp.alloc(nq * maxp * 4);
+        dp2.alloc(nq * maxp * 4);
+        Check(LaunchRotate(st, dq.as<float>(), nq, static_cast<uint32_t>(v.dim), v, drq.as<float>(), true), "rotate");
+        Check(LaunchQueryPrep(st, drq.as<float>(), nq, v, dlut.as<uint8_t>(), dc.as<float>()), "prep");
+        Check(LaunchRoute(st, drq.as<float>(), dc.as<float>(), nq, v, dd.as<float>()), "route");
+        Check(LaunchSelectProbes(st, dd.as<float>(), nq, static_cast<uint32_t>(v.nlist), nullptr, maxp, maxp,
+                                 dp.as<uint32_t>(), dp2.as<float>()),
+              "select");
+        std::vector<uint8_t> hl(nq * E);
+        std::vector<float> hc(nq * 8);
+        if (rot_q) Check(hipMemcpyAsync(rot_q, drq.p, nq * P * 4, hipMemcpyDeviceToHost, st), "dl");
+        Check(hipMemcpyAsync(hl.data(), dlut.p, nq * E, hipMemcpyDeviceToHost, st), "dl");
+        Check(hipMemcpyAsync(hc.data(), dc.p, nq * 32, hipMemcpyDeviceToHost, st), "dl");
+        if (probe) Check(hipMemcpyAsync(probe, dp.p, nq * maxp * 4, hipMemcpyDeviceToHost, st), "dl");
+        if (cdist) Check(hipMemcpyAsync(cdist, dd.p, nq * v.nlist * 4, hipMemcpyDeviceToHost, st), "dl");
+        Check(hipStreamSynchronize(st), "query prep");
+        // Expand the compact LUT to the reference's 64-byte-per-block layout.
+        if (lut)
+            for (uint64_t q = 0; q < nq; ++q)
+                for (int b = 0; b < v.blocks; ++b) {
+                    const uint8_t* s = &hl[q * E + b * 32];
+                    uint8_t* o = lut + q * v.blocks * 64 + b * 64;
+                    std::memcpy(o, s, 16);
+                    std::memcpy(o + 16, s, 16);
+                    std::memcpy(o + 32, s + 16, 16);
+                    std::memcpy(o + 48, s + 16, 16);
+                }
+        if (consts)
+            for (uint64_t q = 0; q < nq; ++q) std::memcpy(consts + q * 4, &hc[q * 8], 16);
+    });
+}
+
+e8q_status e8q_debug_rotate(e8q_ctx* ctx, const e8q_dindex* x, const float* in, uint64_t nrows, float* out) {
+    return Guard([&] {
+        if (!ctx || !x || !in || !out) Throw(E8Q_EINVAL, "null argument");
+        const IndexView& v = x->view;
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        DBuf di, dout;
+        Upload(di, in, nrows * v.dim, st);
+        dout.alloc(nrows * v.padded * 4);
+        Check(LaunchRotate(st, di.as<float>(), nrows, static_cast<uint32_t>(v.dim), v, dout.as<float>(), false), "rotate");
+        Check(hipMemcpyAsync(out, dout.p, nrows * v.padded * 4, hipMemcpyDeviceToHost, st), "dl");
+        Check(hipStreamSynchronize(st), "rotate");
+    });
+}
+
+e8q_status e8q_debug_encode(e8q_ctx* ctx, const e8q_dindex* x, const float* unit, uint64_t nrows, double s,
+                            uint16_t* digits, uint8_t* branch) {
+    return Guard([&] {
+        if (!ctx || !x || !unit || !digits || !branch) Throw(E8Q_EINVAL, "null argument");
+        const IndexView& v = x->view;
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        DBuf du, dd, db;
+        Upload(du, unit, nrows * v.padded, st);
+        dd.alloc(nrows * v.padded * 2);
+        db.alloc(nrows * v.blocks);
+        Check(LaunchEncodeUnits(st, v, x->ev, du.as<float>(), nrows, s, dd.as<uint16_t>(), db.as<uint8_t>()), "encode");
+        Check(hipMemcpyAsync(digits, dd.p, nrows * v.padded * 2, hipMemcpyDeviceToHost, st), "dl");
+        Check(hipMemcpyAsync(branch, db.p, nrows * v.blocks, hipMemcpyDeviceToHost, st), "dl");
+        Check(hipStreamSynchronize(st), "encode");
+    });
+}
+
+}  // extern "C"

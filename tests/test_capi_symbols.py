@@ -1,0 +1,20 @@
+"""The device library loads on a CPU-only host and exports every symbol its
+header declares (no device calls)."""
+import ctypes
+
+import capi
+
+
+def test_library_exports_declared_symbols():
+    names = capi.declared_symbols()
+    assert len(names) >= 14
+    L = ctypes.CDLL(capi.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_header_layout():
+    # Field order/padding of the ctypes mirrors (x86-64 SysV layout).
+    assert ctypes.sizeof(capi.IndexDesc) == 4 * 8 + 4 + 4 + 8 + 7 * 8
+    assert ctypes.sizeof(capi.IndexArrays) == 6 * 8
+    assert ctypes.sizeof(capi.SearchArgs) == 8 * 4 + 8 + 4 * 5 + 4 + 8 + 8 + 8
