@@ -24,7 +24,7 @@ struct PlainArgs {
     int flush_every;
     uint32_t* out_ids;          // [nq][k]
     float* out_dists;
-    unsigned long long* stats;  // [2] candidates, survivors
+    unsigned long long* stats;  // [3] candidates, survivors, batches
 };
 
 hipError_t LaunchRotate(hipStream_t st, const float* x, uint64_t rows, uint32_t dim, const IndexView& ix, float* out,

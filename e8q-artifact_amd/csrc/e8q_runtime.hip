@@ -80,6 +80,7 @@ struct DBuf {
 struct e8q_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // search timing
     DBuf ws;   // per-call workspace, grown on demand
 
     // Bump allocator over ws for one call.
@@ -217,6 +218,7 @@ e8q_status e8q_ctx_create(int device, e8q_ctx** out) {
         try {
             c->Bind();
             Check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+            for (auto& e : c->ev) Check(hipEventCreate(&e), "hipEventCreate");
         } catch (...) {
             delete c;
             throw;
@@ -231,6 +233,8 @@ e8q_status e8q_ctx_destroy(e8q_ctx* ctx) {
         ctx->Bind();
         (void)hipStreamSynchronize(ctx->stream);
         ctx->ws.reset();
+        for (auto& e : ctx->ev)
+            if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -375,19 +379,20 @@ e8q_status e8q_index_encode(e8q_ctx* ctx, e8q_dindex* x, const float* base, cons
         const float* pb = base;
         const float* pc = centroids;
         const uint32_t* pid = cluster_ids;
-        const uint32_t* ps = slot;
+        for (uint64_t i = 0; i < v.n; ++i)
+            if (slot[i] >= v.n) Throw(E8Q_EINVAL, "slot out of range");
         if (!on_device) {
             for (uint64_t i = 0; i < v.n; ++i)
                 if (cluster_ids[i] >= v.nlist) Throw(E8Q_EINVAL, "cluster id out of range");
             Upload(d_base, base, v.n * v.dim, st);
             Upload(d_cent, centroids, v.nlist * v.dim, st);
             Upload(d_cid, cluster_ids, v.n, st);
-            Upload(d_slot, slot, v.n, st);
             pb = d_base.as<float>();
             pc = d_cent.as<float>();
             pid = d_cid.as<uint32_t>();
-            ps = d_slot.as<uint32_t>();
         }
+        Upload(d_slot, slot, v.n, st);
+        const uint32_t* ps = d_slot.as<uint32_t>();
         mu.alloc(v.dim * sizeof(double));
         ob.alloc(v.n * sizeof(float));
         ow.alloc(v.n * sizeof(float));
@@ -428,7 +433,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "unknown search mode");
         const IndexView& v = x->view;
         const uint64_t nq = a->nq;
-        if (stats) *stats = e8q_search_stats{0, 0};
+        if (stats) *stats = e8q_search_stats{0, 0, 0, 0.0, 0.0};
         if (nq == 0) return;
         uint32_t maxp = 0;
         if (a->nprobe_per_query) {
@@ -461,7 +466,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             cv.add(&oids, nq * a->k);
             cv.add(&odist, nq * a->k);
         }
-        cv.add(&dstats, 2);
+        cv.add(&dstats, 4);
         ctx->Take(cv);
         const float* qsrc = a->queries;
         if (!a->queries_on_device) {
@@ -473,7 +478,8 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             oids = a->out_ids;
             odist = a->out_dists;
         }
-        Check(hipMemsetAsync(dstats, 0, 16, st), "memset");
+        Check(hipMemsetAsync(dstats, 0, 32, st), "memset");
+        Check(hipEventRecord(ctx->ev[0], st), "event");
         Check(LaunchRotate(st, qsrc, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
         Check(LaunchQueryPrep(st, rotq, nq, v, lut, consts), "query prep");
         Check(LaunchRoute(st, rotq, consts, nq, v, dist), "route");
@@ -499,17 +505,26 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         pa.out_ids = oids;
         pa.out_dists = odist;
         pa.stats = dstats;
+        Check(hipEventRecord(ctx->ev[1], st), "event");
         Check(LaunchPlainSearch(st, v, pa), "plain search");
+        Check(hipEventRecord(ctx->ev[2], st), "event");
         if (!a->outputs_on_device) {
             Check(hipMemcpyAsync(a->out_ids, oids, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
             Check(hipMemcpyAsync(a->out_dists, odist, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");
         }
-        unsigned long long hs[2] = {0, 0};
-        Check(hipMemcpyAsync(hs, dstats, 16, hipMemcpyDeviceToHost, st), "download stats");
+        unsigned long long hs[4] = {0, 0, 0, 0};
+        Check(hipMemcpyAsync(hs, dstats, 32, hipMemcpyDeviceToHost, st), "download stats");
+        Check(hipEventRecord(ctx->ev[3], st), "event");
         Check(hipStreamSynchronize(st), "search");
         if (stats) {
+            float ms_scan = 0.0F, ms_all = 0.0F;
+            Check(hipEventElapsedTime(&ms_scan, ctx->ev[1], ctx->ev[2]), "event time");
+            Check(hipEventElapsedTime(&ms_all, ctx->ev[0], ctx->ev[2]), "event time");
             stats->candidates = hs[0];
             stats->survivors = hs[1];
+            stats->batches = hs[2];
+            stats->scan_ms = ms_scan;
+            stats->total_ms = ms_all;
         }
     });
 }

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kPlainThreads) void PlainSearchKernel(IndexView ix,
         const float* cst = a.consts + qi * 8;
         const float delta = cst[0], base = cst[1], lut_err = cst[2], score_max = cst[3];
         int cnt = 0, staged = 0;
-        uint64_t n_cand = 0, n_surv = 0;
+        uint64_t n_cand = 0, n_surv = 0, n_batch = 0;
         const uint32_t np = min(static_cast<uint32_t>(ix.nlist), a.nprobe_q ? a.nprobe_q[qi] : a.nprobe_all);
         for (uint32_t p = 0; p < np; ++p) {
             const uint32_t cid = a.probe_ids[qi * a.maxp + p];
@@ -220,6 +220,7 @@ __global__ __launch_bounds__(kPlainThreads) void PlainSearchKernel(IndexView ix,
                     const uint32_t bj = b0 + j, off = bj * 32;
                     const uint32_t n_logic = min(32u, ccount - off);
                     n_cand += n_logic;
+                    n_batch += 1;
                     const float limit0 = S.limit;
                     const float rh = limit0 < d2 ? sqrtf(fmaxf(0.0F, limit0)) : sqrt_d2;
                     if (!a.disable_prune && limit0 < inf) {
@@ -297,6 +298,7 @@ __global__ __launch_bounds__(kPlainThreads) void PlainSearchKernel(IndexView ix,
         if (tid == 0 && a.stats != nullptr) {
             atomicAdd(&a.stats[0], static_cast<unsigned long long>(n_cand));
             atomicAdd(&a.stats[1], static_cast<unsigned long long>(n_surv));
+            atomicAdd(&a.stats[2], static_cast<unsigned long long>(n_batch));
         }
         __syncthreads();
     }

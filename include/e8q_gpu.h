@@ -65,6 +65,9 @@ typedef struct {
 typedef struct {
     uint64_t candidates; /* SearchStats::candidates (e8q.hpp:55-62) */
     uint64_t survivors;  /* SearchStats::survivors                  */
+    uint64_t batches;    /* 32-vector scan records streamed         */
+    double scan_ms;      /* scan kernel time, HIP events on the context stream */
+    double total_ms;     /* device time of the whole search, same events       */
 } e8q_search_stats;
 
 /* Search modes. PLAIN keeps the reference's per-query schedule bit for bit
@@ -115,8 +118,9 @@ e8q_status e8q_index_download(e8q_ctx* ctx, const e8q_dindex* idx, e8q_index_arr
  * fp64 base mean, rotated centroids, per-row rotate / residual / E8
  * nearest-lattice-point encode / fp64 fits / pack, then the per-batch u8
  * banks.  `slot[i]` = cluster_begin[cluster_ids[i]] + rank of row i in its
- * list (host computed, e8q_build.cpp:225-231).  base/centroids/slot/cluster
- * ids are host pointers unless on_device != 0. */
+ * list (host computed, e8q_build.cpp:225-231), always a host array.
+ * base / centroids / cluster_ids are host pointers, or device pointers when
+ * on_device != 0 (then cluster ids are trusted to be < nlist). */
 e8q_status e8q_index_encode(e8q_ctx* ctx, e8q_dindex* idx, const float* base, const float* centroids,
                             const uint32_t* cluster_ids, const uint32_t* slot, int on_device);
 

@@ -50,3 +50,9 @@ $(OUT)/gen_golden: $(dir $(lastword $(MAKEFILE_LIST)))../tests/golden/gen_golden
 	$(CXX) -include cstddef -std=c++17 -O2 -I$(REF)/include $< -o $@ -L$(OUT) -le8qref -Wl,-rpath,'$$ORIGIN'
 
 all: $(OUT)/gen_golden
+
+# CPU baseline timer over the reference library (oracle/ref_bench.cpp).
+$(OUT)/ref_bench: $(dir $(lastword $(MAKEFILE_LIST)))ref_bench.cpp $(OUT)/libe8qref.so
+	$(CXX) -include cstddef -std=c++17 -O2 -fopenmp -I$(REF)/include $< -o $@ -L$(OUT) -le8qref -Wl,-rpath,'$$ORIGIN'
+
+all: $(OUT)/ref_bench

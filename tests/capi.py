@@ -31,7 +31,8 @@ class IndexArrays(ctypes.Structure):
 
 
 class SearchStats(ctypes.Structure):
-    _fields_ = [("candidates", u64), ("survivors", u64)]
+    _fields_ = [("candidates", u64), ("survivors", u64), ("batches", u64), ("scan_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
 
 
 class SearchArgs(ctypes.Structure):
