@@ -96,11 +96,13 @@ def ground_truth(torch, base, queries, k=10, cand=64):
     return out
 
 
-def recall_at(gt, ids, k=10):
-    import numpy as np
+def recall_at(gt, ids, k=10, within=None):
+    """|exact top-k  intersect  first `within` results| / k, averaged (within=k: the
+    reference's recall_at_k, experiments/protocol.py:482-488)."""
+    w = k if within is None else within
     hits = 0
     for i in range(gt.shape[0]):
-        hits += len(set(gt[i, :k].tolist()) & set(ids[i, :k].tolist()))
+        hits += len(set(gt[i, :k].tolist()) & set(ids[i, :w].tolist()))
     return hits / float(k * gt.shape[0])
 
 
@@ -204,14 +206,16 @@ def main():
             ids = ids_dev.cpu().numpy().view(np.uint32)
             ms = 1000.0 * sum(times) / len(times)
             row = {"nprobe": nprobe, "ms_per_step": ms, "qps": nq / (ms / 1000.0),
-                   "recall": recall_at(gt, ids) if gt is not None else -1.0,
+                   "recall": recall_at(gt, ids, 10, k) if gt is not None else -1.0,
+                   "recall_10at10": recall_at(gt, ids, 10, 10) if gt is not None else -1.0,
                    "scan_kernel_ms": sum(scan_ms) / len(scan_ms), "batches": st["batches"],
                    "survivors": st["survivors"], "candidates": st["candidates"]}
             algo = row["batches"] * batch_bytes + row["survivors"] * survivor_bytes
             row["scan_algorithmic_bytes"] = algo
             row["scan_achieved_gbs"] = algo / (row["scan_kernel_ms"] / 1000.0) / 1e9
             rows.append(row)
-            log(f"{mode} nprobe={nprobe}: {ms:.2f} ms/step, {row['qps']:.0f} QPS, recall@10={row['recall']:.4f}, "
+            log(f"{mode} nprobe={nprobe}: {ms:.2f} ms/step, {row['qps']:.0f} QPS, recall 10@{k}={row['recall']:.4f} "
+                f"(10@10 {row['recall_10at10']:.4f}), "
                 f"scan kernel {row['scan_kernel_ms']:.2f} ms = {row['scan_achieved_gbs']:.0f} GB/s")
         results[mode] = {"rows": rows, "qps_at_target": qps_at_recall(rows, args.target) if gt is not None else None}
     if args.profile:
@@ -260,7 +264,11 @@ def main():
                                "10k queries, k=100, nprobe 16..128",
                    "n": args.n, "dim": args.dim, "nlist": args.nlist, "nq": nq, "k": k, "bits": args.bits,
                    "nprobe_grid": grid, "form": best, "operating_nprobe": op["nprobe"],
-                   "recall_at_operating_point": op["recall"]},
+                   "recall_at_operating_point": op["recall"],
+                   "recall_definition": f"recall@10 = fraction of the exact top-10 (fp64) found among the k={k} "
+                                        "results (10@100); the strict first-10 overlap (10@10) is in each grid row "
+                                        "and is capped near 0.91 by the 4-bit codes on this data, for the "
+                                        "reference CPU search as well"},
         "latency_us": {"p50": round(p50, 1), "p99": round(p99, 1), "what": "single-query search() calls"},
         "modes": {m: {"qps_at_target": results[m]["qps_at_target"],
                       "grid": [{kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items()}

@@ -108,6 +108,7 @@ struct e8q_dindex {
     IndexView view{};
     EncodeView ev{};
     float alpha = 1.0F;
+    uint64_t arena_bytes = 0;   // E8AX arena size (the allocation adds prefetch padding)
     std::vector<uint64_t> cluster_begin, arena_begin, ref_arena_begin;
     DBuf signs, mean_rot, centroid_rot, centroid_n2, d_cluster_begin, d_arena_begin, d_ref_arena_begin, arena,
         ref_arena, ids, refine_tab, gate_mid, levels, mlev, mdig, cls_lev, cls_dig;
@@ -301,7 +302,11 @@ e8q_status e8q_index_create(e8q_ctx* ctx, const e8q_index_desc* d, e8q_dindex** 
             x->mean_rot.alloc(d->padded * sizeof(float));
             x->centroid_rot.alloc(L * d->padded * sizeof(float));
             x->centroid_n2.alloc(L * sizeof(float));
-            x->arena.alloc(ab + 64);
+            // Two records of tail padding: the scan prefetches whole two-record
+            // steps, so the last step of the last cluster may read one record
+            // past the arena (never used).
+            x->arena_bytes = ab + 64;
+            x->arena.alloc(ab + 64 + 2 * static_cast<uint64_t>(v.batch_bytes));
             x->ref_arena.alloc(rb + 64);
             x->ids.alloc(std::max<uint64_t>(d->n, 1) * sizeof(uint32_t));
             Check(hipMemsetAsync(x->arena.p, 0, x->arena.bytes, st), "memset");
@@ -341,7 +346,7 @@ e8q_status e8q_index_upload(e8q_ctx* ctx, e8q_dindex* x, const e8q_index_arrays*
         Check(hipMemcpyAsync(x->centroid_rot.p, h->centroid_rot, v.nlist * v.padded * 4, hipMemcpyHostToDevice, st),
               "upload centroids");
         Check(hipMemcpyAsync(x->centroid_n2.p, h->centroid_n2, v.nlist * 4, hipMemcpyHostToDevice, st), "upload cn2");
-        Check(hipMemcpyAsync(x->arena.p, h->arena, x->arena.bytes, hipMemcpyHostToDevice, st), "upload arena");
+        Check(hipMemcpyAsync(x->arena.p, h->arena, x->arena_bytes, hipMemcpyHostToDevice, st), "upload arena");
         Check(hipMemcpyAsync(x->ref_arena.p, h->ref_arena, x->ref_arena.bytes, hipMemcpyHostToDevice, st),
               "upload refine arena");
         Check(hipStreamSynchronize(st), "upload");
@@ -361,7 +366,7 @@ e8q_status e8q_index_download(e8q_ctx* ctx, const e8q_dindex* x, e8q_index_array
         dl(h->mean_rot, x->mean_rot, v.padded * 4, "download mean");
         dl(h->centroid_rot, x->centroid_rot, v.nlist * v.padded * 4, "download centroids");
         dl(h->centroid_n2, x->centroid_n2, v.nlist * 4, "download cn2");
-        dl(h->arena, x->arena, x->arena.bytes, "download arena");
+        dl(h->arena, x->arena, x->arena_bytes, "download arena");
         dl(h->ref_arena, x->ref_arena, x->ref_arena.bytes, "download refine arena");
         Check(hipStreamSynchronize(st), "download");
     });
