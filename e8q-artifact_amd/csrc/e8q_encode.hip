@@ -272,7 +272,7 @@ __global__ __launch_bounds__(kEncThreads) void EncodeRowsKernel(IndexView ix, En
                                                            : ((d8[i] >> (bits - 1)) & 1u);
                 byte |= top << (7 - i);
             }
-            rec[kHdr + blk * 32 + lane] = static_cast<uint8_t>(byte);
+            rec[kHdr + lane * ix.blocks + blk] = static_cast<uint8_t>(byte);   // device layout: [lane][blk]
             if (bits >= 2) {   // refine field (e8q_codec.cpp:827-837)
                 const int cb = cbytes;
                 const uint64_t m = (uint64_t{1} << cb) - 1;
@@ -429,7 +429,43 @@ __global__ void EncodeUnitsKernel(EncodeView ev, const float* __restrict__ unit,
 }
 
 // ---------------------------------------------------------------------------
+// Scan-record rowA layout: the E8AX file stores it [blk][lane] (32 bytes per
+// block); the device stores it [lane][blk] so that one lane's code bytes are
+// one contiguous load.  Headers are not touched.  One workgroup per record,
+// which reads its whole rowA into LDS before writing, so src == dst is fine.
+__global__ __launch_bounds__(256) void RecordLayoutKernel(const uint8_t* src, uint8_t* dst, uint64_t nrec,
+                                                          uint32_t batch_bytes, uint32_t blocks, int to_device) {
+    extern __shared__ uint8_t rbuf[];
+    const uint32_t R = 32u * blocks;
+    for (uint64_t r = blockIdx.x; r < nrec; r += gridDim.x) {
+        const uint8_t* s = src + r * batch_bytes + kHdr;
+        uint8_t* d = dst + r * batch_bytes + kHdr;
+        for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) rbuf[i] = s[i];
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) {
+            if (to_device) {
+                const uint32_t lane = i / blocks, blk = i - lane * blocks;
+                d[i] = rbuf[blk * 32 + lane];
+            } else {
+                const uint32_t blk = i >> 5, lane = i & 31;
+                d[i] = rbuf[lane * blocks + blk];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
+
+hipError_t LaunchRecordLayout(hipStream_t st, const IndexView& ix, const uint8_t* src, uint8_t* dst, uint64_t nrec,
+                              bool to_device) {
+    if (nrec == 0) return hipSuccess;
+    const uint32_t grid = static_cast<uint32_t>(nrec < 65536 ? nrec : 65536);
+    hipLaunchKernelGGL(RecordLayoutKernel, dim3(grid), dim3(256), 32 * static_cast<size_t>(ix.blocks), st, src, dst,
+                       nrec, ix.batch_bytes, static_cast<uint32_t>(ix.blocks), to_device ? 1 : 0);
+    return hipGetLastError();
+}
 
 hipError_t LaunchMean(hipStream_t st, const float* base, uint64_t n, uint32_t dim, double* mu) {
     hipLaunchKernelGGL(MeanKernel, dim3((dim + 63) / 64), dim3(256), 0, st, base, n, dim, mu);

@@ -37,7 +37,14 @@ size_t SelectProbesLds(uint32_t nlist, uint32_t maxp);
 hipError_t LaunchSelectProbes(hipStream_t st, const float* dist, uint64_t nq, uint32_t nlist,
                               const uint32_t* nprobe_q, uint32_t nprobe_all, uint32_t maxp, uint32_t* probe_ids,
                               float* probe_d2);
-hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a);
+// The plain scan reads its parameters from device memory (`dparams`, filled
+// by the launcher on the stream) so that rarely used fields are loaded where
+// they are used instead of occupying scalar registers for the whole kernel.
+struct PlainParams {
+    IndexView ix;
+    PlainArgs a;
+};
+hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a, PlainParams* dparams);
 hipError_t LaunchCoarseAccumulate(hipStream_t st, const uint8_t* rowa, uint64_t nbatch, const uint8_t* lut, int blocks,
                                   int32_t* out);
 
@@ -62,6 +69,10 @@ hipError_t LaunchCentroids(hipStream_t st, const float* centroids, const IndexVi
 hipError_t LaunchEncodeRows(hipStream_t st, const IndexView& ix, const EncodeView& ev, const EncodeArgs& a);
 hipError_t LaunchBanks(hipStream_t st, const IndexView& ix, const float* ob, const float* ow, const float* op,
                        uint8_t* arena);
+// Scan records [nrec] between the E8AX rowA layout ([blk][lane]) and the
+// device layout ([lane][blk]); src may equal dst.
+hipError_t LaunchRecordLayout(hipStream_t st, const IndexView& ix, const uint8_t* src, uint8_t* dst, uint64_t nrec,
+                              bool to_device);
 hipError_t LaunchEncodeUnits(hipStream_t st, const IndexView& ix, const EncodeView& ev, const float* unit,
                              uint64_t rows, double scale, uint16_t* digits, uint8_t* branch);
 

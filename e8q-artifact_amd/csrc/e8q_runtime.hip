@@ -108,7 +108,8 @@ struct e8q_dindex {
     IndexView view{};
     EncodeView ev{};
     float alpha = 1.0F;
-    uint64_t arena_bytes = 0;   // E8AX arena size (the allocation adds prefetch padding)
+    uint64_t arena_bytes = 0;   // E8AX arena size: records + 64-byte tail
+    uint64_t records() const { return view.batch_bytes ? (arena_bytes - 64) / view.batch_bytes : 0; }
     std::vector<uint64_t> cluster_begin, arena_begin, ref_arena_begin;
     DBuf signs, mean_rot, centroid_rot, centroid_n2, d_cluster_begin, d_arena_begin, d_ref_arena_begin, arena,
         ref_arena, ids, refine_tab, gate_mid, levels, mlev, mdig, cls_lev, cls_dig;
@@ -302,11 +303,9 @@ e8q_status e8q_index_create(e8q_ctx* ctx, const e8q_index_desc* d, e8q_dindex** 
             x->mean_rot.alloc(d->padded * sizeof(float));
             x->centroid_rot.alloc(L * d->padded * sizeof(float));
             x->centroid_n2.alloc(L * sizeof(float));
-            // Two records of tail padding: the scan prefetches whole two-record
-            // steps, so the last step of the last cluster may read one record
-            // past the arena (never used).
+            // The E8AX arena carries a 64-byte zero tail (e8q_io.cpp:35-84).
             x->arena_bytes = ab + 64;
-            x->arena.alloc(ab + 64 + 2 * static_cast<uint64_t>(v.batch_bytes));
+            x->arena.alloc(ab + 64);
             x->ref_arena.alloc(rb + 64);
             x->ids.alloc(std::max<uint64_t>(d->n, 1) * sizeof(uint32_t));
             Check(hipMemsetAsync(x->arena.p, 0, x->arena.bytes, st), "memset");
@@ -347,6 +346,8 @@ e8q_status e8q_index_upload(e8q_ctx* ctx, e8q_dindex* x, const e8q_index_arrays*
               "upload centroids");
         Check(hipMemcpyAsync(x->centroid_n2.p, h->centroid_n2, v.nlist * 4, hipMemcpyHostToDevice, st), "upload cn2");
         Check(hipMemcpyAsync(x->arena.p, h->arena, x->arena_bytes, hipMemcpyHostToDevice, st), "upload arena");
+        Check(LaunchRecordLayout(st, v, x->arena.as<uint8_t>(), x->arena.as<uint8_t>(), x->records(), true),
+              "arena layout");
         Check(hipMemcpyAsync(x->ref_arena.p, h->ref_arena, x->ref_arena.bytes, hipMemcpyHostToDevice, st),
               "upload refine arena");
         Check(hipStreamSynchronize(st), "upload");
@@ -366,7 +367,15 @@ e8q_status e8q_index_download(e8q_ctx* ctx, const e8q_dindex* x, e8q_index_array
         dl(h->mean_rot, x->mean_rot, v.padded * 4, "download mean");
         dl(h->centroid_rot, x->centroid_rot, v.nlist * v.padded * 4, "download centroids");
         dl(h->centroid_n2, x->centroid_n2, v.nlist * 4, "download cn2");
-        dl(h->arena, x->arena, x->arena_bytes, "download arena");
+        if (h->arena) {   // back to the E8AX rowA layout, through a scratch copy
+            DBuf tmp;
+            tmp.alloc(x->arena_bytes);
+            Check(hipMemcpyAsync(tmp.p, x->arena.p, x->arena_bytes, hipMemcpyDeviceToDevice, st), "copy arena");
+            Check(LaunchRecordLayout(st, v, tmp.as<uint8_t>(), tmp.as<uint8_t>(), x->records(), false),
+                  "arena layout");
+            Check(hipMemcpyAsync(h->arena, tmp.p, x->arena_bytes, hipMemcpyDeviceToHost, st), "download arena");
+            Check(hipStreamSynchronize(st), "download");
+        }
         dl(h->ref_arena, x->ref_arena, x->ref_arena.bytes, "download refine arena");
         Check(hipStreamSynchronize(st), "download");
     });
@@ -458,6 +467,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         uint8_t* lut = nullptr;
         uint32_t *pids = nullptr, *npq = nullptr, *oids = nullptr;
         unsigned long long* dstats = nullptr;
+        PlainParams* dparams = nullptr;
         e8q_ctx::Carve cv;
         if (!a->queries_on_device) cv.add(&dq, nq * v.dim);
         cv.add(&rotq, nq * P);
@@ -472,6 +482,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             cv.add(&odist, nq * a->k);
         }
         cv.add(&dstats, 4);
+        cv.add(&dparams, 1);
         ctx->Take(cv);
         const float* qsrc = a->queries;
         if (!a->queries_on_device) {
@@ -511,7 +522,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         pa.out_dists = odist;
         pa.stats = dstats;
         Check(hipEventRecord(ctx->ev[1], st), "event");
-        Check(LaunchPlainSearch(st, v, pa), "plain search");
+        Check(LaunchPlainSearch(st, v, pa, dparams), "plain search");
         Check(hipEventRecord(ctx->ev[2], st), "event");
         if (!a->outputs_on_device) {
             Check(hipMemcpyAsync(a->out_ids, oids, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");

@@ -6,14 +6,17 @@
 // FlushSurvivors (e8q_search.cpp:182-207), RefineCoreAvx2 (e8q_search_avx2.cpp:
 // 82-403), InsertHeap/ExtractTopK (e8q_search.cpp:135-180, 295-318).
 //
-// Layout per query: the probed clusters are visited in probe order; each
-// cluster's 32-vector batches are scored two at a time (64 lanes: coarse
-// integer accumulate from the LDS query LUT + the threshold-independent part
-// of the ZPR bound), then batch skip, lane mask and survivor staging against
-// the live top-k threshold, refining the staged rows whenever flush_every
-// rows are waiting and at the end of every cluster.  The top-k is a sorted
-// (dist, id) list in LDS; its k-th entry is the reference heap's root, the
-// prune threshold.
+// Work per query: the probed clusters in probe order; each cluster's
+// 32-vector batches two at a time (one per half-wave).  A step is: integer
+// coarse sums from the LDS query LUT over this lane's rowA bytes (held in
+// registers: the device scan record stores rowA vector-major, so a lane's
+// bytes are one contiguous load), the threshold-free part of the ZPR bound,
+// then the batch skip, lane mask and survivor staging against the live top-k
+// threshold.  The next step's records are loaded into a second register set
+// while a step is scored.  Staged rows are refined whenever flush_every are
+// waiting and at the end of each cluster; the top-k is a sorted (dist, id)
+// list in LDS whose k-th entry is the reference heap's root, the prune
+// threshold.
 #include "e8q_device.h"
 #include "e8q_kernels.h"
 
@@ -21,20 +24,41 @@ namespace e8qdev {
 
 constexpr int kWave = 64;       // one wave per query
 constexpr int kMaxFlush = 256;
-
-struct StagedRow {
-    uint32_t off;   // vector offset inside the current cluster
-    uint32_t id;
-    float g1;
-    float h3;       // d2 + h3 (e8q.hpp:76-79)
-};
+constexpr int kGroup = 32;      // staged rows refined and merged together
 
 __device__ __forceinline__ bool Better(float d1, uint32_t i1, float d2, uint32_t i2) {
     return d1 < d2 || (d1 == d2 && i1 < i2);
 }
 
+__device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float UniformF(float v) {
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+__device__ __forceinline__ uint32_t ReadLane(uint32_t v, uint32_t l) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(l)));
+}
+__device__ __forceinline__ uint64_t ReadLane64(uint32_t lo, uint32_t hi, uint32_t l) {
+    return (static_cast<uint64_t>(ReadLane(hi, l)) << 32) | ReadLane(lo, l);
+}
+
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ uint32_t Dpp(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), CTRL, ROWMASK, 0xF, false));
+}
+
+// Unsigned max over each 32-lane half of the wave; valid in lanes 31 and 63.
+__device__ __forceinline__ uint32_t HalfMax(uint32_t v) {
+    v = max(v, Dpp<0xB1>(v));        // quad_perm [1,0,3,2]
+    v = max(v, Dpp<0x4E>(v));        // quad_perm [2,3,0,1]
+    v = max(v, Dpp<0x124>(v));       // row_ror:4
+    v = max(v, Dpp<0x128>(v));       // row_ror:8
+    v = max(v, Dpp<0x142, 0xA>(v));  // row_bcast:15 into rows 1 and 3
+    return v;
+}
+
 // Digit `i` (0..7) and the branch of one (lane, block) from the rowA byte and
-// the (bits-1)-bit refine field (e8q_scan.cpp:212-265).
+// the (bits-1)-bit refine field (e8q_scan.cpp:34-90).
 __device__ __forceinline__ uint32_t FieldBits(uint64_t lo, uint64_t hi, int pos, int n) {
     const uint64_t m = (uint64_t{1} << n) - 1;
     if (pos + n <= 64) return static_cast<uint32_t>((lo >> pos) & m);
@@ -68,92 +92,204 @@ __device__ __forceinline__ void DecodeDigit(int bits, uint8_t rb, const uint8_t*
     }
 }
 
+// Fixed-size LDS state of one query.
 struct PlainShared {
-    float t1[kWave];             // ZPR bound minus the slack term, per lane
-    float pp[kWave];             // perp (Omega) per lane
-    int32_t amax[2], amin[2];    // batch max/min of the coarse sums
-    float td[kMaxK], nd[kMaxK];  // sorted top-k (dist, id), merge target
+    float tab[512];              // refine level table (bits <= 8)
+    float t1[kWave];             // exact replay: ZPR bound minus the slack term
+    float pp[kWave];             //               perp (Omega) per lane
+    float lbp[2], pam[2];        //               batch bound parts
+    float td[kMaxK], nd[kMaxK];  // sorted top-k (dist, id) and merge target
     uint32_t ti[kMaxK], ni[kMaxK];
-    float cd[32];
-    uint32_t ci[32];
-    float limit;
+    float cd[kGroup];            // refined rows of the group
+    uint32_t ci[kGroup];
+    float ad[kGroup];            // ... those that can enter the top-k
+    uint32_t ai[kGroup];
+    float rg1[kGroup], rh3[kGroup];
+    uint32_t rid[kGroup];
+    uint32_t staged[kMaxFlush];  // staged rows: vector offset inside the cluster
 };
 
-// Refines the staged rows, up to 32 at a time, and merges them into the
-// sorted top-k list.  The rows' code bytes (one rowA byte per block at
-// stride 32, plus the contiguous (bits-1)-byte refine fields) are gathered
-// into LDS with independent loads first; then 8 rows x 8 components per
-// pass decode and accumulate in the AVX2 order (even blocks in acc, odd in
-// acc2, then the 8-lane horizontal sum).
-__device__ void FlushRows(PlainShared& S, const StagedRow* staged, int ns, int& cnt, int k, const IndexView& ix,
-                          const float* qv, const uint8_t* arena_c, const uint8_t* ref_c, uint8_t* rowbuf,
-                          const float* tab) {
+// The query LUT first, at LDS address 0, so lookups fold the block offset
+// into the instruction.
+template <int NQW>
+struct PlainLds {
+    uint8_t lut[NQW > 0 ? NQW * 256 : 4];
+    PlainShared s;
+};
+
+// Probe metadata of 64 consecutive probes, one per lane.
+struct ProbeChunk {
+    uint32_t base;
+    uint64_t nonempty;   // lanes whose cluster has vectors
+    uint32_t cnt, vb_lo, vb_hi, ao_lo, ao_hi, ro_lo, ro_hi;
+    float d2;
+};
+
+__device__ __forceinline__ void LoadChunk(ProbeChunk& c, uint32_t base, const IndexView& ix, const PlainArgs& a,
+                                          uint64_t qi, uint32_t np) {
+    const uint32_t p = base + threadIdx.x;
+    c.base = base;
+    c.cnt = c.vb_lo = c.vb_hi = c.ao_lo = c.ao_hi = c.ro_lo = c.ro_hi = 0;
+    c.d2 = 0.0F;
+    if (p < np) {
+        const uint32_t cid = a.probe_ids[qi * a.maxp + p];
+        const uint64_t vb = ix.cluster_begin[cid];
+        const uint64_t ve = ix.cluster_begin[cid + 1];
+        const uint64_t ao = ix.arena_begin[cid];
+        const uint64_t ro = ix.ref_arena_begin[cid];
+        c.cnt = static_cast<uint32_t>(ve - vb);
+        c.vb_lo = static_cast<uint32_t>(vb);
+        c.vb_hi = static_cast<uint32_t>(vb >> 32);
+        c.ao_lo = static_cast<uint32_t>(ao);
+        c.ao_hi = static_cast<uint32_t>(ao >> 32);
+        c.ro_lo = static_cast<uint32_t>(ro);
+        c.ro_hi = static_cast<uint32_t>(ro >> 32);
+        c.d2 = a.probe_d2[qi * a.maxp + p];
+    }
+    c.nonempty = __ballot(c.cnt > 0);
+}
+
+// First non-empty probe >= from inside the loaded chunk; base + 64 (or np)
+// when there is none there.
+__device__ __forceinline__ uint32_t NextInChunk(const ProbeChunk& c, uint32_t from, uint32_t np) {
+    const uint32_t off = from - c.base;
+    const uint64_t m = off >= 64 ? 0 : (c.nonempty >> off) << off;
+    const uint32_t r = m ? c.base + static_cast<uint32_t>(__ffsll(static_cast<long long>(m))) - 1 : c.base + 64;
+    return min(r, np);
+}
+
+// First non-empty probe >= from, loading chunks as needed; np when none.
+__device__ __forceinline__ uint32_t Settle(ProbeChunk& c, uint32_t from, const IndexView& ix, const PlainArgs& a,
+                                           uint64_t qi, uint32_t np) {
+    uint32_t p = from;
+    while (p < np && p - c.base >= 64) {
+        LoadChunk(c, p, ix, a, qi, np);
+        p = NextInChunk(c, p, np);
+    }
+    return p;
+}
+
+// This lane's share of one step: its rowA bytes (NQW 8-byte words; NQW == 0:
+// any size, read in the coarse loop) and the batch header fields it uses.
+template <int NQW>
+struct Rec {
+    uint2 w[NQW > 0 ? NQW : 1];
+    const uint8_t* rp;           // NQW == 0: this lane's rowA
+    uint32_t hb, hw, hp;         // base / W / Omega bank bytes
+    float4 s0;                   // sc[0..3]
+    float3 s1;                   // sc[4..6]
+};
+
+template <int NQW>
+__device__ __forceinline__ void LoadRec(Rec<NQW>& r, const uint8_t* rec, uint32_t t, int blocks) {
+    const uint8_t* row = rec + kHdr + t * static_cast<uint32_t>(blocks);
+    if constexpr (NQW > 0) {
+        const uint2* rw = reinterpret_cast<const uint2*>(row);
+#pragma unroll
+        for (int w = 0; w < NQW; ++w) r.w[w] = rw[w];
+    } else {
+        r.rp = row;
+    }
+    r.hb = rec[t];
+    r.hw = rec[32 + t];
+    r.hp = rec[64 + t];
+    r.s0 = *reinterpret_cast<const float4*>(rec + 96);
+    r.s1 = *reinterpret_cast<const float3*>(rec + 112);
+}
+
+// Eight blocks of coarse sum: bytes of x.x are blocks 0..3, of x.y 4..7; the
+// LUT holds per block 16 low-nibble then 16 high-nibble entries.
+__device__ __forceinline__ int32_t Coarse8(uint2 x, const uint8_t* l) {
+    int32_t s = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        s += l[b * 32 + __builtin_amdgcn_ubfe(x.x, 8 * b, 4)] + l[b * 32 + 16 + __builtin_amdgcn_ubfe(x.x, 8 * b + 4, 4)];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        s += l[128 + b * 32 + __builtin_amdgcn_ubfe(x.y, 8 * b, 4)] +
+             l[128 + b * 32 + 16 + __builtin_amdgcn_ubfe(x.y, 8 * b + 4, 4)];
+    return s;
+}
+
+template <int NQW>
+__device__ __forceinline__ int32_t Coarse(const Rec<NQW>& r, const uint8_t* lut, int blocks) {
+    int32_t acc = 0;
+    if constexpr (NQW > 0) {
+#pragma unroll
+        for (int w = 0; w < NQW; ++w) acc += Coarse8(r.w[w], lut + w * 256);
+    } else {
+        const uint2* rw = reinterpret_cast<const uint2*>(r.rp);
+        const int nw = blocks / 8;
+        for (int w = 0; w < nw; ++w) acc += Coarse8(rw[w], lut + w * 256);
+    }
+    return acc;
+}
+
+// The open cluster, for the refine.
+struct ClusterRef {
+    const uint8_t* arena_c;
+    const uint8_t* ref_c;
+    uint64_t vb;
+    float d2;
+};
+
+// Refines the staged rows, kGroup at a time, and merges them into the
+// sorted top-k list.  The rows' code bytes (rowA bytes and the
+// (bits-1)-byte refine fields) are gathered into LDS with independent loads;
+// then 8 rows x 8 components per pass decode and accumulate in the AVX2
+// order (even blocks in acc, odd in acc2, then the 8-lane horizontal sum).
+// Only rows that beat the current k-th entry take part in the merge.
+__device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, int k, const IndexView& ix,
+                          const float* qv, const ClusterRef& cr, uint8_t* rowbuf, const float* tab) {
     const int lane = threadIdx.x;
     const int B = ix.blocks;
     const int c = ix.bits - 1;
     const int fbytes = ix.bits >= 2 ? B * c : 0;
-    const int rstride = B + fbytes;
-    for (int rb = 0; rb < ns; rb += 32) {
+    const int rstride = B + fbytes;        // multiple of 8
+    const int wpr = rstride / 8, bw = B / 8;
+    for (int rb = 0; rb < ns; rb += kGroup) {
+        const int nn = min(kGroup, ns - rb);
         __syncthreads();
-        const int nn = min(32, ns - rb);
-        // Loads are issued four at a time per lane before their LDS stores so
-        // several gathers are in flight.
-        const int na = nn * B;
-        for (int e0 = lane; e0 < na; e0 += 4 * kWave) {
-            uint32_t v[4];
+        // Word e of the group: row e % 32, word e / 32 (rowA words, then
+        // refine-field words); four loads in flight per lane.
+        const int nw = kGroup * wpr;
+        for (int e0 = lane; e0 < nw; e0 += 4 * kWave) {
+            uint2 v[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int e = e0 + u * kWave;
-                v[u] = 0;
-                if (e < na) {
-                    const int r = e / B, blk = e - r * B;
-                    const uint32_t off = staged[rb + r].off;
-                    v[u] = arena_c[static_cast<uint64_t>(off >> 5) * ix.batch_bytes + kHdr + blk * 32 + (off & 31)];
+                const int r = e & (kGroup - 1), w = e >> 5;
+                v[u] = make_uint2(0, 0);
+                if (e < nw && r < nn) {
+                    const uint32_t off = S.staged[rb + r];
+                    const uint8_t* src =
+                        w < bw ? cr.arena_c + static_cast<uint64_t>(off >> 5) * ix.batch_bytes + kHdr + (off & 31) * B +
+                                     8 * w
+                               : cr.ref_c + static_cast<uint64_t>(off >> 5) * ix.ref_batch_bytes +
+                                     (off & 31) * ix.refine_stride + 8 * (w - bw);
+                    v[u] = *reinterpret_cast<const uint2*>(src);
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int e = e0 + u * kWave;
-                if (e < na) {
-                    const int r = e / B, blk = e - r * B;
-                    rowbuf[r * rstride + blk] = static_cast<uint8_t>(v[u]);
-                }
+                const int r = e & (kGroup - 1), w = e >> 5;
+                if (e < nw && r < nn) *reinterpret_cast<uint2*>(rowbuf + r * rstride + 8 * w) = v[u];
             }
         }
-        if (fbytes > 0) {
-            const int w8 = fbytes / 8;   // blocks is a multiple of 8: whole 8-byte words
-            const int nw = nn * w8;
-            for (int e0 = lane; e0 < nw; e0 += 4 * kWave) {
-                uint2 v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int e = e0 + u * kWave;
-                    v[u] = make_uint2(0, 0);
-                    if (e < nw) {
-                        const int r = e / w8, w = e - r * w8;
-                        const uint32_t off = staged[rb + r].off;
-                        const uint8_t* src = ref_c + static_cast<uint64_t>(off >> 5) * ix.ref_batch_bytes +
-                                             (off & 31) * ix.refine_stride;
-                        v[u] = reinterpret_cast<const uint2*>(src)[w];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int e = e0 + u * kWave;
-                    if (e < nw) {
-                        const int r = e / w8, w = e - r * w8;
-                        uint2* dst = reinterpret_cast<uint2*>(rowbuf + r * rstride + B + w * 8);
-                        *dst = v[u];
-                    }
-                }
-            }
+        if (lane < nn) {
+            const uint32_t off = S.staged[rb + lane];
+            const float* g1b = reinterpret_cast<const float*>(
+                cr.ref_c + static_cast<uint64_t>(off >> 5) * ix.ref_batch_bytes + ix.refine_stride * 32);
+            S.rg1[lane] = g1b[off & 31];
+            S.rh3[lane] = cr.d2 + g1b[32 + (off & 31)];
+            S.rid[lane] = ix.ids[cr.vb + off];
         }
         __syncthreads();
         for (int sub = 0; sub < nn; sub += 8) {
             const int r = sub + (lane >> 3);
             const int i = lane & 7;
             if (r < nn) {
-                const StagedRow row = staged[rb + r];
                 const uint8_t* plane = rowbuf + r * rstride;
                 const uint8_t* field = plane + B;
                 float acc = 0.0F, acc2 = 0.0F;
@@ -168,333 +304,285 @@ __device__ void FlushRows(PlainShared& S, const StagedRow* staged, int ns, int& 
                 }
                 const float sc = Hsum8Lanes(acc + acc2);
                 if (i == 0) {
-                    S.cd[r] = fmaf(-(2.0F * row.g1), sc, row.h3);
-                    S.ci[r] = row.id;
+                    S.cd[r] = fmaf(-(2.0F * S.rg1[r]), sc, S.rh3[r]);
+                    S.ci[r] = S.rid[r];
                 }
             }
         }
         __syncthreads();
-        // Merge: each list entry moves down by the number of new rows better
-        // than it; each new row lands at (# list entries better) + (# new
-        // rows better).  (dist, id) is a strict total order (ids unique).
-        for (int e = lane; e < cnt; e += kWave) {
-            const float d = S.td[e];
-            const uint32_t id = S.ti[e];
-            int rank = e;
-#pragma unroll 8
-            for (int j = 0; j < nn; ++j) rank += Better(S.cd[j], S.ci[j], d, id) ? 1 : 0;
-            if (rank < k) {
-                S.nd[rank] = d;
-                S.ni[rank] = id;
-            }
-        }
+        // Rows that cannot enter the top-k (not better than a full list's
+        // k-th entry) are dropped; the rest are ranked against the list and
+        // each other ((dist, id) is a strict total order: ids are unique).
+        bool take = false;
+        float d = 0.0F;
+        uint32_t id = 0;
         if (lane < nn) {
-            const float d = S.cd[lane];
-            const uint32_t id = S.ci[lane];
-            int lo = 0, hi = cnt;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (Better(S.td[mid], S.ti[mid], d, id)) lo = mid + 1;
-                else hi = mid;
-            }
-            int rank = lo;
-#pragma unroll 8
-            for (int j = 0; j < nn; ++j) rank += Better(S.cd[j], S.ci[j], d, id) ? 1 : 0;
-            if (rank < k) {
-                S.nd[rank] = d;
-                S.ni[rank] = id;
-            }
+            d = S.cd[lane];
+            id = S.ci[lane];
+            take = cnt < k || Better(d, id, S.td[k - 1], S.ti[k - 1]);
         }
-        __syncthreads();
-        cnt = min(k, cnt + nn);
-        for (int e = lane; e < cnt; e += kWave) {
-            S.td[e] = S.nd[e];
-            S.ti[e] = S.ni[e];
+        const uint64_t am = __ballot(take);
+        const int m = __popcll(am);
+        if (m > 0) {
+            if (take) {
+                const int pos = __popcll(am & ((uint64_t{1} << lane) - 1));
+                S.ad[pos] = d;
+                S.ai[pos] = id;
+            }
+            __syncthreads();
+            for (int e = lane; e < cnt; e += kWave) {
+                const float ed = S.td[e];
+                const uint32_t ei = S.ti[e];
+                int rank = e;
+                for (int q = 0; q < m; ++q) rank += Better(S.ad[q], S.ai[q], ed, ei) ? 1 : 0;
+                if (rank < k) {
+                    S.nd[rank] = ed;
+                    S.ni[rank] = ei;
+                }
+            }
+            if (lane < m) {
+                const float nd0 = S.ad[lane];
+                const uint32_t ni0 = S.ai[lane];
+                int lo = 0, hi = cnt;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (Better(S.td[mid], S.ti[mid], nd0, ni0)) lo = mid + 1;
+                    else hi = mid;
+                }
+                int rank = lo;
+                for (int q = 0; q < m; ++q) rank += Better(S.ad[q], S.ai[q], nd0, ni0) ? 1 : 0;
+                if (rank < k) {
+                    S.nd[rank] = nd0;
+                    S.ni[rank] = ni0;
+                }
+            }
+            __syncthreads();
+            cnt = min(k, cnt + m);
+            for (int e = lane; e < cnt; e += kWave) {
+                S.td[e] = S.nd[e];
+                S.ti[e] = S.ni[e];
+            }
+            __syncthreads();
+            limit = cnt == k ? UniformF(S.td[k - 1]) : __builtin_inff();
         }
-        __syncthreads();
-        if (lane == 0) S.limit = cnt == k ? S.td[k - 1] : __builtin_inff();
     }
     __syncthreads();
 }
 
-__device__ __forceinline__ StagedRow MakeRow(const IndexView& ix, const uint8_t* ref_c, uint64_t vb, uint32_t bj,
-                                             uint32_t t, float d2) {
-    const uint32_t off = bj * 32;
-    const float* g1b =
-        reinterpret_cast<const float*>(ref_c + static_cast<uint64_t>(bj) * ix.ref_batch_bytes + ix.refine_stride * 32);
-    return StagedRow{off + t, ix.ids[vb + off + t], g1b[t], d2 + g1b[32 + t]};
-}
-
-// ---- record prefetch ------------------------------------------------------
-typedef __attribute__((address_space(3))) void* LdsPtr;
-typedef __attribute__((address_space(1))) void* GblPtr;
-
-// Steps of prefetch kept in flight: ~12 KB of records per wave, 1..3 steps.
-__host__ __device__ inline int PrefetchDepth(const IndexView& ix) {
-    const int d = static_cast<int>(12288 / (2 * ix.batch_bytes));
-    return d < 1 ? 1 : (d > 3 ? 3 : d);
-}
-
-// Copies `bytes` (a multiple of 64) global -> LDS with direct-to-LDS loads:
-// one instruction per kilobyte, lane i moving bytes [16i, 16i+16) of it.
-// Every instruction has at least one active lane, so each call issues exactly
-// ceil(bytes / 1024) loads (the count the vmcnt waits rely on).
-__device__ __forceinline__ void CopyToLdsAsync(const uint8_t* src, uint8_t* dst, uint32_t bytes, int lane) {
-    for (uint32_t off = 0; off < bytes; off += 1024) {
-        const uint32_t o = off + static_cast<uint32_t>(lane) * 16;
-        if (o < bytes)
-            __builtin_amdgcn_global_load_lds((GblPtr)(src + o), (LdsPtr)(dst + off), 16, 0, 0);
-    }
-}
-
-// s_waitcnt on vmcnt only (expcnt / lgkmcnt fields at their no-wait maxima).
-template <int N>
-__device__ __forceinline__ void WaitVmN() {
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-
-// Waits until at most n vector-memory operations are outstanding; values
-// beyond the table wait for all (always safe, only slower).
-__device__ __forceinline__ void WaitVm(int n) {
-    switch (n) {
-        case 1: WaitVmN<1>(); break;
-        case 2: WaitVmN<2>(); break;
-        case 3: WaitVmN<3>(); break;
-        case 4: WaitVmN<4>(); break;
-        case 5: WaitVmN<5>(); break;
-        case 6: WaitVmN<6>(); break;
-        case 7: WaitVmN<7>(); break;
-        case 8: WaitVmN<8>(); break;
-        case 9: WaitVmN<9>(); break;
-        case 10: WaitVmN<10>(); break;
-        case 11: WaitVmN<11>(); break;
-        case 12: WaitVmN<12>(); break;
-        default: WaitVmN<0>(); break;
-    }
-}
-
-// Producer side of the step sequence: the same walk as the consumer loop
-// (probes in order, empty clusters skipped, two batches per step).
-struct StepSeq {
-    uint32_t p;              // next probe to open
-    uint32_t b;              // next batch in the open cluster
-    uint32_t nb;             // batches of the open cluster
-    const uint8_t* arena_p;  // its scan records
-};
-
-__device__ __forceinline__ const uint8_t* NextStep(StepSeq& s, const IndexView& ix, const PlainArgs& a, uint64_t qi,
-                                                   uint32_t np) {
-    while (s.b >= s.nb) {
-        if (s.p >= np) return nullptr;
-        const uint32_t cid = a.probe_ids[qi * a.maxp + s.p];
-        const uint64_t cnt = ix.cluster_begin[cid + 1] - ix.cluster_begin[cid];
-        s.nb = static_cast<uint32_t>((cnt + 31) / 32);
-        s.arena_p = ix.arena + ix.arena_begin[cid];
-        s.b = 0;
-        ++s.p;
-    }
-    const uint8_t* src = s.arena_p + static_cast<uint64_t>(s.b) * ix.batch_bytes;
-    s.b += 2;
-    return src;
-}
-
 // One wave per query.  Each step covers two 32-vector batches (lanes 0..31
-// and 32..63): coarse integer sums from the query LUT in LDS, the
-// threshold-free part of the ZPR bound, and a speculative evaluation of both
-// batches against the threshold at the step's start.  When the survivors of
-// the step fit the staging area without a flush, those decisions are the
-// reference's decisions (the threshold cannot move before the next flush)
-// and they are applied at once; otherwise the step is replayed batch by
-// batch, lane by lane, exactly as CollectLanes does.
-__global__ __launch_bounds__(kWave) void PlainSearchKernel(IndexView ix, PlainArgs a) {
-    __shared__ PlainShared S;
-    extern __shared__ float dyn[];
-    float* qv = dyn;                                                    // padded floats
-    uint8_t* lutc = reinterpret_cast<uint8_t*>(dyn + ix.padded);        // blocks*32 bytes
-    float* ltab = reinterpret_cast<float*>(lutc + static_cast<size_t>(ix.blocks) * 32);  // level table (b <= 8)
-    StagedRow* staged_rows = reinterpret_cast<StagedRow*>(ltab + (ix.bits <= 8 ? 512 : 0));
-    uint8_t* rowbuf = reinterpret_cast<uint8_t*>(staged_rows + a.flush_every);
-    uint8_t* ring = rowbuf + ((32 * static_cast<size_t>(ix.blocks) * ix.bits + 15) & ~size_t{15});
-    const uint32_t step_bytes = 2 * ix.batch_bytes;   // one step = two scan records
-    const int depth = PrefetchDepth(ix);
-    const uint32_t nring = static_cast<uint32_t>(depth) + 1;
-    const int ni = static_cast<int>((step_bytes + 1023) / 1024);   // load instructions per step
+// and 32..63): coarse integer sums, the threshold-free part of the ZPR
+// bound, and a speculative evaluation of both batches against the threshold
+// at the step's start.  When the step's survivors fit the staging area
+// without a flush, those decisions are the reference's decisions (the
+// threshold cannot move before the next flush) and they are applied at
+// once; otherwise the step is replayed batch by batch, lane by lane, exactly
+// as CollectLanes does.
+template <int NQW>
+__global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __restrict__ prm) {
+    const IndexView& ix = prm->ix;
+    const PlainArgs& a = prm->a;
+    __shared__ PlainLds<NQW> sh;
+    PlainShared& S = sh.s;
+    extern __shared__ uint8_t dyn[];
+    uint8_t* lutc = NQW > 0 ? sh.lut : dyn;
+    float* qv = reinterpret_cast<float*>(dyn + (NQW > 0 ? 0 : static_cast<size_t>(ix.blocks) * 32));  // padded
+    uint8_t* rowbuf = reinterpret_cast<uint8_t*>(qv + ix.padded);                                     // kGroup rows
     const int lane = threadIdx.x;
     if (ix.bits <= 8)
-        for (int i = lane; i < 512; i += kWave) ltab[i] = ix.refine_tab[i];
-    const float* tab = ix.bits <= 8 ? ltab : ix.refine_tab;
+        for (int i = lane; i < 512; i += kWave) S.tab[i] = ix.refine_tab[i];
+    const float* tab = ix.bits <= 8 ? S.tab : ix.refine_tab;
     const uint32_t j = lane >> 5, t = lane & 31;
     const int k = static_cast<int>(a.k);
     const int fe = a.flush_every;
     const float inf = __builtin_inff();
     const uint32_t P = static_cast<uint32_t>(ix.padded);
     const uint32_t E = static_cast<uint32_t>(ix.blocks) * 32;
+    const uint32_t BB = ix.batch_bytes;
+    const float z = a.z;
+    const bool prune = !a.disable_prune;
     for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
         for (uint32_t i = lane; i < P; i += kWave) qv[i] = a.rotq[qi * P + i];
-        for (uint32_t i = lane; i < E; i += kWave) lutc[i] = a.lut[qi * E + i];
-        if (lane == 0) S.limit = inf;
-        __syncthreads();
+        for (uint32_t i = lane; 4 * i < E; i += kWave)
+            reinterpret_cast<uint32_t*>(lutc)[i] = reinterpret_cast<const uint32_t*>(a.lut + qi * E)[i];
         const float* cst = a.consts + qi * 8;
-        const float delta = cst[0], base = cst[1], lut_err = cst[2], score_max = cst[3];
+        const float delta = UniformF(cst[0]), base = UniformF(cst[1]);
+        const float lut_err = UniformF(cst[2]);
+        const float sml = UniformF(cst[3]) + lut_err;   // score_max + lut_err
+        const uint32_t np = min(static_cast<uint32_t>(ix.nlist), a.nprobe_q ? Uniform(a.nprobe_q[qi]) : a.nprobe_all);
+        __syncthreads();
         int cnt = 0, staged = 0;
-        uint64_t n_cand = 0, n_surv = 0, n_batch = 0;
-        const uint32_t np = min(static_cast<uint32_t>(ix.nlist), a.nprobe_q ? a.nprobe_q[qi] : a.nprobe_all);
-        // Prefetch ring: the records of the next `depth` steps stream into LDS
-        // while the current step is scored.
-        StepSeq seq{0, 0, 0, nullptr};
-        uint32_t issued = 0, consumed = 0;
-        for (int d = 0; d < depth; ++d) {
-            const uint8_t* src = NextStep(seq, ix, a, qi, np);
-            if (!src) break;
-            CopyToLdsAsync(src, ring + (issued % nring) * step_bytes, step_bytes, lane);
-            ++issued;
+        float limit = inf;
+        uint32_t n_cand = 0, n_surv = 0, n_batch = 0;
+        ProbeChunk ch;
+        LoadChunk(ch, 0, ix, a, qi, np);
+        uint32_t cp = Settle(ch, NextInChunk(ch, 0, np), ix, a, qi, np);
+        // The open cluster.
+        uint32_t ccnt = 0, cnb = 0, cb0 = 0;
+        float d2 = 0.0F, sqrt_d2 = 0.0F, zrh = 0.0F;
+        const uint8_t* arena_c = nullptr;
+        auto open = [&](uint32_t p) {
+            const uint32_t l = p - ch.base;
+            ccnt = ReadLane(ch.cnt, l);
+            cnb = (ccnt + 31) / 32;
+            d2 = __uint_as_float(ReadLane(__float_as_uint(ch.d2), l));
+            sqrt_d2 = sqrtf(fmaxf(0.0F, d2));
+            arena_c = ix.arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
+        };
+        // z * r_horizon, r_horizon = min(sqrt(d2), sqrt(limit)) once the heap is live.
+        auto horizon = [&]() { zrh = z * (limit < d2 ? sqrtf(fmaxf(0.0F, limit)) : sqrt_d2); };
+        auto cluster_ref = [&]() {
+            const uint32_t l = cp - ch.base;
+            return ClusterRef{arena_c, ix.ref_arena + ReadLane64(ch.ro_lo, ch.ro_hi, l), ReadLane64(ch.vb_lo, ch.vb_hi, l),
+                              d2};
+        };
+        Rec<NQW> rn;
+        if (cp < np) {
+            open(cp);
+            horizon();
+            if (j < min(2u, cnb)) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(j) * BB, t, ix.blocks);
         }
-        for (uint32_t p = 0; p < np; ++p) {
-            const uint32_t cid = a.probe_ids[qi * a.maxp + p];
-            const float d2 = a.probe_d2[qi * a.maxp + p];
-            const uint64_t vb = ix.cluster_begin[cid];
-            const uint32_t ccount = static_cast<uint32_t>(ix.cluster_begin[cid + 1] - vb);
-            if (ccount == 0) continue;
-            const float sqrt_d2 = sqrtf(fmaxf(0.0F, d2));
-            const uint32_t nb = (ccount + 31) / 32;
-            const uint8_t* arena_c = ix.arena + ix.arena_begin[cid];
-            const uint8_t* ref_c = ix.ref_arena + ix.ref_arena_begin[cid];
-            for (uint32_t b0 = 0; b0 < nb; b0 += 2) {
-                const uint32_t jn = min(2u, nb - b0);
-                const uint32_t bj = b0 + j;
-                // This step's records were requested `issued - consumed - 1`
-                // steps of loads ago: wait for them, then request the step
-                // `depth` ahead into the slot the previous step just freed.
-                WaitVm(ni * static_cast<int>(issued - consumed - 1));
-                __syncthreads();
-                const uint8_t* recs = ring + (consumed % nring) * step_bytes;
-                if (const uint8_t* src = NextStep(seq, ix, a, qi, np)) {
-                    CopyToLdsAsync(src, ring + (issued % nring) * step_bytes, step_bytes, lane);
-                    ++issued;
-                }
-                ++consumed;
-                const float L = S.limit;
-                bool pass = false;
-                if (j < jn) {
-                    const uint8_t* rec = recs + j * ix.batch_bytes;
-                    int32_t acc = 0;
-#pragma unroll 8
-                    for (int blk = 0; blk < ix.blocks; ++blk) {
-                        const uint32_t h = rec[kHdr + blk * 32 + t];
-                        acc += lutc[blk * 32 + (h & 15)] + lutc[blk * 32 + 16 + (h >> 4)];
-                    }
-                    const float* sc = reinterpret_cast<const float*>(rec + 96);
-                    const float pad_w_round = (0.5F * sc[3]) * (score_max + lut_err);
-                    const float vbase = (sc[0] + d2) - pad_w_round;
-                    const float scv = fmaf(static_cast<float>(acc), delta, base);
-                    const float bt = fmaf(static_cast<float>(rec[t]), sc[1], vbase);
-                    const float w = fmaf(static_cast<float>(rec[32 + t]), sc[3], sc[2]);
-                    const float pp = fmaf(static_cast<float>(rec[64 + t]), sc[5], sc[4]);
-                    const float dc = fmaf(-w, scv, bt);
-                    const float t1 = fmaf(-fabsf(w), lut_err, dc);
-                    S.t1[lane] = t1;
-                    S.pp[lane] = pp;
-                    int32_t mx = acc, mn = acc;
-                    for (int o = 16; o > 0; o >>= 1) {
-                        mx = max(mx, __shfl_xor(mx, o, 32));
-                        mn = min(mn, __shfl_xor(mn, o, 32));
-                    }
-                    if (t == 0) {
-                        S.amax[j] = mx;
-                        S.amin[j] = mn;
-                    }
-                    const float rh = L < d2 ? sqrtf(fmaxf(0.0F, L)) : sqrt_d2;
-                    bool skip = false;
-                    if (!a.disable_prune && L < inf) {
-                        const float pad_add = fmaf(lut_err, sc[6], pad_w_round);
-                        const float perp_amax = fmaf(sc[5], 255.0F, sc[4]);
-                        const float s_hi = fmaf(static_cast<float>(mx), delta, base);
-                        const float s_lo = fmaf(static_cast<float>(mn), delta, base);
-                        const float sa = fabsf(s_hi) < fabsf(s_lo) ? fabsf(s_lo) : fabsf(s_hi);
-                        const float lb = fmaf(-perp_amax, rh * a.z, fmaf(-sc[6], sa, (sc[0] + d2) - pad_add));
-                        skip = lb >= L;
-                    }
-                    const uint32_t n_logic = min(32u, ccount - bj * 32);
-                    const float dlo = fmaf(-(a.z * rh), pp, t1);
-                    pass = !skip && t < n_logic && (a.disable_prune || dlo < L);
-                }
-                const uint64_t bal = __ballot(pass);
-                const uint32_t pm0 = static_cast<uint32_t>(bal), pm1 = static_cast<uint32_t>(bal >> 32);
-                const int total = __popc(pm0) + __popc(pm1);
-                if (staged + total <= fe) {
-                    // No flush inside the step: apply the speculative decisions.
-                    if (pass) {
-                        const uint32_t m = j ? pm1 : pm0;
-                        const int rank = staged + (j ? __popc(pm0) : 0) + __popc(m & ((1u << t) - 1u));
-                        staged_rows[rank] = MakeRow(ix, ref_c, vb, bj, t, d2);
-                    }
-                    staged += total;
-                    n_surv += total;
-                    n_batch += jn;
-                    n_cand += min(ccount - b0 * 32, jn * 32);
-                } else {
-                    __syncthreads();
-                    // Exact replay, batch by batch (a flush lands inside this step).
-                    for (uint32_t jj = 0; jj < jn; ++jj) {
-                        const uint32_t bb = b0 + jj, off = bb * 32;
-                        const uint32_t n_logic = min(32u, ccount - off);
-                        n_cand += n_logic;
-                        n_batch += 1;
-                        const float limit0 = S.limit;
-                        const float rh = limit0 < d2 ? sqrtf(fmaxf(0.0F, limit0)) : sqrt_d2;
-                        if (!a.disable_prune && limit0 < inf) {
-                            const float* sc = reinterpret_cast<const float*>(recs + jj * ix.batch_bytes + 96);
-                            const float pad_w_round = (0.5F * sc[3]) * (score_max + lut_err);
-                            const float pad_add = fmaf(lut_err, sc[6], pad_w_round);
-                            const float perp_amax = fmaf(sc[5], 255.0F, sc[4]);
-                            const float s_hi = fmaf(static_cast<float>(S.amax[jj]), delta, base);
-                            const float s_lo = fmaf(static_cast<float>(S.amin[jj]), delta, base);
-                            const float sa = fabsf(s_hi) < fabsf(s_lo) ? fabsf(s_lo) : fabsf(s_hi);
-                            const float lb =
-                                fmaf(-perp_amax, rh * a.z, fmaf(-sc[6], sa, (sc[0] + d2) - pad_add));
-                            if (lb >= limit0) continue;
-                        }
-                        const float slack = a.z * rh;
-                        const bool in32 = lane < 32;
-                        const float dlo = in32 ? fmaf(-slack, S.pp[jj * 32 + lane], S.t1[jj * 32 + lane]) : inf;
-                        const uint32_t lmask = n_logic >= 32 ? 0xFFFFFFFFu : ((1u << n_logic) - 1u);
-                        uint32_t mask = a.disable_prune
-                                            ? lmask
-                                            : (static_cast<uint32_t>(__ballot(in32 && dlo < limit0)) & lmask);
-                        while (mask != 0) {
-                            const float Lc = S.limit;
-                            const uint32_t ps =
-                                a.disable_prune ? mask : (mask & static_cast<uint32_t>(__ballot(in32 && dlo < Lc)));
-                            if (ps == 0) break;
-                            const int room = fe - staged;
-                            const int pc = __popc(ps);
-                            const int take = pc <= room ? pc : room;
-                            if (in32 && ((ps >> lane) & 1u)) {
-                                const int rank = __popc(ps & ((1u << lane) - 1u));
-                                if (rank < take) staged_rows[staged + rank] = MakeRow(ix, ref_c, vb, bb, lane, d2);
-                            }
-                            staged += take;
-                            n_surv += take;
-                            if (pc <= room) break;
-                            // The next passing lane meets a full staging area: flush,
-                            // then stage it without a re-test (CollectLanes' order).
-                            uint32_t rest = ps;
-                            for (int q = 0; q < take; ++q) rest &= rest - 1;
-                            const int pl = __ffs(rest) - 1;
-                            FlushRows(S, staged_rows, staged, cnt, k, ix, qv, arena_c, ref_c, rowbuf, tab);
-                            if (lane == pl) staged_rows[0] = MakeRow(ix, ref_c, vb, bb, pl, d2);
-                            staged = 1;
-                            n_surv += 1;
-                            mask &= (pl >= 31) ? 0u : ~((2u << pl) - 1u);
-                        }
-                    }
-                }
-                __syncthreads();
+        while (cp < np) {
+            // The next step; its records are loaded now when its probe is in
+            // the loaded chunk (otherwise after this step, see below).
+            uint32_t xp = cp, xb0 = cb0 + 2;
+            if (xb0 >= cnb) {
+                xp = NextInChunk(ch, cp + 1, np);
+                xb0 = 0;
             }
-            if (staged > 0) {
-                FlushRows(S, staged_rows, staged, cnt, k, ix, qv, arena_c, ref_c, rowbuf, tab);
+            const bool x_in = xp < np && xp - ch.base < 64;
+            const Rec<NQW> rc = rn;
+            if (x_in) {
+                const uint8_t* xa = arena_c;
+                uint32_t xnb = cnb;
+                if (xp != cp) {
+                    const uint32_t l = xp - ch.base;
+                    xa = ix.arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
+                    xnb = (ReadLane(ch.cnt, l) + 31) / 32;
+                }
+                if (j < min(2u, xnb - xb0)) LoadRec<NQW>(rn, xa + static_cast<uint64_t>(xb0 + j) * BB, t, ix.blocks);
+            }
+
+            const uint32_t jn = min(2u, cnb - cb0);
+            const uint32_t bj = cb0 + j;
+            const bool act = j < jn;
+            const float L = limit;
+            float t1 = inf, pp = 0.0F, lbp = 0.0F, pam = 0.0F;
+            uint32_t sabs = 0;
+            if (act) {
+                const int32_t acc = Coarse<NQW>(rc, lutc, ix.blocks);
+                const float sc0 = rc.s0.x, sc1 = rc.s0.y, sc2 = rc.s0.z, sc3 = rc.s0.w;
+                const float sc4 = rc.s1.x, sc5 = rc.s1.y, sc6 = rc.s1.z;
+                const float pad_w_round = (0.5F * sc3) * sml;
+                const float vbase = (sc0 + d2) - pad_w_round;
+                const float scv = fmaf(static_cast<float>(acc), delta, base);
+                const float bt = fmaf(static_cast<float>(rc.hb), sc1, vbase);
+                const float w = fmaf(static_cast<float>(rc.hw), sc3, sc2);
+                pp = fmaf(static_cast<float>(rc.hp), sc5, sc4);
+                const float dc = fmaf(-w, scv, bt);
+                t1 = fmaf(-fabsf(w), lut_err, dc);
+                // Batch bound: (sc0 + d2 - pad_add) - |W|max * max|S| - Omega_max * z * r_h.
+                const float pad_add = fmaf(lut_err, sc6, pad_w_round);
+                lbp = (sc0 + d2) - pad_add;
+                pam = fmaf(sc5, 255.0F, sc4);
+                sabs = __float_as_uint(fabsf(scv));
+            }
+            // Batch max |S| over its 32 lanes: S is monotone in the integer
+            // sum, so this is max(|S(max sum)|, |S(min sum)|), the
+            // reference's batch bound input (non-negative floats order as
+            // their bit patterns).
+            const uint32_t hm = HalfMax(sabs);
+            const float sa = __uint_as_float(j ? ReadLane(hm, 63) : ReadLane(hm, 31));
+            if (act) lbp = fmaf(-rc.s1.z, sa, lbp);
+            bool skip = false;
+            if (prune && L < inf) skip = fmaf(-pam, zrh, lbp) >= L;
+            const uint32_t n_logic = act ? min(32u, ccnt - bj * 32) : 0u;
+            const float dlo = fmaf(-zrh, pp, t1);
+            const bool pass = act && !skip && t < n_logic && (!prune || dlo < L);
+            const uint64_t bal = __ballot(pass);
+            const uint32_t pm0 = static_cast<uint32_t>(bal), pm1 = static_cast<uint32_t>(bal >> 32);
+            const int total = __popc(pm0) + __popc(pm1);
+            if (staged + total <= fe) {
+                // No flush inside the step: apply the speculative decisions.
+                if (pass) {
+                    const uint32_t m = j ? pm1 : pm0;
+                    const int rank = staged + (j ? __popc(pm0) : 0) + __popc(m & ((1u << t) - 1u));
+                    S.staged[rank] = bj * 32 + t;
+                }
+                staged += total;
+                n_surv += total;
+                n_batch += jn;
+                n_cand += min(ccnt - cb0 * 32, jn * 32);
+            } else {
+                // Exact replay, batch by batch (a flush lands inside this step).
+                S.t1[lane] = t1;
+                S.pp[lane] = pp;
+                if (t == 0) {
+                    S.lbp[j] = lbp;
+                    S.pam[j] = pam;
+                }
+                __syncthreads();
+                const ClusterRef cr = cluster_ref();
+                for (uint32_t jj = 0; jj < jn; ++jj) {
+                    const uint32_t off = (cb0 + jj) * 32;
+                    const uint32_t nl = min(32u, ccnt - off);
+                    n_cand += nl;
+                    n_batch += 1;
+                    const float limit0 = limit;
+                    const float zrh0 = z * (limit0 < d2 ? sqrtf(fmaxf(0.0F, limit0)) : sqrt_d2);
+                    if (prune && limit0 < inf) {
+                        if (fmaf(-S.pam[jj], zrh0, S.lbp[jj]) >= limit0) continue;
+                    }
+                    const bool in32 = lane < 32;
+                    const float dl = in32 ? fmaf(-zrh0, S.pp[jj * 32 + lane], S.t1[jj * 32 + lane]) : inf;
+                    const uint32_t lmask = nl >= 32 ? 0xFFFFFFFFu : ((1u << nl) - 1u);
+                    uint32_t mask = prune ? (static_cast<uint32_t>(__ballot(in32 && dl < limit0)) & lmask) : lmask;
+                    while (mask != 0) {
+                        const float Lc = limit;
+                        const uint32_t ps = prune ? (mask & static_cast<uint32_t>(__ballot(in32 && dl < Lc))) : mask;
+                        if (ps == 0) break;
+                        const int room = fe - staged;
+                        const int pc = __popc(ps);
+                        const int tk = pc <= room ? pc : room;
+                        if (in32 && ((ps >> lane) & 1u)) {
+                            const int rank = __popc(ps & ((1u << lane) - 1u));
+                            if (rank < tk) S.staged[staged + rank] = off + lane;
+                        }
+                        staged += tk;
+                        n_surv += tk;
+                        if (pc <= room) break;
+                        // The next passing lane meets a full staging area: flush,
+                        // then stage it without a re-test (CollectLanes' order).
+                        uint32_t rest = ps;
+                        for (int q = 0; q < tk; ++q) rest &= rest - 1;
+                        const int pl = __ffs(rest) - 1;
+                        FlushRows(S, staged, cnt, limit, k, ix, qv, cr, rowbuf, tab);
+                        if (lane == 0) S.staged[0] = off + static_cast<uint32_t>(pl);
+                        staged = 1;
+                        n_surv += 1;
+                        mask &= (pl >= 31) ? 0u : ~((2u << pl) - 1u);
+                    }
+                }
+                __syncthreads();
+                horizon();
+            }
+            if (cb0 + 2 >= cnb && staged > 0) {   // end of the cluster
+                __syncthreads();
+                FlushRows(S, staged, cnt, limit, k, ix, qv, cluster_ref(), rowbuf, tab);
                 staged = 0;
+                horizon();
             }
+            if (xp >= np) break;
+            if (!x_in) {   // the next probe lies beyond the loaded chunk
+                xp = Settle(ch, xp, ix, a, qi, np);
+                if (xp >= np) break;
+            }
+            if (xp != cp) {
+                open(xp);
+                horizon();
+            }
+            if (!x_in && j < min(2u, cnb)) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(j) * BB, t, ix.blocks);
+            cp = xp;
+            cb0 = xb0;
         }
         for (int i = lane; i < k; i += kWave) {
             a.out_ids[qi * a.k + i] = i < cnt ? S.ti[i] : 0xFFFFFFFFu;
@@ -509,30 +597,45 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(IndexView ix, PlainAr
     }
 }
 
-size_t PlainSearchLds(const IndexView& ix, int flush_every) {
-    return ix.padded * sizeof(float) + static_cast<size_t>(ix.blocks) * 32 + (ix.bits <= 8 ? 512 * sizeof(float) : 0) +
-           static_cast<size_t>(flush_every) * sizeof(StagedRow) +
-           ((32 * static_cast<size_t>(ix.blocks) * static_cast<size_t>(ix.bits) + 15) & ~size_t{15}) +
-           (PrefetchDepth(ix) + 1) * 2 * static_cast<size_t>(ix.batch_bytes);
+size_t PlainSearchLds(const IndexView& ix, int /*flush_every*/) {
+    const size_t nqw = static_cast<size_t>(ix.blocks) / 8;
+    const bool lut_static = nqw == 1 || nqw == 2 || nqw == 12 || nqw == 16;
+    const size_t rstride = static_cast<size_t>(ix.blocks) * static_cast<size_t>(ix.bits);
+    return (lut_static ? 0 : static_cast<size_t>(ix.blocks) * 32) + ix.padded * sizeof(float) + kGroup * rstride;
 }
 
-hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a) {
-    if (a.nq == 0) return hipSuccess;
-    if (a.flush_every < 1 || a.flush_every > kMaxFlush) return hipErrorInvalidValue;
-    const uint32_t grid = static_cast<uint32_t>(a.nq < 262144 ? a.nq : 262144);
-    const size_t dyn = PlainSearchLds(ix, a.flush_every);
+template <int NQW>
+static hipError_t LaunchPlainT(hipStream_t st, const PlainParams* dp, uint32_t grid, size_t dyn) {
     if (dyn > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&PlainSearchKernel),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&PlainSearchKernel<NQW>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dyn));
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(PlainSearchKernel, dim3(grid), dim3(kWave), dyn, st, ix, a);
+    hipLaunchKernelGGL(PlainSearchKernel<NQW>, dim3(grid), dim3(kWave), dyn, st, dp);
     return hipGetLastError();
 }
 
+hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a, PlainParams* dparams) {
+    if (a.nq == 0) return hipSuccess;
+    if (a.flush_every < 1 || a.flush_every > kMaxFlush) return hipErrorInvalidValue;
+    if (ix.blocks % 8 != 0) return hipErrorInvalidValue;
+    const PlainParams hp{ix, a};
+    const hipError_t e = hipMemcpyAsync(dparams, &hp, sizeof(hp), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    const uint32_t grid = static_cast<uint32_t>(a.nq < 262144 ? a.nq : 262144);
+    const size_t dyn = PlainSearchLds(ix, a.flush_every);
+    switch (ix.blocks / 8) {
+        case 1: return LaunchPlainT<1>(st, dparams, grid, dyn);
+        case 2: return LaunchPlainT<2>(st, dparams, grid, dyn);
+        case 12: return LaunchPlainT<12>(st, dparams, grid, dyn);
+        case 16: return LaunchPlainT<16>(st, dparams, grid, dyn);
+        default: return LaunchPlainT<0>(st, dparams, grid, dyn);
+    }
+}
+
 // Coarse accumulate on caller-provided batches and LUTs (the integer kernel
-// on its own, for the exactness tests).  lut is the full 64-byte-per-block
-// reference layout.
+// on its own, for the exactness tests).  rowa is the reference layout
+// [blk][lane]; lut the full 64-byte-per-block reference layout.
 __global__ void CoarseAccumulateKernel(const uint8_t* __restrict__ rowa, uint64_t nbatch,
                                        const uint8_t* __restrict__ lut, int blocks, int32_t* __restrict__ out) {
     const uint64_t b = blockIdx.x;
