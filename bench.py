@@ -58,7 +58,22 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="only build and run the timed plain searches (for rocprofv3)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    """A plain copy of known size: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on it
+    gives the counter-to-bytes factor used for the scan kernel's traffic."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 x {nbytes} bytes")
 
 
 def log(*a):
@@ -160,6 +175,9 @@ def cpu_baseline(idx, queries_np, k, nprobe, gpu_ids, args):
 
 def main():
     args = parse()
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)

@@ -30,6 +30,37 @@ __device__ __forceinline__ bool Better(float d1, uint32_t i1, float d2, uint32_t
     return d1 < d2 || (d1 == d2 && i1 < i2);
 }
 
+// Global-memory views of pointers read from the parameter block: the
+// compiler cannot infer their address space, and generic (flat) loads would
+// also count against the LDS wait counter.
+#define E8Q_GLOBAL __attribute__((address_space(1)))
+// Plain clang vectors (no constructors) for loads through such pointers.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ __forceinline__ const E8Q_GLOBAL T* G(const T* p) {
+    return (const E8Q_GLOBAL T*)(p);
+}
+template <class T>
+__device__ __forceinline__ E8Q_GLOBAL T* Gw(T* p) {
+    return (E8Q_GLOBAL T*)(p);
+}
+
+// Values made opaque to the optimizer where they are used, so that it does
+// not hoist loop-invariant lane masks and parameter loads of the rarely taken
+// paths to the kernel entry, where they would hold scalar registers for the
+// whole kernel.
+__device__ __forceinline__ uint32_t LaneHere() {
+    uint32_t v = threadIdx.x;
+    asm volatile("" : "+v"(v));
+    return v;
+}
+template <class T>
+__device__ __forceinline__ const T* Here(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 __device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float UniformF(float v) {
     return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
@@ -132,11 +163,11 @@ __device__ __forceinline__ void LoadChunk(ProbeChunk& c, uint32_t base, const In
     c.cnt = c.vb_lo = c.vb_hi = c.ao_lo = c.ao_hi = c.ro_lo = c.ro_hi = 0;
     c.d2 = 0.0F;
     if (p < np) {
-        const uint32_t cid = a.probe_ids[qi * a.maxp + p];
-        const uint64_t vb = ix.cluster_begin[cid];
-        const uint64_t ve = ix.cluster_begin[cid + 1];
-        const uint64_t ao = ix.arena_begin[cid];
-        const uint64_t ro = ix.ref_arena_begin[cid];
+        const uint32_t cid = G(a.probe_ids)[qi * a.maxp + p];
+        const uint64_t vb = G(ix.cluster_begin)[cid];
+        const uint64_t ve = G(ix.cluster_begin)[cid + 1];
+        const uint64_t ao = G(ix.arena_begin)[cid];
+        const uint64_t ro = G(ix.ref_arena_begin)[cid];
         c.cnt = static_cast<uint32_t>(ve - vb);
         c.vb_lo = static_cast<uint32_t>(vb);
         c.vb_hi = static_cast<uint32_t>(vb >> 32);
@@ -144,7 +175,7 @@ __device__ __forceinline__ void LoadChunk(ProbeChunk& c, uint32_t base, const In
         c.ao_hi = static_cast<uint32_t>(ao >> 32);
         c.ro_lo = static_cast<uint32_t>(ro);
         c.ro_hi = static_cast<uint32_t>(ro >> 32);
-        c.d2 = a.probe_d2[qi * a.maxp + p];
+        c.d2 = G(a.probe_d2)[qi * a.maxp + p];
     }
     c.nonempty = __ballot(c.cnt > 0);
 }
@@ -173,33 +204,34 @@ __device__ __forceinline__ uint32_t Settle(ProbeChunk& c, uint32_t from, const I
 // any size, read in the coarse loop) and the batch header fields it uses.
 template <int NQW>
 struct Rec {
-    uint2 w[NQW > 0 ? NQW : 1];
+    u32x2 w[NQW > 0 ? NQW : 1];
     const uint8_t* rp;           // NQW == 0: this lane's rowA
     uint32_t hb, hw, hp;         // base / W / Omega bank bytes
-    float4 s0;                   // sc[0..3]
-    float3 s1;                   // sc[4..6]
+    f32x4 s0;                    // sc[0..3]
+    f32x4 s1;                    // sc[4..7]
 };
 
 template <int NQW>
-__device__ __forceinline__ void LoadRec(Rec<NQW>& r, const uint8_t* rec, uint32_t t, int blocks) {
-    const uint8_t* row = rec + kHdr + t * static_cast<uint32_t>(blocks);
+__device__ __forceinline__ void LoadRec(Rec<NQW>& r, const uint8_t* rec_p, uint32_t t, int blocks) {
+    const E8Q_GLOBAL uint8_t* rec = G(rec_p);
+    const uint32_t row = kHdr + t * static_cast<uint32_t>(blocks);
     if constexpr (NQW > 0) {
-        const uint2* rw = reinterpret_cast<const uint2*>(row);
+        const E8Q_GLOBAL u32x2* rw = reinterpret_cast<const E8Q_GLOBAL u32x2*>(rec + row);
 #pragma unroll
         for (int w = 0; w < NQW; ++w) r.w[w] = rw[w];
     } else {
-        r.rp = row;
+        r.rp = rec_p + row;
     }
     r.hb = rec[t];
     r.hw = rec[32 + t];
     r.hp = rec[64 + t];
-    r.s0 = *reinterpret_cast<const float4*>(rec + 96);
-    r.s1 = *reinterpret_cast<const float3*>(rec + 112);
+    r.s0 = *reinterpret_cast<const E8Q_GLOBAL f32x4*>(rec + 96);
+    r.s1 = *reinterpret_cast<const E8Q_GLOBAL f32x4*>(rec + 112);
 }
 
 // Eight blocks of coarse sum: bytes of x.x are blocks 0..3, of x.y 4..7; the
 // LUT holds per block 16 low-nibble then 16 high-nibble entries.
-__device__ __forceinline__ int32_t Coarse8(uint2 x, const uint8_t* l) {
+__device__ __forceinline__ int32_t Coarse8(u32x2 x, const uint8_t* l) {
     int32_t s = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b)
@@ -218,7 +250,7 @@ __device__ __forceinline__ int32_t Coarse(const Rec<NQW>& r, const uint8_t* lut,
 #pragma unroll
         for (int w = 0; w < NQW; ++w) acc += Coarse8(r.w[w], lut + w * 256);
     } else {
-        const uint2* rw = reinterpret_cast<const uint2*>(r.rp);
+        const E8Q_GLOBAL u32x2* rw = reinterpret_cast<const E8Q_GLOBAL u32x2*>(G(r.rp));
         const int nw = blocks / 8;
         for (int w = 0; w < nw; ++w) acc += Coarse8(rw[w], lut + w * 256);
     }
@@ -239,9 +271,14 @@ struct ClusterRef {
 // then 8 rows x 8 components per pass decode and accumulate in the AVX2
 // order (even blocks in acc, odd in acc2, then the 8-lane horizontal sum).
 // Only rows that beat the current k-th entry take part in the merge.
-__device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, int k, const IndexView& ix,
-                          const float* qv, const ClusterRef& cr, uint8_t* rowbuf, const float* tab) {
-    const int lane = threadIdx.x;
+template <bool GEN>
+__device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, const PlainParams* prm_,
+                          const float* qv, const ClusterRef& cr, uint8_t* rowbuf) {
+    const PlainParams* prm = Here(prm_);
+    const IndexView& ix = prm->ix;
+    const int k = static_cast<int>(prm->a.k);
+    const bool tab_lds = ix.bits <= 8;
+    const int lane = static_cast<int>(LaneHere());
     const int B = ix.blocks;
     const int c = ix.bits - 1;
     const int fbytes = ix.bits >= 2 ? B * c : 0;
@@ -254,12 +291,12 @@ __device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, int k,
         // refine-field words); four loads in flight per lane.
         const int nw = kGroup * wpr;
         for (int e0 = lane; e0 < nw; e0 += 4 * kWave) {
-            uint2 v[4];
+            u32x2 v[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int e = e0 + u * kWave;
                 const int r = e & (kGroup - 1), w = e >> 5;
-                v[u] = make_uint2(0, 0);
+                v[u] = u32x2{0u, 0u};
                 if (e < nw && r < nn) {
                     const uint32_t off = S.staged[rb + r];
                     const uint8_t* src =
@@ -267,23 +304,23 @@ __device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, int k,
                                      8 * w
                                : cr.ref_c + static_cast<uint64_t>(off >> 5) * ix.ref_batch_bytes +
                                      (off & 31) * ix.refine_stride + 8 * (w - bw);
-                    v[u] = *reinterpret_cast<const uint2*>(src);
+                    v[u] = *reinterpret_cast<const E8Q_GLOBAL u32x2*>(G(src));
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int e = e0 + u * kWave;
                 const int r = e & (kGroup - 1), w = e >> 5;
-                if (e < nw && r < nn) *reinterpret_cast<uint2*>(rowbuf + r * rstride + 8 * w) = v[u];
+                if (e < nw && r < nn) *reinterpret_cast<u32x2*>(rowbuf + r * rstride + 8 * w) = v[u];
             }
         }
         if (lane < nn) {
             const uint32_t off = S.staged[rb + lane];
-            const float* g1b = reinterpret_cast<const float*>(
-                cr.ref_c + static_cast<uint64_t>(off >> 5) * ix.ref_batch_bytes + ix.refine_stride * 32);
+            const E8Q_GLOBAL float* g1b = reinterpret_cast<const E8Q_GLOBAL float*>(
+                G(cr.ref_c) + static_cast<uint64_t>(off >> 5) * ix.ref_batch_bytes + ix.refine_stride * 32);
             S.rg1[lane] = g1b[off & 31];
             S.rh3[lane] = cr.d2 + g1b[32 + (off & 31)];
-            S.rid[lane] = ix.ids[cr.vb + off];
+            S.rid[lane] = G(ix.ids)[cr.vb + off];
         }
         __syncthreads();
         for (int sub = 0; sub < nn; sub += 8) {
@@ -293,14 +330,38 @@ __device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, int k,
                 const uint8_t* plane = rowbuf + r * rstride;
                 const uint8_t* field = plane + B;
                 float acc = 0.0F, acc2 = 0.0F;
-#pragma unroll 4
-                for (int blk = 0; blk < B; ++blk) {
-                    uint32_t digit;
-                    int br;
-                    DecodeDigit(ix.bits, plane[blk], field + blk * c, i, &digit, &br);
-                    const float lv = tab[static_cast<uint32_t>(br) * ix.tab_stride + digit];
-                    if (blk & 1) acc2 = fmaf(lv, qv[blk * 8 + i], acc2);
-                    else acc = fmaf(lv, qv[blk * 8 + i], acc);
+                if constexpr (!GEN) {   // bits 2..5: the field fits 32 bits
+                    // The block's field lies in the 8 bytes from its 4-byte
+                    // aligned start (the field area of a row is 8-aligned).
+                    const uint32_t fmask = c >= 4 ? ~0u : (1u << (8 * c)) - 1u;
+                    const uint32_t cm = (1u << c) - 1u;
+                    uint32_t pm = 0;                                // LSB positions of digits 0..6
+                    for (int q = 0; q < 7; ++q) pm |= 1u << (c * q);
+                    for (int blk = 0; blk < B; ++blk) {
+                        const uint32_t bo = static_cast<uint32_t>(blk * c);
+                        const uint32_t* wp = reinterpret_cast<const uint32_t*>(field + (bo & ~3u));
+                        const uint64_t win = (static_cast<uint64_t>(wp[1]) << 32) | wp[0];
+                        const uint32_t f = static_cast<uint32_t>(win >> (8 * (bo & 3u))) & fmask;
+                        const uint32_t rbv = plane[blk];
+                        const uint32_t f7 = (f >> (7 * c)) & cm;
+                        const uint32_t br = f7 & 1u;
+                        const uint32_t lo = i < 7 ? ((f >> (c * i)) & cm) : ((f7 & ~1u) | (__popc(f & pm) & 1u));
+                        const uint32_t digit = lo | (((rbv >> (7 - i)) & 1u) << c);
+                        const uint32_t ti = br * ix.tab_stride + digit;
+                        const float lv = tab_lds ? S.tab[ti] : G(ix.refine_tab)[ti];
+                        if (blk & 1) acc2 = fmaf(lv, qv[blk * 8 + i], acc2);
+                        else acc = fmaf(lv, qv[blk * 8 + i], acc);
+                    }
+                } else {
+                    for (int blk = 0; blk < B; ++blk) {
+                        uint32_t digit;
+                        int br;
+                        DecodeDigit(ix.bits, plane[blk], field + blk * c, i, &digit, &br);
+                        const uint32_t ti = static_cast<uint32_t>(br) * ix.tab_stride + digit;
+                        const float lv = tab_lds ? S.tab[ti] : G(ix.refine_tab)[ti];
+                        if (blk & 1) acc2 = fmaf(lv, qv[blk * 8 + i], acc2);
+                        else acc = fmaf(lv, qv[blk * 8 + i], acc);
+                    }
                 }
                 const float sc = Hsum8Lanes(acc + acc2);
                 if (i == 0) {
@@ -377,20 +438,19 @@ __device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, int k,
 // threshold cannot move before the next flush) and they are applied at
 // once; otherwise the step is replayed batch by batch, lane by lane, exactly
 // as CollectLanes does.
-template <int NQW>
+template <int NQW, bool GEN>
 __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __restrict__ prm) {
     const IndexView& ix = prm->ix;
     const PlainArgs& a = prm->a;
     __shared__ PlainLds<NQW> sh;
     PlainShared& S = sh.s;
-    extern __shared__ uint8_t dyn[];
+    extern __shared__ __align__(16) uint8_t dyn[];
     uint8_t* lutc = NQW > 0 ? sh.lut : dyn;
     float* qv = reinterpret_cast<float*>(dyn + (NQW > 0 ? 0 : static_cast<size_t>(ix.blocks) * 32));  // padded
     uint8_t* rowbuf = reinterpret_cast<uint8_t*>(qv + ix.padded);                                     // kGroup rows
     const int lane = threadIdx.x;
     if (ix.bits <= 8)
-        for (int i = lane; i < 512; i += kWave) S.tab[i] = ix.refine_tab[i];
-    const float* tab = ix.bits <= 8 ? S.tab : ix.refine_tab;
+        for (int i = lane; i < 512; i += kWave) S.tab[i] = G(ix.refine_tab)[i];
     const uint32_t j = lane >> 5, t = lane & 31;
     const int k = static_cast<int>(a.k);
     const int fe = a.flush_every;
@@ -401,14 +461,14 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
     const float z = a.z;
     const bool prune = !a.disable_prune;
     for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
-        for (uint32_t i = lane; i < P; i += kWave) qv[i] = a.rotq[qi * P + i];
+        for (uint32_t i = lane; i < P; i += kWave) qv[i] = G(a.rotq)[qi * P + i];
         for (uint32_t i = lane; 4 * i < E; i += kWave)
-            reinterpret_cast<uint32_t*>(lutc)[i] = reinterpret_cast<const uint32_t*>(a.lut + qi * E)[i];
-        const float* cst = a.consts + qi * 8;
+            reinterpret_cast<uint32_t*>(lutc)[i] = reinterpret_cast<const E8Q_GLOBAL uint32_t*>(G(a.lut) + qi * E)[i];
+        const E8Q_GLOBAL float* cst = G(a.consts) + qi * 8;
         const float delta = UniformF(cst[0]), base = UniformF(cst[1]);
         const float lut_err = UniformF(cst[2]);
         const float sml = UniformF(cst[3]) + lut_err;   // score_max + lut_err
-        const uint32_t np = min(static_cast<uint32_t>(ix.nlist), a.nprobe_q ? Uniform(a.nprobe_q[qi]) : a.nprobe_all);
+        const uint32_t np = min(static_cast<uint32_t>(ix.nlist), a.nprobe_q ? Uniform(G(a.nprobe_q)[qi]) : a.nprobe_all);
         __syncthreads();
         int cnt = 0, staged = 0;
         float limit = inf;
@@ -516,7 +576,7 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                 // Exact replay, batch by batch (a flush lands inside this step).
                 S.t1[lane] = t1;
                 S.pp[lane] = pp;
-                if (t == 0) {
+                if ((LaneHere() & 31) == 0) {
                     S.lbp[j] = lbp;
                     S.pam[j] = pam;
                 }
@@ -532,7 +592,7 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                     if (prune && limit0 < inf) {
                         if (fmaf(-S.pam[jj], zrh0, S.lbp[jj]) >= limit0) continue;
                     }
-                    const bool in32 = lane < 32;
+                    const bool in32 = LaneHere() < 32;
                     const float dl = in32 ? fmaf(-zrh0, S.pp[jj * 32 + lane], S.t1[jj * 32 + lane]) : inf;
                     const uint32_t lmask = nl >= 32 ? 0xFFFFFFFFu : ((1u << nl) - 1u);
                     uint32_t mask = prune ? (static_cast<uint32_t>(__ballot(in32 && dl < limit0)) & lmask) : lmask;
@@ -555,8 +615,8 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                         uint32_t rest = ps;
                         for (int q = 0; q < tk; ++q) rest &= rest - 1;
                         const int pl = __ffs(rest) - 1;
-                        FlushRows(S, staged, cnt, limit, k, ix, qv, cr, rowbuf, tab);
-                        if (lane == 0) S.staged[0] = off + static_cast<uint32_t>(pl);
+                        FlushRows<GEN>(S, staged, cnt, limit, prm, qv, cr, rowbuf);
+                        if (LaneHere() == 0) S.staged[0] = off + static_cast<uint32_t>(pl);
                         staged = 1;
                         n_surv += 1;
                         mask &= (pl >= 31) ? 0u : ~((2u << pl) - 1u);
@@ -567,7 +627,7 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
             }
             if (cb0 + 2 >= cnb && staged > 0) {   // end of the cluster
                 __syncthreads();
-                FlushRows(S, staged, cnt, limit, k, ix, qv, cluster_ref(), rowbuf, tab);
+                FlushRows<GEN>(S, staged, cnt, limit, prm, qv, cluster_ref(), rowbuf);
                 staged = 0;
                 horizon();
             }
@@ -585,8 +645,8 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
             cb0 = xb0;
         }
         for (int i = lane; i < k; i += kWave) {
-            a.out_ids[qi * a.k + i] = i < cnt ? S.ti[i] : 0xFFFFFFFFu;
-            a.out_dists[qi * a.k + i] = i < cnt ? S.td[i] : inf;
+            Gw(a.out_ids)[qi * a.k + i] = i < cnt ? S.ti[i] : 0xFFFFFFFFu;
+            Gw(a.out_dists)[qi * a.k + i] = i < cnt ? S.td[i] : inf;
         }
         if (lane == 0 && a.stats != nullptr) {
             atomicAdd(&a.stats[0], static_cast<unsigned long long>(n_cand));
@@ -599,19 +659,20 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
 
 size_t PlainSearchLds(const IndexView& ix, int /*flush_every*/) {
     const size_t nqw = static_cast<size_t>(ix.blocks) / 8;
-    const bool lut_static = nqw == 1 || nqw == 2 || nqw == 12 || nqw == 16;
+    const bool lut_static = (nqw == 1 || nqw == 2 || nqw == 12 || nqw == 16) && ix.bits >= 2 && ix.bits <= 5;
     const size_t rstride = static_cast<size_t>(ix.blocks) * static_cast<size_t>(ix.bits);
-    return (lut_static ? 0 : static_cast<size_t>(ix.blocks) * 32) + ix.padded * sizeof(float) + kGroup * rstride;
+    // + 16: the refine reads each field through an 8-byte window.
+    return (lut_static ? 0 : static_cast<size_t>(ix.blocks) * 32) + ix.padded * sizeof(float) + kGroup * rstride + 16;
 }
 
-template <int NQW>
+template <int NQW, bool GEN>
 static hipError_t LaunchPlainT(hipStream_t st, const PlainParams* dp, uint32_t grid, size_t dyn) {
     if (dyn > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&PlainSearchKernel<NQW>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&PlainSearchKernel<NQW, GEN>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dyn));
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(PlainSearchKernel<NQW>, dim3(grid), dim3(kWave), dyn, st, dp);
+    hipLaunchKernelGGL((PlainSearchKernel<NQW, GEN>), dim3(grid), dim3(kWave), dyn, st, dp);
     return hipGetLastError();
 }
 
@@ -624,12 +685,15 @@ hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArg
     if (e != hipSuccess) return e;
     const uint32_t grid = static_cast<uint32_t>(a.nq < 262144 ? a.nq : 262144);
     const size_t dyn = PlainSearchLds(ix, a.flush_every);
+    // Rates 2..5 (refine fields of 1..4 bytes per block) take the specialized
+    // kernels; other rates and row sizes the general one.
+    if (ix.bits < 2 || ix.bits > 5) return LaunchPlainT<0, true>(st, dparams, grid, dyn);
     switch (ix.blocks / 8) {
-        case 1: return LaunchPlainT<1>(st, dparams, grid, dyn);
-        case 2: return LaunchPlainT<2>(st, dparams, grid, dyn);
-        case 12: return LaunchPlainT<12>(st, dparams, grid, dyn);
-        case 16: return LaunchPlainT<16>(st, dparams, grid, dyn);
-        default: return LaunchPlainT<0>(st, dparams, grid, dyn);
+        case 1: return LaunchPlainT<1, false>(st, dparams, grid, dyn);
+        case 2: return LaunchPlainT<2, false>(st, dparams, grid, dyn);
+        case 12: return LaunchPlainT<12, false>(st, dparams, grid, dyn);
+        case 16: return LaunchPlainT<16, false>(st, dparams, grid, dyn);
+        default: return LaunchPlainT<0, false>(st, dparams, grid, dyn);
     }
 }
 
