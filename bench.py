@@ -227,6 +227,7 @@ def main():
                    "recall": recall_at(gt, ids, 10, k) if gt is not None else -1.0,
                    "recall_10at10": recall_at(gt, ids, 10, 10) if gt is not None else -1.0,
                    "scan_kernel_ms": sum(scan_ms) / len(scan_ms), "batches": st["batches"],
+                   "records_read": st["scanned"],
                    "survivors": st["survivors"], "candidates": st["candidates"]}
             algo = row["batches"] * batch_bytes + row["survivors"] * survivor_bytes
             row["scan_algorithmic_bytes"] = algo
@@ -234,7 +235,8 @@ def main():
             rows.append(row)
             log(f"{mode} nprobe={nprobe}: {ms:.2f} ms/step, {row['qps']:.0f} QPS, recall 10@{k}={row['recall']:.4f} "
                 f"(10@10 {row['recall_10at10']:.4f}), "
-                f"scan kernel {row['scan_kernel_ms']:.2f} ms = {row['scan_achieved_gbs']:.0f} GB/s")
+                f"scan kernel {row['scan_kernel_ms']:.2f} ms = {row['scan_achieved_gbs']:.0f} GB/s, "
+                f"records read {row['records_read']}/{row['batches']}")
         results[mode] = {"rows": rows, "qps_at_target": qps_at_recall(rows, args.target) if gt is not None else None}
     if args.profile:
         return

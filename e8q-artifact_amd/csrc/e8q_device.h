@@ -37,6 +37,11 @@ struct IndexView {
     const uint32_t* ids;
     const float* refine_tab;
     const double* gate_mid;    // [16]
+    // Per cluster, over its scan records: {min base_min (sc0), max W_step
+    // (sc3), max |W|max (sc6), max Omega_max = Omega_min + 255 Omega_step}.
+    // The batch-skip bound is monotone in each, so these give a lower bound
+    // for every batch of the cluster at once.
+    const float4* csum;
 };
 
 // Encoder tables (device): merged sorted level table for b >= 2, the four

@@ -456,7 +456,37 @@ __global__ __launch_bounds__(256) void RecordLayoutKernel(const uint8_t* src, ui
 }
 
 // ---------------------------------------------------------------------------
+// Cluster summaries of the scan-record headers; one wave per cluster.
+__global__ __launch_bounds__(64) void ClusterSummaryKernel(IndexView ix, float4* __restrict__ csum) {
+    const uint64_t c = blockIdx.x;
+    const uint32_t cnt = static_cast<uint32_t>(ix.cluster_begin[c + 1] - ix.cluster_begin[c]);
+    const uint32_t nb = (cnt + 31) / 32;
+    float s0 = __builtin_inff(), s3 = -__builtin_inff(), s6 = -__builtin_inff(), pm = -__builtin_inff();
+    for (uint32_t b = threadIdx.x; b < nb; b += 64) {
+        const float* sc = reinterpret_cast<const float*>(ix.arena + ix.arena_begin[c] +
+                                                         static_cast<uint64_t>(b) * ix.batch_bytes + 96);
+        s0 = fminf(s0, sc[0]);
+        s3 = fmaxf(s3, sc[3]);
+        s6 = fmaxf(s6, sc[6]);
+        pm = fmaxf(pm, fmaf(sc[5], 255.0F, sc[4]));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 = fminf(s0, __shfl_xor(s0, o));
+        s3 = fmaxf(s3, __shfl_xor(s3, o));
+        s6 = fmaxf(s6, __shfl_xor(s6, o));
+        pm = fmaxf(pm, __shfl_xor(pm, o));
+    }
+    if (threadIdx.x == 0) csum[c] = make_float4(s0, s3, s6, pm);
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
+
+hipError_t LaunchClusterSummary(hipStream_t st, const IndexView& ix, float4* csum) {
+    if (ix.nlist == 0) return hipSuccess;
+    hipLaunchKernelGGL(ClusterSummaryKernel, dim3(static_cast<uint32_t>(ix.nlist)), dim3(64), 0, st, ix, csum);
+    return hipGetLastError();
+}
 
 hipError_t LaunchRecordLayout(hipStream_t st, const IndexView& ix, const uint8_t* src, uint8_t* dst, uint64_t nrec,
                               bool to_device) {

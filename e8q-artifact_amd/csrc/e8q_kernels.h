@@ -24,7 +24,7 @@ struct PlainArgs {
     int flush_every;
     uint32_t* out_ids;          // [nq][k]
     float* out_dists;
-    unsigned long long* stats;  // [3] candidates, survivors, batches
+    unsigned long long* stats;  // [4] candidates, survivors, batches, records read
 };
 
 hipError_t LaunchRotate(hipStream_t st, const float* x, uint64_t rows, uint32_t dim, const IndexView& ix, float* out,
@@ -73,6 +73,8 @@ hipError_t LaunchBanks(hipStream_t st, const IndexView& ix, const float* ob, con
 // device layout ([lane][blk]); src may equal dst.
 hipError_t LaunchRecordLayout(hipStream_t st, const IndexView& ix, const uint8_t* src, uint8_t* dst, uint64_t nrec,
                               bool to_device);
+// Fills ix.csum from the scan-record headers (after build or upload).
+hipError_t LaunchClusterSummary(hipStream_t st, const IndexView& ix, float4* csum);
 hipError_t LaunchEncodeUnits(hipStream_t st, const IndexView& ix, const EncodeView& ev, const float* unit,
                              uint64_t rows, double scale, uint16_t* digits, uint8_t* branch);
 

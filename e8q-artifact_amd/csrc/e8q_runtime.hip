@@ -112,12 +112,12 @@ struct e8q_dindex {
     uint64_t records() const { return view.batch_bytes ? (arena_bytes - 64) / view.batch_bytes : 0; }
     std::vector<uint64_t> cluster_begin, arena_begin, ref_arena_begin;
     DBuf signs, mean_rot, centroid_rot, centroid_n2, d_cluster_begin, d_arena_begin, d_ref_arena_begin, arena,
-        ref_arena, ids, refine_tab, gate_mid, levels, mlev, mdig, cls_lev, cls_dig;
+        ref_arena, ids, refine_tab, gate_mid, levels, mlev, mdig, cls_lev, cls_dig, csum;
     uint64_t device_bytes() const {
         return signs.bytes + mean_rot.bytes + centroid_rot.bytes + centroid_n2.bytes + d_cluster_begin.bytes +
                d_arena_begin.bytes + d_ref_arena_begin.bytes + arena.bytes + ref_arena.bytes + ids.bytes +
                refine_tab.bytes + gate_mid.bytes + levels.bytes + mlev.bytes + mdig.bytes + cls_lev.bytes +
-               cls_dig.bytes;
+               cls_dig.bytes + csum.bytes;
     }
 };
 
@@ -200,6 +200,7 @@ void RefreshView(e8q_dindex* x) {
     v.ids = x->ids.as<uint32_t>();
     v.refine_tab = x->refine_tab.as<float>();
     v.gate_mid = x->gate_mid.as<double>();
+    v.csum = x->csum.as<float4>();
 }
 
 }  // namespace
@@ -308,6 +309,7 @@ e8q_status e8q_index_create(e8q_ctx* ctx, const e8q_index_desc* d, e8q_dindex** 
             x->arena.alloc(ab + 64);
             x->ref_arena.alloc(rb + 64);
             x->ids.alloc(std::max<uint64_t>(d->n, 1) * sizeof(uint32_t));
+            x->csum.alloc(L * sizeof(float4));
             Check(hipMemsetAsync(x->arena.p, 0, x->arena.bytes, st), "memset");
             Check(hipMemsetAsync(x->ref_arena.p, 0, x->ref_arena.bytes, st), "memset");
             BuildEncoderTables(x, d->levels, d->bits, st);
@@ -350,6 +352,7 @@ e8q_status e8q_index_upload(e8q_ctx* ctx, e8q_dindex* x, const e8q_index_arrays*
               "arena layout");
         Check(hipMemcpyAsync(x->ref_arena.p, h->ref_arena, x->ref_arena.bytes, hipMemcpyHostToDevice, st),
               "upload refine arena");
+        Check(LaunchClusterSummary(st, v, x->csum.as<float4>()), "cluster summary");
         Check(hipStreamSynchronize(st), "upload");
     });
 }
@@ -432,6 +435,7 @@ e8q_status e8q_index_encode(e8q_ctx* ctx, e8q_dindex* x, const float* base, cons
         a.op = op.as<float>();
         Check(LaunchEncodeRows(st, v, x->ev, a), "encode rows");
         Check(LaunchBanks(st, v, ob.as<float>(), ow.as<float>(), op.as<float>(), x->arena.as<uint8_t>()), "banks");
+        Check(LaunchClusterSummary(st, v, x->csum.as<float4>()), "cluster summary");
         Check(hipStreamSynchronize(st), "encode");
     });
 }
@@ -447,7 +451,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "unknown search mode");
         const IndexView& v = x->view;
         const uint64_t nq = a->nq;
-        if (stats) *stats = e8q_search_stats{0, 0, 0, 0.0, 0.0};
+        if (stats) *stats = e8q_search_stats{0, 0, 0, 0.0, 0.0, 0};
         if (nq == 0) return;
         uint32_t maxp = 0;
         if (a->nprobe_per_query) {
@@ -539,6 +543,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             stats->candidates = hs[0];
             stats->survivors = hs[1];
             stats->batches = hs[2];
+            stats->scanned = hs[3];
             stats->scan_ms = ms_scan;
             stats->total_ms = ms_all;
         }

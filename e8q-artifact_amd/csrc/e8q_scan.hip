@@ -65,6 +65,13 @@ __device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgc
 __device__ __forceinline__ float UniformF(float v) {
     return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
 }
+__device__ __forceinline__ uint64_t Uniform64(uint64_t v) {
+    return (static_cast<uint64_t>(Uniform(static_cast<uint32_t>(v >> 32))) << 32) | Uniform(static_cast<uint32_t>(v));
+}
+template <class T>
+__device__ __forceinline__ const T* UniformPtr(const T* p) {
+    return reinterpret_cast<const T*>(Uniform64(reinterpret_cast<uint64_t>(p)));
+}
 __device__ __forceinline__ uint32_t ReadLane(uint32_t v, uint32_t l) {
     return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(l)));
 }
@@ -148,20 +155,52 @@ struct PlainLds {
     PlainShared s;
 };
 
+// Per-query inputs of the whole-cluster skip test.
+struct SkipConsts {
+    float U;          // upper bound of |S| over every possible code of this query
+    float lut_err;
+    float sml;        // score_max + lut_err
+    float z;
+    bool prune;
+};
+
 // Probe metadata of 64 consecutive probes, one per lane.
 struct ProbeChunk {
     uint32_t base;
     uint64_t nonempty;   // lanes whose cluster has vectors
+    uint64_t active;     // ... and that are not skipped as a whole
     uint32_t cnt, vb_lo, vb_hi, ao_lo, ao_hi, ro_lo, ro_hi;
-    float d2;
+    float d2, sqd2;
+    float lbpc;          // cluster bound before the slack term, with max|S| = U
+    float pamx;          // cluster max of Omega_max
 };
 
+// Whole-cluster skip.  The reference skips a batch when its bound
+//   lb = fma(-Om, z*r_h, fma(-Wmax, max|S|, (sc0 + d2) - fma(lut_err, Wmax, (0.5*sc3)*sml)))
+// reaches the threshold.  Every operation is monotone in its inputs, so the
+// same expression over the cluster's extreme header values (min sc0, max
+// sc3, Wmax, Om) and max|S| := U is a lower bound of every batch's lb; when
+// it reaches the threshold, every batch of the cluster would be skipped and
+// the cluster is passed over without reading its records.  Skips decided
+// with a threshold stay valid as it decreases (lb grows as r_h shrinks).
+__device__ __forceinline__ void Refresh(ProbeChunk& c, float L, const SkipConsts& k) {
+    bool s = false;
+    if (k.prune && L < __builtin_inff()) {
+        const float zr = k.z * (L < c.d2 ? sqrtf(fmaxf(0.0F, L)) : c.sqd2);
+        s = fmaf(-c.pamx, zr, c.lbpc) >= L;
+    }
+    c.active = c.nonempty & ~__ballot(s);
+}
+
+// lane_cand / lane_nb: this lane's share of the candidate and batch counts
+// (every vector and record of every probed cluster, skipped or not).
 __device__ __forceinline__ void LoadChunk(ProbeChunk& c, uint32_t base, const IndexView& ix, const PlainArgs& a,
-                                          uint64_t qi, uint32_t np) {
+                                          uint64_t qi, uint32_t np, float L, const SkipConsts& k,
+                                          uint32_t& lane_cand, uint32_t& lane_nb) {
     const uint32_t p = base + threadIdx.x;
     c.base = base;
     c.cnt = c.vb_lo = c.vb_hi = c.ao_lo = c.ao_hi = c.ro_lo = c.ro_hi = 0;
-    c.d2 = 0.0F;
+    c.d2 = c.sqd2 = c.lbpc = c.pamx = 0.0F;
     if (p < np) {
         const uint32_t cid = G(a.probe_ids)[qi * a.maxp + p];
         const uint64_t vb = G(ix.cluster_begin)[cid];
@@ -176,25 +215,35 @@ __device__ __forceinline__ void LoadChunk(ProbeChunk& c, uint32_t base, const In
         c.ro_lo = static_cast<uint32_t>(ro);
         c.ro_hi = static_cast<uint32_t>(ro >> 32);
         c.d2 = G(a.probe_d2)[qi * a.maxp + p];
+        const f32x4 cs = *reinterpret_cast<const E8Q_GLOBAL f32x4*>(G(ix.csum) + cid);
+        c.sqd2 = sqrtf(fmaxf(0.0F, c.d2));
+        const float pad_w_round = (0.5F * cs.y) * k.sml;
+        const float pad_add = fmaf(k.lut_err, cs.z, pad_w_round);
+        c.lbpc = fmaf(-cs.z, k.U, (cs.x + c.d2) - pad_add);
+        c.pamx = cs.w;
+        lane_cand += c.cnt;
+        lane_nb += (c.cnt + 31) / 32;
     }
     c.nonempty = __ballot(c.cnt > 0);
+    Refresh(c, L, k);
 }
 
-// First non-empty probe >= from inside the loaded chunk; base + 64 (or np)
+// First active probe >= from inside the loaded chunk; base + 64 (or np)
 // when there is none there.
 __device__ __forceinline__ uint32_t NextInChunk(const ProbeChunk& c, uint32_t from, uint32_t np) {
     const uint32_t off = from - c.base;
-    const uint64_t m = off >= 64 ? 0 : (c.nonempty >> off) << off;
+    const uint64_t m = off >= 64 ? 0 : (c.active >> off) << off;
     const uint32_t r = m ? c.base + static_cast<uint32_t>(__ffsll(static_cast<long long>(m))) - 1 : c.base + 64;
     return min(r, np);
 }
 
-// First non-empty probe >= from, loading chunks as needed; np when none.
+// First active probe >= from, loading chunks as needed; np when none.
 __device__ __forceinline__ uint32_t Settle(ProbeChunk& c, uint32_t from, const IndexView& ix, const PlainArgs& a,
-                                           uint64_t qi, uint32_t np) {
+                                           uint64_t qi, uint32_t np, float L, const SkipConsts& k,
+                                           uint32_t& lane_cand, uint32_t& lane_nb) {
     uint32_t p = from;
     while (p < np && p - c.base >= 64) {
-        LoadChunk(c, p, ix, a, qi, np);
+        LoadChunk(c, p, ix, a, qi, np, L, k, lane_cand, lane_nb);
         p = NextInChunk(c, p, np);
     }
     return p;
@@ -438,8 +487,17 @@ __device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, const 
 // threshold cannot move before the next flush) and they are applied at
 // once; otherwise the step is replayed batch by batch, lane by lane, exactly
 // as CollectLanes does.
+// Waves per SIMD the register budget is set for: four for short rows (the
+// cold refine and merge paths take the few spills), three for long rows and
+// the general decode.
 template <int NQW, bool GEN>
-__global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __restrict__ prm) {
+constexpr int PlainWaves() {
+    return (!GEN && NQW >= 0 && NQW <= 2) ? 4 : 3;
+}
+
+template <int NQW, bool GEN>
+__global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKernel(
+    const PlainParams* __restrict__ prm) {
     const IndexView& ix = prm->ix;
     const PlainArgs& a = prm->a;
     __shared__ PlainLds<NQW> sh;
@@ -457,7 +515,10 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
     const float inf = __builtin_inff();
     const uint32_t P = static_cast<uint32_t>(ix.padded);
     const uint32_t E = static_cast<uint32_t>(ix.blocks) * 32;
-    const uint32_t BB = ix.batch_bytes;
+    // Values the scan loop uses, read once (scalar registers).
+    const uint32_t BB = Uniform(ix.batch_bytes);
+    const int blocks = static_cast<int>(Uniform(static_cast<uint32_t>(ix.blocks)));
+    const uint8_t* const arena = UniformPtr(ix.arena);
     const float z = a.z;
     const bool prune = !a.disable_prune;
     for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
@@ -470,12 +531,36 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
         const float sml = UniformF(cst[3]) + lut_err;   // score_max + lut_err
         const uint32_t np = min(static_cast<uint32_t>(ix.nlist), a.nprobe_q ? Uniform(G(a.nprobe_q)[qi]) : a.nprobe_all);
         __syncthreads();
+        // U: the integer sum of any code lies between the sums of the
+        // per-table minima and maxima, and S is monotone in it.
+        int32_t tmin = 0, tmax = 0;
+        for (uint32_t tb = lane; tb < 2u * static_cast<uint32_t>(ix.blocks); tb += kWave) {
+            const uint8_t* tp = lutc + (tb >> 1) * 32 + (tb & 1) * 16;
+            int32_t mn = 255, mx = 0;
+            for (int e = 0; e < 16; ++e) {
+                mn = min(mn, static_cast<int32_t>(tp[e]));
+                mx = max(mx, static_cast<int32_t>(tp[e]));
+            }
+            tmin += mn;
+            tmax += mx;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            tmin += __shfl_xor(tmin, o);
+            tmax += __shfl_xor(tmax, o);
+        }
+        SkipConsts sk;
+        sk.U = UniformF(fmaxf(fabsf(fmaf(static_cast<float>(tmin), delta, base)),
+                              fabsf(fmaf(static_cast<float>(tmax), delta, base))));
+        sk.lut_err = lut_err;
+        sk.sml = sml;
+        sk.z = z;
+        sk.prune = prune;
         int cnt = 0, staged = 0;
         float limit = inf;
-        uint32_t n_cand = 0, n_surv = 0, n_batch = 0;
+        uint32_t n_surv = 0, n_scan = 0, lane_cand = 0, lane_nb = 0;
         ProbeChunk ch;
-        LoadChunk(ch, 0, ix, a, qi, np);
-        uint32_t cp = Settle(ch, NextInChunk(ch, 0, np), ix, a, qi, np);
+        LoadChunk(ch, 0, ix, a, qi, np, limit, sk, lane_cand, lane_nb);
+        uint32_t cp = Settle(ch, NextInChunk(ch, 0, np), ix, a, qi, np, limit, sk, lane_cand, lane_nb);
         // The open cluster.
         uint32_t ccnt = 0, cnb = 0, cb0 = 0;
         float d2 = 0.0F, sqrt_d2 = 0.0F, zrh = 0.0F;
@@ -486,7 +571,7 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
             cnb = (ccnt + 31) / 32;
             d2 = __uint_as_float(ReadLane(__float_as_uint(ch.d2), l));
             sqrt_d2 = sqrtf(fmaxf(0.0F, d2));
-            arena_c = ix.arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
+            arena_c = arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
         };
         // z * r_horizon, r_horizon = min(sqrt(d2), sqrt(limit)) once the heap is live.
         auto horizon = [&]() { zrh = z * (limit < d2 ? sqrtf(fmaxf(0.0F, limit)) : sqrt_d2); };
@@ -496,12 +581,36 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                               d2};
         };
         Rec<NQW> rn;
+        cp = Uniform(cp);
+        ch.base = Uniform(ch.base);
+        ch.active = Uniform64(ch.active);
+        ch.nonempty = Uniform64(ch.nonempty);
         if (cp < np) {
             open(cp);
             horizon();
-            if (j < min(2u, cnb)) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(j) * BB, t, ix.blocks);
+            if (j < min(2u, cnb)) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(j) * BB, t, blocks);
         }
-        while (cp < np) {
+        for (;;) {
+            // The scan state is wave-uniform; saying so keeps it in scalar
+            // registers and the loop on scalar branches (the compiler's
+            // uniformity analysis gives up after the per-lane merge loops).
+            cp = Uniform(cp);
+            if (cp >= np) break;
+            cb0 = Uniform(cb0);
+            cnb = Uniform(cnb);
+            ccnt = Uniform(ccnt);
+            staged = static_cast<int>(Uniform(static_cast<uint32_t>(staged)));
+            cnt = static_cast<int>(Uniform(static_cast<uint32_t>(cnt)));
+            limit = UniformF(limit);
+            d2 = UniformF(d2);
+            sqrt_d2 = UniformF(sqrt_d2);
+            zrh = UniformF(zrh);
+            arena_c = UniformPtr(arena_c);
+            ch.base = Uniform(ch.base);
+            ch.active = Uniform64(ch.active);
+            ch.nonempty = Uniform64(ch.nonempty);
+            n_surv = Uniform(n_surv);
+            n_scan = Uniform(n_scan);
             // The next step; its records are loaded now when its probe is in
             // the loaded chunk (otherwise after this step, see below).
             uint32_t xp = cp, xb0 = cb0 + 2;
@@ -516,20 +625,21 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                 uint32_t xnb = cnb;
                 if (xp != cp) {
                     const uint32_t l = xp - ch.base;
-                    xa = ix.arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
+                    xa = arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
                     xnb = (ReadLane(ch.cnt, l) + 31) / 32;
                 }
-                if (j < min(2u, xnb - xb0)) LoadRec<NQW>(rn, xa + static_cast<uint64_t>(xb0 + j) * BB, t, ix.blocks);
+                if (j < min(2u, xnb - xb0)) LoadRec<NQW>(rn, xa + static_cast<uint64_t>(xb0 + j) * BB, t, blocks);
             }
 
             const uint32_t jn = min(2u, cnb - cb0);
+            n_scan += jn;
             const uint32_t bj = cb0 + j;
             const bool act = j < jn;
             const float L = limit;
             float t1 = inf, pp = 0.0F, lbp = 0.0F, pam = 0.0F;
             uint32_t sabs = 0;
             if (act) {
-                const int32_t acc = Coarse<NQW>(rc, lutc, ix.blocks);
+                const int32_t acc = Coarse<NQW>(rc, lutc, blocks);
                 const float sc0 = rc.s0.x, sc1 = rc.s0.y, sc2 = rc.s0.z, sc3 = rc.s0.w;
                 const float sc4 = rc.s1.x, sc5 = rc.s1.y, sc6 = rc.s1.z;
                 const float pad_w_round = (0.5F * sc3) * sml;
@@ -570,8 +680,6 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                 }
                 staged += total;
                 n_surv += total;
-                n_batch += jn;
-                n_cand += min(ccnt - cb0 * 32, jn * 32);
             } else {
                 // Exact replay, batch by batch (a flush lands inside this step).
                 S.t1[lane] = t1;
@@ -585,8 +693,6 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                 for (uint32_t jj = 0; jj < jn; ++jj) {
                     const uint32_t off = (cb0 + jj) * 32;
                     const uint32_t nl = min(32u, ccnt - off);
-                    n_cand += nl;
-                    n_batch += 1;
                     const float limit0 = limit;
                     const float zrh0 = z * (limit0 < d2 ? sqrtf(fmaxf(0.0F, limit0)) : sqrt_d2);
                     if (prune && limit0 < inf) {
@@ -616,6 +722,7 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                         for (int q = 0; q < tk; ++q) rest &= rest - 1;
                         const int pl = __ffs(rest) - 1;
                         FlushRows<GEN>(S, staged, cnt, limit, prm, qv, cr, rowbuf);
+                        Refresh(ch, limit, sk);
                         if (LaneHere() == 0) S.staged[0] = off + static_cast<uint32_t>(pl);
                         staged = 1;
                         n_surv += 1;
@@ -623,35 +730,68 @@ __global__ __launch_bounds__(kWave) void PlainSearchKernel(const PlainParams* __
                     }
                 }
                 __syncthreads();
+                staged = static_cast<int>(Uniform(static_cast<uint32_t>(staged)));
+                n_surv = Uniform(n_surv);
+                cnt = static_cast<int>(Uniform(static_cast<uint32_t>(cnt)));
+                limit = UniformF(limit);
+                ch.active = Uniform64(ch.active);
                 horizon();
             }
             if (cb0 + 2 >= cnb && staged > 0) {   // end of the cluster
                 __syncthreads();
                 FlushRows<GEN>(S, staged, cnt, limit, prm, qv, cluster_ref(), rowbuf);
+                Refresh(ch, limit, sk);
                 staged = 0;
                 horizon();
             }
+            // The records of the next step are reloaded here when its probe
+            // lies beyond the loaded chunk, or when a flush has just made its
+            // whole cluster skippable (then the next active probe is taken).
+            bool reload = !x_in;
+            if (x_in && xp != cp && !((ch.active >> (xp - ch.base)) & 1u)) {
+                xp = NextInChunk(ch, xp + 1, np);
+                reload = true;
+            }
             if (xp >= np) break;
-            if (!x_in) {   // the next probe lies beyond the loaded chunk
-                xp = Settle(ch, xp, ix, a, qi, np);
+            if (xp - ch.base >= 64) {
+                xp = Settle(ch, xp, ix, a, qi, np, limit, sk, lane_cand, lane_nb);
                 if (xp >= np) break;
             }
             if (xp != cp) {
                 open(xp);
                 horizon();
             }
-            if (!x_in && j < min(2u, cnb)) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(j) * BB, t, ix.blocks);
-            cp = xp;
-            cb0 = xb0;
+            if (reload && j < min(2u, cnb)) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(j) * BB, t, blocks);
+            cp = Uniform(xp);
+            cb0 = Uniform(xb0);
+            cnb = Uniform(cnb);
+            ccnt = Uniform(ccnt);
+            staged = static_cast<int>(Uniform(static_cast<uint32_t>(staged)));
+            cnt = static_cast<int>(Uniform(static_cast<uint32_t>(cnt)));
+            limit = UniformF(limit);
+            d2 = UniformF(d2);
+            sqrt_d2 = UniformF(sqrt_d2);
+            zrh = UniformF(zrh);
+            arena_c = UniformPtr(arena_c);
+            ch.base = Uniform(ch.base);
+            ch.active = Uniform64(ch.active);
+            ch.nonempty = Uniform64(ch.nonempty);
+            n_surv = Uniform(n_surv);
+            n_scan = Uniform(n_scan);
         }
         for (int i = lane; i < k; i += kWave) {
             Gw(a.out_ids)[qi * a.k + i] = i < cnt ? S.ti[i] : 0xFFFFFFFFu;
             Gw(a.out_dists)[qi * a.k + i] = i < cnt ? S.td[i] : inf;
         }
+        for (int o = 32; o > 0; o >>= 1) {
+            lane_cand += __shfl_xor(lane_cand, o);
+            lane_nb += __shfl_xor(lane_nb, o);
+        }
         if (lane == 0 && a.stats != nullptr) {
-            atomicAdd(&a.stats[0], static_cast<unsigned long long>(n_cand));
+            atomicAdd(&a.stats[0], static_cast<unsigned long long>(lane_cand));
             atomicAdd(&a.stats[1], static_cast<unsigned long long>(n_surv));
-            atomicAdd(&a.stats[2], static_cast<unsigned long long>(n_batch));
+            atomicAdd(&a.stats[2], static_cast<unsigned long long>(lane_nb));
+            atomicAdd(&a.stats[3], static_cast<unsigned long long>(n_scan));
         }
         __syncthreads();
     }

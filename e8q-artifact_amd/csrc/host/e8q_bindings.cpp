@@ -109,6 +109,7 @@ class PyE8qIndex {
         s["survivor_rate"] =
             st.candidates > 0 ? static_cast<double>(st.survivors) / static_cast<double>(st.candidates) : 0.0;
         s["batches"] = st.batches;
+        s["scanned"] = st.scanned;
         s["device_scan_ms"] = st.device_scan_ms;
         s["device_total_ms"] = st.device_total_ms;
         return py::make_tuple(ids, dists, s);
@@ -143,6 +144,7 @@ class PyE8qIndex {
         s["candidates"] = st.candidates;
         s["survivors"] = st.survivors;
         s["batches"] = st.batches;
+        s["scanned"] = st.scanned;
         s["device_scan_ms"] = st.device_scan_ms;
         s["device_total_ms"] = st.device_total_ms;
         return s;

@@ -353,6 +353,7 @@ void E8qIndex::SearchBatch(const float* queries, size_t nq, size_t k, size_t npr
         stats->candidates += st.candidates;
         stats->survivors += st.survivors;
         stats->batches += st.batches;
+        stats->scanned += st.scanned;
         stats->device_scan_ms += st.scan_ms;
         stats->device_total_ms += st.total_ms;
     }

@@ -33,7 +33,8 @@ struct SearchStats {
     double prepare_route_us = 0.0;
     double scan_us = 0.0;
     double refine_us = 0.0;
-    size_t batches = 0;         // 32-vector scan records streamed
+    size_t batches = 0;         // 32-vector scan records of the probed clusters
+    size_t scanned = 0;         // of those, records whose codes were read
     double device_scan_ms = 0.0;   // scan kernel, HIP events on the device stream
     double device_total_ms = 0.0;  // whole search on the device stream
 };

@@ -65,9 +65,11 @@ typedef struct {
 typedef struct {
     uint64_t candidates; /* SearchStats::candidates (e8q.hpp:55-62) */
     uint64_t survivors;  /* SearchStats::survivors                  */
-    uint64_t batches;    /* 32-vector scan records streamed         */
+    uint64_t batches;    /* 32-vector scan records of the probed clusters */
     double scan_ms;      /* scan kernel time, HIP events on the context stream */
     double total_ms;     /* device time of the whole search, same events       */
+    uint64_t scanned;    /* of those, records whose codes were read (the rest
+                            belong to clusters passed over as a whole) */
 } e8q_search_stats;
 
 /* Search modes. PLAIN keeps the reference's per-query schedule bit for bit

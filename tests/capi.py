@@ -32,7 +32,7 @@ class IndexArrays(ctypes.Structure):
 
 class SearchStats(ctypes.Structure):
     _fields_ = [("candidates", u64), ("survivors", u64), ("batches", u64), ("scan_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double)]
+                ("total_ms", ctypes.c_double), ("scanned", u64)]
 
 
 class SearchArgs(ctypes.Structure):
