@@ -81,11 +81,11 @@ __device__ __forceinline__ uint64_t ReadLane64(uint32_t lo, uint32_t hi, uint32_
 
 template <int CTRL, int ROWMASK = 0xF>
 __device__ __forceinline__ uint32_t Dpp(uint32_t v) {
-    return static_cast<uint32_t>(
-        __builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), CTRL, ROWMASK, 0xF, false));
+    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), CTRL, ROWMASK, 0xF, true));
 }
 
-// Unsigned max over each 32-lane half of the wave; valid in lanes 31 and 63.
+// Unsigned max over each 32-lane half of the wave; valid in lanes 31 and 63
+// (the other lanes of rows 0 and 2 end with unspecified values).
 __device__ __forceinline__ uint32_t HalfMax(uint32_t v) {
     v = max(v, Dpp<0xB1>(v));        // quad_perm [1,0,3,2]
     v = max(v, Dpp<0x4E>(v));        // quad_perm [2,3,0,1]
@@ -636,24 +636,36 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             const uint32_t bj = cb0 + j;
             const bool act = j < jn;
             const float L = limit;
-            float t1 = inf, pp = 0.0F, lbp = 0.0F, pam = 0.0F;
+            // Header terms of the ZPR bound.  Batch bound:
+            // (sc0 + d2 - pad_add) - |W|max * max|S| - Omega_max * z * r_h.
+            float lbp = 0.0F, pam = 0.0F, vbase = 0.0F;
+            bool uskip = !act;
+            if (act) {
+                const float sc0 = rc.s0.x, sc3 = rc.s0.w, sc4 = rc.s1.x, sc5 = rc.s1.y, sc6 = rc.s1.z;
+                const float pad_w_round = (0.5F * sc3) * sml;
+                vbase = (sc0 + d2) - pad_w_round;
+                const float pad_add = fmaf(lut_err, sc6, pad_w_round);
+                lbp = (sc0 + d2) - pad_add;
+                pam = fmaf(sc5, 255.0F, sc4);
+                // No code has |S| above U and the bound only grows as max|S|
+                // falls, so a batch whose bound with max|S| = U reaches the
+                // threshold is one the reference skips: its coarse sums are
+                // not needed.
+                if (prune && L < inf) uskip = fmaf(-pam, zrh, fmaf(-sc6, sk.U, lbp)) >= L;
+            }
+            if (__ballot(uskip) != ~uint64_t{0}) {   // some batch of the step needs its sums
+            float t1 = inf, pp = 0.0F;
             uint32_t sabs = 0;
             if (act) {
                 const int32_t acc = Coarse<NQW>(rc, lutc, blocks);
-                const float sc0 = rc.s0.x, sc1 = rc.s0.y, sc2 = rc.s0.z, sc3 = rc.s0.w;
-                const float sc4 = rc.s1.x, sc5 = rc.s1.y, sc6 = rc.s1.z;
-                const float pad_w_round = (0.5F * sc3) * sml;
-                const float vbase = (sc0 + d2) - pad_w_round;
+                const float sc1 = rc.s0.y, sc2 = rc.s0.z, sc3 = rc.s0.w;
+                const float sc4 = rc.s1.x, sc5 = rc.s1.y;
                 const float scv = fmaf(static_cast<float>(acc), delta, base);
                 const float bt = fmaf(static_cast<float>(rc.hb), sc1, vbase);
                 const float w = fmaf(static_cast<float>(rc.hw), sc3, sc2);
                 pp = fmaf(static_cast<float>(rc.hp), sc5, sc4);
                 const float dc = fmaf(-w, scv, bt);
                 t1 = fmaf(-fabsf(w), lut_err, dc);
-                // Batch bound: (sc0 + d2 - pad_add) - |W|max * max|S| - Omega_max * z * r_h.
-                const float pad_add = fmaf(lut_err, sc6, pad_w_round);
-                lbp = (sc0 + d2) - pad_add;
-                pam = fmaf(sc5, 255.0F, sc4);
                 sabs = __float_as_uint(fabsf(scv));
             }
             // Batch max |S| over its 32 lanes: S is monotone in the integer
@@ -736,6 +748,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                 limit = UniformF(limit);
                 ch.active = Uniform64(ch.active);
                 horizon();
+            }
             }
             if (cb0 + 2 >= cnb && staged > 0) {   // end of the cluster
                 __syncthreads();
