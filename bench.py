@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--bits", type=int, default=4)
     ap.add_argument("--grid", default="16,32,64,128")
     ap.add_argument("--target", type=float, default=0.95)
-    ap.add_argument("--modes", default="plain")
+    ap.add_argument("--modes", default="plain,batched")
     ap.add_argument("--latency-queries", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-sample", type=int, default=2000)

@@ -12,7 +12,7 @@ struct PlainArgs {
     uint64_t nq;
     const float* rotq;        // [nq][padded] rotated, centered queries
     const uint8_t* lut;       // [nq][blocks*32] compact u8 LUT
-    const float* consts;      // [nq][8]: delta, base, lut_err, score_max, qn2
+    const float* consts;      // [nq][8]: delta, base, lut_err, score_max, qn2, U (max |S| of any code)
     const uint32_t* probe_ids;  // [nq][maxp]
     const float* probe_d2;      // [nq][maxp]
     uint32_t maxp;
@@ -25,6 +25,8 @@ struct PlainArgs {
     uint32_t* out_ids;          // [nq][k]
     float* out_dists;
     unsigned long long* stats;  // [4] candidates, survivors, batches, records read
+    const uint32_t* qsel;       // optional: search only queries qsel[0..nsel)
+    uint64_t nsel;
 };
 
 hipError_t LaunchRotate(hipStream_t st, const float* x, uint64_t rows, uint32_t dim, const IndexView& ix, float* out,
@@ -47,6 +49,83 @@ struct PlainParams {
 hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a, PlainParams* dparams);
 hipError_t LaunchCoarseAccumulate(hipStream_t st, const uint8_t* rowa, uint64_t nbatch, const uint8_t* lut, int blocks,
                                   int32_t* out);
+
+// ---- batched search ---------------------------------------------------------
+// One unit of the persistent batched scan: a tile of the (query, probe)
+// pairs of one cluster against a range of the cluster's 32-vector batches.
+struct BatchItem {
+    uint32_t c;    // cluster
+    uint32_t p0;   // first pair (index into the pair list)
+    uint32_t np;   // pairs in the tile
+    uint32_t b0;   // first batch
+    uint32_t nb;   // batches
+    uint32_t pad[3];
+};
+
+struct BatchedArgs {
+    uint64_t nq;
+    uint32_t maxp;
+    const uint32_t* nprobe_q;   // optional [nq]
+    uint32_t nprobe_all;
+    const uint32_t* probe_ids;  // [nq][maxp]
+    const float* probe_d2;      // [nq][maxp]
+    const uint8_t* lut;         // [nq][blocks*32]
+    const float* consts;        // [nq][8] (see PlainArgs)
+    const float* rotq;          // [nq][padded]
+    uint32_t k;
+    float z;
+    int disable_prune;
+    const uint32_t* first_ids;  // [nq][k] top-k over the first probe
+    const float* first_dists;
+    uint32_t* ccount;           // [nlist] pairs per cluster
+    uint32_t* cfill;            // [nlist] scatter cursor
+    uint32_t* coff;             // [nlist+1] pair offsets
+    uint32_t* ioff;             // [nlist+1] item offsets
+    uint64_t* pairs;            // (query << 32) | probe position, grouped by cluster
+    BatchItem* items;
+    uint64_t items_cap;         // capacity of items (an upper bound of the count)
+    uint32_t* counters;         // [0] items, [1] next item, [2] overflowed queries
+    uint32_t qtile, btile;      // pairs per item, batches per item
+    uint64_t* cand;             // [nq][cap] (probe position << 32) | vector offset
+    uint32_t* ncand;            // [nq]
+    uint32_t cap;
+    uint32_t* overflow;         // [nq] queries whose candidates did not fit
+    uint32_t* out_ids;          // [nq][k]
+    float* out_dists;
+    unsigned long long* stats;  // [4] as PlainArgs
+};
+
+struct BatchedParams {
+    IndexView ix;
+    BatchedArgs a;
+};
+
+constexpr uint32_t kBatchCandCap = 1024;   // candidates kept per query (more: plain search)
+
+// Default item shape of the batched scan: up to 32 batches (~24 KB of
+// records) and up to 32 pairs (~16 KB of LUTs) per item.
+inline void BatchTiles(const IndexView& ix, uint32_t* qtile, uint32_t* btile) {
+    uint32_t bt = (24576u / ix.batch_bytes) & ~1u;
+    bt = bt < 2 ? 2 : (bt > 32 ? 32 : bt);
+    uint32_t qt = 16384u / (static_cast<uint32_t>(ix.blocks) * 32u);
+    qt = qt < 4 ? 4 : (qt > 32 ? 32 : qt);
+    *qtile = qt;
+    *btile = bt;
+}
+
+// The work list of the pairs (q, p), 1 <= p < nprobe(q), of non-empty
+// clusters: count, scan, scatter, items.  Fills the candidate and batch
+// counts of those pairs into stats.
+hipError_t LaunchBatchedWorkList(hipStream_t st, const IndexView& ix, const BatchedArgs& a, BatchedParams* dparams);
+// The persistent scan over the items; `grid` workgroups.
+size_t BatchedScanLds(const IndexView& ix, uint32_t qtile, uint32_t btile);
+hipError_t LaunchBatchedScan(hipStream_t st, const IndexView& ix, const BatchedArgs& a, const BatchedParams* dparams,
+                             uint32_t grid);
+// Refine of every query's candidates merged into its first-probe top-k.
+hipError_t LaunchBatchedRefine(hipStream_t st, const IndexView& ix, const BatchedArgs& a,
+                               const BatchedParams* dparams);
+// Fill a device buffer with a 32-bit value (graph-safe, no memset nodes).
+hipError_t LaunchFill32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
 
 // Build (encode) path.
 struct EncodeArgs {

@@ -65,13 +65,38 @@ __global__ void QueryPrepKernel(const float* __restrict__ rotq, uint64_t nq, Ind
         const float delta = hc > 0.0F ? 2.0F * hc / 255.0F : 1.0F;
         const float qinv = 1.0F / delta;
         const float qoff = hc / delta;
-        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+        auto quant = [&](uint32_t e) {
             const float xq = fmaf(cf[e], qinv, qoff);
             const float r = rintf(xq);
             int iv = (r >= -2147483648.0F && r < 2147483648.0F) ? static_cast<int>(r) : INT32_MIN;
-            iv = iv < 0 ? 0 : (iv > 255 ? 255 : iv);
-            lut[qi * E + e] = static_cast<uint8_t>(iv);
+            return iv < 0 ? 0 : (iv > 255 ? 255 : iv);
+        };
+        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) lut[qi * E + e] = static_cast<uint8_t>(quant(e));
+        // U: no code's coarse score S = acc*delta + base exceeds it in
+        // magnitude, since acc lies between the sums of the 2*B tables'
+        // minima and maxima and S is monotone in acc (the search kernels'
+        // header-only batch and cluster skips use it).
+        int tmin = 0, tmax = 0;
+        for (uint32_t tb = threadIdx.x; tb < 2u * B; tb += blockDim.x) {
+            int mn = 255, mx = 0;
+            for (uint32_t v = 0; v < 16; ++v) {
+                const int iv = quant((tb >> 1) * 32 + (tb & 1) * 16 + v);
+                mn = min(mn, iv);
+                mx = max(mx, iv);
+            }
+            tmin += mn;
+            tmax += mx;
         }
+        for (int o = 32; o > 0; o >>= 1) {
+            tmin += __shfl_xor(tmin, o);
+            tmax += __shfl_xor(tmax, o);
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) {
+            red[2 * (threadIdx.x >> 6)] = __int_as_float(tmin);
+            red[2 * (threadIdx.x >> 6) + 1] = __int_as_float(tmax);
+        }
+        __syncthreads();
         if (threadIdx.x < 8) {
             // QueryNorm2Avx2: 8 FMA chains over d = l, l+8, ..., then Hsum.
             float a = 0.0F;
@@ -84,6 +109,13 @@ __global__ void QueryPrepKernel(const float* __restrict__ rotq, uint64_t nq, Ind
                 c[2] = delta * (sqrtf(static_cast<float>(B)) * 2.0F);
                 c[3] = static_cast<float>(B) * fmaf(delta, 255.0F, hc * 2.0F);
                 c[4] = qn2;
+                int smin = 0, smax = 0;
+                for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) {
+                    smin += __float_as_int(red[2 * w]);
+                    smax += __float_as_int(red[2 * w + 1]);
+                }
+                c[5] = fmaxf(fabsf(fmaf(static_cast<float>(smin), delta, c[1])),
+                             fabsf(fmaf(static_cast<float>(smax), delta, c[1])));
             }
         }
         __syncthreads();

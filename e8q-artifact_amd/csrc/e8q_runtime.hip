@@ -79,6 +79,7 @@ struct DBuf {
 
 struct e8q_ctx {
     int device = 0;
+    int cus = 0;      // compute units (the persistent batched scan's grid)
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // search timing
     DBuf ws;   // per-call workspace, grown on demand
@@ -221,6 +222,7 @@ e8q_status e8q_ctx_create(int device, e8q_ctx** out) {
         try {
             c->Bind();
             Check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+            Check(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
             for (auto& e : c->ev) Check(hipEventCreate(&e), "hipEventCreate");
         } catch (...) {
             delete c;
@@ -487,6 +489,48 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         }
         cv.add(&dstats, 4);
         cv.add(&dparams, 1);
+        // Batched form: first-probe lists, work list, candidates.
+        const bool batched = a->mode == E8Q_MODE_BATCHED;
+        BatchedArgs ba{};
+        BatchedParams* bparams = nullptr;
+        uint32_t* first_ids = nullptr;
+        float* first_dists = nullptr;
+        uint64_t items_cap = 0;
+        if (batched) {
+            BatchTiles(v, &ba.qtile, &ba.btile);
+            uint64_t pairs = 0;
+            if (a->nprobe_per_query) {
+                for (uint64_t i = 0; i < nq; ++i)
+                    pairs += std::min<uint64_t>(a->nprobe_per_query[i], v.nlist) - 1;
+            } else {
+                pairs = nq * (maxp - 1);
+            }
+            uint64_t sum_nbt = 0, max_nbt = 0;
+            for (uint64_t c = 0; c < v.nlist; ++c) {
+                const uint64_t nb = (x->cluster_begin[c + 1] - x->cluster_begin[c] + 31) / 32;
+                const uint64_t nbt = (nb + ba.btile - 1) / ba.btile;
+                sum_nbt += nbt;
+                max_nbt = std::max(max_nbt, nbt);
+            }
+            // items of a cluster = ceil(pairs_c / qtile) * ceil(nb_c / btile)
+            items_cap = std::min((nq + ba.qtile - 1) / ba.qtile * sum_nbt,
+                                 (pairs + ba.qtile - 1) / ba.qtile * max_nbt + sum_nbt);
+            ba.cap = kBatchCandCap;
+            cv.add(&first_ids, nq * a->k);
+            cv.add(&first_dists, nq * a->k);
+            cv.add(&ba.ccount, v.nlist);
+            cv.add(&ba.cfill, v.nlist);
+            cv.add(&ba.coff, v.nlist + 1);
+            cv.add(&ba.ioff, v.nlist + 1);
+            cv.add(&ba.pairs, std::max<uint64_t>(pairs, 1));
+            cv.add(&ba.items, std::max<uint64_t>(items_cap, 1));
+            ba.items_cap = items_cap;
+            cv.add(&ba.counters, 4);
+            cv.add(&ba.cand, nq * ba.cap);
+            cv.add(&ba.ncand, nq);
+            cv.add(&ba.overflow, nq);
+            cv.add(&bparams, 1);
+        }
         ctx->Take(cv);
         const float* qsrc = a->queries;
         if (!a->queries_on_device) {
@@ -526,8 +570,53 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         pa.out_dists = odist;
         pa.stats = dstats;
         Check(hipEventRecord(ctx->ev[1], st), "event");
-        Check(LaunchPlainSearch(st, v, pa, dparams), "plain search");
+        if (!batched) {
+            Check(LaunchPlainSearch(st, v, pa, dparams), "plain search");
+        } else {
+            // Threshold pass: the plain search over each query's first probe.
+            PlainArgs fa = pa;
+            fa.nprobe_q = nullptr;
+            fa.nprobe_all = 1;
+            fa.out_ids = first_ids;
+            fa.out_dists = first_dists;
+            ba.first_ids = first_ids;
+            ba.first_dists = first_dists;
+            Check(LaunchPlainSearch(st, v, fa, dparams), "first-probe search");
+            ba.nq = nq;
+            ba.maxp = maxp;
+            ba.nprobe_q = npq;
+            ba.nprobe_all = maxp;
+            ba.probe_ids = pids;
+            ba.probe_d2 = pd2;
+            ba.lut = lut;
+            ba.consts = consts;
+            ba.rotq = rotq;
+            ba.k = static_cast<uint32_t>(a->k);
+            ba.z = a->z;
+            ba.disable_prune = a->disable_prune ? 1 : 0;
+            ba.out_ids = oids;
+            ba.out_dists = odist;
+            ba.stats = dstats;
+            Check(LaunchFill32(st, ba.ccount, v.nlist, 0), "fill");
+            Check(LaunchFill32(st, ba.ncand, nq, 0), "fill");
+            Check(LaunchBatchedWorkList(st, v, ba, bparams), "work list");
+            const uint32_t grid = static_cast<uint32_t>(std::max(1, ctx->cus) * 2);
+            Check(LaunchBatchedScan(st, v, ba, bparams, grid), "batched scan");
+            Check(LaunchBatchedRefine(st, v, ba, bparams), "batched refine");
+        }
         Check(hipEventRecord(ctx->ev[2], st), "event");
+        if (batched) {   // queries whose candidate lists overflowed: the plain search
+            uint32_t novf = 0;
+            Check(hipMemcpyAsync(&novf, ba.counters + 2, 4, hipMemcpyDeviceToHost, st), "download counters");
+            Check(hipStreamSynchronize(st), "batched search");
+            if (novf > 0) {
+                PlainArgs oa = pa;
+                oa.qsel = ba.overflow;
+                oa.nsel = novf;
+                oa.stats = nullptr;   // their candidates and batches are counted already
+                Check(LaunchPlainSearch(st, v, oa, dparams), "plain search of overflowed queries");
+            }
+        }
         if (!a->outputs_on_device) {
             Check(hipMemcpyAsync(a->out_ids, oids, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
             Check(hipMemcpyAsync(a->out_dists, odist, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");

@@ -17,135 +17,9 @@
 // waiting and at the end of each cluster; the top-k is a sorted (dist, id)
 // list in LDS whose k-th entry is the reference heap's root, the prune
 // threshold.
-#include "e8q_device.h"
-#include "e8q_kernels.h"
+#include "e8q_scan_dev.h"
 
 namespace e8qdev {
-
-constexpr int kWave = 64;       // one wave per query
-constexpr int kMaxFlush = 256;
-constexpr int kGroup = 32;      // staged rows refined and merged together
-
-__device__ __forceinline__ bool Better(float d1, uint32_t i1, float d2, uint32_t i2) {
-    return d1 < d2 || (d1 == d2 && i1 < i2);
-}
-
-// Global-memory views of pointers read from the parameter block: the
-// compiler cannot infer their address space, and generic (flat) loads would
-// also count against the LDS wait counter.
-#define E8Q_GLOBAL __attribute__((address_space(1)))
-// Plain clang vectors (no constructors) for loads through such pointers.
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-template <class T>
-__device__ __forceinline__ const E8Q_GLOBAL T* G(const T* p) {
-    return (const E8Q_GLOBAL T*)(p);
-}
-template <class T>
-__device__ __forceinline__ E8Q_GLOBAL T* Gw(T* p) {
-    return (E8Q_GLOBAL T*)(p);
-}
-
-// Values made opaque to the optimizer where they are used, so that it does
-// not hoist loop-invariant lane masks and parameter loads of the rarely taken
-// paths to the kernel entry, where they would hold scalar registers for the
-// whole kernel.
-__device__ __forceinline__ uint32_t LaneHere() {
-    uint32_t v = threadIdx.x;
-    asm volatile("" : "+v"(v));
-    return v;
-}
-template <class T>
-__device__ __forceinline__ const T* Here(const T* p) {
-    asm volatile("" : "+s"(p));
-    return p;
-}
-
-__device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ float UniformF(float v) {
-    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
-}
-__device__ __forceinline__ uint64_t Uniform64(uint64_t v) {
-    return (static_cast<uint64_t>(Uniform(static_cast<uint32_t>(v >> 32))) << 32) | Uniform(static_cast<uint32_t>(v));
-}
-template <class T>
-__device__ __forceinline__ const T* UniformPtr(const T* p) {
-    return reinterpret_cast<const T*>(Uniform64(reinterpret_cast<uint64_t>(p)));
-}
-__device__ __forceinline__ uint32_t ReadLane(uint32_t v, uint32_t l) {
-    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(l)));
-}
-__device__ __forceinline__ uint64_t ReadLane64(uint32_t lo, uint32_t hi, uint32_t l) {
-    return (static_cast<uint64_t>(ReadLane(hi, l)) << 32) | ReadLane(lo, l);
-}
-
-template <int CTRL, int ROWMASK = 0xF>
-__device__ __forceinline__ uint32_t Dpp(uint32_t v) {
-    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), CTRL, ROWMASK, 0xF, true));
-}
-
-// Unsigned max over each 32-lane half of the wave; valid in lanes 31 and 63
-// (the other lanes of rows 0 and 2 end with unspecified values).
-__device__ __forceinline__ uint32_t HalfMax(uint32_t v) {
-    v = max(v, Dpp<0xB1>(v));        // quad_perm [1,0,3,2]
-    v = max(v, Dpp<0x4E>(v));        // quad_perm [2,3,0,1]
-    v = max(v, Dpp<0x124>(v));       // row_ror:4
-    v = max(v, Dpp<0x128>(v));       // row_ror:8
-    v = max(v, Dpp<0x142, 0xA>(v));  // row_bcast:15 into rows 1 and 3
-    return v;
-}
-
-// Digit `i` (0..7) and the branch of one (lane, block) from the rowA byte and
-// the (bits-1)-bit refine field (e8q_scan.cpp:34-90).
-__device__ __forceinline__ uint32_t FieldBits(uint64_t lo, uint64_t hi, int pos, int n) {
-    const uint64_t m = (uint64_t{1} << n) - 1;
-    if (pos + n <= 64) return static_cast<uint32_t>((lo >> pos) & m);
-    if (pos >= 64) return static_cast<uint32_t>((hi >> (pos - 64)) & m);
-    return static_cast<uint32_t>(((lo >> pos) | (hi << (64 - pos))) & m);
-}
-
-__device__ __forceinline__ void DecodeDigit(int bits, uint8_t rb, const uint8_t* field, int i, uint32_t* digit,
-                                            int* branch) {
-    if (bits == 1) {
-        const uint32_t par = __popc(static_cast<uint32_t>(rb & 0xFE)) & 1u;
-        *digit = i < 7 ? ((rb >> (7 - i)) & 1u) : par;
-        *branch = rb & 1;
-        return;
-    }
-    const int c = bits - 1;
-    uint64_t lo = 0, hi = 0;
-    for (int b = 0; b < c; ++b) {
-        const uint64_t v = field[b];
-        if (b < 8) lo |= v << (8 * b);
-        else hi |= v << (8 * (b - 8));
-    }
-    const uint32_t f7 = FieldBits(lo, hi, c * 7, c);
-    *branch = static_cast<int>(f7 & 1u);
-    if (i < 7) {
-        *digit = FieldBits(lo, hi, c * i, c) | (((rb >> (7 - i)) & 1u) << c);
-    } else {
-        uint32_t par = 0;
-        for (int j = 0; j < 7; ++j) par ^= FieldBits(lo, hi, c * j, 1);
-        *digit = (f7 & ~1u) | par | ((rb & 1u) << c);
-    }
-}
-
-// Fixed-size LDS state of one query.
-struct PlainShared {
-    float tab[512];              // refine level table (bits <= 8)
-    float t1[kWave];             // exact replay: ZPR bound minus the slack term
-    float pp[kWave];             //               perp (Omega) per lane
-    float lbp[2], pam[2];        //               batch bound parts
-    float td[kMaxK], nd[kMaxK];  // sorted top-k (dist, id) and merge target
-    uint32_t ti[kMaxK], ni[kMaxK];
-    float cd[kGroup];            // refined rows of the group
-    uint32_t ci[kGroup];
-    float ad[kGroup];            // ... those that can enter the top-k
-    uint32_t ai[kGroup];
-    float rg1[kGroup], rh3[kGroup];
-    uint32_t rid[kGroup];
-    uint32_t staged[kMaxFlush];  // staged rows: vector offset inside the cluster
-};
 
 // The query LUT first, at LDS address 0, so lookups fold the block offset
 // into the instruction.
@@ -278,20 +152,6 @@ __device__ __forceinline__ void LoadRec(Rec<NQW>& r, const uint8_t* rec_p, uint3
     r.s1 = *reinterpret_cast<const E8Q_GLOBAL f32x4*>(rec + 112);
 }
 
-// Eight blocks of coarse sum: bytes of x.x are blocks 0..3, of x.y 4..7; the
-// LUT holds per block 16 low-nibble then 16 high-nibble entries.
-__device__ __forceinline__ int32_t Coarse8(u32x2 x, const uint8_t* l) {
-    int32_t s = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-        s += l[b * 32 + __builtin_amdgcn_ubfe(x.x, 8 * b, 4)] + l[b * 32 + 16 + __builtin_amdgcn_ubfe(x.x, 8 * b + 4, 4)];
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-        s += l[128 + b * 32 + __builtin_amdgcn_ubfe(x.y, 8 * b, 4)] +
-             l[128 + b * 32 + 16 + __builtin_amdgcn_ubfe(x.y, 8 * b + 4, 4)];
-    return s;
-}
-
 template <int NQW>
 __device__ __forceinline__ int32_t Coarse(const Rec<NQW>& r, const uint8_t* lut, int blocks) {
     int32_t acc = 0;
@@ -304,179 +164,6 @@ __device__ __forceinline__ int32_t Coarse(const Rec<NQW>& r, const uint8_t* lut,
         for (int w = 0; w < nw; ++w) acc += Coarse8(rw[w], lut + w * 256);
     }
     return acc;
-}
-
-// The open cluster, for the refine.
-struct ClusterRef {
-    const uint8_t* arena_c;
-    const uint8_t* ref_c;
-    uint64_t vb;
-    float d2;
-};
-
-// Refines the staged rows, kGroup at a time, and merges them into the
-// sorted top-k list.  The rows' code bytes (rowA bytes and the
-// (bits-1)-byte refine fields) are gathered into LDS with independent loads;
-// then 8 rows x 8 components per pass decode and accumulate in the AVX2
-// order (even blocks in acc, odd in acc2, then the 8-lane horizontal sum).
-// Only rows that beat the current k-th entry take part in the merge.
-template <bool GEN>
-__device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, const PlainParams* prm_,
-                          const float* qv, const ClusterRef& cr, uint8_t* rowbuf) {
-    const PlainParams* prm = Here(prm_);
-    const IndexView& ix = prm->ix;
-    const int k = static_cast<int>(prm->a.k);
-    const bool tab_lds = ix.bits <= 8;
-    const int lane = static_cast<int>(LaneHere());
-    const int B = ix.blocks;
-    const int c = ix.bits - 1;
-    const int fbytes = ix.bits >= 2 ? B * c : 0;
-    const int rstride = B + fbytes;        // multiple of 8
-    const int wpr = rstride / 8, bw = B / 8;
-    for (int rb = 0; rb < ns; rb += kGroup) {
-        const int nn = min(kGroup, ns - rb);
-        __syncthreads();
-        // Word e of the group: row e % 32, word e / 32 (rowA words, then
-        // refine-field words); four loads in flight per lane.
-        const int nw = kGroup * wpr;
-        for (int e0 = lane; e0 < nw; e0 += 4 * kWave) {
-            u32x2 v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = e0 + u * kWave;
-                const int r = e & (kGroup - 1), w = e >> 5;
-                v[u] = u32x2{0u, 0u};
-                if (e < nw && r < nn) {
-                    const uint32_t off = S.staged[rb + r];
-                    const uint8_t* src =
-                        w < bw ? cr.arena_c + static_cast<uint64_t>(off >> 5) * ix.batch_bytes + kHdr + (off & 31) * B +
-                                     8 * w
-                               : cr.ref_c + static_cast<uint64_t>(off >> 5) * ix.ref_batch_bytes +
-                                     (off & 31) * ix.refine_stride + 8 * (w - bw);
-                    v[u] = *reinterpret_cast<const E8Q_GLOBAL u32x2*>(G(src));
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = e0 + u * kWave;
-                const int r = e & (kGroup - 1), w = e >> 5;
-                if (e < nw && r < nn) *reinterpret_cast<u32x2*>(rowbuf + r * rstride + 8 * w) = v[u];
-            }
-        }
-        if (lane < nn) {
-            const uint32_t off = S.staged[rb + lane];
-            const E8Q_GLOBAL float* g1b = reinterpret_cast<const E8Q_GLOBAL float*>(
-                G(cr.ref_c) + static_cast<uint64_t>(off >> 5) * ix.ref_batch_bytes + ix.refine_stride * 32);
-            S.rg1[lane] = g1b[off & 31];
-            S.rh3[lane] = cr.d2 + g1b[32 + (off & 31)];
-            S.rid[lane] = G(ix.ids)[cr.vb + off];
-        }
-        __syncthreads();
-        for (int sub = 0; sub < nn; sub += 8) {
-            const int r = sub + (lane >> 3);
-            const int i = lane & 7;
-            if (r < nn) {
-                const uint8_t* plane = rowbuf + r * rstride;
-                const uint8_t* field = plane + B;
-                float acc = 0.0F, acc2 = 0.0F;
-                if constexpr (!GEN) {   // bits 2..5: the field fits 32 bits
-                    // The block's field lies in the 8 bytes from its 4-byte
-                    // aligned start (the field area of a row is 8-aligned).
-                    const uint32_t fmask = c >= 4 ? ~0u : (1u << (8 * c)) - 1u;
-                    const uint32_t cm = (1u << c) - 1u;
-                    uint32_t pm = 0;                                // LSB positions of digits 0..6
-                    for (int q = 0; q < 7; ++q) pm |= 1u << (c * q);
-                    for (int blk = 0; blk < B; ++blk) {
-                        const uint32_t bo = static_cast<uint32_t>(blk * c);
-                        const uint32_t* wp = reinterpret_cast<const uint32_t*>(field + (bo & ~3u));
-                        const uint64_t win = (static_cast<uint64_t>(wp[1]) << 32) | wp[0];
-                        const uint32_t f = static_cast<uint32_t>(win >> (8 * (bo & 3u))) & fmask;
-                        const uint32_t rbv = plane[blk];
-                        const uint32_t f7 = (f >> (7 * c)) & cm;
-                        const uint32_t br = f7 & 1u;
-                        const uint32_t lo = i < 7 ? ((f >> (c * i)) & cm) : ((f7 & ~1u) | (__popc(f & pm) & 1u));
-                        const uint32_t digit = lo | (((rbv >> (7 - i)) & 1u) << c);
-                        const uint32_t ti = br * ix.tab_stride + digit;
-                        const float lv = tab_lds ? S.tab[ti] : G(ix.refine_tab)[ti];
-                        if (blk & 1) acc2 = fmaf(lv, qv[blk * 8 + i], acc2);
-                        else acc = fmaf(lv, qv[blk * 8 + i], acc);
-                    }
-                } else {
-                    for (int blk = 0; blk < B; ++blk) {
-                        uint32_t digit;
-                        int br;
-                        DecodeDigit(ix.bits, plane[blk], field + blk * c, i, &digit, &br);
-                        const uint32_t ti = static_cast<uint32_t>(br) * ix.tab_stride + digit;
-                        const float lv = tab_lds ? S.tab[ti] : G(ix.refine_tab)[ti];
-                        if (blk & 1) acc2 = fmaf(lv, qv[blk * 8 + i], acc2);
-                        else acc = fmaf(lv, qv[blk * 8 + i], acc);
-                    }
-                }
-                const float sc = Hsum8Lanes(acc + acc2);
-                if (i == 0) {
-                    S.cd[r] = fmaf(-(2.0F * S.rg1[r]), sc, S.rh3[r]);
-                    S.ci[r] = S.rid[r];
-                }
-            }
-        }
-        __syncthreads();
-        // Rows that cannot enter the top-k (not better than a full list's
-        // k-th entry) are dropped; the rest are ranked against the list and
-        // each other ((dist, id) is a strict total order: ids are unique).
-        bool take = false;
-        float d = 0.0F;
-        uint32_t id = 0;
-        if (lane < nn) {
-            d = S.cd[lane];
-            id = S.ci[lane];
-            take = cnt < k || Better(d, id, S.td[k - 1], S.ti[k - 1]);
-        }
-        const uint64_t am = __ballot(take);
-        const int m = __popcll(am);
-        if (m > 0) {
-            if (take) {
-                const int pos = __popcll(am & ((uint64_t{1} << lane) - 1));
-                S.ad[pos] = d;
-                S.ai[pos] = id;
-            }
-            __syncthreads();
-            for (int e = lane; e < cnt; e += kWave) {
-                const float ed = S.td[e];
-                const uint32_t ei = S.ti[e];
-                int rank = e;
-                for (int q = 0; q < m; ++q) rank += Better(S.ad[q], S.ai[q], ed, ei) ? 1 : 0;
-                if (rank < k) {
-                    S.nd[rank] = ed;
-                    S.ni[rank] = ei;
-                }
-            }
-            if (lane < m) {
-                const float nd0 = S.ad[lane];
-                const uint32_t ni0 = S.ai[lane];
-                int lo = 0, hi = cnt;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (Better(S.td[mid], S.ti[mid], nd0, ni0)) lo = mid + 1;
-                    else hi = mid;
-                }
-                int rank = lo;
-                for (int q = 0; q < m; ++q) rank += Better(S.ad[q], S.ai[q], nd0, ni0) ? 1 : 0;
-                if (rank < k) {
-                    S.nd[rank] = nd0;
-                    S.ni[rank] = ni0;
-                }
-            }
-            __syncthreads();
-            cnt = min(k, cnt + m);
-            for (int e = lane; e < cnt; e += kWave) {
-                S.td[e] = S.nd[e];
-                S.ti[e] = S.ni[e];
-            }
-            __syncthreads();
-            limit = cnt == k ? UniformF(S.td[k - 1]) : __builtin_inff();
-        }
-    }
-    __syncthreads();
 }
 
 // One wave per query.  Each step covers two 32-vector batches (lanes 0..31
@@ -521,7 +208,9 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
     const uint8_t* const arena = UniformPtr(ix.arena);
     const float z = a.z;
     const bool prune = !a.disable_prune;
-    for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
+    const uint64_t nrun = a.qsel ? a.nsel : a.nq;
+    for (uint64_t ii = blockIdx.x; ii < nrun; ii += gridDim.x) {
+        const uint64_t qi = a.qsel ? G(a.qsel)[ii] : ii;
         for (uint32_t i = lane; i < P; i += kWave) qv[i] = G(a.rotq)[qi * P + i];
         for (uint32_t i = lane; 4 * i < E; i += kWave)
             reinterpret_cast<uint32_t*>(lutc)[i] = reinterpret_cast<const E8Q_GLOBAL uint32_t*>(G(a.lut) + qi * E)[i];
@@ -531,26 +220,8 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
         const float sml = UniformF(cst[3]) + lut_err;   // score_max + lut_err
         const uint32_t np = min(static_cast<uint32_t>(ix.nlist), a.nprobe_q ? Uniform(G(a.nprobe_q)[qi]) : a.nprobe_all);
         __syncthreads();
-        // U: the integer sum of any code lies between the sums of the
-        // per-table minima and maxima, and S is monotone in it.
-        int32_t tmin = 0, tmax = 0;
-        for (uint32_t tb = lane; tb < 2u * static_cast<uint32_t>(ix.blocks); tb += kWave) {
-            const uint8_t* tp = lutc + (tb >> 1) * 32 + (tb & 1) * 16;
-            int32_t mn = 255, mx = 0;
-            for (int e = 0; e < 16; ++e) {
-                mn = min(mn, static_cast<int32_t>(tp[e]));
-                mx = max(mx, static_cast<int32_t>(tp[e]));
-            }
-            tmin += mn;
-            tmax += mx;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            tmin += __shfl_xor(tmin, o);
-            tmax += __shfl_xor(tmax, o);
-        }
         SkipConsts sk;
-        sk.U = UniformF(fmaxf(fabsf(fmaf(static_cast<float>(tmin), delta, base)),
-                              fabsf(fmaf(static_cast<float>(tmax), delta, base))));
+        sk.U = UniformF(cst[5]);
         sk.lut_err = lut_err;
         sk.sml = sml;
         sk.z = z;
@@ -577,8 +248,8 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
         auto horizon = [&]() { zrh = z * (limit < d2 ? sqrtf(fmaxf(0.0F, limit)) : sqrt_d2); };
         auto cluster_ref = [&]() {
             const uint32_t l = cp - ch.base;
-            return ClusterRef{arena_c, ix.ref_arena + ReadLane64(ch.ro_lo, ch.ro_hi, l), ReadLane64(ch.vb_lo, ch.vb_hi, l),
-                              d2};
+            return ClusterRows{arena_c, ix.ref_arena + ReadLane64(ch.ro_lo, ch.ro_hi, l),
+                               ReadLane64(ch.vb_lo, ch.vb_hi, l), d2, S.staged};
         };
         Rec<NQW> rn;
         cp = Uniform(cp);
@@ -701,7 +372,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                     S.pam[j] = pam;
                 }
                 __syncthreads();
-                const ClusterRef cr = cluster_ref();
+                const ClusterRows cr = cluster_ref();
                 for (uint32_t jj = 0; jj < jn; ++jj) {
                     const uint32_t off = (cb0 + jj) * 32;
                     const uint32_t nl = min(32u, ccnt - off);
@@ -733,7 +404,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                         uint32_t rest = ps;
                         for (int q = 0; q < tk; ++q) rest &= rest - 1;
                         const int pl = __ffs(rest) - 1;
-                        FlushRows<GEN>(S, staged, cnt, limit, prm, qv, cr, rowbuf);
+                        FlushRows<GEN>(S, staged, cnt, limit, &prm->ix, k, qv, cr, rowbuf);
                         Refresh(ch, limit, sk);
                         if (LaneHere() == 0) S.staged[0] = off + static_cast<uint32_t>(pl);
                         staged = 1;
@@ -752,7 +423,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             }
             if (cb0 + 2 >= cnb && staged > 0) {   // end of the cluster
                 __syncthreads();
-                FlushRows<GEN>(S, staged, cnt, limit, prm, qv, cluster_ref(), rowbuf);
+                FlushRows<GEN>(S, staged, cnt, limit, &prm->ix, k, qv, cluster_ref(), rowbuf);
                 Refresh(ch, limit, sk);
                 staged = 0;
                 horizon();
@@ -830,13 +501,14 @@ static hipError_t LaunchPlainT(hipStream_t st, const PlainParams* dp, uint32_t g
 }
 
 hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a, PlainParams* dparams) {
-    if (a.nq == 0) return hipSuccess;
+    const uint64_t nrun = a.qsel ? a.nsel : a.nq;
+    if (nrun == 0) return hipSuccess;
     if (a.flush_every < 1 || a.flush_every > kMaxFlush) return hipErrorInvalidValue;
     if (ix.blocks % 8 != 0) return hipErrorInvalidValue;
     const PlainParams hp{ix, a};
     const hipError_t e = hipMemcpyAsync(dparams, &hp, sizeof(hp), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return e;
-    const uint32_t grid = static_cast<uint32_t>(a.nq < 262144 ? a.nq : 262144);
+    const uint32_t grid = static_cast<uint32_t>(nrun < 262144 ? nrun : 262144);
     const size_t dyn = PlainSearchLds(ix, a.flush_every);
     // Rates 2..5 (refine fields of 1..4 bytes per block) take the specialized
     // kernels; other rates and row sizes the general one.

@@ -64,7 +64,9 @@ typedef struct {
 
 typedef struct {
     uint64_t candidates; /* SearchStats::candidates (e8q.hpp:55-62) */
-    uint64_t survivors;  /* SearchStats::survivors                  */
+    uint64_t survivors;  /* SearchStats::survivors (batched form: vectors refined;
+                            a query searched again by the plain form after its
+                            candidate list overflowed counts its first probe's) */
     uint64_t batches;    /* 32-vector scan records of the probed clusters */
     double scan_ms;      /* scan kernel time, HIP events on the context stream */
     double total_ms;     /* device time of the whole search, same events       */
