@@ -52,10 +52,10 @@ def build_host():
 
 
 def build_oracle():
-    src = os.path.join(ROOT, "oracle", "e8q_oracle.cpp")
+    srcs = [os.path.join(ROOT, "oracle", f) for f in ("e8q_oracle.cpp", "me8q_oracle.cpp")]
     out = os.path.join(ROOT, "oracle", "liborc_e8q.so")
-    run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-fopenmp", src,
-         "-o", out + ".tmp"])
+    run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-fopenmp"] + srcs +
+        ["-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     ref = "/root/reference"
     if os.path.isdir(os.path.join(ref, "src")) and shutil.which("make"):

@@ -26,9 +26,10 @@
 #include <string>
 #include <vector>
 
+#include "orc_common.h"
+
 namespace orc {
 
-constexpr int kMaxBits = 13;
 constexpr double kSigmaC = 1.1180339887498949;  // sqrt(5/4), e8q_codec.hpp:24
 constexpr size_t kMaxK = 128;                   // e8q.hpp:44
 constexpr size_t kHdr = 128;                    // scan-record header, e8q.hpp:193-197
@@ -153,57 +154,6 @@ std::vector<float> RefineTable(int bits, size_t padded) {
 // ---------------------------------------------------------------------------
 // FHT-Kac-4 rotator (rotator.cpp:13-80, kernels_generic.cpp:49-78)
 
-struct Rotator {
-    size_t dim = 0, padded = 0, trunc = 0;
-    float fac = 0.0F;
-    std::vector<uint8_t> signs;
-
-    Rotator() = default;
-    Rotator(size_t d, uint64_t seed) : dim(d) {
-        padded = (d + 63) / 64 * 64;
-        size_t lg = 0;
-        while ((size_t{1} << (lg + 1)) <= d) ++lg;
-        trunc = size_t{1} << lg;
-        fac = 1.0F / std::sqrt(static_cast<float>(trunc));
-        signs.resize(4 * padded / 8);
-        std::mt19937_64 gen(seed);
-        std::uniform_int_distribution<int> dist(0, 255);
-        for (auto& s : signs) s = static_cast<uint8_t>(dist(gen));
-    }
-
-    static void Fht(float* x, size_t len) {
-        for (size_t h = 1; h < len; h <<= 1)
-            for (size_t i = 0; i < len; i += 2 * h)
-                for (size_t j = i; j < i + h; ++j) {
-                    const float a = x[j], b = x[j + h];
-                    x[j] = a + b;
-                    x[j + h] = a - b;
-                }
-    }
-
-    void Rotate(const float* src, float* dst) const {
-        for (size_t i = 0; i < padded; ++i) dst[i] = i < dim ? src[i] : 0.0F;
-        const size_t per_round = padded / 8;
-        for (int r = 0; r < 4; ++r) {
-            const uint8_t* sg = signs.data() + r * per_round;
-            for (size_t i = 0; i < padded; ++i)
-                if ((sg[i >> 3] >> (i & 7)) & 1) dst[i] = -dst[i];
-            float* win = dst + ((trunc == padded || r % 2 == 0) ? 0 : padded - trunc);
-            Fht(win, trunc);
-            for (size_t i = 0; i < trunc; ++i) win[i] *= fac;
-            if (trunc != padded) {
-                const size_t half = padded / 2;
-                for (size_t i = 0; i < half; ++i) {
-                    const float a = dst[i], b = dst[i + half];
-                    dst[i] = a + b;
-                    dst[i + half] = a - b;
-                }
-            }
-        }
-        if (trunc != padded)
-            for (size_t i = 0; i < padded; ++i) dst[i] *= 0.25F;
-    }
-};
 
 // ---------------------------------------------------------------------------
 // E8 nearest-lattice-point encoder (e8q_codec.cpp:142-414, 549-726)
@@ -620,11 +570,6 @@ void Build(Index& ix, const float* base, const float* cent, const uint32_t* cid,
 // ---------------------------------------------------------------------------
 // Query side
 
-// AVX horizontal sum order (simd_helpers.hpp:12-18).
-static inline float Hsum8(const float v[8]) {
-    const float s0 = v[0] + v[4], s1 = v[1] + v[5], s2 = v[2] + v[6], s3 = v[3] + v[7];
-    return (s0 + s2) + (s1 + s3);
-}
 
 // PhiInv / OrthoQuantile (e8q_internal.hpp:77-105); `Phi - p` is contracted.
 static double PhiInv(double p) {

@@ -18,6 +18,17 @@ INDEX_CASES = {
     "d768b4": (700, 768, 6, 4, 5, 0.0, False, 8),
 }
 
+# ME8Q: name -> (n, dim, nlist, avg_bits, seed, prune_alpha, plan_spec, pass_variance,
+#                drop_energy, empty_list, nq)
+ME8Q_CASES = {
+    "m400a4": (1000, 400, 9, 4.0, 0, 0.0, "", False, True, False, 10),
+    "m256sp": (1200, 256, 8, 4.0, 7, 0.01, "64:8,64:3,64:1,64:0", False, True, True, 10),
+    "m128u4": (1000, 128, 6, 4.0, 3, 0.0, "128:4", False, True, False, 10),
+    "m512v3": (900, 512, 7, 3.0, 11, 0.0, "", True, False, False, 10),
+    "m256hi": (900, 256, 5, 4.0, 5, 0.0, "64:13,64:6,128:2", False, True, False, 8),
+    "m64drop": (300, 64, 4, 2.0, 9, 0.0, "", False, True, False, 6),
+}
+
 ROTATE_CASES = [(64, 7), (100, 99), (128, 42), (768, 5), (960, 0x5EED1234), (1024, 1)]
 ENCODE_PADDED = 128
 ENCODE_ROWS = 24
@@ -37,6 +48,30 @@ def mixture(name):
     qlab = rng.integers(0, used, size=nq)
     queries = (centers[qlab] + rng.standard_normal((nq, dim)).astype(np.float32)).astype(np.float32)
     return data, centers, labels, queries
+
+
+def me8q_mixture(name):
+    """Anisotropic mixture with descending per-dimension variance (the PCA
+    domain ME8Q consumes): lambda_i = i^-1.2, centers ~ N(0, 9 diag(lambda)),
+    points = center + N(0, diag(lambda)).  Returns data, centers, labels,
+    queries and the generating variance."""
+    n, dim, nlist, _b, seed, _a, _spec, _pv, _de, empty, nq = ME8Q_CASES[name]
+    rng = np.random.default_rng(1000 + seed)
+    lam = (np.arange(1, dim + 1, dtype=np.float64) ** -1.2).astype(np.float32)
+    sd = np.sqrt(lam).astype(np.float32)
+    centers = (rng.standard_normal((nlist, dim)).astype(np.float32) * (3.0 * sd)).astype(np.float32)
+    used = nlist - 1 if empty else nlist
+    labels = rng.integers(0, used, size=n).astype(np.uint32)
+    data = (centers[labels] + rng.standard_normal((n, dim)).astype(np.float32) * sd).astype(np.float32)
+    data[3] = centers[labels[3]]
+    qlab = rng.integers(0, used, size=nq)
+    queries = (centers[qlab] + rng.standard_normal((nq, dim)).astype(np.float32) * sd).astype(np.float32)
+    return data, centers, labels, queries, lam
+
+
+def me8q_search_grid(name):
+    nlist = ME8Q_CASES[name][2]
+    return [(10, 1), (10, 3), (10, nlist), (100, nlist), (7, nlist)]
 
 
 def search_grid(name):

@@ -12,6 +12,12 @@
 //   route    Q.npy CENT.npy NPROBE OUT_DIST.npy OUT_IDS.npy OUT_CN2.npy
 //   build    DATA.npy CENT.npy IDS.npy BITS SEED ALPHA OUT.e8ax
 //   search   INDEX.e8ax QUERIES.npy K NPROBE DISABLE_PRUNE OUT_PREFIX
+//   mbuild   DATA.npy CENT.npy IDS.npy AVG_BITS SEED ALPHA PLAN_SPEC|- VAR.npy|- DROP_ENERGY
+//            OUT.me8q OUT_PREFIX
+//   msearch  INDEX.me8q QUERIES.npy K NPROBE OUT_PREFIX
+//
+// The ME8Q commands read the index's fast layout (private members) to dump
+// it: the class's `private` is mapped to `public` around its header only.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -22,7 +28,14 @@
 #include <string>
 #include <vector>
 
+#include <array>
+#include <mutex>
+#include <utility>
+
 #include "e8qlib/index/e8q.hpp"
+#define private public
+#include "e8qlib/index/me8q.hpp"
+#undef private
 #include "e8qlib/index/ivf_common.hpp"
 #include "e8qlib/quantization/e8q_codec.hpp"
 #include "e8qlib/utils/rotator.hpp"
@@ -256,6 +269,120 @@ int CmdSearch(char** av) {
     return 0;
 }
 
+int CmdMBuild(char** av) {
+    const Npy data = ReadNpy(av[0]);
+    const Npy cent = ReadNpy(av[1]);
+    const Npy ids = ReadNpy(av[2]);
+    e8qlib::me8q::BuildOptions opt;
+    opt.avg_bits = std::strtof(av[3], nullptr);
+    const uint64_t seed = std::strtoull(av[4], nullptr, 0);
+    if (seed != 0) opt.rot_seed = seed;
+    opt.prune_alpha = std::strtof(av[5], nullptr);
+    if (std::string(av[6]) != "-") opt.plan_spec = av[6];
+    Npy var;
+    const float* varp = nullptr;
+    if (std::string(av[7]) != "-") {
+        var = ReadNpy(av[7]);
+        varp = As<float>(var, "<f4");
+    }
+    opt.drop_energy = std::atoi(av[8]) != 0;
+    opt.threads = 4;
+    const size_t n = data.shape[0], dim = data.shape[1], nl = cent.shape[0];
+    e8qlib::me8q::Me8qIndex idx;
+    idx.Build(As<float>(data, "<f4"), dim, As<float>(cent, "<f4"), dim, As<uint32_t>(ids, "<u4"), n, dim, nl, opt,
+              varp);
+    idx.Save(av[9]);
+    // Fast layout, clusters concatenated.
+    const std::string pre = av[10];
+    std::vector<uint8_t> plane, refine;
+    std::vector<float> gain, vpc;
+    std::vector<double> rcn2;
+    std::vector<uint64_t> off(nl * 4 + 4, 0);
+    for (size_t c = 0; c < nl; ++c) {
+        const auto& f = idx.fast_[c];
+        off[c * 4 + 0] = plane.size();
+        off[c * 4 + 1] = refine.size();
+        off[c * 4 + 2] = gain.size();
+        off[c * 4 + 3] = rcn2.size();
+        plane.insert(plane.end(), f.plane.begin(), f.plane.end());
+        refine.insert(refine.end(), f.refine.begin(), f.refine.end());
+        gain.insert(gain.end(), f.ortho_gain.begin(), f.ortho_gain.end());
+        vpc.insert(vpc.end(), f.ortho_vpc.begin(), f.ortho_vpc.end());
+        rcn2.insert(rcn2.end(), f.rc_n2.begin(), f.rc_n2.end());
+    }
+    off[nl * 4 + 0] = plane.size();
+    off[nl * 4 + 1] = refine.size();
+    off[nl * 4 + 2] = gain.size();
+    off[nl * 4 + 3] = rcn2.size();
+    std::vector<float> cn2(idx.centroid_n2_.begin(), idx.centroid_n2_.end());
+    std::vector<float> crot(idx.cent_rot_.begin(), idx.cent_rot_.end());
+    WriteNpy(pre + "_plane.npy", "|u1", {plane.size()}, plane.data(), plane.size());
+    WriteNpy(pre + "_refine.npy", "|u1", {refine.size()}, refine.data(), refine.size());
+    WriteNpy(pre + "_gain.npy", "<f4", {gain.size()}, gain.data(), gain.size() * 4);
+    WriteNpy(pre + "_vpc.npy", "<f4", {vpc.size()}, vpc.data(), vpc.size() * 4);
+    WriteNpy(pre + "_rcn2.npy", "<f8", {rcn2.size()}, rcn2.data(), rcn2.size() * 8);
+    WriteNpy(pre + "_off.npy", "<u8", {nl + 1, 4}, off.data(), off.size() * 8);
+    WriteNpy(pre + "_cn2.npy", "<f4", {cn2.size()}, cn2.data(), cn2.size() * 4);
+    WriteNpy(pre + "_crot.npy", "<f4", {nl, idx.padded_dim()}, crot.data(), crot.size() * 4);
+    std::vector<int64_t> plan;
+    for (const auto& sg : idx.shared_plan()) {
+        plan.push_back(static_cast<int64_t>(sg.len));
+        plan.push_back(sg.bits);
+    }
+    WriteNpy(pre + "_plan.npy", "<i8", {plan.size() / 2, 2}, plan.data(), plan.size() * 8);
+    const uint64_t rd = idx.route_dim();
+    WriteNpy(pre + "_routedim.npy", "<u8", {1}, &rd, 8);
+    return 0;
+}
+
+int CmdMSearch(char** av) {
+    e8qlib::me8q::Me8qIndex idx;
+    idx.Load(av[0]);
+    const Npy q = ReadNpy(av[1]);
+    const size_t k = std::strtoull(av[2], nullptr, 10);
+    const size_t nprobe = std::strtoull(av[3], nullptr, 10);
+    const std::string pre = av[4];
+    const size_t nq = q.shape[0], dim = q.shape[1], P = idx.padded_dim();
+    const size_t take = std::min(nprobe, idx.num_clusters());
+    std::vector<uint32_t> ids(nq * k), probe(nq * take);
+    std::vector<float> dists(nq * k), rotq(nq * P), pdist(nq * take);
+    std::vector<uint64_t> st(nq * 5);
+    for (size_t i = 0; i < nq; ++i) {
+        e8qlib::me8q::SearchStats stats;
+        idx.SearchIVFFast(As<float>(q, "<f4") + i * dim, k, nprobe, ids.data() + i * k, &stats,
+                          dists.data() + i * k);
+        std::vector<float> qpad;
+        idx.PadQuery(As<float>(q, "<f4") + i * dim, &qpad);
+        std::vector<float> rq(P, 0.0F);
+        idx.PrepareRotatedQuery(qpad.data(), rq.data());
+        std::memcpy(rotq.data() + i * P, rq.data(), P * 4);
+        std::vector<std::pair<float, uint32_t>> cd;
+        idx.RouteQuery(rq.data(), nprobe, &cd);
+        for (size_t p = 0; p < take; ++p) {
+            probe[i * take + p] = cd[p].second;
+            pdist[i * take + p] = cd[p].first;
+        }
+        st[i * 5 + 0] = stats.candidates;
+        st[i * 5 + 1] = stats.survivors;
+        st[i * 5 + 2] = stats.bits_scored;
+        st[i * 5 + 3] = stats.bits_total;
+        st[i * 5 + 4] = stats.pruned;
+    }
+    // The batched entry point must give the same results.
+    std::vector<uint32_t> bids(nq * k);
+    std::vector<float> bd(nq * k);
+    idx.SearchIVFFastBatch(As<float>(q, "<f4"), nq, k, nprobe, bids.data(), nullptr, bd.data(), 3);
+    if (bids != ids || std::memcmp(bd.data(), dists.data(), bd.size() * 4) != 0)
+        throw std::runtime_error("SearchIVFFastBatch differs from SearchIVFFast");
+    WriteNpy(pre + "_ids.npy", "<u4", {nq, k}, ids.data(), ids.size() * 4);
+    WriteNpy(pre + "_dists.npy", "<f4", {nq, k}, dists.data(), dists.size() * 4);
+    WriteNpy(pre + "_stats.npy", "<u8", {nq, 5}, st.data(), st.size() * 8);
+    WriteNpy(pre + "_rotq.npy", "<f4", {nq, P}, rotq.data(), rotq.size() * 4);
+    WriteNpy(pre + "_probe.npy", "<u4", {nq, take}, probe.data(), probe.size() * 4);
+    WriteNpy(pre + "_pdist.npy", "<f4", {nq, take}, pdist.data(), pdist.size() * 4);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -272,6 +399,8 @@ int main(int argc, char** argv) {
         if (cmd == "route" && argc == 8) return CmdRoute(argv + 2);
         if (cmd == "build" && argc == 9) return CmdBuild(argv + 2);
         if (cmd == "search" && argc == 8) return CmdSearch(argv + 2);
+        if (cmd == "mbuild" && argc == 13) return CmdMBuild(argv + 2);
+        if (cmd == "msearch" && argc == 7) return CmdMSearch(argv + 2);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "gen_golden: %s\n", e.what());
         return 1;

@@ -88,6 +88,25 @@ def main():
             res["queries"] = queries
             np.savez(os.path.join(out, f"{name}_search.npz"), **res)
             manifest["index"][name] = cases.digest(data, cent, lab)
+        manifest["me8q"] = {}
+        for name, (n, dim, nlist, avg, seed, alpha, spec, pv, de, _e, _nq) in cases.ME8Q_CASES.items():
+            data, cent, lab, queries, lam = cases.me8q_mixture(name)
+            for fn, arr in (("data", data), ("cent", cent), ("lab", lab), ("var", lam), ("mq", queries)):
+                np.save(t(fn + ".npy"), arr)
+            idx_path = os.path.join(out, f"{name}.me8q")
+            pre = t("m")
+            run("mbuild", t("data.npy"), t("cent.npy"), t("lab.npy"), avg, seed, alpha, spec or "-",
+                t("var.npy") if pv else "-", int(de), idx_path, pre)
+            res = {part: np.load(f"{pre}_{part}.npy")
+                   for part in ("plane", "refine", "gain", "vpc", "rcn2", "off", "cn2", "crot", "plan", "routedim")}
+            for (k, nprobe) in cases.me8q_search_grid(name):
+                run("msearch", idx_path, t("mq.npy"), k, nprobe, pre)
+                tag = f"k{k}_p{nprobe}"
+                for part in ("ids", "dists", "stats", "rotq", "probe", "pdist"):
+                    res[f"{tag}_{part}"] = np.load(f"{pre}_{part}.npy")
+            res["queries"] = queries
+            np.savez_compressed(os.path.join(out, f"{name}_me8q.npz"), **res)
+            manifest["me8q"][name] = cases.digest(data, cent, lab, lam)
     with open(os.path.join(out, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
 

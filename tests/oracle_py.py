@@ -181,3 +181,99 @@ class OracleIndex:
         d = np.empty((nq, k), np.float32)
         lib().orc_scan_all(self.h, _p(q), nq, k, nprobe, _p(ids), _p(d))
         return ids, d
+
+
+# ---------------------------------------------------------------------------
+# ME8Q (oracle/me8q_oracle.cpp)
+
+def _m8_lib():
+    L = lib()
+    if not getattr(L, "_m8_ready", False):
+        vp, sz, i32, u64, f32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64, ctypes.c_float
+        L.orc_m8_build.restype = vp
+        L.orc_m8_build.argtypes = [vp, vp, vp, sz, sz, sz, f32, u64, f32, ctypes.c_char_p, vp, i32]
+        L.orc_m8_free.argtypes = [vp]
+        L.orc_m8_info.argtypes = [vp, vp]
+        L.orc_m8_image.argtypes = [vp, vp]
+        L.orc_m8_plan.argtypes = [vp, vp]
+        L.orc_m8_fast_sizes.argtypes = [vp, sz, vp]
+        L.orc_m8_fast.argtypes = [vp, sz, vp, vp, vp, vp, vp]
+        L.orc_m8_centroids.argtypes = [vp, vp, vp]
+        L.orc_m8_query.argtypes = [vp, vp, sz, vp, vp, vp, vp, vp]
+        L.orc_m8_search.restype = i32
+        L.orc_m8_search.argtypes = [vp, vp, sz, sz, sz, i32, vp, vp, vp]
+        L._m8_ready = True
+    return L
+
+
+class OracleMe8q:
+    """Oracle ME8Q index (build + FastScan search), reference semantics."""
+
+    def __init__(self, h):
+        self.h = h
+        info = np.zeros(11, np.uint64)
+        _m8_lib().orc_m8_info(h, _p(info))
+        (self.n, self.dim, self.padded, self.nlist, self.nrun, self.plane_cols, self.refine_stride, self.stride,
+         self.route_dim, self.image_bytes, plan_len) = (int(v) for v in info)
+        plan = np.zeros((plan_len, 2), np.int64)
+        _m8_lib().orc_m8_plan(h, _p(plan))
+        self.plan = [(int(a), int(b)) for a, b in plan]
+
+    @classmethod
+    def build(cls, data, cent, lab, avg_bits=4.0, seed=0, prune_alpha=0.0, plan_spec="", variance=None,
+              drop_energy=True):
+        data, cent, lab = c32(data), c32(cent), c32(lab, np.uint32)
+        var = c32(variance) if variance is not None else None
+        h = _m8_lib().orc_m8_build(_p(data), _p(cent), _p(lab), data.shape[0], data.shape[1], cent.shape[0],
+                                   float(avg_bits), int(seed), float(prune_alpha), plan_spec.encode(), _p(var),
+                                   int(drop_energy))
+        if not h:
+            raise ValueError("oracle ME8Q build failed")
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _m8_lib().orc_m8_free(self.h)
+            self.h = None
+
+    def image(self):
+        out = np.empty(self.image_bytes, np.uint8)
+        _m8_lib().orc_m8_image(self.h, _p(out))
+        return out
+
+    def fast(self, c):
+        sz = np.zeros(3, np.uint64)
+        _m8_lib().orc_m8_fast_sizes(self.h, c, _p(sz))
+        plane = np.empty(int(sz[0]), np.uint8)
+        refine = np.empty(int(sz[1]), np.uint8)
+        gain = np.empty(int(sz[2]), np.float32)
+        vpc = np.empty(int(sz[2]), np.float32)
+        rcn2 = np.empty(self.nrun, np.float64)
+        _m8_lib().orc_m8_fast(self.h, c, _p(plane), _p(refine), _p(gain), _p(vpc), _p(rcn2))
+        return plane, refine, gain, vpc, rcn2
+
+    def centroids(self):
+        crot = np.empty((self.nlist, self.padded), np.float32)
+        cn2 = np.empty(self.nlist, np.float32)
+        _m8_lib().orc_m8_centroids(self.h, _p(crot), _p(cn2))
+        return crot, cn2
+
+    def query(self, q, nprobe):
+        q = c32(q)
+        take = min(nprobe, self.nlist)
+        rq = np.empty(self.padded, np.float32)
+        lut = np.empty(self.plane_cols * 64, np.uint8)
+        md = np.empty(2, np.float32)
+        probe = np.empty(take, np.uint32)
+        pd = np.empty(take, np.float32)
+        _m8_lib().orc_m8_query(self.h, _p(q), nprobe, _p(rq), _p(lut), _p(md), _p(probe), _p(pd))
+        return rq, lut, md, probe, pd
+
+    def search(self, queries, k, nprobe, threads=8):
+        q = c32(queries)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        st = np.empty((nq, 5), np.uint64)
+        _m8_lib().orc_m8_search(self.h, _p(q), nq, k, nprobe, threads, _p(ids), _p(d), _p(st))
+        return ids, d, st
