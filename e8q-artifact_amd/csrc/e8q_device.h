@@ -74,13 +74,13 @@ __device__ __forceinline__ float Hsum8Lanes(float v) {
 // Rotation: one workgroup per row; the row lives in LDS for the 4 rounds of
 // {sign flip, FHT over the power-of-two window, 1/sqrt(window)} with a Kac
 // walk between rounds when the padded length exceeds the window.
-__device__ inline void RotateInLds(float* buf, const IndexView& ix) {
-    const uint32_t P = static_cast<uint32_t>(ix.padded);
-    const uint32_t T = ix.trunc;
+// P = rotated length (a multiple of 64), T = its power-of-two window,
+// fac = 1/sqrt(T), signs = 4*P/8 bytes.
+__device__ inline void RotateInLds(float* buf, uint32_t P, uint32_t T, float fac, const uint8_t* signs) {
     const bool kac = T != P;
     const uint32_t per_round = P / 8;
     for (int r = 0; r < 4; ++r) {
-        const uint8_t* sg = ix.signs + r * per_round;
+        const uint8_t* sg = signs + r * per_round;
         for (uint32_t i = threadIdx.x; i < P; i += blockDim.x)
             if ((sg[i >> 3] >> (i & 7)) & 1) buf[i] = -buf[i];
         __syncthreads();
@@ -94,7 +94,7 @@ __device__ inline void RotateInLds(float* buf, const IndexView& ix) {
             }
             __syncthreads();
         }
-        for (uint32_t i = threadIdx.x; i < T; i += blockDim.x) win[i] = win[i] * ix.fac;
+        for (uint32_t i = threadIdx.x; i < T; i += blockDim.x) win[i] = win[i] * fac;
         __syncthreads();
         if (kac) {
             const uint32_t half = P / 2;
@@ -110,6 +110,10 @@ __device__ inline void RotateInLds(float* buf, const IndexView& ix) {
         for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = buf[i] * 0.25F;
         __syncthreads();
     }
+}
+
+__device__ inline void RotateInLds(float* buf, const IndexView& ix) {
+    RotateInLds(buf, static_cast<uint32_t>(ix.padded), ix.trunc, ix.fac, ix.signs);
 }
 
 }  // namespace e8qdev

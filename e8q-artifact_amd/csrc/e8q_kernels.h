@@ -35,6 +35,21 @@ hipError_t LaunchQueryPrep(hipStream_t st, const float* rotq, uint64_t nq, const
                            float* consts);
 hipError_t LaunchRoute(hipStream_t st, const float* rotq, const float* consts, uint64_t nq, const IndexView& ix,
                        float* dist);
+// Route distances over the first `dim` coordinates of rows `cstride` apart
+// (ivf_route.cpp:9-27; ME8Q routes over its plan's coded prefix).
+struct RouteArgs {
+    const float* rotq;        // nq rows, qstride floats apart
+    uint32_t qstride;
+    const float* qn2;         // |q|^2 over the prefix, qn2_stride floats apart
+    uint32_t qn2_stride;
+    const float* cent;        // rotated centroids
+    const float* cn2;         // their squared norms
+    uint64_t nlist;
+    uint32_t dim, cstride;
+    uint64_t nq;
+    float* dist;              // [nq][nlist]
+};
+hipError_t LaunchRouteDist(hipStream_t st, const RouteArgs& ra);
 size_t SelectProbesLds(uint32_t nlist, uint32_t maxp);
 hipError_t LaunchSelectProbes(hipStream_t st, const float* dist, uint64_t nq, uint32_t nlist,
                               const uint32_t* nprobe_q, uint32_t nprobe_all, uint32_t maxp, uint32_t* probe_ids,

@@ -129,28 +129,36 @@ __global__ void QueryPrepKernel(const float* __restrict__ rotq, uint64_t nq, Ind
 // Grid: x = centroid tiles of 8 (one 32-lane group each), y = query tiles.
 constexpr int kRouteQT = 8;
 
-__global__ void RouteDistKernel(const float* __restrict__ rotq, const float* __restrict__ consts, uint64_t nq,
-                                IndexView ix, float* __restrict__ dist) {
-    extern __shared__ float qs[];   // kRouteQT * P
-    const uint32_t P = static_cast<uint32_t>(ix.padded);
+__global__ void RouteNoDimKernel(RouteArgs ra) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ra.nq * ra.nlist) return;
+    const uint64_t q = i / ra.nlist, c = i % ra.nlist;
+    float d2 = fmaf(-0.0F, 2.0F, ra.cn2[c] + ra.qn2[q * ra.qn2_stride]);
+    if (d2 < 0.0F) d2 = 0.0F;
+    ra.dist[i] = d2;
+}
+
+__global__ void RouteDistKernel(RouteArgs ra) {
+    extern __shared__ float qs[];   // kRouteQT * D
+    const uint32_t D = ra.dim;
     const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * kRouteQT;
-    const int nqt = static_cast<int>(min<uint64_t>(kRouteQT, nq - q0));
-    for (uint32_t i = threadIdx.x; i < kRouteQT * P; i += blockDim.x) {
-        const uint32_t qi = i / P;
-        qs[i] = qi < static_cast<uint32_t>(nqt) ? rotq[(q0 + qi) * P + (i % P)] : 0.0F;
+    const int nqt = static_cast<int>(min<uint64_t>(kRouteQT, ra.nq - q0));
+    for (uint32_t i = threadIdx.x; i < kRouteQT * D; i += blockDim.x) {
+        const uint32_t qi = i / D;
+        qs[i] = qi < static_cast<uint32_t>(nqt) ? ra.rotq[(q0 + qi) * ra.qstride + (i % D)] : 0.0F;
     }
     __syncthreads();
     const uint32_t lane = threadIdx.x & 31;
     const uint64_t c = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 32) + (threadIdx.x >> 5);
-    const bool valid = c < ix.nlist;
-    const float* cc = ix.centroid_rot + (valid ? c : 0) * P;
+    const bool valid = c < ra.nlist;
+    const float* cc = ra.cent + (valid ? c : 0) * ra.cstride;
     float acc[kRouteQT];
 #pragma unroll
     for (int t = 0; t < kRouteQT; ++t) acc[t] = 0.0F;
-    for (uint32_t d = lane; d < P; d += 32) {
+    for (uint32_t d = lane; d < D; d += 32) {
         const float cv = cc[d];
 #pragma unroll
-        for (int t = 0; t < kRouteQT; ++t) acc[t] = fmaf(qs[t * P + d], cv, acc[t]);
+        for (int t = 0; t < kRouteQT; ++t) acc[t] = fmaf(qs[t * D + d], cv, acc[t]);
     }
 #pragma unroll
     for (int t = 0; t < kRouteQT; ++t) {
@@ -161,10 +169,10 @@ __global__ void RouteDistKernel(const float* __restrict__ rotq, const float* __r
         v = v + __shfl_down(v, 2, 32);    // lanes 0..1
         v = v + __shfl_down(v, 1, 32);    // lane 0
         if (lane == 0 && valid && t < nqt) {
-            const float qn2 = consts[(q0 + t) * 8 + 4];
-            float d2 = fmaf(-v, 2.0F, ix.centroid_n2[c] + qn2);
+            const float qn2 = ra.qn2[(q0 + t) * ra.qn2_stride];
+            float d2 = fmaf(-v, 2.0F, ra.cn2[c] + qn2);
             if (d2 < 0.0F) d2 = 0.0F;
-            dist[(q0 + t) * ix.nlist + c] = d2;
+            ra.dist[(q0 + t) * ra.nlist + c] = d2;
         }
     }
 }
@@ -280,10 +288,37 @@ hipError_t LaunchQueryPrep(hipStream_t st, const float* rotq, uint64_t nq, const
 
 hipError_t LaunchRoute(hipStream_t st, const float* rotq, const float* consts, uint64_t nq, const IndexView& ix,
                        float* dist) {
-    if (nq == 0) return hipSuccess;
-    const dim3 grid(static_cast<uint32_t>((ix.nlist + 7) / 8), static_cast<uint32_t>((nq + kRouteQT - 1) / kRouteQT));
-    hipLaunchKernelGGL(RouteDistKernel, grid, dim3(256), kRouteQT * ix.padded * sizeof(float), st, rotq, consts, nq,
-                       ix, dist);
+    RouteArgs ra{};
+    ra.rotq = rotq;
+    ra.qstride = static_cast<uint32_t>(ix.padded);
+    ra.qn2 = consts + 4;
+    ra.qn2_stride = 8;
+    ra.cent = ix.centroid_rot;
+    ra.cn2 = ix.centroid_n2;
+    ra.nlist = ix.nlist;
+    ra.dim = static_cast<uint32_t>(ix.padded);
+    ra.cstride = static_cast<uint32_t>(ix.padded);
+    ra.nq = nq;
+    ra.dist = dist;
+    return LaunchRouteDist(st, ra);
+}
+
+hipError_t LaunchRouteDist(hipStream_t st, const RouteArgs& ra) {
+    if (ra.nq == 0) return hipSuccess;
+    if (ra.dim == 0) {   // no coded prefix: every centroid scores its norm
+        // dist = (cn2 + qn2) - 0, clamped; one thread per (query, centroid)
+        const uint64_t tot = ra.nq * ra.nlist;
+        hipLaunchKernelGGL(RouteNoDimKernel, dim3(static_cast<uint32_t>((tot + 255) / 256)), dim3(256), 0, st, ra);
+        return hipGetLastError();
+    }
+    const dim3 grid(static_cast<uint32_t>((ra.nlist + 7) / 8), static_cast<uint32_t>((ra.nq + kRouteQT - 1) / kRouteQT));
+    const size_t lds = kRouteQT * static_cast<size_t>(ra.dim) * sizeof(float);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&RouteDistKernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(RouteDistKernel, grid, dim3(256), lds, st, ra);
     return hipGetLastError();
 }
 

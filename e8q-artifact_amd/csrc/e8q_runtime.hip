@@ -19,90 +19,9 @@
 
 using namespace e8qdev;
 
-namespace {
+#include "e8q_rt.h"
 
-thread_local std::string g_err;
-
-struct Fail {
-    e8q_status st;
-    std::string msg;
-};
-
-[[noreturn]] void Throw(e8q_status st, const std::string& m) { throw Fail{st, m}; }
-
-void Check(hipError_t e, const char* what) {
-    if (e != hipSuccess) Throw(e == hipErrorOutOfMemory ? E8Q_ENOMEM : E8Q_ERUNTIME,
-                               std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <class F>
-e8q_status Guard(F&& f) {
-    try {
-        f();
-        return E8Q_OK;
-    } catch (const Fail& x) {
-        g_err = x.msg;
-        return x.st;
-    } catch (const std::bad_alloc&) {
-        g_err = "host allocation failed";
-        return E8Q_ENOMEM;
-    } catch (const std::exception& x) {
-        g_err = x.what();
-        return E8Q_ERUNTIME;
-    }
-}
-
-// Owned device allocation.
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { reset(); }
-    void reset() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t b) {
-        reset();
-        if (b == 0) b = 1;
-        Check(hipMalloc(&p, b), "hipMalloc");
-        bytes = b;
-    }
-    template <class T>
-    T* as() const { return static_cast<T*>(p); }
-};
-
-}  // namespace
-
-struct e8q_ctx {
-    int device = 0;
-    int cus = 0;      // compute units (the persistent batched scan's grid)
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // search timing
-    DBuf ws;   // per-call workspace, grown on demand
-
-    // Bump allocator over ws for one call.
-    struct Carve {
-        std::vector<std::pair<void**, size_t>> want;
-        template <class T>
-        void add(T** p, size_t count) { want.push_back({reinterpret_cast<void**>(p), count * sizeof(T)}); }
-    };
-    void Bind() { Check(hipSetDevice(device), "hipSetDevice"); }
-    void Take(Carve& c) {
-        size_t total = 0;
-        for (auto& w : c.want) total += (w.second + 255) & ~size_t{255};
-        if (ws.bytes < total) ws.alloc(total + total / 4);
-        char* base = ws.as<char>();
-        size_t off = 0;
-        for (auto& w : c.want) {
-            *w.first = base + off;
-            off += (w.second + 255) & ~size_t{255};
-        }
-    }
-};
+using namespace e8qrt;
 
 struct e8q_dindex {
     e8q_ctx* ctx = nullptr;
@@ -124,58 +43,15 @@ struct e8q_dindex {
 
 namespace {
 
-template <class T>
-void Upload(DBuf& b, const T* src, size_t count, hipStream_t st) {
-    b.alloc(count * sizeof(T));
-    if (count) Check(hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, st), "upload");
-}
-
-// Encoder tables from the per-branch level tables (e8q_codec.cpp:125-352):
-// the merged sorted table with its period-4 class interleave verified, and
-// the four (branch, parity) single-class tables sorted by (level, digit).
 void BuildEncoderTables(e8q_dindex* ix, const double* levels, int bits, hipStream_t st) {
+    const EncTabHost t = MakeEncoderTables(levels, bits);
     const uint32_t q = 1u << bits;
-    struct E {
-        double v;
-        uint16_t d;
-        int br;
-    };
-    std::vector<E> m;
-    m.reserve(2 * q);
-    for (uint32_t d = 0; d < q; ++d) {
-        m.push_back({levels[d], static_cast<uint16_t>(d), 0});
-        m.push_back({levels[q + d], static_cast<uint16_t>(d), 1});
-    }
-    std::stable_sort(m.begin(), m.end(), [](const E& a, const E& b) { return a.v < b.v; });
-    bool alt = true;
-    for (size_t i = 0; i < m.size(); ++i) {
-        if (i > 0 && !(m[i].v > m[i - 1].v)) alt = false;
-        if (m[i].br != static_cast<int>(i & 1) || (m[i].d & 1u) != ((i >> 1) & 1u)) alt = false;
-    }
-    std::vector<double> mlev(m.size());
-    std::vector<uint16_t> mdig(m.size());
-    for (size_t i = 0; i < m.size(); ++i) {
-        mlev[i] = m[i].v;
-        mdig[i] = m[i].d;
-    }
-    const uint32_t half = q / 2;
-    std::vector<double> cl(4 * half);
-    std::vector<uint16_t> cd(4 * half);
-    for (int br = 0; br < 2; ++br)
-        for (int par = 0; par < 2; ++par) {
-            std::vector<std::pair<double, uint16_t>> e;
-            for (uint32_t d = par; d < q; d += 2) e.push_back({levels[br * q + d], static_cast<uint16_t>(d)});
-            std::sort(e.begin(), e.end());
-            for (uint32_t i = 0; i < half; ++i) {
-                cl[(br * 2 + par) * half + i] = e[i].first;
-                cd[(br * 2 + par) * half + i] = e[i].second;
-            }
-        }
+    const bool alt = t.merged;
     Upload(ix->levels, levels, 2 * q, st);
-    Upload(ix->mlev, mlev.data(), mlev.size(), st);
-    Upload(ix->mdig, mdig.data(), mdig.size(), st);
-    Upload(ix->cls_lev, cl.data(), cl.size(), st);
-    Upload(ix->cls_dig, cd.data(), cd.size(), st);
+    Upload(ix->mlev, t.mlev.data(), t.mlev.size(), st);
+    Upload(ix->mdig, t.mdig.data(), t.mdig.size(), st);
+    Upload(ix->cls_lev, t.cl.data(), t.cl.size(), st);
+    Upload(ix->cls_dig, t.cd.data(), t.cd.size(), st);
     Check(hipStreamSynchronize(st), "encoder tables");
     ix->ev.bits = bits;
     ix->ev.merged = alt ? 1 : 0;

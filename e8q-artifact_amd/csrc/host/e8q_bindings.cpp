@@ -205,8 +205,11 @@ class PyE8qIndex {
 
 }  // namespace
 
+void RegisterMe8q(py::module_& m);   // me8q_bindings.cpp
+
 PYBIND11_MODULE(_e8qlib, m) {
-    m.doc() = "E8Q IVF index on MI355X (gfx950)";
+    m.doc() = "E8Q / ME8Q IVF indexes on MI355X (gfx950)";
+    RegisterMe8q(m);
     py::class_<PyE8qIndex>(m, "E8qIndex")
         .def(py::init<size_t, size_t, uint64_t>(), py::arg("dim"), py::arg("nbits"), py::arg("seed") = 0)
         .def("build", &PyE8qIndex::build, py::arg("data"), py::arg("centroids"), py::arg("cluster_ids"),

@@ -24,11 +24,6 @@ namespace {
 constexpr uint32_t kMagic = 0x58413845;     // "E8AX"
 constexpr uint32_t kOptMagic = 0x514F5045;  // "EPOQ"
 
-std::mutex& DeviceMutex() {
-    static std::mutex m;
-    return m;
-}
-
 void Check(e8q_status st) {
     if (st == E8Q_OK) return;
     const std::string msg = e8q_last_error();
@@ -43,6 +38,11 @@ size_t RefBatchBytes(size_t blocks, int bits) {
 }
 
 }  // namespace
+
+std::mutex& DeviceMutex() {
+    static std::mutex m;
+    return m;
+}
 
 e8q_ctx* DefaultContext() {
     static e8q_ctx* ctx = [] {

@@ -12,6 +12,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -107,5 +108,7 @@ class E8qIndex {
 
 // Process-wide device context (one per process; device from E8Q_DEVICE, default 0).
 e8q_ctx* DefaultContext();
+// Serializes the calls made on that context (one host thread at a time).
+std::mutex& DeviceMutex();
 
 }  // namespace e8qlib

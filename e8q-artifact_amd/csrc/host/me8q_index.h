@@ -1,0 +1,131 @@
+// me8q_index.h -- e8qlib::me8q::Me8qIndex with a device-resident index.
+//
+// Same public surface as the reference class (include/e8qlib/index/me8q.hpp:
+// 31-136): BuildOptions, SearchStats, Build / Save / Load / SearchIVFFast /
+// SearchIVFFastBatch and the shape getters, the same ME8QIDXX file and the
+// same exceptions.  The plan (SAQ dynamic program or a manual spec) and the
+// rotator signs are computed on the host as the reference does; the encode,
+// the FastScan layout and every search run on the GPU through
+// include/me8q_gpu.h.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+struct e8q_ctx;
+struct e8q_m8index;
+
+namespace e8qlib::me8q {
+
+constexpr size_t kSegDim = 64;      // me8q_plan.hpp:12
+constexpr int kMaxBits = 13;        // me8q_plan.hpp:13
+constexpr size_t kFactorBits = 64;  // me8q_plan.hpp:14
+constexpr size_t kMaxDeviceK = 1024;
+
+struct PlanSeg {
+    size_t len = 0;
+    int bits = 0;
+};
+using Plan = std::vector<PlanSeg>;
+
+size_t PaddedDim(size_t dim);
+
+struct PlanInput {
+    const float* var = nullptr;
+    size_t dim = 0;
+    float avg_bits = 4.0F;
+    int max_bits = kMaxBits;
+    double extra_bits = 0.0;
+};
+// SAQ's DP (me8q_plan.cpp:47-167).
+Plan Allocate(const PlanInput& in);
+// "L:B,L:B,..." (me8q_common.cpp:37-75).
+Plan ParsePlanSpec(const std::string& spec, size_t padded_dim);
+// Per-dimension variance of the base rows (me8q_common.cpp:11-35).
+void DatasetVariance(const float* base, size_t ld, size_t n, size_t dim, std::vector<float>* var);
+
+struct BuildOptions {
+    float avg_bits = 4.0F;
+    std::string plan_spec;
+    uint64_t rot_seed = 0x5EED1234ULL;
+    int max_bits = kMaxBits;
+    bool drop_energy = true;
+    float prune_alpha = 0.0F;
+    double extra_budget_bits = 0.0;
+    size_t threads = 1;
+};
+
+struct SearchStats {
+    size_t candidates = 0;
+    size_t clusters = 0;
+    double route_us = 0.0;
+    double scan_us = 0.0;
+    double rotate_us = 0.0;
+    double lut_us = 0.0;
+    double filter_us = 0.0;
+    double lane_us = 0.0;
+    double rest_us = 0.0;
+    double refine_us = 0.0;
+    size_t survivors = 0;
+    size_t bits_scored = 0;
+    size_t bits_total = 0;
+    size_t pruned = 0;
+    double device_scan_ms = 0.0;
+    double device_total_ms = 0.0;
+};
+
+class Me8qIndex {
+   public:
+    Me8qIndex();
+    ~Me8qIndex();
+    Me8qIndex(Me8qIndex&&) noexcept;
+    Me8qIndex& operator=(Me8qIndex&&) noexcept;
+    Me8qIndex(const Me8qIndex&) = delete;
+    Me8qIndex& operator=(const Me8qIndex&) = delete;
+
+    void Build(const float* base, size_t ld, const float* centroids, size_t ld_c, const uint32_t* cluster_ids,
+               size_t n, size_t dim, size_t num_clusters, const BuildOptions& opt,
+               const float* data_variance = nullptr);
+    void Save(const char* path) const;
+    void Load(const char* path);
+
+    void SearchIVFFast(const float* query, size_t k, size_t nprobe, uint32_t* out_ids, SearchStats* stats = nullptr,
+                       float* out_dists = nullptr) const;
+    void SearchIVFFastBatch(const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* out_ids,
+                            SearchStats* stats, float* out_dists, int num_threads) const;
+    // The device entry: nprobe_per_query (nullable) overrides nprobe; queries
+    // / outputs may be device pointers; per_query_stats (nullable) [nq*5].
+    void SearchDevice(const float* queries, size_t nq, size_t k, size_t nprobe, const uint32_t* nprobe_per_query,
+                      uint32_t* out_ids, float* out_dists, SearchStats* stats, bool queries_on_device,
+                      bool outputs_on_device, uint64_t* per_query_stats = nullptr) const;
+
+    [[nodiscard]] size_t n() const { return n_; }
+    [[nodiscard]] size_t dim() const { return dim_; }
+    [[nodiscard]] size_t padded_dim() const { return padded_; }
+    [[nodiscard]] size_t route_dim() const;
+    [[nodiscard]] size_t num_clusters() const { return nlist_; }
+    [[nodiscard]] const Plan& shared_plan() const { return plan_; }
+    [[nodiscard]] double bits_per_dim() const;
+    [[nodiscard]] size_t memory_bytes() const;
+    [[nodiscard]] size_t device_bytes() const;
+    [[nodiscard]] bool built() const { return dev_ != nullptr; }
+    [[nodiscard]] float prune_alpha() const { return opt_.prune_alpha; }
+    [[nodiscard]] e8q_m8index* device_index() const { return dev_; }
+
+   private:
+    void CreateDevice();
+    void Reset();
+    size_t n_ = 0, dim_ = 0, padded_ = 0, nlist_ = 0, bytes_per_vector_ = 0;
+    BuildOptions opt_;
+    float ortho_alpha_ = 1.0F;
+    Plan plan_;
+    std::vector<std::vector<uint8_t>> run_signs_;
+    std::vector<uint64_t> list_begin_;
+    e8q_ctx* ctx_ = nullptr;
+    e8q_m8index* dev_ = nullptr;
+};
+
+}  // namespace e8qlib::me8q

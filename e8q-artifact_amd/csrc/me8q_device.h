@@ -1,0 +1,124 @@
+// me8q_device.h -- device view of an ME8Q (mixed-rate E8Q) index and the
+// helpers its build and search kernels share.
+//
+// HBM layout (one device index):
+//   codes    n rows x `stride` bytes, the reference's row format
+//            (me8q_build.cpp:103-135): nrun (coef, h2) float pairs, then each
+//            coded run's bit-tight code (rate bytes per 8-dim block); kept as
+//            is so Save() writes it back unchanged.
+//   plane    per 32-vector batch, `rec` bytes: the digit top-bit plane stored
+//            vector-major ([lane][plane_cols], one byte per 8-dim block, so a
+//            lane reads its own bytes 8 at a time), then nrun x 160 bytes of
+//            u8 banks + scales in the reference's format (me8q_fast.cpp:
+//            223-301).  The reference keeps the plane [col][lane].
+//   refine   per vector, `rstride` bytes: nrun x float4 {coef, h2, gain, vpc}
+//            (the reference keeps gain/vpc in separate per-list arrays), then
+//            each run's (rate-1)-bit refine fields (me8q_fast.cpp:97-118).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "e8q_device.h"
+
+namespace e8qdev {
+
+constexpr uint32_t kM8BankBytes = 160;   // per run per batch (me8q.hpp:183)
+constexpr uint32_t kM8MaxRuns = 32;      // the reference's flush4 bound (me8q_fast.cpp:865)
+constexpr uint32_t kM8MaxK = 1024;       // top-k list held in LDS
+
+// One coded run of the plan.
+struct M8Run {
+    uint32_t rate, len, nblk, dim_off;   // bits/dim, dims, blocks, first dim
+    uint32_t blk0;                       // first plane column
+    uint32_t code_off;                   // byte offset of its code in a code row
+    uint32_t ref_off;                    // byte offset of its fields in a device refine row
+    uint32_t trunc;                      // rotator FHT window
+    float fac;                           // 1/sqrt(trunc)
+    float pbar0, pbar1;                  // top-bit class means (ClassModelOf)
+    uint32_t pad;
+    const uint8_t* signs;                // rotator, 4*len/8 bytes
+    const float* lev;                    // float levels [2][2^rate]
+    EncodeView ev;                       // encoder tables at this rate
+};
+
+// One plan segment (coded or dropped), for the dropped-energy sum.
+struct M8Seg {
+    uint32_t off, len, bits, pad;
+};
+
+struct M8View {
+    uint64_t n, dim, padded, nlist;
+    uint32_t nrun, nseg, plane_cols, route_dim;
+    uint32_t stride;      // code row bytes
+    uint32_t rstride;     // device refine row bytes
+    uint32_t rec;         // device plane record bytes
+    int32_t drop_energy;
+    const M8Run* runs;
+    const M8Seg* segs;
+    const uint64_t* list_begin;    // [nlist+1] vectors
+    const uint64_t* batch_begin;   // [nlist+1] 32-vector batches
+    uint32_t* ids;
+    uint8_t* codes;
+    float* centroids;     // [nlist][padded] zero padded, unrotated
+    float* cent_rot;      // [nlist][padded] per-run rotated (dropped dims 0)
+    float* cn2;           // [nlist] fp64 sum of squares, stored float
+    double* rc_n2;        // [nlist][nrun] per-run centroid energy
+    uint8_t* plane;
+    uint8_t* refine;
+};
+
+// `n` (<= 25) bits at bit `pos` of a little-endian byte string.
+__device__ __forceinline__ uint32_t M8Bits(const uint8_t* p, uint32_t pos, uint32_t n) {
+    const uint8_t* b = p + (pos >> 3);
+    const uint32_t w = static_cast<uint32_t>(b[0]) | (static_cast<uint32_t>(b[1]) << 8) |
+                       (static_cast<uint32_t>(b[2]) << 16) | (static_cast<uint32_t>(b[3]) << 24);
+    return n == 0 ? 0u : (w >> (pos & 7)) & ((1u << n) - 1u);
+}
+
+// Unpacks one block packed at `bits` bits per digit (me8q_codec.cpp:186-213):
+// digits 0..6 at bits*i, digit 7's high bits-1 bits at 7*bits with its LSB
+// the parity of digits 0..6's LSBs, the branch at bit 8*bits-1.
+__device__ __forceinline__ void M8Unpack(const uint8_t* in, uint32_t bits, uint32_t d[8], uint32_t* branch) {
+    uint32_t par = 0;
+    for (uint32_t i = 0; i < 7; ++i) {
+        d[i] = M8Bits(in, bits * i, bits);
+        par ^= d[i] & 1u;
+    }
+    d[7] = (M8Bits(in, bits * 7, bits - 1) << 1) | par;
+    *branch = M8Bits(in, 8 * bits - 1, 1);
+}
+
+// Packs one block at `bits` bits per digit into `bits` bytes (PackBlock,
+// me8q_codec.cpp:117-133).  Byte-wise, so any alignment.
+__device__ __forceinline__ void M8Pack(const uint32_t d[8], uint32_t branch, uint32_t bits, uint8_t* out) {
+    for (uint32_t k = 0; k < bits; ++k) {
+        uint32_t byte = 0;
+        for (uint32_t bit = 0; bit < 8; ++bit) {
+            const uint32_t pos = 8 * k + bit;
+            uint32_t v;
+            if (pos == 8 * bits - 1) {
+                v = branch & 1u;
+            } else if (pos < 7 * bits) {
+                v = (d[pos / bits] >> (pos % bits)) & 1u;
+            } else {
+                v = ((d[7] >> 1) >> (pos - 7 * bits)) & 1u;
+            }
+            byte |= v << bit;
+        }
+        out[k] = static_cast<uint8_t>(byte);
+    }
+}
+
+__device__ __forceinline__ uint32_t M8ListOf(const uint64_t* begin, uint64_t nlist, uint64_t i) {
+    uint64_t lo = 0, hi = nlist;   // last c with begin[c] <= i
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (begin[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return static_cast<uint32_t>(lo);
+}
+
+}  // namespace e8qdev

@@ -1,0 +1,571 @@
+// me8q_runtime.hip -- the ME8Q half of libe8q_gpu.so's C ABI
+// (include/me8q_gpu.h): device index lifetime, upload / download, the
+// device build, search, and the test diagnostics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/me8q_gpu.h"
+#include "e8q_kernels.h"
+#include "e8q_rt.h"
+#include "me8q_device.h"
+#include "me8q_kernels.h"
+
+using namespace e8qdev;
+using namespace e8qrt;
+
+struct e8q_m8index {
+    e8q_ctx* ctx = nullptr;
+    M8View view{};
+    std::vector<M8Run> runs;
+    std::vector<M8Seg> segs;
+    std::vector<uint64_t> list_begin, batch_begin;
+    uint64_t nbatch = 0;
+    // refine-row layout of the reference (for the diagnostics)
+    std::vector<uint32_t> ref_run_off;
+    uint32_t ref_stride = 0;
+    DBuf d_runs, d_segs, d_list_begin, d_batch_begin, ids, codes, centroids, cent_rot, cn2, rc_n2, plane, refine,
+        signs, lev;
+    std::vector<DBuf> enc;   // encoder tables, 5 per distinct rate
+    uint64_t device_bytes() const {
+        uint64_t b = d_runs.bytes + d_segs.bytes + d_list_begin.bytes + d_batch_begin.bytes + ids.bytes + codes.bytes +
+                     centroids.bytes + cent_rot.bytes + cn2.bytes + rc_n2.bytes + plane.bytes + refine.bytes +
+                     signs.bytes + lev.bytes;
+        for (const auto& e : enc) b += e.bytes;
+        return b;
+    }
+};
+
+namespace {
+
+void RefreshM8View(e8q_m8index* x) {
+    M8View& v = x->view;
+    v.runs = x->d_runs.as<M8Run>();
+    v.segs = x->d_segs.as<M8Seg>();
+    v.list_begin = x->d_list_begin.as<uint64_t>();
+    v.batch_begin = x->d_batch_begin.as<uint64_t>();
+    v.ids = x->ids.as<uint32_t>();
+    v.codes = x->codes.as<uint8_t>();
+    v.centroids = x->centroids.as<float>();
+    v.cent_rot = x->cent_rot.as<float>();
+    v.cn2 = x->cn2.as<float>();
+    v.rc_n2 = x->rc_n2.as<double>();
+    v.plane = x->plane.as<uint8_t>();
+    v.refine = x->refine.as<uint8_t>();
+}
+
+// Rotated centroids + FastScan layout from codes / centroids already on the device.
+void BuildDerived(e8q_ctx* ctx, e8q_m8index* x) {
+    hipStream_t st = ctx->stream;
+    const M8View& v = x->view;
+    Check(LaunchM8Centroids(st, v), "centroid kernel");
+    Check(hipMemsetAsync(x->plane.p, 0, x->plane.bytes, st), "plane clear");
+    if (v.n > 0 && v.nrun > 0) {
+        DBuf perp;
+        perp.alloc(v.n * v.nrun * sizeof(float));
+        Check(LaunchM8Fast(st, v, perp.as<float>(), x->nbatch), "fast-layout kernels");
+        Check(hipStreamSynchronize(st), "fast layout");
+    } else {
+        Check(hipStreamSynchronize(st), "centroids");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+e8q_status e8q_m8_create(e8q_ctx* ctx, const e8q_m8_desc* d, e8q_m8index** out) {
+    return Guard([&] {
+        if (!ctx || !d || !out) Throw(E8Q_EINVAL, "null argument");
+        if (d->dim == 0 || d->nlist == 0) Throw(E8Q_EINVAL, "empty index shape");
+        if (d->padded != (d->dim + 63) / 64 * 64) Throw(E8Q_EINVAL, "padded must be dim rounded up to 64");
+        if (!d->seg_len || !d->seg_bits || !d->levels || !d->fixed_scale || !d->list_begin)
+            Throw(E8Q_EINVAL, "null table");
+        auto* x = new e8q_m8index();
+        x->ctx = ctx;
+        try {
+            ctx->Bind();
+            hipStream_t st = ctx->stream;
+            M8View& v = x->view;
+            v.n = d->n;
+            v.dim = d->dim;
+            v.padded = d->padded;
+            v.nlist = d->nlist;
+            v.drop_energy = d->drop_energy ? 1 : 0;
+            // plan -> runs
+            uint64_t off = 0, total_signs = 0;
+            uint32_t nrun = 0, code = 0, blk = 0;
+            for (uint32_t s = 0; s < d->nseg; ++s) {
+                const uint64_t len = d->seg_len[s];
+                const int bits = d->seg_bits[s];
+                if (len == 0 || len % 64 != 0 || bits < 0 || bits > 13) Throw(E8Q_EINVAL, "bad plan segment");
+                x->segs.push_back(M8Seg{static_cast<uint32_t>(off), static_cast<uint32_t>(len),
+                                        static_cast<uint32_t>(bits), 0});
+                if (bits > 0) {
+                    M8Run r{};
+                    r.rate = static_cast<uint32_t>(bits);
+                    r.len = static_cast<uint32_t>(len);
+                    r.nblk = static_cast<uint32_t>(len / 8);
+                    r.dim_off = static_cast<uint32_t>(off);
+                    r.blk0 = blk;
+                    uint32_t lg = 0;
+                    while ((uint64_t{1} << (lg + 1)) <= len) ++lg;
+                    r.trunc = 1u << lg;
+                    r.fac = 1.0F / std::sqrt(static_cast<float>(r.trunc));
+                    x->runs.push_back(r);
+                    blk += r.nblk;
+                    total_signs += 4 * len / 8;
+                    ++nrun;
+                }
+                off += len;
+            }
+            if (off != d->padded) Throw(E8Q_EINVAL, "plan does not cover the padded dimension");
+            if (nrun > kM8MaxRuns) Throw(E8Q_EINVAL, "more than 32 coded runs");
+            if (nrun > 0 && !d->signs) Throw(E8Q_EINVAL, "null rotator signs");
+            v.nrun = nrun;
+            v.nseg = d->nseg;
+            v.plane_cols = blk;
+            uint32_t route = static_cast<uint32_t>(d->padded);
+            for (uint32_t s = d->nseg; s-- > 0;) {
+                if (d->seg_bits[s] != 0) break;
+                route -= static_cast<uint32_t>(d->seg_len[s]);
+            }
+            v.route_dim = route;
+            code = nrun * 8;
+            uint32_t ref = (nrun * 16 + 15) & ~15u, refr = nrun * 8;
+            for (auto& r : x->runs) {
+                r.code_off = code;
+                code += r.rate * r.len / 8;
+                r.ref_off = ref;
+                x->ref_run_off.push_back(refr);
+                if (r.rate >= 2) {
+                    ref += (r.rate - 1) * r.nblk;
+                    refr += (r.rate - 1) * r.nblk;
+                }
+            }
+            v.stride = code;
+            v.rstride = (ref + 15) & ~15u;
+            x->ref_stride = refr;
+            v.rec = 32u * v.plane_cols + kM8BankBytes * nrun;
+            // list / batch offsets
+            x->list_begin.assign(d->list_begin, d->list_begin + d->nlist + 1);
+            if (x->list_begin[0] != 0 || x->list_begin[d->nlist] != d->n) Throw(E8Q_EINVAL, "list offsets inconsistent");
+            x->batch_begin.assign(d->nlist + 1, 0);
+            for (uint64_t c = 0; c < d->nlist; ++c) {
+                if (x->list_begin[c + 1] < x->list_begin[c]) Throw(E8Q_EINVAL, "list offsets not monotone");
+                x->batch_begin[c + 1] = x->batch_begin[c] + (x->list_begin[c + 1] - x->list_begin[c] + 31) / 32;
+            }
+            x->nbatch = x->batch_begin[d->nlist];
+            // tables: signs, float levels per run, encoder tables per distinct rate
+            Upload(x->signs, d->signs, total_signs, st);
+            size_t lev_total = 0;
+            for (const auto& r : x->runs) lev_total += size_t{2} << r.rate;
+            std::vector<float> levf;
+            levf.reserve(lev_total);
+            std::vector<size_t> lev_off;
+            size_t lev_base[14] = {};
+            for (int b = 1; b <= 13; ++b) lev_base[b] = (b == 1 ? 0 : lev_base[b - 1] + (size_t{2} << (b - 1)));
+            int enc_of_rate[14];
+            bool merged_of_rate[14] = {};
+            for (int& e : enc_of_rate) e = -1;
+            for (auto& r : x->runs) {
+                const double* L = d->levels + lev_base[r.rate];
+                const uint32_t q = 1u << r.rate;
+                lev_off.push_back(levf.size());
+                for (uint32_t i = 0; i < 2 * q; ++i) levf.push_back(static_cast<float>(L[i]));
+                // class means pbar (ClassModelOf, me8q_codec.cpp:263-280), from the float levels
+                for (int cl = 0; cl < 2; ++cl) {
+                    double sum = 0.0;
+                    for (int br = 0; br < 2; ++br)
+                        for (uint32_t dd = cl * (q / 2); dd < (cl + 1) * (q / 2); ++dd)
+                            sum += static_cast<double>(static_cast<float>(L[br * q + dd]));
+                    const float pb = static_cast<float>(sum / static_cast<double>(q));
+                    (cl == 0 ? r.pbar0 : r.pbar1) = pb;
+                }
+                if (enc_of_rate[r.rate] < 0) {
+                    enc_of_rate[r.rate] = static_cast<int>(x->enc.size());
+                    const EncTabHost t = MakeEncoderTables(L, static_cast<int>(r.rate));
+                    x->enc.resize(x->enc.size() + 5);
+                    DBuf* e = &x->enc[x->enc.size() - 5];
+                    Upload(e[0], L, 2 * q, st);
+                    Upload(e[1], t.mlev.data(), t.mlev.size(), st);
+                    Upload(e[2], t.mdig.data(), t.mdig.size(), st);
+                    Upload(e[3], t.cl.data(), t.cl.size(), st);
+                    Upload(e[4], t.cd.data(), t.cd.size(), st);
+                    merged_of_rate[r.rate] = t.merged;
+                }
+            }
+            Upload(x->lev, levf.data(), levf.size(), st);
+            uint64_t soff = 0;
+            for (size_t i = 0; i < x->runs.size(); ++i) {
+                M8Run& r = x->runs[i];
+                r.signs = x->signs.as<uint8_t>() + soff;
+                soff += 4 * static_cast<uint64_t>(r.len) / 8;
+                r.lev = x->lev.as<float>() + lev_off[i];
+                const DBuf* e = &x->enc[static_cast<size_t>(enc_of_rate[r.rate])];
+                r.ev.bits = static_cast<int>(r.rate);
+                r.ev.merged = merged_of_rate[r.rate] ? 1 : 0;
+                r.ev.mlen = 2u << r.rate;
+                r.ev.scale = d->fixed_scale[r.rate];
+                r.ev.levels = e[0].as<double>();
+                r.ev.mlev = e[1].as<double>();
+                r.ev.mdig = e[2].as<uint16_t>();
+                r.ev.cls_lev = e[3].as<double>();
+                r.ev.cls_dig = e[4].as<uint16_t>();
+            }
+            Upload(x->d_runs, x->runs.data(), x->runs.size(), st);
+            Upload(x->d_segs, x->segs.data(), x->segs.size(), st);
+            Upload(x->d_list_begin, x->list_begin.data(), x->list_begin.size(), st);
+            Upload(x->d_batch_begin, x->batch_begin.data(), x->batch_begin.size(), st);
+            x->ids.alloc(v.n * sizeof(uint32_t));
+            x->codes.alloc(v.n * v.stride + 16);
+            x->centroids.alloc(v.nlist * v.padded * sizeof(float));
+            x->cent_rot.alloc(v.nlist * v.padded * sizeof(float));
+            x->cn2.alloc(v.nlist * sizeof(float));
+            x->rc_n2.alloc(v.nlist * std::max<uint32_t>(nrun, 1) * sizeof(double));
+            x->plane.alloc(x->nbatch * v.rec + 16);
+            x->refine.alloc(v.n * v.rstride + 16);
+            Check(hipMemsetAsync(x->codes.p, 0, x->codes.bytes, st), "clear");
+            Check(hipMemsetAsync(x->refine.p, 0, x->refine.bytes, st), "clear");
+            Check(hipStreamSynchronize(st), "m8 create");
+            RefreshM8View(x);
+        } catch (...) {
+            delete x;
+            throw;
+        }
+        *out = x;
+    });
+}
+
+e8q_status e8q_m8_destroy(e8q_m8index* x) {
+    return Guard([&] {
+        if (!x) return;
+        x->ctx->Bind();
+        delete x;
+    });
+}
+
+uint64_t e8q_m8_device_bytes(const e8q_m8index* x) { return x ? x->device_bytes() : 0; }
+uint64_t e8q_m8_row_bytes(const e8q_m8index* x) { return x ? x->view.stride : 0; }
+
+e8q_status e8q_m8_upload(e8q_ctx* ctx, e8q_m8index* x, const e8q_m8_arrays* h) {
+    return Guard([&] {
+        if (!ctx || !x || !h) Throw(E8Q_EINVAL, "null argument");
+        const M8View& v = x->view;
+        // a plan with no coded run has empty code rows
+        if ((v.n && !h->ids) || !h->centroids || (v.n * v.stride && !h->codes)) Throw(E8Q_EINVAL, "null array");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        if (v.n) Check(hipMemcpyAsync(x->ids.p, h->ids, v.n * 4, hipMemcpyHostToDevice, st), "upload ids");
+        Check(hipMemcpyAsync(x->centroids.p, h->centroids, v.nlist * v.padded * 4, hipMemcpyHostToDevice, st),
+              "upload centroids");
+        if (v.n * v.stride)
+            Check(hipMemcpyAsync(x->codes.p, h->codes, v.n * v.stride, hipMemcpyHostToDevice, st), "upload codes");
+        BuildDerived(ctx, x);
+    });
+}
+
+e8q_status e8q_m8_download(e8q_ctx* ctx, const e8q_m8index* x, e8q_m8_arrays* h) {
+    return Guard([&] {
+        if (!ctx || !x || !h) Throw(E8Q_EINVAL, "null argument");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const M8View& v = x->view;
+        if (h->ids && v.n) Check(hipMemcpyAsync(h->ids, x->ids.p, v.n * 4, hipMemcpyDeviceToHost, st), "download ids");
+        if (h->centroids)
+            Check(hipMemcpyAsync(h->centroids, x->centroids.p, v.nlist * v.padded * 4, hipMemcpyDeviceToHost, st),
+                  "download centroids");
+        if (h->codes && v.n * v.stride)
+            Check(hipMemcpyAsync(h->codes, x->codes.p, v.n * v.stride, hipMemcpyDeviceToHost, st), "download codes");
+        Check(hipStreamSynchronize(st), "download");
+    });
+}
+
+e8q_status e8q_m8_encode(e8q_ctx* ctx, e8q_m8index* x, const float* base, const float* centroids,
+                         const uint32_t* cluster_ids, const uint32_t* rank, int on_device) {
+    return Guard([&] {
+        if (!ctx || !x || !base || !centroids || !cluster_ids || !rank) Throw(E8Q_EINVAL, "null argument");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const M8View& v = x->view;
+        if (!on_device)
+            for (uint64_t i = 0; i < v.n; ++i)
+                if (cluster_ids[i] >= v.nlist) Throw(E8Q_EINVAL, "cluster id out of range");
+        Check(hipMemcpyAsync(x->centroids.p, centroids, v.nlist * v.padded * 4, hipMemcpyHostToDevice, st),
+              "upload centroids");
+        Check(LaunchM8Centroids(st, v), "centroid kernel");
+        float* db = nullptr;
+        uint32_t *dc = nullptr, *dr = nullptr;
+        e8q_ctx::Carve cv;
+        if (!on_device) {
+            cv.add(&db, v.n * v.dim);
+            cv.add(&dc, v.n);
+        }
+        cv.add(&dr, v.n);
+        ctx->Take(cv);
+        if (!on_device) {
+            Check(hipMemcpyAsync(db, base, v.n * v.dim * 4, hipMemcpyHostToDevice, st), "upload base");
+            Check(hipMemcpyAsync(dc, cluster_ids, v.n * 4, hipMemcpyHostToDevice, st), "upload cluster ids");
+        }
+        Check(hipMemcpyAsync(dr, rank, v.n * 4, hipMemcpyHostToDevice, st), "upload ranks");
+        Check(LaunchM8Encode(st, v, on_device ? base : db, on_device ? cluster_ids : dc, dr, v.n), "encode kernel");
+        Check(hipStreamSynchronize(st), "encode");
+        BuildDerived(ctx, x);
+    });
+}
+
+e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e8q_m8_stats* stats,
+                         uint64_t* per_query_stats) {
+    return Guard([&] {
+        if (!ctx || !x || !a) Throw(E8Q_EINVAL, "null argument");
+        if (a->k == 0 || a->k > kM8MaxK) Throw(E8Q_EINVAL, "k must be in [1, 1024]");
+        if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
+        const M8View& v = x->view;
+        const uint64_t nq = a->nq;
+        if (stats) *stats = e8q_m8_stats{0, 0, 0, 0, 0, 0.0, 0.0};
+        if (nq == 0) return;
+        uint32_t maxp = 0;
+        if (a->nprobe_per_query) {
+            for (uint64_t i = 0; i < nq; ++i) {
+                if (a->nprobe_per_query[i] == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+                maxp = std::max<uint32_t>(maxp, a->nprobe_per_query[i]);
+            }
+        } else {
+            if (a->nprobe == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+            maxp = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
+        }
+        maxp = static_cast<uint32_t>(std::min<uint64_t>(maxp, v.nlist));
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const uint64_t P = v.padded, R = v.nrun, PC = v.plane_cols;
+        float *dq = nullptr, *rotq = nullptr, *qc = nullptr, *dist = nullptr, *pd = nullptr, *od = nullptr;
+        double *rc = nullptr, *sqd2 = nullptr;
+        uint8_t* lut = nullptr;
+        uint32_t *probe = nullptr, *npq = nullptr, *oid = nullptr;
+        uint64_t* qst = nullptr;
+        e8q_ctx::Carve cv;
+        if (!a->queries_on_device) cv.add(&dq, nq * v.dim);
+        cv.add(&rotq, nq * P);
+        cv.add(&lut, nq * PC * 32 + 16);
+        cv.add(&qc, nq * 4);
+        cv.add(&rc, nq * std::max<uint64_t>(R, 1) * 4);
+        cv.add(&dist, nq * v.nlist);
+        cv.add(&probe, nq * maxp);
+        cv.add(&pd, nq * maxp);
+        cv.add(&sqd2, nq * maxp * std::max<uint64_t>(R, 1));
+        if (a->nprobe_per_query) cv.add(&npq, nq);
+        if (!a->outputs_on_device) {
+            cv.add(&oid, nq * a->k);
+            cv.add(&od, nq * a->k);
+        }
+        cv.add(&qst, nq * 5);
+        ctx->Take(cv);
+        const float* q = a->queries;
+        if (!a->queries_on_device) {
+            Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+            q = dq;
+        }
+        if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
+        Check(hipEventRecord(ctx->ev[0], st), "event");
+        Check(LaunchM8QueryPrep(st, v, q, nq, rotq, lut, qc, rc), "query prep");
+        RouteArgs ra{};
+        ra.rotq = rotq;
+        ra.qstride = static_cast<uint32_t>(P);
+        ra.qn2 = qc + 2;
+        ra.qn2_stride = 4;
+        ra.cent = v.cent_rot;
+        ra.cn2 = v.cn2;
+        ra.nlist = v.nlist;
+        ra.dim = v.route_dim;
+        ra.cstride = static_cast<uint32_t>(P);
+        ra.nq = nq;
+        ra.dist = dist;
+        Check(LaunchRouteDist(st, ra), "route");
+        Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq,
+                                 static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist)), maxp, probe, pd),
+              "select probes");
+        Check(LaunchM8RunDist(st, v, rotq, rc, probe, npq, static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist)),
+                              maxp, nq, sqd2),
+              "run distances");
+        M8ScanArgs sa{};
+        sa.v = v;
+        sa.rotq = rotq;
+        sa.lut = lut;
+        sa.qc = qc;
+        sa.rc = rc;
+        sa.probe = probe;
+        sa.pdist = pd;
+        sa.sqd2 = sqd2;
+        sa.nprobe_q = npq;
+        sa.nprobe_all = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
+        sa.maxp = maxp;
+        sa.nq = nq;
+        sa.k = static_cast<uint32_t>(a->k);
+        sa.z = a->z;
+        sa.out_ids = a->outputs_on_device ? a->out_ids : oid;
+        sa.out_d = a->outputs_on_device ? a->out_dists : od;
+        sa.qstats = qst;
+        Check(hipEventRecord(ctx->ev[1], st), "event");
+        Check(LaunchM8Scan(st, sa), "scan");
+        Check(hipEventRecord(ctx->ev[2], st), "event");
+        if (!a->outputs_on_device) {
+            Check(hipMemcpyAsync(a->out_ids, oid, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
+            Check(hipMemcpyAsync(a->out_dists, od, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");
+        }
+        std::vector<uint64_t> hs(nq * 5);
+        Check(hipMemcpyAsync(hs.data(), qst, nq * 5 * 8, hipMemcpyDeviceToHost, st), "download stats");
+        Check(hipStreamSynchronize(st), "search");
+        if (per_query_stats) std::memcpy(per_query_stats, hs.data(), hs.size() * 8);
+        if (stats) {
+            for (uint64_t i = 0; i < nq; ++i) {
+                stats->candidates += hs[i * 5];
+                stats->survivors += hs[i * 5 + 1];
+                stats->bits_scored += hs[i * 5 + 2];
+                stats->bits_total += hs[i * 5 + 3];
+                stats->pruned += hs[i * 5 + 4];
+            }
+            float ms = 0.0F;
+            Check(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "event time");
+            stats->scan_ms = ms;
+            Check(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]), "event time");
+            stats->total_ms = ms;
+        }
+    });
+}
+
+e8q_status e8q_m8_debug_fast(e8q_ctx* ctx, const e8q_m8index* x, uint64_t c, uint64_t* sizes3, uint8_t* plane,
+                             uint8_t* refine, float* gain, float* vpc, double* rc_n2) {
+    return Guard([&] {
+        if (!ctx || !x || !sizes3) Throw(E8Q_EINVAL, "null argument");
+        const M8View& v = x->view;
+        if (c >= v.nlist) Throw(E8Q_EINVAL, "list out of range");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const uint64_t R = v.nrun, PC = v.plane_cols;
+        const uint64_t cnt = x->list_begin[c + 1] - x->list_begin[c];
+        const uint64_t nb = x->batch_begin[c + 1] - x->batch_begin[c];
+        const uint64_t rrec = PC * 32 + R * kM8BankBytes;   // reference record bytes
+        sizes3[0] = nb * rrec;
+        sizes3[1] = cnt * x->ref_stride;
+        sizes3[2] = R * cnt;
+        std::vector<uint8_t> dp(nb * v.rec), dr(cnt * v.rstride);
+        std::vector<double> rn(R);
+        if (nb) Check(hipMemcpyAsync(dp.data(), x->plane.as<uint8_t>() + x->batch_begin[c] * v.rec, dp.size(),
+                                     hipMemcpyDeviceToHost, st), "download plane");
+        if (cnt) Check(hipMemcpyAsync(dr.data(), x->refine.as<uint8_t>() + x->list_begin[c] * v.rstride, dr.size(),
+                                      hipMemcpyDeviceToHost, st), "download refine");
+        if (R) Check(hipMemcpyAsync(rn.data(), x->rc_n2.as<double>() + c * R, R * 8, hipMemcpyDeviceToHost, st),
+                     "download rc_n2");
+        Check(hipStreamSynchronize(st), "debug fast");
+        if (plane)
+            for (uint64_t b = 0; b < nb; ++b) {
+                const uint8_t* s = dp.data() + b * v.rec;
+                uint8_t* o = plane + b * rrec;
+                for (uint64_t col = 0; col < PC; ++col)
+                    for (uint64_t l = 0; l < 32; ++l) o[col * 32 + l] = s[l * PC + col];
+                std::memcpy(o + PC * 32, s + 32 * PC, R * kM8BankBytes);
+            }
+        for (uint64_t i = 0; i < cnt; ++i) {
+            const uint8_t* s = dr.data() + i * v.rstride;
+            if (refine) {
+                uint8_t* o = refine + i * x->ref_stride;
+                for (uint64_t r = 0; r < R; ++r) std::memcpy(o + r * 8, s + r * 16, 8);
+                for (uint64_t r = 0; r < R; ++r) {
+                    const M8Run& ru = x->runs[r];
+                    if (ru.rate >= 2) std::memcpy(o + x->ref_run_off[r], s + ru.ref_off, (ru.rate - 1) * ru.nblk);
+                }
+            }
+            for (uint64_t r = 0; r < R; ++r) {
+                float f4[4];
+                std::memcpy(f4, s + r * 16, 16);
+                if (gain) gain[r * cnt + i] = f4[2];
+                if (vpc) vpc[r * cnt + i] = f4[3];
+            }
+        }
+        if (rc_n2) std::memcpy(rc_n2, rn.data(), R * 8);
+    });
+}
+
+e8q_status e8q_m8_debug_centroids(e8q_ctx* ctx, const e8q_m8index* x, float* cent_rot, float* cn2) {
+    return Guard([&] {
+        if (!ctx || !x) Throw(E8Q_EINVAL, "null argument");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const M8View& v = x->view;
+        if (cent_rot) Check(hipMemcpyAsync(cent_rot, x->cent_rot.p, v.nlist * v.padded * 4, hipMemcpyDeviceToHost, st), "d");
+        if (cn2) Check(hipMemcpyAsync(cn2, x->cn2.p, v.nlist * 4, hipMemcpyDeviceToHost, st), "d");
+        Check(hipStreamSynchronize(st), "debug centroids");
+    });
+}
+
+e8q_status e8q_m8_debug_query(e8q_ctx* ctx, const e8q_m8index* x, const float* queries, uint64_t nq, uint64_t nprobe,
+                              float* rot_q, uint8_t* lut, float* lut_md, uint32_t* probe, float* pdist) {
+    return Guard([&] {
+        if (!ctx || !x || !queries) Throw(E8Q_EINVAL, "null argument");
+        if (nprobe == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const M8View& v = x->view;
+        const uint64_t P = v.padded, PC = v.plane_cols, R = std::max<uint64_t>(v.nrun, 1);
+        const uint32_t maxp = static_cast<uint32_t>(std::min<uint64_t>(nprobe, v.nlist));
+        float *dq = nullptr, *rotq = nullptr, *qc = nullptr, *dist = nullptr, *pd = nullptr;
+        double* rc = nullptr;
+        uint8_t* dl = nullptr;
+        uint32_t* pr = nullptr;
+        e8q_ctx::Carve cv;
+        cv.add(&dq, nq * v.dim);
+        cv.add(&rotq, nq * P);
+        cv.add(&dl, nq * PC * 32 + 16);
+        cv.add(&qc, nq * 4);
+        cv.add(&rc, nq * R * 4);
+        cv.add(&dist, nq * v.nlist);
+        cv.add(&pr, nq * maxp);
+        cv.add(&pd, nq * maxp);
+        ctx->Take(cv);
+        Check(hipMemcpyAsync(dq, queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload");
+        Check(LaunchM8QueryPrep(st, v, dq, nq, rotq, dl, qc, rc), "query prep");
+        RouteArgs ra{};
+        ra.rotq = rotq;
+        ra.qstride = static_cast<uint32_t>(P);
+        ra.qn2 = qc + 2;
+        ra.qn2_stride = 4;
+        ra.cent = v.cent_rot;
+        ra.cn2 = v.cn2;
+        ra.nlist = v.nlist;
+        ra.dim = v.route_dim;
+        ra.cstride = static_cast<uint32_t>(P);
+        ra.nq = nq;
+        ra.dist = dist;
+        Check(LaunchRouteDist(st, ra), "route");
+        Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), nullptr, maxp, maxp, pr, pd), "select");
+        std::vector<uint8_t> hl(nq * PC * 32);
+        std::vector<float> hq(nq * 4);
+        if (rot_q) Check(hipMemcpyAsync(rot_q, rotq, nq * P * 4, hipMemcpyDeviceToHost, st), "d");
+        if (!hl.empty()) Check(hipMemcpyAsync(hl.data(), dl, hl.size(), hipMemcpyDeviceToHost, st), "d");
+        Check(hipMemcpyAsync(hq.data(), qc, nq * 16, hipMemcpyDeviceToHost, st), "d");
+        if (probe) Check(hipMemcpyAsync(probe, pr, nq * maxp * 4, hipMemcpyDeviceToHost, st), "d");
+        if (pdist) Check(hipMemcpyAsync(pdist, pd, nq * maxp * 4, hipMemcpyDeviceToHost, st), "d");
+        Check(hipStreamSynchronize(st), "debug query");
+        for (uint64_t i = 0; i < nq; ++i) {
+            if (lut_md) {
+                lut_md[i * 2] = hq[i * 4];
+                lut_md[i * 2 + 1] = hq[i * 4 + 1];
+            }
+            if (lut)
+                for (uint64_t col = 0; col < PC; ++col) {
+                    const uint8_t* s = hl.data() + (i * PC + col) * 32;
+                    uint8_t* o = lut + (i * PC + col) * 64;
+                    for (int e = 0; e < 16; ++e) {
+                        o[e] = o[16 + e] = s[e];
+                        o[32 + e] = o[48 + e] = s[16 + e];
+                    }
+                }
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,634 @@
+// me8q_search.hip -- ME8Q FastScan search on the device (a13-a20):
+//
+//   M8QueryPrepKernel  per query: zero-pad, rotate each coded run with its
+//                      own FHT-Kac-4 (me8q_search.cpp:26-43), the class-mean
+//                      LUT per plane column and its u8 quantization
+//                      (me8q_fast.cpp:627-727, fastscan_avx2.cpp:8-51), the
+//                      route prefix norm and the per-run constants.
+//   M8RunDistKernel    per (query, probed list, run): d2_r = |q_r|^2 +
+//                      |c_r|^2 - 2 q_r.c_r in fp64 (me8q_fast.cpp:756-770).
+//   M8ScanKernel       one wave per query, lists in probe order: per run the
+//                      integer coarse sums of two batches at once (a17), the
+//                      fp32 coarse prune of whole groups of 8 lanes (a18,
+//                      me8q_prune_avx2.cpp:9-98), the fp64 lane test, the
+//                      four-candidate exact cascade (a19, me8q_fast.cpp:
+//                      856-979) and the sorted top-k (a20, me8q_search.cpp:
+//                      121-134).
+//
+// The reference's schedule is sequential per query: the k-th distance only
+// changes when four staged candidates are refined, so the kernel tests all
+// remaining kept lanes of a batch at once against the current threshold,
+// stages the first four that pass (lane order), refines them, and tests
+// the lanes after the last staged one again with the new threshold.  That
+// is the reference's lane-by-lane loop, result for result.
+#include <hip/hip_runtime.h>
+
+#include "e8q_kernels.h"
+#include "me8q_device.h"
+#include "me8q_kernels.h"
+
+namespace e8qdev {
+
+constexpr int kM8PrepThreads = 256;
+
+__global__ __launch_bounds__(kM8PrepThreads) void M8QueryPrepKernel(M8View v, const float* __restrict__ queries,
+                                                                    uint64_t nq, float* __restrict__ rotq,
+                                                                    uint8_t* __restrict__ lut, float* __restrict__ qc,
+                                                                    double* __restrict__ rc) {
+    extern __shared__ float sm[];
+    const uint32_t P = static_cast<uint32_t>(v.padded);
+    const uint32_t PC = v.plane_cols;
+    float* buf = sm;            // padded
+    float* lf = sm + P;         // PC * 32 float LUT (16 low-nibble, 16 high-nibble per column)
+    __shared__ float s_lo[kM8PrepThreads], s_hi[kM8PrepThreads];
+    const int tid = threadIdx.x;
+    for (uint64_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+        for (uint32_t d = tid; d < P; d += kM8PrepThreads) buf[d] = d < v.dim ? queries[qi * v.dim + d] : 0.0F;
+        __syncthreads();
+        for (uint32_t r = 0; r < v.nrun; ++r) {
+            const M8Run& ru = v.runs[r];
+            RotateInLds(buf + ru.dim_off, ru.len, ru.trunc, ru.fac, ru.signs);
+        }
+        for (uint32_t s = 0; s < v.nseg; ++s) {
+            const M8Seg sg = v.segs[s];
+            if (sg.bits != 0) continue;
+            for (uint32_t d = tid; d < sg.len; d += kM8PrepThreads) buf[sg.off + d] = 0.0F;
+        }
+        __syncthreads();
+        for (uint32_t d = tid; d < P; d += kM8PrepThreads) rotq[qi * P + d] = buf[d];
+        // Class-mean LUT.  `qb4*a4 + qb5*a5 + qb6*a6 + qb7*a7` as the
+        // reference build evaluates it: the first product fused into the
+        // first add, then one fused multiply-add per further term.
+        for (uint32_t e = tid; e < PC * 16; e += kM8PrepThreads) {
+            const uint32_t col = e >> 4, val = e & 15;
+            uint32_t r = 0;
+            while (r + 1 < v.nrun && v.runs[r + 1].blk0 <= col) ++r;
+            const M8Run& ru = v.runs[r];
+            const float p0 = ru.pbar0, p1 = ru.pbar1;
+            const float p70 = ru.rate == 1 ? 0.0F : p0, p71 = ru.rate == 1 ? 0.0F : p1;
+            const float* qb = buf + ru.dim_off + (col - ru.blk0) * 8;
+            const float a4 = (val >> 3) & 1 ? p1 : p0, a5 = (val >> 2) & 1 ? p1 : p0, a6 = (val >> 1) & 1 ? p1 : p0;
+            const float a7 = val & 1 ? p71 : p70;
+            const float a3 = val & 1 ? p1 : p0;
+            lf[col * 32 + val] = fmaf(qb[7], a7, fmaf(qb[6], a6, fmaf(qb[4], a4, qb[5] * a5)));
+            lf[col * 32 + 16 + val] = fmaf(qb[3], a3, fmaf(qb[2], a6, fmaf(qb[0], a4, qb[1] * a5)));
+        }
+        __syncthreads();
+        float lo = __builtin_inff(), hi = -__builtin_inff();
+        for (uint32_t e = tid; e < PC * 32; e += kM8PrepThreads) {
+            lo = fminf(lo, lf[e]);
+            hi = fmaxf(hi, lf[e]);
+        }
+        s_lo[tid] = lo;
+        s_hi[tid] = hi;
+        __syncthreads();
+        for (int w = kM8PrepThreads / 2; w > 0; w >>= 1) {
+            if (tid < w) {
+                s_lo[tid] = fminf(s_lo[tid], s_lo[tid + w]);
+                s_hi[tid] = fmaxf(s_hi[tid], s_hi[tid + w]);
+            }
+            __syncthreads();
+        }
+        lo = PC ? s_lo[0] : 0.0F;
+        hi = PC ? s_hi[0] : 0.0F;
+        const float delta = hi > lo ? (hi - lo) / 255.0F : 1.0F;
+        for (uint32_t e = tid; e < PC * 32; e += kM8PrepThreads) {
+            const float t = (lf[e] - lo) / delta + 0.5F;
+            const int iv = static_cast<int>(t);
+            lut[qi * PC * 32 + e] = static_cast<uint8_t>(min(max(iv, 0), 255));
+        }
+        if (tid == 0) {
+            // |q|^2 over the route prefix in the AVX2 order (8 FMA chains).
+            float acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (uint32_t d = 0; d < v.route_dim; d += 8)
+                for (int l = 0; l < 8; ++l) acc8[l] = fmaf(buf[d + l], buf[d + l], acc8[l]);
+            const float qn2 = ((acc8[0] + acc8[4]) + (acc8[2] + acc8[6])) + ((acc8[1] + acc8[5]) + (acc8[3] + acc8[7]));
+            uint32_t bits_row = 0;
+            for (uint32_t r = 0; r < v.nrun; ++r) bits_row += v.runs[r].rate * v.runs[r].len;
+            qc[qi * 4 + 0] = lo;
+            qc[qi * 4 + 1] = delta;
+            qc[qi * 4 + 2] = qn2;
+            qc[qi * 4 + 3] = static_cast<float>(bits_row);
+        }
+        if (tid >= 64 && static_cast<uint32_t>(tid - 64) < v.nrun) {
+            const uint32_t r = tid - 64;
+            const M8Run& ru = v.runs[r];
+            double a = 0.0;
+            for (uint32_t t = 0; t < ru.len; ++t) {
+                const double x = buf[ru.dim_off + t];
+                a = fma(x, x, a);
+            }
+            const double lmx = static_cast<double>(lo) * (2.0 * static_cast<double>(ru.nblk));
+            const float sq = static_cast<float>(sqrt(static_cast<double>(ru.nblk > 1 ? ru.nblk : 1)));
+            const double qerr = static_cast<double>(2.0F * sq * delta);
+            const double hv = fma(2.0 * static_cast<double>(ru.nblk) * 255.0, static_cast<double>(delta), lmx);
+            double* o = rc + (qi * v.nrun + r) * 4;
+            o[0] = a;
+            o[1] = lmx;
+            o[2] = qerr;
+            o[3] = fmax(fabs(lmx), fabs(hv));
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void M8RunDistKernel(M8View v, const float* __restrict__ rotq,
+                                                       const double* __restrict__ rc,
+                                                       const uint32_t* __restrict__ probe,
+                                                       const uint32_t* __restrict__ nprobe_q, uint32_t nprobe_all,
+                                                       uint32_t maxp, uint64_t nq, double* __restrict__ sqd2) {
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t per_q = static_cast<uint64_t>(maxp) * v.nrun;
+    if (t >= nq * per_q) return;
+    const uint64_t qi = t / per_q;
+    const uint32_t p = static_cast<uint32_t>((t % per_q) / v.nrun);
+    const uint32_t r = static_cast<uint32_t>(t % v.nrun);
+    const uint64_t np = min<uint64_t>(nprobe_q ? nprobe_q[qi] : nprobe_all, v.nlist);
+    if (p >= np) return;
+    const uint32_t c = probe[qi * maxp + p];
+    const M8Run& ru = v.runs[r];
+    const float* qr = rotq + qi * v.padded + ru.dim_off;
+    const float* cr = v.cent_rot + static_cast<uint64_t>(c) * v.padded + ru.dim_off;
+    double dot = 0.0;
+    for (uint32_t j = 0; j < ru.len; ++j) dot = fma(static_cast<double>(qr[j]), static_cast<double>(cr[j]), dot);
+    const double d2r = fma(-2.0, dot, rc[(qi * v.nrun + r) * 4] + v.rc_n2[static_cast<uint64_t>(c) * v.nrun + r]);
+    sqd2[t] = sqrt(fmax(d2r, 0.0));
+}
+
+// ---------------------------------------------------------------------------
+// Scan: LDS layout of one wave (one query).
+struct M8ScanLayout {
+    uint32_t q, lut, td, ti, acc, fdc, fpool, fqa, fsc, ftr, tsc, ttr, ssc, str, sS, sord, smean, rq, rqf, ssq,
+        stage, total;
+    __host__ __device__ M8ScanLayout(uint32_t P, uint32_t PC, uint32_t R, uint32_t K) {
+        uint32_t o = 0;
+        auto take = [&](uint32_t bytes, uint32_t align) {
+            o = (o + align - 1) & ~(align - 1);
+            const uint32_t at = o;
+            o += bytes;
+            return at;
+        };
+        const uint32_t R1 = R > 0 ? R : 1;
+        q = take(P * 4, 16);
+        lut = take(PC * 32 + 16, 16);
+        td = take(K * 4, 16);
+        ti = take(K * 4, 16);
+        acc = take(R1 * 64 * 4, 16);
+        fdc = take(64 * 4, 16);
+        fpool = take(64 * 4, 16);
+        fqa = take(64 * 4, 16);
+        fsc = take(R1 * 64 * 4, 16);
+        ftr = take(R1 * 64 * 4, 16);
+        tsc = take(R1 * 8 * 8, 16);
+        ttr = take(R1 * 8 * 8, 16);
+        ssc = take(4 * R1 * 8, 16);
+        str = take(4 * R1 * 8, 16);
+        sS = take(4 * R1 * 4, 16);
+        sord = take(R1 * 4, 16);
+        smean = take(R1 * 8, 16);
+        rq = take(R1 * 3 * 8, 16);      // lut_min_x, qerr, score_max
+        rqf = take(R1 * 2 * 4, 16);     // float lut_min_x, float qerr
+        ssq = take(R1 * 8, 16);         // sqrt(d2_r) of the current list
+        stage = take(4 * 64, 16);       // staged candidates: lane,row,pool,qadd,dist,died,bits
+        total = (o + 15) & ~15u;
+    }
+};
+
+struct M8Stage {         // one staged candidate
+    uint32_t lane, row, died, pad;
+    double pool, qadd, dist;
+    uint64_t bits;
+    uint64_t pad2[2];
+};
+
+// Inserts (d, id) into the sorted list (td, ti) of `cnt` entries (<= K):
+// a full list rejects d unless it is below the worst; a new item goes before
+// equal ones (lower_bound by distance).  Called by the whole wave with
+// uniform arguments.
+__device__ void M8Push(float* td, uint32_t* ti, uint32_t& cnt, uint32_t K, float d, uint32_t id) {
+    const int lane = threadIdx.x;
+    if (cnt >= K && !(d < td[K - 1])) return;
+    if (cnt >= K) cnt = K - 1;
+    uint32_t pos = 0;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
+        const uint32_t e = e0 + lane;
+        pos += __popcll(__ballot(e < cnt && td[e] < d));
+    }
+    if (cnt > pos) {
+        for (int32_t s = static_cast<int32_t>((cnt - 1) & ~63u); s >= static_cast<int32_t>(pos & ~63u); s -= 64) {
+            const uint32_t e = static_cast<uint32_t>(s) + lane;
+            const bool mv = e >= pos && e < cnt;
+            float x = 0.0F;
+            uint32_t y = 0;
+            if (mv) {
+                x = td[e];
+                y = ti[e];
+            }
+            __syncthreads();
+            if (mv) {
+                td[e + 1] = x;
+                ti[e + 1] = y;
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        td[pos] = d;
+        ti[pos] = id;
+    }
+    __syncthreads();
+    ++cnt;
+}
+
+__global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const M8View& v = a.v;
+    const uint32_t P = static_cast<uint32_t>(v.padded), PC = v.plane_cols, R = v.nrun, K = a.k;
+    const M8ScanLayout L(P, PC, R, K);
+    float* q = reinterpret_cast<float*>(smem + L.q);
+    uint8_t* lut = smem + L.lut;
+    float* td = reinterpret_cast<float*>(smem + L.td);
+    uint32_t* ti = reinterpret_cast<uint32_t*>(smem + L.ti);
+    int32_t* sacc = reinterpret_cast<int32_t*>(smem + L.acc);
+    float* fdc = reinterpret_cast<float*>(smem + L.fdc);
+    float* fpool = reinterpret_cast<float*>(smem + L.fpool);
+    float* fqa = reinterpret_cast<float*>(smem + L.fqa);
+    float* fsc = reinterpret_cast<float*>(smem + L.fsc);
+    float* ftr = reinterpret_cast<float*>(smem + L.ftr);
+    double* tsc = reinterpret_cast<double*>(smem + L.tsc);
+    double* ttr = reinterpret_cast<double*>(smem + L.ttr);
+    double* ssc = reinterpret_cast<double*>(smem + L.ssc);
+    double* str = reinterpret_cast<double*>(smem + L.str);
+    float* sS = reinterpret_cast<float*>(smem + L.sS);
+    uint32_t* sord = reinterpret_cast<uint32_t*>(smem + L.sord);
+    double* smean = reinterpret_cast<double*>(smem + L.smean);
+    double* rq = reinterpret_cast<double*>(smem + L.rq);
+    float* rqf = reinterpret_cast<float*>(smem + L.rqf);
+    double* ssq = reinterpret_cast<double*>(smem + L.ssq);
+    M8Stage* stg = reinterpret_cast<M8Stage*>(smem + L.stage);
+    const int lane = threadIdx.x;
+    const float finf = __builtin_inff();
+    const double dinf = __builtin_inf();
+    const float z = a.z;
+    const double dz = z;
+
+    for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
+        for (uint32_t d = lane; d < P; d += 64) q[d] = a.rotq[qi * P + d];
+        for (uint32_t e = lane; e < PC * 8; e += 64)
+            reinterpret_cast<uint32_t*>(lut)[e] = reinterpret_cast<const uint32_t*>(a.lut + qi * PC * 32)[e];
+        for (uint32_t r = lane; r < R; r += 64) {
+            const double* o = a.rc + (qi * R + r) * 4;
+            rq[r * 3 + 0] = o[1];
+            rq[r * 3 + 1] = o[2];
+            rq[r * 3 + 2] = o[3];
+            rqf[r * 2 + 0] = static_cast<float>(o[1]);
+            rqf[r * 2 + 1] = static_cast<float>(o[2]);
+        }
+        const float lut_delta = a.qc[qi * 4 + 1];
+        const uint64_t bits_row = static_cast<uint64_t>(a.qc[qi * 4 + 3]);
+        uint32_t cnt = 0;
+        uint64_t n_cand = 0, n_surv = 0, n_bits = 0, n_btot = 0;
+        __syncthreads();
+        const uint64_t np = min<uint64_t>(a.nprobe_q ? a.nprobe_q[qi] : a.nprobe_all, v.nlist);
+        for (uint64_t p = 0; p < np; ++p) {
+            const uint32_t c = a.probe[qi * a.maxp + p];
+            const float d2f = a.pdist[qi * a.maxp + p];
+            const double d2 = d2f;
+            const uint64_t lb = v.list_begin[c];
+            const uint64_t cntc = v.list_begin[c + 1] - lb;
+            if (R == 0) {   // no coded run: every row estimates to the list distance
+                for (uint64_t i = 0; i < cntc; ++i) M8Push(td, ti, cnt, K, d2f, v.ids[lb + i]);
+                n_cand += cntc;
+                n_surv += cntc;
+                continue;
+            }
+            if (cntc == 0) continue;
+            for (uint32_t r = lane; r < R; r += 64) ssq[r] = a.sqd2[(qi * a.maxp + p) * R + r];
+            const uint64_t nb = (cntc + 31) >> 5;
+            const uint64_t bat0 = v.batch_begin[c];
+            for (uint64_t b0 = 0; b0 < nb; b0 += 2) {
+                {   // integer coarse sums of batches b0 (lanes 0..31) and b0+1 (32..63)
+                    const uint64_t b = b0 + (lane >> 5);
+                    const uint32_t l = lane & 31;
+                    if (b < nb) {
+                        const uint8_t* rowp = v.plane + (bat0 + b) * v.rec + l * PC;
+                        for (uint32_t r = 0; r < R; ++r) {
+                            const M8Run& ru = v.runs[r];
+                            int32_t s = 0;
+                            for (uint32_t j = 0; j < ru.nblk; j += 8) {
+                                const uint2 w = *reinterpret_cast<const uint2*>(rowp + ru.blk0 + j);
+                                const uint8_t* lc = lut + (ru.blk0 + j) * 32;
+#pragma unroll
+                                for (int k2 = 0; k2 < 4; ++k2) {
+                                    const uint32_t x0 = (w.x >> (8 * k2)) & 0xFFu, x1 = (w.y >> (8 * k2)) & 0xFFu;
+                                    s += lc[k2 * 32 + (x0 & 15)] + lc[k2 * 32 + 16 + (x0 >> 4)];
+                                    s += lc[(4 + k2) * 32 + (x1 & 15)] + lc[(4 + k2) * 32 + 16 + (x1 >> 4)];
+                                }
+                            }
+                            sacc[r * 64 + lane] = s;
+                        }
+                    }
+                }
+                __syncthreads();
+                for (uint32_t hh = 0; hh < 2 && b0 + hh < nb; ++hh) {
+                    const uint64_t bb = b0 + hh;
+                    const uint32_t cntb = static_cast<uint32_t>(min<uint64_t>(32, cntc - bb * 32));
+                    const uint32_t covered = cntb >= 8 ? (cntb & ~7u) : 0u;
+                    const uint8_t* rec = v.plane + (bat0 + bb) * v.rec;
+                    const uint8_t* banks = rec + 32u * PC;
+                    const bool mine = static_cast<uint32_t>(lane >> 5) == hh;
+                    const uint32_t l = lane & 31;
+                    const float distk = cnt < K ? finf : td[K - 1];
+                    const float sqk = distk < finf ? sqrtf(fmaxf(0.0F, distk)) : finf;
+                    bool kept = false;
+                    if (mine && l < covered) {   // fp32 coarse prune (AVX2 filter)
+                        float dc = d2f, pool = 0.0F, qa = 0.0F, wpad = 0.0F;
+                        for (uint32_t r = 0; r < R; ++r) {
+                            const uint8_t* bp = banks + r * kM8BankBytes;
+                            const float* sc = reinterpret_cast<const float*>(bp + 128);
+                            const float s_c = fmaf(static_cast<float>(sacc[r * 64 + lane]), lut_delta, rqf[r * 2]);
+                            const float bq = fmaf(static_cast<float>(bp[l]), sc[1], sc[0]);
+                            const float wq = fmaf(static_cast<float>(bp[32 + l]), sc[3], sc[2]);
+                            const float pq = fmaf(static_cast<float>(bp[64 + l]), sc[5], sc[4]);
+                            dc = fmaf(-wq, s_c, dc);
+                            dc = dc + bq;
+                            const float ra = static_cast<float>(ssq[r]);
+                            const float rh = sqk < ra ? sqk : ra;
+                            const float t = pq * rh;
+                            fsc[r * 64 + lane] = s_c;
+                            ftr[r * 64 + lane] = t;
+                            pool = fmaf(t, t, pool);
+                            qa = fmaf(fabsf(wq), rqf[r * 2 + 1], qa);
+                            const float wp = static_cast<float>(0.5 * static_cast<double>(sc[3]) *
+                                                                (rq[r * 3 + 2] + rq[r * 3 + 1]));
+                            wpad = wpad + wp;
+                        }
+                        const float qw = qa + wpad;
+                        const float slack = fmaf(z, sqrtf(pool), qw);
+                        fdc[lane] = dc;
+                        fpool[lane] = pool;
+                        fqa[lane] = qw;
+                        kept = !(dc - slack >= distk);
+                    } else if (mine && l < cntb) {
+                        kept = true;
+                    }
+                    n_cand += cntb;
+                    n_btot += static_cast<uint64_t>(cntb) * bits_row;
+                    const uint32_t keep = static_cast<uint32_t>(__ballot(kept) >> (32 * hh));
+                    __syncthreads();
+                    uint32_t nst = 0, pos = 0;
+                    for (;;) {
+                        const uint32_t rem = pos >= 32 ? 0u : keep & ~((1u << pos) - 1u);
+                        if (rem == 0) break;
+                        const double distk_d = cnt < K ? dinf : static_cast<double>(td[K - 1]);
+                        bool pass = false;
+                        double pool2 = 0.0, qadd = 0.0;
+                        if (mine && ((rem >> l) & 1u)) {
+                            double dcq;
+                            if (l < covered) {
+                                dcq = fdc[lane];
+                                pool2 = fpool[lane];
+                                qadd = fqa[lane];
+                            } else {   // fp64 twin of the filter for the lanes it does not cover
+                                dcq = d2;
+                                const double sqdk = distk_d < dinf ? sqrt(fmax(distk_d, 0.0)) : dinf;
+                                const uint32_t tl = l - covered;
+                                for (uint32_t r = 0; r < R; ++r) {
+                                    const uint8_t* bp = banks + r * kM8BankBytes;
+                                    const float* scl = reinterpret_cast<const float*>(bp + 128);
+                                    const double scr = fma(static_cast<double>(lut_delta),
+                                                           static_cast<double>(sacc[r * 64 + lane]), rq[r * 3]);
+                                    const double bq = fma(static_cast<double>(bp[l]), static_cast<double>(scl[1]),
+                                                          static_cast<double>(scl[0]));
+                                    const double wq = fma(static_cast<double>(bp[32 + l]), static_cast<double>(scl[3]),
+                                                          static_cast<double>(scl[2]));
+                                    const double pq = fma(static_cast<double>(bp[64 + l]), static_cast<double>(scl[5]),
+                                                          static_cast<double>(scl[4]));
+                                    dcq += fma(-wq, scr, bq);
+                                    const double sa = ssq[r];
+                                    const double rh = sqdk < sa ? sqdk : sa;
+                                    const double tr = pq * rh;
+                                    tsc[r * 8 + tl] = scr;
+                                    ttr[r * 8 + tl] = tr;
+                                    pool2 = fma(tr, tr, pool2);
+                                    qadd = fma(fabs(wq), rq[r * 3 + 1], qadd);
+                                    qadd = fma(0.5 * static_cast<double>(scl[3]), rq[r * 3 + 2] + rq[r * 3 + 1], qadd);
+                                }
+                            }
+                            pass = !(dcq - fma(dz, sqrt(pool2), qadd) >= distk_d);
+                        }
+                        const uint32_t pm = static_cast<uint32_t>(__ballot(pass) >> (32 * hh));
+                        const uint32_t need = 4 - nst;
+                        uint32_t sel = 0, m = pm;
+                        for (uint32_t t = 0; t < need && m; ++t) {
+                            const uint32_t low = m & (0u - m);
+                            sel |= low;
+                            m ^= low;
+                        }
+                        if (mine && ((sel >> l) & 1u)) {
+                            const uint32_t slot = nst + __popc(sel & ((1u << l) - 1u));
+                            M8Stage& s = stg[slot];
+                            s.lane = l;
+                            s.row = static_cast<uint32_t>(bb * 32 + l);
+                            s.pool = pool2;
+                            s.qadd = qadd;
+                            for (uint32_t r = 0; r < R; ++r) {
+                                ssc[slot * R + r] = l < covered ? static_cast<double>(fsc[r * 64 + lane]) : tsc[r * 8 + (l - covered)];
+                                str[slot * R + r] = l < covered ? static_cast<double>(ftr[r * 64 + lane]) : ttr[r * 8 + (l - covered)];
+                            }
+                        }
+                        nst += __popc(sel);
+                        __syncthreads();
+                        const bool full = nst == 4;
+                        if (full || (nst > 0 && pm == sel)) {
+                            // refine the staged candidates (flush4)
+                            const uint32_t n = nst;
+                            if (lane == 0) {
+                                for (uint32_t r = 0; r < R; ++r) sord[r] = r;
+                                if (R > 1) {
+                                    for (uint32_t r = 0; r < R; ++r) smean[r] = 0.0;
+                                    for (uint32_t j = 0; j < n; ++j)
+                                        for (uint32_t r = 0; r < R; ++r)
+                                            smean[r] = fma(str[j * R + r], str[j * R + r], smean[r]);
+                                    for (uint32_t r = 0; r < R; ++r)
+                                        smean[r] /= static_cast<double>(v.runs[r].nblk > 1 ? v.runs[r].nblk : 1);
+                                    // insertion sort, descending, stable (GCC's std::sort below 17 elements)
+                                    for (uint32_t i = 1; i < R; ++i) {
+                                        const uint32_t x = sord[i];
+                                        uint32_t j = i;
+                                        while (j > 0 && smean[x] > smean[sord[j - 1]]) {
+                                            sord[j] = sord[j - 1];
+                                            --j;
+                                        }
+                                        sord[j] = x;
+                                    }
+                                }
+                            }
+                            // exact run scores of every (candidate, run): 8 lanes each
+                            const uint32_t pairs = n * R;
+                            const uint32_t g = lane >> 3, ci = lane & 7;
+                            for (uint32_t base = 0; base < pairs; base += 8) {
+                                const uint32_t pidx = base + g;
+                                float acc = 0.0F, acc2 = 0.0F;
+                                if (pidx < pairs) {
+                                    const uint32_t j = pidx / R, r = pidx % R;
+                                    const M8Run& ru = v.runs[r];
+                                    const M8Stage& s = stg[j];
+                                    const uint8_t* pbt = rec + s.lane * PC + ru.blk0;
+                                    const uint8_t* rr = v.refine + (lb + s.row) * v.rstride + ru.ref_off;
+                                    const uint32_t rate = ru.rate, T = rate - 1;
+                                    const float* qb = q + ru.dim_off + ci;
+                                    for (uint32_t blk = 0; blk < ru.nblk; ++blk) {
+                                        const uint32_t pbyte = pbt[blk];
+                                        const uint32_t top = (pbyte >> (7 - ci)) & 1u;
+                                        uint32_t dgt, br;
+                                        if (rate == 1) {
+                                            dgt = ci < 7 ? top : (__popc(pbyte >> 1) & 1u);
+                                            br = pbyte & 1u;
+                                        } else {
+                                            const uint8_t* fb = rr + blk * T;
+                                            uint32_t lo;
+                                            if (ci < 7) {
+                                                lo = M8Bits(fb, T * ci, T);
+                                            } else {
+                                                uint32_t par = 0;
+                                                for (uint32_t jj = 0; jj < 7; ++jj) par ^= M8Bits(fb, T * jj, 1);
+                                                lo = (M8Bits(fb, 7 * T, T - 1) << 1) | par;
+                                            }
+                                            br = M8Bits(fb, 8 * T - 1, 1);
+                                            dgt = lo | (top << T);
+                                        }
+                                        const float lv = ru.lev[(br << rate) + dgt];
+                                        const float qv = qb[blk * 8];
+                                        if (blk & 1) acc2 = fmaf(lv, qv, acc2);
+                                        else acc = fmaf(lv, qv, acc);
+                                    }
+                                }
+                                const float sc8 = Hsum8Lanes(acc + acc2);
+                                if (ci == 0 && pidx < pairs) sS[pidx] = sc8;
+                            }
+                            __syncthreads();
+                            const double distk_c = cnt < K ? dinf : static_cast<double>(td[K - 1]);
+                            if (static_cast<uint32_t>(lane) < n) {   // the cascade, one lane per candidate
+                                const uint32_t j = lane;
+                                M8Stage& s = stg[j];
+                                const float* f4 = reinterpret_cast<const float*>(v.refine + (lb + s.row) * v.rstride);
+                                double dist;
+                                if (R == 1) {
+                                    dist = d2 + static_cast<double>(f4[1]);
+                                } else {
+                                    double dc = d2;
+                                    for (uint32_t r = 0; r < R; ++r) {
+                                        const double coef = f4[r * 4], h2 = f4[r * 4 + 1];
+                                        const double gn = f4[r * 4 + 2], vp = f4[r * 4 + 3];
+                                        dc += fma(-(2.0 * coef), fma(gn, ssc[j * R + r], vp), h2);
+                                    }
+                                    dist = dc;
+                                }
+                                double rem2 = s.pool;
+                                bool live = true;
+                                uint64_t bs = 0;
+                                for (uint32_t oi = 0; oi < R && live; ++oi) {
+                                    const uint32_t r = sord[oi];
+                                    const double se = sS[j * R + r];
+                                    const double coef = f4[r * 4];
+                                    if (R == 1) {
+                                        dist = fma(-(2.0 * coef), se, dist);
+                                    } else {
+                                        const double gn = f4[r * 4 + 2], vp = f4[r * 4 + 3];
+                                        dist = fma(2.0 * coef, fma(gn, ssc[j * R + r], vp) - se, dist);
+                                    }
+                                    const double t = str[j * R + r];
+                                    rem2 = fma(-t, t, rem2);
+                                    bs += static_cast<uint64_t>(v.runs[r].rate) * v.runs[r].len;
+                                    if (fma(-dz, sqrt(fmax(rem2, 0.0)), dist) - s.qadd >= distk_c) live = false;
+                                }
+                                s.dist = dist;
+                                s.died = live ? 0u : 1u;
+                                s.bits = bs;
+                            }
+                            __syncthreads();
+                            for (uint32_t j = 0; j < n; ++j) {
+                                const M8Stage s = stg[j];
+                                n_bits += s.bits;
+                                if (s.died) continue;
+                                ++n_surv;
+                                M8Push(td, ti, cnt, K, static_cast<float>(s.dist), v.ids[lb + s.row]);
+                            }
+                            __syncthreads();
+                            pos = sel ? 32 - __clz(sel) : 32;
+                            if (!full) pos = 32;
+                            nst = 0;
+                        } else {
+                            break;
+                        }
+                    }
+                    if (nst > 0) {
+                        // unreachable: a partial group is refined inside the loop
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        for (uint32_t i = lane; i < K; i += 64) {
+            a.out_ids[qi * K + i] = i < cnt ? ti[i] : 0xFFFFFFFFu;
+            a.out_d[qi * K + i] = i < cnt ? td[i] : finf;
+        }
+        if (lane == 0 && a.qstats) {
+            uint64_t* o = a.qstats + qi * 5;
+            o[0] = n_cand;
+            o[1] = n_surv;
+            o[2] = n_bits;
+            o[3] = n_btot;
+            o[4] = n_cand - n_surv;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+
+hipError_t LaunchM8QueryPrep(hipStream_t st, const M8View& v, const float* queries, uint64_t nq, float* rotq,
+                             uint8_t* lut, float* qc, double* rc) {
+    if (nq == 0) return hipSuccess;
+    const size_t lds = (v.padded + static_cast<size_t>(v.plane_cols) * 32) * sizeof(float);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&M8QueryPrepKernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    const uint32_t grid = static_cast<uint32_t>(nq < 65536 ? nq : 65536);
+    hipLaunchKernelGGL(M8QueryPrepKernel, dim3(grid), dim3(kM8PrepThreads), lds, st, v, queries, nq, rotq, lut, qc,
+                       rc);
+    return hipGetLastError();
+}
+
+hipError_t LaunchM8RunDist(hipStream_t st, const M8View& v, const float* rotq, const double* rc,
+                           const uint32_t* probe, const uint32_t* nprobe_q, uint32_t nprobe_all, uint32_t maxp,
+                           uint64_t nq, double* sqd2) {
+    const uint64_t tot = nq * maxp * v.nrun;
+    if (tot == 0) return hipSuccess;
+    hipLaunchKernelGGL(M8RunDistKernel, dim3(static_cast<uint32_t>((tot + 255) / 256)), dim3(256), 0, st, v, rotq, rc,
+                       probe, nprobe_q, nprobe_all, maxp, nq, sqd2);
+    return hipGetLastError();
+}
+
+size_t M8ScanLds(const M8View& v, uint32_t k) {
+    return M8ScanLayout(static_cast<uint32_t>(v.padded), v.plane_cols, v.nrun, k).total;
+}
+
+hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a) {
+    if (a.nq == 0) return hipSuccess;
+    const size_t lds = M8ScanLds(a.v, a.k);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&M8ScanKernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    const uint32_t grid = static_cast<uint32_t>(a.nq < 1048576 ? a.nq : 1048576);
+    hipLaunchKernelGGL(M8ScanKernel, dim3(grid), dim3(64), lds, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace e8qdev

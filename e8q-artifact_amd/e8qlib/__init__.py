@@ -1,6 +1,7 @@
-"""E8Q IVF index on AMD MI355X (gfx950).
+"""E8Q and ME8Q IVF indexes on AMD MI355X (gfx950).
 
-Drop-in for the reference package's `e8qlib.E8qIndex`: same constructor,
+Drop-in for the reference package's `e8qlib.E8qIndex` and
+`e8qlib.Me8qIndex`: same constructors,
 build/search/save/load signatures and outputs, with build (E8 encode) and
 search running as HIP kernels on the GPU.  There is no CPU fallback: if the
 native extension or its device library is missing, importing this package
@@ -22,6 +23,6 @@ _here = os.path.dirname(os.path.abspath(__file__))
 if not os.path.exists(os.path.join(_here, "libe8q_gpu.so")):
     raise ImportError("e8qlib: libe8q_gpu.so is not built (run __graft_entry__.build())")
 
-from ._e8qlib import E8qIndex  # noqa: E402
+from ._e8qlib import E8qIndex, Me8qIndex  # noqa: E402
 
-__all__ = ["E8qIndex"]
+__all__ = ["E8qIndex", "Me8qIndex"]

@@ -1,0 +1,120 @@
+/* me8q_gpu.h -- C ABI of the ME8Q (mixed-rate E8Q) part of libe8q_gpu.so.
+ *
+ * The reference's ME8Q boundary is the C++ class e8qlib::me8q::Me8qIndex
+ * (include/e8qlib/index/me8q.hpp:69-247) and its pybind11 module
+ * (python_bindings/me8q_bindings.cpp:22-315); there is no FFI.  These entry
+ * points are the device layer those C++ methods now call.  Conventions as in
+ * e8q_gpu.h: plain pointers and sizes, an e8q_status per call, the message
+ * of the last failure in e8q_last_error(), host pointers unless a field says
+ * device, one host thread per context.
+ */
+#ifndef ME8Q_GPU_H_
+#define ME8Q_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "e8q_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct e8q_m8index e8q_m8index; /* device-resident ME8Q index */
+
+/* Static description: shape, plan, rotators, codec tables, list offsets.
+ * Mirrors Me8qIndex's private members (me8q.hpp:204-246); the host computes
+ * the plan (Allocate, me8q_plan.cpp:47-167, or ParsePlanSpec), the rotator
+ * signs (BuildRunRotators, me8q_build.cpp:29-40) and the level tables
+ * (BranchLevels / FixedScale, e8q_codec.cpp:427-509) as Build and Load do. */
+typedef struct {
+    uint64_t n, dim, padded, nlist;   /* padded = dim rounded up to 64 */
+    uint32_t nseg;                    /* plan runs */
+    const uint64_t* seg_len;          /* nseg, multiples of 64 summing to padded */
+    const int32_t* seg_bits;          /* nseg, 0 (dropped) or 1..13 */
+    const uint8_t* signs;             /* the coded runs' rotator sign bytes, 4*len/8 each, in plan order */
+    const double* levels;             /* for b = 1..13: branch 0 then branch 1, 2^b each */
+    const double* fixed_scale;        /* [14], FixedScale(b) at index b */
+    const uint64_t* list_begin;       /* nlist+1 vector offsets (cluster-major ids) */
+    int32_t drop_energy;              /* BuildOptions::drop_energy */
+} e8q_m8_desc;
+
+/* The arrays of an ME8QIDXX file (me8q_io.cpp:29-64) that are not implied by
+ * the desc: ids, the zero-padded (unrotated) centroids, the code rows. */
+typedef struct {
+    uint32_t* ids;        /* n, cluster-major                     */
+    float* centroids;     /* nlist*padded                         */
+    uint8_t* codes;       /* n*row_bytes (e8q_m8_row_bytes)       */
+} e8q_m8_arrays;
+
+typedef struct {
+    const float* queries;             /* nq*dim (device if queries_on_device) */
+    uint64_t nq;
+    uint64_t k;                       /* 1..1024 on the device (the reference has no cap) */
+    uint64_t nprobe;                  /* used when nprobe_per_query is null */
+    const uint32_t* nprobe_per_query; /* optional host array [nq] */
+    float z;                          /* lambda = OrthoQuantile(ortho_alpha, max(4k, 32)), host computed
+                                         (me8q_fast.cpp:444-450) */
+    int32_t queries_on_device;
+    uint32_t* out_ids;                /* nq*k, ascending distance, UINT32_MAX padded */
+    float* out_dists;                 /* nq*k, +inf padded */
+    int32_t outputs_on_device;
+} e8q_m8_search_args;
+
+typedef struct {
+    uint64_t candidates, survivors, bits_scored, bits_total, pruned;   /* me8q::SearchStats (me8q.hpp:54-69) */
+    double scan_ms;    /* scan kernel, HIP events on the context stream */
+    double total_ms;   /* device time of the whole search */
+} e8q_m8_stats;
+
+/* Allocates the device index and its static tables (the allocation half of
+ * Me8qIndex::Build / Load). */
+e8q_status e8q_m8_create(e8q_ctx* ctx, const e8q_m8_desc* desc, e8q_m8index** out);
+e8q_status e8q_m8_destroy(e8q_m8index* idx);
+uint64_t e8q_m8_device_bytes(const e8q_m8index* idx);
+/* Bytes of one code row: nrun (coef, h2) pairs + the runs' codes (me8q_build.cpp:103-135). */
+uint64_t e8q_m8_row_bytes(const e8q_m8index* idx);
+
+/* Uploads the arrays of a loaded file and derives the rotated centroids and
+ * the FastScan layout on the device (Me8qIndex::Load, me8q_io.cpp:85-176,
+ * PrepareCentroidsRotated + BuildFastLayout). */
+e8q_status e8q_m8_upload(e8q_ctx* ctx, e8q_m8index* idx, const e8q_m8_arrays* host);
+/* Downloads them (feeds Me8qIndex::Save, me8q_io.cpp:29-64). */
+e8q_status e8q_m8_download(e8q_ctx* ctx, const e8q_m8index* idx, e8q_m8_arrays* host);
+
+/* The compute of Me8qIndex::Build (me8q_build.cpp:43-311) on the device:
+ * rotated centroids, per-row per-run rotation / E8 encode / pack / fits, then
+ * BuildFastLayout.  `centroids` = nlist*padded zero-padded host array;
+ * rank[i] = position of row i in its list, host array.  base (n*dim) and
+ * cluster_ids (n) are host pointers, or device pointers when on_device. */
+e8q_status e8q_m8_encode(e8q_ctx* ctx, e8q_m8index* idx, const float* base, const float* centroids,
+                         const uint32_t* cluster_ids, const uint32_t* rank, int on_device);
+
+/* Me8qIndex::SearchIVFFast for each query (me8q_fast.cpp:380-1108; the
+ * binding's batch entry point SearchIVFFastBatch gives the same results).
+ * per_query_stats: optional host [nq*5] = candidates, survivors, bits_scored,
+ * bits_total, pruned. */
+e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* idx, const e8q_m8_search_args* args,
+                         e8q_m8_stats* stats, uint64_t* per_query_stats);
+
+/* ---- diagnostics used by tests (reference layouts, host pointers) -------- */
+/* One list's FastLayout as the reference holds it (me8q.hpp:153-184): plane
+ * [batches][plane_cols*32 + nrun*160] ([col][lane] bytes), refine
+ * [cnt][refine_stride], gain / vpc [nrun][cnt], rc_n2 [nrun].  sizes3 =
+ * {plane bytes, refine bytes, gain count}; any array pointer may be null. */
+e8q_status e8q_m8_debug_fast(e8q_ctx* ctx, const e8q_m8index* idx, uint64_t list, uint64_t* sizes3,
+                             uint8_t* plane, uint8_t* refine, float* gain, float* vpc, double* rc_n2);
+/* Rotated centroids [nlist*padded] and their squared norms [nlist]. */
+e8q_status e8q_m8_debug_centroids(e8q_ctx* ctx, const e8q_m8index* idx, float* cent_rot, float* cn2);
+/* Per-query intermediates: rotated query [nq*padded], u8 LUT in the
+ * reference's layout [nq*plane_cols*64], {lut_min, lut_delta} [nq*2], probe
+ * lists [nq*nprobe] and their distances [nq*nprobe]. */
+e8q_status e8q_m8_debug_query(e8q_ctx* ctx, const e8q_m8index* idx, const float* queries, uint64_t nq,
+                              uint64_t nprobe, float* rot_q, uint8_t* lut, float* lut_md, uint32_t* probe,
+                              float* pdist);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ME8Q_GPU_H_ */

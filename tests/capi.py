@@ -13,6 +13,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "e8q-artifact_amd", "e8qlib", "libe8q_gpu.so")
 HEADER = os.path.join(ROOT, "include", "e8q_gpu.h")
+HEADERS = [HEADER, os.path.join(ROOT, "include", "me8q_gpu.h")]
 
 vp = ctypes.c_void_p
 u64 = ctypes.c_uint64
@@ -48,9 +49,12 @@ _lib = None
 
 
 def declared_symbols():
-    """Function names declared in include/e8q_gpu.h."""
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:e8q_status|uint64_t|const char\*)\s+(e8q_\w+)\(", text, re.M)))
+    """Function names declared in include/e8q_gpu.h and include/me8q_gpu.h."""
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        names |= set(re.findall(r"^(?:e8q_status|uint64_t|const char\*)\s+(e8q_\w+)\(", text, re.M))
+    return sorted(names)
 
 
 def lib():
@@ -60,10 +64,12 @@ def lib():
             raise RuntimeError("libe8q_gpu.so not built: run __graft_entry__.build()")
         L = ctypes.CDLL(LIB_PATH)
         L.e8q_last_error.restype = ctypes.c_char_p
-        L.e8q_index_device_bytes.restype = u64
+        u64_ret = ("e8q_index_device_bytes", "e8q_m8_device_bytes", "e8q_m8_row_bytes")
         for name in declared_symbols():
             fn = getattr(L, name)
-            if name not in ("e8q_last_error", "e8q_index_device_bytes"):
+            if name in u64_ret:
+                fn.restype = u64
+            elif name != "e8q_last_error":
                 fn.restype = ctypes.c_int
         _lib = L
     return _lib

@@ -3,6 +3,7 @@ header declares (no device calls)."""
 import ctypes
 
 import capi
+import capi_m8
 
 
 def test_library_exports_declared_symbols():
@@ -18,3 +19,10 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(capi.IndexDesc) == 4 * 8 + 4 + 4 + 8 + 7 * 8
     assert ctypes.sizeof(capi.IndexArrays) == 6 * 8
     assert ctypes.sizeof(capi.SearchArgs) == 8 * 4 + 8 + 4 * 5 + 4 + 8 + 8 + 8
+
+
+def test_me8q_struct_sizes():
+    assert ctypes.sizeof(capi_m8.M8Desc) == 4 * 8 + 8 + 6 * 8 + 8
+    assert ctypes.sizeof(capi_m8.M8Arrays) == 3 * 8
+    assert ctypes.sizeof(capi_m8.M8SearchArgs) == 5 * 8 + 4 + 4 + 8 + 8 + 8
+    assert ctypes.sizeof(capi_m8.M8Stats) == 5 * 8 + 2 * 8
