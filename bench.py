@@ -20,6 +20,12 @@ bytes from the kernel's own batch and survivor counters, duration from HIP
 events on the library's stream), and the reference CPU search timed on this
 host on a bounded query sample.
 
+`--workload me8q` measures the mixed-rate ME8Q configuration instead:
+N = 1M, D = 960 with a power-law spectrum (lambda_i ~ i^-1.2) behind a seeded
+random rotation, taken to the PCA domain (fp64 covariance + eigh, descending)
+as the reference's runner does, 4096 lists = seeded random base rows,
+avg_bits = 4 (SAQ plan), 10k queries, k = 100, nprobe grid 32/64/128/256.
+
 One GPU only: --gpus N with N > 1 exits with a message.
 """
 import argparse
@@ -40,15 +46,17 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("e8q", "me8q"), default="e8q")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--dim", type=int, default=128)
-    ap.add_argument("--nlist", type=int, default=1024)
+    ap.add_argument("--dim", type=int, default=None, help="128 (e8q) / 960 (me8q)")
+    ap.add_argument("--nlist", type=int, default=None, help="1024 (e8q) / 4096 (me8q)")
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--bits", type=int, default=4)
-    ap.add_argument("--grid", default="16,32,64,128")
+    ap.add_argument("--grid", default=None, help="16,32,64,128 (e8q) / 32,64,128,256 (me8q)")
+    ap.add_argument("--avg-bits", type=float, default=4.0)
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--modes", default="plain,batched")
     ap.add_argument("--latency-queries", type=int, default=200)
@@ -61,7 +69,12 @@ def parse():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
-    return ap.parse_args()
+    a = ap.parse_args()
+    me = a.workload == "me8q"
+    a.dim = a.dim or (960 if me else 128)
+    a.nlist = a.nlist or (4096 if me else 1024)
+    a.grid = a.grid or ("32,64,128,256" if me else "16,32,64,128")
+    return a
 
 
 def copy_calibration(nbytes):
@@ -173,6 +186,177 @@ def cpu_baseline(idx, queries_np, k, nprobe, gpu_ids, args):
             "rows_identical_to_gpu_plain": same}
 
 
+def make_me8q_data(torch, n, dim, nlist, nq, seed=42):
+    """Power-law spectrum behind a seeded orthogonal rotation, then the PCA
+    domain: base, queries and list centroids projected on the eigenvectors of
+    the fp64 covariance in descending eigenvalue order; variance = the
+    eigenvalues."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    lam = torch.arange(1, dim + 1, dtype=torch.float64, device=dev) ** -1.2
+    rot, _ = np.linalg.qr(np.random.default_rng(seed).standard_normal((dim, dim)))
+    rot = torch.from_numpy(rot).to(dev).float()
+    sd = lam.sqrt().float()
+    base = (torch.randn(n, dim, generator=g, device=dev) * sd) @ rot.T
+    queries = (torch.randn(nq, dim, generator=g, device=dev) * sd) @ rot.T
+    mean = base.double().mean(0)
+    cov = torch.zeros(dim, dim, dtype=torch.float64, device=dev)
+    for s0 in range(0, n, 131072):
+        x = base[s0:s0 + 131072].double() - mean
+        cov += x.T @ x
+    cov /= n
+    ev, vec = np.linalg.eigh(cov.cpu().numpy())
+    order = np.argsort(ev)[::-1]
+    var = ev[order].astype(np.float32)
+    V = torch.from_numpy(vec[:, order].copy()).to(dev)
+    meanf = mean.float()
+    Vf = V.float()
+    for s0 in range(0, n, 131072):
+        base[s0:s0 + 131072] = (base[s0:s0 + 131072] - meanf) @ Vf
+    queries = (queries - meanf) @ Vf
+    pick = torch.randperm(n, generator=g, device=dev)[:nlist]
+    centers = base[pick].clone()
+    cn = (centers * centers).sum(1)
+    cid = torch.empty(n, dtype=torch.int64, device=dev)
+    for s0 in range(0, n, 65536):
+        x = base[s0:s0 + 65536]
+        cid[s0:s0 + 65536] = (cn[None, :] - 2.0 * (x @ centers.T)).argmin(1)
+    return base.contiguous(), centers.contiguous(), cid, queries.contiguous(), var
+
+
+def run_me8q(args):
+    import numpy as np
+    import torch
+    import e8qlib
+
+    grid = [int(x) for x in args.grid.split(",")]
+    t0 = time.perf_counter()
+    base, centers, cid, queries, var = make_me8q_data(torch, args.n, args.dim, args.nlist, args.nq)
+    gt = None if args.profile else ground_truth(torch, base, queries).cpu().numpy()
+    torch.cuda.synchronize()
+    log(f"data + ground truth: {time.perf_counter() - t0:.1f} s")
+    idx = e8qlib.Me8qIndex(args.dim, args.avg_bits, 0)
+    tb = time.perf_counter()
+    idx.build(base.cpu().numpy(), centers.cpu().numpy(), cid.to(torch.int32).cpu().numpy().astype(np.uint32),
+              variance=var)
+    build_s = time.perf_counter() - tb
+    plan = [tuple(p) for p in idx.plan]
+    log(f"build: {build_s:.2f} s, plan {plan}, {idx.bits_per_dim:.3f} bits/dim")
+    del base
+    torch.cuda.empty_cache()
+    nq, k = args.nq, args.k
+    q_dev = queries.contiguous()
+    ids_dev = torch.empty((nq, k), dtype=torch.int32, device="cuda:0")
+    d_dev = torch.empty((nq, k), dtype=torch.float32, device="cuda:0")
+    coded = [(ln, b) for ln, b in plan if b > 0]
+    plane_cols = sum(ln // 8 for ln, _ in coded)
+    rec_bytes = 32 * plane_cols + 160 * len(coded)
+    refine_stride = 8 * len(coded) + sum((b - 1) * ln // 8 for ln, b in coded if b >= 2)
+    rows = []
+    for nprobe in grid:
+        for _ in range(args.warmup):
+            idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr())
+        torch.cuda.synchronize()
+        times, scan_ms, st = [], [], None
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            st = idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr())
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - ts)
+            scan_ms.append(st["device_scan_ms"])
+        ids = ids_dev.cpu().numpy().view(np.uint32)
+        ms = 1000.0 * sum(times) / len(times)
+        row = {"nprobe": nprobe, "ms_per_step": ms, "qps": nq / (ms / 1000.0),
+               "recall": recall_at(gt, ids, 10, k) if gt is not None else -1.0,
+               "recall_10at10": recall_at(gt, ids, 10, 10) if gt is not None else -1.0,
+               "scan_kernel_ms": sum(scan_ms) / len(scan_ms), "batches": st["batches"],
+               "survivors": st["survivors"], "candidates": st["candidates"], "refined": st["refined"],
+               "groups": st["groups"]}
+        algo = row["batches"] * rec_bytes + row["survivors"] * (refine_stride + 4)
+        row["scan_algorithmic_bytes"] = algo
+        row["scan_achieved_gbs"] = algo / (row["scan_kernel_ms"] / 1000.0) / 1e9
+        rows.append(row)
+        log(f"me8q nprobe={nprobe}: {ms:.2f} ms/step, {row['qps']:.0f} QPS, recall 10@{k}={row['recall']:.4f} "
+            f"(10@10 {row['recall_10at10']:.4f}), scan kernel {row['scan_kernel_ms']:.2f} ms = "
+            f"{row['scan_achieved_gbs']:.0f} GB/s, survivors {row['survivors']}/{row['candidates']}, "
+            f"refined {row['refined']} in {row['groups']} groups")
+    if args.profile:
+        return
+    value = qps_at_recall(rows, args.target)
+    op = next((r for r in rows if r["recall"] >= args.target), rows[-1])
+    lat = []
+    qh = queries[:args.latency_queries].cpu().numpy()
+    for i in range(qh.shape[0]):
+        ts = time.perf_counter()
+        idx.search(qh[i:i + 1], k, op["nprobe"])
+        lat.append((time.perf_counter() - ts) * 1e6)
+    lat.sort()
+    cpu = None
+    if not args.no_cpu:
+        idx.search_device(q_dev.data_ptr(), nq, k, op["nprobe"], ids_dev.data_ptr(), d_dev.data_ptr())
+        torch.cuda.synchronize()
+        cpu = cpu_baseline_me8q(idx, queries.cpu().numpy(), k, op["nprobe"],
+                                ids_dev.cpu().numpy().view(np.uint32), args)
+        log(f"cpu baseline: {cpu}")
+    achieved = op["scan_achieved_gbs"]
+    line = {
+        "metric": "QPS at recall@10>=0.95 (ME8Q mixed-rate plain GPU search, 1 GPU); p50/p99 latency",
+        "value": round(value, 1) if value == value else None,
+        "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * nq / value, 3) if value == value and value > 0 else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 LUT / i32 coarse sums, fp32 scores, fp64 bounds",
+        "data": "synthetic power-law spectrum (lambda_i ~ i^-1.2), seeded rotation, PCA domain, generated on the GPU",
+        "config": {"workload": f"ME8Q mixed-rate scan: N={args.n}, D={args.dim} PCA-domain power law, "
+                               f"{args.nlist} lists, avg_bits={args.avg_bits}, {nq} queries, k={k}, "
+                               f"nprobe {args.grid}",
+                   "plan": plan, "bits_per_dim": idx.bits_per_dim, "operating_nprobe": op["nprobe"],
+                   "recall_at_operating_point": op["recall"],
+                   "recall_definition": f"recall@10 = fraction of the exact top-10 (fp64) among the k={k} results; "
+                                        "10@10 per grid row"},
+        "latency_us": {"p50": round(lat[len(lat) // 2], 1), "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 1),
+                       "what": "single-query search() calls"},
+        "grid": [{kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items()} for r in rows],
+        "build_seconds": round(build_s, 2),
+        "roofline": {"bound": "hbm", "kernel": "M8ScanKernel", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "basis": f"per launch: {op['batches']} scan records x {rec_bytes} B + {op['survivors']} "
+                              f"survivors x {refine_stride + 4} B, / mean kernel time {op['scan_kernel_ms']:.3f} ms"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_me8q(idx, queries_np, k, nprobe, gpu_ids, args):
+    """Reference e8qlib ME8Q CPU search (oracle/_ref/ref_bench, SearchIVFFastBatch) on a bounded sample."""
+    import numpy as np
+    tool = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    if not os.path.exists(tool):
+        return None
+    sample = min(args.cpu_sample, queries_np.shape[0])
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "index.me8q")
+        idx.save(path)
+        qf = os.path.join(tmp, "q.f32")
+        queries_np[:sample].astype(np.float32).tofile(qf)
+        out = os.path.join(tmp, "ids.u32")
+        cmd = [tool, path, qf, str(sample), str(k), str(nprobe), str(args.cpu_threads), str(args.cpu_seconds), out]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr)
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        ids = np.fromfile(out, dtype=np.uint32).reshape(sample, k)
+    same = float((ids == gpu_ids[:sample]).all(axis=1).mean())
+    return {"value": round(res["qps_median"], 2), "unit": "queries/s", "cores": res["threads"], "kind": "reference",
+            "sample": f"{sample} of the {queries_np.shape[0]} queries, nprobe={nprobe}, k={k}, "
+                      f"{res.get('rounds', 1)} rounds after 3 warm-ups, median round QPS "
+                      f"({res['seconds']:.1f} s of CPU search)",
+            "rows_identical_to_gpu_plain": same}
+
+
 def main():
     args = parse()
     if args.copy_calib:
@@ -182,6 +366,9 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "me8q":
+        run_me8q(args)
+        return
     import numpy as np
     import torch
     import e8qlib

@@ -125,11 +125,12 @@ class PyMe8qIndex {
             st.candidates > 0 ? static_cast<double>(st.survivors) / static_cast<double>(st.candidates) : 0.0;
         s["device_scan_ms"] = st.device_scan_ms;
         s["device_total_ms"] = st.device_total_ms;
+        s["batches"] = st.batches;
         return py::make_tuple(ids, dists, s);
     }
 
-    void search_device(uintptr_t q_ptr, size_t nq, size_t k, size_t nprobe, uintptr_t ids_ptr, uintptr_t dists_ptr,
-                       py::object nprobe_per_query) {
+    py::dict search_device(uintptr_t q_ptr, size_t nq, size_t k, size_t nprobe, uintptr_t ids_ptr,
+                           uintptr_t dists_ptr, py::object nprobe_per_query) {
         if (!built_ || !index_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
         if (k == 0) throw std::invalid_argument("k must be positive");
         if (nprobe == 0) throw std::invalid_argument("nprobe must be positive");
@@ -141,10 +142,25 @@ class PyMe8qIndex {
                 throw std::invalid_argument("nprobe_per_query length must match number of queries");
             npq_ptr = npq.data();
         }
-        py::gil_scoped_release nogil;
-        index_->SearchDevice(reinterpret_cast<const float*>(q_ptr), nq, k, nprobe, npq_ptr,
-                             reinterpret_cast<uint32_t*>(ids_ptr), reinterpret_cast<float*>(dists_ptr), nullptr, true,
-                             true);
+        e8qlib::me8q::SearchStats st;
+        {
+            py::gil_scoped_release nogil;
+            index_->SearchDevice(reinterpret_cast<const float*>(q_ptr), nq, k, nprobe, npq_ptr,
+                                 reinterpret_cast<uint32_t*>(ids_ptr), reinterpret_cast<float*>(dists_ptr), &st, true,
+                                 true);
+        }
+        py::dict s;
+        s["candidates"] = st.candidates;
+        s["survivors"] = st.survivors;
+        s["bits_scored"] = st.bits_scored;
+        s["bits_total"] = st.bits_total;
+        s["pruned"] = st.pruned;
+        s["batches"] = st.batches;
+        s["refined"] = st.refined;
+        s["groups"] = st.groups;
+        s["device_scan_ms"] = st.device_scan_ms;
+        s["device_total_ms"] = st.device_total_ms;
+        return s;
     }
 
     void save(const std::string& path) const {

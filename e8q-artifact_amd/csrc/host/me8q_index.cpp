@@ -481,6 +481,9 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
         stats->clusters += nq * std::min(nprobe, nlist_);
         stats->device_scan_ms += st.scan_ms;
         stats->device_total_ms += st.total_ms;
+        stats->batches += st.batches;
+        stats->refined += st.refined;
+        stats->groups += st.groups;
     }
 }
 

@@ -75,6 +75,9 @@ struct SearchStats {
     size_t pruned = 0;
     double device_scan_ms = 0.0;
     double device_total_ms = 0.0;
+    size_t batches = 0;   // 32-vector scan records read
+    size_t refined = 0;   // candidates that entered the exact cascade
+    size_t groups = 0;    // cascade groups (up to four candidates)
 };
 
 class Me8qIndex {

@@ -48,9 +48,11 @@ struct M8ScanArgs {
     float z;
     uint32_t* out_ids;
     float* out_d;
-    uint64_t* qstats;   // [nq][5] candidates, survivors, bits_scored, bits_total, pruned
+    uint64_t* qstats;   // [nq][8] candidates, survivors, bits_scored, bits_total, pruned, batches read,
+                        //         candidates refined, refine groups
+    uint32_t lev_floats;   // total size of the runs' level tables
 };
-size_t M8ScanLds(const M8View& v, uint32_t k);
+size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats);
 hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a);
 
 }  // namespace e8qdev

@@ -326,7 +326,7 @@ e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search
         if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
         const M8View& v = x->view;
         const uint64_t nq = a->nq;
-        if (stats) *stats = e8q_m8_stats{0, 0, 0, 0, 0, 0.0, 0.0};
+        if (stats) *stats = e8q_m8_stats{0, 0, 0, 0, 0, 0.0, 0.0, 0, 0, 0};
         if (nq == 0) return;
         uint32_t maxp = 0;
         if (a->nprobe_per_query) {
@@ -362,7 +362,7 @@ e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search
             cv.add(&oid, nq * a->k);
             cv.add(&od, nq * a->k);
         }
-        cv.add(&qst, nq * 5);
+        cv.add(&qst, nq * 8);
         ctx->Take(cv);
         const float* q = a->queries;
         if (!a->queries_on_device) {
@@ -409,6 +409,9 @@ e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search
         sa.out_ids = a->outputs_on_device ? a->out_ids : oid;
         sa.out_d = a->outputs_on_device ? a->out_dists : od;
         sa.qstats = qst;
+        sa.lev_floats = 0;
+        for (const auto& r : x->runs) sa.lev_floats += 2u << r.rate;
+        if (sa.lev_floats * 4u > 48u * 1024u) sa.lev_floats = 0;   // large tables stay in global memory
         Check(hipEventRecord(ctx->ev[1], st), "event");
         Check(LaunchM8Scan(st, sa), "scan");
         Check(hipEventRecord(ctx->ev[2], st), "event");
@@ -416,17 +419,21 @@ e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search
             Check(hipMemcpyAsync(a->out_ids, oid, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
             Check(hipMemcpyAsync(a->out_dists, od, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");
         }
-        std::vector<uint64_t> hs(nq * 5);
-        Check(hipMemcpyAsync(hs.data(), qst, nq * 5 * 8, hipMemcpyDeviceToHost, st), "download stats");
+        std::vector<uint64_t> hs(nq * 8);
+        Check(hipMemcpyAsync(hs.data(), qst, nq * 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
         Check(hipStreamSynchronize(st), "search");
-        if (per_query_stats) std::memcpy(per_query_stats, hs.data(), hs.size() * 8);
+        if (per_query_stats)
+            for (uint64_t i = 0; i < nq; ++i) std::memcpy(per_query_stats + i * 5, hs.data() + i * 8, 5 * 8);
         if (stats) {
             for (uint64_t i = 0; i < nq; ++i) {
-                stats->candidates += hs[i * 5];
-                stats->survivors += hs[i * 5 + 1];
-                stats->bits_scored += hs[i * 5 + 2];
-                stats->bits_total += hs[i * 5 + 3];
-                stats->pruned += hs[i * 5 + 4];
+                stats->candidates += hs[i * 8];
+                stats->survivors += hs[i * 8 + 1];
+                stats->bits_scored += hs[i * 8 + 2];
+                stats->bits_total += hs[i * 8 + 3];
+                stats->pruned += hs[i * 8 + 4];
+                stats->batches += hs[i * 8 + 5];
+                stats->refined += hs[i * 8 + 6];
+                stats->groups += hs[i * 8 + 7];
             }
             float ms = 0.0F;
             Check(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "event time");

@@ -158,9 +158,9 @@ __global__ __launch_bounds__(256) void M8RunDistKernel(M8View v, const float* __
 // ---------------------------------------------------------------------------
 // Scan: LDS layout of one wave (one query).
 struct M8ScanLayout {
-    uint32_t q, lut, td, ti, acc, fdc, fpool, fqa, fsc, ftr, tsc, ttr, ssc, str, sS, sord, smean, rq, rqf, ssq,
-        stage, total;
-    __host__ __device__ M8ScanLayout(uint32_t P, uint32_t PC, uint32_t R, uint32_t K) {
+    uint32_t q, lut, td, ti, acc, fdc, fpool, fqa, fsc, ftr, tsc, ttr, ssc, str, sord, smean, rq, rqf, ssq,
+        stage, srend, slev, sS32, plist, total;
+    __host__ __device__ M8ScanLayout(uint32_t P, uint32_t PC, uint32_t R, uint32_t K, uint32_t RS, uint32_t LV) {
         uint32_t o = 0;
         auto take = [&](uint32_t bytes, uint32_t align) {
             o = (o + align - 1) & ~(align - 1);
@@ -183,12 +183,15 @@ struct M8ScanLayout {
         ttr = take(R1 * 8 * 8, 16);
         ssc = take(4 * R1 * 8, 16);
         str = take(4 * R1 * 8, 16);
-        sS = take(4 * R1 * 4, 16);
         sord = take(R1 * 4, 16);
         smean = take(R1 * 8, 16);
         rq = take(R1 * 3 * 8, 16);      // lut_min_x, qerr, score_max
         rqf = take(R1 * 2 * 4, 16);     // float lut_min_x, float qerr
         ssq = take(R1 * 8, 16);         // sqrt(d2_r) of the current list
+        srend = take((R1 + 1) * 4, 16); // last plane column + 1 of each run
+        slev = take(LV * 4, 16);        // the runs' level tables, [2][2^rate] each
+        sS32 = take(32 * R1 * 4, 16);   // exact run scores of a batch's passing lanes
+        plist = take(32, 16);           // those lanes, in order
         stage = take(4 * 64, 16);       // staged candidates: lane,row,pool,qadd,dist,died,bits
         total = (o + 15) & ~15u;
     }
@@ -200,6 +203,83 @@ struct M8Stage {         // one staged candidate
     uint64_t bits;
     uint64_t pad2[2];
 };
+
+// One 8-lane group's half of a run score: component `ci` of the blocks of
+// parity `half`, as one FMA chain in block order (the reference keeps even
+// blocks in acc and odd blocks in acc2, me8q_score_avx2.cpp:404-413).  The
+// digit of (block, ci) = its refine field bits (T = rate-1 per digit, digit
+// 7's LSB the parity of digits 0..6's, the branch in the field's top bit)
+// under the plane's top bit; at rate 1 the plane byte is the whole code.
+template <int T>
+__device__ __forceinline__ float M8HalfChainT(const uint8_t* __restrict__ pb, const uint8_t* __restrict__ fr,
+                                              const float* __restrict__ lev, const float* __restrict__ qb,
+                                              uint32_t nblk, uint32_t half, uint32_t ci) {
+    constexpr uint32_t rate = T + 1;
+    uint64_t pml = 0;
+    uint32_t pmh = 0;
+#pragma unroll
+    for (uint32_t jj = 0; jj < 7; ++jj) {
+        if (jj * T < 64) pml |= uint64_t{1} << (jj * T);
+        else pmh |= 1u << (jj * T - 64);
+    }
+    float acc = 0.0F;
+#pragma unroll 2
+    for (uint32_t blk = half; blk < nblk; blk += 2) {
+        const uint32_t pbyte = pb[blk];
+        const uint32_t top = (pbyte >> (7 - ci)) & 1u;
+        uint32_t dgt, br;
+        if constexpr (T == 0) {
+            dgt = ci < 7 ? top : (__popc(pbyte >> 1) & 1u);
+            br = pbyte & 1u;
+        } else {
+            const uint8_t* fb = fr + blk * T;
+            uint64_t flo = 0;
+            uint32_t fhi = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < static_cast<uint32_t>(T); ++b) {
+                const uint32_t byte = fb[b];
+                if (b < 8) flo |= static_cast<uint64_t>(byte) << (8 * b);
+                else fhi |= byte << (8 * (b - 8));
+            }
+            auto field = [&](uint32_t pos, uint32_t nb) -> uint32_t {
+                const uint64_t m = (uint64_t{1} << nb) - 1;
+                if (pos + nb <= 64) return static_cast<uint32_t>((flo >> pos) & m);
+                if (pos >= 64) return static_cast<uint32_t>((fhi >> (pos - 64)) & m);
+                return static_cast<uint32_t>(((flo >> pos) | (static_cast<uint64_t>(fhi) << (64 - pos))) & m);
+            };
+            uint32_t lo;
+            if (ci < 7) {
+                lo = field(T * ci, T);
+            } else {
+                const uint32_t par = (__popcll(flo & pml) + __popc(fhi & pmh)) & 1u;
+                lo = (field(7 * T, T - 1) << 1) | par;
+            }
+            br = field(8 * T - 1, 1);
+            dgt = lo | (top << T);
+        }
+        acc = fmaf(lev[(br << rate) + dgt], qb[blk * 8], acc);
+    }
+    return acc;
+}
+
+__device__ float M8HalfChain(uint32_t rate, const uint8_t* pb, const uint8_t* fr, const float* lev, const float* qb,
+                             uint32_t nblk, uint32_t half, uint32_t ci) {
+    switch (rate) {
+        case 1: return M8HalfChainT<0>(pb, fr, lev, qb, nblk, half, ci);
+        case 2: return M8HalfChainT<1>(pb, fr, lev, qb, nblk, half, ci);
+        case 3: return M8HalfChainT<2>(pb, fr, lev, qb, nblk, half, ci);
+        case 4: return M8HalfChainT<3>(pb, fr, lev, qb, nblk, half, ci);
+        case 5: return M8HalfChainT<4>(pb, fr, lev, qb, nblk, half, ci);
+        case 6: return M8HalfChainT<5>(pb, fr, lev, qb, nblk, half, ci);
+        case 7: return M8HalfChainT<6>(pb, fr, lev, qb, nblk, half, ci);
+        case 8: return M8HalfChainT<7>(pb, fr, lev, qb, nblk, half, ci);
+        case 9: return M8HalfChainT<8>(pb, fr, lev, qb, nblk, half, ci);
+        case 10: return M8HalfChainT<9>(pb, fr, lev, qb, nblk, half, ci);
+        case 11: return M8HalfChainT<10>(pb, fr, lev, qb, nblk, half, ci);
+        case 12: return M8HalfChainT<11>(pb, fr, lev, qb, nblk, half, ci);
+        default: return M8HalfChainT<12>(pb, fr, lev, qb, nblk, half, ci);
+    }
+}
 
 // Inserts (d, id) into the sorted list (td, ti) of `cnt` entries (<= K):
 // a full list rejects d unless it is below the worst; a new item goes before
@@ -244,7 +324,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const M8View& v = a.v;
     const uint32_t P = static_cast<uint32_t>(v.padded), PC = v.plane_cols, R = v.nrun, K = a.k;
-    const M8ScanLayout L(P, PC, R, K);
+    const M8ScanLayout L(P, PC, R, K, v.rstride, a.lev_floats);
     float* q = reinterpret_cast<float*>(smem + L.q);
     uint8_t* lut = smem + L.lut;
     float* td = reinterpret_cast<float*>(smem + L.td);
@@ -259,13 +339,27 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
     double* ttr = reinterpret_cast<double*>(smem + L.ttr);
     double* ssc = reinterpret_cast<double*>(smem + L.ssc);
     double* str = reinterpret_cast<double*>(smem + L.str);
-    float* sS = reinterpret_cast<float*>(smem + L.sS);
     uint32_t* sord = reinterpret_cast<uint32_t*>(smem + L.sord);
     double* smean = reinterpret_cast<double*>(smem + L.smean);
     double* rq = reinterpret_cast<double*>(smem + L.rq);
     float* rqf = reinterpret_cast<float*>(smem + L.rqf);
     double* ssq = reinterpret_cast<double*>(smem + L.ssq);
     M8Stage* stg = reinterpret_cast<M8Stage*>(smem + L.stage);
+    uint32_t* srend = reinterpret_cast<uint32_t*>(smem + L.srend);
+    float* slev = reinterpret_cast<float*>(smem + L.slev);
+    float* sS32 = reinterpret_cast<float*>(smem + L.sS32);
+    uint8_t* plist = smem + L.plist;
+    // level tables of every run, copied once per workgroup; run r's table
+    // starts at slev + lev_off(r)
+    const bool lev_lds = a.lev_floats > 0;   // else the tables are read from global memory
+    if (lev_lds) {
+        uint32_t o = 0;
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t nq2 = 2u << v.runs[r].rate;
+            for (uint32_t e = threadIdx.x; e < nq2; e += 64) slev[o + e] = v.runs[r].lev[e];
+            o += nq2;
+        }
+    }
     const int lane = threadIdx.x;
     const float finf = __builtin_inff();
     const double dinf = __builtin_inf();
@@ -276,6 +370,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
         for (uint32_t d = lane; d < P; d += 64) q[d] = a.rotq[qi * P + d];
         for (uint32_t e = lane; e < PC * 8; e += 64)
             reinterpret_cast<uint32_t*>(lut)[e] = reinterpret_cast<const uint32_t*>(a.lut + qi * PC * 32)[e];
+        for (uint32_t r = lane; r <= R; r += 64) srend[r] = r < R ? v.runs[r].blk0 + v.runs[r].nblk : 0xFFFFFFFFu;
         for (uint32_t r = lane; r < R; r += 64) {
             const double* o = a.rc + (qi * R + r) * 4;
             rq[r * 3 + 0] = o[1];
@@ -287,7 +382,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
         const float lut_delta = a.qc[qi * 4 + 1];
         const uint64_t bits_row = static_cast<uint64_t>(a.qc[qi * 4 + 3]);
         uint32_t cnt = 0;
-        uint64_t n_cand = 0, n_surv = 0, n_bits = 0, n_btot = 0;
+        uint64_t n_cand = 0, n_surv = 0, n_bits = 0, n_btot = 0, n_batch = 0, n_staged = 0, n_flush = 0;
         __syncthreads();
         const uint64_t np = min<uint64_t>(a.nprobe_q ? a.nprobe_q[qi] : a.nprobe_all, v.nlist);
         for (uint64_t p = 0; p < np; ++p) {
@@ -306,27 +401,44 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
             for (uint32_t r = lane; r < R; r += 64) ssq[r] = a.sqd2[(qi * a.maxp + p) * R + r];
             const uint64_t nb = (cntc + 31) >> 5;
             const uint64_t bat0 = v.batch_begin[c];
+            n_batch += nb;
             for (uint64_t b0 = 0; b0 < nb; b0 += 2) {
                 {   // integer coarse sums of batches b0 (lanes 0..31) and b0+1 (32..63)
                     const uint64_t b = b0 + (lane >> 5);
                     const uint32_t l = lane & 31;
                     if (b < nb) {
+                        // runs occupy consecutive plane columns, 8 per word pair: four
+                        // word pairs are loaded before any is summed
                         const uint8_t* rowp = v.plane + (bat0 + b) * v.rec + l * PC;
-                        for (uint32_t r = 0; r < R; ++r) {
-                            const M8Run& ru = v.runs[r];
-                            int32_t s = 0;
-                            for (uint32_t j = 0; j < ru.nblk; j += 8) {
-                                const uint2 w = *reinterpret_cast<const uint2*>(rowp + ru.blk0 + j);
-                                const uint8_t* lc = lut + (ru.blk0 + j) * 32;
+                        uint32_t r = 0, rend = srend[0];
+                        int32_t s = 0;
+                        for (uint32_t j0 = 0; j0 < PC; j0 += 32) {
+                            uint2 w[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const uint32_t col = j0 + 8 * u;
+                                w[u] = col < PC ? *reinterpret_cast<const uint2*>(rowp + col) : uint2{0u, 0u};
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const uint32_t col = j0 + 8 * u;
+                                if (col >= PC) break;
+                                if (col == rend) {
+                                    sacc[r * 64 + lane] = s;
+                                    s = 0;
+                                    ++r;
+                                    rend = srend[r];
+                                }
+                                const uint8_t* lc = lut + col * 32;
 #pragma unroll
                                 for (int k2 = 0; k2 < 4; ++k2) {
-                                    const uint32_t x0 = (w.x >> (8 * k2)) & 0xFFu, x1 = (w.y >> (8 * k2)) & 0xFFu;
+                                    const uint32_t x0 = (w[u].x >> (8 * k2)) & 0xFFu, x1 = (w[u].y >> (8 * k2)) & 0xFFu;
                                     s += lc[k2 * 32 + (x0 & 15)] + lc[k2 * 32 + 16 + (x0 >> 4)];
                                     s += lc[(4 + k2) * 32 + (x1 & 15)] + lc[(4 + k2) * 32 + 16 + (x1 >> 4)];
                                 }
                             }
-                            sacc[r * 64 + lane] = s;
                         }
+                        sacc[r * 64 + lane] = s;
                     }
                 }
                 __syncthreads();
@@ -376,47 +488,93 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                     n_btot += static_cast<uint64_t>(cntb) * bits_row;
                     const uint32_t keep = static_cast<uint32_t>(__ballot(kept) >> (32 * hh));
                     __syncthreads();
+                    // The lane test of a kept lane against the current k-th distance:
+                    // fp32 filter values for the lanes the filter covered, the fp64
+                    // twin for the rest (their slack depends on the threshold).
+                    auto lane_test = [&](double distk_d, double& pool2, double& qadd) -> bool {
+                        double dcq;
+                        if (l < covered) {
+                            dcq = fdc[lane];
+                            pool2 = fpool[lane];
+                            qadd = fqa[lane];
+                        } else {   // fp64 twin of the filter for the lanes it does not cover
+                            dcq = d2;
+                            pool2 = 0.0;
+                            qadd = 0.0;
+                            const double sqdk = distk_d < dinf ? sqrt(fmax(distk_d, 0.0)) : dinf;
+                            const uint32_t tl = l - covered;
+                            for (uint32_t r = 0; r < R; ++r) {
+                                const uint8_t* bp = banks + r * kM8BankBytes;
+                                const float* scl = reinterpret_cast<const float*>(bp + 128);
+                                const double scr = fma(static_cast<double>(lut_delta),
+                                                       static_cast<double>(sacc[r * 64 + lane]), rq[r * 3]);
+                                const double bq = fma(static_cast<double>(bp[l]), static_cast<double>(scl[1]),
+                                                      static_cast<double>(scl[0]));
+                                const double wq = fma(static_cast<double>(bp[32 + l]), static_cast<double>(scl[3]),
+                                                      static_cast<double>(scl[2]));
+                                const double pq = fma(static_cast<double>(bp[64 + l]), static_cast<double>(scl[5]),
+                                                      static_cast<double>(scl[4]));
+                                dcq += fma(-wq, scr, bq);
+                                const double sa = ssq[r];
+                                const double rh = sqdk < sa ? sqdk : sa;
+                                const double tr = pq * rh;
+                                tsc[r * 8 + tl] = scr;
+                                ttr[r * 8 + tl] = tr;
+                                pool2 = fma(tr, tr, pool2);
+                                qadd = fma(fabs(wq), rq[r * 3 + 1], qadd);
+                                qadd = fma(0.5 * static_cast<double>(scl[3]), rq[r * 3 + 2] + rq[r * 3 + 1], qadd);
+                            }
+                        }
+                        return !(dcq - fma(dz, sqrt(pool2), qadd) >= distk_d);
+                    };
+                    // Lanes passing now: the threshold only falls while this batch is
+                    // refined and the test only gets stricter as it falls, so every
+                    // lane staged below is among them.  Their exact run scores are
+                    // computed here at once (they do not depend on the threshold):
+                    // per (lane, run) pair 16 lanes, the even-block and the odd-block
+                    // FMA chains on two 8-lane groups, added and summed in the AVX order.
+                    uint32_t p0;
+                    {
+                        bool pass0 = false;
+                        if (mine && ((keep >> l) & 1u)) {
+                            double pl, qd;
+                            pass0 = lane_test(cnt < K ? dinf : static_cast<double>(td[K - 1]), pl, qd);
+                        }
+                        p0 = static_cast<uint32_t>(__ballot(pass0) >> (32 * hh));
+                        if (pass0) plist[__popc(p0 & ((1u << l) - 1u))] = static_cast<uint8_t>(l);
+                        __syncthreads();
+                        const uint32_t np0 = __popc(p0);
+                        const uint32_t npairs = R * np0;
+                        const uint32_t half = (lane >> 3) & 1u, ci = lane & 7u;
+                        for (uint32_t t0 = 0; t0 < npairs; t0 += 4) {
+                            const uint32_t pair = t0 + (static_cast<uint32_t>(lane) >> 4);
+                            float acc = 0.0F;
+                            uint32_t lj = 0, r = 0;
+                            if (pair < npairs) {
+                                r = pair / np0;
+                                lj = plist[pair - r * np0];
+                                const M8Run& ru = v.runs[r];
+                                uint32_t lo_off = 0;
+                                for (uint32_t r2 = 0; r2 < r; ++r2) lo_off += 2u << v.runs[r2].rate;
+                                const float* lev = lev_lds ? slev + lo_off : ru.lev;
+                                const uint8_t* pb = rec + lj * PC + ru.blk0;
+                                const uint8_t* fr = v.refine + (lb + bb * 32 + lj) * v.rstride + ru.ref_off;
+                                acc = M8HalfChain(ru.rate, pb, fr, lev, q + ru.dim_off + ci, ru.nblk, half, ci);
+                            }
+                            const float other = __shfl_xor(acc, 8);
+                            const float sc8 = Hsum8Lanes(acc + other);
+                            if (ci == 0 && half == 0 && pair < npairs) sS32[lj * R + r] = sc8;
+                        }
+                        __syncthreads();
+                    }
                     uint32_t nst = 0, pos = 0;
                     for (;;) {
-                        const uint32_t rem = pos >= 32 ? 0u : keep & ~((1u << pos) - 1u);
+                        const uint32_t rem = pos >= 32 ? 0u : keep & p0 & ~((1u << pos) - 1u);
                         if (rem == 0) break;
                         const double distk_d = cnt < K ? dinf : static_cast<double>(td[K - 1]);
                         bool pass = false;
                         double pool2 = 0.0, qadd = 0.0;
-                        if (mine && ((rem >> l) & 1u)) {
-                            double dcq;
-                            if (l < covered) {
-                                dcq = fdc[lane];
-                                pool2 = fpool[lane];
-                                qadd = fqa[lane];
-                            } else {   // fp64 twin of the filter for the lanes it does not cover
-                                dcq = d2;
-                                const double sqdk = distk_d < dinf ? sqrt(fmax(distk_d, 0.0)) : dinf;
-                                const uint32_t tl = l - covered;
-                                for (uint32_t r = 0; r < R; ++r) {
-                                    const uint8_t* bp = banks + r * kM8BankBytes;
-                                    const float* scl = reinterpret_cast<const float*>(bp + 128);
-                                    const double scr = fma(static_cast<double>(lut_delta),
-                                                           static_cast<double>(sacc[r * 64 + lane]), rq[r * 3]);
-                                    const double bq = fma(static_cast<double>(bp[l]), static_cast<double>(scl[1]),
-                                                          static_cast<double>(scl[0]));
-                                    const double wq = fma(static_cast<double>(bp[32 + l]), static_cast<double>(scl[3]),
-                                                          static_cast<double>(scl[2]));
-                                    const double pq = fma(static_cast<double>(bp[64 + l]), static_cast<double>(scl[5]),
-                                                          static_cast<double>(scl[4]));
-                                    dcq += fma(-wq, scr, bq);
-                                    const double sa = ssq[r];
-                                    const double rh = sqdk < sa ? sqdk : sa;
-                                    const double tr = pq * rh;
-                                    tsc[r * 8 + tl] = scr;
-                                    ttr[r * 8 + tl] = tr;
-                                    pool2 = fma(tr, tr, pool2);
-                                    qadd = fma(fabs(wq), rq[r * 3 + 1], qadd);
-                                    qadd = fma(0.5 * static_cast<double>(scl[3]), rq[r * 3 + 2] + rq[r * 3 + 1], qadd);
-                                }
-                            }
-                            pass = !(dcq - fma(dz, sqrt(pool2), qadd) >= distk_d);
-                        }
+                        if (mine && ((rem >> l) & 1u)) pass = lane_test(distk_d, pool2, qadd);
                         const uint32_t pm = static_cast<uint32_t>(__ballot(pass) >> (32 * hh));
                         const uint32_t need = 4 - nst;
                         uint32_t sel = 0, m = pm;
@@ -443,6 +601,8 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                         if (full || (nst > 0 && pm == sel)) {
                             // refine the staged candidates (flush4)
                             const uint32_t n = nst;
+                            n_staged += n;
+                            ++n_flush;
                             if (lane == 0) {
                                 for (uint32_t r = 0; r < R; ++r) sord[r] = r;
                                 if (R > 1) {
@@ -463,49 +623,6 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                                         sord[j] = x;
                                     }
                                 }
-                            }
-                            // exact run scores of every (candidate, run): 8 lanes each
-                            const uint32_t pairs = n * R;
-                            const uint32_t g = lane >> 3, ci = lane & 7;
-                            for (uint32_t base = 0; base < pairs; base += 8) {
-                                const uint32_t pidx = base + g;
-                                float acc = 0.0F, acc2 = 0.0F;
-                                if (pidx < pairs) {
-                                    const uint32_t j = pidx / R, r = pidx % R;
-                                    const M8Run& ru = v.runs[r];
-                                    const M8Stage& s = stg[j];
-                                    const uint8_t* pbt = rec + s.lane * PC + ru.blk0;
-                                    const uint8_t* rr = v.refine + (lb + s.row) * v.rstride + ru.ref_off;
-                                    const uint32_t rate = ru.rate, T = rate - 1;
-                                    const float* qb = q + ru.dim_off + ci;
-                                    for (uint32_t blk = 0; blk < ru.nblk; ++blk) {
-                                        const uint32_t pbyte = pbt[blk];
-                                        const uint32_t top = (pbyte >> (7 - ci)) & 1u;
-                                        uint32_t dgt, br;
-                                        if (rate == 1) {
-                                            dgt = ci < 7 ? top : (__popc(pbyte >> 1) & 1u);
-                                            br = pbyte & 1u;
-                                        } else {
-                                            const uint8_t* fb = rr + blk * T;
-                                            uint32_t lo;
-                                            if (ci < 7) {
-                                                lo = M8Bits(fb, T * ci, T);
-                                            } else {
-                                                uint32_t par = 0;
-                                                for (uint32_t jj = 0; jj < 7; ++jj) par ^= M8Bits(fb, T * jj, 1);
-                                                lo = (M8Bits(fb, 7 * T, T - 1) << 1) | par;
-                                            }
-                                            br = M8Bits(fb, 8 * T - 1, 1);
-                                            dgt = lo | (top << T);
-                                        }
-                                        const float lv = ru.lev[(br << rate) + dgt];
-                                        const float qv = qb[blk * 8];
-                                        if (blk & 1) acc2 = fmaf(lv, qv, acc2);
-                                        else acc = fmaf(lv, qv, acc);
-                                    }
-                                }
-                                const float sc8 = Hsum8Lanes(acc + acc2);
-                                if (ci == 0 && pidx < pairs) sS[pidx] = sc8;
                             }
                             __syncthreads();
                             const double distk_c = cnt < K ? dinf : static_cast<double>(td[K - 1]);
@@ -530,7 +647,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                                 uint64_t bs = 0;
                                 for (uint32_t oi = 0; oi < R && live; ++oi) {
                                     const uint32_t r = sord[oi];
-                                    const double se = sS[j * R + r];
+                                    const double se = sS32[s.lane * R + r];
                                     const double coef = f4[r * 4];
                                     if (R == 1) {
                                         dist = fma(-(2.0 * coef), se, dist);
@@ -556,15 +673,11 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                                 M8Push(td, ti, cnt, K, static_cast<float>(s.dist), v.ids[lb + s.row]);
                             }
                             __syncthreads();
-                            pos = sel ? 32 - __clz(sel) : 32;
-                            if (!full) pos = 32;
+                            pos = full ? 32 - __clz(sel) : 32;
                             nst = 0;
                         } else {
                             break;
                         }
-                    }
-                    if (nst > 0) {
-                        // unreachable: a partial group is refined inside the loop
                     }
                     __syncthreads();
                 }
@@ -575,12 +688,15 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
             a.out_d[qi * K + i] = i < cnt ? td[i] : finf;
         }
         if (lane == 0 && a.qstats) {
-            uint64_t* o = a.qstats + qi * 5;
+            uint64_t* o = a.qstats + qi * 8;
             o[0] = n_cand;
             o[1] = n_surv;
             o[2] = n_bits;
             o[3] = n_btot;
             o[4] = n_cand - n_surv;
+            o[5] = n_batch;
+            o[6] = n_staged;
+            o[7] = n_flush;
         }
         __syncthreads();
     }
@@ -614,13 +730,13 @@ hipError_t LaunchM8RunDist(hipStream_t st, const M8View& v, const float* rotq, c
     return hipGetLastError();
 }
 
-size_t M8ScanLds(const M8View& v, uint32_t k) {
-    return M8ScanLayout(static_cast<uint32_t>(v.padded), v.plane_cols, v.nrun, k).total;
+size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats) {
+    return M8ScanLayout(static_cast<uint32_t>(v.padded), v.plane_cols, v.nrun, k, v.rstride, lev_floats).total;
 }
 
 hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a) {
     if (a.nq == 0) return hipSuccess;
-    const size_t lds = M8ScanLds(a.v, a.k);
+    const size_t lds = M8ScanLds(a.v, a.k, a.lev_floats);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&M8ScanKernel),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));

@@ -65,6 +65,9 @@ typedef struct {
     uint64_t candidates, survivors, bits_scored, bits_total, pruned;   /* me8q::SearchStats (me8q.hpp:54-69) */
     double scan_ms;    /* scan kernel, HIP events on the context stream */
     double total_ms;   /* device time of the whole search */
+    uint64_t batches;  /* 32-vector scan records read (plane + banks) */
+    uint64_t refined;  /* candidates that entered the exact cascade */
+    uint64_t groups;   /* cascade groups (up to four candidates each) */
 } e8q_m8_stats;
 
 /* Allocates the device index and its static tables (the allocation half of

@@ -5,7 +5,9 @@
 // timing protocol (3 warm-ups, then rounds until >= 9 rounds and >= the
 // minimum time; median round QPS: experiments/protocol.py:559-597).
 //
-//   ref_bench INDEX.e8ax QUERIES.f32 NQ K NPROBE THREADS MIN_SECONDS OUT_IDS.u32
+//   ref_bench INDEX.e8ax|INDEX.me8q QUERIES.f32 NQ K NPROBE THREADS MIN_SECONDS OUT_IDS.u32
+// An ME8QIDXX file is searched with Me8qIndex::SearchIVFFastBatch, the
+// reference binding's entry for nq > 1 (me8q_bindings.cpp:133-142).
 // QUERIES.f32 / OUT_IDS.u32 are raw little-endian arrays [NQ][dim] / [NQ][K].
 // Prints one JSON object on stdout.
 #include <omp.h>
@@ -17,21 +19,37 @@
 #include <cstdlib>
 #include <vector>
 
+#include <cstring>
+#include <functional>
+
 #include "e8qlib/index/e8q.hpp"
+#include "e8qlib/index/me8q.hpp"
 
 int main(int argc, char** argv) {
     if (argc != 9) {
         std::fprintf(stderr, "usage: ref_bench INDEX QUERIES.f32 NQ K NPROBE THREADS MIN_SECONDS OUT_IDS\n");
         return 2;
     }
+    char magic[8] = {};
+    {
+        FILE* mf = std::fopen(argv[1], "rb");
+        if (!mf || std::fread(magic, 1, 8, mf) != 8) {
+            std::fprintf(stderr, "cannot read index\n");
+            return 1;
+        }
+        std::fclose(mf);
+    }
+    const bool me8q = std::memcmp(magic, "ME8QIDXX", 8) == 0;
     e8qlib::E8qIndex idx;
-    idx.Load(argv[1]);
+    e8qlib::me8q::Me8qIndex midx;
+    if (me8q) midx.Load(argv[1]);
+    else idx.Load(argv[1]);
     const size_t nq = std::strtoull(argv[3], nullptr, 10);
     const size_t k = std::strtoull(argv[4], nullptr, 10);
     const size_t nprobe = std::strtoull(argv[5], nullptr, 10);
     const int threads = std::atoi(argv[6]);
     const double min_s = std::atof(argv[7]);
-    const size_t dim = idx.dim();
+    const size_t dim = me8q ? midx.dim() : idx.dim();
     std::vector<float> q(nq * dim);
     FILE* f = std::fopen(argv[2], "rb");
     if (!f || std::fread(q.data(), 4, q.size(), f) != q.size()) {
@@ -43,6 +61,10 @@ int main(int argc, char** argv) {
     std::vector<float> dists(nq * k);
     auto round = [&]() {
         const auto t0 = std::chrono::steady_clock::now();
+        if (me8q) {
+            midx.SearchIVFFastBatch(q.data(), nq, k, nprobe, ids.data(), nullptr, dists.data(), threads);
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
 #pragma omp parallel num_threads(threads)
         {
             e8qlib::QueryScratch scratch;

@@ -30,7 +30,8 @@ class M8SearchArgs(ctypes.Structure):
 
 class M8Stats(ctypes.Structure):
     _fields_ = [("candidates", u64), ("survivors", u64), ("bits_scored", u64), ("bits_total", u64),
-                ("pruned", u64), ("scan_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+                ("pruned", u64), ("scan_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("batches", u64),
+                ("refined", u64), ("groups", u64)]
 
 
 def tables():

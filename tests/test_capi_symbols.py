@@ -25,4 +25,4 @@ def test_me8q_struct_sizes():
     assert ctypes.sizeof(capi_m8.M8Desc) == 4 * 8 + 8 + 6 * 8 + 8
     assert ctypes.sizeof(capi_m8.M8Arrays) == 3 * 8
     assert ctypes.sizeof(capi_m8.M8SearchArgs) == 5 * 8 + 4 + 4 + 8 + 8 + 8
-    assert ctypes.sizeof(capi_m8.M8Stats) == 5 * 8 + 2 * 8
+    assert ctypes.sizeof(capi_m8.M8Stats) == 5 * 8 + 2 * 8 + 3 * 8
