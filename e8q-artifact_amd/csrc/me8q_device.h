@@ -27,6 +27,7 @@ namespace e8qdev {
 constexpr uint32_t kM8BankBytes = 160;   // per run per batch (me8q.hpp:183)
 constexpr uint32_t kM8MaxRuns = 32;      // the reference's flush4 bound (me8q_fast.cpp:865)
 constexpr uint32_t kM8MaxK = 1024;       // top-k list held in LDS
+constexpr uint32_t kM8LdsLevRate = 7;    // level tables up to this rate are staged in LDS
 
 // One coded run of the plan.
 struct M8Run {

@@ -410,8 +410,8 @@ e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search
         sa.out_d = a->outputs_on_device ? a->out_dists : od;
         sa.qstats = qst;
         sa.lev_floats = 0;
-        for (const auto& r : x->runs) sa.lev_floats += 2u << r.rate;
-        if (sa.lev_floats * 4u > 48u * 1024u) sa.lev_floats = 0;   // large tables stay in global memory
+        for (const auto& r : x->runs)
+            if (r.rate <= kM8LdsLevRate) sa.lev_floats += 2u << r.rate;
         Check(hipEventRecord(ctx->ev[1], st), "event");
         Check(LaunchM8Scan(st, sa), "scan");
         Check(hipEventRecord(ctx->ev[2], st), "event");

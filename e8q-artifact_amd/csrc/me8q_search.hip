@@ -351,10 +351,13 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
     uint8_t* plist = smem + L.plist;
     // level tables of every run, copied once per workgroup; run r's table
     // starts at slev + lev_off(r)
-    const bool lev_lds = a.lev_floats > 0;   // else the tables are read from global memory
-    if (lev_lds) {
+    // Level tables of the runs at rates <= kM8LdsLevRate (at most 256
+    // floats each) live in LDS, run after run; wider tables are read from
+    // global memory.
+    {
         uint32_t o = 0;
         for (uint32_t r = 0; r < R; ++r) {
+            if (v.runs[r].rate > kM8LdsLevRate) continue;
             const uint32_t nq2 = 2u << v.runs[r].rate;
             for (uint32_t e = threadIdx.x; e < nq2; e += 64) slev[o + e] = v.runs[r].lev[e];
             o += nq2;
@@ -555,8 +558,9 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                                 lj = plist[pair - r * np0];
                                 const M8Run& ru = v.runs[r];
                                 uint32_t lo_off = 0;
-                                for (uint32_t r2 = 0; r2 < r; ++r2) lo_off += 2u << v.runs[r2].rate;
-                                const float* lev = lev_lds ? slev + lo_off : ru.lev;
+                                for (uint32_t r2 = 0; r2 < r; ++r2)
+                                    if (v.runs[r2].rate <= kM8LdsLevRate) lo_off += 2u << v.runs[r2].rate;
+                                const float* lev = ru.rate <= kM8LdsLevRate ? slev + lo_off : ru.lev;
                                 const uint8_t* pb = rec + lj * PC + ru.blk0;
                                 const uint8_t* fr = v.refine + (lb + bb * 32 + lj) * v.rstride + ru.ref_off;
                                 acc = M8HalfChain(ru.rate, pb, fr, lev, q + ru.dim_off + ci, ru.nblk, half, ci);
