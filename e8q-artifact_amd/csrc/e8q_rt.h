@@ -100,11 +100,15 @@ struct e8q_ctx {
         std::vector<std::pair<void**, size_t>> want;
         template <class T>
         void add(T** p, size_t count) { want.push_back({reinterpret_cast<void**>(p), count * sizeof(T)}); }
+        size_t bytes() const {
+            size_t total = 0;
+            for (auto& w : want) total += (w.second + 255) & ~size_t{255};
+            return total;
+        }
     };
     void Bind() { e8qrt::Check(hipSetDevice(device), "hipSetDevice"); }
     void Take(Carve& c) {
-        size_t total = 0;
-        for (auto& w : c.want) total += (w.second + 255) & ~size_t{255};
+        const size_t total = c.bytes();
         if (ws.bytes < total) ws.alloc(total + total / 4);
         char* base = ws.as<char>();
         size_t off = 0;

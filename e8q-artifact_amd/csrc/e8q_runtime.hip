@@ -121,6 +121,17 @@ e8q_status e8q_ctx_destroy(e8q_ctx* ctx) {
     });
 }
 
+e8q_status e8q_ctx_reserve(e8q_ctx* ctx, uint64_t bytes) {
+    return Guard([&] {
+        if (!ctx) Throw(E8Q_EINVAL, "ctx is null");
+        ctx->Bind();
+        Check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        if (ctx->ws.bytes < bytes) ctx->ws.alloc(bytes);
+    });
+}
+
+uint64_t e8q_ctx_workspace_bytes(const e8q_ctx* ctx) { return ctx ? ctx->ws.bytes : 0; }
+
 e8q_status e8q_ctx_synchronize(e8q_ctx* ctx) {
     return Guard([&] {
         if (!ctx) Throw(E8Q_EINVAL, "ctx is null");
@@ -316,6 +327,32 @@ e8q_status e8q_index_encode(e8q_ctx* ctx, e8q_dindex* x, const float* base, cons
         Check(LaunchClusterSummary(st, v, x->csum.as<float4>()), "cluster summary");
         Check(hipStreamSynchronize(st), "encode");
     });
+}
+
+uint64_t e8q_search_workspace_bytes(const e8q_dindex* x, uint64_t nq, uint64_t k, uint64_t max_nprobe) {
+    if (!x) return 0;
+    const IndexView& v = x->view;
+    const uint64_t maxp = std::min<uint64_t>(max_nprobe, v.nlist);
+    const uint64_t P = v.padded, E = static_cast<uint64_t>(v.blocks) * 32;
+    float *dq = nullptr, *rotq = nullptr, *consts = nullptr, *dist = nullptr, *pd2 = nullptr, *odist = nullptr;
+    uint8_t* lut = nullptr;
+    uint32_t *pids = nullptr, *npq = nullptr, *oids = nullptr;
+    unsigned long long* dstats = nullptr;
+    PlainParams* dparams = nullptr;
+    e8q_ctx::Carve cv;   // the plain form's carve in e8q_search, host queries and outputs
+    cv.add(&dq, nq * v.dim);
+    cv.add(&rotq, nq * P);
+    cv.add(&lut, nq * E);
+    cv.add(&consts, nq * 8);
+    cv.add(&dist, nq * v.nlist);
+    cv.add(&pids, nq * maxp);
+    cv.add(&pd2, nq * maxp);
+    cv.add(&npq, nq);
+    cv.add(&oids, nq * k);
+    cv.add(&odist, nq * k);
+    cv.add(&dstats, 4);
+    cv.add(&dparams, 1);
+    return cv.bytes();
 }
 
 e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* a, e8q_search_stats* stats) {

@@ -193,6 +193,8 @@ class PyE8qIndex {
     size_t n() const { return index_ ? index_->n() : 0; }
     size_t num_clusters() const { return index_ ? index_->num_clusters() : 0; }
     size_t memory_bytes() const { return index_ ? index_->memory_bytes() : 0; }
+    // The C++ index behind this object (for the serving loop).
+    uintptr_t native() const { return reinterpret_cast<uintptr_t>(index_.get()); }
     size_t device_bytes() const { return index_ ? index_->device_bytes() : 0; }
     bool disable_prune() const { return index_ ? index_->disable_prune() : false; }
     float prune_alpha() const { return index_ ? index_->prune_alpha() : 0.0F; }
@@ -206,6 +208,7 @@ class PyE8qIndex {
 }  // namespace
 
 void RegisterMe8q(py::module_& m);   // me8q_bindings.cpp
+void RegisterServe(py::module_& m);  // serve_bindings.cpp
 
 PYBIND11_MODULE(_e8qlib, m) {
     m.doc() = "E8Q / ME8Q IVF indexes on MI355X (gfx950)";
@@ -233,5 +236,7 @@ PYBIND11_MODULE(_e8qlib, m) {
         .def_property_readonly("memory_bytes", &PyE8qIndex::memory_bytes)
         .def_property_readonly("device_bytes", &PyE8qIndex::device_bytes)
         .def_property_readonly("disable_prune", &PyE8qIndex::disable_prune)
-        .def_property_readonly("prune_alpha", &PyE8qIndex::prune_alpha);
+        .def_property_readonly("prune_alpha", &PyE8qIndex::prune_alpha)
+        .def("_native", &PyE8qIndex::native);
+    RegisterServe(m);
 }

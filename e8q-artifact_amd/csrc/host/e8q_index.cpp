@@ -359,6 +359,29 @@ void E8qIndex::SearchBatch(const float* queries, size_t nq, size_t k, size_t npr
     }
 }
 
+void E8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k,
+                               const uint32_t* nprobe_per_query, uint32_t* out_ids, float* out_dists) const {
+    if (dev_ == nullptr) throw std::runtime_error("E8qIndex must be built or loaded before search");
+    if (k == 0 || k > kMaxK) throw std::invalid_argument("k must be in [1, " + std::to_string(kMaxK) + "]");
+    e8q_search_args a{};
+    a.queries = queries;
+    a.nq = nq;
+    a.k = k;
+    a.nprobe = 1;
+    a.nprobe_per_query = nprobe_per_query;
+    a.z = hostmath::OrthoQuantile(alpha_, std::max<size_t>(4 * k, 32));
+    a.disable_prune = disable_prune_ ? 1 : 0;
+    a.flush_every = hostmath::FlushEvery();
+    a.mode = E8Q_MODE_PLAIN;
+    a.out_ids = out_ids;
+    a.out_dists = out_dists;
+    Check(e8q_search(ctx, dev_, &a, nullptr));
+}
+
+uint64_t E8qIndex::SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const {
+    return dev_ ? e8q_search_workspace_bytes(dev_, nq, k, max_nprobe) : 0;
+}
+
 void E8qIndex::Search(const float* query, size_t k, size_t nprobe, uint32_t* out_ids, QueryScratch& scratch,
                       SearchStats* stats, float* out_dists) const {
     (void)scratch;

@@ -75,6 +75,13 @@ class E8qIndex {
     void SearchBatch(const float* queries, size_t nq, size_t k, size_t nprobe, const uint32_t* nprobe_per_query,
                      uint32_t* out_ids, float* out_dists, SearchStats* stats, SearchMode mode,
                      bool queries_on_device = false, bool outputs_on_device = false) const;
+    // The plain-form search on a caller-owned device context (a serving
+    // worker's): no process-wide lock, so calls on different contexts run
+    // concurrently.  Host queries and outputs, per-query nprobe.
+    void SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k, const uint32_t* nprobe_per_query,
+                         uint32_t* out_ids, float* out_dists) const;
+    // Device scratch such a search carves from its context's workspace.
+    [[nodiscard]] uint64_t SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const;
     // Top-k of the refine estimator over every stored vector (no pruning).
     void ExhaustiveTopK(const float* query, size_t k, std::vector<std::pair<float, uint32_t>>& out) const;
 

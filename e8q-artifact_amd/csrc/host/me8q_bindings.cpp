@@ -187,6 +187,8 @@ class PyMe8qIndex {
     double bits_per_dim() const { return index_ ? index_->bits_per_dim() : 0.0; }
     size_t memory_bytes() const { return index_ ? index_->memory_bytes() : 0; }
     size_t device_bytes() const { return index_ ? index_->device_bytes() : 0; }
+    // The C++ index behind this object (for the serving loop).
+    uintptr_t native() const { return reinterpret_cast<uintptr_t>(index_.get()); }
     std::vector<std::pair<size_t, int>> plan() const {
         std::vector<std::pair<size_t, int>> out;
         if (!index_) return out;
@@ -225,5 +227,6 @@ void RegisterMe8q(py::module_& m) {
         .def_property_readonly("bits_per_dim", &PyMe8qIndex::bits_per_dim)
         .def_property_readonly("memory_bytes", &PyMe8qIndex::memory_bytes)
         .def_property_readonly("device_bytes", &PyMe8qIndex::device_bytes)
-        .def_property_readonly("plan", &PyMe8qIndex::plan);
+        .def_property_readonly("plan", &PyMe8qIndex::plan)
+        .def("_native", &PyMe8qIndex::native);
 }

@@ -487,6 +487,26 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
     }
 }
 
+void Me8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k,
+                                const uint32_t* nprobe_per_query, uint32_t* out_ids, float* out_dists) const {
+    if (!dev_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
+    if (k == 0 || k > kMaxDeviceK) throw std::invalid_argument("k must be in [1, 1024] on the device");
+    e8q_m8_search_args a{};
+    a.queries = queries;
+    a.nq = nq;
+    a.k = k;
+    a.nprobe = 1;
+    a.nprobe_per_query = nprobe_per_query;
+    a.z = hostmath::OrthoQuantile(static_cast<double>(ortho_alpha_), std::max<size_t>(4 * k, 32));
+    a.out_ids = out_ids;
+    a.out_dists = out_dists;
+    Check(e8q_m8_search(ctx, dev_, &a, nullptr, nullptr));
+}
+
+uint64_t Me8qIndex::SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const {
+    return dev_ ? e8q_m8_search_workspace_bytes(dev_, nq, k, max_nprobe) : 0;
+}
+
 void Me8qIndex::SearchIVFFastBatch(const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* out_ids,
                                    SearchStats* stats, float* out_dists, int /*num_threads*/) const {
     if (nq == 0) return;

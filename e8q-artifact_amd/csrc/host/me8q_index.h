@@ -105,6 +105,12 @@ class Me8qIndex {
                       uint32_t* out_ids, float* out_dists, SearchStats* stats, bool queries_on_device,
                       bool outputs_on_device, uint64_t* per_query_stats = nullptr) const;
 
+    // Search on a caller-owned device context (a serving worker's): no
+    // process-wide lock.  Host queries and outputs, per-query nprobe.
+    void SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k, const uint32_t* nprobe_per_query,
+                         uint32_t* out_ids, float* out_dists) const;
+    [[nodiscard]] uint64_t SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const;
+
     [[nodiscard]] size_t n() const { return n_; }
     [[nodiscard]] size_t dim() const { return dim_; }
     [[nodiscard]] size_t padded_dim() const { return padded_; }

@@ -74,9 +74,46 @@ void BuildDerived(e8q_ctx* ctx, e8q_m8index* x) {
     }
 }
 
+// Device scratch of one search, carved from the context workspace.
+struct M8Bufs {
+    float *dq, *rotq, *qc, *dist, *pd, *od;
+    double *rc, *sqd2;
+    uint8_t* lut;
+    uint32_t *probe, *npq, *oid;
+    uint64_t* qst;
+};
+
+void M8Carve(e8q_ctx::Carve& cv, M8Bufs& b, const M8View& v, uint64_t nq, uint64_t k, uint64_t maxp, bool q_host,
+             bool out_host, bool per_query_nprobe) {
+    const uint64_t R1 = std::max<uint64_t>(v.nrun, 1);
+    if (q_host) cv.add(&b.dq, nq * v.dim);
+    cv.add(&b.rotq, nq * v.padded);
+    cv.add(&b.lut, nq * v.plane_cols * 32 + 16);
+    cv.add(&b.qc, nq * 4);
+    cv.add(&b.rc, nq * R1 * 4);
+    cv.add(&b.dist, nq * v.nlist);
+    cv.add(&b.probe, nq * maxp);
+    cv.add(&b.pd, nq * maxp);
+    cv.add(&b.sqd2, nq * maxp * R1);
+    if (per_query_nprobe) cv.add(&b.npq, nq);
+    if (out_host) {
+        cv.add(&b.oid, nq * k);
+        cv.add(&b.od, nq * k);
+    }
+    cv.add(&b.qst, nq * 8);
+}
+
 }  // namespace
 
 extern "C" {
+
+uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* x, uint64_t nq, uint64_t k, uint64_t max_nprobe) {
+    if (!x) return 0;
+    M8Bufs b{};
+    e8q_ctx::Carve cv;
+    M8Carve(cv, b, x->view, nq, k, std::min<uint64_t>(max_nprobe, x->view.nlist), true, true, true);
+    return cv.bytes();
+}
 
 e8q_status e8q_m8_create(e8q_ctx* ctx, const e8q_m8_desc* d, e8q_m8index** out) {
     return Guard([&] {
@@ -341,29 +378,16 @@ e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search
         maxp = static_cast<uint32_t>(std::min<uint64_t>(maxp, v.nlist));
         ctx->Bind();
         hipStream_t st = ctx->stream;
-        const uint64_t P = v.padded, R = v.nrun, PC = v.plane_cols;
-        float *dq = nullptr, *rotq = nullptr, *qc = nullptr, *dist = nullptr, *pd = nullptr, *od = nullptr;
-        double *rc = nullptr, *sqd2 = nullptr;
-        uint8_t* lut = nullptr;
-        uint32_t *probe = nullptr, *npq = nullptr, *oid = nullptr;
-        uint64_t* qst = nullptr;
+        const uint64_t P = v.padded, R = v.nrun;
+        M8Bufs b{};
         e8q_ctx::Carve cv;
-        if (!a->queries_on_device) cv.add(&dq, nq * v.dim);
-        cv.add(&rotq, nq * P);
-        cv.add(&lut, nq * PC * 32 + 16);
-        cv.add(&qc, nq * 4);
-        cv.add(&rc, nq * std::max<uint64_t>(R, 1) * 4);
-        cv.add(&dist, nq * v.nlist);
-        cv.add(&probe, nq * maxp);
-        cv.add(&pd, nq * maxp);
-        cv.add(&sqd2, nq * maxp * std::max<uint64_t>(R, 1));
-        if (a->nprobe_per_query) cv.add(&npq, nq);
-        if (!a->outputs_on_device) {
-            cv.add(&oid, nq * a->k);
-            cv.add(&od, nq * a->k);
-        }
-        cv.add(&qst, nq * 8);
+        M8Carve(cv, b, v, nq, a->k, maxp, !a->queries_on_device, !a->outputs_on_device, a->nprobe_per_query != nullptr);
         ctx->Take(cv);
+        float *dq = b.dq, *rotq = b.rotq, *qc = b.qc, *dist = b.dist, *pd = b.pd, *od = b.od;
+        double *rc = b.rc, *sqd2 = b.sqd2;
+        uint8_t* lut = b.lut;
+        uint32_t *probe = b.probe, *npq = b.npq, *oid = b.oid;
+        uint64_t* qst = b.qst;
         const float* q = a->queries;
         if (!a->queries_on_device) {
             Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");

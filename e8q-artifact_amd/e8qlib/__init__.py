@@ -23,6 +23,6 @@ _here = os.path.dirname(os.path.abspath(__file__))
 if not os.path.exists(os.path.join(_here, "libe8q_gpu.so")):
     raise ImportError("e8qlib: libe8q_gpu.so is not built (run __graft_entry__.build())")
 
-from ._e8qlib import E8qIndex, Me8qIndex  # noqa: E402
+from ._e8qlib import E8qIndex, Me8qIndex, Server  # noqa: E402
 
-__all__ = ["E8qIndex", "Me8qIndex"]
+__all__ = ["E8qIndex", "Me8qIndex", "Server"]

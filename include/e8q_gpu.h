@@ -101,6 +101,11 @@ typedef struct {
 e8q_status e8q_ctx_create(int device, e8q_ctx** out);
 e8q_status e8q_ctx_destroy(e8q_ctx* ctx);
 e8q_status e8q_ctx_synchronize(e8q_ctx* ctx);
+/* Grows the context's device workspace to `bytes` now, so that later calls
+ * whose scratch fits it allocate nothing (serving admission; no reference
+ * counterpart: the reference allocates per-thread host scratch). */
+e8q_status e8q_ctx_reserve(e8q_ctx* ctx, uint64_t bytes);
+uint64_t e8q_ctx_workspace_bytes(const e8q_ctx* ctx);
 const char* e8q_last_error(void);
 
 /* ---- index lifetime --------------------------------------------------- */
@@ -134,6 +139,10 @@ e8q_status e8q_index_encode(e8q_ctx* ctx, e8q_dindex* idx, const float* base, co
  * (dist, id) and padded with UINT32_MAX / +inf. */
 e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* idx, const e8q_search_args* args,
                       e8q_search_stats* stats);
+/* Device scratch of one plain-form search of nq host queries with per-query
+ * nprobe <= max_nprobe and host outputs (what e8q_search carves from the
+ * context workspace); the serving loop admits work by it. */
+uint64_t e8q_search_workspace_bytes(const e8q_dindex* idx, uint64_t nq, uint64_t k, uint64_t max_nprobe);
 
 /* ---- diagnostics used by tests ------------------------------------------ */
 /* Coarse integer accumulate of one 32-lane batch (AccumulateCoarseHi,

@@ -99,6 +99,9 @@ e8q_status e8q_m8_encode(e8q_ctx* ctx, e8q_m8index* idx, const float* base, cons
  * bits_total, pruned. */
 e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* idx, const e8q_m8_search_args* args,
                          e8q_m8_stats* stats, uint64_t* per_query_stats);
+/* Device scratch of one search of nq host queries with per-query nprobe <=
+ * max_nprobe and host outputs, carved from the context workspace. */
+uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* idx, uint64_t nq, uint64_t k, uint64_t max_nprobe);
 
 /* ---- diagnostics used by tests (reference layouts, host pointers) -------- */
 /* One list's FastLayout as the reference holds it (me8q.hpp:153-184): plane

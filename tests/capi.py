@@ -64,7 +64,8 @@ def lib():
             raise RuntimeError("libe8q_gpu.so not built: run __graft_entry__.build()")
         L = ctypes.CDLL(LIB_PATH)
         L.e8q_last_error.restype = ctypes.c_char_p
-        u64_ret = ("e8q_index_device_bytes", "e8q_m8_device_bytes", "e8q_m8_row_bytes")
+        u64_ret = ("e8q_index_device_bytes", "e8q_m8_device_bytes", "e8q_m8_row_bytes", "e8q_ctx_workspace_bytes",
+                   "e8q_search_workspace_bytes", "e8q_m8_search_workspace_bytes")
         for name in declared_symbols():
             fn = getattr(L, name)
             if name in u64_ret:
