@@ -46,7 +46,12 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("e8q", "me8q"), default="e8q")
+    ap.add_argument("--workload", choices=("e8q", "me8q", "serve"), default="e8q")
+    ap.add_argument("--serve-index", choices=("e8q", "me8q"), default="e8q")
+    ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-rates", default="5,20,40,80",
+                    help="offered request rates as multiples of the plain path's request rate")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -70,7 +75,7 @@ def parse():
                     help="only run a device-to-device copy of BYTES three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     a = ap.parse_args()
-    me = a.workload == "me8q"
+    me = a.workload == "me8q" or (a.workload == "serve" and a.serve_index == "me8q")
     a.dim = a.dim or (960 if me else 128)
     a.nlist = a.nlist or (4096 if me else 1024)
     a.grid = a.grid or ("32,64,128,256" if me else "16,32,64,128")
@@ -357,6 +362,112 @@ def cpu_baseline_me8q(idx, queries_np, k, nprobe, gpu_ids, args):
             "rows_identical_to_gpu_plain": same}
 
 
+def run_serve(args):
+    """Continuous-batching serving (e8qlib.Server) against the plain path (one
+    search() call per request, back to back) on the same request stream:
+    sizes log-uniform in [1, 256] queries, nprobe log-uniform over
+    {8, ..., 256}, k uniform over {10, 100}, Poisson arrivals."""
+    import threading
+
+    import numpy as np
+    import torch
+    import e8qlib
+
+    rng = np.random.default_rng(7)
+    t0 = time.perf_counter()
+    if args.serve_index == "me8q":
+        base, centers, cid, queries, var = make_me8q_data(torch, args.n, args.dim, args.nlist, args.nq)
+        idx = e8qlib.Me8qIndex(args.dim, args.avg_bits, 0)
+        idx.build(base.cpu().numpy(), centers.cpu().numpy(), cid.to(torch.int32).cpu().numpy().astype(np.uint32),
+                  variance=var)
+        workload = f"ME8Q N={args.n} D={args.dim} {args.nlist} lists avg_bits={args.avg_bits}"
+    else:
+        base, centers, cid, queries = make_data(torch, args.n, args.dim, args.nlist, args.nq)
+        idx = e8qlib.E8qIndex(args.dim, args.bits, 42)
+        idx.build(base.cpu().numpy(), centers.cpu().numpy(), cid.to(torch.int32).cpu().numpy().astype(np.uint32))
+        workload = f"E8Q N={args.n} D={args.dim} {args.nlist} lists b={args.bits}"
+    log(f"index: {workload}, {time.perf_counter() - t0:.1f} s")
+    qpool = queries.cpu().numpy()
+    del base
+    nprobes = [p for p in (8, 16, 32, 64, 128, 256) if p <= args.nlist]
+    reqs = []
+    for _ in range(args.serve_requests):
+        nq = int(np.exp(rng.uniform(0.0, np.log(256.0))))
+        rows = rng.integers(0, qpool.shape[0], nq)
+        reqs.append((np.ascontiguousarray(qpool[rows]), int(rng.choice([10, 100])), int(rng.choice(nprobes))))
+    total_q = sum(r[0].shape[0] for r in reqs)
+    # plain path: one blocking search per request
+    for q, k, npb in reqs[:20]:
+        idx.search(q, k, npb)
+    ts = time.perf_counter()
+    plain_lat = []
+    for q, k, npb in reqs:
+        a = time.perf_counter()
+        idx.search(q, k, npb)
+        plain_lat.append((time.perf_counter() - a) * 1e6)
+    plain_s = time.perf_counter() - ts
+    plain_rps = len(reqs) / plain_s
+    plain_qps = total_q / plain_s
+    plain_lat.sort()
+    log(f"plain: {plain_qps:.0f} QPS ({plain_rps:.0f} requests/s), service p50 {plain_lat[len(plain_lat) // 2]:.0f} us")
+    srv = e8qlib.Server(idx, workers=args.serve_workers, max_batch=8192)
+    warm = [srv.submit(q, k, npb) for q, k, npb in reqs[:50]]
+    for t in warm:
+        srv.wait(t)
+    runs = []
+    for mult in [float(x) for x in args.serve_rates.split(",")]:
+        rate = plain_rps * mult
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        tickets = [None] * len(reqs)
+        lat = [0.0] * len(reqs)
+        done_t = [0.0] * len(reqs)
+
+        def waiter():
+            for i in range(len(reqs)):
+                while tickets[i] is None:
+                    time.sleep(0.0002)
+                _, _, info = srv.wait(tickets[i])
+                lat[i] = info["latency_us"]
+                done_t[i] = time.perf_counter()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        nxt = start
+        for i, (q, k, npb) in enumerate(reqs):
+            nxt += gaps[i]
+            d = nxt - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(q, k, npb)
+        th.join()
+        span = max(done_t) - start
+        ls = sorted(lat)
+        run = {"offered_rps": round(rate, 1), "offered_x_plain": mult, "qps": round(total_q / span, 1),
+               "requests_per_s": round(len(reqs) / span, 1), "p50_us": round(ls[len(ls) // 2], 1),
+               "p99_us": round(ls[min(len(ls) - 1, int(0.99 * len(ls)))], 1)}
+        runs.append(run)
+        log(f"served at {mult}x plain rate: {run['qps']:.0f} QPS, p50 {run['p50_us']:.0f} us, p99 {run['p99_us']:.0f} us")
+    st = srv.stats()
+    srv.close()
+    best = max(runs, key=lambda r: r["qps"])
+    line = {
+        "metric": "served QPS with p50/p99 latency (continuous batching vs one search call per request, 1 GPU)",
+        "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": len(runs), "warmup": 1,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "same as the index's plain search", "data": "synthetic, generated on the GPU",
+        "config": {"workload": f"served {workload}: {len(reqs)} requests ({total_q} queries), sizes log-uniform "
+                               f"[1,256], nprobe log-uniform {nprobes}, k in {{10,100}}, Poisson arrivals",
+                   "workers": args.serve_workers},
+        "plain": {"qps": round(plain_qps, 1), "requests_per_s": round(plain_rps, 1),
+                  "service_p50_us": round(plain_lat[len(plain_lat) // 2], 1),
+                  "service_p99_us": round(plain_lat[min(len(plain_lat) - 1, int(0.99 * len(plain_lat)))], 1)},
+        "served_runs": runs, "served_over_plain": round(best["qps"] / plain_qps, 2), "server_stats": st,
+        "latency_us": {"p50": best["p50_us"], "p99": best["p99_us"], "what": "per request, submit to results"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calib:
@@ -368,6 +479,9 @@ def main():
         sys.exit(2)
     if args.workload == "me8q":
         run_me8q(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
         return
     import numpy as np
     import torch
