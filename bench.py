@@ -5,7 +5,7 @@ Workload (the plain GPU E8Q configuration): N = 1M base vectors, D = 128,
 float32 Gaussian mixture (1024 centers ~ N(0, 4I), points = center + N(0, I)),
 IVF lists = the 1024 mixture centers with cluster_ids = nearest center,
 E8Q b = 4, 10k queries from the same mixture, k = 100, nprobe grid
-16/32/64/128.  Synthetic data generated on the GPU from fixed seeds.
+2..128 (powers of two).  Synthetic data generated on the GPU from fixed seeds.
 
 A step is one search call over the 10k-query batch (queries and outputs
 resident in HBM).  For every search form and nprobe the script runs W warm-up
@@ -60,10 +60,10 @@ def parse():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--bits", type=int, default=4)
-    ap.add_argument("--grid", default=None, help="16,32,64,128 (e8q) / 32,64,128,256 (me8q)")
+    ap.add_argument("--grid", default=None, help="2,4,...,128 (e8q) / 32,64,128,256 (me8q)")
     ap.add_argument("--avg-bits", type=float, default=4.0)
     ap.add_argument("--target", type=float, default=0.95)
-    ap.add_argument("--modes", default="plain,batched")
+    ap.add_argument("--modes", default="plain,batched,batched_graph")
     ap.add_argument("--latency-queries", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-sample", type=int, default=2000)
@@ -78,7 +78,7 @@ def parse():
     me = a.workload == "me8q" or (a.workload == "serve" and a.serve_index == "me8q")
     a.dim = a.dim or (960 if me else 128)
     a.nlist = a.nlist or (4096 if me else 1024)
-    a.grid = a.grid or ("32,64,128,256" if me else "16,32,64,128")
+    a.grid = a.grid or ("32,64,128,256" if me else "2,4,8,16,32,64,128")
     return a
 
 
@@ -306,7 +306,15 @@ def run_me8q(args):
         cpu = cpu_baseline_me8q(idx, queries.cpu().numpy(), k, op["nprobe"],
                                 ids_dev.cpu().numpy().view(np.uint32), args)
         log(f"cpu baseline: {cpu}")
-    achieved = op["scan_achieved_gbs"]
+    # Roofline of the plain scan kernel (its own events, eager launches) at
+    # the largest grid nprobe (the scan-bound regime), and at the plain
+    # form's operating point.
+    rop = opp = op
+    if "plain" in results:
+        prow = results["plain"]["rows"]
+        opp = next((r for r in prow if r["recall"] >= args.target), prow[-1])
+        rop = max(prow, key=lambda r: r["nprobe"])
+    achieved = rop["scan_achieved_gbs"]
     line = {
         "metric": "QPS at recall@10>=0.95 (ME8Q mixed-rate plain GPU search, 1 GPU); p50/p99 latency",
         "value": round(value, 1) if value == value else None,
@@ -510,15 +518,19 @@ def main():
     results = {}
     for mode in modes:
         rows = []
+        # "<form>_graph": the same form replayed as a HIP graph (the warmup
+        # calls run it eagerly, then capture it).
+        form = mode[:-len("_graph")] if mode.endswith("_graph") else mode
+        e8qlib.set_search_graphs(mode != form)
         for nprobe in grid:
-            for _ in range(args.warmup):
-                idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr(), mode)
+            for _ in range(max(args.warmup, 2 if mode != form else 0)):
+                idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr(), form)
             torch.cuda.synchronize()
             times, scan_ms, st = [], [], None
             for _ in range(args.steps):
                 torch.cuda.synchronize()
                 ts = time.perf_counter()
-                st = idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr(), mode)
+                st = idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr(), form)
                 torch.cuda.synchronize()
                 times.append(time.perf_counter() - ts)
                 scan_ms.append(st["device_scan_ms"])
@@ -539,6 +551,9 @@ def main():
                 f"scan kernel {row['scan_kernel_ms']:.2f} ms = {row['scan_achieved_gbs']:.0f} GB/s, "
                 f"records read {row['records_read']}/{row['batches']}")
         results[mode] = {"rows": rows, "qps_at_target": qps_at_recall(rows, args.target) if gt is not None else None}
+        if mode != form:
+            results[mode]["graph"] = e8qlib.search_context_info()
+            e8qlib.set_search_graphs(False)
     if args.profile:
         return
     best = max(results, key=lambda m: results[m]["qps_at_target"] or 0.0)
@@ -567,7 +582,15 @@ def main():
         cpu = cpu_baseline(idx, queries.cpu().numpy(), k, op["nprobe"], plain_ids, args)
         log(f"cpu baseline: {cpu}")
 
-    achieved = op["scan_achieved_gbs"]
+    # Roofline of the plain scan kernel (its own events, eager launches) at
+    # the largest grid nprobe (the scan-bound regime), and at the plain
+    # form's operating point.
+    rop = opp = op
+    if "plain" in results:
+        prow = results["plain"]["rows"]
+        opp = next((r for r in prow if r["recall"] >= args.target), prow[-1])
+        rop = max(prow, key=lambda r: r["nprobe"])
+    achieved = rop["scan_achieved_gbs"]
     line = {
         "metric": "QPS at recall@10>=0.95 (plain vs batched/served, 1 GPU); p50/p99 latency",
         "value": round(value, 1) if value == value else None,
@@ -582,7 +605,7 @@ def main():
         "dtype": "u8 LUT / i32 coarse sums, fp32 refine",
         "data": "synthetic Gaussian mixture generated on the GPU (seed 42)",
         "config": {"workload": "plain GPU E8Q scan: N=1M, D=128 Gaussian mixture, 1024 lists, b=4, "
-                               "10k queries, k=100, nprobe 16..128",
+                               f"10k queries, k=100, nprobe {args.grid}",
                    "n": args.n, "dim": args.dim, "nlist": args.nlist, "nq": nq, "k": k, "bits": args.bits,
                    "nprobe_grid": grid, "form": best, "operating_nprobe": op["nprobe"],
                    "recall_at_operating_point": op["recall"],
@@ -598,8 +621,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "PlainSearchKernel", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "basis": f"per launch: {op['batches']} scan records x {batch_bytes} B + {op['survivors']} "
-                              f"survivors x {survivor_bytes} B, / mean kernel time {op['scan_kernel_ms']:.3f} ms"},
+                     "basis": f"plain form at nprobe={rop['nprobe']}, per launch: {rop['batches']} scan records x "
+                              f"{batch_bytes} B + {rop['survivors']} survivors x {survivor_bytes} B, / mean kernel "
+                              f"time {rop['scan_kernel_ms']:.3f} ms",
+                     "at_operating_point": {"nprobe": opp["nprobe"], "achieved": round(opp["scan_achieved_gbs"], 1),
+                                            "frac": round(opp["scan_achieved_gbs"] / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

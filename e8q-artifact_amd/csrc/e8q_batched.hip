@@ -160,10 +160,14 @@ __global__ __launch_bounds__(256) void ItemsKernel(const BatchedParams* __restri
     }
 }
 
-hipError_t LaunchBatchedWorkList(hipStream_t st, const IndexView& ix, const BatchedArgs& a, BatchedParams* dparams) {
-    const BatchedParams hp{ix, a};
-    hipError_t e = hipMemcpyAsync(dparams, &hp, sizeof(hp), hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return e;
+hipError_t LaunchBatchedWorkList(hipStream_t st, const IndexView& ix, const BatchedArgs& a, BatchedParams* dparams,
+                                 bool upload) {
+    hipError_t e = hipSuccess;
+    if (upload) {
+        const BatchedParams hp{ix, a};
+        e = hipMemcpyAsync(dparams, &hp, sizeof(hp), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+    }
     const uint64_t total = a.nq * a.maxp;
     const uint32_t g = static_cast<uint32_t>(std::min<uint64_t>((total + 255) / 256, 2048));
     hipLaunchKernelGGL(PairCountKernel, dim3(g), dim3(256), 0, st, dparams);

@@ -61,7 +61,8 @@ struct PlainParams {
     IndexView ix;
     PlainArgs a;
 };
-hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a, PlainParams* dparams);
+hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a, PlainParams* dparams,
+                             bool upload = true);
 hipError_t LaunchCoarseAccumulate(hipStream_t st, const uint8_t* rowa, uint64_t nbatch, const uint8_t* lut, int blocks,
                                   int32_t* out);
 
@@ -131,7 +132,8 @@ inline void BatchTiles(const IndexView& ix, uint32_t* qtile, uint32_t* btile) {
 // The work list of the pairs (q, p), 1 <= p < nprobe(q), of non-empty
 // clusters: count, scan, scatter, items.  Fills the candidate and batch
 // counts of those pairs into stats.
-hipError_t LaunchBatchedWorkList(hipStream_t st, const IndexView& ix, const BatchedArgs& a, BatchedParams* dparams);
+hipError_t LaunchBatchedWorkList(hipStream_t st, const IndexView& ix, const BatchedArgs& a, BatchedParams* dparams,
+                                 bool upload = true);
 // The persistent scan over the items; `grid` workgroups.
 size_t BatchedScanLds(const IndexView& ix, uint32_t qtile, uint32_t btile);
 hipError_t LaunchBatchedScan(hipStream_t st, const IndexView& ix, const BatchedArgs& a, const BatchedParams* dparams,

@@ -95,6 +95,54 @@ struct e8q_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // search timing
     DBuf ws;   // per-call workspace, grown on demand
 
+    // Replay of a search's device sequence as a HIP graph, per call shape.
+    // A shape's first call runs eagerly and registers the shape; its second
+    // call captures the sequence, later ones replay it.
+    struct GraphEntry {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t last_use = 0;
+    };
+    static constexpr size_t kMaxGraphs = 16;
+    bool graphs = false;
+    std::vector<GraphEntry> gcache;
+    uint64_t gclock = 0, captures = 0, replays = 0;
+    GraphEntry* FindGraph(const std::vector<uint64_t>& key) {
+        for (auto& g : gcache)
+            if (g.key == key) {
+                g.last_use = ++gclock;
+                return &g;
+            }
+        return nullptr;
+    }
+    GraphEntry* AddGraph(const std::vector<uint64_t>& key) {
+        if (gcache.size() >= kMaxGraphs) {   // evict the least recently used
+            auto lru = std::min_element(gcache.begin(), gcache.end(),
+                                        [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+            if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+            gcache.erase(lru);
+        }
+        gcache.push_back(GraphEntry{key, nullptr, ++gclock});
+        return &gcache.back();
+    }
+    void ClearGraphs() {
+        for (auto& g : gcache)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        gcache.clear();
+    }
+
+    // Shape of the batched scan: fixed by the caller (nonzero fix_*), else
+    // chosen by timing candidates on the first batched call per (index
+    // shape, power-of-two query-count bucket).
+    uint32_t fix_qtile = 0, fix_btile = 0, fix_grid = 0;
+    struct Tuned {
+        uint64_t shape[4];   // batch bytes, blocks, nlist, query bucket
+        uint32_t qtile, btile, grid;
+    };
+    std::vector<Tuned> tuned;
+    uint32_t last_qtile = 0, last_btile = 0, last_grid = 0;
+    uint64_t tunings = 0;
+
     // Bump allocator over ws for one call.
     struct Carve {
         std::vector<std::pair<void**, size_t>> want;

@@ -83,6 +83,65 @@ void RefreshView(e8q_dindex* x) {
 }  // namespace
 
 // ===========================================================================
+namespace {
+
+uint64_t FloatBits(float f) {
+    uint32_t u = 0;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+// Power-of-two bucket of a query count (the tuner's cache key).
+uint64_t QueryBucket(uint64_t nq) {
+    uint64_t b = 0;
+    while ((uint64_t{1} << b) < nq) ++b;
+    return b;
+}
+
+// Upper bound of the batched scan's work items at a tile shape: over the
+// lists, ceil(pairs_c / qtile) * ceil(batches_c / btile).
+uint64_t ItemsCap(const e8q_dindex* x, uint64_t nq, uint64_t pairs, uint32_t qtile, uint32_t btile) {
+    uint64_t sum_nbt = 0, max_nbt = 0;
+    for (uint64_t c = 0; c < x->view.nlist; ++c) {
+        const uint64_t nb = (x->cluster_begin[c + 1] - x->cluster_begin[c] + 31) / 32;
+        const uint64_t nbt = (nb + btile - 1) / btile;
+        sum_nbt += nbt;
+        max_nbt = std::max(max_nbt, nbt);
+    }
+    return std::min((nq + qtile - 1) / qtile * sum_nbt, (pairs + qtile - 1) / qtile * max_nbt + sum_nbt);
+}
+
+struct ScanShape {
+    uint32_t qtile, btile, grid;
+};
+
+bool ShapeFits(const IndexView& v, uint32_t qtile, uint32_t btile) {
+    return qtile >= 1 && qtile <= 64 && btile >= 2 && btile <= 64 && (btile & 1u) == 0 &&
+           BatchedScanLds(v, qtile, btile) <= 160 * 1024;
+}
+
+// The tuner's candidates: pairs and batches per item at half, once and
+// twice the default shape (default first, so a tie keeps it).
+std::vector<ScanShape> ShapeCandidates(const IndexView& v, uint32_t grid) {
+    uint32_t q0 = 0, b0 = 0;
+    BatchTiles(v, &q0, &b0);
+    std::vector<ScanShape> out{{q0, b0, grid}};
+    for (uint32_t qm = 0; qm < 3; ++qm)
+        for (uint32_t bm = 0; bm < 3; ++bm) {
+            const uint32_t qt = std::clamp<uint32_t>(qm == 0 ? q0 / 2 : q0 << (qm - 1), 4, 32);
+            const uint32_t bt = std::clamp<uint32_t>((bm == 0 ? b0 / 2 : b0 << (bm - 1)) & ~1u, 2, 32);
+            if (!ShapeFits(v, qt, bt)) continue;
+            bool dup = false;
+            for (const ScanShape& c : out) dup = dup || (c.qtile == qt && c.btile == bt);
+            if (!dup) out.push_back({qt, bt, grid});
+        }
+    return out;
+}
+
+constexpr uint64_t kTuneMinQueries = 64;   // smaller calls use the default shape untuned
+
+}  // namespace
+
 extern "C" {
 
 const char* e8q_last_error(void) { return g_err.c_str(); }
@@ -113,6 +172,7 @@ e8q_status e8q_ctx_destroy(e8q_ctx* ctx) {
         if (!ctx) return;
         ctx->Bind();
         (void)hipStreamSynchronize(ctx->stream);
+        ctx->ClearGraphs();
         ctx->ws.reset();
         for (auto& e : ctx->ev)
             if (e) (void)hipEventDestroy(e);
@@ -355,6 +415,38 @@ uint64_t e8q_search_workspace_bytes(const e8q_dindex* x, uint64_t nq, uint64_t k
     return cv.bytes();
 }
 
+
+e8q_status e8q_ctx_set_graphs(e8q_ctx* ctx, int enable) {
+    return Guard([&] {
+        if (!ctx) Throw(E8Q_EINVAL, "ctx is null");
+        ctx->Bind();
+        Check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        ctx->graphs = enable != 0;
+        if (!ctx->graphs) ctx->ClearGraphs();
+    });
+}
+
+e8q_status e8q_ctx_set_batched_shape(e8q_ctx* ctx, uint32_t qtile, uint32_t btile, uint32_t grid) {
+    return Guard([&] {
+        if (!ctx) Throw(E8Q_EINVAL, "ctx is null");
+        if ((qtile == 0) != (btile == 0)) Throw(E8Q_EINVAL, "qtile and btile are set together");
+        if (qtile != 0 && (qtile > 64 || btile < 2 || btile > 64 || (btile & 1u)))
+            Throw(E8Q_EINVAL, "qtile must be in [1, 64] and btile even in [2, 64]");
+        if (grid > 65536) Throw(E8Q_EINVAL, "grid must be <= 65536");
+        ctx->fix_qtile = qtile;
+        ctx->fix_btile = btile;
+        ctx->fix_grid = grid;
+    });
+}
+
+e8q_status e8q_ctx_get_info(const e8q_ctx* ctx, e8q_ctx_info* out) {
+    return Guard([&] {
+        if (!ctx || !out) Throw(E8Q_EINVAL, "null argument");
+        *out = e8q_ctx_info{ctx->last_qtile, ctx->last_btile, ctx->last_grid, ctx->graphs ? 1u : 0u,
+                            ctx->tunings, ctx->captures, ctx->replays};
+    });
+}
+
 e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* a, e8q_search_stats* stats) {
     return Guard([&] {
         if (!ctx || !x || !a) Throw(E8Q_EINVAL, "null argument");
@@ -379,13 +471,16 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             maxp = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
         }
         maxp = static_cast<uint32_t>(std::min<uint64_t>(maxp, v.nlist));
+        const uint32_t nprobe_all = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
+        const size_t sel_lds = SelectProbesLds(static_cast<uint32_t>(v.nlist), maxp);
+        if (sel_lds > 160 * 1024) Throw(E8Q_EINVAL, "nlist too large for the probe selector");
         ctx->Bind();
         hipStream_t st = ctx->stream;
         const uint64_t P = v.padded, E = static_cast<uint64_t>(v.blocks) * 32;
         float *dq = nullptr, *rotq = nullptr, *consts = nullptr, *dist = nullptr, *pd2 = nullptr, *odist = nullptr;
         uint8_t* lut = nullptr;
         uint32_t *pids = nullptr, *npq = nullptr, *oids = nullptr;
-        unsigned long long* dstats = nullptr;
+        unsigned long long *dstats = nullptr, *tstats = nullptr;
         PlainParams* dparams = nullptr;
         e8q_ctx::Carve cv;
         if (!a->queries_on_device) cv.add(&dq, nq * v.dim);
@@ -402,32 +497,47 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         }
         cv.add(&dstats, 4);
         cv.add(&dparams, 1);
-        // Batched form: first-probe lists, work list, candidates.
+        // Batched form: its scan shape (fixed, tuned before, or to be tuned
+        // now), first-probe lists, work list, candidates.
         const bool batched = a->mode == E8Q_MODE_BATCHED;
         BatchedArgs ba{};
         BatchedParams* bparams = nullptr;
         uint32_t* first_ids = nullptr;
         float* first_dists = nullptr;
-        uint64_t items_cap = 0;
+        uint64_t pairs = 0;
+        ScanShape shape{0, 0, 0};
+        std::vector<ScanShape> cands;   // non-empty: tune in this call
+        const uint64_t tkey[4] = {v.batch_bytes, static_cast<uint64_t>(v.blocks), v.nlist, QueryBucket(nq)};
         if (batched) {
-            BatchTiles(v, &ba.qtile, &ba.btile);
-            uint64_t pairs = 0;
+            const uint32_t grid0 = static_cast<uint32_t>(std::max(1, ctx->cus) * 2);
+            if (ctx->fix_qtile != 0) {
+                shape = {ctx->fix_qtile, ctx->fix_btile, ctx->fix_grid ? ctx->fix_grid : grid0};
+                if (!ShapeFits(v, shape.qtile, shape.btile))
+                    Throw(E8Q_EINVAL, "batched shape needs more than 160 KiB of LDS for this index");
+            } else {
+                bool found = false;
+                for (const auto& t : ctx->tuned)
+                    if (std::equal(t.shape, t.shape + 4, tkey)) {
+                        shape = {t.qtile, t.btile, t.grid};
+                        found = true;
+                    }
+                if (!found) {
+                    cands = ShapeCandidates(v, grid0);
+                    shape = cands[0];
+                    if (nq < kTuneMinQueries) cands.clear();
+                }
+                if (ctx->fix_grid != 0) shape.grid = ctx->fix_grid;
+            }
             if (a->nprobe_per_query) {
                 for (uint64_t i = 0; i < nq; ++i)
                     pairs += std::min<uint64_t>(a->nprobe_per_query[i], v.nlist) - 1;
             } else {
                 pairs = nq * (maxp - 1);
             }
-            uint64_t sum_nbt = 0, max_nbt = 0;
-            for (uint64_t c = 0; c < v.nlist; ++c) {
-                const uint64_t nb = (x->cluster_begin[c + 1] - x->cluster_begin[c] + 31) / 32;
-                const uint64_t nbt = (nb + ba.btile - 1) / ba.btile;
-                sum_nbt += nbt;
-                max_nbt = std::max(max_nbt, nbt);
-            }
-            // items of a cluster = ceil(pairs_c / qtile) * ceil(nb_c / btile)
-            items_cap = std::min((nq + ba.qtile - 1) / ba.qtile * sum_nbt,
-                                 (pairs + ba.qtile - 1) / ba.qtile * max_nbt + sum_nbt);
+            uint64_t items_cap = ItemsCap(x, nq, pairs, shape.qtile, shape.btile);
+            for (const ScanShape& c : cands) items_cap = std::max(items_cap, ItemsCap(x, nq, pairs, c.qtile, c.btile));
+            ba.qtile = shape.qtile;
+            ba.btile = shape.btile;
             ba.cap = kBatchCandCap;
             cv.add(&first_ids, nq * a->k);
             cv.add(&first_dists, nq * a->k);
@@ -437,34 +547,20 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             cv.add(&ba.ioff, v.nlist + 1);
             cv.add(&ba.pairs, std::max<uint64_t>(pairs, 1));
             cv.add(&ba.items, std::max<uint64_t>(items_cap, 1));
-            ba.items_cap = items_cap;
+            ba.items_cap = ItemsCap(x, nq, pairs, shape.qtile, shape.btile);
             cv.add(&ba.counters, 4);
             cv.add(&ba.cand, nq * ba.cap);
             cv.add(&ba.ncand, nq);
             cv.add(&ba.overflow, nq);
             cv.add(&bparams, 1);
+            if (!cands.empty()) cv.add(&tstats, 4);
         }
         ctx->Take(cv);
-        const float* qsrc = a->queries;
-        if (!a->queries_on_device) {
-            Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
-            qsrc = dq;
-        }
-        if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
+        const float* qsrc = a->queries_on_device ? a->queries : dq;
         if (a->outputs_on_device) {
             oids = a->out_ids;
             odist = a->out_dists;
         }
-        Check(hipMemsetAsync(dstats, 0, 32, st), "memset");
-        Check(hipEventRecord(ctx->ev[0], st), "event");
-        Check(LaunchRotate(st, qsrc, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
-        Check(LaunchQueryPrep(st, rotq, nq, v, lut, consts), "query prep");
-        Check(LaunchRoute(st, rotq, consts, nq, v, dist), "route");
-        const size_t sel_lds = SelectProbesLds(static_cast<uint32_t>(v.nlist), maxp);
-        if (sel_lds > 160 * 1024) Throw(E8Q_EINVAL, "nlist too large for the probe selector");
-        Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq,
-                                 static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist)), maxp, pids, pd2),
-              "select probes");
         PlainArgs pa{};
         pa.nq = nq;
         pa.rotq = rotq;
@@ -482,19 +578,16 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         pa.out_ids = oids;
         pa.out_dists = odist;
         pa.stats = dstats;
-        Check(hipEventRecord(ctx->ev[1], st), "event");
-        if (!batched) {
-            Check(LaunchPlainSearch(st, v, pa, dparams), "plain search");
-        } else {
-            // Threshold pass: the plain search over each query's first probe.
-            PlainArgs fa = pa;
-            fa.nprobe_q = nullptr;
-            fa.nprobe_all = 1;
-            fa.out_ids = first_ids;
-            fa.out_dists = first_dists;
+        // Threshold pass of the batched form: the plain search over each
+        // query's first probe.
+        PlainArgs fa = pa;
+        fa.nprobe_q = nullptr;
+        fa.nprobe_all = 1;
+        fa.out_ids = first_ids;
+        fa.out_dists = first_dists;
+        if (batched) {
             ba.first_ids = first_ids;
             ba.first_dists = first_dists;
-            Check(LaunchPlainSearch(st, v, fa, dparams), "first-probe search");
             ba.nq = nq;
             ba.maxp = maxp;
             ba.nprobe_q = npq;
@@ -510,12 +603,154 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             ba.out_ids = oids;
             ba.out_dists = odist;
             ba.stats = dstats;
-            Check(LaunchFill32(st, ba.ccount, v.nlist, 0), "fill");
-            Check(LaunchFill32(st, ba.ncand, nq, 0), "fill");
-            Check(LaunchBatchedWorkList(st, v, ba, bparams), "work list");
-            const uint32_t grid = static_cast<uint32_t>(std::max(1, ctx->cus) * 2);
-            Check(LaunchBatchedScan(st, v, ba, bparams, grid), "batched scan");
-            Check(LaunchBatchedRefine(st, v, ba, bparams), "batched refine");
+        }
+        auto batched_tail = [&](const BatchedArgs& b, uint32_t grid, bool upload) {
+            Check(LaunchFill32(st, b.ccount, v.nlist, 0), "fill");
+            Check(LaunchFill32(st, b.ncand, nq, 0), "fill");
+            Check(LaunchBatchedWorkList(st, v, b, bparams, upload), "work list");
+            Check(LaunchBatchedScan(st, v, b, bparams, grid), "batched scan");
+            Check(LaunchBatchedRefine(st, v, b, bparams), "batched refine");
+        };
+        // Times the batched tail at each candidate shape (twice, keeping the
+        // faster), then the persistent grid at the best tile; results do not
+        // depend on the shape.  Counts go to a scratch stats block.
+        auto tune = [&] {
+            auto time_shape = [&](const ScanShape& c) {
+                BatchedArgs t = ba;
+                t.qtile = c.qtile;
+                t.btile = c.btile;
+                t.items_cap = ItemsCap(x, nq, pairs, c.qtile, c.btile);
+                t.stats = tstats;
+                float best = 3.0e38F;
+                for (int rep = 0; rep < 2; ++rep) {
+                    Check(hipEventRecord(ctx->ev[2], st), "event");
+                    batched_tail(t, c.grid, true);
+                    Check(hipEventRecord(ctx->ev[3], st), "event");
+                    Check(hipEventSynchronize(ctx->ev[3]), "tuning");
+                    float ms = 0.0F;
+                    Check(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "event time");
+                    best = std::min(best, ms);
+                }
+                return best;
+            };
+            ScanShape best = cands[0];
+            float best_ms = time_shape(best);
+            for (size_t i = 1; i < cands.size(); ++i) {
+                const float ms = time_shape(cands[i]);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = cands[i];
+                }
+            }
+            if (ctx->fix_grid == 0) {
+                const uint32_t cu = static_cast<uint32_t>(std::max(1, ctx->cus));
+                for (uint32_t g : {cu, 4 * cu}) {
+                    const ScanShape c{best.qtile, best.btile, g};
+                    const float ms = time_shape(c);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
+                    }
+                }
+            }
+            ctx->tuned.push_back({{tkey[0], tkey[1], tkey[2], tkey[3]}, best.qtile, best.btile, best.grid});
+            ++ctx->tunings;
+            shape = best;
+            if (ctx->fix_grid != 0) shape.grid = ctx->fix_grid;
+            ba.qtile = shape.qtile;
+            ba.btile = shape.btile;
+            ba.items_cap = ItemsCap(x, nq, pairs, shape.qtile, shape.btile);
+        };
+        // The device sequence from the query rotation to the results.  With
+        // upload false the launchers' parameter blocks are already on the
+        // device (a graph capture records kernel launches only).
+        auto device_seq = [&](bool upload, bool eager) {
+            Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(dstats), 8, 0), "fill");
+            Check(LaunchRotate(st, qsrc, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
+            Check(LaunchQueryPrep(st, rotq, nq, v, lut, consts), "query prep");
+            Check(LaunchRoute(st, rotq, consts, nq, v, dist), "route");
+            Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq, nprobe_all, maxp, pids, pd2),
+                  "select probes");
+            if (eager) Check(hipEventRecord(ctx->ev[1], st), "event");
+            if (!batched) {
+                Check(LaunchPlainSearch(st, v, pa, dparams, upload), "plain search");
+                return;
+            }
+            Check(LaunchPlainSearch(st, v, fa, dparams, upload), "first-probe search");
+            if (eager && !cands.empty()) tune();
+            batched_tail(ba, shape.grid, upload);
+        };
+        // Graph replay: the key is everything the captured launches bake in.
+        e8q_ctx::GraphEntry* ge = nullptr;
+        if (ctx->graphs && cands.empty()) {
+            static_assert(sizeof(IndexView) % 8 == 0, "IndexView is compared as 64-bit words");
+            std::vector<uint64_t> key{reinterpret_cast<uint64_t>(x),
+                                      static_cast<uint64_t>(a->mode),
+                                      nq,
+                                      a->k,
+                                      maxp,
+                                      nprobe_all,
+                                      npq != nullptr ? 1u : 0u,
+                                      FloatBits(a->z),
+                                      static_cast<uint64_t>(pa.disable_prune),
+                                      static_cast<uint64_t>(flush),
+                                      reinterpret_cast<uint64_t>(qsrc),
+                                      reinterpret_cast<uint64_t>(oids),
+                                      reinterpret_cast<uint64_t>(odist),
+                                      reinterpret_cast<uint64_t>(ctx->ws.p),
+                                      ctx->ws.bytes,
+                                      shape.qtile,
+                                      shape.btile,
+                                      shape.grid,
+                                      ba.items_cap,
+                                      pairs};
+            const size_t w0 = key.size();
+            key.resize(w0 + sizeof(IndexView) / 8);
+            std::memcpy(key.data() + w0, &v, sizeof(IndexView));
+            ge = ctx->FindGraph(key);
+            if (!ge) ctx->AddGraph(key);   // first call of this shape runs eagerly
+        }
+        if (!a->queries_on_device)
+            Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+        if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
+        if (ge) {
+            const PlainParams hp{v, batched ? fa : pa};
+            Check(hipMemcpyAsync(dparams, &hp, sizeof(hp), hipMemcpyHostToDevice, st), "upload parameters");
+            if (batched) {
+                const BatchedParams hb{v, ba};
+                Check(hipMemcpyAsync(bparams, &hb, sizeof(hb), hipMemcpyHostToDevice, st), "upload parameters");
+            }
+            if (!ge->exec) {
+                Check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+                try {
+                    device_seq(false, false);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                hipGraph_t g = nullptr;
+                Check(hipStreamEndCapture(st, &g), "end capture");
+                hipGraphExec_t ex = nullptr;
+                const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                Check(e, "graph instantiate");
+                ge->exec = ex;
+                ++ctx->captures;
+            }
+            Check(hipEventRecord(ctx->ev[0], st), "event");
+            Check(hipEventRecord(ctx->ev[1], st), "event");
+            Check(hipGraphLaunch(ge->exec, st), "graph launch");
+            ++ctx->replays;
+        } else {
+            Check(hipEventRecord(ctx->ev[0], st), "event");
+            device_seq(true, true);
+        }
+        if (batched) {
+            ctx->last_qtile = shape.qtile;
+            ctx->last_btile = shape.btile;
+            ctx->last_grid = shape.grid;
         }
         Check(hipEventRecord(ctx->ev[2], st), "event");
         if (batched) {   // queries whose candidate lists overflowed: the plain search

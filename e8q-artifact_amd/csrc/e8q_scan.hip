@@ -500,14 +500,17 @@ static hipError_t LaunchPlainT(hipStream_t st, const PlainParams* dp, uint32_t g
     return hipGetLastError();
 }
 
-hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a, PlainParams* dparams) {
+hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArgs& a, PlainParams* dparams,
+                             bool upload) {
     const uint64_t nrun = a.qsel ? a.nsel : a.nq;
     if (nrun == 0) return hipSuccess;
     if (a.flush_every < 1 || a.flush_every > kMaxFlush) return hipErrorInvalidValue;
     if (ix.blocks % 8 != 0) return hipErrorInvalidValue;
-    const PlainParams hp{ix, a};
-    const hipError_t e = hipMemcpyAsync(dparams, &hp, sizeof(hp), hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return e;
+    if (upload) {
+        const PlainParams hp{ix, a};
+        const hipError_t e = hipMemcpyAsync(dparams, &hp, sizeof(hp), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+    }
     const uint32_t grid = static_cast<uint32_t>(nrun < 262144 ? nrun : 262144);
     const size_t dyn = PlainSearchLds(ix, a.flush_every);
     // Rates 2..5 (refine fields of 1..4 bytes per block) take the specialized

@@ -238,5 +238,22 @@ PYBIND11_MODULE(_e8qlib, m) {
         .def_property_readonly("disable_prune", &PyE8qIndex::disable_prune)
         .def_property_readonly("prune_alpha", &PyE8qIndex::prune_alpha)
         .def("_native", &PyE8qIndex::native);
+    // Launch options of the process-wide device context.
+    m.def("set_search_graphs", &e8qlib::SetSearchGraphs, py::arg("on"),
+          "Replay repeated search shapes as HIP graphs (first call eager, second captures).");
+    m.def("set_batched_shape", &e8qlib::SetBatchedShape, py::arg("qtile") = 0, py::arg("btile") = 0,
+          py::arg("grid") = 0, "Fix the batched scan's (pairs, batches) per item and grid; zeros: autotuned.");
+    m.def("search_context_info", [] {
+        const e8qlib::ContextInfo i = e8qlib::GetContextInfo();
+        py::dict d;
+        d["qtile"] = i.qtile;
+        d["btile"] = i.btile;
+        d["grid"] = i.grid;
+        d["graphs"] = i.graphs;
+        d["tunings"] = i.tunings;
+        d["captures"] = i.captures;
+        d["replays"] = i.replays;
+        return d;
+    });
     RegisterServe(m);
 }

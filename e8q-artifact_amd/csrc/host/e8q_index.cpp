@@ -54,6 +54,31 @@ e8q_ctx* DefaultContext() {
     return ctx;
 }
 
+void SetSearchGraphs(bool on) {
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    Check(e8q_ctx_set_graphs(DefaultContext(), on ? 1 : 0));
+}
+
+void SetBatchedShape(uint32_t qtile, uint32_t btile, uint32_t grid) {
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    Check(e8q_ctx_set_batched_shape(DefaultContext(), qtile, btile, grid));
+}
+
+ContextInfo GetContextInfo() {
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    e8q_ctx_info i{};
+    Check(e8q_ctx_get_info(DefaultContext(), &i));
+    ContextInfo o;
+    o.qtile = i.qtile;
+    o.btile = i.btile;
+    o.grid = i.grid;
+    o.graphs = i.graphs != 0;
+    o.tunings = i.tunings;
+    o.captures = i.captures;
+    o.replays = i.replays;
+    return o;
+}
+
 E8qIndex::E8qIndex() = default;
 
 E8qIndex::~E8qIndex() { Reset(); }

@@ -118,4 +118,15 @@ e8q_ctx* DefaultContext();
 // Serializes the calls made on that context (one host thread at a time).
 std::mutex& DeviceMutex();
 
+// Launch options of that context: graph replay of repeated search shapes,
+// and the batched form's scan shape (all zero: the context's autotuner).
+void SetSearchGraphs(bool on);
+void SetBatchedShape(uint32_t qtile, uint32_t btile, uint32_t grid);
+struct ContextInfo {
+    uint32_t qtile = 0, btile = 0, grid = 0;   // shape of the last batched search
+    bool graphs = false;
+    uint64_t tunings = 0, captures = 0, replays = 0;
+};
+ContextInfo GetContextInfo();
+
 }  // namespace e8qlib

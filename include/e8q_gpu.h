@@ -68,7 +68,8 @@ typedef struct {
                             a query searched again by the plain form after its
                             candidate list overflowed counts its first probe's) */
     uint64_t batches;    /* 32-vector scan records of the probed clusters */
-    double scan_ms;      /* scan kernel time, HIP events on the context stream */
+    double scan_ms;      /* scan kernel time, HIP events on the context stream
+                            (a graph replay: the whole graph)                   */
     double total_ms;     /* device time of the whole search, same events       */
     uint64_t scanned;    /* of those, records whose codes were read (the rest
                             belong to clusters passed over as a whole) */
@@ -106,6 +107,28 @@ e8q_status e8q_ctx_synchronize(e8q_ctx* ctx);
  * counterpart: the reference allocates per-thread host scratch). */
 e8q_status e8q_ctx_reserve(e8q_ctx* ctx, uint64_t bytes);
 uint64_t e8q_ctx_workspace_bytes(const e8q_ctx* ctx);
+
+/* Graph replay of e8q_search on this context (no reference counterpart: the
+ * reference makes no device launches).  With graphs on, the first search of
+ * a call shape (sizes, k, nprobe, buffers, workspace) runs eagerly, the
+ * second captures its launches from the query rotation to the results as one
+ * HIP graph and later ones replay it.  A replayed search reports the graph's
+ * time as both scan_ms and total_ms.  Off by default. */
+e8q_status e8q_ctx_set_graphs(e8q_ctx* ctx, int enable);
+/* Shape of the batched form's persistent scan: qtile (query, list) pairs and
+ * btile 32-vector batches per work item, grid workgroups.  qtile = btile = 0:
+ * the context's autotuner picks the tile (and, with grid = 0, the grid) per
+ * index shape and power-of-two query-count bucket by timing candidates on the
+ * first batched search of at least 64 queries.  Results do not depend on the
+ * shape. */
+e8q_status e8q_ctx_set_batched_shape(e8q_ctx* ctx, uint32_t qtile, uint32_t btile, uint32_t grid);
+typedef struct {
+    uint32_t qtile, btile, grid; /* shape of the last batched search */
+    uint32_t graphs;             /* e8q_ctx_set_graphs state          */
+    uint64_t tunings;            /* autotuner runs                    */
+    uint64_t captures, replays;  /* graph captures and launches       */
+} e8q_ctx_info;
+e8q_status e8q_ctx_get_info(const e8q_ctx* ctx, e8q_ctx_info* out);
 const char* e8q_last_error(void);
 
 /* ---- index lifetime --------------------------------------------------- */

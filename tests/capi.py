@@ -43,6 +43,11 @@ class SearchArgs(ctypes.Structure):
                 ("out_dists", vp), ("outputs_on_device", ctypes.c_int32)]
 
 
+class CtxInfo(ctypes.Structure):
+    _fields_ = [("qtile", ctypes.c_uint32), ("btile", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("graphs", ctypes.c_uint32), ("tunings", u64), ("captures", u64), ("replays", u64)]
+
+
 MODE_PLAIN, MODE_BATCHED = 0, 1
 
 _lib = None
@@ -89,6 +94,17 @@ class Ctx:
     def __init__(self, device=0):
         self.h = vp()
         check(lib().e8q_ctx_create(device, ctypes.byref(self.h)))
+
+    def set_graphs(self, on):
+        check(lib().e8q_ctx_set_graphs(self.h, int(bool(on))))
+
+    def set_batched_shape(self, qtile=0, btile=0, grid=0):
+        check(lib().e8q_ctx_set_batched_shape(self.h, qtile, btile, grid))
+
+    def info(self):
+        i = CtxInfo()
+        check(lib().e8q_ctx_get_info(self.h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in CtxInfo._fields_}
 
     def close(self):
         if self.h:

@@ -19,6 +19,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(capi.IndexDesc) == 4 * 8 + 4 + 4 + 8 + 7 * 8
     assert ctypes.sizeof(capi.IndexArrays) == 6 * 8
     assert ctypes.sizeof(capi.SearchArgs) == 8 * 4 + 8 + 4 * 5 + 4 + 8 + 8 + 8
+    assert ctypes.sizeof(capi.CtxInfo) == 4 * 4 + 3 * 8
 
 
 def test_me8q_struct_sizes():
