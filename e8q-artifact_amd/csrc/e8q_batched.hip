@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void PairCountKernel(const BatchedParams* __re
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total; i += gridDim.x * 256ull) {
         const uint64_t q = i / a.maxp;
         const uint32_t p = static_cast<uint32_t>(i - q * a.maxp);
-        if (p == 0 || p >= NprobeOf(ix, a, q)) continue;
+        if (p < a.p_lo || p >= NprobeOf(ix, a, q) || (a.qmask != nullptr && a.qmask[q] == 0)) continue;
         const uint32_t c = a.probe_ids[i];
         const uint32_t cnt = static_cast<uint32_t>(ix.cluster_begin[c + 1] - ix.cluster_begin[c]);
         if (cnt == 0) continue;
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void PairScatterKernel(const BatchedParams* __
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total; i += gridDim.x * 256ull) {
         const uint64_t q = i / a.maxp;
         const uint32_t p = static_cast<uint32_t>(i - q * a.maxp);
-        if (p == 0 || p >= NprobeOf(ix, a, q)) continue;
+        if (p < a.p_lo || p >= NprobeOf(ix, a, q) || (a.qmask != nullptr && a.qmask[q] == 0)) continue;
         const uint32_t c = a.probe_ids[i];
         if (ix.cluster_begin[c + 1] == ix.cluster_begin[c]) continue;
         const uint32_t pos = atomicAdd(&a.cfill[c], 1u);

@@ -109,6 +109,8 @@ struct BatchedArgs {
     uint32_t* out_ids;          // [nq][k]
     float* out_dists;
     unsigned long long* stats;  // [4] as PlainArgs
+    uint32_t p_lo;              // first probe position of the work list (1: the first probe is the threshold pass)
+    const uint8_t* qmask;       // optional [nq]: only queries with a nonzero entry get pairs
 };
 
 struct BatchedParams {
@@ -141,6 +143,43 @@ hipError_t LaunchBatchedScan(hipStream_t st, const IndexView& ix, const BatchedA
 // Refine of every query's candidates merged into its first-probe top-k.
 hipError_t LaunchBatchedRefine(hipStream_t st, const IndexView& ix, const BatchedArgs& a,
                                const BatchedParams* dparams);
+// ---- MFMA form (e8q_mfma.hip) ----------------------------------------------
+// Exact estimator over every probed vector as a split-BF16 MFMA contraction:
+// S(q, v) = sum_i q_i L(d_i) with the branch folded by the level table's
+// reflection (L1[d] = -L0[Q-1-d]), codes as one-hot {-1, 0, +1} rows of
+// K = 16 per dimension, the query side q_i L0[l] split into hi + lo bf16,
+// fp32 accumulation.
+constexpr uint32_t kMfmaQTile = 32;    // pairs per item (the MFMA's M)
+constexpr uint32_t kMfmaBTile = 8;     // batches per item (4 waves x 2 tiles)
+constexpr uint32_t kMfmaMaxBits = 4;   // 2^bits folded levels fit one K = 16 slice
+
+struct MfmaExtra {
+    const uint16_t* qexp;       // [nq][padded][32] bf16 bits: 16 hi, then 16 lo
+    const float* thr;           // [nq] candidate bound
+    const uint8_t* mcodes;      // [records][blocks][32][8] folded level | branch << 7
+    uint64_t* cand;             // [rows][cap] (monotone dist bits << 32) | id
+    uint32_t* ncand;            // [nq]
+    uint32_t cap;
+    const uint32_t* qslot;      // optional [nq]: candidate row of each query
+    const uint32_t* qsel;       // optional [nsel]: queries the select kernel handles
+    uint64_t nsel;
+};
+
+struct MfmaParams {
+    BatchedParams bp;   // first, so that the work-list kernels read it as theirs
+    MfmaExtra m;
+};
+
+// Folded one-hot code layout and per-list max |g1| of an index.
+hipError_t LaunchMfmaCodes(hipStream_t st, const IndexView& ix, uint64_t records, uint8_t* mcodes, float* g1max);
+// Per query: the hi/lo bf16 query expansion, and the candidate bound
+// thr = tau (first-probe k-th distance) + a bound of both forms' rounding.
+hipError_t LaunchMfmaQuery(hipStream_t st, const IndexView& ix, const float* rotq, uint64_t nq, uint32_t k,
+                           const float* first_dists, const uint32_t* probe_ids, uint32_t maxp, const float* g1max,
+                           float lmax, uint16_t* qexp, float* thr);
+hipError_t LaunchMfmaScan(hipStream_t st, const MfmaParams* dp, uint32_t grid);
+hipError_t LaunchMfmaSelect(hipStream_t st, const MfmaParams* dp, uint64_t nrun, uint32_t k);
+
 // Fill a device buffer with a 32-bit value (graph-safe, no memset nodes).
 hipError_t LaunchFill32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
 

@@ -1,0 +1,393 @@
+// e8q_mfma.hip -- the MFMA form of the E8Q search: the refine estimator of
+// every vector of every probed list as a split-BF16 matrix contraction.
+//
+// The reference scores a survivor with S = sum_i q_i * level[br][d_i] / sqrt(P)
+// (RefineCoreAvx2, e8q_search_avx2.cpp:82-403) and dist = h3' - 2 g1 S.  The
+// level table is not linear in the digit, but it is reflection-symmetric,
+// L1[d] = -L0[Q-1-d] (e8q_test.cpp:35-47, e8q_search_avx512.cpp:180-188), so
+// with  l_i = br ? Q-1-d_i : d_i  and  s_i = br ? -1 : +1
+//     S = sum_i sum_l  (q_i L0[l]) * (s_i [l == l_i])
+// -- a contraction over K = 16 P of the query expansion Qexp[q, 16 i + l] =
+// q_i L0[l] against one-hot code rows with entries in {-1, 0, +1}, exact in
+// BF16.  Qexp is split into hi + lo BF16 (hi = bf16(x), lo = bf16(x - hi)),
+// both contracted against the same code tile into one FP32 accumulator:
+// v_mfma_f32_32x32x16_bf16, one instruction per dimension (its K = 16 is one
+// dimension's folded levels) per 32 pairs x 32 vectors.  FP64 is not used at
+// search time; the build-time fits (g1, h3) are the reference's.
+//
+// Candidates: all vectors of the pair's list whose MFMA distance is within
+// thr_q = tau_q + margin_q, tau_q being the k-th fp32 refine distance of the
+// query's first probe (the threshold pass) and margin_q a bound of the two
+// forms' rounding difference for that list's vectors; the k vectors behind
+// tau_q therefore lie within thr_q in MFMA distance, so the top-k by MFMA
+// distance is among the candidates.  One wave per query then selects it.
+#include <algorithm>
+
+#include "e8q_scan_dev.h"
+
+namespace e8qdev {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Index side: folded codes, one byte per dimension (level index in bits 0-3,
+// branch in bit 7), [record][block][lane][8] so that a lane reads a block's
+// eight dimensions with one 8-byte load.  Decode as DecodeSplitBlockLane
+// (e8q_scan.cpp:34-90): d_i = rowA bit (7-i) << (b-1) | field_i; digit 7's
+// field LSB is the parity of digits 0..6's and its stored LSB the branch.
+
+__device__ __forceinline__ uint32_t ListOfRecord(const IndexView& ix, uint64_t rec) {
+    const uint64_t byte = rec * ix.batch_bytes;
+    uint64_t lo = 0, hi = ix.nlist;   // last c with arena_begin[c] <= byte
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (ix.arena_begin[mid] <= byte) lo = mid;
+        else hi = mid;
+    }
+    return static_cast<uint32_t>(lo);
+}
+
+__global__ __launch_bounds__(256) void MfmaCodesKernel(IndexView ix, uint64_t records, uint8_t* __restrict__ mcodes,
+                                                        float* __restrict__ g1max) {
+    const uint64_t tid = blockIdx.x * 256ull + threadIdx.x;
+    if (tid >= records * 32) return;
+    const uint64_t rec = tid >> 5;
+    const uint32_t t = static_cast<uint32_t>(tid & 31);
+    const uint32_t c = ListOfRecord(ix, rec);
+    const uint64_t b = (rec * ix.batch_bytes - ix.arena_begin[c]) / ix.batch_bytes;   // batch within the list
+    const uint64_t cnt = ix.cluster_begin[c + 1] - ix.cluster_begin[c];
+    const int B = ix.blocks, cb = ix.bits - 1;
+    const uint32_t Q = 1u << ix.bits;
+    const uint8_t* rowa = ix.arena + rec * ix.batch_bytes + kHdr + static_cast<uint64_t>(t) * B;
+    const uint8_t* rr = ix.ref_arena + ix.ref_arena_begin[c] + b * ix.ref_batch_bytes;
+    const uint8_t* field = rr + static_cast<uint64_t>(t) * ix.refine_stride;
+    uint8_t* out = mcodes + rec * 32ull * ix.padded + static_cast<uint64_t>(t) * 8;
+    for (int blk = 0; blk < B; ++blk) {
+        const uint32_t rb = rowa[blk];
+        uint32_t d[8], br;
+        if (ix.bits == 1) {
+            uint32_t par = 0;
+            for (int i = 0; i < 7; ++i) {
+                d[i] = (rb >> (7 - i)) & 1u;
+                par ^= d[i];
+            }
+            d[7] = par;
+            br = rb & 1u;
+        } else {
+            uint64_t f = 0;   // 8 fields of cb <= 3 bits
+            for (int i = 0; i < cb; ++i) f |= static_cast<uint64_t>(field[blk * cb + i]) << (8 * i);
+            const uint32_t m = (1u << cb) - 1u;
+            uint32_t par = 0;
+            for (int i = 0; i < 7; ++i) {
+                const uint32_t fi = static_cast<uint32_t>(f >> (cb * i)) & m;
+                par ^= fi & 1u;
+                d[i] = fi | (((rb >> (7 - i)) & 1u) << cb);
+            }
+            const uint32_t f7 = static_cast<uint32_t>(f >> (cb * 7)) & m;
+            br = f7 & 1u;
+            d[7] = (f7 & ~1u) | par | ((rb & 1u) << cb);
+        }
+        uint64_t w = 0;
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t l = br ? Q - 1u - d[i] : d[i];
+            w |= static_cast<uint64_t>(l | (br << 7)) << (8 * i);
+        }
+        *reinterpret_cast<uint64_t*>(out + static_cast<uint64_t>(blk) * 256) = w;
+    }
+    if (b * 32 + t < cnt) {   // |g1| of stored vectors; non-negative floats order as their bits
+        const float g1 = reinterpret_cast<const float*>(rr + 32ull * ix.refine_stride)[t];
+        atomicMax(reinterpret_cast<uint32_t*>(g1max) + c, __float_as_uint(fabsf(g1)));
+    }
+}
+
+hipError_t LaunchMfmaCodes(hipStream_t st, const IndexView& ix, uint64_t records, uint8_t* mcodes, float* g1max) {
+    if (records == 0) return hipSuccess;
+    if (ix.bits < 1 || ix.bits > static_cast<int>(kMfmaMaxBits)) return hipErrorInvalidValue;
+    const hipError_t e = LaunchFill32(st, reinterpret_cast<uint32_t*>(g1max), ix.nlist, 0);
+    if (e != hipSuccess) return e;
+    const uint64_t n = records * 32;
+    hipLaunchKernelGGL(MfmaCodesKernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, st, ix, records,
+                       mcodes, g1max);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Query side.
+
+// Round to nearest even (no NaN reaches here).
+__device__ __forceinline__ uint32_t Bf16Bits(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// Qexp rows: thread per (query, dimension), 32 bf16 = 64 bytes.
+__global__ __launch_bounds__(256) void MfmaQexpKernel(const float* __restrict__ rotq, uint64_t nq, uint32_t P,
+                                                       const float* __restrict__ tab, uint32_t Q,
+                                                       uint16_t* __restrict__ qexp) {
+    const uint64_t tid = blockIdx.x * 256ull + threadIdx.x;
+    if (tid >= nq * P) return;
+    const float qi = rotq[tid];
+    uint32_t hi[8], lo[8];
+#pragma unroll
+    for (int l2 = 0; l2 < 8; ++l2) {
+        uint32_t h2 = 0, o2 = 0;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const uint32_t l = 2 * l2 + u;
+            const float x = l < Q ? qi * tab[l] : 0.0F;
+            const uint32_t hb = Bf16Bits(x);
+            const uint32_t lb = Bf16Bits(x - __uint_as_float(hb << 16));
+            h2 |= hb << (16 * u);
+            o2 |= lb << (16 * u);
+        }
+        hi[l2] = h2;
+        lo[l2] = o2;
+    }
+    u32x4* o = reinterpret_cast<u32x4*>(qexp + tid * 32);
+    o[0] = u32x4{hi[0], hi[1], hi[2], hi[3]};
+    o[1] = u32x4{hi[4], hi[5], hi[6], hi[7]};
+    o[2] = u32x4{lo[0], lo[1], lo[2], lo[3]};
+    o[3] = u32x4{lo[4], lo[5], lo[6], lo[7]};
+}
+
+// thr_q = tau + margin, one wave per query.  Both the MFMA score and the
+// fp32 refine chain are within (2^-15 + (3P + 32) 2^-24) * sum_i |q_i| Lmax of
+// the exact S (split residual 2^-17 per term, fp32 rounding of the products
+// and of at most 2P + 16 accumulations, generously rounded up), and the
+// distance moves by 2 |g1| times that; |g1| is bounded by the first list's
+// max.  2^-20 |tau| covers the final fused multiply-add.
+__global__ __launch_bounds__(kWave) void MfmaThrKernel(const float* __restrict__ rotq, uint32_t P, uint32_t k,
+                                                        const float* __restrict__ first_dists,
+                                                        const uint32_t* __restrict__ probe_ids, uint32_t maxp,
+                                                        const float* __restrict__ g1max, float lmax,
+                                                        float* __restrict__ thr) {
+    const uint64_t q = blockIdx.x;
+    float s = 0.0F;
+    for (uint32_t i = threadIdx.x; i < P; i += kWave) s += fabsf(rotq[q * P + i]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (threadIdx.x == 0) {
+        const float tau = first_dists[q * k + k - 1];
+        const float eps = 3.0517578e-05F + static_cast<float>(3 * P + 32) * 5.9604645e-08F;
+        const float g1 = g1max[probe_ids[q * maxp]];
+        const float margin = 2.0F * g1 * (s * 1.001F) * lmax * eps + fabsf(tau) * 9.5367432e-07F;
+        thr[q] = tau + margin;   // +inf when the first list holds fewer than k vectors
+    }
+}
+
+hipError_t LaunchMfmaQuery(hipStream_t st, const IndexView& ix, const float* rotq, uint64_t nq, uint32_t k,
+                           const float* first_dists, const uint32_t* probe_ids, uint32_t maxp, const float* g1max,
+                           float lmax, uint16_t* qexp, float* thr) {
+    if (nq == 0) return hipSuccess;
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    const uint64_t n = nq * P;
+    hipLaunchKernelGGL(MfmaQexpKernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, st, rotq, nq, P,
+                       ix.refine_tab, 1u << ix.bits, qexp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(MfmaThrKernel, dim3(static_cast<uint32_t>(nq)), dim3(kWave), 0, st, rotq, P, k, first_dists,
+                       probe_ids, maxp, g1max, lmax, thr);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The contraction.  Items of the batched work list with 32 pairs x 8
+// batches; four waves per workgroup, each scoring 32 pairs x two 32-vector
+// tiles.  Per dimension a lane supplies 8 bf16 of the A operand (query row
+// lane % 32, levels 8 (lane / 32) .. +7, hi and lo) and 8 of the B operand
+// (vector column lane % 32, the same levels); accumulator register r holds
+// row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32.
+
+__device__ __forceinline__ u32x4 OneHot(uint32_t code, uint32_t h) {
+    const uint32_t local = (code & 15u) - 8u * h;   // wraps above 7 when outside this half
+    const uint32_t v = local < 8u ? ((0x3F80u | ((code >> 7) << 15)) << ((local & 1u) << 4)) : 0u;
+    const uint32_t dw = local >> 1;
+    return u32x4{dw == 0 ? v : 0u, dw == 1 ? v : 0u, dw == 2 ? v : 0u, dw == 3 ? v : 0u};
+}
+
+__device__ __forceinline__ f32x16 Mfma(u32x4 a, u32x4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+__device__ __forceinline__ uint64_t DistKey(float d, uint32_t id) {
+    const uint32_t u = __float_as_uint(d);
+    const uint32_t m = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return (static_cast<uint64_t>(m) << 32) | id;
+}
+
+__device__ __forceinline__ float KeyDist(uint64_t key) {
+    const uint32_t m = static_cast<uint32_t>(key >> 32);
+    return __uint_as_float((m & 0x80000000u) ? (m & 0x7FFFFFFFu) : ~m);
+}
+
+constexpr int kMfmaThreads = 256;
+
+__global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams* __restrict__ prm) {
+    const IndexView& ix = prm->bp.ix;
+    const BatchedArgs& a = prm->bp.a;
+    const MfmaExtra& m = prm->m;
+    __shared__ uint32_t s_it, s_q[kMfmaQTile];
+    __shared__ float s_d2[kMfmaQTile], s_thr[kMfmaQTile];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t h = lane >> 5, n = lane & 31;
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    const int B = ix.blocks;
+    const uint32_t nitems = Uniform(static_cast<uint32_t>(min<uint64_t>(*G(a.counters), a.items_cap)));
+    unsigned long long staged = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_it = atomicAdd(&a.counters[1], 1u);
+        __syncthreads();
+        const uint32_t it = Uniform(s_it);
+        if (it >= nitems) break;
+        const E8Q_GLOBAL BatchItem* item = G(a.items) + it;
+        const uint32_t c = Uniform(item->c), np = Uniform(item->np), nb = Uniform(item->nb), b0 = Uniform(item->b0);
+        const uint32_t p0 = Uniform(item->p0);
+        if (threadIdx.x < np) {
+            const uint64_t pr = G(a.pairs)[p0 + threadIdx.x];
+            const uint32_t q = static_cast<uint32_t>(pr >> 32), p = static_cast<uint32_t>(pr);
+            s_q[threadIdx.x] = q;
+            s_d2[threadIdx.x] = G(a.probe_d2)[static_cast<uint64_t>(q) * a.maxp + p];
+            s_thr[threadIdx.x] = G(m.thr)[q];
+        }
+        __syncthreads();
+        if (wave < nb) {   // wave-uniform: this wave's tiles are batches wave and wave + 4
+            const bool two = wave + 4 < nb;
+            const uint64_t ccnt = G(ix.cluster_begin)[c + 1] - G(ix.cluster_begin)[c];
+            const uint64_t rec0 = G(ix.arena_begin)[c] / ix.batch_bytes + b0;
+            const E8Q_GLOBAL uint8_t* code0 = G(m.mcodes) + (rec0 + wave) * 32ull * P + n * 8;
+            const E8Q_GLOBAL uint8_t* code1 = G(m.mcodes) + (rec0 + (two ? wave + 4 : wave)) * 32ull * P + n * 8;
+            const bool qv = n < np;
+            const E8Q_GLOBAL uint16_t* qa = G(m.qexp) + static_cast<uint64_t>(s_q[qv ? n : 0]) * P * 32 + 8 * h;
+            f32x16 acc0 = {}, acc1 = {};
+            for (int blk = 0; blk < B; ++blk) {
+                const uint64_t w0 = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(code0 + blk * 256);
+                const uint64_t w1 = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(code1 + blk * 256);
+                const E8Q_GLOBAL u32x4* ap = reinterpret_cast<const E8Q_GLOBAL u32x4*>(qa + blk * 8 * 32);
+                u32x4 ahi[8], alo[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    ahi[e] = ap[e * 4];
+                    alo[e] = ap[e * 4 + 2];
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const u32x4 zero = {0u, 0u, 0u, 0u};
+                    const u32x4 xh = qv ? ahi[e] : zero, xl = qv ? alo[e] : zero;
+                    const u32x4 b0v = OneHot(static_cast<uint32_t>(w0 >> (8 * e)) & 0xFFu, h);
+                    if (two) {   // interleaved, so that no MFMA waits on the previous one's result
+                        const u32x4 b1v = OneHot(static_cast<uint32_t>(w1 >> (8 * e)) & 0xFFu, h);
+                        acc0 = Mfma(xh, b0v, acc0);
+                        acc1 = Mfma(xh, b1v, acc1);
+                        acc0 = Mfma(xl, b0v, acc0);
+                        acc1 = Mfma(xl, b1v, acc1);
+                    } else {
+                        acc0 = Mfma(xh, b0v, acc0);
+                        acc0 = Mfma(xl, b0v, acc0);
+                    }
+                }
+            }
+            // Epilogue: distances, the candidate test, appends.
+            for (int tile = 0; tile < (two ? 2 : 1); ++tile) {
+                const uint32_t bj = b0 + wave + 4 * tile;   // batch within the list
+                const uint64_t vo = static_cast<uint64_t>(bj) * 32 + n;
+                if (vo >= ccnt) continue;
+                const E8Q_GLOBAL uint8_t* rr =
+                    G(ix.ref_arena) + G(ix.ref_arena_begin)[c] + static_cast<uint64_t>(bj) * ix.ref_batch_bytes;
+                const E8Q_GLOBAL float* g1p = reinterpret_cast<const E8Q_GLOBAL float*>(rr + 32ull * ix.refine_stride);
+                const float g1 = g1p[n], h3 = g1p[32 + n];
+                const uint32_t id = G(ix.ids)[G(ix.cluster_begin)[c] + vo];
+                const f32x16 acc = tile ? acc1 : acc0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const uint32_t row = 8 * (r / 4) + 4 * h + (r % 4);
+                    if (row >= np) continue;
+                    const float d = fmaf(-(2.0F * g1), acc[r], s_d2[row] + h3) + 0.0F;   // -0 -> +0
+                    if (!(d <= s_thr[row])) continue;
+                    const uint32_t q = s_q[row];
+                    const uint32_t slot = atomicAdd(&m.ncand[q], 1u);
+                    const uint64_t crow = m.qslot ? G(m.qslot)[q] : q;
+                    if (slot < m.cap) Gw(m.cand)[crow * m.cap + slot] = DistKey(d, id);
+                }
+            }
+        }
+        if (threadIdx.x == 0) staged += nb;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && a.stats) atomicAdd(&a.stats[3], staged);
+}
+
+hipError_t LaunchMfmaScan(hipStream_t st, const MfmaParams* dp, uint32_t grid) {
+    hipLaunchKernelGGL(MfmaScanKernel, dim3(grid), dim3(kMfmaThreads), 0, st, dp);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Top-k of each query's candidates, one wave per query: a bitonic sort in
+// LDS of the best k so far plus the next chunk, repeated over the list.
+
+constexpr uint32_t kSelChunk = 1024;
+
+__global__ __launch_bounds__(kWave) void MfmaSelectKernel(const MfmaParams* __restrict__ prm) {
+    const BatchedArgs& a = prm->bp.a;
+    const MfmaExtra& m = prm->m;
+    __shared__ uint64_t buf[kSelChunk];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t k = a.k;
+    const uint64_t nrun = m.qsel ? m.nsel : a.nq;
+    for (uint64_t j = blockIdx.x; j < nrun; j += gridDim.x) {
+        const uint32_t q = m.qsel ? G(m.qsel)[j] : static_cast<uint32_t>(j);
+        const uint32_t n = Uniform(*G(m.ncand + q));
+        if (n > m.cap) {   // searched again with a longer list
+            if (lane == 0) a.overflow[atomicAdd(&a.counters[2], 1u)] = q;
+            continue;
+        }
+        const uint64_t crow = m.qslot ? G(m.qslot)[q] : q;
+        const E8Q_GLOBAL uint64_t* cl = G(m.cand) + crow * m.cap;
+        // chunk size: a power of two >= k + min(n, room), at least 64
+        uint32_t ch = 64;
+        while (ch < kSelChunk && ch < k + n) ch <<= 1;
+        const uint32_t room = ch - k;
+        for (uint32_t i = lane; i < k; i += kWave) buf[i] = ~0ull;
+        uint32_t s = 0;
+        do {
+            const uint32_t take = min(room, n - s);
+            for (uint32_t i = lane; i < room; i += kWave) buf[k + i] = i < take ? cl[s + i] : ~0ull;
+            __syncthreads();
+            for (uint32_t size = 2; size <= ch; size <<= 1)
+                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (uint32_t i = lane; i < ch / 2; i += kWave) {
+                        const uint32_t lo = 2 * i - (i & (stride - 1));
+                        const uint32_t hi = lo + stride;
+                        const bool up = (lo & size) == 0;
+                        const uint64_t x = buf[lo], y = buf[hi];
+                        if ((x > y) == up) {
+                            buf[lo] = y;
+                            buf[hi] = x;
+                        }
+                    }
+                    __syncthreads();
+                }
+            s += take;
+        } while (s < n);
+        for (uint32_t i = lane; i < k; i += kWave) {
+            const uint64_t key = buf[i];
+            const bool ok = key != ~0ull;
+            Gw(a.out_ids)[static_cast<uint64_t>(q) * k + i] = ok ? static_cast<uint32_t>(key) : 0xFFFFFFFFu;
+            Gw(a.out_dists)[static_cast<uint64_t>(q) * k + i] = ok ? KeyDist(key) : __builtin_inff();
+        }
+        if (lane == 0 && a.stats) atomicAdd(&a.stats[1], static_cast<unsigned long long>(n));
+        __syncthreads();
+    }
+}
+
+hipError_t LaunchMfmaSelect(hipStream_t st, const MfmaParams* dp, uint64_t nrun, uint32_t k) {
+    if (nrun == 0) return hipSuccess;
+    if (k == 0 || k > 512) return hipErrorInvalidValue;
+    const uint32_t grid = static_cast<uint32_t>(nrun < 262144 ? nrun : 262144);
+    hipLaunchKernelGGL(MfmaSelectKernel, dim3(grid), dim3(kWave), 0, st, dp);
+    return hipGetLastError();
+}
+
+}  // namespace e8qdev

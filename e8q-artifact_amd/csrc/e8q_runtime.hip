@@ -33,11 +33,19 @@ struct e8q_dindex {
     std::vector<uint64_t> cluster_begin, arena_begin, ref_arena_begin;
     DBuf signs, mean_rot, centroid_rot, centroid_n2, d_cluster_begin, d_arena_begin, d_ref_arena_begin, arena,
         ref_arena, ids, refine_tab, gate_mid, levels, mlev, mdig, cls_lev, cls_dig, csum;
+    // MFMA form: folded one-hot code layout and per-list max |g1|, built on
+    // its first search and dropped when the arrays change.
+    struct Mfma {
+        DBuf codes, g1max;
+        float lmax = 0.0F;
+        bool ready = false;
+    };
+    mutable Mfma mfma;
     uint64_t device_bytes() const {
         return signs.bytes + mean_rot.bytes + centroid_rot.bytes + centroid_n2.bytes + d_cluster_begin.bytes +
                d_arena_begin.bytes + d_ref_arena_begin.bytes + arena.bytes + ref_arena.bytes + ids.bytes +
                refine_tab.bytes + gate_mid.bytes + levels.bytes + mlev.bytes + mdig.bytes + cls_lev.bytes +
-               cls_dig.bytes + csum.bytes;
+               cls_dig.bytes + csum.bytes + mfma.codes.bytes + mfma.g1max.bytes;
     }
 };
 
@@ -139,6 +147,37 @@ std::vector<ScanShape> ShapeCandidates(const IndexView& v, uint32_t grid) {
 }
 
 constexpr uint64_t kTuneMinQueries = 64;   // smaller calls use the default shape untuned
+constexpr uint32_t kMfmaCap = 2048;         // MFMA-form candidates kept per query (more: a second pass)
+
+// Builds the MFMA form's code layout of an index on its first MFMA search.
+// The fold needs the refine table's reflection symmetry, checked here.
+void PrepareMfma(e8q_ctx* ctx, const e8q_dindex* x) {
+    if (x->mfma.ready) return;
+    const IndexView& v = x->view;
+    if (v.bits < 1 || v.bits > static_cast<int>(kMfmaMaxBits)) Throw(E8Q_EINVAL, "the MFMA form needs bits <= 4");
+    hipStream_t st = ctx->stream;
+    const uint32_t Q = 1u << v.bits;
+    std::vector<float> tab(2ull * v.tab_stride);
+    Check(hipMemcpyAsync(tab.data(), v.refine_tab, tab.size() * 4, hipMemcpyDeviceToHost, st), "download table");
+    Check(hipStreamSynchronize(st), "download table");
+    float lmax = 0.0F;
+    for (uint32_t d = 0; d < Q; ++d) {
+        if (tab[v.tab_stride + d] != -tab[Q - 1 - d])
+            Throw(E8Q_EINVAL, "the MFMA form needs a reflection-symmetric level table");
+        lmax = std::max(lmax, std::fabs(tab[d]));
+    }
+    for (uint64_t c = 0; c < v.nlist; ++c)
+        if (x->arena_begin[c] % v.batch_bytes != 0) Throw(E8Q_EINVAL, "scan records are not record-aligned");
+    const uint64_t records = x->records();
+    e8q_dindex::Mfma m;
+    m.codes.alloc(records * 32 * v.padded);
+    m.g1max.alloc(v.nlist * 4);
+    Check(LaunchMfmaCodes(st, v, records, m.codes.as<uint8_t>(), m.g1max.as<float>()), "mfma codes");
+    Check(hipStreamSynchronize(st), "mfma codes");
+    m.lmax = lmax;
+    m.ready = true;
+    x->mfma = std::move(m);
+}
 
 }  // namespace
 
@@ -285,6 +324,7 @@ uint64_t e8q_index_device_bytes(const e8q_dindex* idx) { return idx ? idx->devic
 
 e8q_status e8q_index_upload(e8q_ctx* ctx, e8q_dindex* x, const e8q_index_arrays* h) {
     return Guard([&] {
+        if (x) x->mfma = e8q_dindex::Mfma{};
         if (!ctx || !x || !h || !h->ids || !h->mean_rot || !h->centroid_rot || !h->centroid_n2 || !h->arena ||
             !h->ref_arena)
             Throw(E8Q_EINVAL, "null argument");
@@ -336,6 +376,7 @@ e8q_status e8q_index_download(e8q_ctx* ctx, const e8q_dindex* x, e8q_index_array
 e8q_status e8q_index_encode(e8q_ctx* ctx, e8q_dindex* x, const float* base, const float* centroids,
                             const uint32_t* cluster_ids, const uint32_t* slot, int on_device) {
     return Guard([&] {
+        if (x) x->mfma = e8q_dindex::Mfma{};
         if (!ctx || !x || !base || !centroids || !cluster_ids || !slot) Throw(E8Q_EINVAL, "null argument");
         const IndexView& v = x->view;
         if (v.n == 0) Throw(E8Q_EINVAL, "empty build set");
@@ -455,7 +496,8 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
         const int flush = a->flush_every > 0 ? a->flush_every : 32;
         if (flush > 256) Throw(E8Q_EINVAL, "flush_every must be <= 256");
-        if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "unknown search mode");
+        if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED && a->mode != E8Q_MODE_MFMA)
+            Throw(E8Q_EINVAL, "unknown search mode");
         const IndexView& v = x->view;
         const uint64_t nq = a->nq;
         if (stats) *stats = e8q_search_stats{0, 0, 0, 0.0, 0.0, 0};
@@ -476,6 +518,9 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         if (sel_lds > 160 * 1024) Throw(E8Q_EINVAL, "nlist too large for the probe selector");
         ctx->Bind();
         hipStream_t st = ctx->stream;
+        const bool batched = a->mode == E8Q_MODE_BATCHED, mfma = a->mode == E8Q_MODE_MFMA;
+        const bool listed = batched || mfma;   // forms over the (query, list) work list
+        if (mfma) PrepareMfma(ctx, x);
         const uint64_t P = v.padded, E = static_cast<uint64_t>(v.blocks) * 32;
         float *dq = nullptr, *rotq = nullptr, *consts = nullptr, *dist = nullptr, *pd2 = nullptr, *odist = nullptr;
         uint8_t* lut = nullptr;
@@ -497,10 +542,15 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         }
         cv.add(&dstats, 4);
         cv.add(&dparams, 1);
-        // Batched form: its scan shape (fixed, tuned before, or to be tuned
-        // now), first-probe lists, work list, candidates.
-        const bool batched = a->mode == E8Q_MODE_BATCHED;
+        // Batched and MFMA forms: the scan shape (batched: fixed, tuned
+        // before, or to be tuned now), first-probe lists, work list,
+        // candidates.
         BatchedArgs ba{};
+        MfmaExtra mx{};
+        MfmaParams* mparams = nullptr;
+        uint16_t* m_qexp = nullptr;
+        float* m_thr = nullptr;
+        uint64_t* m_cand = nullptr;
         BatchedParams* bparams = nullptr;
         uint32_t* first_ids = nullptr;
         float* first_dists = nullptr;
@@ -508,7 +558,9 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         ScanShape shape{0, 0, 0};
         std::vector<ScanShape> cands;   // non-empty: tune in this call
         const uint64_t tkey[4] = {v.batch_bytes, static_cast<uint64_t>(v.blocks), v.nlist, QueryBucket(nq)};
-        if (batched) {
+        if (mfma) {
+            shape = {kMfmaQTile, kMfmaBTile, static_cast<uint32_t>(std::max(1, ctx->cus) * 2)};
+        } else if (batched) {
             const uint32_t grid0 = static_cast<uint32_t>(std::max(1, ctx->cus) * 2);
             if (ctx->fix_qtile != 0) {
                 shape = {ctx->fix_qtile, ctx->fix_btile, ctx->fix_grid ? ctx->fix_grid : grid0};
@@ -528,17 +580,20 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
                 }
                 if (ctx->fix_grid != 0) shape.grid = ctx->fix_grid;
             }
+        }
+        if (listed) {
+            const uint64_t p_lo = batched ? 1 : 0;   // the MFMA form scores the first probe too
             if (a->nprobe_per_query) {
                 for (uint64_t i = 0; i < nq; ++i)
-                    pairs += std::min<uint64_t>(a->nprobe_per_query[i], v.nlist) - 1;
+                    pairs += std::min<uint64_t>(a->nprobe_per_query[i], v.nlist) - p_lo;
             } else {
-                pairs = nq * (maxp - 1);
+                pairs = nq * (maxp - p_lo);
             }
             uint64_t items_cap = ItemsCap(x, nq, pairs, shape.qtile, shape.btile);
             for (const ScanShape& c : cands) items_cap = std::max(items_cap, ItemsCap(x, nq, pairs, c.qtile, c.btile));
             ba.qtile = shape.qtile;
             ba.btile = shape.btile;
-            ba.cap = kBatchCandCap;
+            ba.cap = batched ? kBatchCandCap : kMfmaCap;
             cv.add(&first_ids, nq * a->k);
             cv.add(&first_dists, nq * a->k);
             cv.add(&ba.ccount, v.nlist);
@@ -549,13 +604,29 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             cv.add(&ba.items, std::max<uint64_t>(items_cap, 1));
             ba.items_cap = ItemsCap(x, nq, pairs, shape.qtile, shape.btile);
             cv.add(&ba.counters, 4);
-            cv.add(&ba.cand, nq * ba.cap);
             cv.add(&ba.ncand, nq);
             cv.add(&ba.overflow, nq);
-            cv.add(&bparams, 1);
-            if (!cands.empty()) cv.add(&tstats, 4);
+            if (batched) {
+                cv.add(&ba.cand, nq * ba.cap);
+                cv.add(&bparams, 1);
+            } else {
+                cv.add(&m_qexp, nq * P * 32);
+                cv.add(&m_thr, nq);
+                cv.add(&m_cand, nq * kMfmaCap);
+                cv.add(&mparams, 1);
+            }
+            if (!cands.empty() || mfma) cv.add(&tstats, 4);
         }
         ctx->Take(cv);
+        if (mfma) {   // carve pointers are final only after Take
+            bparams = reinterpret_cast<BatchedParams*>(mparams);   // its first member
+            mx.qexp = m_qexp;
+            mx.thr = m_thr;
+            mx.cand = m_cand;
+            mx.mcodes = x->mfma.codes.as<uint8_t>();
+            mx.ncand = ba.ncand;
+            mx.cap = kMfmaCap;
+        }
         const float* qsrc = a->queries_on_device ? a->queries : dq;
         if (a->outputs_on_device) {
             oids = a->out_ids;
@@ -585,7 +656,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         fa.nprobe_all = 1;
         fa.out_ids = first_ids;
         fa.out_dists = first_dists;
-        if (batched) {
+        if (listed) {
             ba.first_ids = first_ids;
             ba.first_dists = first_dists;
             ba.nq = nq;
@@ -603,7 +674,10 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             ba.out_ids = oids;
             ba.out_dists = odist;
             ba.stats = dstats;
+            ba.p_lo = mfma ? 0 : 1;
+            ba.qmask = nullptr;
         }
+        if (mfma) fa.stats = tstats;   // the work list counts the first probe's candidates
         auto batched_tail = [&](const BatchedArgs& b, uint32_t grid, bool upload) {
             Check(LaunchFill32(st, b.ccount, v.nlist, 0), "fill");
             Check(LaunchFill32(st, b.ncand, nq, 0), "fill");
@@ -672,11 +746,26 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq, nprobe_all, maxp, pids, pd2),
                   "select probes");
             if (eager) Check(hipEventRecord(ctx->ev[1], st), "event");
-            if (!batched) {
+            if (!listed) {
                 Check(LaunchPlainSearch(st, v, pa, dparams, upload), "plain search");
                 return;
             }
             Check(LaunchPlainSearch(st, v, fa, dparams, upload), "first-probe search");
+            if (mfma) {
+                Check(LaunchMfmaQuery(st, v, rotq, nq, static_cast<uint32_t>(a->k), first_dists, pids, maxp,
+                                      x->mfma.g1max.as<float>(), x->mfma.lmax, m_qexp, m_thr),
+                      "mfma query");
+                if (upload) {
+                    const MfmaParams hm{{v, ba}, mx};
+                    Check(hipMemcpyAsync(mparams, &hm, sizeof(hm), hipMemcpyHostToDevice, st), "upload parameters");
+                }
+                Check(LaunchFill32(st, ba.ccount, v.nlist, 0), "fill");
+                Check(LaunchFill32(st, ba.ncand, nq, 0), "fill");
+                Check(LaunchBatchedWorkList(st, v, ba, bparams, false), "work list");
+                Check(LaunchMfmaScan(st, mparams, shape.grid), "mfma scan");
+                Check(LaunchMfmaSelect(st, mparams, nq, static_cast<uint32_t>(a->k)), "mfma select");
+                return;
+            }
             if (eager && !cands.empty()) tune();
             batched_tail(ba, shape.grid, upload);
         };
@@ -714,11 +803,14 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
         if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
         if (ge) {
-            const PlainParams hp{v, batched ? fa : pa};
+            const PlainParams hp{v, listed ? fa : pa};
             Check(hipMemcpyAsync(dparams, &hp, sizeof(hp), hipMemcpyHostToDevice, st), "upload parameters");
             if (batched) {
                 const BatchedParams hb{v, ba};
                 Check(hipMemcpyAsync(bparams, &hb, sizeof(hb), hipMemcpyHostToDevice, st), "upload parameters");
+            } else if (mfma) {
+                const MfmaParams hm{{v, ba}, mx};
+                Check(hipMemcpyAsync(mparams, &hm, sizeof(hm), hipMemcpyHostToDevice, st), "upload parameters");
             }
             if (!ge->exec) {
                 Check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -747,7 +839,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             Check(hipEventRecord(ctx->ev[0], st), "event");
             device_seq(true, true);
         }
-        if (batched) {
+        if (listed) {
             ctx->last_qtile = shape.qtile;
             ctx->last_btile = shape.btile;
             ctx->last_grid = shape.grid;
@@ -765,6 +857,50 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
                 Check(LaunchPlainSearch(st, v, oa, dparams), "plain search of overflowed queries");
             }
         }
+        DBuf r_cand, r_slot, r_mask, r_sel;   // the MFMA form's second pass (held until the final sync)
+        uint64_t extra_survivors = 0;
+        if (mfma) {   // queries with more candidates than a list holds: again, with lists of their length
+            uint32_t novf = 0;
+            Check(hipMemcpyAsync(&novf, ba.counters + 2, 4, hipMemcpyDeviceToHost, st), "download counters");
+            Check(hipStreamSynchronize(st), "mfma search");
+            if (novf > nq) Throw(E8Q_ERUNTIME, "mfma search: overflow count exceeds the query count");
+            if (novf > 0) {
+                std::vector<uint32_t> ovf(novf), cnt(nq);
+                Check(hipMemcpyAsync(ovf.data(), ba.overflow, novf * 4ull, hipMemcpyDeviceToHost, st), "download");
+                Check(hipMemcpyAsync(cnt.data(), ba.ncand, nq * 4, hipMemcpyDeviceToHost, st), "download");
+                Check(hipStreamSynchronize(st), "download");
+                uint32_t cap2 = 0;
+                std::vector<uint32_t> slot(nq, 0xFFFFFFFFu);
+                std::vector<uint8_t> mask(nq, 0);
+                for (uint32_t j = 0; j < novf; ++j) {
+                    if (ovf[j] >= nq) Throw(E8Q_ERUNTIME, "mfma search: bad overflow entry");
+                    extra_survivors += cnt[ovf[j]];
+                    cap2 = std::max(cap2, cnt[ovf[j]]);
+                    slot[ovf[j]] = j;
+                    mask[ovf[j]] = 1;
+                }
+                r_cand.alloc(static_cast<uint64_t>(novf) * cap2 * 8);
+                Upload(r_slot, slot.data(), nq, st);
+                Upload(r_mask, mask.data(), nq, st);
+                Upload(r_sel, ovf.data(), novf, st);
+                BatchedArgs b2 = ba;
+                b2.qmask = r_mask.as<uint8_t>();
+                b2.stats = tstats;   // counted already
+                MfmaExtra m2 = mx;
+                m2.cand = r_cand.as<uint64_t>();
+                m2.cap = cap2;
+                m2.qslot = r_slot.as<uint32_t>();
+                m2.qsel = r_sel.as<uint32_t>();
+                m2.nsel = novf;
+                const MfmaParams hm{{v, b2}, m2};
+                Check(hipMemcpyAsync(mparams, &hm, sizeof(hm), hipMemcpyHostToDevice, st), "upload parameters");
+                Check(LaunchFill32(st, ba.ccount, v.nlist, 0), "fill");
+                Check(LaunchFill32(st, ba.ncand, nq, 0), "fill");
+                Check(LaunchBatchedWorkList(st, v, b2, bparams, false), "work list");
+                Check(LaunchMfmaScan(st, mparams, shape.grid), "mfma scan");
+                Check(LaunchMfmaSelect(st, mparams, novf, static_cast<uint32_t>(a->k)), "mfma select");
+            }
+        }
         if (!a->outputs_on_device) {
             Check(hipMemcpyAsync(a->out_ids, oids, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
             Check(hipMemcpyAsync(a->out_dists, odist, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");
@@ -778,7 +914,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             Check(hipEventElapsedTime(&ms_scan, ctx->ev[1], ctx->ev[2]), "event time");
             Check(hipEventElapsedTime(&ms_all, ctx->ev[0], ctx->ev[2]), "event time");
             stats->candidates = hs[0];
-            stats->survivors = hs[1];
+            stats->survivors = hs[1] + extra_survivors;
             stats->batches = hs[2];
             stats->scanned = hs[3];
             stats->scan_ms = ms_scan;

@@ -3,7 +3,7 @@
 // E8qIndex.  Same constructor, method and argument names, same validation
 // order and messages, same sentinel-prefilled (ids uint32, dists float32)
 // outputs and stats dict.  Extra keyword arguments: `mode` ("plain" |
-// "batched") and `nprobe_per_query`.  `num_threads` is accepted and unused:
+// "batched" | "mfma") and `nprobe_per_query`.  `num_threads` is accepted and unused:
 // the device sizes its own parallelism.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -34,7 +34,8 @@ py::array_t<T, py::array::c_style | py::array::forcecast> Ensure(py::handle v, c
 e8qlib::SearchMode ParseMode(const std::string& m) {
     if (m == "plain") return e8qlib::SearchMode::kPlain;
     if (m == "batched") return e8qlib::SearchMode::kBatched;
-    throw std::invalid_argument("mode must be 'plain' or 'batched'");
+    if (m == "mfma") return e8qlib::SearchMode::kMfma;
+    throw std::invalid_argument("mode must be 'plain', 'batched' or 'mfma'");
 }
 
 class PyE8qIndex {

@@ -364,7 +364,9 @@ void E8qIndex::SearchBatch(const float* queries, size_t nq, size_t k, size_t npr
     a.z = hostmath::OrthoQuantile(alpha_, std::max<size_t>(4 * k, 32));
     a.disable_prune = disable_prune_ ? 1 : 0;
     a.flush_every = hostmath::FlushEvery();
-    a.mode = mode == SearchMode::kBatched ? E8Q_MODE_BATCHED : E8Q_MODE_PLAIN;
+    a.mode = mode == SearchMode::kBatched ? E8Q_MODE_BATCHED
+             : mode == SearchMode::kMfma  ? E8Q_MODE_MFMA
+                                          : E8Q_MODE_PLAIN;
     a.queries_on_device = queries_on_device ? 1 : 0;
     a.out_ids = out_ids;
     a.out_dists = out_dists;

@@ -47,7 +47,10 @@ class QueryScratch {
     bool disable_prune = false;
 };
 
-enum class SearchMode { kPlain = 0, kBatched = 1 };
+// kMfma: the unpruned estimator over every probed vector as a split-BF16
+// MFMA contraction (bits <= 4); distances within its rounding of the fp32
+// refine's.
+enum class SearchMode { kPlain = 0, kBatched = 1, kMfma = 2 };
 
 class E8qIndex {
    public:

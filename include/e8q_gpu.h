@@ -78,8 +78,13 @@ typedef struct {
 /* Search modes. PLAIN keeps the reference's per-query schedule bit for bit
  * (clusters in probe order, survivors refined every flush_every rows against
  * the live top-k threshold: e8q_search.cpp:215-293); BATCHED runs the whole
- * query set as one (query, cluster) work list. */
-enum { E8Q_MODE_PLAIN = 0, E8Q_MODE_BATCHED = 1 };
+ * query set as one (query, cluster) work list.  MFMA scores every vector of
+ * every probed list with the refine estimator as a split-BF16 matrix
+ * contraction (bits <= 4, FP32 accumulation; no pruning, so its reference
+ * result is the unpruned estimator top-k, within the contraction's rounding)
+ * and reports those distances; survivors then counts the vectors that passed
+ * its candidate bound. */
+enum { E8Q_MODE_PLAIN = 0, E8Q_MODE_BATCHED = 1, E8Q_MODE_MFMA = 2 };
 
 typedef struct {
     const float* queries;          /* nq*dim (host, or device if queries_on_device) */

@@ -24,6 +24,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "orc_common.h"
@@ -732,40 +733,47 @@ struct Row {
     uint32_t id;
 };
 
+// Digits and branch of block `blk` of a row (DecodeSplitBlockLane,
+// e8q_scan.cpp:34-90): d_i = rowA bit (7-i) << (b-1) | field_i, digit 7's
+// field LSB the parity of digits 0..6's, its stored LSB the branch.
+void DecodeBlock(const Index& ix, const Row& r, int blk, int d[8], int* br) {
+    const int bits = ix.bits, c = bits - 1;
+    const uint8_t rb = r.plane[static_cast<size_t>(blk) * 32 + r.lane];
+    if (bits == 1) {
+        int par = 0;
+        for (int i = 0; i < 7; ++i) {
+            d[i] = (rb >> (7 - i)) & 1;
+            par ^= d[i];
+        }
+        d[7] = par;
+        *br = rb & 1;
+        return;
+    }
+    unsigned __int128 x = 0;
+    for (int i = 0; i < c; ++i)
+        x |= static_cast<unsigned __int128>(r.refine[static_cast<size_t>(blk) * c + i]) << (8 * i);
+    const unsigned __int128 m = (static_cast<unsigned __int128>(1) << c) - 1;
+    int par = 0;
+    for (int i = 0; i < 7; ++i) {
+        const int f = static_cast<int>((x >> (c * i)) & m);
+        par ^= f & 1;
+        d[i] = f | (((rb >> (7 - i)) & 1) << c);
+    }
+    const int f7 = static_cast<int>((x >> (c * 7)) & m);
+    *br = f7 & 1;
+    d[7] = (f7 & ~1) | par | ((rb & 1) << c);
+}
+
 // Exact per-row score in the AVX2 refine order (e8q_search_avx2.cpp:82-403):
 // component i of even blocks in acc[i], of odd blocks in acc2[i], FMA
 // chains, then Hsum8(acc + acc2).
 float RefineScore(const Index& ix, const Row& r, const float* q) {
-    const int bits = ix.bits, c = bits - 1;
-    const size_t stride = bits >= 9 ? (size_t{1} << bits) : 256;
+    const size_t stride = ix.bits >= 9 ? (size_t{1} << ix.bits) : 256;
     float acc[8] = {}, acc2[8] = {};
     for (int blk = 0; blk < ix.blocks; ++blk) {
-        const uint8_t rb = r.plane[static_cast<size_t>(blk) * 32 + r.lane];
         int d[8];
         int br;
-        if (bits == 1) {
-            int par = 0;
-            for (int i = 0; i < 7; ++i) {
-                d[i] = (rb >> (7 - i)) & 1;
-                par ^= d[i];
-            }
-            d[7] = par;
-            br = rb & 1;
-        } else {
-            unsigned __int128 x = 0;
-            for (int i = 0; i < c; ++i)
-                x |= static_cast<unsigned __int128>(r.refine[static_cast<size_t>(blk) * c + i]) << (8 * i);
-            const unsigned __int128 m = (static_cast<unsigned __int128>(1) << c) - 1;
-            int par = 0;
-            for (int i = 0; i < 7; ++i) {
-                const int f = static_cast<int>((x >> (c * i)) & m);
-                par ^= f & 1;
-                d[i] = f | (((rb >> (7 - i)) & 1) << c);
-            }
-            const int f7 = static_cast<int>((x >> (c * 7)) & m);
-            br = f7 & 1;
-            d[7] = (f7 & ~1) | par | ((rb & 1) << c);
-        }
+        DecodeBlock(ix, r, blk, d, &br);
         const float* tab = ix.refine_tab.data() + static_cast<size_t>(br) * stride;
         float* dst = (blk & 1) ? acc2 : acc;
         for (int i = 0; i < 8; ++i) dst[i] = std::fma(tab[d[i]], q[blk * 8 + i], dst[i]);
@@ -773,6 +781,21 @@ float RefineScore(const Index& ix, const Row& r, const float* q) {
     float s[8];
     for (int i = 0; i < 8; ++i) s[i] = acc[i] + acc2[i];
     return Hsum8(s);
+}
+
+// The same score in fp64: the float table entries and query components,
+// exact products, one fp64 sum in dimension order.
+double RefineScore64(const Index& ix, const Row& r, const float* q) {
+    const size_t stride = ix.bits >= 9 ? (size_t{1} << ix.bits) : 256;
+    double s = 0.0;
+    for (int blk = 0; blk < ix.blocks; ++blk) {
+        int d[8];
+        int br;
+        DecodeBlock(ix, r, blk, d, &br);
+        const float* tab = ix.refine_tab.data() + static_cast<size_t>(br) * stride;
+        for (int i = 0; i < 8; ++i) s += static_cast<double>(tab[d[i]]) * static_cast<double>(q[blk * 8 + i]);
+    }
+    return s;
 }
 
 struct Stats {
@@ -907,6 +930,38 @@ void ScanAllTopK(const Index& ix, const float* query, size_t k, size_t nprobe, u
     for (size_t i = 0; i < k; ++i) {
         out_ids[i] = i < take ? all[i].second : std::numeric_limits<uint32_t>::max();
         out_dists[i] = i < take ? all[i].first : std::numeric_limits<float>::infinity();
+    }
+}
+
+// The estimator of ScanAllTopK in fp64 for given ids of the probed lists:
+// ((double) d2_c + h3) - 2 g1 S64 (NaN for an id outside them).
+void Dist64(const Index& ix, const float* query, size_t nprobe, const uint32_t* ids, size_t n, double* out) {
+    std::vector<float> q(ix.padded);
+    RotateQuery(ix, query, q.data());
+    std::vector<float> cdist(ix.nlist);
+    std::vector<uint32_t> probe;
+    Route(ix, q.data(), nprobe, cdist.data(), probe);
+    std::unordered_map<uint32_t, size_t> want;
+    for (size_t j = 0; j < n; ++j) {
+        want[ids[j]] = j;
+        out[j] = std::numeric_limits<double>::quiet_NaN();
+    }
+    for (uint32_t c : probe) {
+        const uint64_t vb = ix.cluster_begin[c];
+        const size_t cnt = ix.cluster_begin[c + 1] - vb;
+        for (size_t i = 0; i < cnt; ++i) {
+            const auto it = want.find(ix.ids[vb + i]);
+            if (it == want.end()) continue;
+            const uint8_t* bp = ix.arena.data() + ix.arena_begin[c] + (i / 32) * ix.batch_bytes;
+            const uint8_t* rr = ix.ref_arena.data() + ix.ref_arena_begin[c] + (i / 32) * ix.ref_batch_bytes;
+            const size_t t = i % 32;
+            const float g1 = reinterpret_cast<const float*>(rr + ix.RefG1Off())[t];
+            const float h3 = reinterpret_cast<const float*>(rr + ix.RefH3Off())[t];
+            Row r{ix.bits >= 2 ? rr + t * ix.RefineStride() : nullptr, bp + kHdr, static_cast<uint32_t>(t), g1, h3,
+                  ix.ids[vb + i]};
+            const double S = RefineScore64(ix, r, q.data());
+            out[it->second] = (static_cast<double>(cdist[c]) + static_cast<double>(h3)) - 2.0 * static_cast<double>(g1) * S;
+        }
     }
 }
 
@@ -1066,6 +1121,15 @@ void orc_query_prep(void* h, const float* query, size_t k, size_t nprobe, float*
     }
     if (probe) std::memcpy(probe, pr.data(), pr.size() * 4);
     if (cdist) std::memcpy(cdist, cd.data(), cd.size() * 4);
+}
+
+// fp64 distances of `k` given ids per query (NaN: not in the probed lists).
+void orc_dist64(void* h, const float* queries, size_t nq, size_t nprobe, const uint32_t* ids, size_t k, double* out) {
+    const auto* ix = static_cast<orc::Index*>(h);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long i = 0; i < static_cast<long long>(nq); ++i)
+        orc::Dist64(*ix, queries + static_cast<size_t>(i) * ix->dim, nprobe, ids + static_cast<size_t>(i) * k, k,
+                    out + static_cast<size_t>(i) * k);
 }
 
 void orc_scan_all(void* h, const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* ids, float* dists) {

@@ -42,6 +42,7 @@ def lib():
         L.orc_search.argtypes = [vp, vp, sz, sz, sz, i32, i32, vp, vp, vp]
         L.orc_query_prep.argtypes = [vp, vp, sz, sz, vp, vp, vp, vp, vp]
         L.orc_scan_all.argtypes = [vp, vp, sz, sz, sz, vp, vp]
+        L.orc_dist64.argtypes = [vp, vp, sz, sz, vp, sz, vp]
         _lib = L
     return _lib
 
@@ -181,6 +182,14 @@ class OracleIndex:
         d = np.empty((nq, k), np.float32)
         lib().orc_scan_all(self.h, _p(q), nq, k, nprobe, _p(ids), _p(d))
         return ids, d
+
+    def dist64(self, queries, nprobe, ids):
+        """fp64 estimator distances of the given ids (NaN: not in the probed lists)."""
+        q = c32(queries)
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.empty(ids.shape, np.float64)
+        lib().orc_dist64(self.h, _p(q), q.shape[0], nprobe, _p(ids), ids.shape[1], _p(out))
+        return out
 
 
 # ---------------------------------------------------------------------------

@@ -244,3 +244,36 @@ def test_estimator_against_float64():
         true = ((data[ids[i]].astype(np.float64) - q[i]) ** 2).sum(axis=1)
         rel = np.abs(d[i] - true) / true
         assert np.median(rel) < 0.05
+
+
+@pytest.mark.parametrize("name", ["d128b4", "d100b1", "d128b2", "d64b9", "d768b4"])
+def test_fp64_estimator_matches_fp32(name):
+    # The fp64 restatement of the estimator (the MFMA form's error yardstick)
+    # shares the pinned decode with the fp32 refine; the two differ only by
+    # fp32 rounding of the products and sums: a wrong digit, branch, table
+    # entry or sign would move a distance by far more than that.
+    ix = orc.OracleIndex.from_e8ax(_case_index(name))
+    q = gl(f"{name}_search.npz")["queries"]
+    nl = ix.nlist
+    for nprobe in (1, nl):
+        ids, d = ix.scan_all(q, 10, nprobe)
+        d64 = ix.dist64(q, nprobe, ids)
+        ok = ids != np.iinfo(np.uint32).max
+        assert np.all(np.isfinite(d64[ok])) and np.all(np.isnan(d64[~ok]))
+        scale = np.abs(d64[ok]).max()
+        err = np.abs(d[ok].astype(np.float64) - d64[ok])
+        assert err.max() <= 1e-5 * scale, (name, err.max() / scale)
+        # an id outside the probed lists has no distance
+        if nprobe == 1 and ix.n > 0:
+            far = np.setdiff1d(np.arange(ix.n, dtype=np.uint32), ids[0])[:1]
+            probe = ix.query_prep(q[0], 10, 1)[3]
+            d_far = ix.dist64(q[:1], 1, far[None, :])
+            assert np.isnan(d_far[0, 0]) or far[0] in _members(ix, probe)
+
+
+def _members(ix, lists):
+    a = ix.arrays()
+    out = []
+    for c in lists:
+        out.extend(a["ids"][a["cluster_begin"][c]:a["cluster_begin"][c + 1]].tolist())
+    return set(out)
