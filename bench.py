@@ -46,7 +46,9 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("e8q", "me8q", "serve"), default="e8q")
+    ap.add_argument("--workload", choices=("e8q", "me8q", "serve", "zipf"), default="e8q",
+                    help="zipf: the batched / MFMA E8Q configuration, 10M x 768, 16384 Zipf(1.1) lists, "
+                         "k=10, per-query nprobe in {16,32,64,128}")
     ap.add_argument("--serve-index", choices=("e8q", "me8q"), default="e8q")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-workers", type=int, default=3)
@@ -54,16 +56,17 @@ def parse():
                     help="offered request rates as multiples of the plain path's request rate")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--n", type=int, default=None, help="1M (e8q, me8q) / 10M (zipf)")
     ap.add_argument("--dim", type=int, default=None, help="128 (e8q) / 960 (me8q)")
     ap.add_argument("--nlist", type=int, default=None, help="1024 (e8q) / 4096 (me8q)")
     ap.add_argument("--nq", type=int, default=10_000)
-    ap.add_argument("--k", type=int, default=100)
+    ap.add_argument("--k", type=int, default=None, help="100 (e8q, me8q) / 10 (zipf)")
     ap.add_argument("--bits", type=int, default=4)
     ap.add_argument("--grid", default=None, help="2,4,...,128 (e8q) / 32,64,128,256 (me8q)")
     ap.add_argument("--avg-bits", type=float, default=4.0)
     ap.add_argument("--target", type=float, default=0.95)
-    ap.add_argument("--modes", default="plain,batched,batched_graph")
+    ap.add_argument("--modes", default=None,
+                    help="plain,batched,batched_graph (e8q) / plain,batched,mfma,mfma_graph (zipf)")
     ap.add_argument("--latency-queries", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-sample", type=int, default=2000)
@@ -76,8 +79,12 @@ def parse():
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     a = ap.parse_args()
     me = a.workload == "me8q" or (a.workload == "serve" and a.serve_index == "me8q")
-    a.dim = a.dim or (960 if me else 128)
-    a.nlist = a.nlist or (4096 if me else 1024)
+    zipf = a.workload == "zipf"
+    a.n = a.n or (10_000_000 if zipf else 1_000_000)
+    a.k = a.k or (10 if zipf else 100)
+    a.modes = a.modes or ("plain,batched,mfma,mfma_graph" if zipf else "plain,batched,batched_graph")
+    a.dim = a.dim or (960 if me else 768 if zipf else 128)
+    a.nlist = a.nlist or (4096 if me else 16384 if zipf else 1024)
     a.grid = a.grid or ("32,64,128,256" if me else "2,4,8,16,32,64,128")
     return a
 
@@ -476,6 +483,157 @@ def run_serve(args):
     print(json.dumps(line), flush=True)
 
 
+def make_zipf_data(torch, n, dim, nlist, nq, seed=7, s=1.1):
+    """List sizes proportional to c^-s over nlist lists (summing to n), centers ~
+    N(0, 4I), points = center + N(0, I), cluster_ids = the generating list,
+    centroids = per-list means; queries drawn like the points (list chosen in
+    proportion to its size)."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    w = torch.arange(1, nlist + 1, dtype=torch.float64) ** (-s)
+    sizes = torch.floor(w / w.sum() * n).to(torch.int64)
+    sizes[:n - int(sizes.sum())] += 1
+    centers = torch.randn(nlist, dim, generator=g, device=dev) * 2.0
+    lab = torch.repeat_interleave(torch.arange(nlist, device=dev), sizes.to(dev))
+    base = torch.empty(n, dim, device=dev)
+    for st in range(0, n, 1 << 20):
+        e = min(n, st + (1 << 20))
+        base[st:e] = centers[lab[st:e]] + torch.randn(e - st, dim, generator=g, device=dev)
+    cent = torch.zeros(nlist, dim, device=dev, dtype=torch.float64)
+    for st in range(0, n, 1 << 20):
+        e = min(n, st + (1 << 20))
+        cent.index_add_(0, lab[st:e], base[st:e].double())
+    cent = (cent / sizes.to(dev).clamp(min=1)[:, None].double()).float()
+    pick = torch.randint(0, n, (nq,), generator=g, device=dev)
+    queries = centers[lab[pick]] + torch.randn(nq, dim, generator=g, device=dev)
+    return base, cent.contiguous(), lab.to(torch.int32), queries.contiguous(), sizes
+
+
+def ground_truth_chunked(torch, base, queries, k=10, cand=64, qchunk=1024, bchunk=1 << 21):
+    """Exact top-k: fp32 GEMM candidates per base chunk, merged, re-ranked in fp64."""
+    n = base.shape[0]
+    bn = torch.empty(n, device=base.device)
+    for st in range(0, n, bchunk):
+        x = base[st:st + bchunk].double()
+        bn[st:st + bchunk] = (x * x).sum(1).float()
+    out = torch.empty(queries.shape[0], k, dtype=torch.int64, device=base.device)
+    for qs in range(0, queries.shape[0], qchunk):
+        q = queries[qs:qs + qchunk]
+        best_d, best_i = None, None
+        for st in range(0, n, bchunk):
+            d = bn[None, st:st + bchunk] - 2.0 * (q @ base[st:st + bchunk].T)
+            td, ti = d.topk(cand, dim=1, largest=False)
+            ti = ti + st
+            if best_d is None:
+                best_d, best_i = td, ti
+            else:
+                md = torch.cat([best_d, td], 1)
+                mi = torch.cat([best_i, ti], 1)
+                sel = md.topk(cand, dim=1, largest=False).indices
+                best_d, best_i = torch.gather(md, 1, sel), torch.gather(mi, 1, sel)
+        exact = ((base[best_i].double() - q[:, None, :].double()) ** 2).sum(2)
+        order = exact.argsort(dim=1)[:, :k]
+        out[qs:qs + qchunk] = torch.gather(best_i, 1, order)
+    return out
+
+
+def run_zipf(args):
+    """The batched / MFMA configuration: every form over one 10k-query batch with
+    per-query nprobe uniform in {16, 32, 64, 128}."""
+    import numpy as np
+    import torch
+    import e8qlib
+
+    t0 = time.perf_counter()
+    base, cent, lab, queries, sizes = make_zipf_data(torch, args.n, args.dim, args.nlist, args.nq)
+    torch.cuda.synchronize()
+    log(f"zipf data: {time.perf_counter() - t0:.1f} s, largest list {int(sizes.max())}, smallest {int(sizes.min())}")
+    t0 = time.perf_counter()
+    gt = ground_truth_chunked(torch, base, queries, k=10).cpu().numpy()
+    torch.cuda.synchronize()
+    log(f"ground truth: {time.perf_counter() - t0:.1f} s")
+    idx = e8qlib.E8qIndex(args.dim, args.bits, 42)
+    lab_np = lab.cpu().numpy().astype(np.uint32)
+    tb = time.perf_counter()
+    idx.build_device(base.data_ptr(), cent.data_ptr(), args.nlist, lab_np, lab.data_ptr())
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - tb
+    log(f"build: {build_s:.2f} s ({args.n / build_s / 1e6:.2f} M vectors/s, base in HBM)")
+    del base
+    torch.cuda.empty_cache()
+    nq, k = args.nq, args.k
+    rng = np.random.default_rng(11)
+    npq = rng.choice(np.array([16, 32, 64, 128], np.uint32), size=nq)
+    q_dev = queries.contiguous()
+    ids_dev = torch.empty((nq, k), dtype=torch.int32, device="cuda:0")
+    d_dev = torch.empty((nq, k), dtype=torch.float32, device="cuda:0")
+    P = (args.dim + 63) // 64 * 64
+    K = P * 16
+    results = {}
+    for mode in [m for m in args.modes.split(",") if m]:
+        form = mode[:-len("_graph")] if mode.endswith("_graph") else mode
+        e8qlib.set_search_graphs(mode != form)
+        for _ in range(max(args.warmup, 2 if mode != form else 0)):
+            idx.search_device(q_dev.data_ptr(), nq, k, 128, ids_dev.data_ptr(), d_dev.data_ptr(), form, npq)
+        torch.cuda.synchronize()
+        times, scan_ms, st = [], [], None
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            st = idx.search_device(q_dev.data_ptr(), nq, k, 128, ids_dev.data_ptr(), d_dev.data_ptr(), form, npq)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - ts)
+            scan_ms.append(st["device_scan_ms"])
+        if mode != form:
+            info = e8qlib.search_context_info()
+            e8qlib.set_search_graphs(False)
+        ids = ids_dev.cpu().numpy().view(np.uint32)
+        ms = 1000.0 * sum(times) / len(times)
+        row = {"ms_per_step": ms, "qps": nq / (ms / 1000.0), "recall": recall_at(gt, ids, 10, 10),
+               "scan_kernel_ms": sum(scan_ms) / len(scan_ms), "candidates": st["candidates"],
+               "survivors": st["survivors"], "batches": st["batches"], "records_read": st["scanned"]}
+        if form == "mfma" and mode == form:
+            flops = st["candidates"] * 4.0 * K
+            row["contraction_tflops"] = flops / (row["scan_kernel_ms"] / 1000.0) / 1e12
+            row["useful_tflops"] = st["candidates"] * 2.0 * P / (row["scan_kernel_ms"] / 1000.0) / 1e12
+        if mode != form:
+            row["graph"] = info
+        results[mode] = row
+        log(f"{mode}: {ms:.2f} ms/step, {row['qps']:.0f} QPS, recall@10 {row['recall']:.4f}, "
+            f"scan kernel {row['scan_kernel_ms']:.2f} ms, candidates {row['candidates']}, "
+            f"survivors {row['survivors']}"
+            + (f", contraction {row['contraction_tflops']:.0f} TFLOP/s" if "contraction_tflops" in row else ""))
+    ok = {m: r for m, r in results.items() if r["recall"] >= args.target}
+    best = max(ok, key=lambda m: ok[m]["qps"]) if ok else max(results, key=lambda m: results[m]["recall"])
+    plain_qps = results.get("plain", {}).get("qps")
+    mf = results.get("mfma")
+    line = {
+        "metric": "QPS at recall@10>=0.95 (plain vs batched/MFMA, 1 GPU)",
+        "value": round(results[best]["qps"], 1) if ok else None,
+        "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(results[best]["ms_per_step"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 hi+lo MFMA / fp32 accumulate (mfma); u8 / i32 / fp32 (plain, batched)",
+        "data": "synthetic Zipf(1.1) Gaussian mixture generated on the GPU (seed 7)",
+        "config": {"workload": f"batched MFMA E8Q: N={args.n}, D={args.dim}, {args.nlist} Zipf lists, b={args.bits}, "
+                               f"{nq} queries, k={k}, nprobe per query uniform in {{16,32,64,128}}",
+                   "form": best, "largest_list": int(sizes.max()), "smallest_list": int(sizes.min())},
+        "speedup_vs_plain": round(results[best]["qps"] / plain_qps, 2) if plain_qps and ok else None,
+        "forms": {m: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items()}
+                  for m, r in results.items()},
+        "build_seconds": round(build_s, 2),
+        "roofline": ({"bound": "mfma", "kernel": "MfmaScanKernel", "achieved": round(mf["contraction_tflops"], 1),
+                      "peak": 2500.0, "unit": "TFLOP/s", "frac": round(mf["contraction_tflops"] / 2500.0, 4),
+                      "traffic": None,
+                      "basis": f"per launch {mf['candidates']} (query, vector) pairs x 4 K flops, K = {P} x 16 "
+                               f"(one-hot) and x2 for hi+lo; useful 2 P flops per pair: "
+                               f"{mf['useful_tflops']:.2f} TFLOP/s; / mean kernel time {mf['scan_kernel_ms']:.3f} ms"}
+                     if mf else None),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calib:
@@ -490,6 +648,9 @@ def main():
         return
     if args.workload == "serve":
         run_serve(args)
+        return
+    if args.workload == "zipf":
+        run_zipf(args)
         return
     import numpy as np
     import torch

@@ -745,7 +745,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             Check(LaunchRoute(st, rotq, consts, nq, v, dist), "route");
             Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq, nprobe_all, maxp, pids, pd2),
                   "select probes");
-            if (eager) Check(hipEventRecord(ctx->ev[1], st), "event");
+            if (eager && !mfma) Check(hipEventRecord(ctx->ev[1], st), "event");
             if (!listed) {
                 Check(LaunchPlainSearch(st, v, pa, dparams, upload), "plain search");
                 return;
@@ -762,7 +762,9 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
                 Check(LaunchFill32(st, ba.ccount, v.nlist, 0), "fill");
                 Check(LaunchFill32(st, ba.ncand, nq, 0), "fill");
                 Check(LaunchBatchedWorkList(st, v, ba, bparams, false), "work list");
+                if (eager) Check(hipEventRecord(ctx->ev[1], st), "event");   // the contraction alone
                 Check(LaunchMfmaScan(st, mparams, shape.grid), "mfma scan");
+                if (eager) Check(hipEventRecord(ctx->ev[3], st), "event");
                 Check(LaunchMfmaSelect(st, mparams, nq, static_cast<uint32_t>(a->k)), "mfma select");
                 return;
             }
@@ -907,11 +909,11 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         }
         unsigned long long hs[4] = {0, 0, 0, 0};
         Check(hipMemcpyAsync(hs, dstats, 32, hipMemcpyDeviceToHost, st), "download stats");
-        Check(hipEventRecord(ctx->ev[3], st), "event");
         Check(hipStreamSynchronize(st), "search");
         if (stats) {
             float ms_scan = 0.0F, ms_all = 0.0F;
-            Check(hipEventElapsedTime(&ms_scan, ctx->ev[1], ctx->ev[2]), "event time");
+            const bool contraction = mfma && !ge;   // eager MFMA form: the contraction kernel alone
+            Check(hipEventElapsedTime(&ms_scan, ctx->ev[1], ctx->ev[contraction ? 3 : 2]), "event time");
             Check(hipEventElapsedTime(&ms_all, ctx->ev[0], ctx->ev[2]), "event time");
             stats->candidates = hs[0];
             stats->survivors = hs[1] + extra_survivors;

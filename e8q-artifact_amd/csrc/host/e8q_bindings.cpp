@@ -66,6 +66,26 @@ class PyE8qIndex {
         index_ = std::move(idx);
     }
 
+    // Build from rows already in device memory (base [n][dim] and centroids
+    // [num_clusters][dim] float32 device pointers, cluster ids both as a host
+    // array and as a device pointer): no host copy of the base.
+    void build_device(uintptr_t base_ptr, uintptr_t centroids_ptr, size_t num_clusters, py::handle cluster_ids,
+                      uintptr_t cluster_ids_ptr, float prune_alpha) {
+        auto ids = Ensure<uint32_t>(cluster_ids, "cluster_ids", 1);
+        if (!base_ptr || !centroids_ptr || !cluster_ids_ptr) throw std::invalid_argument("null device pointer");
+        auto idx = std::make_unique<e8qlib::E8qIndex>();
+        e8qlib::E8qBuildOptions opt;
+        opt.prune_alpha = prune_alpha;
+        {
+            py::gil_scoped_release nogil;
+            idx->BuildFromDevice(reinterpret_cast<const float*>(base_ptr), reinterpret_cast<const float*>(centroids_ptr),
+                                 ids.data(), reinterpret_cast<const uint32_t*>(cluster_ids_ptr),
+                                 static_cast<size_t>(ids.shape(0)), dim_, num_clusters, static_cast<int>(nbits_), seed_,
+                                 opt);
+        }
+        index_ = std::move(idx);
+    }
+
     py::tuple search(py::handle queries, size_t k, size_t nprobe, size_t num_threads, bool return_stats,
                      const std::string& mode, py::object nprobe_per_query) {
         (void)num_threads;
@@ -218,6 +238,8 @@ PYBIND11_MODULE(_e8qlib, m) {
         .def(py::init<size_t, size_t, uint64_t>(), py::arg("dim"), py::arg("nbits"), py::arg("seed") = 0)
         .def("build", &PyE8qIndex::build, py::arg("data"), py::arg("centroids"), py::arg("cluster_ids"),
              py::arg("num_threads") = 1, py::arg("prune_alpha") = 0.0F)
+        .def("build_device", &PyE8qIndex::build_device, py::arg("base_ptr"), py::arg("centroids_ptr"),
+             py::arg("num_clusters"), py::arg("cluster_ids"), py::arg("cluster_ids_ptr"), py::arg("prune_alpha") = 0.0F)
         .def("search", &PyE8qIndex::search, py::arg("queries"), py::arg("k"), py::arg("nprobe"),
              py::arg("num_threads") = 1, py::arg("return_stats") = false, py::arg("mode") = "plain",
              py::arg("nprobe_per_query") = py::none())

@@ -69,7 +69,8 @@ typedef struct {
                             candidate list overflowed counts its first probe's) */
     uint64_t batches;    /* 32-vector scan records of the probed clusters */
     double scan_ms;      /* scan kernel time, HIP events on the context stream
-                            (a graph replay: the whole graph)                   */
+                            (MFMA form: the contraction kernel; a graph
+                            replay: the whole graph)                            */
     double total_ms;     /* device time of the whole search, same events       */
     uint64_t scanned;    /* of those, records whose codes were read (the rest
                             belong to clusters passed over as a whole) */
