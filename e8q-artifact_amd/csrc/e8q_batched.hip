@@ -137,26 +137,34 @@ __global__ __launch_bounds__(256) void PairScatterKernel(const BatchedParams* __
     }
 }
 
-// Items of each cluster: pair tiles x batch ranges, pair-tile major.
+// Items of each cluster: pair tiles x batch ranges, pair-tile major.  One
+// thread per item (a head list can hold 1e5 items): its list is the last one
+// whose first item precedes it.
 __global__ __launch_bounds__(256) void ItemsKernel(const BatchedParams* __restrict__ prm) {
     const IndexView& ix = prm->ix;
     const BatchedArgs& a = prm->a;
-    for (uint64_t c = blockIdx.x * 256ull + threadIdx.x; c < ix.nlist; c += gridDim.x * 256ull) {
+    const uint64_t L = ix.nlist;
+    const uint64_t total = min<uint64_t>(a.ioff[L], a.items_cap);
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total; i += gridDim.x * 256ull) {
+        uint64_t lo = 0, hi = L;   // last c in [0, L) with ioff[c] <= i
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (a.ioff[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t c = static_cast<uint32_t>(lo);
         const uint32_t n = a.ccount[c];
-        if (n == 0) continue;
         const uint32_t nb = static_cast<uint32_t>((ix.cluster_begin[c + 1] - ix.cluster_begin[c] + 31) / 32);
-        const uint32_t nbt = (nb + a.btile - 1) / a.btile, nqt = (n + a.qtile - 1) / a.qtile;
-        BatchItem* out = a.items + a.ioff[c];
-        for (uint32_t qt = 0; qt < nqt; ++qt)
-            for (uint32_t bt = 0; bt < nbt; ++bt) {
-                BatchItem it{};
-                it.c = static_cast<uint32_t>(c);
-                it.p0 = a.coff[c] + qt * a.qtile;
-                it.np = min(a.qtile, n - qt * a.qtile);
-                it.b0 = bt * a.btile;
-                it.nb = min(a.btile, nb - bt * a.btile);
-                if (a.ioff[c] + qt * nbt + bt < a.items_cap) out[qt * nbt + bt] = it;
-            }
+        const uint32_t nbt = (nb + a.btile - 1) / a.btile;
+        const uint32_t local = static_cast<uint32_t>(i - a.ioff[c]);
+        const uint32_t qt = local / nbt, bt = local - qt * nbt;
+        BatchItem it{};
+        it.c = c;
+        it.p0 = a.coff[c] + qt * a.qtile;
+        it.np = min(a.qtile, n - qt * a.qtile);
+        it.b0 = bt * a.btile;
+        it.nb = min(a.btile, nb - bt * a.btile);
+        a.items[i] = it;
     }
 }
 
@@ -176,8 +184,8 @@ hipError_t LaunchBatchedWorkList(hipStream_t st, const IndexView& ix, const Batc
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(PairScatterKernel, dim3(g), dim3(256), 0, st, dparams);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const uint32_t gc = static_cast<uint32_t>(std::min<uint64_t>((ix.nlist + 255) / 256, 1024));
-    hipLaunchKernelGGL(ItemsKernel, dim3(gc), dim3(256), 0, st, dparams);
+    const uint32_t gc = static_cast<uint32_t>(std::min<uint64_t>((a.items_cap + 255) / 256, 4096));
+    hipLaunchKernelGGL(ItemsKernel, dim3(std::max<uint32_t>(gc, 1)), dim3(256), 0, st, dparams);
     return hipGetLastError();
 }
 

@@ -27,6 +27,7 @@ struct PlainArgs {
     unsigned long long* stats;  // [4] candidates, survivors, batches, records read
     const uint32_t* qsel;       // optional: search only queries qsel[0..nsel)
     uint64_t nsel;
+    uint32_t max_vectors;       // nonzero (a multiple of 32): scan at most this many vectors of each list
 };
 
 hipError_t LaunchRotate(hipStream_t st, const float* x, uint64_t rows, uint32_t dim, const IndexView& ix, float* out,
@@ -150,12 +151,14 @@ hipError_t LaunchBatchedRefine(hipStream_t st, const IndexView& ix, const Batche
 // K = 16 per dimension, the query side q_i L0[l] split into hi + lo bf16,
 // fp32 accumulation.
 constexpr uint32_t kMfmaQTile = 32;    // pairs per item (the MFMA's M)
-constexpr uint32_t kMfmaBTile = 8;     // batches per item (4 waves x 2 tiles)
+constexpr uint32_t kMfmaBTile = 16;    // batches per item (4 waves x 4 tiles)
 constexpr uint32_t kMfmaMaxBits = 4;   // 2^bits folded levels fit one K = 16 slice
+constexpr uint32_t kMfmaDynK = 32;     // k up to this: the bound falls during the scan
 
 struct MfmaExtra {
     const uint16_t* qexp;       // [nq][padded][32] bf16 bits: 16 hi, then 16 lo
-    const float* thr;           // [nq] candidate bound
+    uint32_t* thr;              // [nq] candidate bound as a monotone distance key
+    uint32_t dyn;               // 1: each tile lowers thr to its k-th distance
     const uint8_t* mcodes;      // [records][blocks][32][8] folded level | branch << 7
     uint64_t* cand;             // [rows][cap] (monotone dist bits << 32) | id
     uint32_t* ncand;            // [nq]
@@ -172,11 +175,12 @@ struct MfmaParams {
 
 // Folded one-hot code layout and per-list max |g1| of an index.
 hipError_t LaunchMfmaCodes(hipStream_t st, const IndexView& ix, uint64_t records, uint8_t* mcodes, float* g1max);
-// Per query: the hi/lo bf16 query expansion, and the candidate bound
-// thr = tau (first-probe k-th distance) + a bound of both forms' rounding.
+// Per query: the hi/lo bf16 query expansion, and (first_dists non-null) the
+// candidate bound thr = tau (first-probe k-th distance) + a bound of both
+// forms' rounding, as a monotone key.
 hipError_t LaunchMfmaQuery(hipStream_t st, const IndexView& ix, const float* rotq, uint64_t nq, uint32_t k,
                            const float* first_dists, const uint32_t* probe_ids, uint32_t maxp, const float* g1max,
-                           float lmax, uint16_t* qexp, float* thr);
+                           float lmax, uint16_t* qexp, uint32_t* thr);
 hipError_t LaunchMfmaScan(hipStream_t st, const MfmaParams* dp, uint32_t grid);
 hipError_t LaunchMfmaSelect(hipStream_t st, const MfmaParams* dp, uint64_t nrun, uint32_t k);
 

@@ -15,12 +15,15 @@
 // dimension's folded levels) per 32 pairs x 32 vectors.  FP64 is not used at
 // search time; the build-time fits (g1, h3) are the reference's.
 //
-// Candidates: all vectors of the pair's list whose MFMA distance is within
-// thr_q = tau_q + margin_q, tau_q being the k-th fp32 refine distance of the
-// query's first probe (the threshold pass) and margin_q a bound of the two
-// forms' rounding difference for that list's vectors; the k vectors behind
-// tau_q therefore lie within thr_q in MFMA distance, so the top-k by MFMA
-// distance is among the candidates.  One wave per query then selects it.
+// Candidates: the vectors whose MFMA distance is within the query's bound
+// thr_q.  For k <= 32 the bound starts at +inf and every scored tile lowers
+// it to the tile's k-th smallest distance for that query (atomicMin): any k
+// scored vectors bound the final k-th distance from above, so every vector
+// of the final top-k passes whenever it is scored.  For larger k it is
+// tau_q + margin_q, tau_q being the k-th fp32 refine distance of the query's
+// first probe (the threshold pass) and margin_q a bound of the two forms'
+// rounding difference for that list's vectors, so the k vectors behind tau_q
+// lie within it in MFMA distance.  One wave per query then selects the top-k.
 #include <algorithm>
 
 #include "e8q_scan_dev.h"
@@ -116,6 +119,13 @@ hipError_t LaunchMfmaCodes(hipStream_t st, const IndexView& ix, uint64_t records
 // ---------------------------------------------------------------------------
 // Query side.
 
+// Order-preserving map of a float distance to 32 bits (negative values
+// reversed below the positive ones).
+__device__ __forceinline__ uint32_t DistKey32(float d) {
+    const uint32_t u = __float_as_uint(d);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
 // Round to nearest even (no NaN reaches here).
 __device__ __forceinline__ uint32_t Bf16Bits(float x) {
     const uint32_t u = __float_as_uint(x);
@@ -162,7 +172,7 @@ __global__ __launch_bounds__(kWave) void MfmaThrKernel(const float* __restrict__
                                                         const float* __restrict__ first_dists,
                                                         const uint32_t* __restrict__ probe_ids, uint32_t maxp,
                                                         const float* __restrict__ g1max, float lmax,
-                                                        float* __restrict__ thr) {
+                                                        uint32_t* __restrict__ thr) {
     const uint64_t q = blockIdx.x;
     float s = 0.0F;
     for (uint32_t i = threadIdx.x; i < P; i += kWave) s += fabsf(rotq[q * P + i]);
@@ -172,32 +182,36 @@ __global__ __launch_bounds__(kWave) void MfmaThrKernel(const float* __restrict__
         const float eps = 3.0517578e-05F + static_cast<float>(3 * P + 32) * 5.9604645e-08F;
         const float g1 = g1max[probe_ids[q * maxp]];
         const float margin = 2.0F * g1 * (s * 1.001F) * lmax * eps + fabsf(tau) * 9.5367432e-07F;
-        thr[q] = tau + margin;   // +inf when the first list holds fewer than k vectors
+        thr[q] = DistKey32(tau + margin);   // +inf when the first list holds fewer than k vectors
     }
 }
 
 hipError_t LaunchMfmaQuery(hipStream_t st, const IndexView& ix, const float* rotq, uint64_t nq, uint32_t k,
                            const float* first_dists, const uint32_t* probe_ids, uint32_t maxp, const float* g1max,
-                           float lmax, uint16_t* qexp, float* thr) {
+                           float lmax, uint16_t* qexp, uint32_t* thr) {
     if (nq == 0) return hipSuccess;
     const uint32_t P = static_cast<uint32_t>(ix.padded);
     const uint64_t n = nq * P;
     hipLaunchKernelGGL(MfmaQexpKernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, st, rotq, nq, P,
                        ix.refine_tab, 1u << ix.bits, qexp);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || first_dists == nullptr) return e;
     hipLaunchKernelGGL(MfmaThrKernel, dim3(static_cast<uint32_t>(nq)), dim3(kWave), 0, st, rotq, P, k, first_dists,
                        probe_ids, maxp, g1max, lmax, thr);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// The contraction.  Items of the batched work list with 32 pairs x 8
-// batches; four waves per workgroup, each scoring 32 pairs x two 32-vector
-// tiles.  Per dimension a lane supplies 8 bf16 of the A operand (query row
-// lane % 32, levels 8 (lane / 32) .. +7, hi and lo) and 8 of the B operand
-// (vector column lane % 32, the same levels); accumulator register r holds
-// row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32.
+// The contraction.  Items of the batched work list with 32 pairs x 16
+// batches; four waves per workgroup, each scoring the 32 pairs against four
+// 32-vector tiles (batches wave, wave + 4, wave + 8, wave + 12), so that one
+// load of the A operand feeds eight MFMAs.  Per dimension a lane supplies 8
+// bf16 of the A operand (query row lane % 32, levels 8 (lane / 32) .. +7, hi
+// and lo) and 8 of each B operand (vector column lane % 32, the same
+// levels); accumulator register r holds row 8 (r / 4) + 4 (lane / 32) + r % 4,
+// column lane % 32.
+
+constexpr int kMfmaTiles = 4;
 
 __device__ __forceinline__ u32x4 OneHot(uint32_t code, uint32_t h) {
     const uint32_t local = (code & 15u) - 8u * h;   // wraps above 7 when outside this half
@@ -211,15 +225,23 @@ __device__ __forceinline__ f32x16 Mfma(u32x4 a, u32x4 b, f32x16 c) {
                                                    0, 0, 0);
 }
 
-__device__ __forceinline__ uint64_t DistKey(float d, uint32_t id) {
-    const uint32_t u = __float_as_uint(d);
-    const uint32_t m = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-    return (static_cast<uint64_t>(m) << 32) | id;
-}
-
 __device__ __forceinline__ float KeyDist(uint64_t key) {
     const uint32_t m = static_cast<uint32_t>(key >> 32);
     return __uint_as_float((m & 0x80000000u) ? (m & 0x7FFFFFFFu) : ~m);
+}
+
+// The k-th smallest (1 <= k <= 32) of the 32 values held by the lanes of
+// this lane's half of the wave: a bitonic sort across the half.
+__device__ __forceinline__ uint32_t HalfKth(uint32_t v, uint32_t n, uint32_t lane, uint32_t k) {
+#pragma unroll
+    for (uint32_t size = 2; size <= 32; size <<= 1)
+#pragma unroll
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), static_cast<int>(stride)));
+            const bool keep_min = ((n & size) == 0) == ((n & stride) == 0);
+            v = keep_min ? min(v, o) : max(v, o);
+        }
+    return static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>((lane & 32u) | (k - 1))));
 }
 
 constexpr int kMfmaThreads = 256;
@@ -228,12 +250,14 @@ __global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams*
     const IndexView& ix = prm->bp.ix;
     const BatchedArgs& a = prm->bp.a;
     const MfmaExtra& m = prm->m;
-    __shared__ uint32_t s_it, s_q[kMfmaQTile];
-    __shared__ float s_d2[kMfmaQTile], s_thr[kMfmaQTile];
+    __shared__ uint32_t s_it, s_q[kMfmaQTile], s_thr[kMfmaQTile];
+    __shared__ float s_d2[kMfmaQTile];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t h = lane >> 5, n = lane & 31;
     const uint32_t P = static_cast<uint32_t>(ix.padded);
     const int B = ix.blocks;
+    const uint32_t k = a.k;
+    const bool dyn = m.dyn != 0;
     const uint32_t nitems = Uniform(static_cast<uint32_t>(min<uint64_t>(*G(a.counters), a.items_cap)));
     unsigned long long staged = 0;
     for (;;) {
@@ -249,21 +273,24 @@ __global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams*
             const uint32_t q = static_cast<uint32_t>(pr >> 32), p = static_cast<uint32_t>(pr);
             s_q[threadIdx.x] = q;
             s_d2[threadIdx.x] = G(a.probe_d2)[static_cast<uint64_t>(q) * a.maxp + p];
-            s_thr[threadIdx.x] = G(m.thr)[q];
+            s_thr[threadIdx.x] = m.thr[q];
         }
         __syncthreads();
-        if (wave < nb) {   // wave-uniform: this wave's tiles are batches wave and wave + 4
-            const bool two = wave + 4 < nb;
+        if (wave < nb) {   // wave-uniform: this wave's tiles are batches wave + 4 t, t < nt
+            const uint32_t nt = min(static_cast<uint32_t>(kMfmaTiles), (nb - wave + 3) / 4);
             const uint64_t ccnt = G(ix.cluster_begin)[c + 1] - G(ix.cluster_begin)[c];
-            const uint64_t rec0 = G(ix.arena_begin)[c] / ix.batch_bytes + b0;
-            const E8Q_GLOBAL uint8_t* code0 = G(m.mcodes) + (rec0 + wave) * 32ull * P + n * 8;
-            const E8Q_GLOBAL uint8_t* code1 = G(m.mcodes) + (rec0 + (two ? wave + 4 : wave)) * 32ull * P + n * 8;
+            const uint64_t rec0 = G(ix.arena_begin)[c] / ix.batch_bytes + b0 + wave;
+            const E8Q_GLOBAL uint8_t* code[kMfmaTiles];
+#pragma unroll
+            for (int t = 0; t < kMfmaTiles; ++t)   // a tile past nt reads the first one (unused)
+                code[t] = G(m.mcodes) + (rec0 + 4 * (t < static_cast<int>(nt) ? t : 0)) * 32ull * P + n * 8;
             const bool qv = n < np;
             const E8Q_GLOBAL uint16_t* qa = G(m.qexp) + static_cast<uint64_t>(s_q[qv ? n : 0]) * P * 32 + 8 * h;
-            f32x16 acc0 = {}, acc1 = {};
+            f32x16 acc[kMfmaTiles] = {};
             for (int blk = 0; blk < B; ++blk) {
-                const uint64_t w0 = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(code0 + blk * 256);
-                const uint64_t w1 = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(code1 + blk * 256);
+                uint64_t w[kMfmaTiles];
+#pragma unroll
+                for (int t = 0; t < kMfmaTiles; ++t) w[t] = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(code[t] + blk * 256);
                 const E8Q_GLOBAL u32x4* ap = reinterpret_cast<const E8Q_GLOBAL u32x4*>(qa + blk * 8 * 32);
                 u32x4 ahi[8], alo[8];
 #pragma unroll
@@ -275,40 +302,61 @@ __global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams*
                 for (int e = 0; e < 8; ++e) {
                     const u32x4 zero = {0u, 0u, 0u, 0u};
                     const u32x4 xh = qv ? ahi[e] : zero, xl = qv ? alo[e] : zero;
-                    const u32x4 b0v = OneHot(static_cast<uint32_t>(w0 >> (8 * e)) & 0xFFu, h);
-                    if (two) {   // interleaved, so that no MFMA waits on the previous one's result
-                        const u32x4 b1v = OneHot(static_cast<uint32_t>(w1 >> (8 * e)) & 0xFFu, h);
-                        acc0 = Mfma(xh, b0v, acc0);
-                        acc1 = Mfma(xh, b1v, acc1);
-                        acc0 = Mfma(xl, b0v, acc0);
-                        acc1 = Mfma(xl, b1v, acc1);
-                    } else {
-                        acc0 = Mfma(xh, b0v, acc0);
-                        acc0 = Mfma(xl, b0v, acc0);
-                    }
+                    u32x4 bv[kMfmaTiles];
+#pragma unroll
+                    for (int t = 0; t < kMfmaTiles; ++t) bv[t] = OneHot(static_cast<uint32_t>(w[t] >> (8 * e)) & 0xFFu, h);
+                    // hi for every tile, then lo: no MFMA waits on the previous one's result
+#pragma unroll
+                    for (int t = 0; t < kMfmaTiles; ++t)
+                        if (t < static_cast<int>(nt)) acc[t] = Mfma(xh, bv[t], acc[t]);
+#pragma unroll
+                    for (int t = 0; t < kMfmaTiles; ++t)
+                        if (t < static_cast<int>(nt)) acc[t] = Mfma(xl, bv[t], acc[t]);
                 }
             }
-            // Epilogue: distances, the candidate test, appends.
-            for (int tile = 0; tile < (two ? 2 : 1); ++tile) {
-                const uint32_t bj = b0 + wave + 4 * tile;   // batch within the list
+            // Epilogue: distances, the bound, the candidate test, appends.
+            uint32_t tk[16];   // this lane's rows' bounds
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t row = 8 * (r / 4) + 4 * h + (r % 4);
+                tk[r] = row < np ? s_thr[row] : 0u;
+            }
+#pragma unroll
+            for (int t = 0; t < kMfmaTiles; ++t) {
+                if (t >= static_cast<int>(nt)) break;
+                const uint32_t bj = b0 + wave + 4 * t;   // batch within the list
                 const uint64_t vo = static_cast<uint64_t>(bj) * 32 + n;
-                if (vo >= ccnt) continue;
-                const E8Q_GLOBAL uint8_t* rr =
-                    G(ix.ref_arena) + G(ix.ref_arena_begin)[c] + static_cast<uint64_t>(bj) * ix.ref_batch_bytes;
-                const E8Q_GLOBAL float* g1p = reinterpret_cast<const E8Q_GLOBAL float*>(rr + 32ull * ix.refine_stride);
-                const float g1 = g1p[n], h3 = g1p[32 + n];
-                const uint32_t id = G(ix.ids)[G(ix.cluster_begin)[c] + vo];
-                const f32x16 acc = tile ? acc1 : acc0;
+                const bool vv = vo < ccnt;
+                float g1 = 0.0F, h3 = 0.0F;
+                uint32_t id = 0;
+                if (vv) {
+                    const E8Q_GLOBAL uint8_t* rr =
+                        G(ix.ref_arena) + G(ix.ref_arena_begin)[c] + static_cast<uint64_t>(bj) * ix.ref_batch_bytes;
+                    const E8Q_GLOBAL float* g1p =
+                        reinterpret_cast<const E8Q_GLOBAL float*>(rr + 32ull * ix.refine_stride);
+                    g1 = g1p[n];
+                    h3 = g1p[32 + n];
+                    id = G(ix.ids)[G(ix.cluster_begin)[c] + vo];
+                }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const uint32_t row = 8 * (r / 4) + 4 * h + (r % 4);
-                    if (row >= np) continue;
-                    const float d = fmaf(-(2.0F * g1), acc[r], s_d2[row] + h3) + 0.0F;   // -0 -> +0
-                    if (!(d <= s_thr[row])) continue;
+                    const bool ok = vv && row < np;
+                    const float d = fmaf(-(2.0F * g1), acc[t][r], s_d2[row < np ? row : 0] + h3) + 0.0F;   // -0 -> +0
+                    const uint32_t key = ok ? DistKey32(d) : 0xFFFFFFFFu;
+                    if (dyn) tk[r] = min(tk[r], HalfKth(key, n, lane, k));
+                    if (!ok || key > tk[r]) continue;
                     const uint32_t q = s_q[row];
                     const uint32_t slot = atomicAdd(&m.ncand[q], 1u);
                     const uint64_t crow = m.qslot ? G(m.qslot)[q] : q;
-                    if (slot < m.cap) Gw(m.cand)[crow * m.cap + slot] = DistKey(d, id);
+                    if (slot < m.cap) Gw(m.cand)[crow * m.cap + slot] = (static_cast<uint64_t>(key) << 32) | id;
+                }
+            }
+            if (dyn && n == 0) {   // one lane per half publishes its rows' bounds
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const uint32_t row = 8 * (r / 4) + 4 * h + (r % 4);
+                    if (row < np && tk[r] < s_thr[row]) atomicMin(&m.thr[s_q[row]], tk[r]);
                 }
             }
         }

@@ -147,7 +147,8 @@ std::vector<ScanShape> ShapeCandidates(const IndexView& v, uint32_t grid) {
 }
 
 constexpr uint64_t kTuneMinQueries = 64;   // smaller calls use the default shape untuned
-constexpr uint32_t kMfmaCap = 2048;         // MFMA-form candidates kept per query (more: a second pass)
+constexpr uint32_t kMfmaCap = 4096;         // MFMA-form candidates kept per query (more: a second pass)
+constexpr uint32_t kMfmaTauVectors = 16384;  // the MFMA form's threshold pass reads this much of the first list
 
 // Builds the MFMA form's code layout of an index on its first MFMA search.
 // The fold needs the refine table's reflection symmetry, checked here.
@@ -549,7 +550,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
         MfmaExtra mx{};
         MfmaParams* mparams = nullptr;
         uint16_t* m_qexp = nullptr;
-        float* m_thr = nullptr;
+        uint32_t* m_thr = nullptr;
         uint64_t* m_cand = nullptr;
         BatchedParams* bparams = nullptr;
         uint32_t* first_ids = nullptr;
@@ -622,6 +623,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             bparams = reinterpret_cast<BatchedParams*>(mparams);   // its first member
             mx.qexp = m_qexp;
             mx.thr = m_thr;
+            mx.dyn = a->k <= kMfmaDynK ? 1u : 0u;
             mx.cand = m_cand;
             mx.mcodes = x->mfma.codes.as<uint8_t>();
             mx.ncand = ba.ncand;
@@ -677,7 +679,10 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             ba.p_lo = mfma ? 0 : 1;
             ba.qmask = nullptr;
         }
-        if (mfma) fa.stats = tstats;   // the work list counts the first probe's candidates
+        if (mfma) {
+            fa.stats = tstats;   // the work list counts the first probe's candidates
+            fa.max_vectors = kMfmaTauVectors;
+        }
         auto batched_tail = [&](const BatchedArgs& b, uint32_t grid, bool upload) {
             Check(LaunchFill32(st, b.ccount, v.nlist, 0), "fill");
             Check(LaunchFill32(st, b.ncand, nq, 0), "fill");
@@ -750,8 +755,11 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
                 Check(LaunchPlainSearch(st, v, pa, dparams, upload), "plain search");
                 return;
             }
-            Check(LaunchPlainSearch(st, v, fa, dparams, upload), "first-probe search");
             if (mfma) {
+                // The bound: the threshold pass over (the first 16384 vectors
+                // of) the first probe, plus the rounding margin; for k <= 32
+                // every scored tile lowers it further.
+                Check(LaunchPlainSearch(st, v, fa, dparams, upload), "first-probe search");
                 Check(LaunchMfmaQuery(st, v, rotq, nq, static_cast<uint32_t>(a->k), first_dists, pids, maxp,
                                       x->mfma.g1max.as<float>(), x->mfma.lmax, m_qexp, m_thr),
                       "mfma query");
@@ -768,6 +776,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
                 Check(LaunchMfmaSelect(st, mparams, nq, static_cast<uint32_t>(a->k)), "mfma select");
                 return;
             }
+            Check(LaunchPlainSearch(st, v, fa, dparams, upload), "first-probe search");
             if (eager && !cands.empty()) tune();
             batched_tail(ba, shape.grid, upload);
         };

@@ -82,6 +82,7 @@ __device__ __forceinline__ void LoadChunk(ProbeChunk& c, uint32_t base, const In
         const uint64_t ao = G(ix.arena_begin)[cid];
         const uint64_t ro = G(ix.ref_arena_begin)[cid];
         c.cnt = static_cast<uint32_t>(ve - vb);
+        if (a.max_vectors != 0 && c.cnt > a.max_vectors) c.cnt = a.max_vectors;
         c.vb_lo = static_cast<uint32_t>(vb);
         c.vb_hi = static_cast<uint32_t>(vb >> 32);
         c.ao_lo = static_cast<uint32_t>(ao);

@@ -130,7 +130,7 @@ def test_mixture_recall_not_below_plain(ctx):
 
 def test_uneven_nprobe_and_second_pass(ctx):
     # One list of 6000 vectors and several tiny ones: a query whose first list
-    # holds fewer than k vectors has no bound, keeps every candidate (> 2048
+    # holds fewer than k vectors has no bound, keeps every candidate (> 4096
     # of them) and is scored again with a list of its length.
     sizes = np.array([6000, 40, 900, 3, 1200, 0, 700, 350], np.int64)
     ix, q = _mixture_index(ctx, 0, 64, 8, 4, 8, 60, sizes=sizes)
@@ -141,14 +141,14 @@ def test_uneven_nprobe_and_second_pass(ctx):
         print(f"mfma skewed k={k}", flush=True)
         res = ix.search(q, k, 8, _z(k), mode=MODE_MFMA)
         _check(oix, q, k, 8, res, f"skewed k={k}")
-        if k == 50:   # one query at a time: some keep more than the 2048 a list holds
+        if k == 50:   # one query at a time: some keep more than the 4096 a list holds
             surv = []
             for i in range(q.shape[0]):
                 one = ix.search(q[i:i + 1], k, 8, _z(k), mode=MODE_MFMA)
                 np.testing.assert_array_equal(one[0][0], res[0][i])
                 np.testing.assert_array_equal(one[1][0], res[1][i])
                 surv.append(one[2][1])
-            assert max(surv) > 2048, "the second pass was not exercised"
+            assert max(surv) > 4096, "the second pass was not exercised"
         resq = ix.search(q, k, 8, _z(k), mode=MODE_MFMA, nprobe_per_query=npq)
         for p in (1, 2, 5, 8):
             sel = npq == p
