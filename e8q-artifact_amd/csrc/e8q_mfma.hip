@@ -36,8 +36,8 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // Index side: folded codes, one byte per dimension (level index in bits 0-3,
-// branch in bit 7), [record][block][lane][8] so that a lane reads a block's
-// eight dimensions with one 8-byte load.  Decode as DecodeSplitBlockLane
+// branch in bit 4: the row of the one-hot table below), [record][block][lane][8]
+// so that a lane reads a block's eight dimensions with one 8-byte load.  Decode as DecodeSplitBlockLane
 // (e8q_scan.cpp:34-90): d_i = rowA bit (7-i) << (b-1) | field_i; digit 7's
 // field LSB is the parity of digits 0..6's and its stored LSB the branch.
 
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void MfmaCodesKernel(IndexView ix, uint64_t re
         uint64_t w = 0;
         for (int i = 0; i < 8; ++i) {
             const uint32_t l = br ? Q - 1u - d[i] : d[i];
-            w |= static_cast<uint64_t>(l | (br << 7)) << (8 * i);
+            w |= static_cast<uint64_t>(l | (br << 4)) << (8 * i);
         }
         *reinterpret_cast<uint64_t*>(out + static_cast<uint64_t>(blk) * 256) = w;
     }
@@ -213,11 +213,18 @@ hipError_t LaunchMfmaQuery(hipStream_t st, const IndexView& ix, const float* rot
 
 constexpr int kMfmaTiles = 4;
 
-__device__ __forceinline__ u32x4 OneHot(uint32_t code, uint32_t h) {
-    const uint32_t local = (code & 15u) - 8u * h;   // wraps above 7 when outside this half
-    const uint32_t v = local < 8u ? ((0x3F80u | ((code >> 7) << 15)) << ((local & 1u) << 4)) : 0u;
-    const uint32_t dw = local >> 1;
-    return u32x4{dw == 0 ? v : 0u, dw == 1 ? v : 0u, dw == 2 ? v : 0u, dw == 3 ? v : 0u};
+// The one-hot B slices: entry 32 h + code holds the 8 bf16 of levels
+// 8 h .. 8 h + 7 for a folded code (level l, branch s): -1 or +1 at l, else 0.
+__device__ __forceinline__ void FillOneHotTable(u32x4* tab) {
+    for (uint32_t e = threadIdx.x; e < 64; e += blockDim.x) {
+        const uint32_t h = e >> 5, code = e & 31, l = code & 15, s = code >> 4;
+        uint32_t dw[4] = {0u, 0u, 0u, 0u};
+        if (l >= 8 * h && l < 8 * h + 8) {
+            const uint32_t j = l - 8 * h;
+            dw[j >> 1] = (s ? 0xBF80u : 0x3F80u) << (16 * (j & 1));
+        }
+        tab[e] = u32x4{dw[0], dw[1], dw[2], dw[3]};
+    }
 }
 
 __device__ __forceinline__ f32x16 Mfma(u32x4 a, u32x4 b, f32x16 c) {
@@ -252,8 +259,11 @@ __global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams*
     const MfmaExtra& m = prm->m;
     __shared__ uint32_t s_it, s_q[kMfmaQTile], s_thr[kMfmaQTile];
     __shared__ float s_d2[kMfmaQTile];
+    __shared__ u32x4 s_onehot[64];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t h = lane >> 5, n = lane & 31;
+    FillOneHotTable(s_onehot);
+    const u32x4* onehot = s_onehot + 32 * h;   // this half's levels
     const uint32_t P = static_cast<uint32_t>(ix.padded);
     const int B = ix.blocks;
     const uint32_t k = a.k;
@@ -300,11 +310,12 @@ __global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams*
                 }
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const u32x4 zero = {0u, 0u, 0u, 0u};
-                    const u32x4 xh = qv ? ahi[e] : zero, xl = qv ? alo[e] : zero;
+                    // Rows of pairs past np hold another query's values: their
+                    // results are never read.
+                    const u32x4 xh = ahi[e], xl = alo[e];
                     u32x4 bv[kMfmaTiles];
 #pragma unroll
-                    for (int t = 0; t < kMfmaTiles; ++t) bv[t] = OneHot(static_cast<uint32_t>(w[t] >> (8 * e)) & 0xFFu, h);
+                    for (int t = 0; t < kMfmaTiles; ++t) bv[t] = onehot[static_cast<uint32_t>(w[t] >> (8 * e)) & 31u];
                     // hi for every tile, then lo: no MFMA waits on the previous one's result
 #pragma unroll
                     for (int t = 0; t < kMfmaTiles; ++t)
