@@ -126,8 +126,12 @@ __global__ void QueryPrepKernel(const float* __restrict__ rotq, uint64_t nq, Ind
 // Route distances d2[q][c] = (qn2 + cn2) - 2*dot, dot in the AVX2
 // RouteGemv order: lane l of a 32-lane group accumulates the FMA chain over
 // d = l, l+32, ...; then ((a0+a1)+(a2+a3)) per 8-lane slice and the Hsum.
-// Grid: x = centroid tiles of 8 (one 32-lane group each), y = query tiles.
-constexpr int kRouteQT = 8;
+// Grid: x = centroid tiles of 32 (eight 32-lane groups of four centroids),
+// y = query tiles of 16 (staged in LDS).  A lane runs the chains of its four
+// centroids against the 16 queries at once: per dimension step four centroid
+// loads and 16 LDS reads feed 64 FMAs.  Each (query, centroid) pair keeps
+// its own chain and reduction order.
+constexpr int kRouteCT = 4;   // centroids per 32-lane group
 
 __global__ void RouteNoDimKernel(RouteArgs ra) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -138,7 +142,8 @@ __global__ void RouteNoDimKernel(RouteArgs ra) {
     ra.dist[i] = d2;
 }
 
-__global__ void RouteDistKernel(RouteArgs ra) {
+template <int kRouteQT>
+__global__ __launch_bounds__(256) void RouteDistKernel(RouteArgs ra) {
     extern __shared__ float qs[];   // kRouteQT * D
     const uint32_t D = ra.dim;
     const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * kRouteQT;
@@ -149,30 +154,43 @@ __global__ void RouteDistKernel(RouteArgs ra) {
     }
     __syncthreads();
     const uint32_t lane = threadIdx.x & 31;
-    const uint64_t c = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 32) + (threadIdx.x >> 5);
-    const bool valid = c < ra.nlist;
-    const float* cc = ra.cent + (valid ? c : 0) * ra.cstride;
-    float acc[kRouteQT];
+    const uint64_t c0 = (static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 32) + (threadIdx.x >> 5)) * kRouteCT;
+    const float* cc[kRouteCT];
 #pragma unroll
-    for (int t = 0; t < kRouteQT; ++t) acc[t] = 0.0F;
+    for (int u = 0; u < kRouteCT; ++u) cc[u] = ra.cent + (c0 + u < ra.nlist ? c0 + u : 0) * ra.cstride;
+    float acc[kRouteCT][kRouteQT];
+#pragma unroll
+    for (int u = 0; u < kRouteCT; ++u)
+#pragma unroll
+        for (int t = 0; t < kRouteQT; ++t) acc[u][t] = 0.0F;
     for (uint32_t d = lane; d < D; d += 32) {
-        const float cv = cc[d];
+        float cv[kRouteCT];
 #pragma unroll
-        for (int t = 0; t < kRouteQT; ++t) acc[t] = fmaf(qs[t * D + d], cv, acc[t]);
+        for (int u = 0; u < kRouteCT; ++u) cv[u] = cc[u][d];
+#pragma unroll
+        for (int t = 0; t < kRouteQT; ++t) {
+            const float qv = qs[t * D + d];
+#pragma unroll
+            for (int u = 0; u < kRouteCT; ++u) acc[u][t] = fmaf(qv, cv[u], acc[u][t]);
+        }
     }
 #pragma unroll
-    for (int t = 0; t < kRouteQT; ++t) {
-        float v = acc[t];
-        v = v + __shfl_down(v, 8, 32);    // lanes 0..7: a0+a1, lanes 16..23: a2+a3
-        v = v + __shfl_down(v, 16, 32);   // lanes 0..7: (a0+a1)+(a2+a3)
-        v = v + __shfl_down(v, 4, 32);    // lanes 0..3: s[i]+s[i+4]
-        v = v + __shfl_down(v, 2, 32);    // lanes 0..1
-        v = v + __shfl_down(v, 1, 32);    // lane 0
-        if (lane == 0 && valid && t < nqt) {
-            const float qn2 = ra.qn2[(q0 + t) * ra.qn2_stride];
-            float d2 = fmaf(-v, 2.0F, ra.cn2[c] + qn2);
-            if (d2 < 0.0F) d2 = 0.0F;
-            ra.dist[(q0 + t) * ra.nlist + c] = d2;
+    for (int u = 0; u < kRouteCT; ++u) {
+        const uint64_t c = c0 + u;
+#pragma unroll
+        for (int t = 0; t < kRouteQT; ++t) {
+            float v = acc[u][t];
+            v = v + __shfl_down(v, 8, 32);    // lanes 0..7: a0+a1, lanes 16..23: a2+a3
+            v = v + __shfl_down(v, 16, 32);   // lanes 0..7: (a0+a1)+(a2+a3)
+            v = v + __shfl_down(v, 4, 32);    // lanes 0..3: s[i]+s[i+4]
+            v = v + __shfl_down(v, 2, 32);    // lanes 0..1
+            v = v + __shfl_down(v, 1, 32);    // lane 0
+            if (lane == 0 && c < ra.nlist && t < nqt) {
+                const float qn2 = ra.qn2[(q0 + t) * ra.qn2_stride];
+                float d2 = fmaf(-v, 2.0F, ra.cn2[c] + qn2);
+                if (d2 < 0.0F) d2 = 0.0F;
+                ra.dist[(q0 + t) * ra.nlist + c] = d2;
+            }
         }
     }
 }
@@ -311,15 +329,23 @@ hipError_t LaunchRouteDist(hipStream_t st, const RouteArgs& ra) {
         hipLaunchKernelGGL(RouteNoDimKernel, dim3(static_cast<uint32_t>((tot + 255) / 256)), dim3(256), 0, st, ra);
         return hipGetLastError();
     }
-    const dim3 grid(static_cast<uint32_t>((ra.nlist + 7) / 8), static_cast<uint32_t>((ra.nq + kRouteQT - 1) / kRouteQT));
-    const size_t lds = kRouteQT * static_cast<size_t>(ra.dim) * sizeof(float);
-    if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&RouteDistKernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(RouteDistKernel, grid, dim3(256), lds, st, ra);
-    return hipGetLastError();
+    const uint64_t per_block = 8 * kRouteCT;   // eight 32-lane groups
+    auto go = [&](auto kern, uint32_t qt) -> hipError_t {
+        const dim3 grid(static_cast<uint32_t>((ra.nlist + per_block - 1) / per_block),
+                        static_cast<uint32_t>((ra.nq + qt - 1) / qt));
+        const size_t lds = qt * static_cast<size_t>(ra.dim) * sizeof(float);
+        if (lds > 160 * 1024) return hipErrorInvalidValue;
+        if (lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, ra);
+        return hipGetLastError();
+    };
+    // 16 queries per tile; 4 above D = 2560 (the query tile lives in LDS).
+    if (16ull * ra.dim * sizeof(float) <= 160 * 1024) return go(RouteDistKernel<16>, 16);
+    return go(RouteDistKernel<4>, 4);
 }
 
 size_t SelectProbesLds(uint32_t nlist, uint32_t maxp) {
