@@ -5,7 +5,7 @@ Workload (the plain GPU E8Q configuration): N = 1M base vectors, D = 128,
 float32 Gaussian mixture (1024 centers ~ N(0, 4I), points = center + N(0, I)),
 IVF lists = the 1024 mixture centers with cluster_ids = nearest center,
 E8Q b = 4, 10k queries from the same mixture, k = 100, nprobe grid
-2..128 (powers of two).  Synthetic data generated on the GPU from fixed seeds.
+16/32/64/128.  Synthetic data generated on the GPU from fixed seeds.
 
 A step is one search call over the 10k-query batch (queries and outputs
 resident in HBM).  For every search form and nprobe the script runs W warm-up
@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--k", type=int, default=None, help="100 (e8q, me8q) / 10 (zipf)")
     ap.add_argument("--bits", type=int, default=4)
-    ap.add_argument("--grid", default=None, help="2,4,...,128 (e8q) / 32,64,128,256 (me8q)")
+    ap.add_argument("--grid", default=None, help="16,32,64,128 (e8q) / 32,64,128,256 (me8q)")
     ap.add_argument("--avg-bits", type=float, default=4.0)
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--modes", default=None,
@@ -85,7 +85,7 @@ def parse():
     a.modes = a.modes or ("plain,batched,mfma,mfma_graph" if zipf else "plain,batched,batched_graph")
     a.dim = a.dim or (960 if me else 768 if zipf else 128)
     a.nlist = a.nlist or (4096 if me else 16384 if zipf else 1024)
-    a.grid = a.grid or ("32,64,128,256" if me else "2,4,8,16,32,64,128")
+    a.grid = a.grid or ("32,64,128,256" if me else "16,32,64,128")
     return a
 
 
@@ -105,11 +105,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_data(torch, n, dim, nlist, nq, seed=42):
+def make_data(torch, n, dim, nlist, nq, seed=42, center_sd=2.0):
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
-    centers = torch.randn(nlist, dim, generator=g, device=dev) * 2.0
+    centers = torch.randn(nlist, dim, generator=g, device=dev) * center_sd
     lab = torch.randint(0, nlist, (n,), generator=g, device=dev)
     base = centers[lab] + torch.randn(n, dim, generator=g, device=dev)
     qlab = torch.randint(0, nlist, (nq,), generator=g, device=dev)
@@ -400,7 +400,7 @@ def run_serve(args):
         base, centers, cid, queries = make_data(torch, args.n, args.dim, args.nlist, args.nq)
         idx = e8qlib.E8qIndex(args.dim, args.bits, 42)
         idx.build(base.cpu().numpy(), centers.cpu().numpy(), cid.to(torch.int32).cpu().numpy().astype(np.uint32))
-        workload = f"E8Q N={args.n} D={args.dim} {args.nlist} lists b={args.bits}"
+        workload = f"E8Q N={args.n} D={args.dim} {args.nlist} lists (centers N(0,4I)) b={args.bits}"
     log(f"index: {workload}, {time.perf_counter() - t0:.1f} s")
     qpool = queries.cpu().numpy()
     del base
@@ -483,30 +483,35 @@ def run_serve(args):
     print(json.dumps(line), flush=True)
 
 
-def make_zipf_data(torch, n, dim, nlist, nq, seed=7, s=1.1):
-    """List sizes proportional to c^-s over nlist lists (summing to n), centers ~
-    N(0, 4I), points = center + N(0, I), cluster_ids = the generating list,
-    centroids = per-list means; queries drawn like the points (list chosen in
-    proportion to its size)."""
+def make_zipf_data(torch, n, dim, nlist, nq, seed=7, s=1.1, spectrum=1.0):
+    """Unit-normalized anisotropic vectors in Zipf-sized lists: list sizes
+    proportional to c^-s over nlist lists (summing to n); per-dimension
+    standard deviations sqrt(lambda_i), lambda_i = i^-spectrum; centers ~
+    N(0, 4 Lambda), points = center + N(0, Lambda), then scaled to unit norm;
+    cluster_ids = the generating list, centroids = per-list means; queries
+    drawn like the points (list chosen in proportion to its size)."""
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     w = torch.arange(1, nlist + 1, dtype=torch.float64) ** (-s)
     sizes = torch.floor(w / w.sum() * n).to(torch.int64)
     sizes[:n - int(sizes.sum())] += 1
-    centers = torch.randn(nlist, dim, generator=g, device=dev) * 2.0
+    sd = (torch.arange(1, dim + 1, dtype=torch.float64, device=dev) ** (-spectrum)).sqrt().float()
+    centers = torch.randn(nlist, dim, generator=g, device=dev) * (2.0 * sd)
     lab = torch.repeat_interleave(torch.arange(nlist, device=dev), sizes.to(dev))
     base = torch.empty(n, dim, device=dev)
     for st in range(0, n, 1 << 20):
         e = min(n, st + (1 << 20))
-        base[st:e] = centers[lab[st:e]] + torch.randn(e - st, dim, generator=g, device=dev)
+        x = centers[lab[st:e]] + torch.randn(e - st, dim, generator=g, device=dev) * sd
+        base[st:e] = x / x.norm(dim=1, keepdim=True)
     cent = torch.zeros(nlist, dim, device=dev, dtype=torch.float64)
     for st in range(0, n, 1 << 20):
         e = min(n, st + (1 << 20))
         cent.index_add_(0, lab[st:e], base[st:e].double())
     cent = (cent / sizes.to(dev).clamp(min=1)[:, None].double()).float()
     pick = torch.randint(0, n, (nq,), generator=g, device=dev)
-    queries = centers[lab[pick]] + torch.randn(nq, dim, generator=g, device=dev)
+    queries = centers[lab[pick]] + torch.randn(nq, dim, generator=g, device=dev) * sd
+    queries = queries / queries.norm(dim=1, keepdim=True)
     return base, cent.contiguous(), lab.to(torch.int32), queries.contiguous(), sizes
 
 
@@ -614,7 +619,7 @@ def run_zipf(args):
         "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(results[best]["ms_per_step"], 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 hi+lo MFMA / fp32 accumulate (mfma); u8 / i32 / fp32 (plain, batched)",
-        "data": "synthetic Zipf(1.1) Gaussian mixture generated on the GPU (seed 7)",
+        "data": "synthetic unit-normalized anisotropic mixture (lambda_i ~ i^-1), Zipf(1.1) list sizes, generated on the GPU (seed 7)",
         "config": {"workload": f"batched MFMA E8Q: N={args.n}, D={args.dim}, {args.nlist} Zipf lists, b={args.bits}, "
                                f"{nq} queries, k={k}, nprobe per query uniform in {{16,32,64,128}}",
                    "form": best, "largest_list": int(sizes.max()), "smallest_list": int(sizes.min())},
@@ -764,7 +769,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8 LUT / i32 coarse sums, fp32 refine",
-        "data": "synthetic Gaussian mixture generated on the GPU (seed 42)",
+        "data": "synthetic Gaussian mixture generated on the GPU (seed 42, centers N(0, 4I), points center + N(0, I))",
         "config": {"workload": "plain GPU E8Q scan: N=1M, D=128 Gaussian mixture, 1024 lists, b=4, "
                                f"10k queries, k=100, nprobe {args.grid}",
                    "n": args.n, "dim": args.dim, "nlist": args.nlist, "nq": nq, "k": k, "bits": args.bits,
