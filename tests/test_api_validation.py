@@ -53,3 +53,14 @@ def test_load_rejects_bad_files(tmp_path):
         e8qlib.E8qIndex.load(str(p))
     with pytest.raises(RuntimeError, match="cannot open index file"):
         e8qlib.E8qIndex.load(str(tmp_path / "missing.e8ax"))
+
+
+def test_extension_surface():
+    # Search forms, launch options and serving beside the reference's surface.
+    for name in ("set_search_graphs", "set_batched_shape", "search_context_info", "Server"):
+        assert hasattr(e8qlib, name), name
+    ix = e8qlib.E8qIndex(16, 4)
+    for name in ("search_device", "build_device"):
+        assert hasattr(ix, name), name
+    with pytest.raises(ValueError, match="null device pointer"):
+        ix.build_device(0, 0, 2, np.zeros(4, np.uint32), 0)
