@@ -260,6 +260,11 @@ __global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams*
     __shared__ uint32_t s_it, s_q[kMfmaQTile], s_thr[kMfmaQTile];
     __shared__ float s_d2[kMfmaQTile];
     __shared__ u32x4 s_onehot[64];
+    // The query operand of one 8-dimension block for the item's 32 pairs, two
+    // buffers: [buffer][dimension][pair][4 x 16 bytes: hi 0-7, hi 8-15, lo 0-7,
+    // lo 8-15] with a 16-byte pad per row (rows 80 bytes apart spread a
+    // half-wave's reads over the banks).
+    __shared__ u32x4 s_a[2 * 8 * kMfmaQTile * 5];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t h = lane >> 5, n = lane & 31;
     FillOneHotTable(s_onehot);
@@ -286,33 +291,44 @@ __global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams*
             s_thr[threadIdx.x] = m.thr[q];
         }
         __syncthreads();
-        if (wave < nb) {   // wave-uniform: this wave's tiles are batches wave + 4 t, t < nt
-            const uint32_t nt = min(static_cast<uint32_t>(kMfmaTiles), (nb - wave + 3) / 4);
-            const uint64_t ccnt = G(ix.cluster_begin)[c + 1] - G(ix.cluster_begin)[c];
-            const uint64_t rec0 = G(ix.arena_begin)[c] / ix.batch_bytes + b0 + wave;
-            const E8Q_GLOBAL uint8_t* code[kMfmaTiles];
+        // Every wave stages the query operand; waves past nb then skip the
+        // contraction (wave-uniform).  This wave's tiles: batches wave + 4 t.
+        const bool active = wave < nb;
+        const uint32_t nt = active ? min(static_cast<uint32_t>(kMfmaTiles), (nb - wave + 3) / 4) : 0u;
+        const uint64_t ccnt = G(ix.cluster_begin)[c + 1] - G(ix.cluster_begin)[c];
+        const uint64_t rec0 = G(ix.arena_begin)[c] / ix.batch_bytes + b0 + (active ? wave : 0u);
+        const E8Q_GLOBAL uint8_t* code[kMfmaTiles];
 #pragma unroll
-            for (int t = 0; t < kMfmaTiles; ++t)   // a tile past nt reads the first one (unused)
-                code[t] = G(m.mcodes) + (rec0 + 4 * (t < static_cast<int>(nt) ? t : 0)) * 32ull * P + n * 8;
-            const bool qv = n < np;
-            const E8Q_GLOBAL uint16_t* qa = G(m.qexp) + static_cast<uint64_t>(s_q[qv ? n : 0]) * P * 32 + 8 * h;
-            f32x16 acc[kMfmaTiles] = {};
-            for (int blk = 0; blk < B; ++blk) {
+        for (int t = 0; t < kMfmaTiles; ++t)   // a tile past nt reads the first one (unused)
+            code[t] = G(m.mcodes) + (rec0 + 4 * (t < static_cast<int>(nt) ? t : 0)) * 32ull * P + n * 8;
+        // Staging: thread t carries pair t / 8, dimension t % 8 of each block
+        // (pairs past np carry the first pair's row: computed, never read).
+        const uint32_t mq = threadIdx.x >> 3, eq = threadIdx.x & 7;
+        const E8Q_GLOBAL u32x4* asrc = reinterpret_cast<const E8Q_GLOBAL u32x4*>(G(m.qexp)) +
+                                       (static_cast<uint64_t>(s_q[mq < np ? mq : 0]) * P + eq) * 4;
+        u32x4* adst = s_a + (eq * kMfmaQTile + mq) * 5;   // + buffer * 8 * 32 * 5
+        u32x4 pf[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pf[i] = asrc[i];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) adst[i] = pf[i];
+        __syncthreads();
+        f32x16 acc[kMfmaTiles] = {};
+        for (int blk = 0; blk < B; ++blk) {
+            const uint32_t cur = static_cast<uint32_t>(blk) & 1u;
+            const bool more = blk + 1 < B;
+            if (more) {   // the next block's operand, in flight during this block's MFMAs
+#pragma unroll
+                for (int i = 0; i < 4; ++i) pf[i] = asrc[static_cast<uint64_t>(blk + 1) * 32 + i];
+            }
+            if (active) {
                 uint64_t w[kMfmaTiles];
 #pragma unroll
                 for (int t = 0; t < kMfmaTiles; ++t) w[t] = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(code[t] + blk * 256);
-                const E8Q_GLOBAL u32x4* ap = reinterpret_cast<const E8Q_GLOBAL u32x4*>(qa + blk * 8 * 32);
-                u32x4 ahi[8], alo[8];
+                const u32x4* arow = s_a + (cur * 8 * kMfmaQTile + n) * 5;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    ahi[e] = ap[e * 4];
-                    alo[e] = ap[e * 4 + 2];
-                }
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    // Rows of pairs past np hold another query's values: their
-                    // results are never read.
-                    const u32x4 xh = ahi[e], xl = alo[e];
+                    const u32x4 xh = arow[e * kMfmaQTile * 5 + h], xl = arow[e * kMfmaQTile * 5 + 2 + h];
                     u32x4 bv[kMfmaTiles];
 #pragma unroll
                     for (int t = 0; t < kMfmaTiles; ++t) bv[t] = onehot[static_cast<uint32_t>(w[t] >> (8 * e)) & 31u];
@@ -325,6 +341,14 @@ __global__ __launch_bounds__(kMfmaThreads) void MfmaScanKernel(const MfmaParams*
                         if (t < static_cast<int>(nt)) acc[t] = Mfma(xl, bv[t], acc[t]);
                 }
             }
+            if (more) {   // the other buffer was last read before the previous barrier
+                u32x4* d = adst + (cur ^ 1u) * 8 * kMfmaQTile * 5;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) d[i] = pf[i];
+            }
+            __syncthreads();
+        }
+        if (active) {
             // Epilogue: distances, the bound, the candidate test, appends.
             uint32_t tk[16];   // this lane's rows' bounds
 #pragma unroll
