@@ -34,6 +34,12 @@ py::array_t<T, py::array::c_style | py::array::forcecast> EnsureArr(py::handle v
 
 using e8qlib::me8q::Me8qIndex;
 
+int ParseMode(const std::string& mode) {
+    if (mode == "plain") return 0;
+    if (mode == "batched") return 1;
+    throw std::invalid_argument("mode must be 'plain' or 'batched'");
+}
+
 class PyMe8qIndex {
    public:
     PyMe8qIndex(size_t dim, float avg_bits, uint64_t seed) : dim_(dim), avg_bits_(avg_bits), seed_(seed) {
@@ -77,8 +83,9 @@ class PyMe8qIndex {
     }
 
     py::tuple search(py::handle queries, size_t k, size_t nprobe, size_t num_threads, bool return_stats,
-                     py::object nprobe_per_query) {
+                     py::object nprobe_per_query, const std::string& mode) {
         (void)num_threads;
+        const int m = ParseMode(mode);
         auto q = EnsureArr<float>(queries, "queries", 2);
         if (!built_ || !index_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
         if (static_cast<size_t>(q.shape(1)) != dim_) throw std::invalid_argument("query dimension does not match index dim");
@@ -103,7 +110,7 @@ class PyMe8qIndex {
         if (nq > 0) {
             py::gil_scoped_release nogil;
             index_->SearchDevice(q.data(), nq, k, nprobe, npq_ptr, ids.mutable_data(), dists.mutable_data(), &st,
-                                 false, false);
+                                 false, false, nullptr, m);
         }
         if (!return_stats) return py::make_tuple(ids, dists);
         const double dn = nq > 0 ? static_cast<double>(nq) : 1.0;
@@ -126,11 +133,16 @@ class PyMe8qIndex {
         s["device_scan_ms"] = st.device_scan_ms;
         s["device_total_ms"] = st.device_total_ms;
         s["batches"] = st.batches;
+        s["refined"] = st.refined;
+        s["form"] = st.form == 1 ? "batched" : "plain";
+        s["overflowed"] = st.overflowed;
+        s["pairs_scored"] = st.pairs_scored;
         return py::make_tuple(ids, dists, s);
     }
 
     py::dict search_device(uintptr_t q_ptr, size_t nq, size_t k, size_t nprobe, uintptr_t ids_ptr,
-                           uintptr_t dists_ptr, py::object nprobe_per_query) {
+                           uintptr_t dists_ptr, py::object nprobe_per_query, const std::string& mode) {
+        const int m = ParseMode(mode);
         if (!built_ || !index_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
         if (k == 0) throw std::invalid_argument("k must be positive");
         if (nprobe == 0) throw std::invalid_argument("nprobe must be positive");
@@ -147,7 +159,7 @@ class PyMe8qIndex {
             py::gil_scoped_release nogil;
             index_->SearchDevice(reinterpret_cast<const float*>(q_ptr), nq, k, nprobe, npq_ptr,
                                  reinterpret_cast<uint32_t*>(ids_ptr), reinterpret_cast<float*>(dists_ptr), &st, true,
-                                 true);
+                                 true, nullptr, m);
         }
         py::dict s;
         s["candidates"] = st.candidates;
@@ -160,6 +172,9 @@ class PyMe8qIndex {
         s["groups"] = st.groups;
         s["device_scan_ms"] = st.device_scan_ms;
         s["device_total_ms"] = st.device_total_ms;
+        s["form"] = st.form == 1 ? "batched" : "plain";
+        s["overflowed"] = st.overflowed;
+        s["pairs_scored"] = st.pairs_scored;
         return s;
     }
 
@@ -213,10 +228,11 @@ void RegisterMe8q(py::module_& m) {
              py::arg("num_threads") = 1, py::arg("variance") = py::none(), py::arg("drop_energy") = true,
              py::arg("prune_alpha") = 0.0F, py::arg("plan_spec") = "", py::arg("extra_budget_bits") = 0.0)
         .def("search", &PyMe8qIndex::search, py::arg("queries"), py::arg("k"), py::arg("nprobe"),
-             py::arg("num_threads") = 1, py::arg("return_stats") = false, py::arg("nprobe_per_query") = py::none())
+             py::arg("num_threads") = 1, py::arg("return_stats") = false, py::arg("nprobe_per_query") = py::none(),
+             py::arg("mode") = "plain")
         .def("search_device", &PyMe8qIndex::search_device, py::arg("queries_ptr"), py::arg("nq"), py::arg("k"),
              py::arg("nprobe"), py::arg("out_ids_ptr"), py::arg("out_dists_ptr"),
-             py::arg("nprobe_per_query") = py::none())
+             py::arg("nprobe_per_query") = py::none(), py::arg("mode") = "plain")
         .def("save", &PyMe8qIndex::save, py::arg("path"))
         .def_static("load", &PyMe8qIndex::load, py::arg("path"))
         .def_property_readonly("dim", &PyMe8qIndex::dim)

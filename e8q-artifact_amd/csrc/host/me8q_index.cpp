@@ -453,7 +453,7 @@ size_t Me8qIndex::device_bytes() const { return dev_ ? e8q_m8_device_bytes(dev_)
 
 void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t nprobe, const uint32_t* nprobe_per_query,
                              uint32_t* out_ids, float* out_dists, SearchStats* stats, bool queries_on_device,
-                             bool outputs_on_device, uint64_t* per_query_stats) const {
+                             bool outputs_on_device, uint64_t* per_query_stats, int mode) const {
     if (!dev_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
     if (k > kMaxDeviceK) throw std::invalid_argument("k above the device limit of 1024");
     e8q_m8_search_args a{};
@@ -467,6 +467,7 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
     a.out_ids = out_ids;
     a.out_dists = out_dists;
     a.outputs_on_device = outputs_on_device ? 1 : 0;
+    a.mode = mode;
     e8q_m8_stats st{};
     {
         std::lock_guard<std::mutex> lk(DeviceMutex());
@@ -484,11 +485,15 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
         stats->batches += st.batches;
         stats->refined += st.refined;
         stats->groups += st.groups;
+        stats->form = st.form;
+        stats->overflowed += st.overflowed;
+        stats->pairs_scored += st.pairs_scored;
     }
 }
 
 void Me8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k,
-                                const uint32_t* nprobe_per_query, uint32_t* out_ids, float* out_dists) const {
+                                const uint32_t* nprobe_per_query, uint32_t* out_ids, float* out_dists,
+                                int mode) const {
     if (!dev_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
     if (k == 0 || k > kMaxDeviceK) throw std::invalid_argument("k must be in [1, 1024] on the device");
     e8q_m8_search_args a{};
@@ -500,11 +505,12 @@ void Me8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, s
     a.z = hostmath::OrthoQuantile(static_cast<double>(ortho_alpha_), std::max<size_t>(4 * k, 32));
     a.out_ids = out_ids;
     a.out_dists = out_dists;
+    a.mode = mode;
     Check(e8q_m8_search(ctx, dev_, &a, nullptr, nullptr));
 }
 
-uint64_t Me8qIndex::SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const {
-    return dev_ ? e8q_m8_search_workspace_bytes(dev_, nq, k, max_nprobe) : 0;
+uint64_t Me8qIndex::SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe, int mode) const {
+    return dev_ ? e8q_m8_search_workspace_bytes(dev_, nq, k, max_nprobe, mode) : 0;
 }
 
 void Me8qIndex::SearchIVFFastBatch(const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* out_ids,

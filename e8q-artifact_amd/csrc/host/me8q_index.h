@@ -78,6 +78,9 @@ struct SearchStats {
     size_t batches = 0;   // 32-vector scan records read
     size_t refined = 0;   // candidates that entered the exact cascade
     size_t groups = 0;    // cascade groups (up to four candidates)
+    int form = 0;         // the device form that ran (E8Q_MODE_PLAIN / E8Q_MODE_BATCHED)
+    size_t overflowed = 0;     // batched form: queries searched again by the plain form
+    size_t pairs_scored = 0;   // batched form: (query, vector) pairs of the MFMA contraction
 };
 
 class Me8qIndex {
@@ -100,16 +103,17 @@ class Me8qIndex {
     void SearchIVFFastBatch(const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* out_ids,
                             SearchStats* stats, float* out_dists, int num_threads) const;
     // The device entry: nprobe_per_query (nullable) overrides nprobe; queries
-    // / outputs may be device pointers; per_query_stats (nullable) [nq*5].
+    // / outputs may be device pointers; per_query_stats (nullable) [nq*5];
+    // mode: E8Q_MODE_PLAIN (the reference's schedule) or E8Q_MODE_BATCHED.
     void SearchDevice(const float* queries, size_t nq, size_t k, size_t nprobe, const uint32_t* nprobe_per_query,
                       uint32_t* out_ids, float* out_dists, SearchStats* stats, bool queries_on_device,
-                      bool outputs_on_device, uint64_t* per_query_stats = nullptr) const;
+                      bool outputs_on_device, uint64_t* per_query_stats = nullptr, int mode = 0) const;
 
     // Search on a caller-owned device context (a serving worker's): no
     // process-wide lock.  Host queries and outputs, per-query nprobe.
     void SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k, const uint32_t* nprobe_per_query,
-                         uint32_t* out_ids, float* out_dists) const;
-    [[nodiscard]] uint64_t SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const;
+                         uint32_t* out_ids, float* out_dists, int mode = 0) const;
+    [[nodiscard]] uint64_t SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe, int mode = 0) const;
 
     [[nodiscard]] size_t n() const { return n_; }
     [[nodiscard]] size_t dim() const { return dim_; }

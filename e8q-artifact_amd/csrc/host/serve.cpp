@@ -7,6 +7,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <thread>
+#include <type_traits>
 
 #include "e8q_gpu.h"
 #include "e8q_index.h"
@@ -215,21 +216,25 @@ void CheckSt(e8q_status st) {
 template <class Index>
 class DeviceBackend final : public Backend {
    public:
-    explicit DeviceBackend(const Index& ix) : ix_(ix) { CheckSt(e8q_ctx_create(DeviceOrdinal(), &ctx_)); }
+    DeviceBackend(const Index& ix, int mode) : ix_(ix), mode_(mode) { CheckSt(e8q_ctx_create(DeviceOrdinal(), &ctx_)); }
     ~DeviceBackend() override {
         if (ctx_) (void)e8q_ctx_destroy(ctx_);
     }
     size_t dim() const override { return ix_.dim(); }
     uint64_t WorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const override {
-        return ix_.SearchWorkspaceBytes(nq, k, max_nprobe);
+        if constexpr (std::is_same_v<Index, me8q::Me8qIndex>) return ix_.SearchWorkspaceBytes(nq, k, max_nprobe, mode_);
+        else return ix_.SearchWorkspaceBytes(nq, k, max_nprobe);
     }
     void Reserve(uint64_t bytes) override { CheckSt(e8q_ctx_reserve(ctx_, bytes)); }
     void Search(const BatchView& b) override {
-        ix_.SearchOnContext(ctx_, b.queries, b.nq, b.k, b.nprobe, b.ids, b.dists);
+        if constexpr (std::is_same_v<Index, me8q::Me8qIndex>)
+            ix_.SearchOnContext(ctx_, b.queries, b.nq, b.k, b.nprobe, b.ids, b.dists, mode_);
+        else ix_.SearchOnContext(ctx_, b.queries, b.nq, b.k, b.nprobe, b.ids, b.dists);
     }
 
    private:
     const Index& ix_;
+    int mode_ = 0;
     e8q_ctx* ctx_ = nullptr;
 };
 
@@ -237,13 +242,13 @@ class DeviceBackend final : public Backend {
 
 std::unique_ptr<Server> MakeE8qServer(const E8qIndex& ix, const Options& opt) {
     std::vector<std::unique_ptr<Backend>> w;
-    for (int i = 0; i < opt.workers; ++i) w.push_back(std::make_unique<DeviceBackend<E8qIndex>>(ix));
+    for (int i = 0; i < opt.workers; ++i) w.push_back(std::make_unique<DeviceBackend<E8qIndex>>(ix, 0));
     return std::make_unique<Server>(std::move(w), opt);
 }
 
 std::unique_ptr<Server> MakeMe8qServer(const me8q::Me8qIndex& ix, const Options& opt) {
     std::vector<std::unique_ptr<Backend>> w;
-    for (int i = 0; i < opt.workers; ++i) w.push_back(std::make_unique<DeviceBackend<me8q::Me8qIndex>>(ix));
+    for (int i = 0; i < opt.workers; ++i) w.push_back(std::make_unique<DeviceBackend<me8q::Me8qIndex>>(ix, opt.mode));
     return std::make_unique<Server>(std::move(w), opt);
 }
 

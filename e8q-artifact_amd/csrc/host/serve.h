@@ -60,6 +60,9 @@ struct Options {
     uint64_t workspace_bytes = 0;
     // Queries waiting to be admitted; Submit blocks while the queue is full.
     size_t max_queued_queries = size_t{1} << 20;
+    // Device form of an ME8Q index's batches: E8Q_MODE_PLAIN (0) or
+    // E8Q_MODE_BATCHED (1).  An E8Q index always runs the plain form.
+    int mode = 0;
 };
 
 struct Completed {

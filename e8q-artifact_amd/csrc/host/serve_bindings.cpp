@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "e8q_gpu.h"
 #include "e8q_index.h"
 #include "me8q_index.h"
 #include "serve.h"
@@ -27,13 +28,16 @@ using e8qlib::serve::Server;
 class PyServer {
    public:
     PyServer(py::object index, int workers, size_t max_batch, size_t max_k, size_t max_nprobe, uint64_t workspace_bytes,
-             size_t max_queued)
+             size_t max_queued, const std::string& mode)
         : index_(std::move(index)) {
         const std::string kind = py::str(index_.attr("__class__").attr("__name__"));
         const uintptr_t ptr = index_.attr("_native")().cast<uintptr_t>();
         if (ptr == 0) throw std::runtime_error("Server: the index must be built or loaded");
-        const Options opt = MakeOptions(workers, max_batch, max_k, max_nprobe, workspace_bytes, max_queued);
+        Options opt = MakeOptions(workers, max_batch, max_k, max_nprobe, workspace_bytes, max_queued);
+        if (mode == "batched") opt.mode = E8Q_MODE_BATCHED;
+        else if (mode != "plain") throw std::invalid_argument("mode must be 'plain' or 'batched'");
         if (kind == "E8qIndex") {
+            if (opt.mode != E8Q_MODE_PLAIN) throw std::invalid_argument("Server: an E8qIndex is served by the plain form");
             if (max_k > e8qlib::kMaxK) throw std::invalid_argument("max_k must be <= 128 for E8qIndex");
             py::gil_scoped_release nogil;
             srv_ = e8qlib::serve::MakeE8qServer(*reinterpret_cast<const e8qlib::E8qIndex*>(ptr), opt);
@@ -132,9 +136,10 @@ class PyServer {
 
 void RegisterServe(py::module_& m) {
     py::class_<PyServer>(m, "Server")
-        .def(py::init<py::object, int, size_t, size_t, size_t, uint64_t, size_t>(), py::arg("index"),
-             py::arg("workers") = 2, py::arg("max_batch") = 4096, py::arg("max_k") = 128, py::arg("max_nprobe") = 256,
-             py::arg("workspace_bytes") = 0, py::arg("max_queued") = size_t{1} << 20)
+        .def(py::init<py::object, int, size_t, size_t, size_t, uint64_t, size_t, const std::string&>(),
+             py::arg("index"), py::arg("workers") = 2, py::arg("max_batch") = 4096, py::arg("max_k") = 128,
+             py::arg("max_nprobe") = 256, py::arg("workspace_bytes") = 0, py::arg("max_queued") = size_t{1} << 20,
+             py::arg("mode") = "plain")
         .def_static("_sim", &PyServer::Sim, py::arg("dim"), py::arg("bytes_per_query"), py::arg("delay_us"),
                     py::arg("workers") = 2, py::arg("max_batch") = 4096, py::arg("max_k") = 128,
                     py::arg("max_nprobe") = 256, py::arg("workspace_bytes") = 0,

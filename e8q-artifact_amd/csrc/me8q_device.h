@@ -122,4 +122,82 @@ __device__ __forceinline__ uint32_t M8ListOf(const uint64_t* begin, uint64_t nli
     return static_cast<uint32_t>(lo);
 }
 
+// One 8-lane group's half of a run score: component `ci` of the blocks of
+// parity `half`, as one FMA chain in block order (the reference keeps even
+// blocks in acc and odd blocks in acc2, me8q_score_avx2.cpp:404-413).  The
+// digit of (block, ci) = its refine field bits (T = rate-1 per digit, digit
+// 7's LSB the parity of digits 0..6's, the branch in the field's top bit)
+// under the plane's top bit; at rate 1 the plane byte is the whole code.
+template <int T>
+__device__ __forceinline__ float M8HalfChainT(const uint8_t* __restrict__ pb, const uint8_t* __restrict__ fr,
+                                              const float* __restrict__ lev, const float* __restrict__ qb,
+                                              uint32_t nblk, uint32_t half, uint32_t ci) {
+    constexpr uint32_t rate = T + 1;
+    uint64_t pml = 0;
+    uint32_t pmh = 0;
+#pragma unroll
+    for (uint32_t jj = 0; jj < 7; ++jj) {
+        if (jj * T < 64) pml |= uint64_t{1} << (jj * T);
+        else pmh |= 1u << (jj * T - 64);
+    }
+    float acc = 0.0F;
+#pragma unroll 2
+    for (uint32_t blk = half; blk < nblk; blk += 2) {
+        const uint32_t pbyte = pb[blk];
+        const uint32_t top = (pbyte >> (7 - ci)) & 1u;
+        uint32_t dgt, br;
+        if constexpr (T == 0) {
+            dgt = ci < 7 ? top : (__popc(pbyte >> 1) & 1u);
+            br = pbyte & 1u;
+        } else {
+            const uint8_t* fb = fr + blk * T;
+            uint64_t flo = 0;
+            uint32_t fhi = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < static_cast<uint32_t>(T); ++b) {
+                const uint32_t byte = fb[b];
+                if (b < 8) flo |= static_cast<uint64_t>(byte) << (8 * b);
+                else fhi |= byte << (8 * (b - 8));
+            }
+            auto field = [&](uint32_t pos, uint32_t nb) -> uint32_t {
+                const uint64_t m = (uint64_t{1} << nb) - 1;
+                if (pos + nb <= 64) return static_cast<uint32_t>((flo >> pos) & m);
+                if (pos >= 64) return static_cast<uint32_t>((fhi >> (pos - 64)) & m);
+                return static_cast<uint32_t>(((flo >> pos) | (static_cast<uint64_t>(fhi) << (64 - pos))) & m);
+            };
+            uint32_t lo;
+            if (ci < 7) {
+                lo = field(T * ci, T);
+            } else {
+                const uint32_t par = (__popcll(flo & pml) + __popc(fhi & pmh)) & 1u;
+                lo = (field(7 * T, T - 1) << 1) | par;
+            }
+            br = field(8 * T - 1, 1);
+            dgt = lo | (top << T);
+        }
+        acc = fmaf(lev[(br << rate) + dgt], qb[blk * 8], acc);
+    }
+    return acc;
+}
+
+__device__ inline float M8HalfChain(uint32_t rate, const uint8_t* pb, const uint8_t* fr, const float* lev, const float* qb,
+                             uint32_t nblk, uint32_t half, uint32_t ci) {
+    switch (rate) {
+        case 1: return M8HalfChainT<0>(pb, fr, lev, qb, nblk, half, ci);
+        case 2: return M8HalfChainT<1>(pb, fr, lev, qb, nblk, half, ci);
+        case 3: return M8HalfChainT<2>(pb, fr, lev, qb, nblk, half, ci);
+        case 4: return M8HalfChainT<3>(pb, fr, lev, qb, nblk, half, ci);
+        case 5: return M8HalfChainT<4>(pb, fr, lev, qb, nblk, half, ci);
+        case 6: return M8HalfChainT<5>(pb, fr, lev, qb, nblk, half, ci);
+        case 7: return M8HalfChainT<6>(pb, fr, lev, qb, nblk, half, ci);
+        case 8: return M8HalfChainT<7>(pb, fr, lev, qb, nblk, half, ci);
+        case 9: return M8HalfChainT<8>(pb, fr, lev, qb, nblk, half, ci);
+        case 10: return M8HalfChainT<9>(pb, fr, lev, qb, nblk, half, ci);
+        case 11: return M8HalfChainT<10>(pb, fr, lev, qb, nblk, half, ci);
+        case 12: return M8HalfChainT<11>(pb, fr, lev, qb, nblk, half, ci);
+        default: return M8HalfChainT<12>(pb, fr, lev, qb, nblk, half, ci);
+    }
+}
+
+
 }  // namespace e8qdev

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 
+#include "e8q_kernels.h"
 #include "me8q_device.h"
 
 namespace e8qdev {
@@ -51,8 +52,72 @@ struct M8ScanArgs {
     uint64_t* qstats;   // [nq][8] candidates, survivors, bits_scored, bits_total, pruned, batches read,
                         //         candidates refined, refine groups
     uint32_t lev_floats;   // total size of the runs' level tables
+    // Threshold pass of the batched form (budget > 0): lists in probe order
+    // until `budget` 32-vector batches are scanned (the last list possibly in
+    // part); cover[q] = (p << 32) | b, the first (probe, batch) not scanned.
+    uint32_t budget;
+    uint64_t* cover;
+    const uint32_t* qsel;   // optional [nsel]: the queries to search (else 0..nq-1)
+    uint64_t nsel;
 };
 size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats);
 hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a);
+
+// ---- batched form (me8q_batch.hip) ----------------------------------------
+// Every query of a call as one (query, list) work list.  A threshold pass
+// (the plain scan over each query's first `budget` batches in probe order)
+// gives tau_q, the k-th distance it found; a persistent kernel then computes
+// the integer coarse sums of every remaining (query, vector) pair as an int8
+// MFMA contraction (one-hot nibble codes against the u8 LUT shifted to
+// int8: exact), applies the reference's coarse bound against tau_q and
+// appends the passing vectors to the query's candidate list; the candidates'
+// exact distances (the reference's arithmetic, with its cascade bound
+// against tau_q) and a top-k selection merged with the threshold pass's list
+// follow.
+constexpr uint32_t kM8BatchRuns = 4;     // coded runs the batched form supports
+constexpr uint32_t kM8BatchQTile = 32;   // pairs per work item (the MFMA's M)
+constexpr uint32_t kM8BatchBTile = 8;    // batches per work item (4 waves x 2 tiles)
+constexpr uint32_t kM8RefineChunk = 64;  // candidates per refine task
+constexpr uint32_t kM8SelMaxK = 512;     // top-k of the selection kernel
+
+struct M8BatchArgs {
+    M8View v;
+    const uint8_t* lut;       // [nq][plane_cols*32]
+    const float* qc;          // [nq][4]
+    const double* rc;         // [nq][nrun][4]
+    const float* rotq;        // [nq][padded]
+    const uint32_t* probe;    // [nq][maxp]
+    const float* pdist;       // [nq][maxp]
+    const double* sqd2;       // [nq][maxp][nrun]
+    uint32_t maxp, k;
+    uint64_t nq;
+    float z;
+    int32_t cascade;          // 1: the reference's cascade bound against tau_q in the refine
+    const uint32_t* first_ids;   // [nq][k] threshold pass (ascending, UINT32_MAX / +inf padded)
+    const float* first_d;
+    const uint64_t* cover;       // [nq] (p << 32) | b: first (probe, batch) the threshold pass left
+    const BatchItem* items;
+    const uint64_t* pairs;
+    uint64_t items_cap;
+    uint32_t* counters;       // [0] items, [1] next item, [2] overflowed queries, [3] next refine task
+    uint64_t* cand;           // [nq][cap] (probe position << 32) | vector position in its list
+    uint32_t* ncand;          // [nq]
+    uint32_t cap;
+    uint32_t* toff;           // [nq+1] refine task offsets
+    uint64_t* surv;           // [nq][scap] (monotone distance key << 32) | id
+    uint32_t* nsurv;          // [nq]
+    uint32_t scap;
+    uint32_t* overflow;       // [nq]
+    uint32_t* out_ids;        // [nq][k]
+    float* out_d;
+    unsigned long long* stats;   // [8] pairs scored, candidates, refined, survivors, bits scored, items, -, -
+    uint32_t* chk;            // optional [nq]: sum over scored pairs of (pos+1) * sum_r (r+1) * coarse_sum_r, mod 2^32
+    uint32_t lev_floats;
+};
+hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid);
+hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a);
+size_t M8RefineLds(const M8View& v, uint32_t lev_floats);
+hipError_t LaunchM8Refine(hipStream_t st, const M8BatchArgs& a, uint32_t grid);
+hipError_t LaunchM8Select(hipStream_t st, const M8BatchArgs& a);
 
 }  // namespace e8qdev

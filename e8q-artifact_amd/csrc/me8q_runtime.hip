@@ -81,7 +81,66 @@ struct M8Bufs {
     uint8_t* lut;
     uint32_t *probe, *npq, *oid;
     uint64_t* qst;
+    // batched form
+    uint32_t *fid, *ccount, *cfill, *coff, *ioff, *counters, *ncand, *toff, *nsurv, *ovf, *chk;
+    float* fd;
+    uint64_t *cover, *pairs, *cand, *surv;
+    BatchItem* items;
+    unsigned long long *wstats, *bstats;
+    BatchedParams* wparams;
 };
+
+constexpr uint32_t kM8CandCap = 32768;   // candidates per query of the batched form (more: plain form)
+constexpr uint32_t kM8SurvCap = 4096;    // exact distances below tau per query (more: plain form)
+constexpr uint32_t kM8DefaultBudget = 16;
+
+// Whether the batched form runs for this index and k.
+bool M8BatchedFits(const e8q_m8index* x, uint64_t k) {
+    const M8View& v = x->view;
+    if (v.nrun < 1 || v.nrun > kM8BatchRuns || k > kM8SelMaxK || (v.plane_cols & 7u) != 0) return false;
+    uint32_t cols = 0;
+    for (const auto& r : x->runs) cols += r.nblk;
+    return cols == v.plane_cols;
+}
+
+// Work items of the batched form: an upper bound of their count (the list
+// count of each pair tile times the batch tiles of its list).
+uint64_t M8ItemsCap(const e8q_m8index* x, uint64_t nq, uint64_t pairs) {
+    uint64_t sum_nbt = 0, max_nbt = 0;
+    for (uint64_t c = 0; c < x->view.nlist; ++c) {
+        const uint64_t nb = (x->list_begin[c + 1] - x->list_begin[c] + 31) / 32;
+        const uint64_t nbt = (nb + kM8BatchBTile - 1) / kM8BatchBTile;
+        sum_nbt += nbt;
+        max_nbt = std::max(max_nbt, nbt);
+    }
+    const uint64_t qt = kM8BatchQTile;
+    return std::min((nq + qt - 1) / qt * sum_nbt, (pairs + qt - 1) / qt * max_nbt + sum_nbt);
+}
+
+void M8CarveBatched(e8q_ctx::Carve& cv, M8Bufs& b, const e8q_m8index* x, uint64_t nq, uint64_t k, uint64_t pairs,
+                    bool chk) {
+    const M8View& v = x->view;
+    cv.add(&b.fid, nq * k);
+    cv.add(&b.fd, nq * k);
+    cv.add(&b.cover, nq);
+    cv.add(&b.ccount, v.nlist);
+    cv.add(&b.cfill, v.nlist);
+    cv.add(&b.coff, v.nlist + 1);
+    cv.add(&b.ioff, v.nlist + 1);
+    cv.add(&b.pairs, std::max<uint64_t>(pairs, 1));
+    cv.add(&b.items, std::max<uint64_t>(M8ItemsCap(x, nq, pairs), 1));
+    cv.add(&b.counters, 4);
+    cv.add(&b.ncand, nq);
+    cv.add(&b.cand, nq * kM8CandCap);
+    cv.add(&b.toff, nq + 1);
+    cv.add(&b.nsurv, nq);
+    cv.add(&b.surv, nq * kM8SurvCap);
+    cv.add(&b.ovf, nq);
+    cv.add(&b.wstats, 4);
+    cv.add(&b.bstats, 8);
+    cv.add(&b.wparams, 1);
+    if (chk) cv.add(&b.chk, nq);
+}
 
 void M8Carve(e8q_ctx::Carve& cv, M8Bufs& b, const M8View& v, uint64_t nq, uint64_t k, uint64_t maxp, bool q_host,
              bool out_host, bool per_query_nprobe) {
@@ -103,15 +162,261 @@ void M8Carve(e8q_ctx::Carve& cv, M8Bufs& b, const M8View& v, uint64_t nq, uint64
     cv.add(&b.qst, nq * 8);
 }
 
+// One search: plain (the reference's per-query schedule) or batched.  chk
+// (host [nq], batched only): the coarse-sum checksum of the diagnostic.
+void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e8q_m8_stats* stats,
+              uint64_t* per_query_stats, uint32_t* chk) {
+    if (!ctx || !x || !a) Throw(E8Q_EINVAL, "null argument");
+    if (a->k == 0 || a->k > kM8MaxK) Throw(E8Q_EINVAL, "k must be in [1, 1024]");
+    if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
+    if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "mode must be plain or batched");
+    const M8View& v = x->view;
+    const uint64_t nq = a->nq;
+    if (stats) *stats = e8q_m8_stats{0, 0, 0, 0, 0, 0.0, 0.0, 0, 0, 0, E8Q_MODE_PLAIN, 0, 0};
+    if (nq == 0) return;
+    uint32_t maxp = 0;
+    uint64_t pairs = 0;
+    if (a->nprobe_per_query) {
+        for (uint64_t i = 0; i < nq; ++i) {
+            if (a->nprobe_per_query[i] == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+            maxp = std::max<uint32_t>(maxp, a->nprobe_per_query[i]);
+            pairs += std::min<uint64_t>(a->nprobe_per_query[i], v.nlist);
+        }
+    } else {
+        if (a->nprobe == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+        maxp = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
+        pairs = nq * maxp;
+    }
+    maxp = static_cast<uint32_t>(std::min<uint64_t>(maxp, v.nlist));
+    const bool batched = a->mode == E8Q_MODE_BATCHED && M8BatchedFits(x, a->k);
+    ctx->Bind();
+    hipStream_t st = ctx->stream;
+    const uint64_t P = v.padded;
+    M8Bufs b{};
+    e8q_ctx::Carve cv;
+    M8Carve(cv, b, v, nq, a->k, maxp, !a->queries_on_device, !a->outputs_on_device, a->nprobe_per_query != nullptr);
+    if (batched) M8CarveBatched(cv, b, x, nq, a->k, pairs, chk != nullptr);
+    ctx->Take(cv);
+    float *dq = b.dq, *rotq = b.rotq, *qc = b.qc, *dist = b.dist, *pd = b.pd, *od = b.od;
+    double *rc = b.rc, *sqd2 = b.sqd2;
+    uint8_t* lut = b.lut;
+    uint32_t *probe = b.probe, *npq = b.npq, *oid = b.oid;
+    uint64_t* qst = b.qst;
+    const float* q = a->queries;
+    if (!a->queries_on_device) {
+        Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+        q = dq;
+    }
+    if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
+    const uint32_t nprobe_all = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
+    Check(hipEventRecord(ctx->ev[0], st), "event");
+    Check(LaunchM8QueryPrep(st, v, q, nq, rotq, lut, qc, rc), "query prep");
+    RouteArgs ra{};
+    ra.rotq = rotq;
+    ra.qstride = static_cast<uint32_t>(P);
+    ra.qn2 = qc + 2;
+    ra.qn2_stride = 4;
+    ra.cent = v.cent_rot;
+    ra.cn2 = v.cn2;
+    ra.nlist = v.nlist;
+    ra.dim = v.route_dim;
+    ra.cstride = static_cast<uint32_t>(P);
+    ra.nq = nq;
+    ra.dist = dist;
+    Check(LaunchRouteDist(st, ra), "route");
+    Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq, nprobe_all, maxp, probe, pd),
+          "select probes");
+    Check(LaunchM8RunDist(st, v, rotq, rc, probe, npq, nprobe_all, maxp, nq, sqd2), "run distances");
+    uint32_t* out_ids = a->outputs_on_device ? a->out_ids : oid;
+    float* out_d = a->outputs_on_device ? a->out_dists : od;
+    M8ScanArgs sa{};
+    sa.v = v;
+    sa.rotq = rotq;
+    sa.lut = lut;
+    sa.qc = qc;
+    sa.rc = rc;
+    sa.probe = probe;
+    sa.pdist = pd;
+    sa.sqd2 = sqd2;
+    sa.nprobe_q = npq;
+    sa.nprobe_all = nprobe_all;
+    sa.maxp = maxp;
+    sa.nq = nq;
+    sa.k = static_cast<uint32_t>(a->k);
+    sa.z = a->z;
+    sa.out_ids = out_ids;
+    sa.out_d = out_d;
+    sa.qstats = qst;
+    sa.lev_floats = 0;
+    for (const auto& r : x->runs)
+        if (r.rate <= kM8LdsLevRate) sa.lev_floats += 2u << r.rate;
+    std::vector<unsigned long long> bst(8, 0), wst(4, 0);
+    uint64_t novf = 0;
+    if (!batched) {
+        Check(hipEventRecord(ctx->ev[1], st), "event");
+        Check(LaunchM8Scan(st, sa), "scan");
+        Check(hipEventRecord(ctx->ev[2], st), "event");
+        Check(hipEventRecord(ctx->ev[3], st), "event");
+    } else {
+        // 1. threshold pass: the plain scan over each query's first `budget` batches
+        M8ScanArgs fa = sa;
+        fa.budget = a->batched_budget ? a->batched_budget : kM8DefaultBudget;
+        fa.cover = b.cover;
+        fa.out_ids = b.fid;
+        fa.out_d = b.fd;
+        Check(LaunchM8Scan(st, fa), "threshold pass");
+        // 2. the (query, list) work list: every probe, grouped by list
+        Check(LaunchFill32(st, b.ccount, v.nlist, 0), "clear");
+        Check(LaunchFill32(st, b.ncand, nq, 0), "clear");
+        Check(LaunchFill32(st, b.nsurv, nq, 0), "clear");
+        Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.wstats), 8, 0), "clear");
+        Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.bstats), 16, 0), "clear");
+        if (chk) Check(LaunchFill32(st, b.chk, nq, 0), "clear");
+        IndexView lv{};   // the work-list kernels read the list count and offsets only
+        lv.nlist = v.nlist;
+        lv.cluster_begin = v.list_begin;
+        BatchedArgs w{};
+        w.nq = nq;
+        w.maxp = maxp;
+        w.nprobe_q = npq;
+        w.nprobe_all = nprobe_all;
+        w.probe_ids = probe;
+        w.k = static_cast<uint32_t>(a->k);
+        w.ccount = b.ccount;
+        w.cfill = b.cfill;
+        w.coff = b.coff;
+        w.ioff = b.ioff;
+        w.pairs = b.pairs;
+        w.items = b.items;
+        w.items_cap = M8ItemsCap(x, nq, pairs);
+        w.counters = b.counters;
+        w.qtile = kM8BatchQTile;
+        w.btile = kM8BatchBTile;
+        w.stats = b.wstats;
+        w.p_lo = 0;
+        Check(LaunchBatchedWorkList(st, lv, w, b.wparams, true), "work list");
+        // 3. the MFMA contraction and the coarse bound
+        M8BatchArgs ba{};
+        ba.v = v;
+        ba.lut = lut;
+        ba.qc = qc;
+        ba.rc = rc;
+        ba.rotq = rotq;
+        ba.probe = probe;
+        ba.pdist = pd;
+        ba.sqd2 = sqd2;
+        ba.maxp = maxp;
+        ba.k = static_cast<uint32_t>(a->k);
+        ba.nq = nq;
+        ba.z = a->z;
+        ba.cascade = 1;
+        ba.first_ids = b.fid;
+        ba.first_d = b.fd;
+        ba.cover = b.cover;
+        ba.items = b.items;
+        ba.pairs = b.pairs;
+        ba.items_cap = w.items_cap;
+        ba.counters = b.counters;
+        ba.cand = b.cand;
+        ba.ncand = b.ncand;
+        ba.cap = kM8CandCap;
+        ba.toff = b.toff;
+        ba.surv = b.surv;
+        ba.nsurv = b.nsurv;
+        ba.scap = kM8SurvCap;
+        ba.overflow = b.ovf;
+        ba.out_ids = out_ids;
+        ba.out_d = out_d;
+        ba.stats = b.bstats;
+        ba.chk = chk ? b.chk : nullptr;
+        ba.lev_floats = sa.lev_floats;
+        const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
+        Check(hipEventRecord(ctx->ev[1], st), "event");
+        Check(LaunchM8MfmaScan(st, ba, 2 * cus), "mfma scan");
+        Check(hipEventRecord(ctx->ev[2], st), "event");
+        // 4. exact cascade of the candidates, 5. top-k
+        Check(LaunchM8RefineTasks(st, ba), "refine tasks");
+        Check(LaunchM8Refine(st, ba, 4 * cus), "refine");
+        Check(LaunchM8Select(st, ba), "select");
+        uint32_t cnt[4] = {0, 0, 0, 0};
+        Check(hipMemcpyAsync(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost, st), "download counters");
+        Check(hipMemcpyAsync(bst.data(), b.bstats, 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
+        Check(hipMemcpyAsync(wst.data(), b.wstats, 4 * 8, hipMemcpyDeviceToHost, st), "download stats");
+        Check(hipStreamSynchronize(st), "batched search");
+        if (cnt[0] > w.items_cap) Throw(E8Q_ERUNTIME, "batched search: work items exceed their bound");
+        novf = cnt[2];
+        if (novf > nq) Throw(E8Q_ERUNTIME, "batched search: overflow count exceeds the query count");
+        if (novf > 0) {   // those queries: the plain form
+            M8ScanArgs oa = sa;
+            oa.qsel = b.ovf;
+            oa.nsel = novf;
+            Check(LaunchM8Scan(st, oa), "plain search of overflowed queries");
+        }
+        Check(hipEventRecord(ctx->ev[3], st), "event");
+        if (chk) Check(hipMemcpyAsync(chk, b.chk, nq * 4, hipMemcpyDeviceToHost, st), "download checksums");
+    }
+    if (!a->outputs_on_device) {
+        Check(hipMemcpyAsync(a->out_ids, oid, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
+        Check(hipMemcpyAsync(a->out_dists, od, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");
+    }
+    std::vector<uint64_t> hs(nq * 8);
+    Check(hipMemcpyAsync(hs.data(), qst, nq * 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
+    Check(hipStreamSynchronize(st), "search");
+    if (per_query_stats) {
+        if (batched) std::fill(per_query_stats, per_query_stats + nq * 5, 0ull);
+        else
+            for (uint64_t i = 0; i < nq; ++i) std::memcpy(per_query_stats + i * 5, hs.data() + i * 8, 5 * 8);
+    }
+    if (stats) {
+        uint64_t s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // threshold pass (batched) or the whole search (plain)
+        for (uint64_t i = 0; i < nq; ++i)
+            for (int f = 0; f < 8; ++f) s8[f] += hs[i * 8 + f];
+        if (!batched) {
+            stats->candidates = s8[0];
+            stats->survivors = s8[1];
+            stats->bits_scored = s8[2];
+            stats->bits_total = s8[3];
+            stats->pruned = s8[4];
+            stats->batches = s8[5];
+            stats->refined = s8[6];
+            stats->groups = s8[7];
+        } else {
+            // candidates: the vectors of every probed list; the threshold
+            // pass's refined / survivors / bits plus the batched ones
+            uint64_t bits_row = 0;
+            for (const auto& r : x->runs) bits_row += static_cast<uint64_t>(r.rate) * r.len;
+            stats->form = E8Q_MODE_BATCHED;
+            stats->candidates = wst[0];
+            stats->batches = wst[2];
+            stats->refined = s8[6] + bst[2];
+            stats->survivors = s8[1] + bst[3];
+            stats->bits_scored = s8[2] + bst[4];
+            stats->bits_total = wst[0] * bits_row;
+            stats->pruned = stats->candidates > stats->survivors ? stats->candidates - stats->survivors : 0;
+            stats->groups = s8[7];
+            stats->overflowed = novf;
+            stats->pairs_scored = bst[0];
+        }
+        float ms = 0.0F;
+        Check(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "event time");
+        stats->scan_ms = ms;
+        Check(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]), "event time");
+        stats->total_ms = ms;
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
-uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* x, uint64_t nq, uint64_t k, uint64_t max_nprobe) {
+uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* x, uint64_t nq, uint64_t k, uint64_t max_nprobe,
+                                        int32_t mode) {
     if (!x) return 0;
     M8Bufs b{};
     e8q_ctx::Carve cv;
-    M8Carve(cv, b, x->view, nq, k, std::min<uint64_t>(max_nprobe, x->view.nlist), true, true, true);
+    const uint64_t maxp = std::min<uint64_t>(max_nprobe, x->view.nlist);
+    M8Carve(cv, b, x->view, nq, k, maxp, true, true, true);
+    if (mode == E8Q_MODE_BATCHED && M8BatchedFits(x, k)) M8CarveBatched(cv, b, x, nq, k, nq * maxp, false);
     return cv.bytes();
 }
 
@@ -357,114 +662,16 @@ e8q_status e8q_m8_encode(e8q_ctx* ctx, e8q_m8index* x, const float* base, const 
 
 e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e8q_m8_stats* stats,
                          uint64_t* per_query_stats) {
+    return Guard([&] { M8Search(ctx, x, a, stats, per_query_stats, nullptr); });
+}
+
+e8q_status e8q_m8_debug_batch_checksum(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a,
+                                       uint32_t* chk) {
     return Guard([&] {
-        if (!ctx || !x || !a) Throw(E8Q_EINVAL, "null argument");
-        if (a->k == 0 || a->k > kM8MaxK) Throw(E8Q_EINVAL, "k must be in [1, 1024]");
-        if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
-        const M8View& v = x->view;
-        const uint64_t nq = a->nq;
-        if (stats) *stats = e8q_m8_stats{0, 0, 0, 0, 0, 0.0, 0.0, 0, 0, 0};
-        if (nq == 0) return;
-        uint32_t maxp = 0;
-        if (a->nprobe_per_query) {
-            for (uint64_t i = 0; i < nq; ++i) {
-                if (a->nprobe_per_query[i] == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
-                maxp = std::max<uint32_t>(maxp, a->nprobe_per_query[i]);
-            }
-        } else {
-            if (a->nprobe == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
-            maxp = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
-        }
-        maxp = static_cast<uint32_t>(std::min<uint64_t>(maxp, v.nlist));
-        ctx->Bind();
-        hipStream_t st = ctx->stream;
-        const uint64_t P = v.padded, R = v.nrun;
-        M8Bufs b{};
-        e8q_ctx::Carve cv;
-        M8Carve(cv, b, v, nq, a->k, maxp, !a->queries_on_device, !a->outputs_on_device, a->nprobe_per_query != nullptr);
-        ctx->Take(cv);
-        float *dq = b.dq, *rotq = b.rotq, *qc = b.qc, *dist = b.dist, *pd = b.pd, *od = b.od;
-        double *rc = b.rc, *sqd2 = b.sqd2;
-        uint8_t* lut = b.lut;
-        uint32_t *probe = b.probe, *npq = b.npq, *oid = b.oid;
-        uint64_t* qst = b.qst;
-        const float* q = a->queries;
-        if (!a->queries_on_device) {
-            Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
-            q = dq;
-        }
-        if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
-        Check(hipEventRecord(ctx->ev[0], st), "event");
-        Check(LaunchM8QueryPrep(st, v, q, nq, rotq, lut, qc, rc), "query prep");
-        RouteArgs ra{};
-        ra.rotq = rotq;
-        ra.qstride = static_cast<uint32_t>(P);
-        ra.qn2 = qc + 2;
-        ra.qn2_stride = 4;
-        ra.cent = v.cent_rot;
-        ra.cn2 = v.cn2;
-        ra.nlist = v.nlist;
-        ra.dim = v.route_dim;
-        ra.cstride = static_cast<uint32_t>(P);
-        ra.nq = nq;
-        ra.dist = dist;
-        Check(LaunchRouteDist(st, ra), "route");
-        Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq,
-                                 static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist)), maxp, probe, pd),
-              "select probes");
-        Check(LaunchM8RunDist(st, v, rotq, rc, probe, npq, static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist)),
-                              maxp, nq, sqd2),
-              "run distances");
-        M8ScanArgs sa{};
-        sa.v = v;
-        sa.rotq = rotq;
-        sa.lut = lut;
-        sa.qc = qc;
-        sa.rc = rc;
-        sa.probe = probe;
-        sa.pdist = pd;
-        sa.sqd2 = sqd2;
-        sa.nprobe_q = npq;
-        sa.nprobe_all = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
-        sa.maxp = maxp;
-        sa.nq = nq;
-        sa.k = static_cast<uint32_t>(a->k);
-        sa.z = a->z;
-        sa.out_ids = a->outputs_on_device ? a->out_ids : oid;
-        sa.out_d = a->outputs_on_device ? a->out_dists : od;
-        sa.qstats = qst;
-        sa.lev_floats = 0;
-        for (const auto& r : x->runs)
-            if (r.rate <= kM8LdsLevRate) sa.lev_floats += 2u << r.rate;
-        Check(hipEventRecord(ctx->ev[1], st), "event");
-        Check(LaunchM8Scan(st, sa), "scan");
-        Check(hipEventRecord(ctx->ev[2], st), "event");
-        if (!a->outputs_on_device) {
-            Check(hipMemcpyAsync(a->out_ids, oid, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
-            Check(hipMemcpyAsync(a->out_dists, od, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");
-        }
-        std::vector<uint64_t> hs(nq * 8);
-        Check(hipMemcpyAsync(hs.data(), qst, nq * 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
-        Check(hipStreamSynchronize(st), "search");
-        if (per_query_stats)
-            for (uint64_t i = 0; i < nq; ++i) std::memcpy(per_query_stats + i * 5, hs.data() + i * 8, 5 * 8);
-        if (stats) {
-            for (uint64_t i = 0; i < nq; ++i) {
-                stats->candidates += hs[i * 8];
-                stats->survivors += hs[i * 8 + 1];
-                stats->bits_scored += hs[i * 8 + 2];
-                stats->bits_total += hs[i * 8 + 3];
-                stats->pruned += hs[i * 8 + 4];
-                stats->batches += hs[i * 8 + 5];
-                stats->refined += hs[i * 8 + 6];
-                stats->groups += hs[i * 8 + 7];
-            }
-            float ms = 0.0F;
-            Check(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "event time");
-            stats->scan_ms = ms;
-            Check(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]), "event time");
-            stats->total_ms = ms;
-        }
+        if (!chk) Throw(E8Q_EINVAL, "null argument");
+        if (!a || a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "the checksum needs the batched form");
+        if (!x || !M8BatchedFits(x, a->k)) Throw(E8Q_EINVAL, "the batched form does not run for this index and k");
+        M8Search(ctx, x, a, nullptr, nullptr, chk);
     });
 }
 

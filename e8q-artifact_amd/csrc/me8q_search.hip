@@ -204,83 +204,6 @@ struct M8Stage {         // one staged candidate
     uint64_t pad2[2];
 };
 
-// One 8-lane group's half of a run score: component `ci` of the blocks of
-// parity `half`, as one FMA chain in block order (the reference keeps even
-// blocks in acc and odd blocks in acc2, me8q_score_avx2.cpp:404-413).  The
-// digit of (block, ci) = its refine field bits (T = rate-1 per digit, digit
-// 7's LSB the parity of digits 0..6's, the branch in the field's top bit)
-// under the plane's top bit; at rate 1 the plane byte is the whole code.
-template <int T>
-__device__ __forceinline__ float M8HalfChainT(const uint8_t* __restrict__ pb, const uint8_t* __restrict__ fr,
-                                              const float* __restrict__ lev, const float* __restrict__ qb,
-                                              uint32_t nblk, uint32_t half, uint32_t ci) {
-    constexpr uint32_t rate = T + 1;
-    uint64_t pml = 0;
-    uint32_t pmh = 0;
-#pragma unroll
-    for (uint32_t jj = 0; jj < 7; ++jj) {
-        if (jj * T < 64) pml |= uint64_t{1} << (jj * T);
-        else pmh |= 1u << (jj * T - 64);
-    }
-    float acc = 0.0F;
-#pragma unroll 2
-    for (uint32_t blk = half; blk < nblk; blk += 2) {
-        const uint32_t pbyte = pb[blk];
-        const uint32_t top = (pbyte >> (7 - ci)) & 1u;
-        uint32_t dgt, br;
-        if constexpr (T == 0) {
-            dgt = ci < 7 ? top : (__popc(pbyte >> 1) & 1u);
-            br = pbyte & 1u;
-        } else {
-            const uint8_t* fb = fr + blk * T;
-            uint64_t flo = 0;
-            uint32_t fhi = 0;
-#pragma unroll
-            for (uint32_t b = 0; b < static_cast<uint32_t>(T); ++b) {
-                const uint32_t byte = fb[b];
-                if (b < 8) flo |= static_cast<uint64_t>(byte) << (8 * b);
-                else fhi |= byte << (8 * (b - 8));
-            }
-            auto field = [&](uint32_t pos, uint32_t nb) -> uint32_t {
-                const uint64_t m = (uint64_t{1} << nb) - 1;
-                if (pos + nb <= 64) return static_cast<uint32_t>((flo >> pos) & m);
-                if (pos >= 64) return static_cast<uint32_t>((fhi >> (pos - 64)) & m);
-                return static_cast<uint32_t>(((flo >> pos) | (static_cast<uint64_t>(fhi) << (64 - pos))) & m);
-            };
-            uint32_t lo;
-            if (ci < 7) {
-                lo = field(T * ci, T);
-            } else {
-                const uint32_t par = (__popcll(flo & pml) + __popc(fhi & pmh)) & 1u;
-                lo = (field(7 * T, T - 1) << 1) | par;
-            }
-            br = field(8 * T - 1, 1);
-            dgt = lo | (top << T);
-        }
-        acc = fmaf(lev[(br << rate) + dgt], qb[blk * 8], acc);
-    }
-    return acc;
-}
-
-__device__ float M8HalfChain(uint32_t rate, const uint8_t* pb, const uint8_t* fr, const float* lev, const float* qb,
-                             uint32_t nblk, uint32_t half, uint32_t ci) {
-    switch (rate) {
-        case 1: return M8HalfChainT<0>(pb, fr, lev, qb, nblk, half, ci);
-        case 2: return M8HalfChainT<1>(pb, fr, lev, qb, nblk, half, ci);
-        case 3: return M8HalfChainT<2>(pb, fr, lev, qb, nblk, half, ci);
-        case 4: return M8HalfChainT<3>(pb, fr, lev, qb, nblk, half, ci);
-        case 5: return M8HalfChainT<4>(pb, fr, lev, qb, nblk, half, ci);
-        case 6: return M8HalfChainT<5>(pb, fr, lev, qb, nblk, half, ci);
-        case 7: return M8HalfChainT<6>(pb, fr, lev, qb, nblk, half, ci);
-        case 8: return M8HalfChainT<7>(pb, fr, lev, qb, nblk, half, ci);
-        case 9: return M8HalfChainT<8>(pb, fr, lev, qb, nblk, half, ci);
-        case 10: return M8HalfChainT<9>(pb, fr, lev, qb, nblk, half, ci);
-        case 11: return M8HalfChainT<10>(pb, fr, lev, qb, nblk, half, ci);
-        case 12: return M8HalfChainT<11>(pb, fr, lev, qb, nblk, half, ci);
-        default: return M8HalfChainT<12>(pb, fr, lev, qb, nblk, half, ci);
-    }
-}
-
 // Inserts (d, id) into the sorted list (td, ti) of `cnt` entries (<= K):
 // a full list rejects d unless it is below the worst; a new item goes before
 // equal ones (lower_bound by distance).  Called by the whole wave with
@@ -369,7 +292,9 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
     const float z = a.z;
     const double dz = z;
 
-    for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
+    const uint64_t nrun_q = a.qsel ? a.nsel : a.nq;
+    for (uint64_t qj = blockIdx.x; qj < nrun_q; qj += gridDim.x) {
+        const uint64_t qi = a.qsel ? a.qsel[qj] : qj;
         for (uint32_t d = lane; d < P; d += 64) q[d] = a.rotq[qi * P + d];
         for (uint32_t e = lane; e < PC * 8; e += 64)
             reinterpret_cast<uint32_t*>(lut)[e] = reinterpret_cast<const uint32_t*>(a.lut + qi * PC * 32)[e];
@@ -388,6 +313,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
         uint64_t n_cand = 0, n_surv = 0, n_bits = 0, n_btot = 0, n_batch = 0, n_staged = 0, n_flush = 0;
         __syncthreads();
         const uint64_t np = min<uint64_t>(a.nprobe_q ? a.nprobe_q[qi] : a.nprobe_all, v.nlist);
+        uint64_t used = 0, p_stop = np, b_stop = 0;   // the threshold pass's budget and where it stopped
         for (uint64_t p = 0; p < np; ++p) {
             const uint32_t c = a.probe[qi * a.maxp + p];
             const float d2f = a.pdist[qi * a.maxp + p];
@@ -402,7 +328,8 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
             }
             if (cntc == 0) continue;
             for (uint32_t r = lane; r < R; r += 64) ssq[r] = a.sqd2[(qi * a.maxp + p) * R + r];
-            const uint64_t nb = (cntc + 31) >> 5;
+            const uint64_t nbl = (cntc + 31) >> 5;
+            const uint64_t nb = a.budget ? min<uint64_t>(nbl, a.budget - used) : nbl;
             const uint64_t bat0 = v.batch_begin[c];
             n_batch += nb;
             for (uint64_t b0 = 0; b0 < nb; b0 += 2) {
@@ -686,7 +613,16 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                     __syncthreads();
                 }
             }
+            if (a.budget) {
+                used += nb;
+                if (used >= a.budget) {
+                    p_stop = nb == nbl ? p + 1 : p;
+                    b_stop = nb == nbl ? 0 : nb;
+                    break;
+                }
+            }
         }
+        if (lane == 0 && a.cover) a.cover[qi] = (p_stop << 32) | b_stop;
         for (uint32_t i = lane; i < K; i += 64) {
             a.out_ids[qi * K + i] = i < cnt ? ti[i] : 0xFFFFFFFFu;
             a.out_d[qi * K + i] = i < cnt ? td[i] : finf;
@@ -739,14 +675,15 @@ size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats) {
 }
 
 hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a) {
-    if (a.nq == 0) return hipSuccess;
+    const uint64_t nrun = a.qsel ? a.nsel : a.nq;
+    if (nrun == 0) return hipSuccess;
     const size_t lds = M8ScanLds(a.v, a.k, a.lev_floats);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&M8ScanKernel),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
     }
-    const uint32_t grid = static_cast<uint32_t>(a.nq < 1048576 ? a.nq : 1048576);
+    const uint32_t grid = static_cast<uint32_t>(nrun < 1048576 ? nrun : 1048576);
     hipLaunchKernelGGL(M8ScanKernel, dim3(grid), dim3(64), lds, st, a);
     return hipGetLastError();
 }

@@ -59,6 +59,16 @@ typedef struct {
     uint32_t* out_ids;                /* nq*k, ascending distance, UINT32_MAX padded */
     float* out_dists;                 /* nq*k, +inf padded */
     int32_t outputs_on_device;
+    int32_t mode;                     /* E8Q_MODE_PLAIN (0): the reference's per-query schedule, bit for
+                                         bit.  E8Q_MODE_BATCHED (1): the whole query set as one (query, list)
+                                         work list -- a threshold pass over each query's first lists, the
+                                         integer coarse sums of every other (query, vector) pair as an exact
+                                         int8 MFMA contraction tested with the reference's coarse bound
+                                         against that fixed threshold, the exact cascade of the passing
+                                         vectors, a top-k merge.  Needs 1..4 coded runs and k <= 512; other
+                                         indexes and k run the plain form (stats.form says which ran).
+                                         No reference counterpart: the reference searches query by query. */
+    uint32_t batched_budget;          /* threshold pass length in 32-vector batches (0: 16) */
 } e8q_m8_search_args;
 
 typedef struct {
@@ -67,7 +77,11 @@ typedef struct {
     double total_ms;   /* device time of the whole search */
     uint64_t batches;  /* 32-vector scan records read (plane + banks) */
     uint64_t refined;  /* candidates that entered the exact cascade */
-    uint64_t groups;   /* cascade groups (up to four candidates each) */
+    uint64_t groups;   /* cascade groups (up to four candidates each; plain form and threshold pass) */
+    int32_t form;      /* the form that ran: E8Q_MODE_PLAIN or E8Q_MODE_BATCHED */
+    uint64_t overflowed;   /* batched form: queries whose candidate lists overflowed, searched again
+                              by the plain form */
+    uint64_t pairs_scored; /* batched form: (query, vector) pairs of the MFMA contraction */
 } e8q_m8_stats;
 
 /* Allocates the device index and its static tables (the allocation half of
@@ -100,10 +114,18 @@ e8q_status e8q_m8_encode(e8q_ctx* ctx, e8q_m8index* idx, const float* base, cons
 e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* idx, const e8q_m8_search_args* args,
                          e8q_m8_stats* stats, uint64_t* per_query_stats);
 /* Device scratch of one search of nq host queries with per-query nprobe <=
- * max_nprobe and host outputs, carved from the context workspace. */
-uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* idx, uint64_t nq, uint64_t k, uint64_t max_nprobe);
+ * max_nprobe and host outputs in `mode`, carved from the context workspace. */
+uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* idx, uint64_t nq, uint64_t k, uint64_t max_nprobe,
+                                       int32_t mode);
 
 /* ---- diagnostics used by tests (reference layouts, host pointers) -------- */
+/* The batched form with a checksum of its integer coarse sums: chk[q] =
+ * sum over the (q, vector) pairs of q's probed lists of (pos + 1) *
+ * sum_r (r + 1) * S_r, mod 2^32, where pos is the vector's position in its
+ * list and S_r its coarse sum of run r (AccumulateCoarseHi over the run's
+ * plane columns, me8q_fast.cpp:800-818).  chk: host [nq]. */
+e8q_status e8q_m8_debug_batch_checksum(e8q_ctx* ctx, const e8q_m8index* idx, const e8q_m8_search_args* args,
+                                       uint32_t* chk);
 /* One list's FastLayout as the reference holds it (me8q.hpp:153-184): plane
  * [batches][plane_cols*32 + nrun*160] ([col][lane] bytes), refine
  * [cnt][refine_stride], gain / vpc [nrun][cnt], rc_n2 [nrun].  sizes3 =

@@ -25,13 +25,16 @@ class M8Arrays(ctypes.Structure):
 class M8SearchArgs(ctypes.Structure):
     _fields_ = [("queries", vp), ("nq", u64), ("k", u64), ("nprobe", u64), ("nprobe_per_query", vp),
                 ("z", ctypes.c_float), ("queries_on_device", ctypes.c_int32), ("out_ids", vp), ("out_dists", vp),
-                ("outputs_on_device", ctypes.c_int32)]
+                ("outputs_on_device", ctypes.c_int32), ("mode", ctypes.c_int32), ("batched_budget", ctypes.c_uint32)]
 
 
 class M8Stats(ctypes.Structure):
     _fields_ = [("candidates", u64), ("survivors", u64), ("bits_scored", u64), ("bits_total", u64),
                 ("pruned", u64), ("scan_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("batches", u64),
-                ("refined", u64), ("groups", u64)]
+                ("refined", u64), ("groups", u64), ("form", ctypes.c_int32), ("overflowed", u64),
+                ("pairs_scored", u64)]
+
+PLAIN, BATCHED = 0, 1
 
 
 def tables():
@@ -121,20 +124,37 @@ class M8Index:
         check(lib().e8q_m8_debug_query(self.ctx.h, self.h, p(q), nq, nprobe, p(rq), p(lut), p(md), p(probe), p(pd)))
         return rq, lut, md, probe, pd
 
-    def search(self, queries, k, nprobe, z, nprobe_per_query=None):
+    def _args(self, q, k, nprobe, z, npq, mode, budget, ids, d):
+        s = M8SearchArgs()
+        s.queries, s.nq, s.k, s.nprobe, s.nprobe_per_query = p(q), q.shape[0], k, nprobe, p(npq)
+        s.z, s.queries_on_device, s.outputs_on_device = z, 0, 0
+        s.out_ids, s.out_dists = p(ids), p(d)
+        s.mode, s.batched_budget = mode, budget
+        return s
+
+    def search(self, queries, k, nprobe, z, nprobe_per_query=None, mode=PLAIN, budget=0):
         q = np.ascontiguousarray(queries, np.float32)
         nq = q.shape[0]
         ids = np.empty((nq, k), np.uint32)
         d = np.empty((nq, k), np.float32)
         st = np.empty((nq, 5), np.uint64)
         npq = None if nprobe_per_query is None else np.ascontiguousarray(nprobe_per_query, np.uint32)
-        s = M8SearchArgs()
-        s.queries, s.nq, s.k, s.nprobe, s.nprobe_per_query = p(q), nq, k, nprobe, p(npq)
-        s.z, s.queries_on_device, s.outputs_on_device = z, 0, 0
-        s.out_ids, s.out_dists = p(ids), p(d)
+        s = self._args(q, k, nprobe, z, npq, mode, budget, ids, d)
         tot = M8Stats()
         check(lib().e8q_m8_search(self.ctx.h, self.h, ctypes.byref(s), ctypes.byref(tot), p(st)))
         return ids, d, st, tot
+
+    def batch_checksum(self, queries, k, nprobe, z, nprobe_per_query=None, budget=0):
+        """Per-query checksum of the batched form's integer coarse sums."""
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        npq = None if nprobe_per_query is None else np.ascontiguousarray(nprobe_per_query, np.uint32)
+        s = self._args(q, k, nprobe, z, npq, BATCHED, budget, ids, d)
+        chk = np.empty(nq, np.uint32)
+        check(lib().e8q_m8_debug_batch_checksum(self.ctx.h, self.h, ctypes.byref(s), p(chk)))
+        return chk
 
     def close(self):
         if self.h:
