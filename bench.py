@@ -46,9 +46,11 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("e8q", "me8q", "serve", "zipf"), default="e8q",
+    ap.add_argument("--workload", choices=("e8q", "me8q", "serve", "zipf", "me8q_zipf"), default="e8q",
                     help="zipf: the batched / MFMA E8Q configuration, 10M x 768, 16384 Zipf(1.1) lists, "
-                         "k=10, per-query nprobe in {16,32,64,128}")
+                         "k=10, per-query nprobe in {16,32,64,128}; me8q_zipf: the served ME8Q configuration, "
+                         "10M x 1024 PCA-domain power law, 16384 Zipf(1.1) lists, avg_bits 4, plain vs batched "
+                         "over an nprobe grid, then a Poisson request stream through the server")
     ap.add_argument("--serve-index", choices=("e8q", "me8q"), default="e8q")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-workers", type=int, default=3)
@@ -68,6 +70,13 @@ def parse():
     ap.add_argument("--modes", default=None,
                     help="plain,batched,batched_graph (e8q) / plain,batched,mfma,mfma_graph (zipf)")
     ap.add_argument("--latency-queries", type=int, default=200)
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain form (me8q_zipf)")
+    ap.add_argument("--plain-budget", type=float, default=60.0,
+                    help="me8q_zipf: seconds per plain step above which larger nprobe points are skipped")
+    ap.add_argument("--serve-load", default="0.25,0.5,0.75",
+                    help="me8q_zipf: offered request rates as fractions of the measured serving capacity")
+    ap.add_argument("--serve-plain-requests", type=int, default=40,
+                    help="me8q_zipf: requests of the stream timed through the plain path")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -80,12 +89,14 @@ def parse():
     a = ap.parse_args()
     me = a.workload == "me8q" or (a.workload == "serve" and a.serve_index == "me8q")
     zipf = a.workload == "zipf"
-    a.n = a.n or (10_000_000 if zipf else 1_000_000)
+    mz = a.workload == "me8q_zipf"
+    a.n = a.n or (10_000_000 if zipf or mz else 1_000_000)
     a.k = a.k or (10 if zipf else 100)
-    a.modes = a.modes or ("plain,batched,mfma,mfma_graph" if zipf else "plain,batched,batched_graph")
-    a.dim = a.dim or (960 if me else 768 if zipf else 128)
-    a.nlist = a.nlist or (4096 if me else 16384 if zipf else 1024)
-    a.grid = a.grid or ("32,64,128,256" if me else "16,32,64,128")
+    a.modes = a.modes or ("plain,batched,mfma,mfma_graph" if zipf else "plain,batched" if a.workload == "me8q"
+                          else "batched,plain" if mz else "plain,batched,batched_graph")
+    a.dim = a.dim or (960 if me else 768 if zipf else 1024 if mz else 128)
+    a.nlist = a.nlist or (4096 if me else 16384 if zipf or mz else 1024)
+    a.grid = a.grid or ("32,64,128,256" if me else "8,16,32,64,128,256" if mz else "16,32,64,128")
     return a
 
 
@@ -238,12 +249,151 @@ def make_me8q_data(torch, n, dim, nlist, nq, seed=42):
     return base.contiguous(), centers.contiguous(), cid, queries.contiguous(), var
 
 
+def me8q_layout(plan):
+    """Scan record and refine row bytes of a plan (per 32-vector batch / vector)."""
+    coded = [(ln, b) for ln, b in plan if b > 0]
+    plane_cols = sum(ln // 8 for ln, _ in coded)
+    rec_bytes = 32 * plane_cols + 160 * len(coded)
+    refine_stride = 8 * len(coded) + sum((b - 1) * ln // 8 for ln, b in coded if b >= 2)
+    return plane_cols, rec_bytes, refine_stride
+
+
+def me8q_forms(idx, q_dev, nq, k, grid, modes, gt, args, plan, plain_budget_s=None):
+    """Every form over the nprobe grid: W warm-up and K timed search_device
+    calls per point (the plain form at the largest grid points, whose steps
+    take seconds on head lists, with one warm-up at most)."""
+    import numpy as np
+    import torch
+    plane_cols, rec_bytes, refine_stride = me8q_layout(plan)
+    ids_dev = torch.empty((nq, k), dtype=torch.int32, device="cuda:0")
+    d_dev = torch.empty((nq, k), dtype=torch.float32, device="cuda:0")
+    results = {}
+    for mode in modes:
+        rows = []
+        for nprobe in grid:
+            steps, warm = args.steps, args.warmup
+            if mode == "plain" and plain_budget_s is not None:
+                steps, warm = max(1, min(steps, args.plain_steps)), min(warm, 1)
+            log(f"me8q {mode} nprobe={nprobe}: {warm} warm-up + {steps} timed steps")
+            for _ in range(warm):
+                idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr(), mode=mode)
+            torch.cuda.synchronize()
+            times, scan_ms, tot_ms, st = [], [], [], None
+            for _ in range(steps):
+                torch.cuda.synchronize()
+                ts = time.perf_counter()
+                st = idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr(),
+                                       mode=mode)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - ts)
+                scan_ms.append(st["device_scan_ms"])
+                tot_ms.append(st["device_total_ms"])
+            ids = ids_dev.cpu().numpy().view(np.uint32)
+            ms = 1000.0 * sum(times) / len(times)
+            row = {"nprobe": nprobe, "ms_per_step": ms, "qps": nq / (ms / 1000.0),
+                   "recall": recall_at(gt, ids, 10, k) if gt is not None else -1.0,
+                   "recall_10at10": recall_at(gt, ids, 10, 10) if gt is not None else -1.0,
+                   "scan_kernel_ms": sum(scan_ms) / len(scan_ms), "device_total_ms": sum(tot_ms) / len(tot_ms),
+                   "batches": st["batches"], "survivors": st["survivors"], "candidates": st["candidates"],
+                   "refined": st["refined"], "groups": st["groups"], "form": st["form"],
+                   "overflowed": st["overflowed"], "pairs_scored": st["pairs_scored"]}
+            if mode == "plain":
+                algo = row["batches"] * rec_bytes + row["survivors"] * (refine_stride + 4)
+                row["scan_algorithmic_bytes"] = algo
+                row["scan_achieved_gbs"] = algo / (row["scan_kernel_ms"] / 1000.0) / 1e9
+                extra = f"scan kernel {row['scan_kernel_ms']:.2f} ms = {row['scan_achieved_gbs']:.0f} GB/s"
+            else:
+                # int8 MFMA work: 32 one-hot entries per plane column per pair, 2 ops each
+                ops = row["pairs_scored"] * 2 * 32 * plane_cols
+                row["mfma_int8_ops"] = ops
+                row["mfma_tops"] = ops / (row["scan_kernel_ms"] / 1000.0) / 1e12
+                extra = (f"MFMA scan {row['scan_kernel_ms']:.2f} ms = {row['mfma_tops']:.0f} TOP/s int8, "
+                         f"overflowed {row['overflowed']}")
+            rows.append(row)
+            log(f"me8q {mode} nprobe={nprobe}: {ms:.2f} ms/step, {row['qps']:.0f} QPS, recall 10@{k}="
+                f"{row['recall']:.4f} (10@10 {row['recall_10at10']:.4f}), {extra}, survivors "
+                f"{row['survivors']}/{row['candidates']}, refined {row['refined']}")
+            if mode == "plain" and plain_budget_s is not None and ms / 1000.0 > plain_budget_s:
+                log(f"me8q plain: {ms / 1000.0:.1f} s per step, larger nprobe points skipped")
+                break
+        results[mode] = {"rows": rows, "qps_at_target": qps_at_recall(rows, args.target) if gt is not None else None}
+    return results
+
+
+def me8q_line(metric, workload, data, args, results, plan, bits_per_dim, k, nq, build_s, lat, cpu, extra_cfg):
+    """The JSON line of an ME8Q workload: value = the best form's QPS at the
+    recall target; roofline = the batched form's MFMA contraction (int8
+    TOP/s) when it ran, else the plain scan's HBM rate."""
+    plane_cols, rec_bytes, refine_stride = me8q_layout(plan)
+    best_mode = max(results, key=lambda m: (results[m]["qps_at_target"] or 0.0))
+    value = results[best_mode]["qps_at_target"]
+    rows = results[best_mode]["rows"]
+    op = next((r for r in rows if r["recall"] >= args.target), rows[-1])
+    modes = {m: {"qps_at_target": (round(r["qps_at_target"], 1) if r["qps_at_target"] == r["qps_at_target"]
+                                   and r["qps_at_target"] is not None else None),
+                 "grid": [{kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in row.items()}
+                          for row in r["rows"]]} for m, r in results.items()}
+    if "batched" in results:
+        br = results["batched"]["rows"]
+        rop = max(br, key=lambda r: r["pairs_scored"])
+        roof = {"bound": "mfma", "kernel": "M8MfmaScanKernel (v_mfma_i32_32x32x32_i8)",
+                "achieved": round(rop["mfma_tops"], 1), "peak": 5000.0, "unit": "TOP/s",
+                "frac": round(rop["mfma_tops"] / 5000.0, 4), "traffic": None,
+                "basis": f"batched form at nprobe={rop['nprobe']}, per launch: {rop['pairs_scored']} (query, vector) "
+                         f"pairs x 2 x 32 x {plane_cols} plane columns int8 ops / mean kernel time "
+                         f"{rop['scan_kernel_ms']:.3f} ms (HIP events)"}
+    else:
+        pr = results["plain"]["rows"]
+        rop = max(pr, key=lambda r: r["nprobe"])
+        achieved = rop["scan_achieved_gbs"]
+        roof = {"bound": "hbm", "kernel": "M8ScanKernel", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "basis": f"per launch: {rop['batches']} scan records x {rec_bytes} B + {rop['survivors']} "
+                         f"survivors x {refine_stride + 4} B, / mean kernel time {rop['scan_kernel_ms']:.3f} ms"}
+    line = {
+        "metric": metric,
+        "value": round(value, 1) if value is not None and value == value else None,
+        "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * nq / value, 3) if value and value == value else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 LUT / i32 coarse sums (int8 MFMA in the batched form), fp32 scores, fp64 bounds",
+        "data": data,
+        "config": dict({"workload": workload, "plan": plan, "bits_per_dim": bits_per_dim, "form": best_mode,
+                        "operating_nprobe": op["nprobe"], "recall_at_operating_point": op["recall"],
+                        "recall_definition": f"recall@10 = fraction of the exact top-10 (fp64) among the k={k} "
+                                             "results; 10@10 per grid row"}, **extra_cfg),
+        "latency_us": lat,
+        "modes": modes,
+        "build_seconds": round(build_s, 2),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if "plain" in results and "batched" in results:
+        p, b = results["plain"]["qps_at_target"], results["batched"]["qps_at_target"]
+        if p and b and p == p and b == b:
+            line["batched_over_plain"] = round(b / p, 2)
+    return line
+
+
+def single_query_latency(idx, queries, k, nprobe, count, mode="plain"):
+    lat = []
+    qh = queries[:count].cpu().numpy()
+    for i in range(qh.shape[0]):
+        ts = time.perf_counter()
+        idx.search(qh[i:i + 1], k, nprobe, mode=mode)
+        lat.append((time.perf_counter() - ts) * 1e6)
+    lat.sort()
+    return {"p50": round(lat[len(lat) // 2], 1), "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 1),
+            "what": f"single-query search() calls ({mode} form, nprobe={nprobe})"}
+
+
 def run_me8q(args):
     import numpy as np
     import torch
     import e8qlib
 
     grid = [int(x) for x in args.grid.split(",")]
+    modes = [m for m in args.modes.split(",") if m]
     t0 = time.perf_counter()
     base, centers, cid, queries, var = make_me8q_data(torch, args.n, args.dim, args.nlist, args.nq)
     gt = None if args.profile else ground_truth(torch, base, queries).cpu().numpy()
@@ -260,93 +410,29 @@ def run_me8q(args):
     torch.cuda.empty_cache()
     nq, k = args.nq, args.k
     q_dev = queries.contiguous()
-    ids_dev = torch.empty((nq, k), dtype=torch.int32, device="cuda:0")
-    d_dev = torch.empty((nq, k), dtype=torch.float32, device="cuda:0")
-    coded = [(ln, b) for ln, b in plan if b > 0]
-    plane_cols = sum(ln // 8 for ln, _ in coded)
-    rec_bytes = 32 * plane_cols + 160 * len(coded)
-    refine_stride = 8 * len(coded) + sum((b - 1) * ln // 8 for ln, b in coded if b >= 2)
-    rows = []
-    for nprobe in grid:
-        for _ in range(args.warmup):
-            idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr())
-        torch.cuda.synchronize()
-        times, scan_ms, st = [], [], None
-        for _ in range(args.steps):
-            torch.cuda.synchronize()
-            ts = time.perf_counter()
-            st = idx.search_device(q_dev.data_ptr(), nq, k, nprobe, ids_dev.data_ptr(), d_dev.data_ptr())
-            torch.cuda.synchronize()
-            times.append(time.perf_counter() - ts)
-            scan_ms.append(st["device_scan_ms"])
-        ids = ids_dev.cpu().numpy().view(np.uint32)
-        ms = 1000.0 * sum(times) / len(times)
-        row = {"nprobe": nprobe, "ms_per_step": ms, "qps": nq / (ms / 1000.0),
-               "recall": recall_at(gt, ids, 10, k) if gt is not None else -1.0,
-               "recall_10at10": recall_at(gt, ids, 10, 10) if gt is not None else -1.0,
-               "scan_kernel_ms": sum(scan_ms) / len(scan_ms), "batches": st["batches"],
-               "survivors": st["survivors"], "candidates": st["candidates"], "refined": st["refined"],
-               "groups": st["groups"]}
-        algo = row["batches"] * rec_bytes + row["survivors"] * (refine_stride + 4)
-        row["scan_algorithmic_bytes"] = algo
-        row["scan_achieved_gbs"] = algo / (row["scan_kernel_ms"] / 1000.0) / 1e9
-        rows.append(row)
-        log(f"me8q nprobe={nprobe}: {ms:.2f} ms/step, {row['qps']:.0f} QPS, recall 10@{k}={row['recall']:.4f} "
-            f"(10@10 {row['recall_10at10']:.4f}), scan kernel {row['scan_kernel_ms']:.2f} ms = "
-            f"{row['scan_achieved_gbs']:.0f} GB/s, survivors {row['survivors']}/{row['candidates']}, "
-            f"refined {row['refined']} in {row['groups']} groups")
+    results = me8q_forms(idx, q_dev, nq, k, grid, modes, gt, args, plan)
     if args.profile:
         return
-    value = qps_at_recall(rows, args.target)
+    best = max(results, key=lambda m: (results[m]["qps_at_target"] or 0.0))
+    rows = results[best]["rows"]
     op = next((r for r in rows if r["recall"] >= args.target), rows[-1])
-    lat = []
-    qh = queries[:args.latency_queries].cpu().numpy()
-    for i in range(qh.shape[0]):
-        ts = time.perf_counter()
-        idx.search(qh[i:i + 1], k, op["nprobe"])
-        lat.append((time.perf_counter() - ts) * 1e6)
-    lat.sort()
+    lat = single_query_latency(idx, queries, k, op["nprobe"], args.latency_queries, best)
     cpu = None
-    if not args.no_cpu:
-        idx.search_device(q_dev.data_ptr(), nq, k, op["nprobe"], ids_dev.data_ptr(), d_dev.data_ptr())
-        torch.cuda.synchronize()
-        cpu = cpu_baseline_me8q(idx, queries.cpu().numpy(), k, op["nprobe"],
-                                ids_dev.cpu().numpy().view(np.uint32), args)
-        log(f"cpu baseline: {cpu}")
-    # Roofline of the plain scan kernel (its own events, eager launches) at
-    # the largest grid nprobe (the scan-bound regime), and at the plain
-    # form's operating point.
-    rop = opp = op
-    if "plain" in results:
+    if not args.no_cpu and "plain" in results:
         prow = results["plain"]["rows"]
         opp = next((r for r in prow if r["recall"] >= args.target), prow[-1])
-        rop = max(prow, key=lambda r: r["nprobe"])
-    achieved = rop["scan_achieved_gbs"]
-    line = {
-        "metric": "QPS at recall@10>=0.95 (ME8Q mixed-rate plain GPU search, 1 GPU); p50/p99 latency",
-        "value": round(value, 1) if value == value else None,
-        "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * nq / value, 3) if value == value and value > 0 else None,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u8 LUT / i32 coarse sums, fp32 scores, fp64 bounds",
-        "data": "synthetic power-law spectrum (lambda_i ~ i^-1.2), seeded rotation, PCA domain, generated on the GPU",
-        "config": {"workload": f"ME8Q mixed-rate scan: N={args.n}, D={args.dim} PCA-domain power law, "
-                               f"{args.nlist} lists, avg_bits={args.avg_bits}, {nq} queries, k={k}, "
-                               f"nprobe {args.grid}",
-                   "plan": plan, "bits_per_dim": idx.bits_per_dim, "operating_nprobe": op["nprobe"],
-                   "recall_at_operating_point": op["recall"],
-                   "recall_definition": f"recall@10 = fraction of the exact top-10 (fp64) among the k={k} results; "
-                                        "10@10 per grid row"},
-        "latency_us": {"p50": round(lat[len(lat) // 2], 1), "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 1),
-                       "what": "single-query search() calls"},
-        "grid": [{kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in r.items()} for r in rows],
-        "build_seconds": round(build_s, 2),
-        "roofline": {"bound": "hbm", "kernel": "M8ScanKernel", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "basis": f"per launch: {op['batches']} scan records x {rec_bytes} B + {op['survivors']} "
-                              f"survivors x {refine_stride + 4} B, / mean kernel time {op['scan_kernel_ms']:.3f} ms"},
-        "cpu_baseline": cpu,
-    }
+        ids_dev = torch.empty((nq, k), dtype=torch.int32, device="cuda:0")
+        d_dev = torch.empty((nq, k), dtype=torch.float32, device="cuda:0")
+        idx.search_device(q_dev.data_ptr(), nq, k, opp["nprobe"], ids_dev.data_ptr(), d_dev.data_ptr())
+        torch.cuda.synchronize()
+        cpu = cpu_baseline_me8q(idx, queries.cpu().numpy(), k, opp["nprobe"],
+                                ids_dev.cpu().numpy().view(np.uint32), args)
+        log(f"cpu baseline: {cpu}")
+    line = me8q_line("QPS at recall@10>=0.95 (ME8Q mixed-rate search, plain vs batched, 1 GPU); p50/p99 latency",
+                     f"ME8Q mixed-rate scan: N={args.n}, D={args.dim} PCA-domain power law, {args.nlist} lists, "
+                     f"avg_bits={args.avg_bits}, {nq} queries, k={k}, nprobe {args.grid}",
+                     "synthetic power-law spectrum (lambda_i ~ i^-1.2), seeded rotation, PCA domain, generated on the GPU",
+                     args, results, plan, idx.bits_per_dim, k, nq, build_s, lat, cpu, {})
     print(json.dumps(line), flush=True)
 
 
@@ -639,6 +725,198 @@ def run_zipf(args):
     print(json.dumps(line), flush=True)
 
 
+def pca_domain(torch, base, queries, cent, chunk=131072):
+    """The reference runner's PCA step (fp64 covariance, eigh, descending),
+    applied in place to base and to queries / centroids; returns the
+    eigenvalues (the per-dimension variance)."""
+    import numpy as np
+    n, dim = base.shape
+    dev = base.device
+    mean = torch.zeros(dim, dtype=torch.float64, device=dev)
+    for s0 in range(0, n, chunk):
+        mean += base[s0:s0 + chunk].double().sum(0)
+    mean /= n
+    cov = torch.zeros(dim, dim, dtype=torch.float64, device=dev)
+    for s0 in range(0, n, chunk):
+        x = base[s0:s0 + chunk].double() - mean
+        cov += x.T @ x
+    cov /= n
+    ev, vec = np.linalg.eigh(cov.cpu().numpy())
+    order = np.argsort(ev)[::-1]
+    V = torch.from_numpy(vec[:, order].copy()).to(dev).float()
+    meanf = mean.float()
+    for s0 in range(0, n, chunk):
+        base[s0:s0 + chunk] = (base[s0:s0 + chunk] - meanf) @ V
+    return ((queries - meanf) @ V).contiguous(), ((cent - meanf) @ V).contiguous(), ev[order].astype(np.float32)
+
+
+def serve_stream(rng, qpool, count, nprobes):
+    """Requests: sizes log-uniform in [1, 256], nprobe uniform over the
+    log-spaced grid, k uniform over {10, 100}."""
+    import numpy as np
+    reqs = []
+    for _ in range(count):
+        nq = int(np.exp(rng.uniform(0.0, np.log(256.0))))
+        rows = rng.integers(0, qpool.shape[0], nq)
+        reqs.append((np.ascontiguousarray(qpool[rows]), int(rng.choice([10, 100])), int(rng.choice(nprobes))))
+    return reqs
+
+
+def serve_run(srv, reqs, rate, rng):
+    """Submits the stream with exponential gaps at `rate` requests/s (rate
+    None: all at once) and waits; QPS over the span, p50/p99 per request.
+    Logs a progress line every 200 submitted requests."""
+    import threading
+    tickets = [None] * len(reqs)
+    lat = [0.0] * len(reqs)
+    done_t = [0.0] * len(reqs)
+    gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else [0.0] * len(reqs)
+
+    def waiter():
+        for i in range(len(reqs)):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            _, _, info = srv.wait(tickets[i])
+            lat[i] = info["latency_us"]
+            done_t[i] = time.perf_counter()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    start = time.perf_counter()
+    nxt = start
+    for i, (q, k, npb) in enumerate(reqs):
+        nxt += gaps[i]
+        d = nxt - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets[i] = srv.submit(q, k, npb)
+        if (i + 1) % 200 == 0:
+            log(f"  submitted {i + 1}/{len(reqs)} requests")
+    th.join()
+    span = max(done_t) - start
+    ls = sorted(lat)
+    total_q = sum(r[0].shape[0] for r in reqs)
+    return {"offered_rps": round(rate, 1) if rate else "saturation", "qps": round(total_q / span, 1),
+            "requests_per_s": round(len(reqs) / span, 1), "p50_us": round(ls[len(ls) // 2], 1),
+            "p99_us": round(ls[min(len(ls) - 1, int(0.99 * len(ls)))], 1)}
+
+
+def run_me8q_zipf(args):
+    """The served ME8Q configuration: 10M x 1024 unit-normalized vectors with a
+    power-law spectrum (lambda_i ~ i^-1.1) in 16384 Zipf(1.1)-sized lists,
+    taken to the PCA domain, ME8Q at avg_bits 4 built on the device; plain vs
+    batched QPS at recall over an nprobe grid on a 10k-query batch, then a
+    Poisson request stream (sizes log-uniform in [1, 256], nprobe over
+    {8..256}, k in {10, 100}) through the continuous-batching server against
+    one plain search call per request."""
+    import numpy as np
+    import torch
+    import e8qlib
+
+    grid = [int(x) for x in args.grid.split(",")]
+    modes = [m for m in args.modes.split(",") if m]
+    t0 = time.perf_counter()
+    base, cent, lab, queries, sizes = make_zipf_data(torch, args.n, args.dim, args.nlist, args.nq, spectrum=1.1)
+    torch.cuda.synchronize()
+    log(f"zipf data: {time.perf_counter() - t0:.1f} s, largest list {int(sizes.max())}, smallest {int(sizes.min())}")
+    t0 = time.perf_counter()
+    queries, cent, var = pca_domain(torch, base, queries, cent)
+    torch.cuda.synchronize()
+    log(f"PCA domain: {time.perf_counter() - t0:.1f} s, variance head {var[:4]}")
+    gt = None
+    if not args.profile:
+        t0 = time.perf_counter()
+        gt = ground_truth_chunked(torch, base, queries, k=10).cpu().numpy()
+        torch.cuda.synchronize()
+        log(f"ground truth: {time.perf_counter() - t0:.1f} s")
+    idx = e8qlib.Me8qIndex(args.dim, args.avg_bits, 0)
+    lab_dev = lab.contiguous()
+    tb = time.perf_counter()
+    idx.build_device(base.data_ptr(), cent.cpu().numpy(), lab_dev.cpu().numpy().view(np.uint32), lab_dev.data_ptr(),
+                     variance=var)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - tb
+    plan = [tuple(p) for p in idx.plan]
+    log(f"build: {build_s:.2f} s, plan {plan}, {idx.bits_per_dim:.3f} bits/dim, {idx.device_bytes / 1e9:.2f} GB on device")
+    del base, lab_dev
+    torch.cuda.empty_cache()
+    nq, k = args.nq, args.k
+    q_dev = queries.contiguous()
+    results = me8q_forms(idx, q_dev, nq, k, grid, modes, gt, args, plan, plain_budget_s=args.plain_budget)
+    if args.profile:
+        return
+    # serving
+    rng = np.random.default_rng(7)
+    qpool = queries.cpu().numpy()
+    nprobes = [p for p in (8, 16, 32, 64, 128, 256) if p <= args.nlist]
+    reqs = serve_stream(rng, qpool, args.serve_requests, nprobes)
+    total_q = sum(r[0].shape[0] for r in reqs)
+    sample = reqs[:args.serve_plain_requests]
+    log(f"serving: {len(reqs)} requests, {total_q} queries; plain path on the first {len(sample)}")
+    idx.search(sample[0][0], sample[0][1], sample[0][2])
+    plain_lat = []
+    ts = time.perf_counter()
+    timed = 0
+    for q, kk, npb in sample:
+        a = time.perf_counter()
+        idx.search(q, kk, npb)
+        plain_lat.append((time.perf_counter() - a) * 1e6)
+        timed += 1
+        log(f"plain request {timed}: {q.shape[0]} queries, k={kk}, nprobe={npb}, {plain_lat[-1] / 1e3:.1f} ms")
+        if time.perf_counter() - ts > args.plain_budget and timed >= 3:
+            break
+    sample = sample[:timed]
+    plain_s = time.perf_counter() - ts
+    plain_rps = len(sample) / plain_s
+    plain_qps = sum(r[0].shape[0] for r in sample) / plain_s
+    plain_lat.sort()
+    log(f"plain: {plain_qps:.0f} QPS ({plain_rps:.2f} requests/s), service p50 {plain_lat[len(plain_lat) // 2]:.0f} us")
+    srv = e8qlib.Server(idx, workers=args.serve_workers, max_batch=4096, mode="batched")
+    warm = [srv.submit(q, kk, npb) for q, kk, npb in reqs[:20]]
+    for t in warm:
+        srv.wait(t)
+    # capacity first (every request submitted at once), then Poisson streams
+    # offered at fractions of it, each bounded to ~20 s of offered traffic
+    sat = serve_run(srv, reqs, None, rng)
+    sat["offered"] = "saturation"
+    log(f"served (saturation): {sat['qps']:.0f} QPS, {sat['requests_per_s']:.1f} requests/s, "
+        f"p50 {sat['p50_us']:.0f} us, p99 {sat['p99_us']:.0f} us")
+    runs = [sat]
+    for frac in [float(x) for x in args.serve_load.split(",")]:
+        rate = sat["requests_per_s"] * frac
+        sub = reqs[:max(50, min(len(reqs), int(rate * 20.0)))]
+        run = serve_run(srv, sub, rate, rng)
+        run["offered"] = f"{frac:g} x capacity"
+        run["offered_x_plain"] = round(rate / plain_rps, 1)
+        runs.append(run)
+        log(f"served ({run['offered']}, {run['offered_x_plain']}x plain): {run['qps']:.0f} QPS, "
+            f"{run['requests_per_s']:.1f} requests/s, p50 {run['p50_us']:.0f} us, p99 {run['p99_us']:.0f} us")
+    sst = srv.stats()
+    srv.close()
+    best = max(runs, key=lambda r: r["qps"])
+    lat_run = runs[-1] if len(runs) > 1 else runs[0]   # the highest offered fraction below capacity
+    serving = {"requests": len(reqs), "queries": total_q, "workers": args.serve_workers,
+               "stream": "sizes log-uniform [1,256], nprobe uniform over " + str(nprobes) +
+                         ", k in {10,100}, Poisson arrivals (and one run with every request submitted at once)",
+               "plain": {"qps": round(plain_qps, 1), "requests_per_s": round(plain_rps, 3),
+                         "requests_timed": len(sample),
+                         "service_p50_us": round(plain_lat[len(plain_lat) // 2], 1),
+                         "service_p99_us": round(plain_lat[min(len(plain_lat) - 1, int(0.99 * len(plain_lat)))], 1)},
+               "served_runs": runs, "served_over_plain": round(best["qps"] / plain_qps, 2), "server_stats": sst}
+    line = me8q_line("QPS at recall@10>=0.95 (ME8Q, plain vs batched/served, 1 GPU); p50/p99 latency",
+                     f"served ME8Q: N={args.n}, D={args.dim} unit-normalized power law (lambda_i ~ i^-1.1) in the "
+                     f"PCA domain, {args.nlist} Zipf(1.1) lists, avg_bits={args.avg_bits}, {nq} queries, k={k}, "
+                     f"nprobe {args.grid}",
+                     "synthetic, generated on the GPU (seed 7); PCA by fp64 covariance + eigh",
+                     args, results, plan, idx.bits_per_dim, k, nq, build_s,
+                     {"p50": lat_run["p50_us"], "p99": lat_run["p99_us"],
+                      "what": f"per request through the server, Poisson arrivals at {lat_run['offered']} "
+                              f"({lat_run['qps']:.0f} QPS served)"},
+                     None, {"list_sizes": {"largest": int(sizes.max()), "smallest": int(sizes.min())}})
+    line["serving"] = serving
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calib:
@@ -656,6 +934,9 @@ def main():
         return
     if args.workload == "zipf":
         run_zipf(args)
+        return
+    if args.workload == "me8q_zipf":
+        run_me8q_zipf(args)
         return
     import numpy as np
     import torch

@@ -94,6 +94,7 @@ struct e8q_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // search timing
     DBuf ws;   // per-call workspace, grown on demand
+    DBuf pool; // candidate pool of the batched ME8Q form, sized per call after its count pass
 
     // Replay of a search's device sequence as a HIP graph, per call shape.
     // A shape's first call runs eagerly and registers the shape; its second

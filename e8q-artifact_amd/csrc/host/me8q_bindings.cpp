@@ -82,6 +82,42 @@ class PyMe8qIndex {
         built_ = true;
     }
 
+    // build() with the rows already in device memory (base_ptr: n x dim
+    // float32, cluster_ids_ptr: n uint32); centroids, cluster_ids and the
+    // variance (or a plan_spec) as host arrays.
+    void build_device(uintptr_t base_ptr, py::handle centroids, py::handle cluster_ids, uintptr_t cluster_ids_ptr,
+                      py::handle variance, bool drop_energy, float prune_alpha, const std::string& plan_spec,
+                      double extra_budget_bits) {
+        auto c = EnsureArr<float>(centroids, "centroids", 2);
+        auto ids = EnsureArr<uint32_t>(cluster_ids, "cluster_ids", 1);
+        if (!base_ptr || !cluster_ids_ptr) throw std::invalid_argument("null device pointer");
+        if (static_cast<size_t>(c.shape(1)) != dim_)
+            throw std::invalid_argument("centroid dimension does not match index dim");
+        py::array_t<float, py::array::c_style | py::array::forcecast> var;
+        const float* varp = nullptr;
+        if (!variance.is_none()) {
+            var = EnsureArr<float>(variance, "variance", 1);
+            if (static_cast<size_t>(var.shape(0)) != dim_) throw std::invalid_argument("variance length must match dim");
+            varp = var.data();
+        }
+        e8qlib::me8q::BuildOptions opt;
+        opt.avg_bits = avg_bits_;
+        opt.drop_energy = drop_energy;
+        opt.prune_alpha = prune_alpha;
+        opt.plan_spec = plan_spec;
+        opt.extra_budget_bits = extra_budget_bits;
+        if (seed_ != 0) opt.rot_seed = seed_;
+        auto idx = std::make_unique<Me8qIndex>();
+        {
+            py::gil_scoped_release nogil;
+            idx->BuildFromDevice(reinterpret_cast<const float*>(base_ptr), c.data(), ids.data(),
+                                 reinterpret_cast<const uint32_t*>(cluster_ids_ptr), static_cast<size_t>(ids.shape(0)),
+                                 dim_, static_cast<size_t>(c.shape(0)), opt, varp);
+        }
+        index_ = std::move(idx);
+        built_ = true;
+    }
+
     py::tuple search(py::handle queries, size_t k, size_t nprobe, size_t num_threads, bool return_stats,
                      py::object nprobe_per_query, const std::string& mode) {
         (void)num_threads;
@@ -227,6 +263,10 @@ void RegisterMe8q(py::module_& m) {
         .def("build", &PyMe8qIndex::build, py::arg("data"), py::arg("centroids"), py::arg("cluster_ids"),
              py::arg("num_threads") = 1, py::arg("variance") = py::none(), py::arg("drop_energy") = true,
              py::arg("prune_alpha") = 0.0F, py::arg("plan_spec") = "", py::arg("extra_budget_bits") = 0.0)
+        .def("build_device", &PyMe8qIndex::build_device, py::arg("base_ptr"), py::arg("centroids"),
+             py::arg("cluster_ids"), py::arg("cluster_ids_ptr"), py::arg("variance") = py::none(),
+             py::arg("drop_energy") = true, py::arg("prune_alpha") = 0.0F, py::arg("plan_spec") = "",
+             py::arg("extra_budget_bits") = 0.0)
         .def("search", &PyMe8qIndex::search, py::arg("queries"), py::arg("k"), py::arg("nprobe"),
              py::arg("num_threads") = 1, py::arg("return_stats") = false, py::arg("nprobe_per_query") = py::none(),
              py::arg("mode") = "plain")

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <limits>
@@ -241,6 +242,22 @@ void Me8qIndex::CreateDevice() {
 
 void Me8qIndex::Build(const float* base, size_t ld, const float* centroids, size_t ld_c, const uint32_t* cluster_ids,
                       size_t n, size_t dim, size_t num_clusters, const BuildOptions& opt, const float* data_variance) {
+    BuildImpl(base, ld, centroids, ld_c, cluster_ids, nullptr, n, dim, num_clusters, opt, data_variance);
+}
+
+void Me8qIndex::BuildFromDevice(const float* base_device, const float* centroids, const uint32_t* cluster_ids_host,
+                                const uint32_t* cluster_ids_device, size_t n, size_t dim, size_t num_clusters,
+                                const BuildOptions& opt, const float* data_variance) {
+    if (cluster_ids_device == nullptr) throw std::invalid_argument("Me8qIndex::BuildFromDevice: null device cluster ids");
+    if (data_variance == nullptr && opt.plan_spec.empty())
+        throw std::invalid_argument("Me8qIndex::BuildFromDevice: the plan needs a variance or a plan_spec");
+    BuildImpl(base_device, dim, centroids, dim, cluster_ids_host, cluster_ids_device, n, dim, num_clusters, opt,
+              data_variance);
+}
+
+void Me8qIndex::BuildImpl(const float* base, size_t ld, const float* centroids, size_t ld_c,
+                          const uint32_t* cluster_ids, const uint32_t* cluster_ids_device, size_t n, size_t dim,
+                          size_t num_clusters, const BuildOptions& opt, const float* data_variance) {
     if (base == nullptr || centroids == nullptr || cluster_ids == nullptr || n == 0 || dim == 0 || num_clusters == 0)
         throw std::invalid_argument("Me8qIndex::Build: empty input");
     if (opt.max_bits < 1 || opt.max_bits > kMaxBits)
@@ -304,7 +321,10 @@ void Me8qIndex::Build(const float* base, size_t ld, const float* centroids, size
         rows = packed.data();
     }
     std::lock_guard<std::mutex> lk(DeviceMutex());
-    Check(e8q_m8_encode(ctx_, dev_, rows, cent.data(), cluster_ids, rank.data(), 0));
+    if (cluster_ids_device != nullptr)
+        Check(e8q_m8_encode(ctx_, dev_, rows, cent.data(), cluster_ids_device, rank.data(), 1));
+    else
+        Check(e8q_m8_encode(ctx_, dev_, rows, cent.data(), cluster_ids, rank.data(), 0));
 }
 
 void Me8qIndex::Save(const char* path) const {
@@ -451,6 +471,15 @@ size_t Me8qIndex::memory_bytes() const {
 
 size_t Me8qIndex::device_bytes() const { return dev_ ? e8q_m8_device_bytes(dev_) : 0; }
 
+namespace {
+// Threshold-pass length of the batched form in 32-vector batches
+// (E8Q_M8_BATCH_BUDGET; 0 = the library default).
+uint32_t BatchedBudget() {
+    const char* e = std::getenv("E8Q_M8_BATCH_BUDGET");
+    return e ? static_cast<uint32_t>(std::strtoul(e, nullptr, 10)) : 0u;
+}
+}  // namespace
+
 void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t nprobe, const uint32_t* nprobe_per_query,
                              uint32_t* out_ids, float* out_dists, SearchStats* stats, bool queries_on_device,
                              bool outputs_on_device, uint64_t* per_query_stats, int mode) const {
@@ -468,6 +497,7 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
     a.out_dists = out_dists;
     a.outputs_on_device = outputs_on_device ? 1 : 0;
     a.mode = mode;
+    a.batched_budget = BatchedBudget();
     e8q_m8_stats st{};
     {
         std::lock_guard<std::mutex> lk(DeviceMutex());
@@ -506,6 +536,7 @@ void Me8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, s
     a.out_ids = out_ids;
     a.out_dists = out_dists;
     a.mode = mode;
+    a.batched_budget = BatchedBudget();
     Check(e8q_m8_search(ctx, dev_, &a, nullptr, nullptr));
 }
 

@@ -95,6 +95,13 @@ class Me8qIndex {
     void Build(const float* base, size_t ld, const float* centroids, size_t ld_c, const uint32_t* cluster_ids,
                size_t n, size_t dim, size_t num_clusters, const BuildOptions& opt,
                const float* data_variance = nullptr);
+    // Same, with the base rows (n x dim, dense) and a copy of cluster_ids in
+    // device memory; centroids, cluster_ids and variance on the host.  The
+    // plan needs data_variance (or opt.plan_spec): the rows are not read on
+    // the host.
+    void BuildFromDevice(const float* base_device, const float* centroids, const uint32_t* cluster_ids_host,
+                         const uint32_t* cluster_ids_device, size_t n, size_t dim, size_t num_clusters,
+                         const BuildOptions& opt, const float* data_variance);
     void Save(const char* path) const;
     void Load(const char* path);
 
@@ -129,6 +136,9 @@ class Me8qIndex {
     [[nodiscard]] e8q_m8index* device_index() const { return dev_; }
 
    private:
+    void BuildImpl(const float* base, size_t ld, const float* centroids, size_t ld_c, const uint32_t* cluster_ids,
+                   const uint32_t* cluster_ids_device, size_t n, size_t dim, size_t num_clusters,
+                   const BuildOptions& opt, const float* data_variance);
     void CreateDevice();
     void Reset();
     size_t n_ = 0, dim_ = 0, padded_ = 0, nlist_ = 0, bytes_per_vector_ = 0;

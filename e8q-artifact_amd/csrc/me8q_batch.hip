@@ -24,15 +24,17 @@
 //      queries x 32 vectors and the integer sums are exact (minus 256 per
 //      column, added back).  Per run the kernel then evaluates the
 //      reference's coarse bound terms, and after the last run tests
-//      dc - (z sqrt(pool) + qa) against tau_q; passing vectors are appended
-//      to the query's candidate list;
+//      dc - (z sqrt(pool) + qa) against tau_q.  The kernel runs twice: the
+//      first pass counts each query's passing vectors, a scan turns the
+//      counts into offsets in one candidate pool, the second pass (the same
+//      arithmetic, so the same vectors pass) writes them there;
 //   3. refine: per candidate the coarse terms again in the reference's own
 //      arithmetic (fp32 for lanes its AVX2 filter covers, fp64 for the rest),
 //      the run order of the reference's cascade for a group of one
 //      candidate (decreasing t^2 / nblk), the exact run scores (the plain
 //      kernel's even/odd block chains and AVX horizontal sums) and the
 //      cascade's death test against tau_q; the survivors' exact distances
-//      below tau_q go to a short per-query list;
+//      below tau_q are written over the task's own pool slots;
 //   4. select: one wave per query sorts its threshold-pass list and its
 //      survivors by (distance, id) and writes the top-k.
 //
@@ -40,8 +42,7 @@
 // exact estimator top-k over the probed lists, as the plain form's.  With a
 // finite z the bounds are the reference's with tau_q in place of the live
 // threshold, which is at least the reference's threshold after the same
-// vectors.  Queries whose candidate or survivor list overflows are searched
-// again by the plain kernel.
+// vectors.
 #include <hip/hip_runtime.h>
 
 #include "e8q_scan_dev.h"
@@ -72,7 +73,7 @@ __device__ __forceinline__ float M8KeyDist(uint32_t m) {
 constexpr int kM8Tiles = 2;
 constexpr uint32_t kM8ARow = 48;   // bytes per staged (column, pair) row: 32 + 16 pad
 
-template <bool kChk>
+template <bool kChk, bool kWrite>
 __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     const M8View& v = a.v;
     __shared__ uint32_t s_it;
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     __shared__ float s_lmx[kM8BatchRuns][kM8BatchQTile], s_qe[kM8BatchRuns][kM8BatchQTile];
     __shared__ float s_ra[kM8BatchRuns][kM8BatchQTile], s_wf[kM8BatchRuns][kM8BatchQTile];
     __shared__ __align__(16) uint32_t s_oh[16 * 4];
+    __shared__ uint32_t s_hist[kChk || kWrite ? 1 : kM8BatchQTile * kM8TauBins];   // count pass
     // The query operand of eight plane columns for the item's 32 pairs, two
     // buffers: [buffer][column][pair][32 bytes + pad], already int8.
     __shared__ __align__(16) uint8_t s_a[2 * 8 * kM8BatchQTile * kM8ARow];
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
         if (tid < kM8BatchQTile) {   // rows past np carry the first pair (computed, never appended)
             const uint64_t pr = G(a.pairs)[p0 + (tid < np ? tid : 0u)];
             const uint32_t q = static_cast<uint32_t>(pr >> 32), p = static_cast<uint32_t>(pr);
-            const float tau = G(a.first_d)[static_cast<uint64_t>(q) * k + k - 1];
+            const float tau = kWrite ? G(a.tau)[q] : G(a.first_d)[static_cast<uint64_t>(q) * k + k - 1];
             const uint64_t cv = G(a.cover)[q];
             s_q[tid] = q;
             s_p[tid] = p;
@@ -121,6 +123,8 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                 s_ra[r][tid] = static_cast<float>(G(a.sqd2)[(static_cast<uint64_t>(q) * maxp + p) * R + r]);
             }
         }
+        if (!kChk && !kWrite)
+            for (uint32_t e = tid; e < kM8BatchQTile * kM8TauBins; e += 256) s_hist[e] = 0u;
         __syncthreads();
         const uint32_t nt = (wave < nb ? 1u : 0u) + (wave + 4 < nb ? 1u : 0u);
         const uint64_t lb = G(v.list_begin)[c], cnt = G(v.list_begin)[c + 1] - lb;
@@ -237,14 +241,29 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                     const float tau = s_tau[row];
                     const float margin = 2.4414062e-4F * (fabsf(dc[t][i]) + fabsf(slack) + fabsf(tau));
                     const bool kept = scored && fresh && !(dc[t][i] - slack >= tau + margin);
+                    if (!kChk && !kWrite && scored && fresh && tau < finf) {
+                        // the coarse upper bound (the reference's bound read the
+                        // other way) into the query's histogram over [0, tau0)
+                        const float ub = dc[t][i] + slack;
+                        if (ub < tau) {
+                            const uint32_t bin = min(kM8TauBins - 1,
+                                                     static_cast<uint32_t>(static_cast<float>(kM8TauBins) * fmaxf(ub, 0.0F) / tau));
+                            atomicAdd(&s_hist[row * kM8TauBins + bin], 1u);
+                        }
+                    }
                     const uint64_t m = __ballot(kept);
                     const uint32_t hm = static_cast<uint32_t>(m >> (32 * h));
-                    uint32_t base = 0;
-                    if (n == 0 && hm != 0) base = atomicAdd(&a.ncand[q], static_cast<uint32_t>(__popc(hm)));
-                    base = static_cast<uint32_t>(__shfl(static_cast<int>(base), static_cast<int>(lane & 32u)));
-                    if (kept) {
-                        const uint32_t slot = base + __popc(hm & ((1u << n) - 1u));
-                        if (slot < a.cap) Gw(a.cand)[static_cast<uint64_t>(q) * a.cap + slot] = (static_cast<uint64_t>(p) << 32) | pos;
+                    if (!kWrite) {
+                        if (n == 0 && hm != 0) atomicAdd(&a.ncand[q], static_cast<uint32_t>(__popc(hm)));
+                    } else {
+                        uint32_t base = 0;
+                        if (n == 0 && hm != 0) base = atomicAdd(&a.nfill[q], static_cast<uint32_t>(__popc(hm)));
+                        base = static_cast<uint32_t>(__shfl(static_cast<int>(base), static_cast<int>(lane & 32u)));
+                        if (kept) {
+                            const uint32_t slot = base + __popc(hm & ((1u << n) - 1u));
+                            if (slot < G(a.ncand)[q])   // the count pass saw the same vectors pass
+                                Gw(a.pool)[G(a.coff)[q] + slot] = (static_cast<uint64_t>(p) << 32) | pos;
+                        }
                     }
                     if (lane == 0) n_cand += __popcll(m);
                     if (kChk) {
@@ -259,67 +278,208 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                 }
             }
         }
+        if (!kChk && !kWrite) {   // the item's histogram counts into the queries' histograms
+            __syncthreads();
+            for (uint32_t e = tid; e < kM8BatchQTile * kM8TauBins; e += 256) {
+                const uint32_t row = e / kM8TauBins, x = s_hist[e];
+                if (x != 0 && row < np) atomicAdd(&a.hist[static_cast<uint64_t>(s_q[row]) * kM8TauBins + e % kM8TauBins], x);
+            }
+        }
         if (tid == 0) ++n_items;
         __syncthreads();
     }
-    if (lane == 0 && a.stats) {
+    if (kWrite && lane == 0 && a.stats) {
         atomicAdd(&a.stats[0], n_pairs);
         atomicAdd(&a.stats[1], n_cand);
         if (tid == 0) atomicAdd(&a.stats[5], n_items);
     }
 }
 
-hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid) {
+hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool write) {
     if (a.v.nrun < 1 || a.v.nrun > kM8BatchRuns || (a.v.plane_cols & 7u) != 0) return hipErrorInvalidValue;
-    if (a.chk) hipLaunchKernelGGL(M8MfmaScanKernel<true>, dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(M8MfmaScanKernel<false>, dim3(grid), dim3(256), 0, st, a);
+    if (write) hipLaunchKernelGGL((M8MfmaScanKernel<false, true>), dim3(grid), dim3(256), 0, st, a);
+    else if (a.chk) hipLaunchKernelGGL((M8MfmaScanKernel<true, false>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((M8MfmaScanKernel<false, false>), dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Refine tasks: ceil(candidates / kM8RefineChunk) per query (none for a query
-// whose list overflowed), exclusive offsets in toff; one workgroup of 1024
-// threads, each owning a contiguous run of queries.
+// Pool offsets (coff, from the count pass's ncand) and refine tasks (toff,
+// ceil(count / kM8RefineChunk) per query); one workgroup of 1024 threads,
+// each owning a contiguous run of queries.
 
-__global__ __launch_bounds__(1024) void M8TaskScanKernel(M8BatchArgs a) {
-    __shared__ uint32_t sp[1024];
+__global__ __launch_bounds__(1024) void M8TaskScanKernel(M8BatchArgs a, const uint32_t* __restrict__ count,
+                                                         int with_offsets) {
+    __shared__ unsigned long long sc[1024];
+    __shared__ uint32_t st[1024];
     const uint64_t nq = a.nq;
     const uint64_t per = (nq + 1023) / 1024;
     const uint64_t q0 = min<uint64_t>(nq, threadIdx.x * per), q1 = min<uint64_t>(nq, q0 + per);
-    auto tasks = [&](uint64_t q) -> uint32_t {
-        const uint32_t nc = a.ncand[q];
-        return nc > a.cap ? 0u : (nc + kM8RefineChunk - 1) / kM8RefineChunk;
-    };
-    uint32_t tot = 0;
-    for (uint64_t q = q0; q < q1; ++q) tot += tasks(q);
-    sp[threadIdx.x] = tot;
+    unsigned long long tc = 0;
+    uint32_t tt = 0;
+    for (uint64_t q = q0; q < q1; ++q) {
+        tc += a.ncand[q];
+        tt += (count[q] + kM8RefineChunk - 1) / kM8RefineChunk;
+    }
+    sc[threadIdx.x] = tc;
+    st[threadIdx.x] = tt;
     __syncthreads();
     for (uint32_t o = 1; o < 1024; o <<= 1) {
-        const uint32_t x = threadIdx.x >= o ? sp[threadIdx.x - o] : 0u;
+        const unsigned long long xc = threadIdx.x >= o ? sc[threadIdx.x - o] : 0ull;
+        const uint32_t xt = threadIdx.x >= o ? st[threadIdx.x - o] : 0u;
         __syncthreads();
-        sp[threadIdx.x] += x;
+        sc[threadIdx.x] += xc;
+        st[threadIdx.x] += xt;
         __syncthreads();
     }
-    uint32_t off = sp[threadIdx.x] - tot;
+    uint64_t oc = sc[threadIdx.x] - tc;
+    uint32_t ot = st[threadIdx.x] - tt;
     for (uint64_t q = q0; q < q1; ++q) {
-        a.toff[q] = off;
-        off += tasks(q);
+        if (with_offsets) a.coff[q] = oc;
+        a.toff[q] = ot;
+        oc += a.ncand[q];
+        ot += (count[q] + kM8RefineChunk - 1) / kM8RefineChunk;
     }
     if (threadIdx.x == 1023) {
-        a.toff[nq] = sp[1023];
+        if (with_offsets) a.coff[nq] = sc[1023];
+        a.toff[nq] = st[1023];
         a.counters[3] = 0;
     }
 }
 
-hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a) {
-    hipLaunchKernelGGL(M8TaskScanKernel, dim3(1), dim3(1024), 0, st, a);
+hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a, const uint32_t* count, bool with_offsets) {
+    hipLaunchKernelGGL(M8TaskScanKernel, dim3(1), dim3(1024), 0, st, a, count, with_offsets ? 1 : 0);
+    return hipGetLastError();
+}
+
+// tau[q]: tau0, lowered to the upper edge of the first histogram bin at
+// which the count of coarse upper bounds below it (and of the threshold
+// pass's own exact distances) reaches k.
+__global__ __launch_bounds__(256) void M8TauKernel(M8BatchArgs a) {
+    const uint64_t q = blockIdx.x * 256ull + threadIdx.x;
+    if (q >= a.nq) return;
+    const uint32_t k = a.k;
+    const float tau0 = a.first_d[q * k + k - 1];
+    float tau = tau0;
+    if (tau0 < __builtin_inff() && tau0 > 0.0F) {
+        uint32_t h[kM8TauBins];
+        for (uint32_t b = 0; b < kM8TauBins; ++b) h[b] = a.hist[q * kM8TauBins + b];
+        for (uint32_t i = 0; i < k; ++i) {
+            const float d = a.first_d[q * k + i];
+            if (d < tau0)
+                ++h[min(kM8TauBins - 1, static_cast<uint32_t>(static_cast<float>(kM8TauBins) * fmaxf(d, 0.0F) / tau0))];
+        }
+        uint32_t c = 0;
+        for (uint32_t b = 0; b < kM8TauBins; ++b) {
+            c += h[b];
+            if (c >= k) {
+                tau = fminf(tau0, static_cast<float>(b + 1) * (tau0 / static_cast<float>(kM8TauBins)));
+                break;
+            }
+        }
+    }
+    a.tau[q] = tau;
+}
+
+hipError_t LaunchM8Tau(hipStream_t st, const M8BatchArgs& a) {
+    if (a.nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(M8TauKernel, dim3(static_cast<uint32_t>((a.nq + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Refine.  A workgroup of 256 threads takes one task (up to 64 candidates of
-// one query); 16 groups of 16 lanes each take four of them, one at a time.
-// LDS: the query's rotated vector, its u8 LUT, the runs' level tables.
+// Refine.  One wave takes one task (up to 64 candidates of one query), one
+// lane per candidate, so the loads of 64 candidates are in flight together.
+// LDS: the query's rotated vector, its u8 LUT, the level tables of the runs
+// at rates <= kM8LdsLevRate.
+
+// The exact score of one run for one vector, computed by one lane in the
+// order the reference's AVX2 code uses (me8q_score_avx2.cpp: even blocks in
+// one accumulator and odd blocks in another, per component, added, then the
+// horizontal sum): e[ci] / o[ci] are the FMA chains of component ci over the
+// even / odd blocks in block order.  Digit decode as M8HalfChainT.  Runs are
+// multiples of 64 dimensions, so a run holds whole groups of eight blocks, and
+// its plane bytes and fields start 8-byte aligned in their rows.
+template <int T>
+__device__ __forceinline__ float M8LaneScoreT(const uint8_t* __restrict__ pb, const uint8_t* __restrict__ fr,
+                                              const float* __restrict__ lev, const float* __restrict__ qr,
+                                              uint32_t nblk) {
+    constexpr uint32_t rate = T + 1;
+    constexpr uint32_t NW = T > 0 ? T : 1;
+    // `nb` bits at bit `pos` of the little-endian words w (positions are
+    // compile-time constants once the block loop is unrolled)
+    auto bits = [](const uint64_t* w, uint32_t pos, uint32_t nb) -> uint32_t {
+        const uint32_t wi = pos >> 6, off = pos & 63u;
+        uint64_t x = w[wi] >> off;
+        if (off + nb > 64) x |= w[wi + 1] << (64 - off);
+        return static_cast<uint32_t>(x) & ((1u << nb) - 1u);
+    };
+    float e[8] = {0, 0, 0, 0, 0, 0, 0, 0}, o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // eight blocks at a time: their plane bytes (one 8-byte load) and their
+    // fields (8 T bytes, T 8-byte loads), all loads in flight together
+    for (uint32_t g8 = 0; g8 < nblk; g8 += 8) {
+        const uint64_t pw = *reinterpret_cast<const uint64_t*>(pb + g8);
+        uint64_t fw[NW];
+#pragma unroll
+        for (uint32_t i = 0; i < NW; ++i)
+            fw[i] = T > 0 ? *reinterpret_cast<const uint64_t*>(fr + static_cast<uint64_t>(g8) * T + 8 * i) : 0ull;
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            const uint32_t pbyte = static_cast<uint32_t>(pw >> (8 * b)) & 0xFFu;
+            uint32_t dg[8], br;
+            if constexpr (T == 0) {
+#pragma unroll
+                for (uint32_t ci = 0; ci < 7; ++ci) dg[ci] = (pbyte >> (7 - ci)) & 1u;
+                dg[7] = __popc(pbyte >> 1) & 1u;
+                br = pbyte & 1u;
+            } else {
+                const uint32_t base = 8 * b * T;
+                uint32_t par = 0;
+#pragma unroll
+                for (uint32_t ci = 0; ci < 7; ++ci) {
+                    const uint32_t f = bits(fw, base + T * ci, T);
+                    par ^= f & 1u;
+                    dg[ci] = f | (((pbyte >> (7 - ci)) & 1u) << T);
+                }
+                const uint32_t f7 = T > 1 ? bits(fw, base + 7 * T, T - 1) : 0u;
+                dg[7] = ((f7 << 1) | par) | ((pbyte & 1u) << T);
+                br = bits(fw, base + 8 * T - 1, 1);
+            }
+            const float* lb = lev + (br << rate);
+            const float* qb = qr + (g8 + b) * 8;
+            if (b & 1u) {
+#pragma unroll
+                for (uint32_t ci = 0; ci < 8; ++ci) o[ci] = fmaf(lb[dg[ci]], qb[ci], o[ci]);
+            } else {
+#pragma unroll
+                for (uint32_t ci = 0; ci < 8; ++ci) e[ci] = fmaf(lb[dg[ci]], qb[ci], e[ci]);
+            }
+        }
+    }
+    float v[8];
+#pragma unroll
+    for (uint32_t ci = 0; ci < 8; ++ci) v[ci] = e[ci] + o[ci];
+    return ((v[0] + v[4]) + (v[2] + v[6])) + ((v[1] + v[5]) + (v[3] + v[7]));
+}
+
+__device__ float M8LaneScore(uint32_t rate, const uint8_t* pb, const uint8_t* fr, const float* lev, const float* qr,
+                             uint32_t nblk) {
+    switch (rate) {
+        case 1: return M8LaneScoreT<0>(pb, fr, lev, qr, nblk);
+        case 2: return M8LaneScoreT<1>(pb, fr, lev, qr, nblk);
+        case 3: return M8LaneScoreT<2>(pb, fr, lev, qr, nblk);
+        case 4: return M8LaneScoreT<3>(pb, fr, lev, qr, nblk);
+        case 5: return M8LaneScoreT<4>(pb, fr, lev, qr, nblk);
+        case 6: return M8LaneScoreT<5>(pb, fr, lev, qr, nblk);
+        case 7: return M8LaneScoreT<6>(pb, fr, lev, qr, nblk);
+        case 8: return M8LaneScoreT<7>(pb, fr, lev, qr, nblk);
+        case 9: return M8LaneScoreT<8>(pb, fr, lev, qr, nblk);
+        case 10: return M8LaneScoreT<9>(pb, fr, lev, qr, nblk);
+        case 11: return M8LaneScoreT<10>(pb, fr, lev, qr, nblk);
+        case 12: return M8LaneScoreT<11>(pb, fr, lev, qr, nblk);
+        default: return M8LaneScoreT<12>(pb, fr, lev, qr, nblk);
+    }
+}
 
 struct M8RefLayout {
     uint32_t q, lut, lev, total;
@@ -331,7 +491,8 @@ struct M8RefLayout {
     }
 };
 
-__global__ __launch_bounds__(256) void M8RefineKernel(M8BatchArgs a) {
+template <bool kFirst>
+__global__ __launch_bounds__(64) void M8RefineKernel(M8BatchArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const M8View& v = a.v;
     const uint32_t P = static_cast<uint32_t>(v.padded), PC = v.plane_cols, R = v.nrun, k = a.k, maxp = a.maxp;
@@ -339,14 +500,13 @@ __global__ __launch_bounds__(256) void M8RefineKernel(M8BatchArgs a) {
     float* sq = reinterpret_cast<float*>(smem + L.q);
     uint8_t* slut = smem + L.lut;
     float* slev = reinterpret_cast<float*>(smem + L.lev);
-    __shared__ uint32_t s_task, s_q;
-    const uint32_t tid = threadIdx.x, g = tid >> 4, j = tid & 15;
+    const uint32_t lane = threadIdx.x;
     {   // level tables of the runs at rates <= kM8LdsLevRate, run after run
         uint32_t o = 0;
         for (uint32_t r = 0; r < R; ++r) {
             if (v.runs[r].rate > kM8LdsLevRate) continue;
             const uint32_t nq2 = 2u << v.runs[r].rate;
-            for (uint32_t e = tid; e < nq2; e += 256) slev[o + e] = v.runs[r].lev[e];
+            for (uint32_t e = lane; e < nq2; e += 64) slev[o + e] = v.runs[r].lev[e];
             o += nq2;
         }
     }
@@ -357,44 +517,49 @@ __global__ __launch_bounds__(256) void M8RefineKernel(M8BatchArgs a) {
     uint32_t qcur = 0xFFFFFFFFu;
     unsigned long long n_ref = 0, n_surv = 0, n_bits = 0;
     for (;;) {
-        if (tid == 0) {
-            const uint32_t t = atomicAdd(&a.counters[3], 1u);
-            s_task = t;
-            if (t < ntask) {   // its query: the last q with toff[q] <= t
-                uint64_t lo = 0, hi = a.nq;
-                while (hi - lo > 1) {
-                    const uint64_t mid = (lo + hi) >> 1;
-                    if (a.toff[mid] <= t) lo = mid;
-                    else hi = mid;
-                }
-                s_q = static_cast<uint32_t>(lo);
-            }
-        }
-        __syncthreads();
-        const uint32_t task = Uniform(s_task);
+        uint32_t task = 0;
+        if (lane == 0) task = atomicAdd(&a.counters[3], 1u);
+        task = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(task), 0)));
         if (task >= ntask) break;
-        const uint32_t q = Uniform(s_q);
-        if (q != qcur) {
-            for (uint32_t d = tid; d < P; d += 256) sq[d] = a.rotq[static_cast<uint64_t>(q) * P + d];
-            for (uint32_t e = tid; e < PC * 8; e += 256)
-                reinterpret_cast<uint32_t*>(slut)[e] = reinterpret_cast<const uint32_t*>(a.lut + static_cast<uint64_t>(q) * PC * 32)[e];
-            qcur = q;
+        uint32_t q = 0;
+        if (lane == 0) {   // the task's query: the last q with toff[q] <= task
+            uint64_t lo = 0, hi = a.nq;
+            while (hi - lo > 1) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (a.toff[mid] <= task) lo = mid;
+                else hi = mid;
+            }
+            q = static_cast<uint32_t>(lo);
         }
-        __syncthreads();
-        const uint32_t nc = a.ncand[q];
+        q = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(q), 0)));
+        if (q != qcur) {
+            __syncthreads();   // the previous query's data is no longer read
+            for (uint32_t d = lane; d < P; d += 64) sq[d] = a.rotq[static_cast<uint64_t>(q) * P + d];
+            for (uint32_t e = lane; e < PC * 8; e += 64)
+                reinterpret_cast<uint32_t*>(slut)[e] =
+                    reinterpret_cast<const uint32_t*>(a.lut + static_cast<uint64_t>(q) * PC * 32)[e];
+            qcur = q;
+            __syncthreads();
+        }
+        const uint32_t nc = a.nfill[q];
         const uint32_t c0 = (task - a.toff[q]) * kM8RefineChunk;
         const uint32_t c1 = min(nc, c0 + kM8RefineChunk);
-        const float tau = a.first_d[static_cast<uint64_t>(q) * k + k - 1];
+        uint64_t* slots = a.pool + a.coff[q] + c0;   // this task's candidates, then its survivors
+        const bool has = c0 + lane < c1;
+        const uint64_t ent = has ? slots[lane] : 0ull;
+        __syncthreads();   // every candidate is read before survivors are written over them
+        const float tau = a.tau[q];
         const double taud = tau < finf ? static_cast<double>(tau) : dinf;
         const float sqk = tau < finf ? sqrtf(fmaxf(0.0F, tau)) : finf;
         const double sqdk = taud < dinf ? sqrt(fmax(taud, 0.0)) : dinf;
         const float lut_delta = a.qc[static_cast<uint64_t>(q) * 4 + 1];
         const double* rcq = a.rc + static_cast<uint64_t>(q) * R * 4;
-        for (uint32_t ci = c0 + g; ci < c0 + 4 * 16 && ci - g < c1; ci += 16) {
-            const bool has = ci < c1;
-            const uint64_t e = has ? a.cand[static_cast<uint64_t>(q) * a.cap + ci] : 0ull;
-            const uint32_t p = static_cast<uint32_t>(e >> 32), pos = static_cast<uint32_t>(e);
-            const uint32_t c = has ? a.probe[static_cast<uint64_t>(q) * maxp + p] : 0u;
+        bool live = false;
+        float dsurv = 0.0F;
+        uint32_t idsurv = 0;
+        if (has) {
+            const uint32_t p = static_cast<uint32_t>(ent >> 32), pos = static_cast<uint32_t>(ent);
+            const uint32_t c = a.probe[static_cast<uint64_t>(q) * maxp + p];
             const uint64_t lb = v.list_begin[c], cnt = v.list_begin[c + 1] - lb;
             const uint32_t bj = pos >> 5, l = pos & 31u;
             const uint32_t cntb = static_cast<uint32_t>(min<uint64_t>(32, cnt - static_cast<uint64_t>(bj) * 32));
@@ -402,66 +567,60 @@ __global__ __launch_bounds__(256) void M8RefineKernel(M8BatchArgs a) {
             const uint8_t* rec = v.plane + (v.batch_begin[c] + bj) * v.rec;
             const uint8_t* prow = rec + static_cast<uint64_t>(l) * PC;
             const uint8_t* banks = rec + 32ull * PC;
-            // integer coarse sums of every run: lane j sums 8-column chunks j, j + 16, ...
+            const double d2 = static_cast<double>(a.pdist[static_cast<uint64_t>(q) * maxp + p]);
+            const double* sqd2 = a.sqd2 + (static_cast<uint64_t>(q) * maxp + p) * R;
+            // integer coarse sums of each run (AccumulateCoarseHi over its columns)
             int32_t part[kM8BatchRuns] = {0, 0, 0, 0};
-            if (has) {
-                for (uint32_t ch = j; ch < PC / 8; ch += 16) {
-                    const uint2 w = *reinterpret_cast<const uint2*>(prow + ch * 8);
-                    int32_t s = 0;
+            for (uint32_t r = 0; r < R; ++r) {
+                int32_t s = 0;
+                const uint32_t b0 = v.runs[r].blk0, b1 = b0 + v.runs[r].nblk;
+                for (uint32_t col = b0; col < b1; col += 8) {
+                    const uint64_t w = *reinterpret_cast<const uint64_t*>(prow + col);
 #pragma unroll
                     for (int b = 0; b < 8; ++b) {
-                        const uint32_t byte = ((b < 4 ? w.x : w.y) >> (8 * (b & 3))) & 0xFFu;
-                        const uint8_t* lc = slut + (ch * 8 + b) * 32;
+                        const uint32_t byte = static_cast<uint32_t>(w >> (8 * b)) & 0xFFu;
+                        const uint8_t* lc = slut + (col + b) * 32;
                         s += lc[byte & 15u] + lc[16 + (byte >> 4)];
                     }
-#pragma unroll
-                    for (uint32_t r = 0; r < kM8BatchRuns; ++r)
-                        if (r < R && ch * 8 >= v.runs[r].blk0 && ch * 8 < v.runs[r].blk0 + v.runs[r].nblk) part[r] += s;
                 }
+                part[r] = s;
             }
-#pragma unroll
-            for (uint32_t r = 0; r < kM8BatchRuns; ++r)
-                for (int o = 8; o > 0; o >>= 1) part[r] += __shfl_xor(part[r], o, 16);
             // the coarse terms in the reference's arithmetic, against tau
             double ssc[kM8BatchRuns], str[kM8BatchRuns], pool = 0.0, qadd = 0.0;
-            const double d2 = has ? static_cast<double>(a.pdist[static_cast<uint64_t>(q) * maxp + p]) : 0.0;
-            const double* sqd2 = a.sqd2 + (static_cast<uint64_t>(q) * maxp + p) * R;
-            if (has) {
-                if (l < covered) {   // the AVX2 filter's fp32 values
-                    float pl = 0.0F, qa = 0.0F, wpad = 0.0F;
-                    for (uint32_t r = 0; r < R; ++r) {
-                        const uint8_t* bp = banks + r * kM8BankBytes;
-                        const float* sc = reinterpret_cast<const float*>(bp + 128);
-                        const float s_c = fmaf(static_cast<float>(part[r]), lut_delta, static_cast<float>(rcq[r * 4 + 1]));
-                        const float wq = fmaf(static_cast<float>(bp[32 + l]), sc[3], sc[2]);
-                        const float pq = fmaf(static_cast<float>(bp[64 + l]), sc[5], sc[4]);
-                        const float ra = static_cast<float>(sqd2[r]);
-                        const float rh = sqk < ra ? sqk : ra;
-                        const float t = pq * rh;
-                        ssc[r] = s_c;
-                        str[r] = t;
-                        pl = fmaf(t, t, pl);
-                        qa = fmaf(fabsf(wq), static_cast<float>(rcq[r * 4 + 2]), qa);
-                        wpad = wpad + static_cast<float>(0.5 * static_cast<double>(sc[3]) * (rcq[r * 4 + 3] + rcq[r * 4 + 2]));
-                    }
-                    pool = pl;
-                    qadd = qa + wpad;
-                } else {   // the fp64 twin for the lanes the filter does not cover
-                    for (uint32_t r = 0; r < R; ++r) {
-                        const uint8_t* bp = banks + r * kM8BankBytes;
-                        const float* scl = reinterpret_cast<const float*>(bp + 128);
-                        const double scr = fma(static_cast<double>(lut_delta), static_cast<double>(part[r]), rcq[r * 4 + 1]);
-                        const double wq = fma(static_cast<double>(bp[32 + l]), static_cast<double>(scl[3]), static_cast<double>(scl[2]));
-                        const double pq = fma(static_cast<double>(bp[64 + l]), static_cast<double>(scl[5]), static_cast<double>(scl[4]));
-                        const double sa = sqd2[r];
-                        const double rh = sqdk < sa ? sqdk : sa;
-                        const double tr = pq * rh;
-                        ssc[r] = scr;
-                        str[r] = tr;
-                        pool = fma(tr, tr, pool);
-                        qadd = fma(fabs(wq), rcq[r * 4 + 2], qadd);
-                        qadd = fma(0.5 * static_cast<double>(scl[3]), rcq[r * 4 + 3] + rcq[r * 4 + 2], qadd);
-                    }
+            if (l < covered) {   // the AVX2 filter's fp32 values
+                float pl = 0.0F, qa = 0.0F, wpad = 0.0F;
+                for (uint32_t r = 0; r < R; ++r) {
+                    const uint8_t* bp = banks + r * kM8BankBytes;
+                    const float* sc = reinterpret_cast<const float*>(bp + 128);
+                    const float s_c = fmaf(static_cast<float>(part[r]), lut_delta, static_cast<float>(rcq[r * 4 + 1]));
+                    const float wq = fmaf(static_cast<float>(bp[32 + l]), sc[3], sc[2]);
+                    const float pq = fmaf(static_cast<float>(bp[64 + l]), sc[5], sc[4]);
+                    const float ra = static_cast<float>(sqd2[r]);
+                    const float rh = sqk < ra ? sqk : ra;
+                    const float t = pq * rh;
+                    ssc[r] = s_c;
+                    str[r] = t;
+                    pl = fmaf(t, t, pl);
+                    qa = fmaf(fabsf(wq), static_cast<float>(rcq[r * 4 + 2]), qa);
+                    wpad = wpad + static_cast<float>(0.5 * static_cast<double>(sc[3]) * (rcq[r * 4 + 3] + rcq[r * 4 + 2]));
+                }
+                pool = pl;
+                qadd = qa + wpad;
+            } else {   // the fp64 twin for the lanes the filter does not cover
+                for (uint32_t r = 0; r < R; ++r) {
+                    const uint8_t* bp = banks + r * kM8BankBytes;
+                    const float* scl = reinterpret_cast<const float*>(bp + 128);
+                    const double scr = fma(static_cast<double>(lut_delta), static_cast<double>(part[r]), rcq[r * 4 + 1]);
+                    const double wq = fma(static_cast<double>(bp[32 + l]), static_cast<double>(scl[3]), static_cast<double>(scl[2]));
+                    const double pq = fma(static_cast<double>(bp[64 + l]), static_cast<double>(scl[5]), static_cast<double>(scl[4]));
+                    const double sa = sqd2[r];
+                    const double rh = sqdk < sa ? sqdk : sa;
+                    const double tr = pq * rh;
+                    ssc[r] = scr;
+                    str[r] = tr;
+                    pool = fma(tr, tr, pool);
+                    qadd = fma(fabs(wq), rcq[r * 4 + 2], qadd);
+                    qadd = fma(0.5 * static_cast<double>(scl[3]), rcq[r * 4 + 3] + rcq[r * 4 + 2], qadd);
                 }
             }
             // the cascade's run order for a group of one: decreasing t^2 / nblk, stable
@@ -480,37 +639,32 @@ __global__ __launch_bounds__(256) void M8RefineKernel(M8BatchArgs a) {
                     ord[m] = x;
                 }
             }
-            const float* f4 = reinterpret_cast<const float*>(v.refine + (lb + pos) * v.rstride);
-            double dist = 0.0;
-            if (has) {
-                if (R == 1) {
-                    dist = d2 + static_cast<double>(f4[1]);
-                } else {
-                    double dcq = d2;
-                    for (uint32_t r = 0; r < R; ++r) {
-                        const double coef = f4[r * 4], h2 = f4[r * 4 + 1];
-                        const double gn = f4[r * 4 + 2], vp = f4[r * 4 + 3];
-                        dcq += fma(-(2.0 * coef), fma(gn, ssc[r], vp), h2);
-                    }
-                    dist = dcq;
+            const uint8_t* row = v.refine + (lb + pos) * v.rstride;
+            const float* f4 = reinterpret_cast<const float*>(row);
+            double dist;
+            if (R == 1) {
+                dist = d2 + static_cast<double>(f4[1]);
+            } else {
+                double dcq = d2;
+                for (uint32_t r = 0; r < R; ++r) {
+                    const double coef = f4[r * 4], h2 = f4[r * 4 + 1];
+                    const double gn = f4[r * 4 + 2], vp = f4[r * 4 + 3];
+                    dcq += fma(-(2.0 * coef), fma(gn, ssc[r], vp), h2);
                 }
+                dist = dcq;
             }
             double rem2 = pool;
-            bool live = has;
+            live = true;
             uint64_t bits = 0;
-            const uint32_t half = j >> 3, cmp = j & 7u;
-            for (uint32_t oi = 0; oi < R; ++oi) {
-                if (!live) break;
+            uint32_t lo_off[kM8BatchRuns] = {0, 0, 0, 0};
+            for (uint32_t r = 1; r < R; ++r)
+                lo_off[r] = lo_off[r - 1] + (v.runs[r - 1].rate <= kM8LdsLevRate ? 2u << v.runs[r - 1].rate : 0u);
+            const uint32_t nsteps = kFirst ? 1u : R;
+            for (uint32_t oi = 0; oi < nsteps && live; ++oi) {
                 const uint32_t r = ord[oi];
                 const M8Run& ru = v.runs[r];
-                uint32_t lo_off = 0;
-                for (uint32_t r2 = 0; r2 < r; ++r2)
-                    if (v.runs[r2].rate <= kM8LdsLevRate) lo_off += 2u << v.runs[r2].rate;
-                const float* lev = ru.rate <= kM8LdsLevRate ? slev + lo_off : ru.lev;
-                const uint8_t* fr = v.refine + (lb + pos) * v.rstride + ru.ref_off;
-                const float acc = M8HalfChain(ru.rate, prow + ru.blk0, fr, lev, sq + ru.dim_off + cmp, ru.nblk, half, cmp);
-                const float other = __shfl_xor(acc, 8, 16);
-                const double se = Hsum8Lanes(acc + other);
+                const float* lev = ru.rate <= kM8LdsLevRate ? slev + lo_off[r] : ru.lev;
+                const double se = M8LaneScore(ru.rate, prow + ru.blk0, row + ru.ref_off, lev, sq + ru.dim_off, ru.nblk);
                 const double coef = f4[r * 4];
                 if (R == 1) {
                     dist = fma(-(2.0 * coef), se, dist);
@@ -520,25 +674,41 @@ __global__ __launch_bounds__(256) void M8RefineKernel(M8BatchArgs a) {
                 }
                 const double t = str[r];
                 rem2 = fma(-t, t, rem2);
-                bits += static_cast<uint64_t>(ru.rate) * ru.len;
+                if (kFirst || oi > 0) bits += static_cast<uint64_t>(ru.rate) * ru.len;
                 if (a.cascade && fma(-dz, sqrt(fmax(rem2, 0.0)), dist) - qadd >= taud) live = false;
             }
-            if (has && j == 0) {
-                ++n_ref;
-                n_bits += bits;
-                const float d = static_cast<float>(dist);
-                if (live && (d < tau || !(tau < finf))) {
-                    ++n_surv;
-                    const uint32_t slot = atomicAdd(&a.nsurv[q], 1u);
-                    if (slot < a.scap)
-                        a.surv[static_cast<uint64_t>(q) * a.scap + slot] =
-                            (static_cast<uint64_t>(M8Key(d)) << 32) | v.ids[lb + pos];
-                }
+            if (kFirst) ++n_ref;
+            n_bits += bits;
+            if (!kFirst || R == 1) {
+                dsurv = static_cast<float>(dist);
+                live = live && (dsurv < tau || !(tau < finf));
+                if (live) idsurv = v.ids[lb + pos];
             }
         }
-        __syncthreads();
+        if (kFirst && R > 1) {   // the cascade continues in the second phase
+            const uint64_t m = __ballot(live);
+            uint32_t base = 0;
+            if (lane == 0 && m != 0) base = atomicAdd(&a.ncont[q], static_cast<uint32_t>(__popcll(m)));
+            base = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(base), 0)));
+            if (live) a.cont[a.coff[q] + base + __popcll(m & ((uint64_t{1} << lane) - 1))] = ent;
+            if (lane == 0) a.tsurv[task] = 0u;
+            continue;
+        }
+        // survivors, compacted over the task's first slots
+        const uint64_t m = __ballot(live);
+        if (live) {
+            slots[__popcll(m & ((uint64_t{1} << lane) - 1))] = (static_cast<uint64_t>(M8Key(dsurv)) << 32) | idsurv;
+            ++n_surv;
+        }
+        if (lane == 0) a.tsurv[task] = static_cast<uint32_t>(__popcll(m));
     }
-    if (j == 0 && a.stats && (n_ref | n_surv)) {
+    // per-wave totals
+    for (int o = 32; o > 0; o >>= 1) {
+        n_ref += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n_ref), o));
+        n_surv += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n_surv), o));
+        n_bits += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n_bits), o));
+    }
+    if (lane == 0 && a.stats && (n_ref | n_surv)) {
         atomicAdd(&a.stats[2], n_ref);
         atomicAdd(&a.stats[3], n_surv);
         atomicAdd(&a.stats[4], n_bits);
@@ -549,21 +719,24 @@ size_t M8RefineLds(const M8View& v, uint32_t lev_floats) {
     return M8RefLayout(static_cast<uint32_t>(v.padded), v.plane_cols, lev_floats).total;
 }
 
-hipError_t LaunchM8Refine(hipStream_t st, const M8BatchArgs& a, uint32_t grid) {
+hipError_t LaunchM8Refine(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool first) {
     const size_t lds = M8RefineLds(a.v, a.lev_floats);
+    const void* fn = first ? reinterpret_cast<const void*>(&M8RefineKernel<true>)
+                           : reinterpret_cast<const void*>(&M8RefineKernel<false>);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&M8RefineKernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(M8RefineKernel, dim3(grid), dim3(256), lds, st, a);
+    if (first) hipLaunchKernelGGL(M8RefineKernel<true>, dim3(grid), dim3(64), lds, st, a);
+    else hipLaunchKernelGGL(M8RefineKernel<false>, dim3(grid), dim3(64), lds, st, a);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Top-k of each query: its threshold-pass list and its survivors, sorted by
-// (distance, id) with a bitonic sort in LDS of the best k so far plus the
-// next chunk.  One wave per query.
+// Top-k of each query: its threshold-pass list and its survivors (at the
+// start of each of its refine tasks' pool slots), sorted by (distance, id)
+// with a bitonic sort in LDS of the best k so far plus the next chunk.  One
+// wave per query.
 
 constexpr uint32_t kM8SelChunk = 1024;
 
@@ -572,25 +745,37 @@ __global__ __launch_bounds__(64) void M8SelectKernel(M8BatchArgs a) {
     const uint32_t lane = threadIdx.x, k = a.k;
     const float finf = __builtin_inff();
     for (uint64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
-        const uint32_t nc = Uniform(a.ncand[q]), ns = Uniform(a.nsurv[q]);
-        if (nc > a.cap || ns > a.scap) {   // searched again by the plain kernel
-            if (lane == 0) a.overflow[atomicAdd(&a.counters[2], 1u)] = static_cast<uint32_t>(q);
-            continue;
-        }
-        const uint64_t* sl = a.surv + q * a.scap;
-        // the threshold-pass list first (it is already sorted), then the survivors
+        const uint32_t t0 = Uniform(a.toff[q]), t1 = Uniform(a.toff[q + 1]);
+        uint32_t ns = 0;
+        for (uint32_t t = t0; t < t1; ++t) ns += a.tsurv[t];
+        ns = Uniform(ns);
+        const uint64_t* slots = a.pool + a.coff[q];
         const uint32_t n = k + ns;
         uint32_t ch = 64;
         while (ch < kM8SelChunk && ch < n) ch <<= 1;
         const uint32_t room = ch - k;
-        for (uint32_t i = lane; i < k; i += 64) {
+        for (uint32_t i = lane; i < k; i += 64) {   // the threshold-pass list
             const float d = a.first_d[q * k + i];
             buf[i] = d < finf ? (static_cast<uint64_t>(M8Key(d)) << 32) | a.first_ids[q * k + i] : ~0ull;
         }
-        uint32_t s = 0;
+        // survivors task by task: task t holds tsurv[t] of them at slots (t - t0) * chunk
+        uint32_t t = t0, done = 0, s = 0;
         do {
-            const uint32_t take = min(room, ns - s);
-            for (uint32_t i = lane; i < room; i += 64) buf[k + i] = i < take ? sl[s + i] : ~0ull;
+            uint32_t fill = 0;
+            while (fill < room && t < t1) {
+                const uint32_t nt = Uniform(a.tsurv[t]);
+                const uint32_t take = min(room - fill, nt - done);
+                const uint64_t* src = slots + static_cast<uint64_t>(t - t0) * kM8RefineChunk + done;
+                for (uint32_t i = lane; i < take; i += 64) buf[k + fill + i] = src[i];
+                fill += take;
+                done += take;
+                if (done == nt) {
+                    ++t;
+                    done = 0;
+                }
+            }
+            for (uint32_t i = fill + lane; i < room; i += 64) buf[k + i] = ~0ull;
+            s += fill;
             __syncthreads();
             for (uint32_t size = 2; size <= ch; size <<= 1)
                 for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
@@ -606,7 +791,6 @@ __global__ __launch_bounds__(64) void M8SelectKernel(M8BatchArgs a) {
                     }
                     __syncthreads();
                 }
-            s += take;
         } while (s < ns);
         for (uint32_t i = lane; i < k; i += 64) {
             const uint64_t key = buf[i];

@@ -79,6 +79,7 @@ constexpr uint32_t kM8BatchQTile = 32;   // pairs per work item (the MFMA's M)
 constexpr uint32_t kM8BatchBTile = 8;    // batches per work item (4 waves x 2 tiles)
 constexpr uint32_t kM8RefineChunk = 64;  // candidates per refine task
 constexpr uint32_t kM8SelMaxK = 512;     // top-k of the selection kernel
+constexpr uint32_t kM8TauBins = 64;      // histogram of coarse upper bounds over [0, tau0)
 
 struct M8BatchArgs {
     M8View v;
@@ -99,25 +100,38 @@ struct M8BatchArgs {
     const BatchItem* items;
     const uint64_t* pairs;
     uint64_t items_cap;
-    uint32_t* counters;       // [0] items, [1] next item, [2] overflowed queries, [3] next refine task
-    uint64_t* cand;           // [nq][cap] (probe position << 32) | vector position in its list
-    uint32_t* ncand;          // [nq]
-    uint32_t cap;
+    uint32_t* counters;       // [0] items, [1] next item, [3] next refine task
+    uint32_t* ncand;          // [nq] candidates at tau0 (count pass)
+    uint32_t* nfill;          // [nq] candidates written (write pass)
+    float* tau;               // [nq] threshold of the write pass and the refine (<= tau0)
+    uint32_t* hist;           // [nq][kM8TauBins] count pass: coarse upper bounds below tau0, by bin
+    uint64_t* coff;           // [nq+1] each query's first pool slot
     uint32_t* toff;           // [nq+1] refine task offsets
-    uint64_t* surv;           // [nq][scap] (monotone distance key << 32) | id
-    uint32_t* nsurv;          // [nq]
-    uint32_t scap;
-    uint32_t* overflow;       // [nq]
+    uint64_t* pool;           // [coff[nq]] (probe position << 32) | vector position in its list; the
+                              // refine overwrites each task's slots with its survivors, (key << 32) | id
+    uint32_t* tsurv;          // [toff[nq]] survivors of each refine task
+    uint64_t* cont;           // first refine phase: candidates alive after their first run, at coff[q]
+    uint32_t* ncont;          // [nq] their count
     uint32_t* out_ids;        // [nq][k]
     float* out_d;
     unsigned long long* stats;   // [8] pairs scored, candidates, refined, survivors, bits scored, items, -, -
     uint32_t* chk;            // optional [nq]: sum over scored pairs of (pos+1) * sum_r (r+1) * coarse_sum_r, mod 2^32
     uint32_t lev_floats;
 };
-hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid);
-hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a);
+// write = 0: count each query's candidates (ncand); write = 1: append them to
+// the pool at coff[q] (same arithmetic, so the same vectors pass).
+hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool write);
+// Pool offsets coff from ncand (with_offsets) and refine tasks toff from
+// `count`; coff[nq] = pool entries, toff[nq] = refine tasks.
+hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a, const uint32_t* count, bool with_offsets);
+// tau[q] from the count pass's histogram: the upper edge of the bin where
+// the coarse upper bounds reach k, if below tau0.
+hipError_t LaunchM8Tau(hipStream_t st, const M8BatchArgs& a);
 size_t M8RefineLds(const M8View& v, uint32_t lev_floats);
-hipError_t LaunchM8Refine(hipStream_t st, const M8BatchArgs& a, uint32_t grid);
+// first = true (indexes of two or more runs): each candidate of a.pool gets
+// the first run of its cascade; those still alive go to a.cont.  first =
+// false: the whole cascade, survivors written over the task's slots.
+hipError_t LaunchM8Refine(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool first);
 hipError_t LaunchM8Select(hipStream_t st, const M8BatchArgs& a);
 
 }  // namespace e8qdev

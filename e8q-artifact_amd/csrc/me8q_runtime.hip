@@ -82,16 +82,15 @@ struct M8Bufs {
     uint32_t *probe, *npq, *oid;
     uint64_t* qst;
     // batched form
-    uint32_t *fid, *ccount, *cfill, *coff, *ioff, *counters, *ncand, *toff, *nsurv, *ovf, *chk;
-    float* fd;
-    uint64_t *cover, *pairs, *cand, *surv;
+    uint32_t *fid, *ccount, *cfill, *coff, *ioff, *counters, *ncand, *nfill, *ncont, *toff, *toff2, *chk, *hist;
+    float *fd, *tau;
+    uint64_t *cover, *pairs, *poff;
     BatchItem* items;
     unsigned long long *wstats, *bstats;
     BatchedParams* wparams;
 };
 
-constexpr uint32_t kM8CandCap = 32768;   // candidates per query of the batched form (more: plain form)
-constexpr uint32_t kM8SurvCap = 4096;    // exact distances below tau per query (more: plain form)
+constexpr uint64_t kM8PoolMax = uint64_t{1} << 31;   // candidate pool entries (more: the plain form)
 constexpr uint32_t kM8DefaultBudget = 16;
 
 // Whether the batched form runs for this index and k.
@@ -131,11 +130,13 @@ void M8CarveBatched(e8q_ctx::Carve& cv, M8Bufs& b, const e8q_m8index* x, uint64_
     cv.add(&b.items, std::max<uint64_t>(M8ItemsCap(x, nq, pairs), 1));
     cv.add(&b.counters, 4);
     cv.add(&b.ncand, nq);
-    cv.add(&b.cand, nq * kM8CandCap);
+    cv.add(&b.nfill, nq);
+    cv.add(&b.tau, nq);
+    cv.add(&b.hist, nq * kM8TauBins);
+    cv.add(&b.ncont, nq);
+    cv.add(&b.poff, nq + 1);
     cv.add(&b.toff, nq + 1);
-    cv.add(&b.nsurv, nq);
-    cv.add(&b.surv, nq * kM8SurvCap);
-    cv.add(&b.ovf, nq);
+    cv.add(&b.toff2, nq + 1);
     cv.add(&b.wstats, 4);
     cv.add(&b.bstats, 8);
     cv.add(&b.wparams, 1);
@@ -268,7 +269,6 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         // 2. the (query, list) work list: every probe, grouped by list
         Check(LaunchFill32(st, b.ccount, v.nlist, 0), "clear");
         Check(LaunchFill32(st, b.ncand, nq, 0), "clear");
-        Check(LaunchFill32(st, b.nsurv, nq, 0), "clear");
         Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.wstats), 8, 0), "clear");
         Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.bstats), 16, 0), "clear");
         if (chk) Check(LaunchFill32(st, b.chk, nq, 0), "clear");
@@ -295,7 +295,11 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         w.stats = b.wstats;
         w.p_lo = 0;
         Check(LaunchBatchedWorkList(st, lv, w, b.wparams, true), "work list");
-        // 3. the MFMA contraction and the coarse bound
+        // 3. the MFMA contraction and the coarse bound: a count pass, the pool
+        // offsets, a write pass
+        Check(LaunchFill32(st, b.nfill, nq, 0), "clear");
+        Check(LaunchFill32(st, b.hist, nq * kM8TauBins, 0), "clear");
+        Check(LaunchFill32(st, b.ncont, nq, 0), "clear");
         M8BatchArgs ba{};
         ba.v = v;
         ba.lut = lut;
@@ -317,42 +321,69 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         ba.pairs = b.pairs;
         ba.items_cap = w.items_cap;
         ba.counters = b.counters;
-        ba.cand = b.cand;
         ba.ncand = b.ncand;
-        ba.cap = kM8CandCap;
+        ba.nfill = b.nfill;
+        ba.tau = b.tau;
+        ba.hist = b.hist;
+        ba.coff = b.poff;
         ba.toff = b.toff;
-        ba.surv = b.surv;
-        ba.nsurv = b.nsurv;
-        ba.scap = kM8SurvCap;
-        ba.overflow = b.ovf;
         ba.out_ids = out_ids;
         ba.out_d = out_d;
         ba.stats = b.bstats;
         ba.chk = chk ? b.chk : nullptr;
         ba.lev_floats = sa.lev_floats;
         const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
-        Check(hipEventRecord(ctx->ev[1], st), "event");
-        Check(LaunchM8MfmaScan(st, ba, 2 * cus), "mfma scan");
-        Check(hipEventRecord(ctx->ev[2], st), "event");
-        // 4. exact cascade of the candidates, 5. top-k
-        Check(LaunchM8RefineTasks(st, ba), "refine tasks");
-        Check(LaunchM8Refine(st, ba, 4 * cus), "refine");
-        Check(LaunchM8Select(st, ba), "select");
-        uint32_t cnt[4] = {0, 0, 0, 0};
+        Check(LaunchM8MfmaScan(st, ba, 2 * cus, false), "mfma count pass");
+        Check(LaunchM8Tau(st, ba), "thresholds");
+        Check(LaunchM8RefineTasks(st, ba, b.ncand, true), "pool offsets");
+        uint64_t total = 0;
+        uint32_t ntask = 0, cnt[4] = {0, 0, 0, 0};
+        Check(hipMemcpyAsync(&total, b.poff + nq, 8, hipMemcpyDeviceToHost, st), "download pool size");
+        Check(hipMemcpyAsync(&ntask, b.toff + nq, 4, hipMemcpyDeviceToHost, st), "download task count");
         Check(hipMemcpyAsync(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost, st), "download counters");
-        Check(hipMemcpyAsync(bst.data(), b.bstats, 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
-        Check(hipMemcpyAsync(wst.data(), b.wstats, 4 * 8, hipMemcpyDeviceToHost, st), "download stats");
-        Check(hipStreamSynchronize(st), "batched search");
+        Check(hipStreamSynchronize(st), "batched count pass");
         if (cnt[0] > w.items_cap) Throw(E8Q_ERUNTIME, "batched search: work items exceed their bound");
-        novf = cnt[2];
-        if (novf > nq) Throw(E8Q_ERUNTIME, "batched search: overflow count exceeds the query count");
-        if (novf > 0) {   // those queries: the plain form
-            M8ScanArgs oa = sa;
-            oa.qsel = b.ovf;
-            oa.nsel = novf;
-            Check(LaunchM8Scan(st, oa), "plain search of overflowed queries");
+        if (total > kM8PoolMax) {   // too many candidates for one pool: the plain form
+            novf = nq;
+            Check(hipEventRecord(ctx->ev[1], st), "event");
+            Check(LaunchM8Scan(st, sa), "plain search");
+            Check(hipEventRecord(ctx->ev[2], st), "event");
+        } else {
+            // pool: the candidates, the candidates alive after their first run,
+            // then each refine phase's per-task survivor counts
+            const uint64_t pool_b = (total * 8 + 255) & ~uint64_t{255};
+            const uint64_t task_b = (static_cast<uint64_t>(ntask) * 4 + 255) & ~uint64_t{255};
+            const uint64_t need = 2 * pool_b + 2 * task_b + 256;
+            if (ctx->pool.bytes < need) ctx->pool.alloc(need + need / 4);
+            char* pb = ctx->pool.as<char>();
+            ba.pool = reinterpret_cast<uint64_t*>(pb);
+            ba.cont = reinterpret_cast<uint64_t*>(pb + pool_b);
+            ba.ncont = b.ncont;
+            ba.tsurv = reinterpret_cast<uint32_t*>(pb + 2 * pool_b);
+            Check(LaunchFill32(st, b.counters + 1, 1, 0), "clear");
+            Check(hipEventRecord(ctx->ev[1], st), "event");
+            Check(LaunchM8MfmaScan(st, ba, 2 * cus, true), "mfma write pass");
+            Check(hipEventRecord(ctx->ev[2], st), "event");
+            Check(LaunchM8RefineTasks(st, ba, b.nfill, false), "refine tasks");
+            // 4. exact cascade of the candidates: every candidate's first run,
+            // then the whole cascade of those still alive, 5. top-k
+            Check(LaunchM8Refine(st, ba, 24 * cus, true), "refine, first runs");
+            if (v.nrun > 1) {
+                M8BatchArgs b2 = ba;
+                b2.pool = ba.cont;
+                b2.nfill = b.ncont;
+                b2.toff = b.toff2;
+                b2.tsurv = reinterpret_cast<uint32_t*>(pb + 2 * pool_b + task_b);
+                Check(LaunchM8RefineTasks(st, b2, b.ncont, false), "refine tasks");
+                Check(LaunchM8Refine(st, b2, 24 * cus, false), "refine, cascades");
+                Check(LaunchM8Select(st, b2), "select");
+            } else {
+                Check(LaunchM8Select(st, ba), "select");
+            }
         }
         Check(hipEventRecord(ctx->ev[3], st), "event");
+        Check(hipMemcpyAsync(bst.data(), b.bstats, 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
+        Check(hipMemcpyAsync(wst.data(), b.wstats, 4 * 8, hipMemcpyDeviceToHost, st), "download stats");
         if (chk) Check(hipMemcpyAsync(chk, b.chk, nq * 4, hipMemcpyDeviceToHost, st), "download checksums");
     }
     if (!a->outputs_on_device) {
@@ -362,8 +393,9 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     std::vector<uint64_t> hs(nq * 8);
     Check(hipMemcpyAsync(hs.data(), qst, nq * 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
     Check(hipStreamSynchronize(st), "search");
+    const bool ran_batched = batched && novf == 0;   // else the plain form answered every query
     if (per_query_stats) {
-        if (batched) std::fill(per_query_stats, per_query_stats + nq * 5, 0ull);
+        if (ran_batched) std::fill(per_query_stats, per_query_stats + nq * 5, 0ull);
         else
             for (uint64_t i = 0; i < nq; ++i) std::memcpy(per_query_stats + i * 5, hs.data() + i * 8, 5 * 8);
     }
@@ -371,7 +403,8 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         uint64_t s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // threshold pass (batched) or the whole search (plain)
         for (uint64_t i = 0; i < nq; ++i)
             for (int f = 0; f < 8; ++f) s8[f] += hs[i * 8 + f];
-        if (!batched) {
+        if (!ran_batched) {
+            stats->overflowed = novf;
             stats->candidates = s8[0];
             stats->survivors = s8[1];
             stats->bits_scored = s8[2];

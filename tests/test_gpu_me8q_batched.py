@@ -169,7 +169,7 @@ def test_pruned_distances_exact(ctx, case):
     assert rec >= 0.97
 
 
-def test_uneven_nprobe_budget_and_overflow(ctx):
+def test_uneven_nprobe_and_budget(ctx):
     sizes = np.array([3000, 5, 0, 700, 1, 64, 33, 2500, 31, 900])
     ix, f, q = _index(ctx, 77, 0, 256, 10, 4.0, "64:6,128:4,64:2", sizes)
     npq = np.random.default_rng(5).choice(np.array([1, 2, 3, 7, 10], np.uint32), size=q.shape[0])
@@ -183,18 +183,21 @@ def test_uneven_nprobe_budget_and_overflow(ctx):
         assert int(np.max(_ulps(bd, pd))) <= 1
 
 
-def test_candidate_overflow_falls_back_to_plain(ctx):
-    # 40000 vectors in 4 lists, all probed, nothing pruned: every query has
-    # more candidates than its list holds (32768) and is searched again by
-    # the plain form
+def test_every_vector_a_candidate(ctx):
+    # 40000 vectors in 4 lists, all probed, nothing pruned, and a threshold
+    # pass shorter than k (tau = +inf): every vector is a candidate and a
+    # survivor of its query
     ix, f, q = _index(ctx, 13, 40000, 128, 4, 4.0, "128:4")
     q = q[:8]
-    print("me8q batched overflow", flush=True)
+    print("me8q batched every vector", flush=True)
     pi, pd, _, _ = ix.search(q, 50, 4, INF)
     bi, bd, _, bst = ix.search(q, 50, 4, INF, mode=BATCHED, budget=1)
-    assert bst.overflowed == q.shape[0]
+    assert bst.form == BATCHED and bst.overflowed == 0
+    assert bst.refined >= q.shape[0] * (40000 - 32)
+    pi, pd = _sorted_rows(pi, pd)
+    bi, bd = _sorted_rows(bi, bd)
     np.testing.assert_array_equal(bi, pi)
-    np.testing.assert_array_equal(bd.view(np.uint32), pd.view(np.uint32))
+    assert int(np.max(_ulps(bd, pd))) <= 1
 
 
 def test_unsupported_shapes_run_plain(ctx):
