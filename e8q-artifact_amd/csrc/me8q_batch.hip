@@ -83,6 +83,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     __shared__ float s_ra[kM8BatchRuns][kM8BatchQTile], s_wf[kM8BatchRuns][kM8BatchQTile];
     __shared__ __align__(16) uint32_t s_oh[16 * 4];
     __shared__ uint32_t s_hist[kChk || kWrite ? 1 : kM8BatchQTile * kM8TauBins];   // count pass
+    __shared__ uint32_t s_abin[kM8BatchQTile], s_na[kM8BatchQTile];                   // write pass
     // The query operand of eight plane columns for the item's 32 pairs, two
     // buffers: [buffer][column][pair][32 bytes + pad], already int8.
     __shared__ __align__(16) uint8_t s_a[2 * 8 * kM8BatchQTile * kM8ARow];
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
         if (tid < kM8BatchQTile) {   // rows past np carry the first pair (computed, never appended)
             const uint64_t pr = G(a.pairs)[p0 + (tid < np ? tid : 0u)];
             const uint32_t q = static_cast<uint32_t>(pr >> 32), p = static_cast<uint32_t>(pr);
-            const float tau = kWrite ? G(a.tau)[q] : G(a.first_d)[static_cast<uint64_t>(q) * k + k - 1];
+            const float tau = G(a.first_d)[static_cast<uint64_t>(q) * k + k - 1];   // tau0
             const uint64_t cv = G(a.cover)[q];
             s_q[tid] = q;
             s_p[tid] = p;
@@ -113,6 +114,10 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             s_bs[tid] = static_cast<uint32_t>(cv);
             s_tau[tid] = tau;
             s_sqk[tid] = tau < finf ? sqrtf(fmaxf(0.0F, tau)) : finf;
+            if (kWrite) {
+                s_abin[tid] = G(a.abin)[q];
+                s_na[tid] = G(a.na)[q];
+            }
             s_d2[tid] = G(a.pdist)[static_cast<uint64_t>(q) * maxp + p];
             s_dl[tid] = G(a.qc)[static_cast<uint64_t>(q) * 4 + 1];
             for (uint32_t r = 0; r < R; ++r) {
@@ -239,30 +244,35 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                     const bool fresh = p > s_ps[row] || (p == s_ps[row] && bj >= s_bs[row]);
                     const float slack = fmaf(z, sqrtf(pool[t][i]), qw[t][i]);
                     const float tau = s_tau[row];
-                    const float margin = 2.4414062e-4F * (fabsf(dc[t][i]) + fabsf(slack) + fabsf(tau));
-                    const bool kept = scored && fresh && !(dc[t][i] - slack >= tau + margin);
-                    if (!kChk && !kWrite && scored && fresh && tau < finf) {
-                        // the coarse upper bound (the reference's bound read the
-                        // other way) into the query's histogram over [0, tau0)
-                        const float ub = dc[t][i] + slack;
-                        if (ub < tau) {
-                            const uint32_t bin = min(kM8TauBins - 1,
-                                                     static_cast<uint32_t>(static_cast<float>(kM8TauBins) * fmaxf(ub, 0.0F) / tau));
-                            atomicAdd(&s_hist[row * kM8TauBins + bin], 1u);
-                        }
-                    }
+                    const float dm = dc[t][i] - slack - 2.4414062e-4F * (fabsf(dc[t][i]) + fabsf(slack));
+                    const bool kept = scored && fresh && !(dm >= tau + 2.4414062e-4F * fabsf(tau));
+                    // the candidate's bin by its coarse estimate (set A: the lowest bins)
+                    const uint32_t bin = tau < finf ? min(kM8TauBins - 1, static_cast<uint32_t>(
+                                                              static_cast<float>(kM8TauBins) * fmaxf(dc[t][i], 0.0F) / tau))
+                                                    : 0u;
+                    if (!kChk && !kWrite && kept) atomicAdd(&s_hist[row * kM8TauBins + bin], 1u);
                     const uint64_t m = __ballot(kept);
                     const uint32_t hm = static_cast<uint32_t>(m >> (32 * h));
                     if (!kWrite) {
                         if (n == 0 && hm != 0) atomicAdd(&a.ncand[q], static_cast<uint32_t>(__popc(hm)));
                     } else {
-                        uint32_t base = 0;
-                        if (n == 0 && hm != 0) base = atomicAdd(&a.nfill[q], static_cast<uint32_t>(__popc(hm)));
-                        base = static_cast<uint32_t>(__shfl(static_cast<int>(base), static_cast<int>(lane & 32u)));
+                        const bool in_a = kept && bin <= s_abin[row];
+                        const uint32_t ma = static_cast<uint32_t>(__ballot(in_a) >> (32 * h));
+                        const uint32_t mb = hm & ~ma;
+                        uint32_t ba = 0, bb = 0;
+                        if (n == 0 && ma != 0) ba = atomicAdd(&a.fill_a[q], static_cast<uint32_t>(__popc(ma)));
+                        if (n == 0 && mb != 0) bb = atomicAdd(&a.fill_b[q], static_cast<uint32_t>(__popc(mb)));
+                        ba = static_cast<uint32_t>(__shfl(static_cast<int>(ba), static_cast<int>(lane & 32u)));
+                        bb = static_cast<uint32_t>(__shfl(static_cast<int>(bb), static_cast<int>(lane & 32u)));
                         if (kept) {
-                            const uint32_t slot = base + __popc(hm & ((1u << n) - 1u));
-                            if (slot < G(a.ncand)[q])   // the count pass saw the same vectors pass
-                                Gw(a.pool)[G(a.coff)[q] + slot] = (static_cast<uint64_t>(p) << 32) | pos;
+                            const uint32_t na = s_na[row];
+                            const uint32_t below = (1u << n) - 1u;
+                            const uint32_t slot = in_a ? ba + __popc(ma & below) : na + bb + __popc(mb & below);
+                            if (in_a ? slot < na : slot < G(a.ncand)[q]) {   // the count pass saw the same vectors pass
+                                const uint64_t at = G(a.coff)[q] + slot;
+                                Gw(a.pool)[at] = (static_cast<uint64_t>(p) << 32) | pos;
+                                Gw(a.pdlo)[at] = dm;
+                            }
                         }
                     }
                     if (lane == 0) n_cand += __popcll(m);
@@ -352,38 +362,70 @@ hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a, const uint3
     return hipGetLastError();
 }
 
-// tau[q]: tau0, lowered to the upper edge of the first histogram bin at
-// which the count of coarse upper bounds below it (and of the threshold
-// pass's own exact distances) reaches k.
-__global__ __launch_bounds__(256) void M8TauKernel(M8BatchArgs a) {
+// Set A of each query: the candidates in the lowest histogram bins (by
+// coarse estimate) that hold at least kM8SetA of them, all candidates when
+// there are fewer.  tau = tau0 for A's refine.
+__global__ __launch_bounds__(256) void M8SplitKernel(M8BatchArgs a) {
     const uint64_t q = blockIdx.x * 256ull + threadIdx.x;
     if (q >= a.nq) return;
-    const uint32_t k = a.k;
-    const float tau0 = a.first_d[q * k + k - 1];
-    float tau = tau0;
-    if (tau0 < __builtin_inff() && tau0 > 0.0F) {
-        uint32_t h[kM8TauBins];
-        for (uint32_t b = 0; b < kM8TauBins; ++b) h[b] = a.hist[q * kM8TauBins + b];
-        for (uint32_t i = 0; i < k; ++i) {
-            const float d = a.first_d[q * k + i];
-            if (d < tau0)
-                ++h[min(kM8TauBins - 1, static_cast<uint32_t>(static_cast<float>(kM8TauBins) * fmaxf(d, 0.0F) / tau0))];
-        }
-        uint32_t c = 0;
-        for (uint32_t b = 0; b < kM8TauBins; ++b) {
-            c += h[b];
-            if (c >= k) {
-                tau = fminf(tau0, static_cast<float>(b + 1) * (tau0 / static_cast<float>(kM8TauBins)));
-                break;
-            }
-        }
+    uint32_t c = 0, b = 0;
+    for (; b < kM8TauBins; ++b) {
+        c += a.hist[q * kM8TauBins + b];
+        if (c >= kM8SetA) break;
     }
-    a.tau[q] = tau;
+    if (b == kM8TauBins) {   // fewer than kM8SetA candidates (or none binned): all in A
+        b = kM8TauBins - 1;
+        c = a.ncand[q];
+    }
+    a.abin[q] = b;
+    a.na[q] = c;
+    a.tau[q] = a.first_d[q * a.k + a.k - 1];
 }
 
-hipError_t LaunchM8Tau(hipStream_t st, const M8BatchArgs& a) {
+hipError_t LaunchM8Split(hipStream_t st, const M8BatchArgs& a) {
     if (a.nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(M8TauKernel, dim3(static_cast<uint32_t>((a.nq + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(M8SplitKernel, dim3(static_cast<uint32_t>((a.nq + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+// tau = min(tau0, the k-th exact distance after A), then each query's B
+// compacted in place (one wave per query, in order, so a write never passes
+// a read) to the candidates whose lower bound at tau0 is below tau: the
+// bound only rises as the threshold falls, so no vector the bound keeps at
+// tau is dropped.
+__global__ __launch_bounds__(64) void M8CompactKernel(M8BatchArgs a) {
+    const uint32_t lane = threadIdx.x;
+    for (uint64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
+        const float tau0 = a.tau[q], t1 = a.first_d[q * a.k + a.k - 1];
+        const float tau = t1 < tau0 ? t1 : tau0;
+        const float lim = tau + 2.4414062e-4F * fabsf(tau);
+        const uint64_t base = a.coff[q] + a.na[q];
+        const uint32_t nb = Uniform(a.ncand[q] - a.na[q]);
+        uint32_t w = 0;
+        for (uint32_t r0 = 0; r0 < nb; r0 += 64) {
+            const uint32_t r = r0 + lane;
+            uint64_t e = 0;
+            bool keep = false;
+            if (r < nb) {
+                e = a.pool[base + r];
+                keep = !(a.pdlo[base + r] >= lim);
+            }
+            const uint64_t m = __ballot(keep);
+            if (keep) a.pool[base + w + __popcll(m & ((uint64_t{1} << lane) - 1))] = e;
+            w += static_cast<uint32_t>(__popcll(m));
+        }
+        if (lane == 0) {
+            a.tau[q] = tau;
+            a.coff_b[q] = base;
+            a.nfill[q] = w;
+        }
+    }
+}
+
+hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a) {
+    if (a.nq == 0) return hipSuccess;
+    const uint32_t grid = static_cast<uint32_t>(a.nq < 65536 ? a.nq : 65536);
+    hipLaunchKernelGGL(M8CompactKernel, dim3(grid), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

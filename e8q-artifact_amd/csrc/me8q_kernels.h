@@ -79,7 +79,8 @@ constexpr uint32_t kM8BatchQTile = 32;   // pairs per work item (the MFMA's M)
 constexpr uint32_t kM8BatchBTile = 8;    // batches per work item (4 waves x 2 tiles)
 constexpr uint32_t kM8RefineChunk = 64;  // candidates per refine task
 constexpr uint32_t kM8SelMaxK = 512;     // top-k of the selection kernel
-constexpr uint32_t kM8TauBins = 64;      // histogram of coarse upper bounds over [0, tau0)
+constexpr uint32_t kM8TauBins = 64;      // histogram of candidates' coarse estimates over [0, tau0)
+constexpr uint32_t kM8SetA = 4096;       // candidates per query refined first (their exact top-k gives tau1)
 
 struct M8BatchArgs {
     M8View v;
@@ -102,9 +103,15 @@ struct M8BatchArgs {
     uint64_t items_cap;
     uint32_t* counters;       // [0] items, [1] next item, [3] next refine task
     uint32_t* ncand;          // [nq] candidates at tau0 (count pass)
-    uint32_t* nfill;          // [nq] candidates written (write pass)
-    float* tau;               // [nq] threshold of the write pass and the refine (<= tau0)
-    uint32_t* hist;           // [nq][kM8TauBins] count pass: coarse upper bounds below tau0, by bin
+    uint32_t* nfill;          // [nq] candidates of the refine launch (its count array)
+    float* tau;               // [nq] threshold of the refine (<= tau0)
+    uint32_t* hist;           // [nq][kM8TauBins] count pass: candidates by coarse estimate, bins over [0, tau0)
+    uint32_t* abin;           // [nq] last histogram bin of the query's set A
+    uint32_t* na;             // [nq] |A|: candidates with the best coarse estimates, refined first
+    uint32_t* fill_a;         // [nq] write cursors of A and of the rest (B)
+    uint32_t* fill_b;
+    float* pdlo;              // [pool] each candidate's coarse lower bound at tau0 (less a rounding margin)
+    uint64_t* coff_b;         // [nq] first slot of B; after the compaction, B's survivors of the bound at tau1
     uint64_t* coff;           // [nq+1] each query's first pool slot
     uint32_t* toff;           // [nq+1] refine task offsets
     uint64_t* pool;           // [coff[nq]] (probe position << 32) | vector position in its list; the
@@ -124,9 +131,13 @@ hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid,
 // Pool offsets coff from ncand (with_offsets) and refine tasks toff from
 // `count`; coff[nq] = pool entries, toff[nq] = refine tasks.
 hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a, const uint32_t* count, bool with_offsets);
-// tau[q] from the count pass's histogram: the upper edge of the bin where
-// the coarse upper bounds reach k, if below tau0.
-hipError_t LaunchM8Tau(hipStream_t st, const M8BatchArgs& a);
+// Per query from the count pass's histogram: set A = the candidates in the
+// lowest bins holding at least kM8SetA of them (abin, na); tau = tau0.
+hipError_t LaunchM8Split(hipStream_t st, const M8BatchArgs& a);
+// After A's exact top-k is in first_ids / first_d: tau = min(tau0, its k-th
+// distance), and each query's B (at coff_b) compacted in place to the
+// candidates whose lower bound is below tau (count in nfill).
+hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a);
 size_t M8RefineLds(const M8View& v, uint32_t lev_floats);
 // first = true (indexes of two or more runs): each candidate of a.pool gets
 // the first run of its cascade; those still alive go to a.cont.  first =

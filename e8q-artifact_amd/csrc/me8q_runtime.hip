@@ -83,6 +83,8 @@ struct M8Bufs {
     uint64_t* qst;
     // batched form
     uint32_t *fid, *ccount, *cfill, *coff, *ioff, *counters, *ncand, *nfill, *ncont, *toff, *toff2, *chk, *hist;
+    uint32_t *abin, *na, *fill_a, *fill_b;
+    uint64_t* coff_b;
     float *fd, *tau;
     uint64_t *cover, *pairs, *poff;
     BatchItem* items;
@@ -134,6 +136,11 @@ void M8CarveBatched(e8q_ctx::Carve& cv, M8Bufs& b, const e8q_m8index* x, uint64_
     cv.add(&b.tau, nq);
     cv.add(&b.hist, nq * kM8TauBins);
     cv.add(&b.ncont, nq);
+    cv.add(&b.abin, nq);
+    cv.add(&b.na, nq);
+    cv.add(&b.fill_a, nq);
+    cv.add(&b.fill_b, nq);
+    cv.add(&b.coff_b, nq);
     cv.add(&b.poff, nq + 1);
     cv.add(&b.toff, nq + 1);
     cv.add(&b.toff2, nq + 1);
@@ -299,7 +306,8 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         // offsets, a write pass
         Check(LaunchFill32(st, b.nfill, nq, 0), "clear");
         Check(LaunchFill32(st, b.hist, nq * kM8TauBins, 0), "clear");
-        Check(LaunchFill32(st, b.ncont, nq, 0), "clear");
+        Check(LaunchFill32(st, b.fill_a, nq, 0), "clear");
+        Check(LaunchFill32(st, b.fill_b, nq, 0), "clear");
         M8BatchArgs ba{};
         ba.v = v;
         ba.lut = lut;
@@ -325,6 +333,11 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         ba.nfill = b.nfill;
         ba.tau = b.tau;
         ba.hist = b.hist;
+        ba.abin = b.abin;
+        ba.na = b.na;
+        ba.fill_a = b.fill_a;
+        ba.fill_b = b.fill_b;
+        ba.coff_b = b.coff_b;
         ba.coff = b.poff;
         ba.toff = b.toff;
         ba.out_ids = out_ids;
@@ -334,7 +347,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         ba.lev_floats = sa.lev_floats;
         const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
         Check(LaunchM8MfmaScan(st, ba, 2 * cus, false), "mfma count pass");
-        Check(LaunchM8Tau(st, ba), "thresholds");
+        Check(LaunchM8Split(st, ba), "candidate split");
         Check(LaunchM8RefineTasks(st, ba, b.ncand, true), "pool offsets");
         uint64_t total = 0;
         uint32_t ntask = 0, cnt[4] = {0, 0, 0, 0};
@@ -349,37 +362,54 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             Check(LaunchM8Scan(st, sa), "plain search");
             Check(hipEventRecord(ctx->ev[2], st), "event");
         } else {
-            // pool: the candidates, the candidates alive after their first run,
-            // then each refine phase's per-task survivor counts
+            // pool: the candidates (A then B per query), their lower bounds,
+            // the candidates alive after their first run, then the per-task
+            // survivor counts of the four refine launches
             const uint64_t pool_b = (total * 8 + 255) & ~uint64_t{255};
+            const uint64_t dlo_b = (total * 4 + 255) & ~uint64_t{255};
             const uint64_t task_b = (static_cast<uint64_t>(ntask) * 4 + 255) & ~uint64_t{255};
-            const uint64_t need = 2 * pool_b + 2 * task_b + 256;
+            const uint64_t need = 2 * pool_b + dlo_b + 4 * task_b + 256;
             if (ctx->pool.bytes < need) ctx->pool.alloc(need + need / 4);
             char* pb = ctx->pool.as<char>();
             ba.pool = reinterpret_cast<uint64_t*>(pb);
             ba.cont = reinterpret_cast<uint64_t*>(pb + pool_b);
+            ba.pdlo = reinterpret_cast<float*>(pb + 2 * pool_b);
             ba.ncont = b.ncont;
-            ba.tsurv = reinterpret_cast<uint32_t*>(pb + 2 * pool_b);
+            uint32_t* ts = reinterpret_cast<uint32_t*>(pb + 2 * pool_b + dlo_b);
+            const uint64_t tsn = task_b / 4;
             Check(LaunchFill32(st, b.counters + 1, 1, 0), "clear");
             Check(hipEventRecord(ctx->ev[1], st), "event");
             Check(LaunchM8MfmaScan(st, ba, 2 * cus, true), "mfma write pass");
             Check(hipEventRecord(ctx->ev[2], st), "event");
-            Check(LaunchM8RefineTasks(st, ba, b.nfill, false), "refine tasks");
-            // 4. exact cascade of the candidates: every candidate's first run,
-            // then the whole cascade of those still alive, 5. top-k
-            Check(LaunchM8Refine(st, ba, 24 * cus, true), "refine, first runs");
-            if (v.nrun > 1) {
-                M8BatchArgs b2 = ba;
-                b2.pool = ba.cont;
-                b2.nfill = b.ncont;
-                b2.toff = b.toff2;
-                b2.tsurv = reinterpret_cast<uint32_t*>(pb + 2 * pool_b + task_b);
-                Check(LaunchM8RefineTasks(st, b2, b.ncont, false), "refine tasks");
-                Check(LaunchM8Refine(st, b2, 24 * cus, false), "refine, cascades");
-                Check(LaunchM8Select(st, b2), "select");
-            } else {
-                Check(LaunchM8Select(st, ba), "select");
-            }
+            // 4. exact cascades (every candidate's first run, then the whole
+            // cascade of those still alive) and a top-k merge, first of each
+            // query's set A (its exact top-k gives tau1), then of the rest
+            // whose lower bound is below tau1
+            auto refine = [&](M8BatchArgs r, uint32_t* tsurv1, uint32_t* tsurv2, uint32_t* sel_ids, float* sel_d) {
+                r.tsurv = tsurv1;
+                r.toff = b.toff;
+                Check(LaunchFill32(st, b.ncont, nq, 0), "clear");
+                Check(LaunchM8RefineTasks(st, r, r.nfill, false), "refine tasks");
+                Check(LaunchM8Refine(st, r, 24 * cus, true), "refine, first runs");
+                if (v.nrun > 1) {
+                    r.pool = r.cont;
+                    r.nfill = b.ncont;
+                    r.toff = b.toff2;
+                    r.tsurv = tsurv2;
+                    Check(LaunchM8RefineTasks(st, r, b.ncont, false), "refine tasks");
+                    Check(LaunchM8Refine(st, r, 24 * cus, false), "refine, cascades");
+                }
+                r.out_ids = sel_ids;
+                r.out_d = sel_d;
+                Check(LaunchM8Select(st, r), "select");
+            };
+            M8BatchArgs ra = ba;   // set A
+            ra.nfill = b.na;
+            refine(ra, ts, ts + tsn, b.fid, b.fd);
+            Check(LaunchM8Compact(st, ba), "compaction");
+            M8BatchArgs rb = ba;   // the rest
+            rb.coff = b.coff_b;
+            refine(rb, ts + 2 * tsn, ts + 3 * tsn, out_ids, out_d);
         }
         Check(hipEventRecord(ctx->ev[3], st), "event");
         Check(hipMemcpyAsync(bst.data(), b.bstats, 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
