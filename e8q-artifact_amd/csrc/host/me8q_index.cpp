@@ -478,6 +478,12 @@ uint32_t BatchedBudget() {
     const char* e = std::getenv("E8Q_M8_BATCH_BUDGET");
     return e ? static_cast<uint32_t>(std::strtoul(e, nullptr, 10)) : 0u;
 }
+// Candidates per query the batched form refines first (E8Q_M8_SET_A; 0 =
+// the library default).
+uint32_t BatchedSetA() {
+    const char* e = std::getenv("E8Q_M8_SET_A");
+    return e ? static_cast<uint32_t>(std::strtoul(e, nullptr, 10)) : 0u;
+}
 }  // namespace
 
 void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t nprobe, const uint32_t* nprobe_per_query,
@@ -498,6 +504,7 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
     a.outputs_on_device = outputs_on_device ? 1 : 0;
     a.mode = mode;
     a.batched_budget = BatchedBudget();
+    a.batched_set_a = BatchedSetA();
     e8q_m8_stats st{};
     {
         std::lock_guard<std::mutex> lk(DeviceMutex());
@@ -537,6 +544,7 @@ void Me8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, s
     a.out_dists = out_dists;
     a.mode = mode;
     a.batched_budget = BatchedBudget();
+    a.batched_set_a = BatchedSetA();
     Check(e8q_m8_search(ctx, dev_, &a, nullptr, nullptr));
 }
 

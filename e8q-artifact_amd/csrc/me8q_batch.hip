@@ -78,9 +78,10 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     const M8View& v = a.v;
     __shared__ uint32_t s_it;
     __shared__ uint32_t s_q[kM8BatchQTile], s_p[kM8BatchQTile], s_ps[kM8BatchQTile], s_bs[kM8BatchQTile];
-    __shared__ float s_tau[kM8BatchQTile], s_sqk[kM8BatchQTile], s_d2[kM8BatchQTile], s_dl[kM8BatchQTile];
-    __shared__ float s_lmx[kM8BatchRuns][kM8BatchQTile], s_qe[kM8BatchRuns][kM8BatchQTile];
-    __shared__ float s_ra[kM8BatchRuns][kM8BatchQTile], s_wf[kM8BatchRuns][kM8BatchQTile];
+    __shared__ float s_tau[kM8BatchQTile], s_d2[kM8BatchQTile], s_dl[kM8BatchQTile];
+    // per (run, pair): {lut_min_x + 256 nblk delta (the int8 shift folded in), qerr,
+    // (score_max + qerr) / 2, min(sqrt(tau0), sqrt(d2_r))^2}
+    __shared__ float4 s_rv[kM8BatchRuns][kM8BatchQTile];
     __shared__ __align__(16) uint32_t s_oh[16 * 4];
     __shared__ uint32_t s_hist[kChk || kWrite ? 1 : kM8BatchQTile * kM8TauBins];   // count pass
     __shared__ uint32_t s_abin[kM8BatchQTile], s_na[kM8BatchQTile];                   // write pass
@@ -113,19 +114,20 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             s_ps[tid] = static_cast<uint32_t>(cv >> 32);
             s_bs[tid] = static_cast<uint32_t>(cv);
             s_tau[tid] = tau;
-            s_sqk[tid] = tau < finf ? sqrtf(fmaxf(0.0F, tau)) : finf;
             if (kWrite) {
                 s_abin[tid] = G(a.abin)[q];
                 s_na[tid] = G(a.na)[q];
             }
             s_d2[tid] = G(a.pdist)[static_cast<uint64_t>(q) * maxp + p];
-            s_dl[tid] = G(a.qc)[static_cast<uint64_t>(q) * 4 + 1];
+            const float dl = G(a.qc)[static_cast<uint64_t>(q) * 4 + 1];
+            const float sk = tau < finf ? sqrtf(fmaxf(0.0F, tau)) : finf;
+            s_dl[tid] = dl;
             for (uint32_t r = 0; r < R; ++r) {
                 const E8Q_GLOBAL double* o = G(a.rc) + (static_cast<uint64_t>(q) * R + r) * 4;
-                s_lmx[r][tid] = static_cast<float>(o[1]);
-                s_qe[r][tid] = static_cast<float>(o[2]);
-                s_wf[r][tid] = static_cast<float>(0.5 * (o[3] + o[2]));
-                s_ra[r][tid] = static_cast<float>(G(a.sqd2)[(static_cast<uint64_t>(q) * maxp + p) * R + r]);
+                const float ra = static_cast<float>(G(a.sqd2)[(static_cast<uint64_t>(q) * maxp + p) * R + r]);
+                const float rh = sk < ra ? sk : ra;
+                s_rv[r][tid] = make_float4(fmaf(256.0F * static_cast<float>(v.runs[r].nblk), dl, static_cast<float>(o[1])),
+                                           static_cast<float>(o[2]), static_cast<float>(0.5 * (o[3] + o[2])), rh * rh);
             }
         }
         if (!kChk && !kWrite)
@@ -208,18 +210,18 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                         const float bq = fmaf(static_cast<float>(bp[n]), sc[1], sc[0]);
                         const float wq = fmaf(static_cast<float>(bp[32 + n]), sc[3], sc[2]);
                         const float pq = fmaf(static_cast<float>(bp[64 + n]), sc[5], sc[4]);
-                        const float aw = fabsf(wq), s3 = sc[3];
+                        const float aw = fabsf(wq), s3 = sc[3], pq2 = pq * pq;
 #pragma unroll
                         for (int i = 0; i < 16; ++i) {
+                            // the filter's terms, regrouped (the test carries a rounding margin):
+                            // s_c from the shifted sum, (pq rh)^2 as pq^2 rh^2
                             const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
-                            const int32_t iv = acc[t][i] + shift;
-                            const float s_c = fmaf(static_cast<float>(iv), s_dl[row], s_lmx[r][row]);
+                            const float4 rv = s_rv[r][row];
+                            const float s_c = fmaf(static_cast<float>(acc[t][i]), s_dl[row], rv.x);
                             dc[t][i] = fmaf(-wq, s_c, dc[t][i]) + bq;
-                            const float ra = s_ra[r][row], sk = s_sqk[row];
-                            const float tt = pq * (sk < ra ? sk : ra);
-                            pool[t][i] = fmaf(tt, tt, pool[t][i]);
-                            qw[t][i] = fmaf(aw, s_qe[r][row], qw[t][i]) + s3 * s_wf[r][row];
-                            if (kChk) ck[t][i] += (r + 1) * static_cast<uint32_t>(iv);
+                            pool[t][i] = fmaf(pq2, rv.w, pool[t][i]);
+                            qw[t][i] = fmaf(s3, rv.z, fmaf(aw, rv.y, qw[t][i]));
+                            if (kChk) ck[t][i] += (r + 1) * static_cast<uint32_t>(acc[t][i] + shift);
                         }
                         acc[t] = i32x16{};
                     }
@@ -363,7 +365,7 @@ hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a, const uint3
 }
 
 // Set A of each query: the candidates in the lowest histogram bins (by
-// coarse estimate) that hold at least kM8SetA of them, all candidates when
+// coarse estimate) that hold at least set_a of them, all candidates when
 // there are fewer.  tau = tau0 for A's refine.
 __global__ __launch_bounds__(256) void M8SplitKernel(M8BatchArgs a) {
     const uint64_t q = blockIdx.x * 256ull + threadIdx.x;
@@ -371,9 +373,9 @@ __global__ __launch_bounds__(256) void M8SplitKernel(M8BatchArgs a) {
     uint32_t c = 0, b = 0;
     for (; b < kM8TauBins; ++b) {
         c += a.hist[q * kM8TauBins + b];
-        if (c >= kM8SetA) break;
+        if (c >= a.set_a) break;
     }
-    if (b == kM8TauBins) {   // fewer than kM8SetA candidates (or none binned): all in A
+    if (b == kM8TauBins) {   // fewer than set_a candidates (or none binned): all in A
         b = kM8TauBins - 1;
         c = a.ncand[q];
     }

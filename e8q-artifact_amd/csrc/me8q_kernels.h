@@ -80,7 +80,7 @@ constexpr uint32_t kM8BatchBTile = 8;    // batches per work item (4 waves x 2 t
 constexpr uint32_t kM8RefineChunk = 64;  // candidates per refine task
 constexpr uint32_t kM8SelMaxK = 512;     // top-k of the selection kernel
 constexpr uint32_t kM8TauBins = 64;      // histogram of candidates' coarse estimates over [0, tau0)
-constexpr uint32_t kM8SetA = 4096;       // candidates per query refined first (their exact top-k gives tau1)
+constexpr uint32_t kM8SetA = 2048;       // candidates per query refined first (their exact top-k gives tau1)
 
 struct M8BatchArgs {
     M8View v;
@@ -124,6 +124,7 @@ struct M8BatchArgs {
     unsigned long long* stats;   // [8] pairs scored, candidates, refined, survivors, bits scored, items, -, -
     uint32_t* chk;            // optional [nq]: sum over scored pairs of (pos+1) * sum_r (r+1) * coarse_sum_r, mod 2^32
     uint32_t lev_floats;
+    uint32_t set_a;           // candidates per query of set A (kM8SetA)
 };
 // write = 0: count each query's candidates (ncand); write = 1: append them to
 // the pool at coff[q] (same arithmetic, so the same vectors pass).

@@ -345,6 +345,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         ba.stats = b.bstats;
         ba.chk = chk ? b.chk : nullptr;
         ba.lev_floats = sa.lev_floats;
+        ba.set_a = a->batched_set_a ? a->batched_set_a : kM8SetA;
         const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
         Check(LaunchM8MfmaScan(st, ba, 2 * cus, false), "mfma count pass");
         Check(LaunchM8Split(st, ba), "candidate split");

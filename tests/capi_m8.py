@@ -25,7 +25,8 @@ class M8Arrays(ctypes.Structure):
 class M8SearchArgs(ctypes.Structure):
     _fields_ = [("queries", vp), ("nq", u64), ("k", u64), ("nprobe", u64), ("nprobe_per_query", vp),
                 ("z", ctypes.c_float), ("queries_on_device", ctypes.c_int32), ("out_ids", vp), ("out_dists", vp),
-                ("outputs_on_device", ctypes.c_int32), ("mode", ctypes.c_int32), ("batched_budget", ctypes.c_uint32)]
+                ("outputs_on_device", ctypes.c_int32), ("mode", ctypes.c_int32), ("batched_budget", ctypes.c_uint32),
+                ("batched_set_a", ctypes.c_uint32)]
 
 
 class M8Stats(ctypes.Structure):
