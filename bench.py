@@ -296,7 +296,7 @@ def me8q_forms(idx, q_dev, nq, k, grid, modes, gt, args, plan, plain_budget_s=No
                    "scan_kernel_ms": sum(scan_ms) / len(scan_ms), "device_total_ms": sum(tot_ms) / len(tot_ms),
                    "batches": st["batches"], "survivors": st["survivors"], "candidates": st["candidates"],
                    "refined": st["refined"], "groups": st["groups"], "form": st["form"],
-                   "overflowed": st["overflowed"], "pairs_scored": st["pairs_scored"]}
+                   "overflowed": st["overflowed"], "pairs_scored": st["pairs_scored"], "items": st.get("items", 0)}
             if mode == "plain":
                 algo = row["batches"] * rec_bytes + row["survivors"] * (refine_stride + 4)
                 row["scan_algorithmic_bytes"] = algo
@@ -307,8 +307,10 @@ def me8q_forms(idx, q_dev, nq, k, grid, modes, gt, args, plan, plain_budget_s=No
                 ops = row["pairs_scored"] * 2 * 32 * plane_cols
                 row["mfma_int8_ops"] = ops
                 row["mfma_tops"] = ops / (row["scan_kernel_ms"] / 1000.0) / 1e12
+                fill = row["pairs_scored"] / max(1, row["items"] * 32 * 256)
+                row["item_fill"] = fill
                 extra = (f"MFMA scan {row['scan_kernel_ms']:.2f} ms = {row['mfma_tops']:.0f} TOP/s int8, "
-                         f"overflowed {row['overflowed']}")
+                         f"{row['items']} items ({100 * fill:.1f}% of their 32 x 256 pairs scored)")
             rows.append(row)
             log(f"me8q {mode} nprobe={nprobe}: {ms:.2f} ms/step, {row['qps']:.0f} QPS, recall 10@{k}="
                 f"{row['recall']:.4f} (10@10 {row['recall_10at10']:.4f}), {extra}, survivors "

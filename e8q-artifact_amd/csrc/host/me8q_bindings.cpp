@@ -173,6 +173,7 @@ class PyMe8qIndex {
         s["form"] = st.form == 1 ? "batched" : "plain";
         s["overflowed"] = st.overflowed;
         s["pairs_scored"] = st.pairs_scored;
+        s["items"] = st.items;
         return py::make_tuple(ids, dists, s);
     }
 
@@ -211,6 +212,7 @@ class PyMe8qIndex {
         s["form"] = st.form == 1 ? "batched" : "plain";
         s["overflowed"] = st.overflowed;
         s["pairs_scored"] = st.pairs_scored;
+        s["items"] = st.items;
         return s;
     }
 

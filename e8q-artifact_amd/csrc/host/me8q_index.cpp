@@ -525,6 +525,7 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
         stats->form = st.form;
         stats->overflowed += st.overflowed;
         stats->pairs_scored += st.pairs_scored;
+        stats->items += st.items;
     }
 }
 

@@ -81,6 +81,7 @@ struct SearchStats {
     int form = 0;         // the device form that ran (E8Q_MODE_PLAIN / E8Q_MODE_BATCHED)
     size_t overflowed = 0;     // batched form: queries searched again by the plain form
     size_t pairs_scored = 0;   // batched form: (query, vector) pairs of the MFMA contraction
+    size_t items = 0;          // batched form: its work items
 };
 
 class Me8qIndex {

@@ -71,7 +71,19 @@ __device__ __forceinline__ float M8KeyDist(uint32_t m) {
 // l % 32.
 
 constexpr int kM8Tiles = 2;
-constexpr uint32_t kM8ARow = 48;   // bytes per staged (column, pair) row: 32 + 16 pad
+constexpr uint32_t kM8StageBytes = 8 * 1024;   // one stage of the query operand: 8 columns x 2 x 32 x 16 B
+
+// s_waitcnt vmcnt(n) for a wave-uniform n <= 4: this wave's vector memory
+// operations, all but the n most recently issued, have completed.
+__device__ __forceinline__ void WaitVmcnt(uint32_t n) {
+    switch (n) {
+        case 0: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+        case 1: asm volatile("s_waitcnt vmcnt(1)" ::: "memory"); break;
+        case 2: asm volatile("s_waitcnt vmcnt(2)" ::: "memory"); break;
+        case 3: asm volatile("s_waitcnt vmcnt(3)" ::: "memory"); break;
+        default: asm volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+    }
+}
 
 template <bool kChk, bool kWrite>
 __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
@@ -85,9 +97,12 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     __shared__ __align__(16) uint32_t s_oh[16 * 4];
     __shared__ uint32_t s_hist[kChk || kWrite ? 1 : kM8BatchQTile * kM8TauBins];   // count pass
     __shared__ uint32_t s_abin[kM8BatchQTile], s_na[kM8BatchQTile];                   // write pass
-    // The query operand of eight plane columns for the item's 32 pairs, two
-    // buffers: [buffer][column][pair][32 bytes + pad], already int8.
-    __shared__ __align__(16) uint8_t s_a[2 * 8 * kM8BatchQTile * kM8ARow];
+    __shared__ uint64_t s_bal[4][kM8Tiles][16][2];       // per wave, tile, accumulator row: kept / set-A ballots
+    __shared__ uint32_t s_off[4][kM8Tiles][kM8BatchQTile][2];   // a tile's first append of each pair (A, B)
+    __shared__ uint32_t s_base[kM8BatchQTile][2];        // each pair's first slot of the item (A, B)
+    // The query operand, eight plane columns per stage, a ring of three
+    // stages: [stage][column][half][pair][16 bytes] of the int8 LUT.
+    __shared__ __align__(16) uint8_t s_a[3 * kM8StageBytes];
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t h = lane >> 5, n = lane & 31;
     if (tid < 16)
@@ -142,16 +157,18 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             const uint32_t bj = b0 + wave + 4 * (t < static_cast<int>(nt) ? t : 0);
             prow[t] = G(v.plane) + (bat0 + bj) * v.rec + static_cast<uint64_t>(n) * PC;
         }
-        // Staging: thread t carries pair t / 8, column t % 8 of each 8-column block.
-        const uint32_t mq = tid >> 3, eq = tid & 7;
-        const E8Q_GLOBAL u32x4* asrc =
-            reinterpret_cast<const E8Q_GLOBAL u32x4*>(G(a.lut) + static_cast<uint64_t>(s_q[mq]) * PC * 32 + eq * 32);
-        uint8_t* adst = s_a + (eq * kM8BatchQTile + mq) * kM8ARow;
-        const uint32_t kFlip = 0x80808080u;
-        u32x4 pf0 = asrc[0], pf1 = asrc[1];
-        *reinterpret_cast<u32x4*>(adst) = pf0 ^ kFlip;
-        *reinterpret_cast<u32x4*>(adst + 16) = pf1 ^ kFlip;
-        __syncthreads();
+        // The query operand goes straight to LDS (global_load_lds_dwordx4, 16
+        // bytes per lane, no registers), two stages ahead: wave w copies columns
+        // 2w and 2w + 1 of a stage, lane l pair l & 31, half l >> 5, so a lane
+        // (pair n, half h) of the MFMA reads its 16 bytes at (h * 32 + n) * 16 of
+        // a column (consecutive lanes, consecutive banks).
+        const E8Q_GLOBAL uint8_t* lsrc =
+            G(a.lut8) + static_cast<uint64_t>(s_q[lane & 31]) * PC * 32 + (lane >> 5) * 16;
+        auto stage = [&](uint32_t sb) {
+            uint8_t* dst = s_a + (sb % 3) * kM8StageBytes + (2 * wave) * 1024;
+            __builtin_amdgcn_global_load_lds(lsrc + (sb * 8 + 2 * wave) * 32, dst, 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(lsrc + (sb * 8 + 2 * wave + 1) * 32, dst + 1024, 16, 0, 0);
+        };
         i32x16 acc[kM8Tiles];
         float dc[kM8Tiles][16], pool[kM8Tiles][16], qw[kM8Tiles][16];
         uint32_t ck[kM8Tiles][16];
@@ -168,36 +185,42 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
         }
         uint32_t r = 0, rend = R > 0 ? Uniform(v.runs[0].blk0 + v.runs[0].nblk) : PC;
         const uint32_t ncb = PC / 8;
-        for (uint32_t cb = 0; cb < ncb; ++cb) {
-            const uint32_t cur = cb & 1u;
-            const bool more = cb + 1 < ncb;
-            if (more) {   // the next block's operand, in flight during this block's MFMAs
-                pf0 = asrc[(cb + 1) * 16];
-                pf1 = asrc[(cb + 1) * 16 + 1];
-            }
-            if (nt > 0) {
-                uint64_t w[kM8Tiles];
+        // prologue: stages 0 and 1, then this wave's plane words of block 0
+        stage(0);
+        if (ncb > 1) stage(1);
+        uint64_t wcur[kM8Tiles] = {0ull, 0ull}, wnext[kM8Tiles] = {0ull, 0ull};
 #pragma unroll
-                for (int t = 0; t < kM8Tiles; ++t) w[t] = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(prow[t] + cb * 8);
-                const uint8_t* arow = s_a + cur * 8 * kM8BatchQTile * kM8ARow + n * kM8ARow + 16 * h;
+        for (int t = 0; t < kM8Tiles; ++t)
+            if (t < static_cast<int>(nt)) wcur[t] = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(prow[t]);
+        for (uint32_t cb = 0; cb < ncb; ++cb) {
+            // Issued after stage cb's copies: this wave's plane words of block cb
+            // (nt loads) and stage cb + 1's copies (2, when there is one); all
+            // older operations are done once no more than those are in flight.
+            WaitVmcnt(cb + 1 < ncb ? nt + 2 : 0);
+            __syncthreads();   // every wave's copies of stage cb are in LDS
+            if (nt > 0 && cb + 1 < ncb) {
+#pragma unroll
+                for (int t = 0; t < kM8Tiles; ++t)
+                    if (t < static_cast<int>(nt))
+                        wnext[t] = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(prow[t] + (cb + 1) * 8);
+            }
+            if (cb + 2 < ncb) stage(cb + 2);   // its ring slot was last read before this barrier
+            if (nt > 0) {
+                const uint8_t* arow = s_a + (cb % 3) * kM8StageBytes + (h * 32 + n) * 16;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const i32x4 av = *reinterpret_cast<const i32x4*>(arow + e * kM8BatchQTile * kM8ARow);
+                    const i32x4 av = *reinterpret_cast<const i32x4*>(arow + e * 1024);
 #pragma unroll
                     for (int t = 0; t < kM8Tiles; ++t) {
                         if (t >= static_cast<int>(nt)) break;
-                        const uint32_t nib = static_cast<uint32_t>(w[t] >> (8 * e + 4 * h)) & 15u;
+                        const uint32_t nib = static_cast<uint32_t>(wcur[t] >> (8 * e + 4 * h)) & 15u;
                         const i32x4 bv = *reinterpret_cast<const i32x4*>(s_oh + nib * 4);
                         acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[t], 0, 0, 0);
                     }
                 }
+#pragma unroll
+                for (int t = 0; t < kM8Tiles; ++t) wcur[t] = wnext[t];
             }
-            if (more) {   // the other buffer was last read before the previous barrier
-                uint8_t* d = adst + (cur ^ 1u) * 8 * kM8BatchQTile * kM8ARow;
-                *reinterpret_cast<u32x4*>(d) = pf0 ^ kFlip;
-                *reinterpret_cast<u32x4*>(d + 16) = pf1 ^ kFlip;
-            }
-            __syncthreads();
             if ((cb + 1) * 8 == rend) {   // run r ends here: its terms of the coarse bound
                 if (nt > 0) {
                     const int32_t shift = 256 * static_cast<int32_t>(Uniform(v.runs[r].nblk));
@@ -230,63 +253,109 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                 rend = r < R ? Uniform(v.runs[r].blk0 + v.runs[r].nblk) : PC;
             }
         }
-        if (nt > 0) {   // the bound test and the appends
-            const float z = a.z;
+        // The bound test of every (pair, vector) of the wave's tiles, ballots
+        // to LDS; then one thread per pair adds up its appends over the item's
+        // tiles (one atomic per query and item), and the write pass places each
+        // vector at its query's base + the tile's offset + its rank in the ballot.
+        const float z = a.z;
+        auto test = [&](int t, int i, float& dm, uint32_t& bin) -> bool {
+            const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
+            const uint32_t bj = b0 + wave + 4 * t;
+            const uint64_t pos = static_cast<uint64_t>(bj) * 32 + n;
+            const uint32_t p = s_p[row];
+            const bool scored = pos < cnt && row < np;
+            const bool fresh = p > s_ps[row] || (p == s_ps[row] && bj >= s_bs[row]);
+            const float slack = fmaf(z, sqrtf(pool[t][i]), qw[t][i]);
+            const float tau = s_tau[row];
+            dm = dc[t][i] - slack - 2.4414062e-4F * (fabsf(dc[t][i]) + fabsf(slack));
+            bin = tau < finf ? min(kM8TauBins - 1,
+                                   static_cast<uint32_t>(static_cast<float>(kM8TauBins) * fmaxf(dc[t][i], 0.0F) / tau))
+                             : 0u;
+            return scored && fresh && !(dm >= tau + 2.4414062e-4F * fabsf(tau));
+        };
+        if (nt > 0) {
 #pragma unroll
             for (int t = 0; t < kM8Tiles; ++t) {
                 if (t >= static_cast<int>(nt)) break;
                 const uint32_t bj = b0 + wave + 4 * t;
                 const uint64_t pos = static_cast<uint64_t>(bj) * 32 + n;
-                const bool vv = pos < cnt;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
-                    const uint32_t q = s_q[row], p = s_p[row];
-                    const bool scored = vv && row < np;
-                    const bool fresh = p > s_ps[row] || (p == s_ps[row] && bj >= s_bs[row]);
-                    const float slack = fmaf(z, sqrtf(pool[t][i]), qw[t][i]);
-                    const float tau = s_tau[row];
-                    const float dm = dc[t][i] - slack - 2.4414062e-4F * (fabsf(dc[t][i]) + fabsf(slack));
-                    const bool kept = scored && fresh && !(dm >= tau + 2.4414062e-4F * fabsf(tau));
-                    // the candidate's bin by its coarse estimate (set A: the lowest bins)
-                    const uint32_t bin = tau < finf ? min(kM8TauBins - 1, static_cast<uint32_t>(
-                                                              static_cast<float>(kM8TauBins) * fmaxf(dc[t][i], 0.0F) / tau))
-                                                    : 0u;
+                    float dm;
+                    uint32_t bin;
+                    const bool kept = test(t, i, dm, bin);
                     if (!kChk && !kWrite && kept) atomicAdd(&s_hist[row * kM8TauBins + bin], 1u);
-                    const uint64_t m = __ballot(kept);
-                    const uint32_t hm = static_cast<uint32_t>(m >> (32 * h));
-                    if (!kWrite) {
-                        if (n == 0 && hm != 0) atomicAdd(&a.ncand[q], static_cast<uint32_t>(__popc(hm)));
-                    } else {
-                        const bool in_a = kept && bin <= s_abin[row];
-                        const uint32_t ma = static_cast<uint32_t>(__ballot(in_a) >> (32 * h));
-                        const uint32_t mb = hm & ~ma;
-                        uint32_t ba = 0, bb = 0;
-                        if (n == 0 && ma != 0) ba = atomicAdd(&a.fill_a[q], static_cast<uint32_t>(__popc(ma)));
-                        if (n == 0 && mb != 0) bb = atomicAdd(&a.fill_b[q], static_cast<uint32_t>(__popc(mb)));
-                        ba = static_cast<uint32_t>(__shfl(static_cast<int>(ba), static_cast<int>(lane & 32u)));
-                        bb = static_cast<uint32_t>(__shfl(static_cast<int>(bb), static_cast<int>(lane & 32u)));
-                        if (kept) {
-                            const uint32_t na = s_na[row];
-                            const uint32_t below = (1u << n) - 1u;
-                            const uint32_t slot = in_a ? ba + __popc(ma & below) : na + bb + __popc(mb & below);
-                            if (in_a ? slot < na : slot < G(a.ncand)[q]) {   // the count pass saw the same vectors pass
-                                const uint64_t at = G(a.coff)[q] + slot;
-                                Gw(a.pool)[at] = (static_cast<uint64_t>(p) << 32) | pos;
-                                Gw(a.pdlo)[at] = dm;
-                            }
-                        }
+                    const uint64_t mk = __ballot(kept);
+                    const uint64_t ma = kWrite ? __ballot(kept && bin <= s_abin[row]) : 0ull;
+                    if (lane == 0) {
+                        s_bal[wave][t][i][0] = mk;
+                        s_bal[wave][t][i][1] = ma;
+                        n_cand += __popcll(mk);
                     }
-                    if (lane == 0) n_cand += __popcll(m);
                     if (kChk) {
-                        uint32_t x = scored ? ck[t][i] * static_cast<uint32_t>(pos + 1) : 0u;
+                        uint32_t x = pos < cnt && row < np ? ck[t][i] * static_cast<uint32_t>(pos + 1) : 0u;
                         for (int o = 16; o > 0; o >>= 1) x += static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o));
-                        if (n == 0 && row < np) atomicAdd(&a.chk[q], x);
+                        if (n == 0 && row < np) atomicAdd(&a.chk[s_q[row]], x);
                     }
                 }
                 if (lane == 0) {   // vectors of this tile within the list x pairs
                     const uint64_t at = static_cast<uint64_t>(bj) * 32;
                     n_pairs += (cnt > at ? min<uint64_t>(32, cnt - at) : 0) * np;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < np) {   // pair tid: its row's appends over the item's tiles
+            const uint32_t m = tid, hh = (m / 4) & 1u;   // accumulator row 8 (i / 4) + 4 h + i % 4
+            const uint32_t i = (m / 8) * 4 + (m % 4);
+            uint32_t ta = 0, tb = 0;
+            for (uint32_t w = 0; w < 4; ++w)
+                for (uint32_t t = 0; t < static_cast<uint32_t>(kM8Tiles); ++t) {
+                    if (w + 4 * t >= nb) continue;
+                    const uint32_t mk = static_cast<uint32_t>(s_bal[w][t][i][0] >> (32 * hh));
+                    const uint32_t ma = static_cast<uint32_t>(s_bal[w][t][i][1] >> (32 * hh));
+                    s_off[w][t][m][0] = ta;
+                    s_off[w][t][m][1] = tb;
+                    ta += __popc(mk & ma);
+                    tb += __popc(mk & ~ma);
+                }
+            const uint32_t q = s_q[m];
+            if (!kWrite) {
+                if (ta + tb) atomicAdd(&a.ncand[q], ta + tb);
+            } else {
+                s_base[m][0] = ta ? atomicAdd(&a.fill_a[q], ta) : 0u;
+                s_base[m][1] = tb ? atomicAdd(&a.fill_b[q], tb) : 0u;
+            }
+        }
+        if (kWrite) {
+            __syncthreads();
+            if (nt > 0) {
+#pragma unroll
+                for (int t = 0; t < kM8Tiles; ++t) {
+                    if (t >= static_cast<int>(nt)) break;
+                    const uint32_t bj = b0 + wave + 4 * t;
+                    const uint64_t pos = static_cast<uint64_t>(bj) * 32 + n;
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
+                        float dm;
+                        uint32_t bin;
+                        if (!test(t, i, dm, bin)) continue;
+                        const uint32_t q = s_q[row];
+                        const bool in_a = bin <= s_abin[row];
+                        const uint32_t below = (1u << n) - 1u;
+                        const uint32_t mk = static_cast<uint32_t>(s_bal[wave][t][i][0] >> (32 * h));
+                        const uint32_t ma = static_cast<uint32_t>(s_bal[wave][t][i][1] >> (32 * h)) & mk;
+                        const uint32_t slot = in_a ? s_base[row][0] + s_off[wave][t][row][0] + __popc(ma & below)
+                                                   : s_na[row] + s_base[row][1] + s_off[wave][t][row][1] +
+                                                         __popc(mk & ~ma & below);
+                        if (in_a ? slot < s_na[row] : slot < G(a.ncand)[q]) {   // the count pass saw the same vectors pass
+                            const uint64_t at = G(a.coff)[q] + slot;
+                            Gw(a.pool)[at] = (static_cast<uint64_t>(s_p[row]) << 32) | pos;
+                            Gw(a.pdlo)[at] = dm;
+                        }
+                    }
                 }
             }
         }
@@ -305,6 +374,20 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
         atomicAdd(&a.stats[1], n_cand);
         if (tid == 0) atomicAdd(&a.stats[5], n_items);
     }
+}
+
+__global__ __launch_bounds__(256) void M8LutInt8Kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                       uint64_t words) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < words; i += gridDim.x * 256ull) out[i] = in[i] ^ 0x80808080u;
+}
+
+hipError_t LaunchM8LutInt8(hipStream_t st, const uint8_t* in, uint8_t* out, uint64_t bytes) {
+    const uint64_t words = bytes / 4;   // plane_cols * 32 bytes per query: whole words
+    if (words == 0) return hipSuccess;
+    const uint64_t g = (words + 255) / 256;
+    hipLaunchKernelGGL(M8LutInt8Kernel, dim3(static_cast<uint32_t>(g < 8192 ? g : 8192)), dim3(256), 0, st,
+                       reinterpret_cast<const uint32_t*>(in), reinterpret_cast<uint32_t*>(out), words);
+    return hipGetLastError();
 }
 
 hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool write) {

@@ -85,6 +85,7 @@ constexpr uint32_t kM8SetA = 2048;       // candidates per query refined first (
 struct M8BatchArgs {
     M8View v;
     const uint8_t* lut;       // [nq][plane_cols*32]
+    const uint8_t* lut8;      // the same as int8 (u8 ^ 0x80 = u8 - 128): the contraction's operand
     const float* qc;          // [nq][4]
     const double* rc;         // [nq][nrun][4]
     const float* rotq;        // [nq][padded]
@@ -129,6 +130,8 @@ struct M8BatchArgs {
 // write = 0: count each query's candidates (ncand); write = 1: append them to
 // the pool at coff[q] (same arithmetic, so the same vectors pass).
 hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool write);
+// out[i] = in[i] ^ 0x80 over nq * plane_cols * 32 bytes (the int8 LUT).
+hipError_t LaunchM8LutInt8(hipStream_t st, const uint8_t* in, uint8_t* out, uint64_t bytes);
 // Pool offsets coff from ncand (with_offsets) and refine tasks toff from
 // `count`; coff[nq] = pool entries, toff[nq] = refine tasks.
 hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a, const uint32_t* count, bool with_offsets);

@@ -87,6 +87,7 @@ struct M8Bufs {
     uint64_t* coff_b;
     float *fd, *tau;
     uint64_t *cover, *pairs, *poff;
+    uint8_t* lut8;
     BatchItem* items;
     unsigned long long *wstats, *bstats;
     BatchedParams* wparams;
@@ -122,6 +123,7 @@ void M8CarveBatched(e8q_ctx::Carve& cv, M8Bufs& b, const e8q_m8index* x, uint64_
                     bool chk) {
     const M8View& v = x->view;
     cv.add(&b.fid, nq * k);
+    cv.add(&b.lut8, nq * v.plane_cols * 32);
     cv.add(&b.fd, nq * k);
     cv.add(&b.cover, nq);
     cv.add(&b.ccount, v.nlist);
@@ -180,7 +182,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "mode must be plain or batched");
     const M8View& v = x->view;
     const uint64_t nq = a->nq;
-    if (stats) *stats = e8q_m8_stats{0, 0, 0, 0, 0, 0.0, 0.0, 0, 0, 0, E8Q_MODE_PLAIN, 0, 0};
+    if (stats) *stats = e8q_m8_stats{0, 0, 0, 0, 0, 0.0, 0.0, 0, 0, 0, E8Q_MODE_PLAIN, 0, 0, 0};
     if (nq == 0) return;
     uint32_t maxp = 0;
     uint64_t pairs = 0;
@@ -311,6 +313,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         M8BatchArgs ba{};
         ba.v = v;
         ba.lut = lut;
+        ba.lut8 = b.lut8;
         ba.qc = qc;
         ba.rc = rc;
         ba.rotq = rotq;
@@ -347,6 +350,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         ba.lev_floats = sa.lev_floats;
         ba.set_a = a->batched_set_a ? a->batched_set_a : kM8SetA;
         const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
+        Check(LaunchM8LutInt8(st, lut, b.lut8, nq * v.plane_cols * 32), "int8 LUT");
         Check(LaunchM8MfmaScan(st, ba, 2 * cus, false), "mfma count pass");
         Check(LaunchM8Split(st, ba), "candidate split");
         Check(LaunchM8RefineTasks(st, ba, b.ncand, true), "pool offsets");
@@ -460,6 +464,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             stats->groups = s8[7];
             stats->overflowed = novf;
             stats->pairs_scored = bst[0];
+            stats->items = bst[5];
         }
         float ms = 0.0F;
         Check(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "event time");

@@ -83,6 +83,7 @@ typedef struct {
     uint64_t overflowed;   /* batched form: queries whose candidate lists overflowed, searched again
                               by the plain form */
     uint64_t pairs_scored; /* batched form: (query, vector) pairs of the MFMA contraction */
+    uint64_t items;        /* batched form: its work items (32 pairs x 8 batches at most) */
 } e8q_m8_stats;
 
 /* Allocates the device index and its static tables (the allocation half of
