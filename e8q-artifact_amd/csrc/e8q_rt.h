@@ -95,6 +95,8 @@ struct e8q_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // search timing
     DBuf ws;   // per-call workspace, grown on demand
     DBuf pool; // candidate pool of the batched ME8Q form, sized per call after its count pass
+    DBuf recs; // records of the batched ME8Q count pass
+    uint64_t rec_want = 0;   // records the last batched ME8Q call needed beyond `recs`
 
     // Replay of a search's device sequence as a HIP graph, per call shape.
     // A shape's first call runs eagerly and registers the shape; its second

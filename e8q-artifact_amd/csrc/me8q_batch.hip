@@ -45,6 +45,8 @@
 // vectors.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "e8q_scan_dev.h"
 #include "me8q_device.h"
 #include "me8q_kernels.h"
@@ -54,6 +56,7 @@ namespace e8qdev {
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // Order-preserving 32-bit key of a float distance (negatives below positives).
 __device__ __forceinline__ uint32_t M8Key(float d) {
@@ -62,6 +65,18 @@ __device__ __forceinline__ uint32_t M8Key(float d) {
 }
 __device__ __forceinline__ float M8KeyDist(uint32_t m) {
     return __uint_as_float((m & 0x80000000u) ? (m & 0x7FFFFFFFu) : ~m);
+}
+
+// A record's second word: the tau0 bin (< 64) in bits 26..31, the lower bound
+// in bits 0..25 as its float's sign and top 25 magnitude bits, the magnitude
+// rounded toward -inf (down for a positive bound, up for a negative one), so
+// the value read back is never above the bound.
+__device__ __forceinline__ uint32_t M8RecPack(float dm, uint32_t bin) {
+    const uint32_t u = __float_as_uint(dm), m = u & 0x7FFFFFFFu, s = u >> 31;
+    return (bin << 26) | (s << 25) | (s ? min((m + 63u) >> 6, 0x1FFFFFFu) : m >> 6);
+}
+__device__ __forceinline__ float M8RecBound(uint32_t w) {
+    return __uint_as_float(((w >> 25) & 1u) << 31 | (w & 0x1FFFFFFu) << 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -85,28 +100,59 @@ __device__ __forceinline__ void WaitVmcnt(uint32_t n) {
     }
 }
 
+// Rounding margin of the coarse bound's regrouped evaluation: 2^-18 of the
+// sum of the magnitudes of its terms (d2, the bq, each |wq| times a bound on
+// its run's coarse score, the slack), on top of 2^-12 (|dc| + |slack|).  This
+// kernel's fp32 evaluation and the reference's per-run one (fp32 or fp64)
+// each stay within a few tens of 2^-24 of that sum, so every vector the
+// reference's bound keeps at a threshold is kept here.
+constexpr float kM8MarginAbs = 3.814697265625e-06F;   // 2^-18
+constexpr float kM8MarginRel = 2.4414062e-4F;         // 2^-12
+static_assert(kM8BatchRuns == 4, "the per-pair run tables are float4");
+
+// The coarse bound per (pair, vector): dc = d2 + sum_r (bq_r - wq_r s_r) with
+// s_r = S_r delta + lut_min_x_r the run's coarse score, slack = z sqrt(sum_r
+// pq_r^2 rh_r^2) + sum_r (|wq_r| qerr_r + s3_r m_r) (me8q_prune_avx2.cpp:
+// 9-98).  With S_r = acc_r + 256 nblk_r (acc_r the int8 contraction's sum),
+// s_r = acc_r delta + x_r, so sum_r wq_r s_r = delta T + sum_r wq_r x_r where
+// T = sum_r wq_r acc_r: T is the only per-(pair, vector) quantity the column
+// loop carries; the other sums over runs of a per-pair factor times a
+// per-vector factor are themselves small contractions (K = runs), done at the
+// end of the item with v_mfma_f32_32x32x2_f32 (two runs per instruction, lane
+// l supplying run 2j + l / 32 of pair l % 32 and of vector l % 32), whose
+// 32 x 32 result has the int8 MFMA's accumulator layout.
 template <bool kChk, bool kWrite>
 __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     const M8View& v = a.v;
     __shared__ uint32_t s_it;
-    __shared__ uint32_t s_q[kM8BatchQTile], s_p[kM8BatchQTile], s_ps[kM8BatchQTile], s_bs[kM8BatchQTile];
-    __shared__ float s_tau[kM8BatchQTile], s_d2[kM8BatchQTile], s_dl[kM8BatchQTile];
-    // per (run, pair): {lut_min_x + 256 nblk delta (the int8 shift folded in), qerr,
-    // (score_max + qerr) / 2, min(sqrt(tau0), sqrt(d2_r))^2}
+    __shared__ uint32_t s_q[kM8BatchQTile], s_p[kM8BatchQTile];
+    __shared__ float s_d2[kM8BatchQTile];
+    // per pair: {delta, tau0 + 2^-12 |tau0|, 64 / tau0 (0 when tau0 is +inf), as bits:
+    // the first batch of its probe that the threshold pass has not scored (~0 past np)}
+    __shared__ float4 s_rowf[kM8BatchQTile];
+    // per (run, pair): {x = lut_min_x + 256 nblk delta, qerr, (score_max + qerr) / 2,
+    // min(sqrt(tau0), sqrt(d2_r))^2}; zero for runs past the index's
     __shared__ float4 s_rv[kM8BatchRuns][kM8BatchQTile];
+    // per pair, per run: |lut_min_x| + 512 nblk |delta| >= |s_r| (either form)
+    __shared__ float4 s_rm[kM8BatchQTile];
     __shared__ __align__(16) uint32_t s_oh[16 * 4];
     __shared__ uint32_t s_hist[kChk || kWrite ? 1 : kM8BatchQTile * kM8TauBins];   // count pass
     __shared__ uint32_t s_abin[kM8BatchQTile], s_na[kM8BatchQTile];                   // write pass
     __shared__ uint64_t s_bal[4][kM8Tiles][16][2];       // per wave, tile, accumulator row: kept / set-A ballots
     __shared__ uint32_t s_off[4][kM8Tiles][kM8BatchQTile][2];   // a tile's first append of each pair (A, B)
     __shared__ uint32_t s_base[kM8BatchQTile][2];        // each pair's first slot of the item (A, B)
+    __shared__ uint32_t s_rs[kM8BatchQTile];             // count pass: each pair's first record of the item
+    __shared__ unsigned long long s_rb;                  // count pass: the item's first record
+    // count pass: each (wave, tile, accumulator row, lane)'s record word
+    __shared__ uint32_t s_rw[kWrite ? 1 : 4][kM8Tiles][16][64];
     // The query operand, eight plane columns per stage, a ring of three
     // stages: [stage][column][half][pair][16 bytes] of the int8 LUT.
     __shared__ __align__(16) uint8_t s_a[3 * kM8StageBytes];
-    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t tid = threadIdx.x, wave = Uniform(tid >> 6), lane = tid & 63;
     const uint32_t h = lane >> 5, n = lane & 31;
     if (tid < 16)
         for (uint32_t j = 0; j < 4; ++j) s_oh[tid * 4 + j] = (tid >> 2) == j ? (1u << (8 * (tid & 3))) : 0u;
+    const uint8_t* s_ohb = reinterpret_cast<const uint8_t*>(s_oh);
     const uint32_t PC = v.plane_cols, R = v.nrun, k = a.k, maxp = a.maxp;
     const float finf = __builtin_inff();
     const uint32_t nitems = Uniform(static_cast<uint32_t>(min<uint64_t>(*G(a.counters), a.items_cap)));
@@ -124,11 +170,9 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             const uint32_t q = static_cast<uint32_t>(pr >> 32), p = static_cast<uint32_t>(pr);
             const float tau = G(a.first_d)[static_cast<uint64_t>(q) * k + k - 1];   // tau0
             const uint64_t cv = G(a.cover)[q];
+            const uint32_t ps = static_cast<uint32_t>(cv >> 32), bs = static_cast<uint32_t>(cv);
             s_q[tid] = q;
             s_p[tid] = p;
-            s_ps[tid] = static_cast<uint32_t>(cv >> 32);
-            s_bs[tid] = static_cast<uint32_t>(cv);
-            s_tau[tid] = tau;
             if (kWrite) {
                 s_abin[tid] = G(a.abin)[q];
                 s_na[tid] = G(a.na)[q];
@@ -136,14 +180,28 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             s_d2[tid] = G(a.pdist)[static_cast<uint64_t>(q) * maxp + p];
             const float dl = G(a.qc)[static_cast<uint64_t>(q) * 4 + 1];
             const float sk = tau < finf ? sqrtf(fmaxf(0.0F, tau)) : finf;
-            s_dl[tid] = dl;
-            for (uint32_t r = 0; r < R; ++r) {
-                const E8Q_GLOBAL double* o = G(a.rc) + (static_cast<uint64_t>(q) * R + r) * 4;
-                const float ra = static_cast<float>(G(a.sqd2)[(static_cast<uint64_t>(q) * maxp + p) * R + r]);
-                const float rh = sk < ra ? sk : ra;
-                s_rv[r][tid] = make_float4(fmaf(256.0F * static_cast<float>(v.runs[r].nblk), dl, static_cast<float>(o[1])),
-                                           static_cast<float>(o[2]), static_cast<float>(0.5 * (o[3] + o[2])), rh * rh);
+            // a vector is new to the query unless the threshold pass scored it
+            // (probe positions before ps, and ps's batches before bs)
+            const uint32_t first = tid >= np ? ~0u : p > ps ? 0u : p == ps ? bs : ~0u;
+            s_rowf[tid] = make_float4(dl, tau + kM8MarginRel * fabsf(tau), tau < finf ? 64.0F / tau : 0.0F,
+                                      __uint_as_float(first));
+            float rm[kM8BatchRuns];
+#pragma unroll
+            for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                float4 rv = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+                rm[r] = 0.0F;
+                if (r < R) {
+                    const E8Q_GLOBAL double* o = G(a.rc) + (static_cast<uint64_t>(q) * R + r) * 4;
+                    const float ra = static_cast<float>(G(a.sqd2)[(static_cast<uint64_t>(q) * maxp + p) * R + r]);
+                    const float rh = sk < ra ? sk : ra;
+                    const float nbf = static_cast<float>(v.runs[r].nblk), lmx = static_cast<float>(o[1]);
+                    rv = make_float4(fmaf(256.0F * nbf, dl, lmx), static_cast<float>(o[2]),
+                                     static_cast<float>(0.5 * (o[3] + o[2])), rh * rh);
+                    rm[r] = fmaf(512.0F * nbf, fabsf(dl), fabsf(lmx));
+                }
+                s_rv[r][tid] = rv;
             }
+            s_rm[tid] = make_float4(rm[0], rm[1], rm[2], rm[3]);
         }
         if (!kChk && !kWrite)
             for (uint32_t e = tid; e < kM8BatchQTile * kM8TauBins; e += 256) s_hist[e] = 0u;
@@ -156,6 +214,34 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
         for (int t = 0; t < kM8Tiles; ++t) {   // a tile past nt reads the first one (unused)
             const uint32_t bj = b0 + wave + 4 * (t < static_cast<int>(nt) ? t : 0);
             prow[t] = G(v.plane) + (bat0 + bj) * v.rec + static_cast<uint64_t>(n) * PC;
+        }
+        // The per-vector factors of the bound (each tile's batch, this lane's
+        // vector), from the scan banks (me8q.hpp:175-184), computed before the
+        // column loop; zero for runs past the index's and tiles past nt.
+        float wq[kM8Tiles][kM8BatchRuns], pq2[kM8Tiles][kM8BatchRuns], s3[kM8Tiles][kM8BatchRuns];
+        float bqs[kM8Tiles], bqa[kM8Tiles];
+#pragma unroll
+        for (int t = 0; t < kM8Tiles; ++t) {
+            bqs[t] = 0.0F;
+            bqa[t] = 0.0F;
+            const uint32_t bj = b0 + wave + 4 * t;
+#pragma unroll
+            for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                wq[t][r] = 0.0F;
+                pq2[t][r] = 0.0F;
+                s3[t][r] = 0.0F;
+                if (t < static_cast<int>(nt) && r < R) {
+                    const E8Q_GLOBAL uint8_t* bp = G(v.plane) + (bat0 + bj) * v.rec + 32ull * PC + r * kM8BankBytes;
+                    const E8Q_GLOBAL float* sc = reinterpret_cast<const E8Q_GLOBAL float*>(bp + 128);
+                    const float bq = fmaf(static_cast<float>(bp[n]), sc[1], sc[0]);
+                    const float pq = fmaf(static_cast<float>(bp[64 + n]), sc[5], sc[4]);
+                    wq[t][r] = fmaf(static_cast<float>(bp[32 + n]), sc[3], sc[2]);
+                    pq2[t][r] = pq * pq;
+                    s3[t][r] = sc[3];
+                    bqs[t] += bq;
+                    bqa[t] += fabsf(bq);
+                }
+            }
         }
         // The query operand goes straight to LDS (global_load_lds_dwordx4, 16
         // bytes per lane, no registers), two stages ahead: wave w copies columns
@@ -170,19 +256,68 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             __builtin_amdgcn_global_load_lds(lsrc + (sb * 8 + 2 * wave + 1) * 32, dst + 1024, 16, 0, 0);
         };
         i32x16 acc[kM8Tiles];
-        float dc[kM8Tiles][16], pool[kM8Tiles][16], qw[kM8Tiles][16];
+        float T[kM8Tiles][16];
         uint32_t ck[kM8Tiles][16];
 #pragma unroll
         for (int t = 0; t < kM8Tiles; ++t) {
             acc[t] = i32x16{};
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                dc[t][i] = s_d2[8 * (i / 4) + 4 * h + (i % 4)];
-                pool[t][i] = 0.0F;
-                qw[t][i] = 0.0F;
+                T[t][i] = 0.0F;
                 ck[t][i] = 0u;
             }
         }
+        // Eight plane columns of NT tiles: per column one A read (shared by the
+        // tiles) and per tile the one-hot B row of the vector's nibble, read
+        // one column ahead of its MFMA.  A plane byte's nibble for half h,
+        // times 16 (the B row's LDS offset), is byte e of the word's masked
+        // nibbles shifted up by four.
+        auto block = [&](auto ntc, uint32_t cb, const uint64_t* w) {
+            constexpr int NT = decltype(ntc)::value;
+            const uint8_t* arow = s_a + (cb % 3) * kM8StageBytes + (h * 32 + n) * 16;
+            uint32_t lo[NT], hi[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const uint64_t x = (w[t] >> (4 * h)) & 0x0F0F0F0F0F0F0F0Full;
+                lo[t] = static_cast<uint32_t>(x) << 4;
+                hi[t] = static_cast<uint32_t>(x >> 32) << 4;
+            }
+            auto brow = [&](int t, int e) -> i32x4 {
+                const uint32_t off = __builtin_amdgcn_ubfe(e < 4 ? lo[t] : hi[t], 8 * (e & 3), 8);
+                return *reinterpret_cast<const i32x4*>(s_ohb + off);
+            };
+            i32x4 av = *reinterpret_cast<const i32x4*>(arow);
+            i32x4 bv[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) bv[t] = brow(t, 0);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                i32x4 avn = av, bvn[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) bvn[t] = bv[t];
+                if (e < 7) {
+                    avn = *reinterpret_cast<const i32x4*>(arow + (e + 1) * 1024);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) bvn[t] = brow(t, e + 1);
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv[t], acc[t], 0, 0, 0);
+                av = avn;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) bv[t] = bvn[t];
+            }
+            // issue order: column 0's reads, then per column the next column's
+            // offsets and reads before this column's MFMAs, so an MFMA waits
+            // only for reads issued one column earlier
+            __builtin_amdgcn_sched_group_barrier(0x100, NT + 1, 0);
+#pragma unroll
+            for (int e = 0; e < 7; ++e) {
+                __builtin_amdgcn_sched_group_barrier(0x002, 2 * NT, 0);
+                __builtin_amdgcn_sched_group_barrier(0x100, NT + 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);
+        };
         uint32_t r = 0, rend = R > 0 ? Uniform(v.runs[0].blk0 + v.runs[0].nblk) : PC;
         const uint32_t ncb = PC / 8;
         // prologue: stages 0 and 1, then this wave's plane words of block 0
@@ -205,49 +340,21 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                         wnext[t] = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(prow[t] + (cb + 1) * 8);
             }
             if (cb + 2 < ncb) stage(cb + 2);   // its ring slot was last read before this barrier
-            if (nt > 0) {
-                const uint8_t* arow = s_a + (cb % 3) * kM8StageBytes + (h * 32 + n) * 16;
+            if (nt == 2) block(std::integral_constant<int, 2>{}, cb, wcur);
+            else if (nt == 1) block(std::integral_constant<int, 1>{}, cb, wcur);
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const i32x4 av = *reinterpret_cast<const i32x4*>(arow + e * 1024);
+            for (int t = 0; t < kM8Tiles; ++t) wcur[t] = wnext[t];
+            if ((cb + 1) * 8 == rend) {   // run r ends here: T += wq_r acc_r
+                const int32_t shift = 256 * static_cast<int32_t>(Uniform(v.runs[r].nblk));
 #pragma unroll
-                    for (int t = 0; t < kM8Tiles; ++t) {
-                        if (t >= static_cast<int>(nt)) break;
-                        const uint32_t nib = static_cast<uint32_t>(wcur[t] >> (8 * e + 4 * h)) & 15u;
-                        const i32x4 bv = *reinterpret_cast<const i32x4*>(s_oh + nib * 4);
-                        acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[t], 0, 0, 0);
+                for (int t = 0; t < kM8Tiles; ++t) {
+                    const float wr = r == 0 ? wq[t][0] : r == 1 ? wq[t][1] : r == 2 ? wq[t][2] : wq[t][3];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        T[t][i] = fmaf(wr, static_cast<float>(acc[t][i]), T[t][i]);
+                        if (kChk) ck[t][i] += (r + 1) * static_cast<uint32_t>(acc[t][i] + shift);
                     }
-                }
-#pragma unroll
-                for (int t = 0; t < kM8Tiles; ++t) wcur[t] = wnext[t];
-            }
-            if ((cb + 1) * 8 == rend) {   // run r ends here: its terms of the coarse bound
-                if (nt > 0) {
-                    const int32_t shift = 256 * static_cast<int32_t>(Uniform(v.runs[r].nblk));
-#pragma unroll
-                    for (int t = 0; t < kM8Tiles; ++t) {
-                        if (t >= static_cast<int>(nt)) break;
-                        const uint32_t bj = b0 + wave + 4 * t;
-                        const E8Q_GLOBAL uint8_t* bp = G(v.plane) + (bat0 + bj) * v.rec + 32ull * PC + r * kM8BankBytes;
-                        const E8Q_GLOBAL float* sc = reinterpret_cast<const E8Q_GLOBAL float*>(bp + 128);
-                        const float bq = fmaf(static_cast<float>(bp[n]), sc[1], sc[0]);
-                        const float wq = fmaf(static_cast<float>(bp[32 + n]), sc[3], sc[2]);
-                        const float pq = fmaf(static_cast<float>(bp[64 + n]), sc[5], sc[4]);
-                        const float aw = fabsf(wq), s3 = sc[3], pq2 = pq * pq;
-#pragma unroll
-                        for (int i = 0; i < 16; ++i) {
-                            // the filter's terms, regrouped (the test carries a rounding margin):
-                            // s_c from the shifted sum, (pq rh)^2 as pq^2 rh^2
-                            const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
-                            const float4 rv = s_rv[r][row];
-                            const float s_c = fmaf(static_cast<float>(acc[t][i]), s_dl[row], rv.x);
-                            dc[t][i] = fmaf(-wq, s_c, dc[t][i]) + bq;
-                            pool[t][i] = fmaf(pq2, rv.w, pool[t][i]);
-                            qw[t][i] = fmaf(s3, rv.z, fmaf(aw, rv.y, qw[t][i]));
-                            if (kChk) ck[t][i] += (r + 1) * static_cast<uint32_t>(acc[t][i] + shift);
-                        }
-                        acc[t] = i32x16{};
-                    }
+                    acc[t] = i32x16{};
                 }
                 ++r;
                 rend = r < R ? Uniform(v.runs[r].blk0 + v.runs[r].nblk) : PC;
@@ -258,20 +365,50 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
         // tiles (one atomic per query and item), and the write pass places each
         // vector at its query's base + the tile's offset + its rank in the ballot.
         const float z = a.z;
-        auto test = [&](int t, int i, float& dm, uint32_t& bin) -> bool {
+        // MFMA operands of the sums: this lane's pair row n and vector n, runs h
+        // and 2 + h (a pair's factors are the same for both tiles)
+        const float4 ra0 = s_rv[h][n], ra1 = s_rv[2 + h][n];
+        const float4 rmv = s_rm[n];
+        const float rm0 = h ? rmv.y : rmv.x, rm1 = h ? rmv.w : rmv.z, d2n = s_d2[n];
+        struct Sums {
+            f32x16 e, m, p, q;
+        };
+        // e = d2 + sum_r (bq_r - wq_r x_r); m = |d2| + sum_r (|bq_r| + |wq_r| rm_r);
+        // p = sum_r pq_r^2 rh_r^2; q = sum_r (|wq_r| qerr_r + s3_r m_r)
+        auto sums = [&](int t) -> Sums {
+            const f32x16 zero = {};
+            const float w0 = h ? wq[t][1] : wq[t][0], w1 = h ? wq[t][3] : wq[t][2];
+            const float p0 = h ? pq2[t][1] : pq2[t][0], p1 = h ? pq2[t][3] : pq2[t][2];
+            const float t0 = h ? s3[t][1] : s3[t][0], t1 = h ? s3[t][3] : s3[t][2];
+            Sums o;
+            o.e = __builtin_amdgcn_mfma_f32_32x32x2f32(h ? 1.0F : d2n, h ? bqs[t] : 1.0F, zero, 0, 0, 0);
+            o.e = __builtin_amdgcn_mfma_f32_32x32x2f32(-ra0.x, w0, o.e, 0, 0, 0);
+            o.e = __builtin_amdgcn_mfma_f32_32x32x2f32(-ra1.x, w1, o.e, 0, 0, 0);
+            o.m = __builtin_amdgcn_mfma_f32_32x32x2f32(h ? 1.0F : fabsf(d2n), h ? bqa[t] : 1.0F, zero, 0, 0, 0);
+            o.m = __builtin_amdgcn_mfma_f32_32x32x2f32(rm0, fabsf(w0), o.m, 0, 0, 0);
+            o.m = __builtin_amdgcn_mfma_f32_32x32x2f32(rm1, fabsf(w1), o.m, 0, 0, 0);
+            o.p = __builtin_amdgcn_mfma_f32_32x32x2f32(ra0.w, p0, zero, 0, 0, 0);
+            o.p = __builtin_amdgcn_mfma_f32_32x32x2f32(ra1.w, p1, o.p, 0, 0, 0);
+            o.q = __builtin_amdgcn_mfma_f32_32x32x2f32(ra0.y, fabsf(w0), zero, 0, 0, 0);
+            o.q = __builtin_amdgcn_mfma_f32_32x32x2f32(ra0.z, t0, o.q, 0, 0, 0);
+            o.q = __builtin_amdgcn_mfma_f32_32x32x2f32(ra1.y, fabsf(w1), o.q, 0, 0, 0);
+            o.q = __builtin_amdgcn_mfma_f32_32x32x2f32(ra1.z, t1, o.q, 0, 0, 0);
+            return o;
+        };
+        // The test of accumulator element i of tile t: dc = e - delta T, the
+        // slack, the lower bound dm (less the margin) against tau0, the tau0 bin
+        // of dc; scored (in the list, a pair of the item) and not scored by the
+        // threshold pass.
+        auto test = [&](int t, int i, const Sums& sm, float& dm, uint32_t& bin) -> bool {
             const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
             const uint32_t bj = b0 + wave + 4 * t;
             const uint64_t pos = static_cast<uint64_t>(bj) * 32 + n;
-            const uint32_t p = s_p[row];
-            const bool scored = pos < cnt && row < np;
-            const bool fresh = p > s_ps[row] || (p == s_ps[row] && bj >= s_bs[row]);
-            const float slack = fmaf(z, sqrtf(pool[t][i]), qw[t][i]);
-            const float tau = s_tau[row];
-            dm = dc[t][i] - slack - 2.4414062e-4F * (fabsf(dc[t][i]) + fabsf(slack));
-            bin = tau < finf ? min(kM8TauBins - 1,
-                                   static_cast<uint32_t>(static_cast<float>(kM8TauBins) * fmaxf(dc[t][i], 0.0F) / tau))
-                             : 0u;
-            return scored && fresh && !(dm >= tau + 2.4414062e-4F * fabsf(tau));
+            const float4 rf = s_rowf[row];
+            const float dc = fmaf(-rf.x, T[t][i], sm.e[i]);
+            const float slack = fmaf(z, sqrtf(sm.p[i]), sm.q[i]);
+            dm = dc - slack - (kM8MarginRel * (fabsf(dc) + fabsf(slack)) + kM8MarginAbs * (sm.m[i] + fabsf(slack)));
+            bin = static_cast<uint32_t>(fminf(fmaxf(dc, 0.0F) * rf.z, static_cast<float>(kM8TauBins - 1)));
+            return bj >= __float_as_uint(rf.w) && pos < cnt && !(dm >= rf.y);
         };
         if (nt > 0) {
 #pragma unroll
@@ -279,13 +416,17 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                 if (t >= static_cast<int>(nt)) break;
                 const uint32_t bj = b0 + wave + 4 * t;
                 const uint64_t pos = static_cast<uint64_t>(bj) * 32 + n;
+                const Sums sm = sums(t);
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
                     float dm;
                     uint32_t bin;
-                    const bool kept = test(t, i, dm, bin);
-                    if (!kChk && !kWrite && kept) atomicAdd(&s_hist[row * kM8TauBins + bin], 1u);
+                    const bool kept = test(t, i, sm, dm, bin);
+                    if (!kWrite) {
+                        if (!kChk && kept) atomicAdd(&s_hist[row * kM8TauBins + bin], 1u);
+                        s_rw[kWrite ? 0 : wave][t][i][lane] = M8RecPack(dm, bin);
+                    }
                     const uint64_t mk = __ballot(kept);
                     const uint64_t ma = kWrite ? __ballot(kept && bin <= s_abin[row]) : 0ull;
                     if (lane == 0) {
@@ -306,10 +447,10 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             }
         }
         __syncthreads();
+        uint32_t ta = 0, tb = 0;
         if (tid < np) {   // pair tid: its row's appends over the item's tiles
             const uint32_t m = tid, hh = (m / 4) & 1u;   // accumulator row 8 (i / 4) + 4 h + i % 4
             const uint32_t i = (m / 8) * 4 + (m % 4);
-            uint32_t ta = 0, tb = 0;
             for (uint32_t w = 0; w < 4; ++w)
                 for (uint32_t t = 0; t < static_cast<uint32_t>(kM8Tiles); ++t) {
                     if (w + 4 * t >= nb) continue;
@@ -328,6 +469,46 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                 s_base[m][1] = tb ? atomicAdd(&a.fill_b[q], tb) : 0u;
             }
         }
+        if (!kWrite && a.rec != nullptr) {
+            // Records: the item's block of the record array (one atomic), each
+            // pair's records contiguous in it, a pair's tiles in (wave, tile)
+            // order, the words staged in LDS by the test above.
+            if (tid < kM8BatchQTile) {   // wave 0, lanes 0..31 = the pairs (count 0 past np)
+                const uint32_t tot = ta + tb;
+                uint32_t x = tot;
+                for (uint32_t o = 1; o < 32; o <<= 1) {
+                    const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(x), o, 32));
+                    if (n >= o) x += y;
+                }
+                const uint32_t all = static_cast<uint32_t>(__shfl(static_cast<int>(x), 31, 32));
+                unsigned long long rb = 0;
+                if (n == 0) {
+                    rb = all ? atomicAdd(a.rcount, static_cast<unsigned long long>(all)) : 0ull;
+                    s_rb = rb;
+                }
+                const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(rb)), 0, 32));
+                s_rs[n] = x - tot;
+                a.itab[static_cast<uint64_t>(it) * kM8BatchQTile + n] = make_uint2(lo + x - tot, tot);
+            }
+            __syncthreads();
+            const unsigned long long rb = s_rb;
+            if (nt > 0) {
+#pragma unroll
+                for (int t = 0; t < kM8Tiles; ++t) {
+                    if (t >= static_cast<int>(nt)) break;
+                    const uint32_t bj = b0 + wave + 4 * t;
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
+                        const uint32_t mk = static_cast<uint32_t>(s_bal[wave][t][i][0] >> (32 * h));
+                        if (!((mk >> n) & 1u)) continue;
+                        const unsigned long long at =
+                            rb + s_rs[row] + s_off[wave][t][row][1] + __popc(mk & ((1u << n) - 1u));
+                        if (at < a.rec_cap) a.rec[at] = make_uint2(bj * 32 + n, s_rw[kWrite ? 0 : wave][t][i][lane]);
+                    }
+                }
+            }
+        }
         if (kWrite) {
             __syncthreads();
             if (nt > 0) {
@@ -336,12 +517,13 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                     if (t >= static_cast<int>(nt)) break;
                     const uint32_t bj = b0 + wave + 4 * t;
                     const uint64_t pos = static_cast<uint64_t>(bj) * 32 + n;
+                    const Sums sm = sums(t);
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
                         const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
                         float dm;
                         uint32_t bin;
-                        if (!test(t, i, dm, bin)) continue;
+                        if (!test(t, i, sm, dm, bin)) continue;
                         const uint32_t q = s_q[row];
                         const bool in_a = bin <= s_abin[row];
                         const uint32_t below = (1u << n) - 1u;
@@ -369,7 +551,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
         if (tid == 0) ++n_items;
         __syncthreads();
     }
-    if (kWrite && lane == 0 && a.stats) {
+    if (!kWrite && lane == 0 && a.stats) {
         atomicAdd(&a.stats[0], n_pairs);
         atomicAdd(&a.stats[1], n_cand);
         if (tid == 0) atomicAdd(&a.stats[5], n_items);
@@ -395,6 +577,65 @@ hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid,
     if (write) hipLaunchKernelGGL((M8MfmaScanKernel<false, true>), dim3(grid), dim3(256), 0, st, a);
     else if (a.chk) hipLaunchKernelGGL((M8MfmaScanKernel<true, false>), dim3(grid), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((M8MfmaScanKernel<false, false>), dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Placement: one wave per (item, pair) takes the pair's records of the item,
+// 64 at a time, keeps those of the set it places (set A: tau0 bin <=
+// abin[q]; B: the rest, with a lower bound below min(tau0, tau1) -- the
+// compaction's test), takes their slots with one atomic and writes the pool
+// entries there.
+
+__global__ __launch_bounds__(256) void M8PlaceKernel(M8BatchArgs a, int set_b) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nitems = min<uint64_t>(*G(a.counters), a.items_cap);
+    const uint64_t units = nitems * kM8BatchQTile;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t wstride = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+    uint32_t* fill = set_b ? a.fill_b : a.fill_a;
+    for (uint64_t u = Uniform64((static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6); u < units;
+         u += wstride) {
+        const uint32_t it = static_cast<uint32_t>(u / kM8BatchQTile), m = static_cast<uint32_t>(u % kM8BatchQTile);
+        const E8Q_GLOBAL BatchItem* item = G(a.items) + it;
+        if (m >= Uniform(item->np)) continue;
+        const E8Q_GLOBAL uint2* ep = G(a.itab) + u;
+        const uint32_t start = Uniform(ep->x), cnt = Uniform(ep->y);
+        if (cnt == 0) continue;
+        const uint64_t pr = G(a.pairs)[Uniform(item->p0) + m];
+        const uint32_t q = static_cast<uint32_t>(pr >> 32), p = static_cast<uint32_t>(pr);
+        const uint32_t abin = G(a.abin)[q], na = G(a.na)[q];
+        const uint32_t room = set_b ? G(a.ncand)[q] - na : na;
+        const uint64_t base = G(a.coff)[q] + (set_b ? na : 0u);
+        float lim = 0.0F;
+        if (set_b) {
+            const float tau0 = G(a.tau)[q], t1 = G(a.first_d)[static_cast<uint64_t>(q) * a.k + a.k - 1];
+            const float tau = t1 < tau0 ? t1 : tau0;
+            lim = tau + 2.4414062e-4F * fabsf(tau);
+        }
+        for (uint32_t j = 0; j < cnt; j += 64) {
+            uint2 r = make_uint2(0u, 0u);
+            bool keep = false;
+            if (j + lane < cnt) {
+                const E8Q_GLOBAL uint2* rp = G(a.rec) + static_cast<uint64_t>(start) + j + lane;
+                r = make_uint2(rp->x, rp->y);
+                const bool in_a = (r.y >> 26) <= abin;
+                keep = set_b ? !in_a && !(M8RecBound(r.y) >= lim) : in_a;
+            }
+            const uint64_t mk = __ballot(keep);
+            if (mk == 0) continue;
+            uint32_t s0 = 0;
+            if (lane == 0) s0 = atomicAdd(&fill[q], static_cast<uint32_t>(__popcll(mk)));
+            s0 = static_cast<uint32_t>(__shfl(static_cast<int>(s0), 0));
+            const uint32_t slot = s0 + static_cast<uint32_t>(__popcll(mk & below));
+            if (keep && slot < room) Gw(a.pool)[base + slot] = (static_cast<uint64_t>(p) << 32) | r.x;
+        }
+    }
+}
+
+hipError_t LaunchM8Place(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool set_b) {
+    if (a.rec == nullptr || a.itab == nullptr) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(M8PlaceKernel, dim3(grid), dim3(256), 0, st, a, set_b ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -478,15 +719,15 @@ hipError_t LaunchM8Split(hipStream_t st, const M8BatchArgs& a) {
 // a read) to the candidates whose lower bound at tau0 is below tau: the
 // bound only rises as the threshold falls, so no vector the bound keeps at
 // tau is dropped.
-__global__ __launch_bounds__(64) void M8CompactKernel(M8BatchArgs a) {
+__global__ __launch_bounds__(64) void M8CompactKernel(M8BatchArgs a, int placed) {
     const uint32_t lane = threadIdx.x;
     for (uint64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
         const float tau0 = a.tau[q], t1 = a.first_d[q * a.k + a.k - 1];
         const float tau = t1 < tau0 ? t1 : tau0;
         const float lim = tau + 2.4414062e-4F * fabsf(tau);
         const uint64_t base = a.coff[q] + a.na[q];
-        const uint32_t nb = Uniform(a.ncand[q] - a.na[q]);
-        uint32_t w = 0;
+        const uint32_t nb = placed ? 0u : Uniform(a.ncand[q] - a.na[q]);
+        uint32_t w = placed ? a.fill_b[q] : 0u;
         for (uint32_t r0 = 0; r0 < nb; r0 += 64) {
             const uint32_t r = r0 + lane;
             uint64_t e = 0;
@@ -507,10 +748,10 @@ __global__ __launch_bounds__(64) void M8CompactKernel(M8BatchArgs a) {
     }
 }
 
-hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a) {
+hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a, bool placed) {
     if (a.nq == 0) return hipSuccess;
     const uint32_t grid = static_cast<uint32_t>(a.nq < 65536 ? a.nq : 65536);
-    hipLaunchKernelGGL(M8CompactKernel, dim3(grid), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(M8CompactKernel, dim3(grid), dim3(64), 0, st, a, placed ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -126,10 +126,23 @@ struct M8BatchArgs {
     uint32_t* chk;            // optional [nq]: sum over scored pairs of (pos+1) * sum_r (r+1) * coarse_sum_r, mod 2^32
     uint32_t lev_floats;
     uint32_t set_a;           // candidates per query of set A (kM8SetA)
+    // Records of the count pass (null: none written): each passing (pair,
+    // vector) as {position in its list, tau0 bin << 26 | its lower bound
+    // rounded down to 26 bits}, a pair's records of an item contiguous.
+    uint2* rec;
+    uint64_t rec_cap;         // records that fit; past it the count goes on and nothing is written
+    unsigned long long* rcount;   // [1] records appended (all of them, also past rec_cap)
+    uint2* itab;              // [items_cap * kM8BatchQTile] per (item, pair): {first record, count}
 };
-// write = 0: count each query's candidates (ncand); write = 1: append them to
-// the pool at coff[q] (same arithmetic, so the same vectors pass).
+// write = 0: count each query's candidates (ncand), and with a.rec write
+// their records; write = 1: append them to the pool at coff[q] (same
+// arithmetic, so the same vectors pass).
 hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool write);
+// The count pass's records into the pool, from the records instead of a
+// second contraction: set_b = 0, set A at coff[q] (count fill_a); set_b = 1,
+// after A's refine, the rest whose lower bound is below min(tau0, tau1) at
+// coff[q] + na[q] (count fill_b).
+hipError_t LaunchM8Place(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool set_b);
 // out[i] = in[i] ^ 0x80 over nq * plane_cols * 32 bytes (the int8 LUT).
 hipError_t LaunchM8LutInt8(hipStream_t st, const uint8_t* in, uint8_t* out, uint64_t bytes);
 // Pool offsets coff from ncand (with_offsets) and refine tasks toff from
@@ -139,9 +152,10 @@ hipError_t LaunchM8RefineTasks(hipStream_t st, const M8BatchArgs& a, const uint3
 // lowest bins holding at least kM8SetA of them (abin, na); tau = tau0.
 hipError_t LaunchM8Split(hipStream_t st, const M8BatchArgs& a);
 // After A's exact top-k is in first_ids / first_d: tau = min(tau0, its k-th
-// distance), and each query's B (at coff_b) compacted in place to the
-// candidates whose lower bound is below tau (count in nfill).
-hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a);
+// distance), coff_b = coff + na, and each query's B compacted in place to the
+// candidates whose lower bound is below tau (count in nfill); placed = 1: B
+// was placed from the records with that filter, nfill = fill_b.
+hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a, bool placed);
 size_t M8RefineLds(const M8View& v, uint32_t lev_floats);
 // first = true (indexes of two or more runs): each candidate of a.pool gets
 // the first run of its cascade; those still alive go to a.cont.  first =

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -89,11 +90,20 @@ struct M8Bufs {
     uint64_t *cover, *pairs, *poff;
     uint8_t* lut8;
     BatchItem* items;
-    unsigned long long *wstats, *bstats;
+    unsigned long long *wstats, *bstats, *rcount;
+    uint2* itab;
     BatchedParams* wparams;
 };
 
 constexpr uint64_t kM8PoolMax = uint64_t{1} << 31;   // candidate pool entries (more: the plain form)
+// Records of the count pass (8 B each): the first call of a context reserves
+// up to kM8RecFirst of them, a call that needed more raises the next call's
+// reserve to 1.25 x its count, up to kM8RecMax (2^31 = 16 GiB); a call whose
+// records do not fit places its candidates with the write pass.  The
+// environment variable E8Q_M8_RECORD_CAP lowers the limit (0: always the
+// write pass).
+constexpr uint64_t kM8RecFirst = uint64_t{1} << 28;
+constexpr uint64_t kM8RecMax = uint64_t{1} << 31;
 constexpr uint32_t kM8DefaultBudget = 16;
 
 // Whether the batched form runs for this index and k.
@@ -148,8 +158,19 @@ void M8CarveBatched(e8q_ctx::Carve& cv, M8Bufs& b, const e8q_m8index* x, uint64_
     cv.add(&b.toff2, nq + 1);
     cv.add(&b.wstats, 4);
     cv.add(&b.bstats, 8);
+    cv.add(&b.rcount, 1);
+    cv.add(&b.itab, std::max<uint64_t>(M8ItemsCap(x, nq, pairs), 1) * kM8BatchQTile);
     cv.add(&b.wparams, 1);
     if (chk) cv.add(&b.chk, nq);
+}
+
+// Records the count pass of this call may write.
+uint64_t M8RecordCap(e8q_ctx* ctx, uint64_t items_cap) {
+    uint64_t lim = kM8RecMax;
+    if (const char* e = std::getenv("E8Q_M8_RECORD_CAP")) lim = std::min<uint64_t>(lim, std::strtoull(e, nullptr, 10));
+    const uint64_t bound = items_cap * kM8BatchQTile * kM8BatchBTile * 32;   // every pair of every item
+    const uint64_t want = std::max(kM8RecFirst, ctx->rec_want);
+    return std::min({lim, bound, want});
 }
 
 void M8Carve(e8q_ctx::Carve& cv, M8Bufs& b, const M8View& v, uint64_t nq, uint64_t k, uint64_t maxp, bool q_host,
@@ -350,17 +371,32 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         ba.lev_floats = sa.lev_floats;
         ba.set_a = a->batched_set_a ? a->batched_set_a : kM8SetA;
         const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
+        // records of the count pass, when the reserve holds any
+        const uint64_t rcap = M8RecordCap(ctx, w.items_cap);
+        if (rcap > 0 && ctx->recs.bytes < rcap * sizeof(uint2)) ctx->recs.alloc(rcap * sizeof(uint2));
+        ba.rec = rcap > 0 ? ctx->recs.as<uint2>() : nullptr;
+        ba.rec_cap = rcap;
+        ba.rcount = b.rcount;
+        ba.itab = b.itab;
+        Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.rcount), 2, 0), "clear");
         Check(LaunchM8LutInt8(st, lut, b.lut8, nq * v.plane_cols * 32), "int8 LUT");
+        Check(hipEventRecord(ctx->ev[1], st), "event");
         Check(LaunchM8MfmaScan(st, ba, 2 * cus, false), "mfma count pass");
+        Check(hipEventRecord(ctx->ev[2], st), "event");
         Check(LaunchM8Split(st, ba), "candidate split");
         Check(LaunchM8RefineTasks(st, ba, b.ncand, true), "pool offsets");
-        uint64_t total = 0;
+        uint64_t total = 0, nrec = 0;
         uint32_t ntask = 0, cnt[4] = {0, 0, 0, 0};
         Check(hipMemcpyAsync(&total, b.poff + nq, 8, hipMemcpyDeviceToHost, st), "download pool size");
         Check(hipMemcpyAsync(&ntask, b.toff + nq, 4, hipMemcpyDeviceToHost, st), "download task count");
         Check(hipMemcpyAsync(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost, st), "download counters");
+        Check(hipMemcpyAsync(&nrec, b.rcount, 8, hipMemcpyDeviceToHost, st), "download record count");
         Check(hipStreamSynchronize(st), "batched count pass");
         if (cnt[0] > w.items_cap) Throw(E8Q_ERUNTIME, "batched search: work items exceed their bound");
+        const bool placed = ba.rec != nullptr && nrec <= rcap;
+        if (ba.rec != nullptr && nrec != total && nrec <= rcap)
+            Throw(E8Q_ERUNTIME, "batched search: records and candidate counts differ");
+        if (!placed && nrec > rcap && nrec <= kM8RecMax) ctx->rec_want = nrec + nrec / 4;
         if (total > kM8PoolMax) {   // too many candidates for one pool: the plain form
             novf = nq;
             Check(hipEventRecord(ctx->ev[1], st), "event");
@@ -382,10 +418,12 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             ba.ncont = b.ncont;
             uint32_t* ts = reinterpret_cast<uint32_t*>(pb + 2 * pool_b + dlo_b);
             const uint64_t tsn = task_b / 4;
-            Check(LaunchFill32(st, b.counters + 1, 1, 0), "clear");
-            Check(hipEventRecord(ctx->ev[1], st), "event");
-            Check(LaunchM8MfmaScan(st, ba, 2 * cus, true), "mfma write pass");
-            Check(hipEventRecord(ctx->ev[2], st), "event");
+            if (placed) {
+                Check(LaunchM8Place(st, ba, 8 * cus, false), "placement of set A");
+            } else {
+                Check(LaunchFill32(st, b.counters + 1, 1, 0), "clear");
+                Check(LaunchM8MfmaScan(st, ba, 2 * cus, true), "mfma write pass");
+            }
             // 4. exact cascades (every candidate's first run, then the whole
             // cascade of those still alive) and a top-k merge, first of each
             // query's set A (its exact top-k gives tau1), then of the rest
@@ -411,7 +449,8 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             M8BatchArgs ra = ba;   // set A
             ra.nfill = b.na;
             refine(ra, ts, ts + tsn, b.fid, b.fd);
-            Check(LaunchM8Compact(st, ba), "compaction");
+            if (placed) Check(LaunchM8Place(st, ba, 8 * cus, true), "placement of the rest");
+            Check(LaunchM8Compact(st, ba, placed), "compaction");
             M8BatchArgs rb = ba;   // the rest
             rb.coff = b.coff_b;
             refine(rb, ts + 2 * tsn, ts + 3 * tsn, out_ids, out_d);

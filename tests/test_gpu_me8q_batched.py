@@ -200,6 +200,26 @@ def test_every_vector_a_candidate(ctx):
     assert int(np.max(_ulps(bd, pd))) <= 1
 
 
+@pytest.mark.parametrize("case", [0, 1, 3])
+@pytest.mark.parametrize("cap", ["0", "100"])
+def test_records_match_write_pass(ctx, monkeypatch, case, cap):
+    # The count pass's records placed into the pool, against the write pass
+    # (a second contraction) that runs when the records do not fit: the same
+    # candidates with the same bounds and bins, so the same answer bit for bit.
+    seed, n, dim, nlist, avg, spec, sizes = CASES[case]
+    ix, f, q = _index(ctx, seed, n, dim, nlist, avg, spec, sizes)
+    k, nprobe, z = 10, max(2, nlist // 2), _z(10, 0.0)
+    print(f"me8q batched records vs write pass case {case} cap {cap}", flush=True)
+    ri, rd, _, rst = ix.search(q, k, nprobe, z, mode=BATCHED)
+    monkeypatch.setenv("E8Q_M8_RECORD_CAP", cap)
+    wi, wd, _, wst = ix.search(q, k, nprobe, z, mode=BATCHED)
+    monkeypatch.delenv("E8Q_M8_RECORD_CAP")
+    assert rst.form == BATCHED and wst.form == BATCHED
+    np.testing.assert_array_equal(ri, wi)
+    np.testing.assert_array_equal(rd.view(np.uint32), wd.view(np.uint32))
+    assert (rst.refined, rst.survivors, rst.pairs_scored) == (wst.refined, wst.survivors, wst.pairs_scored)
+
+
 def test_unsupported_shapes_run_plain(ctx):
     ix, f, q = _index(ctx, 9, 3000, 384, 6, 4.0, "64:8,64:6,64:5,64:4,128:2")   # five coded runs
     print("me8q batched five runs", flush=True)
