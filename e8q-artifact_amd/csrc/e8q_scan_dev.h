@@ -21,6 +21,8 @@ __device__ __forceinline__ bool Better(float d1, uint32_t i1, float d2, uint32_t
 // compiler cannot infer their address space, and generic (flat) loads would
 // also count against the LDS wait counter.
 #define E8Q_GLOBAL __attribute__((address_space(1)))
+// LDS views of generic pointers into __shared__ arrays (ds_read, not flat loads).
+#define E8Q_LDS __attribute__((address_space(3)))
 // Plain clang vectors (no constructors) for loads through such pointers.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));

@@ -768,10 +768,10 @@ hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a, bool placed) {
 // even / odd blocks in block order.  Digit decode as M8HalfChainT.  Runs are
 // multiples of 64 dimensions, so a run holds whole groups of eight blocks, and
 // its plane bytes and fields start 8-byte aligned in their rows.
-template <int T>
-__device__ __forceinline__ float M8LaneScoreT(const uint8_t* __restrict__ pb, const uint8_t* __restrict__ fr,
-                                              const float* __restrict__ lev, const float* __restrict__ qr,
-                                              uint32_t nblk) {
+template <int T, class LevP>
+__device__ __forceinline__ float M8LaneScoreT(const E8Q_GLOBAL uint8_t* __restrict__ pb,
+                                              const E8Q_GLOBAL uint8_t* __restrict__ fr, LevP lev,
+                                              const E8Q_GLOBAL float* __restrict__ qr, uint32_t nblk) {
     constexpr uint32_t rate = T + 1;
     constexpr uint32_t NW = T > 0 ? T : 1;
     // `nb` bits at bit `pos` of the little-endian words w (positions are
@@ -786,11 +786,12 @@ __device__ __forceinline__ float M8LaneScoreT(const uint8_t* __restrict__ pb, co
     // eight blocks at a time: their plane bytes (one 8-byte load) and their
     // fields (8 T bytes, T 8-byte loads), all loads in flight together
     for (uint32_t g8 = 0; g8 < nblk; g8 += 8) {
-        const uint64_t pw = *reinterpret_cast<const uint64_t*>(pb + g8);
+        const uint64_t pw = *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(pb + g8);
         uint64_t fw[NW];
 #pragma unroll
         for (uint32_t i = 0; i < NW; ++i)
-            fw[i] = T > 0 ? *reinterpret_cast<const uint64_t*>(fr + static_cast<uint64_t>(g8) * T + 8 * i) : 0ull;
+            fw[i] = T > 0 ? *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(fr + static_cast<uint64_t>(g8) * T + 8 * i)
+                          : 0ull;
 #pragma unroll
         for (uint32_t b = 0; b < 8; ++b) {
             const uint32_t pbyte = static_cast<uint32_t>(pw >> (8 * b)) & 0xFFu;
@@ -813,8 +814,8 @@ __device__ __forceinline__ float M8LaneScoreT(const uint8_t* __restrict__ pb, co
                 dg[7] = ((f7 << 1) | par) | ((pbyte & 1u) << T);
                 br = bits(fw, base + 8 * T - 1, 1);
             }
-            const float* lb = lev + (br << rate);
-            const float* qb = qr + (g8 + b) * 8;
+            const auto lb = lev + (br << rate);
+            const E8Q_GLOBAL float* qb = qr + (g8 + b) * 8;
             if (b & 1u) {
 #pragma unroll
                 for (uint32_t ci = 0; ci < 8; ++ci) o[ci] = fmaf(lb[dg[ci]], qb[ci], o[ci]);
@@ -830,8 +831,11 @@ __device__ __forceinline__ float M8LaneScoreT(const uint8_t* __restrict__ pb, co
     return ((v[0] + v[4]) + (v[2] + v[6])) + ((v[1] + v[5]) + (v[3] + v[7]));
 }
 
-__device__ float M8LaneScore(uint32_t rate, const uint8_t* pb, const uint8_t* fr, const float* lev, const float* qr,
-                             uint32_t nblk) {
+// The score of one run by its rate, with the level tables in LDS (LevP =
+// const E8Q_LDS float*) or in global memory.
+template <class LevP>
+__device__ float M8LaneScore(uint32_t rate, const E8Q_GLOBAL uint8_t* pb, const E8Q_GLOBAL uint8_t* fr, LevP lev,
+                             const E8Q_GLOBAL float* qr, uint32_t nblk) {
     switch (rate) {
         case 1: return M8LaneScoreT<0>(pb, fr, lev, qr, nblk);
         case 2: return M8LaneScoreT<1>(pb, fr, lev, qr, nblk);
@@ -849,34 +853,40 @@ __device__ float M8LaneScore(uint32_t rate, const uint8_t* pb, const uint8_t* fr
     }
 }
 
+// LDS of a refine workgroup: the level tables of the runs staged there
+// (lev_floats floats, run r at lev_off[r]), then each wave's u8 LUT.
 struct M8RefLayout {
-    uint32_t q, lut, lev, total;
-    __host__ __device__ M8RefLayout(uint32_t P, uint32_t PC, uint32_t LV) {
-        q = 0;
-        lut = P * 4;
-        lev = (lut + PC * 32 + 15) & ~15u;
-        total = (lev + LV * 4 + 15) & ~15u;
+    uint32_t lut, total;
+    __host__ __device__ M8RefLayout(uint32_t PC, uint32_t LV) {
+        lut = (LV * 4 + 15) & ~15u;
+        total = lut + kM8RefineWaves * PC * 32;
     }
 };
 
+// Orders this wave's LDS accesses before and after it (a wave's LDS
+// operations are performed in order; this keeps the compiler from moving
+// them across).
+__device__ __forceinline__ void WaveLdsSync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <bool kFirst>
-__global__ __launch_bounds__(64) void M8RefineKernel(M8BatchArgs a) {
+__global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8BatchArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const M8View& v = a.v;
     const uint32_t P = static_cast<uint32_t>(v.padded), PC = v.plane_cols, R = v.nrun, k = a.k, maxp = a.maxp;
-    const M8RefLayout L(P, PC, a.lev_floats);
-    float* sq = reinterpret_cast<float*>(smem + L.q);
-    uint8_t* slut = smem + L.lut;
-    float* slev = reinterpret_cast<float*>(smem + L.lev);
-    const uint32_t lane = threadIdx.x;
-    {   // level tables of the runs at rates <= kM8LdsLevRate, run after run
-        uint32_t o = 0;
+    const M8RefLayout L(PC, a.ref_lev_floats);
+    const uint32_t lane = threadIdx.x & 63, wave = Uniform(threadIdx.x >> 6);
+    uint8_t* slut = smem + L.lut + wave * PC * 32;   // this wave's query LUT
+    {   // the staged level tables, by the whole workgroup
+        float* slev = reinterpret_cast<float*>(smem);
         for (uint32_t r = 0; r < R; ++r) {
-            if (v.runs[r].rate > kM8LdsLevRate) continue;
+            if (a.ref_lev_off[r] == kM8LevGlobal) continue;
             const uint32_t nq2 = 2u << v.runs[r].rate;
-            for (uint32_t e = lane; e < nq2; e += 64) slev[o + e] = v.runs[r].lev[e];
-            o += nq2;
+            for (uint32_t e = threadIdx.x; e < nq2; e += 64 * kM8RefineWaves) slev[a.ref_lev_off[r] + e] = v.runs[r].lev[e];
         }
+        __syncthreads();
     }
     const float finf = __builtin_inff();
     const double dinf = __builtin_inf();
@@ -884,38 +894,47 @@ __global__ __launch_bounds__(64) void M8RefineKernel(M8BatchArgs a) {
     const uint32_t ntask = Uniform(a.toff[a.nq]);
     uint32_t qcur = 0xFFFFFFFFu;
     unsigned long long n_ref = 0, n_surv = 0, n_bits = 0;
+    // Tasks are taken kM8RefineGrab at a time (consecutive tasks mostly share
+    // a query: one search for it, its LDS data reused).
+    uint32_t task = 0, tend = 0, q = 0, qend = 0;
     for (;;) {
-        uint32_t task = 0;
-        if (lane == 0) task = atomicAdd(&a.counters[3], 1u);
-        task = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(task), 0)));
-        if (task >= ntask) break;
-        uint32_t q = 0;
-        if (lane == 0) {   // the task's query: the last q with toff[q] <= task
-            uint64_t lo = 0, hi = a.nq;
-            while (hi - lo > 1) {
-                const uint64_t mid = (lo + hi) >> 1;
-                if (a.toff[mid] <= task) lo = mid;
-                else hi = mid;
+        if (task == tend) {
+            uint32_t t0 = 0;
+            if (lane == 0) t0 = atomicAdd(&a.counters[3], kM8RefineGrab);
+            task = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(t0), 0)));
+            if (task >= ntask) break;
+            tend = min(ntask, task + kM8RefineGrab);
+            uint32_t qs = 0;
+            if (lane == 0) {   // the task's query: the last q with toff[q] <= task
+                uint64_t lo = 0, hi = a.nq;
+                while (hi - lo > 1) {
+                    const uint64_t mid = (lo + hi) >> 1;
+                    if (a.toff[mid] <= task) lo = mid;
+                    else hi = mid;
+                }
+                qs = static_cast<uint32_t>(lo);
             }
-            q = static_cast<uint32_t>(lo);
+            q = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(qs), 0)));
+            qend = Uniform(a.toff[q + 1]);
         }
-        q = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(q), 0)));
+        while (task >= qend) {   // the next query with tasks (toff is non-decreasing)
+            ++q;
+            qend = Uniform(a.toff[q + 1]);
+        }
         if (q != qcur) {
-            __syncthreads();   // the previous query's data is no longer read
-            for (uint32_t d = lane; d < P; d += 64) sq[d] = a.rotq[static_cast<uint64_t>(q) * P + d];
+            WaveLdsSync();   // the previous query's LUT is no longer read
             for (uint32_t e = lane; e < PC * 8; e += 64)
                 reinterpret_cast<uint32_t*>(slut)[e] =
                     reinterpret_cast<const uint32_t*>(a.lut + static_cast<uint64_t>(q) * PC * 32)[e];
             qcur = q;
-            __syncthreads();
+            WaveLdsSync();
         }
         const uint32_t nc = a.nfill[q];
         const uint32_t c0 = (task - a.toff[q]) * kM8RefineChunk;
         const uint32_t c1 = min(nc, c0 + kM8RefineChunk);
         uint64_t* slots = a.pool + a.coff[q] + c0;   // this task's candidates, then its survivors
         const bool has = c0 + lane < c1;
-        const uint64_t ent = has ? slots[lane] : 0ull;
-        __syncthreads();   // every candidate is read before survivors are written over them
+        const uint64_t ent = has ? slots[lane] : 0ull;   // read before survivors are written over it
         const float tau = a.tau[q];
         const double taud = tau < finf ? static_cast<double>(tau) : dinf;
         const float sqk = tau < finf ? sqrtf(fmaxf(0.0F, tau)) : finf;
@@ -1024,15 +1043,19 @@ __global__ __launch_bounds__(64) void M8RefineKernel(M8BatchArgs a) {
             double rem2 = pool;
             live = true;
             uint64_t bits = 0;
-            uint32_t lo_off[kM8BatchRuns] = {0, 0, 0, 0};
-            for (uint32_t r = 1; r < R; ++r)
-                lo_off[r] = lo_off[r - 1] + (v.runs[r - 1].rate <= kM8LdsLevRate ? 2u << v.runs[r - 1].rate : 0u);
+            const E8Q_GLOBAL float* qrot = G(a.rotq) + static_cast<uint64_t>(q) * P;
             const uint32_t nsteps = kFirst ? 1u : R;
             for (uint32_t oi = 0; oi < nsteps && live; ++oi) {
                 const uint32_t r = ord[oi];
                 const M8Run& ru = v.runs[r];
-                const float* lev = ru.rate <= kM8LdsLevRate ? slev + lo_off[r] : ru.lev;
-                const double se = M8LaneScore(ru.rate, prow + ru.blk0, row + ru.ref_off, lev, sq + ru.dim_off, ru.nblk);
+                const E8Q_GLOBAL uint8_t* pb = G(prow) + ru.blk0;
+                const E8Q_GLOBAL uint8_t* fr = G(row) + ru.ref_off;
+                const uint32_t lo = a.ref_lev_off[r];
+                const double se =
+                    lo != kM8LevGlobal
+                        ? M8LaneScore(ru.rate, pb, fr, (const E8Q_LDS float*)(reinterpret_cast<const float*>(smem)) + lo,
+                                      qrot + ru.dim_off, ru.nblk)
+                        : M8LaneScore(ru.rate, pb, fr, G(ru.lev), qrot + ru.dim_off, ru.nblk);
                 const double coef = f4[r * 4];
                 if (R == 1) {
                     dist = fma(-(2.0 * coef), se, dist);
@@ -1060,6 +1083,7 @@ __global__ __launch_bounds__(64) void M8RefineKernel(M8BatchArgs a) {
             base = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(base), 0)));
             if (live) a.cont[a.coff[q] + base + __popcll(m & ((uint64_t{1} << lane) - 1))] = ent;
             if (lane == 0) a.tsurv[task] = 0u;
+            ++task;
             continue;
         }
         // survivors, compacted over the task's first slots
@@ -1069,6 +1093,7 @@ __global__ __launch_bounds__(64) void M8RefineKernel(M8BatchArgs a) {
             ++n_surv;
         }
         if (lane == 0) a.tsurv[task] = static_cast<uint32_t>(__popcll(m));
+        ++task;
     }
     // per-wave totals
     for (int o = 32; o > 0; o >>= 1) {
@@ -1083,20 +1108,18 @@ __global__ __launch_bounds__(64) void M8RefineKernel(M8BatchArgs a) {
     }
 }
 
-size_t M8RefineLds(const M8View& v, uint32_t lev_floats) {
-    return M8RefLayout(static_cast<uint32_t>(v.padded), v.plane_cols, lev_floats).total;
-}
+size_t M8RefineLds(const M8View& v, uint32_t lev_floats) { return M8RefLayout(v.plane_cols, lev_floats).total; }
 
 hipError_t LaunchM8Refine(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool first) {
-    const size_t lds = M8RefineLds(a.v, a.lev_floats);
+    const size_t lds = M8RefineLds(a.v, a.ref_lev_floats);
     const void* fn = first ? reinterpret_cast<const void*>(&M8RefineKernel<true>)
                            : reinterpret_cast<const void*>(&M8RefineKernel<false>);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
     }
-    if (first) hipLaunchKernelGGL(M8RefineKernel<true>, dim3(grid), dim3(64), lds, st, a);
-    else hipLaunchKernelGGL(M8RefineKernel<false>, dim3(grid), dim3(64), lds, st, a);
+    if (first) hipLaunchKernelGGL(M8RefineKernel<true>, dim3(grid), dim3(64 * kM8RefineWaves), lds, st, a);
+    else hipLaunchKernelGGL(M8RefineKernel<false>, dim3(grid), dim3(64 * kM8RefineWaves), lds, st, a);
     return hipGetLastError();
 }
 

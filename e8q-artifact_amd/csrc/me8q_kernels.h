@@ -78,6 +78,10 @@ constexpr uint32_t kM8BatchRuns = 4;     // coded runs the batched form supports
 constexpr uint32_t kM8BatchQTile = 32;   // pairs per work item (the MFMA's M)
 constexpr uint32_t kM8BatchBTile = 8;    // batches per work item (4 waves x 2 tiles)
 constexpr uint32_t kM8RefineChunk = 64;  // candidates per refine task
+constexpr uint32_t kM8RefineGrab = 8;    // refine tasks a wave takes at a time
+constexpr uint32_t kM8RefineWaves = 4;   // waves of a refine workgroup (one LDS copy of the level tables)
+constexpr uint32_t kM8RefLevMax = 12288; // level-table floats (48 KiB) the refine stages in LDS
+constexpr uint32_t kM8LevGlobal = 0xFFFFFFFFu;   // a run whose level table the refine reads from global memory
 constexpr uint32_t kM8SelMaxK = 512;     // top-k of the selection kernel
 constexpr uint32_t kM8TauBins = 64;      // histogram of candidates' coarse estimates over [0, tau0)
 constexpr uint32_t kM8SetA = 2048;       // candidates per query refined first (their exact top-k gives tau1)
@@ -126,6 +130,10 @@ struct M8BatchArgs {
     uint32_t* chk;            // optional [nq]: sum over scored pairs of (pos+1) * sum_r (r+1) * coarse_sum_r, mod 2^32
     uint32_t lev_floats;
     uint32_t set_a;           // candidates per query of set A (kM8SetA)
+    // Refine: each run's level table offset in LDS (floats; kM8LevGlobal: read
+    // from global memory) and the staged total, at most kM8RefLevMax.
+    uint32_t ref_lev_off[kM8BatchRuns];
+    uint32_t ref_lev_floats;
     // Records of the count pass (null: none written): each passing (pair,
     // vector) as {position in its list, tau0 bin << 26 | its lower bound
     // rounded down to 26 bits}, a pair's records of an item contiguous.

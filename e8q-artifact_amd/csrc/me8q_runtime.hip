@@ -369,6 +369,18 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         ba.stats = b.bstats;
         ba.chk = chk ? b.chk : nullptr;
         ba.lev_floats = sa.lev_floats;
+        {   // the level tables the refine stages in LDS: run by run while they fit
+            uint32_t o = 0;
+            for (uint32_t r = 0; r < kM8BatchRuns; ++r) ba.ref_lev_off[r] = kM8LevGlobal;
+            for (uint32_t r = 0; r < v.nrun && r < kM8BatchRuns; ++r) {
+                const uint32_t nq2 = 2u << x->runs[r].rate;
+                if (o + nq2 <= kM8RefLevMax) {
+                    ba.ref_lev_off[r] = o;
+                    o += nq2;
+                }
+            }
+            ba.ref_lev_floats = o;
+        }
         ba.set_a = a->batched_set_a ? a->batched_set_a : kM8SetA;
         const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
         // records of the count pass, when the reserve holds any
@@ -433,14 +445,14 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
                 r.toff = b.toff;
                 Check(LaunchFill32(st, b.ncont, nq, 0), "clear");
                 Check(LaunchM8RefineTasks(st, r, r.nfill, false), "refine tasks");
-                Check(LaunchM8Refine(st, r, 24 * cus, true), "refine, first runs");
+                Check(LaunchM8Refine(st, r, 8 * cus, true), "refine, first runs");
                 if (v.nrun > 1) {
                     r.pool = r.cont;
                     r.nfill = b.ncont;
                     r.toff = b.toff2;
                     r.tsurv = tsurv2;
                     Check(LaunchM8RefineTasks(st, r, b.ncont, false), "refine tasks");
-                    Check(LaunchM8Refine(st, r, 24 * cus, false), "refine, cascades");
+                    Check(LaunchM8Refine(st, r, 8 * cus, false), "refine, cascades");
                 }
                 r.out_ids = sel_ids;
                 r.out_d = sel_d;
