@@ -132,6 +132,7 @@ __global__ void QueryPrepKernel(const float* __restrict__ rotq, uint64_t nq, Ind
 // loads and 16 LDS reads feed 64 FMAs.  Each (query, centroid) pair keeps
 // its own chain and reduction order.
 constexpr int kRouteCT = 4;   // centroids per 32-lane group
+constexpr int kRoutePf = 4;   // dimension steps of centroid loads in flight
 
 __global__ void RouteNoDimKernel(RouteArgs ra) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -163,16 +164,55 @@ __global__ __launch_bounds__(256) void RouteDistKernel(RouteArgs ra) {
     for (int u = 0; u < kRouteCT; ++u)
 #pragma unroll
         for (int t = 0; t < kRouteQT; ++t) acc[u][t] = 0.0F;
-    for (uint32_t d = lane; d < D; d += 32) {
-        float cv[kRouteCT];
-#pragma unroll
-        for (int u = 0; u < kRouteCT; ++u) cv[u] = cc[u][d];
+    // The chains step d = lane, lane + 32, ...: full steps (every lane in
+    // range) with the centroid values loaded kRoutePf steps ahead in a
+    // register ring, then the last partial step.
+    auto step = [&](uint32_t d, const float* cv) {
 #pragma unroll
         for (int t = 0; t < kRouteQT; ++t) {
             const float qv = qs[t * D + d];
 #pragma unroll
             for (int u = 0; u < kRouteCT; ++u) acc[u][t] = fmaf(qv, cv[u], acc[u][t]);
         }
+    };
+    const uint32_t nfull = D / 32;
+    uint32_t m = 0;
+    if (nfull >= static_cast<uint32_t>(kRoutePf)) {
+        float ring[kRoutePf][kRouteCT];
+#pragma unroll
+        for (int j = 0; j < kRoutePf; ++j)
+#pragma unroll
+            for (int u = 0; u < kRouteCT; ++u) ring[j][u] = cc[u][lane + 32 * j];
+        for (; m + 2 * kRoutePf <= nfull; m += kRoutePf) {   // ring: steps m .. m + kRoutePf - 1
+            float next[kRoutePf][kRouteCT];   // steps m + kRoutePf .., loaded before this group's FMAs
+#pragma unroll
+            for (int j = 0; j < kRoutePf; ++j)
+#pragma unroll
+                for (int u = 0; u < kRouteCT; ++u) next[j][u] = cc[u][lane + 32 * (m + j + kRoutePf)];
+#pragma unroll
+            for (int j = 0; j < kRoutePf; ++j) step(lane + 32 * (m + j), ring[j]);
+#pragma unroll
+            for (int j = 0; j < kRoutePf; ++j)
+#pragma unroll
+                for (int u = 0; u < kRouteCT; ++u) ring[j][u] = next[j][u];
+        }
+#pragma unroll
+        for (int j = 0; j < kRoutePf; ++j) step(lane + 32 * (m + j), ring[j]);
+        m += kRoutePf;
+    }
+    for (; m < nfull; ++m) {
+        const uint32_t d = lane + 32 * m;
+        float cv[kRouteCT];
+#pragma unroll
+        for (int u = 0; u < kRouteCT; ++u) cv[u] = cc[u][d];
+        step(d, cv);
+    }
+    if (lane + 32 * nfull < D) {
+        const uint32_t d = lane + 32 * nfull;
+        float cv[kRouteCT];
+#pragma unroll
+        for (int u = 0; u < kRouteCT; ++u) cv[u] = cc[u][d];
+        step(d, cv);
     }
 #pragma unroll
     for (int u = 0; u < kRouteCT; ++u) {
