@@ -384,7 +384,7 @@ hipError_t LaunchRouteDist(hipStream_t st, const RouteArgs& ra) {
         return hipGetLastError();
     };
     // 16 queries per tile; 4 above D = 2560 (the query tile lives in LDS).
-    if (16ull * ra.dim * sizeof(float) <= 160 * 1024) return go(RouteDistKernel<16>, 16);
+    if (8ull * ra.dim * sizeof(float) <= 160 * 1024) return go(RouteDistKernel<8>, 8);
     return go(RouteDistKernel<4>, 4);
 }
 

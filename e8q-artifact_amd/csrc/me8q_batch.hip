@@ -73,7 +73,7 @@ __device__ __forceinline__ float M8KeyDist(uint32_t m) {
 // the value read back is never above the bound.
 __device__ __forceinline__ uint32_t M8RecPack(float dm, uint32_t bin) {
     const uint32_t u = __float_as_uint(dm), m = u & 0x7FFFFFFFu, s = u >> 31;
-    return (bin << 26) | (s << 25) | (s ? min((m + 63u) >> 6, 0x1FFFFFFu) : m >> 6);
+    return (bin << 26) | (s << 25) | min((m + (63u & (0u - s))) >> 6, 0x1FFFFFFu);
 }
 __device__ __forceinline__ float M8RecBound(uint32_t w) {
     return __uint_as_float(((w >> 25) & 1u) << 31 | (w & 0x1FFFFFFu) << 6);
@@ -278,12 +278,12 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             uint32_t lo[NT], hi[NT];
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const uint64_t x = (w[t] >> (4 * h)) & 0x0F0F0F0F0F0F0F0Full;
-                lo[t] = static_cast<uint32_t>(x) << 4;
-                hi[t] = static_cast<uint32_t>(x >> 32) << 4;
+                const uint64_t x = ((w[t] >> (4 * h)) & 0x0F0F0F0F0F0F0F0Full) << 4;
+                lo[t] = static_cast<uint32_t>(x);
+                hi[t] = static_cast<uint32_t>(x >> 32);
             }
-            auto brow = [&](int t, int e) -> i32x4 {
-                const uint32_t off = __builtin_amdgcn_ubfe(e < 4 ? lo[t] : hi[t], 8 * (e & 3), 8);
+            auto brow = [&](int t, int e) -> i32x4 {   // byte e & 3 of the word, zero-extended (v_perm_b32)
+                const uint32_t off = __builtin_amdgcn_perm(0u, e < 4 ? lo[t] : hi[t], 0x0C0C0C00u | (e & 3));
                 return *reinterpret_cast<const i32x4*>(s_ohb + off);
             };
             i32x4 av = *reinterpret_cast<const i32x4*>(arow);
@@ -405,7 +405,8 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             const uint64_t pos = static_cast<uint64_t>(bj) * 32 + n;
             const float4 rf = s_rowf[row];
             const float dc = fmaf(-rf.x, T[t][i], sm.e[i]);
-            const float slack = fmaf(z, sqrtf(sm.p[i]), sm.q[i]);
+            // v_sqrt_f32 (within 1 ulp): the margin's 2^-18 |slack| covers it
+            const float slack = fmaf(z, __builtin_amdgcn_sqrtf(sm.p[i]), sm.q[i]);
             dm = dc - slack - (kM8MarginRel * (fabsf(dc) + fabsf(slack)) + kM8MarginAbs * (sm.m[i] + fabsf(slack)));
             bin = static_cast<uint32_t>(fminf(fmaxf(dc, 0.0F) * rf.z, static_cast<float>(kM8TauBins - 1)));
             return bj >= __float_as_uint(rf.w) && pos < cnt && !(dm >= rf.y);
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                     const uint64_t ma = kWrite ? __ballot(kept && bin <= s_abin[row]) : 0ull;
                     if (lane == 0) {
                         s_bal[wave][t][i][0] = mk;
-                        s_bal[wave][t][i][1] = ma;
+                        if (kWrite) s_bal[wave][t][i][1] = ma;
                         n_cand += __popcll(mk);
                     }
                     if (kChk) {
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                 for (uint32_t t = 0; t < static_cast<uint32_t>(kM8Tiles); ++t) {
                     if (w + 4 * t >= nb) continue;
                     const uint32_t mk = static_cast<uint32_t>(s_bal[w][t][i][0] >> (32 * hh));
-                    const uint32_t ma = static_cast<uint32_t>(s_bal[w][t][i][1] >> (32 * hh));
+                    const uint32_t ma = kWrite ? static_cast<uint32_t>(s_bal[w][t][i][1] >> (32 * hh)) : 0u;
                     s_off[w][t][m][0] = ta;
                     s_off[w][t][m][1] = tb;
                     ta += __popc(mk & ma);

@@ -84,7 +84,7 @@ constexpr uint32_t kM8RefLevMax = 12288; // level-table floats (48 KiB) the refi
 constexpr uint32_t kM8LevGlobal = 0xFFFFFFFFu;   // a run whose level table the refine reads from global memory
 constexpr uint32_t kM8SelMaxK = 512;     // top-k of the selection kernel
 constexpr uint32_t kM8TauBins = 64;      // histogram of candidates' coarse estimates over [0, tau0)
-constexpr uint32_t kM8SetA = 2048;       // candidates per query refined first (their exact top-k gives tau1)
+constexpr uint32_t kM8SetA = 1024;       // candidates per query refined first (their exact top-k gives tau1)
 
 struct M8BatchArgs {
     M8View v;

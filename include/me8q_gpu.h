@@ -69,7 +69,7 @@ typedef struct {
                                          indexes and k run the plain form (stats.form says which ran).
                                          No reference counterpart: the reference searches query by query. */
     uint32_t batched_budget;          /* threshold pass length in 32-vector batches (0: 16) */
-    uint32_t batched_set_a;           /* candidates per query refined first for the exact threshold (0: 2048) */
+    uint32_t batched_set_a;           /* candidates per query refined first for the exact threshold (0: 1024) */
 } e8q_m8_search_args;
 
 typedef struct {
