@@ -142,6 +142,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     __shared__ uint32_t s_off[4][kM8Tiles][kM8BatchQTile][2];   // a tile's first append of each pair (A, B)
     __shared__ uint32_t s_base[kM8BatchQTile][2];        // each pair's first slot of the item (A, B)
     __shared__ uint32_t s_rs[kM8BatchQTile];             // count pass: each pair's first record of the item
+    __shared__ uint32_t s_minbin[kM8BatchQTile];         // count pass: each pair's smallest tau0 bin
     __shared__ unsigned long long s_rb;                  // count pass: the item's first record
     // count pass: each (wave, tile, accumulator row, lane)'s record word
     __shared__ uint32_t s_rw[kWrite ? 1 : 4][kM8Tiles][16][64];
@@ -474,6 +475,17 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
             // Records: the item's block of the record array (one atomic), each
             // pair's records contiguous in it, a pair's tiles in (wave, tile)
             // order, the words staged in LDS by the test above.
+            if (wave == 0) {   // each pair's smallest tau0 bin: its lowest non-empty histogram bin
+                uint32_t mb = ~0u;
+                if (!kChk)
+                    for (uint32_t r = 0; r < kM8BatchQTile; ++r) {
+                        const uint64_t nz = __ballot(s_hist[r * kM8TauBins + lane] != 0u);   // lane = bin
+                        if (lane == r) mb = nz ? static_cast<uint32_t>(__builtin_ctzll(nz)) : ~0u;
+                    }
+                else
+                    mb = 0u;   // no histogram in the checksum pass: no unit is skipped
+                if (lane < kM8BatchQTile) s_minbin[lane] = mb;
+            }
             if (tid < kM8BatchQTile) {   // wave 0, lanes 0..31 = the pairs (count 0 past np)
                 const uint32_t tot = ta + tb;
                 uint32_t x = tot;
@@ -489,7 +501,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                 }
                 const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(rb)), 0, 32));
                 s_rs[n] = x - tot;
-                a.itab[static_cast<uint64_t>(it) * kM8BatchQTile + n] = make_uint2(lo + x - tot, tot);
+                a.itab[static_cast<uint64_t>(it) * kM8BatchQTile + n] = make_uint2(lo + x - tot, tot | (s_minbin[n] << 16));
             }
             __syncthreads();
             const unsigned long long rb = s_rb;
@@ -601,11 +613,12 @@ __global__ __launch_bounds__(256) void M8PlaceKernel(M8BatchArgs a, int set_b) {
         const E8Q_GLOBAL BatchItem* item = G(a.items) + it;
         if (m >= Uniform(item->np)) continue;
         const E8Q_GLOBAL uint2* ep = G(a.itab) + u;
-        const uint32_t start = Uniform(ep->x), cnt = Uniform(ep->y);
+        const uint32_t start = Uniform(ep->x), cnt = Uniform(ep->y) & 0xFFFFu, minbin = Uniform(ep->y) >> 16;
         if (cnt == 0) continue;
         const uint64_t pr = G(a.pairs)[Uniform(item->p0) + m];
         const uint32_t q = static_cast<uint32_t>(pr >> 32), p = static_cast<uint32_t>(pr);
         const uint32_t abin = G(a.abin)[q], na = G(a.na)[q];
+        if (!set_b && minbin > abin) continue;   // none of its records is in set A
         const uint32_t room = set_b ? G(a.ncand)[q] - na : na;
         const uint64_t base = G(a.coff)[q] + (set_b ? na : 0u);
         float lim = 0.0F;

@@ -140,7 +140,8 @@ struct M8BatchArgs {
     uint2* rec;
     uint64_t rec_cap;         // records that fit; past it the count goes on and nothing is written
     unsigned long long* rcount;   // [1] records appended (all of them, also past rec_cap)
-    uint2* itab;              // [items_cap * kM8BatchQTile] per (item, pair): {first record, count}
+    uint2* itab;              // [items_cap * kM8BatchQTile] per (item, pair): {first record,
+                              // count | smallest tau0 bin of its records << 16 (0xFFFF: none binned)}
 };
 // write = 0: count each query's candidates (ncand), and with a.rec write
 // their records; write = 1: append them to the pool at coff[q] (same
