@@ -126,13 +126,14 @@ __global__ void QueryPrepKernel(const float* __restrict__ rotq, uint64_t nq, Ind
 // Route distances d2[q][c] = (qn2 + cn2) - 2*dot, dot in the AVX2
 // RouteGemv order: lane l of a 32-lane group accumulates the FMA chain over
 // d = l, l+32, ...; then ((a0+a1)+(a2+a3)) per 8-lane slice and the Hsum.
-// Grid: x = centroid tiles of 32 (eight 32-lane groups of four centroids),
-// y = query tiles of 16 (staged in LDS).  A lane runs the chains of its four
-// centroids against the 16 queries at once: per dimension step four centroid
-// loads and 16 LDS reads feed 64 FMAs.  Each (query, centroid) pair keeps
-// its own chain and reduction order.
-constexpr int kRouteCT = 4;   // centroids per 32-lane group
-constexpr int kRoutePf = 4;   // dimension steps of centroid loads in flight
+// Grid: x = centroid tiles of 8 kRouteCT (eight 32-lane groups of kRouteCT
+// centroids), y = query tiles of kRouteQT (staged in LDS).  A lane runs the
+// chains of its kRouteCT centroids against the kRouteQT queries at once: per
+// dimension step kRouteCT centroid loads and kRouteQT LDS reads feed
+// kRouteCT * kRouteQT FMAs.  Each (query, centroid) pair keeps its own chain
+// and reduction order.
+constexpr int kRouteCT = 8;   // centroids per 32-lane group
+constexpr int kRoutePf = 2;   // dimension steps of centroid loads in flight
 
 __global__ void RouteNoDimKernel(RouteArgs ra) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
