@@ -36,6 +36,10 @@ hipError_t LaunchQueryPrep(hipStream_t st, const float* rotq, uint64_t nq, const
                            float* consts);
 hipError_t LaunchRoute(hipStream_t st, const float* rotq, const float* consts, uint64_t nq, const IndexView& ix,
                        float* dist);
+// Route + top-nprobe of the E8Q index (LaunchRouteSelect over its route arguments).
+hipError_t LaunchRouteProbes(hipStream_t st, const float* rotq, const float* consts, uint64_t nq, const IndexView& ix,
+                             float* dist, const uint32_t* nprobe_q, uint32_t nprobe_all, uint32_t maxp,
+                             uint32_t* probe_ids, float* probe_d2);
 // Route distances over the first `dim` coordinates of rows `cstride` apart
 // (ivf_route.cpp:9-27; ME8Q routes over its plan's coded prefix).
 struct RouteArgs {
@@ -51,6 +55,13 @@ struct RouteArgs {
     float* dist;              // [nq][nlist]
 };
 hipError_t LaunchRouteDist(hipStream_t st, const RouteArgs& ra);
+// Route distances and the top-nprobe per query in one step: the two-stage
+// selection (approximate distances by an fp32 MFMA contraction, the
+// reference-order distances of the candidates its error bound leaves) where
+// the shape allows, else LaunchRouteDist + LaunchSelectProbes; the same probe
+// lists and distances either way.  ra.dist is scratch [nq][nlist].
+hipError_t LaunchRouteSelect(hipStream_t st, const RouteArgs& ra, const uint32_t* nprobe_q, uint32_t nprobe_all,
+                             uint32_t maxp, uint32_t* probe_ids, float* probe_d2);
 size_t SelectProbesLds(uint32_t nlist, uint32_t maxp);
 hipError_t LaunchSelectProbes(hipStream_t st, const float* dist, uint64_t nq, uint32_t nlist,
                               const uint32_t* nprobe_q, uint32_t nprobe_all, uint32_t maxp, uint32_t* probe_ids,

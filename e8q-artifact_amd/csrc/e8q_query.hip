@@ -133,6 +133,16 @@ __global__ void QueryPrepKernel(const float* __restrict__ rotq, uint64_t nq, Ind
 // kRouteCT * kRouteQT FMAs.  Each (query, centroid) pair keeps its own chain
 // and reduction order.
 constexpr int kRouteCT = 8;   // centroids per 32-lane group
+
+// The 32-lane reduction of the chains, lane 0 holding the dot product.
+__device__ __forceinline__ float RouteReduce32(float v) {
+    v = v + __shfl_down(v, 8, 32);    // lanes 0..7: a0+a1, lanes 16..23: a2+a3
+    v = v + __shfl_down(v, 16, 32);   // lanes 0..7: (a0+a1)+(a2+a3)
+    v = v + __shfl_down(v, 4, 32);    // lanes 0..3: s[i]+s[i+4]
+    v = v + __shfl_down(v, 2, 32);    // lanes 0..1
+    v = v + __shfl_down(v, 1, 32);    // lane 0
+    return v;
+}
 constexpr int kRoutePf = 2;   // dimension steps of centroid loads in flight
 
 __global__ void RouteNoDimKernel(RouteArgs ra) {
@@ -220,12 +230,7 @@ __global__ __launch_bounds__(256) void RouteDistKernel(RouteArgs ra) {
         const uint64_t c = c0 + u;
 #pragma unroll
         for (int t = 0; t < kRouteQT; ++t) {
-            float v = acc[u][t];
-            v = v + __shfl_down(v, 8, 32);    // lanes 0..7: a0+a1, lanes 16..23: a2+a3
-            v = v + __shfl_down(v, 16, 32);   // lanes 0..7: (a0+a1)+(a2+a3)
-            v = v + __shfl_down(v, 4, 32);    // lanes 0..3: s[i]+s[i+4]
-            v = v + __shfl_down(v, 2, 32);    // lanes 0..1
-            v = v + __shfl_down(v, 1, 32);    // lane 0
+            const float v = RouteReduce32(acc[u][t]);
             if (lane == 0 && c < ra.nlist && t < nqt) {
                 const float qn2 = ra.qn2[(q0 + t) * ra.qn2_stride];
                 float d2 = fmaf(-v, 2.0F, ra.cn2[c] + qn2);
@@ -325,6 +330,179 @@ __global__ void SelectProbesKernel(const float* __restrict__ dist, uint64_t nq, 
 }
 
 // ---------------------------------------------------------------------------
+// Two-stage probe selection, with the same result as RouteDistKernel +
+// SelectProbesKernel.  Stage 1 (RouteApproxKernel): every (query, centroid)
+// distance as an fp32 MFMA contraction, a = max(0, qn2 + cn2 - 2 dot).
+// Stage 2 (SelectProbesTwoStageKernel), per query: the m-th smallest key
+// over a; U = the largest a + E over those m, where E = errf (qn2 + cn2)
+// bounds |a - d| for d the reference-order distance (both within
+// (D/2 + D/32 + 13) 2^-24 (qn2 + cn2); errf is four times that); the m
+// smallest d are at most U, so every centroid with d <= U has a - E <= U:
+// those candidates get d recomputed in the reference's chain order
+// (RouteDistKernel's chains and reduction), the rest the largest key, and
+// the exact top-m is selected from the keys as before.
+
+typedef float f32x16r __attribute__((ext_vector_type(16)));
+
+// Workgroup: 32 queries x 128 centroids, one 32 x 32 tile per wave; lane l
+// supplies row l % 32 and dimensions h * D/2 + s (h = l / 32) of the K = 2
+// operand, so each lane streams its own row as float4 loads.
+__global__ __launch_bounds__(256) void RouteApproxKernel(RouteArgs ra) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 31, h = lane >> 5;
+    const uint32_t half = ra.dim / 2, n4 = half / 4;
+    const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * 32;
+    const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * 128 + wave * 32;
+    const uint64_t qa = min<uint64_t>(q0 + r, ra.nq - 1), ca = min<uint64_t>(c0 + r, ra.nlist - 1);
+    const float4* qp = reinterpret_cast<const float4*>(ra.rotq + qa * ra.qstride + h * half);
+    const float4* cp = reinterpret_cast<const float4*>(ra.cent + ca * ra.cstride + h * half);
+    f32x16r acc = {};
+    float4 qv = qp[0], cv = cp[0];
+    for (uint32_t s4 = 0; s4 < n4; ++s4) {
+        const uint32_t nx = s4 + 1 < n4 ? s4 + 1 : s4;   // the next step's operands (the last one reloads)
+        const float4 qn = qp[nx], cn = cp[nx];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv.x, cv.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv.y, cv.y, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv.z, cv.z, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv.w, cv.w, acc, 0, 0, 0);
+        qv = qn;
+        cv = cn;
+    }
+    const uint64_t c = c0 + r;   // accumulator register i: query row 8 (i / 4) + 4 h + i % 4, centroid c
+    if (c >= ra.nlist) return;
+    const float cn2 = ra.cn2[c];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint64_t q = q0 + 8 * (i / 4) + 4 * h + (i % 4);
+        if (q < ra.nq) ra.dist[q * ra.nlist + c] = fmaxf(0.0F, fmaf(-2.0F, acc[i], cn2 + ra.qn2[q * ra.qn2_stride]));
+    }
+}
+
+// The m-th smallest key (bits[c] << 32 | c) over nlist, m >= 1, by an 8-bit
+// radix select from the top (keys are unique).
+__device__ uint64_t RadixSelectKey(const uint32_t* bits, uint32_t nlist, uint32_t m, uint32_t* hist,
+                                   uint32_t* s_pick) {
+    uint64_t prefix = 0, pmask = 0;
+    uint32_t need = m;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < nlist; c += blockDim.x) {
+            const uint64_t key = KeyOf(bits, c);
+            if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0, b = 0;
+            for (; b < 256; ++b) {
+                if (acc + hist[b] >= need) break;
+                acc += hist[b];
+            }
+            s_pick[0] = b;
+            s_pick[1] = need - acc;
+        }
+        __syncthreads();
+        prefix |= static_cast<uint64_t>(s_pick[0]) << shift;
+        pmask |= static_cast<uint64_t>(255) << shift;
+        need = s_pick[1];
+        __syncthreads();
+    }
+    return prefix;
+}
+
+__global__ void SelectProbesTwoStageKernel(RouteArgs ra, const uint32_t* __restrict__ nprobe_q, uint32_t nprobe_all,
+                                           uint32_t maxp, uint32_t* __restrict__ probe_ids,
+                                           float* __restrict__ probe_d2, float errf) {
+    extern __shared__ uint32_t sm[];
+    const uint32_t nlist = static_cast<uint32_t>(ra.nlist), D = ra.dim;
+    uint32_t* bits = sm;                                                     // nlist
+    uint64_t* sel = reinterpret_cast<uint64_t*>(sm + ((nlist + 1) & ~1u));   // pow2 >= maxp
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_pick[2];
+    __shared__ uint32_t s_cnt, s_u;
+    __shared__ uint32_t s_cand[kSelThreads], s_ncand;
+    uint32_t sortn = 1;
+    while (sortn < maxp) sortn <<= 1;
+    const uint32_t lane = threadIdx.x & 31, grp = threadIdx.x >> 5, ngrp = blockDim.x >> 5;
+    for (uint64_t qi = blockIdx.x; qi < ra.nq; qi += gridDim.x) {
+        const uint32_t m = min(nlist, nprobe_q ? nprobe_q[qi] : nprobe_all);
+        if (m > 0) {
+            const float qn2 = ra.qn2[qi * ra.qn2_stride];
+            for (uint32_t c = threadIdx.x; c < nlist; c += blockDim.x) bits[c] = FloatBits(ra.dist[qi * nlist + c]);
+            if (threadIdx.x == 0) s_u = 0u;
+            __syncthreads();
+            const uint64_t K1 = RadixSelectKey(bits, nlist, m, hist, s_pick);
+            // U: the largest a + E over the m smallest keys (non-negative floats
+            // order as their bits)
+            for (uint32_t c = threadIdx.x; c < nlist; c += blockDim.x)
+                if (KeyOf(bits, c) <= K1)
+                    atomicMax(&s_u, __float_as_uint(__uint_as_float(bits[c]) + errf * (qn2 + ra.cn2[c])));
+            __syncthreads();
+            const float U = __uint_as_float(s_u);
+            const float* qrow = ra.rotq + qi * ra.qstride;
+            // candidates, a chunk of kSelThreads centroids at a time: compacted,
+            // then their reference-order distances, one 32-lane group each
+            for (uint32_t b0 = 0; b0 < nlist; b0 += blockDim.x) {
+                if (threadIdx.x == 0) s_ncand = 0;
+                __syncthreads();
+                const uint32_t c = b0 + threadIdx.x;
+                if (c < nlist) {
+                    const float a = __uint_as_float(bits[c]);
+                    if (a - errf * (qn2 + ra.cn2[c]) <= U) s_cand[atomicAdd(&s_ncand, 1u)] = c;
+                    else bits[c] = 0xFFFFFFFFu;
+                }
+                __syncthreads();
+                const uint32_t nc = s_ncand;
+                for (uint32_t j = grp; j < nc; j += ngrp) {
+                    const uint32_t cc = s_cand[j];
+                    const float* crow = ra.cent + static_cast<uint64_t>(cc) * ra.cstride;
+                    float acc = 0.0F;
+                    for (uint32_t d = lane; d < D; d += 32) acc = fmaf(qrow[d], crow[d], acc);
+                    const float v = RouteReduce32(acc);
+                    if (lane == 0) {
+                        float d2 = fmaf(-v, 2.0F, ra.cn2[cc] + qn2);
+                        if (d2 < 0.0F) d2 = 0.0F;
+                        bits[cc] = FloatBits(d2);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const uint64_t K = m > 0 ? RadixSelectKey(bits, nlist, m, hist, s_pick) : 0ull;
+        if (threadIdx.x == 0) s_cnt = 0;
+        for (uint32_t i = threadIdx.x; i < sortn; i += blockDim.x) sel[i] = ~0ull;
+        __syncthreads();
+        if (m > 0)
+            for (uint32_t c = threadIdx.x; c < nlist; c += blockDim.x) {
+                const uint64_t key = KeyOf(bits, c);
+                if (key <= K) sel[atomicAdd(&s_cnt, 1u)] = key;
+            }
+        __syncthreads();
+        for (uint32_t k2 = 2; k2 <= sortn; k2 <<= 1) {   // bitonic sort of sortn keys (ascending)
+            for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < sortn; i += blockDim.x) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint64_t x = sel[i], y = sel[l];
+                        const bool up = (i & k2) == 0;
+                        if ((x > y) == up) {
+                            sel[i] = y;
+                            sel[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < maxp; i += blockDim.x) {
+            const bool ok = i < m;
+            probe_ids[qi * maxp + i] = ok ? static_cast<uint32_t>(sel[i]) : 0xFFFFFFFFu;
+            probe_d2[qi * maxp + i] = ok ? __uint_as_float(static_cast<uint32_t>(sel[i] >> 32)) : 0.0F;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 
 hipError_t LaunchRotate(hipStream_t st, const float* x, uint64_t rows, uint32_t dim, const IndexView& ix,
@@ -345,8 +523,7 @@ hipError_t LaunchQueryPrep(hipStream_t st, const float* rotq, uint64_t nq, const
     return hipGetLastError();
 }
 
-hipError_t LaunchRoute(hipStream_t st, const float* rotq, const float* consts, uint64_t nq, const IndexView& ix,
-                       float* dist) {
+static RouteArgs E8qRouteArgs(const float* rotq, const float* consts, uint64_t nq, const IndexView& ix, float* dist) {
     RouteArgs ra{};
     ra.rotq = rotq;
     ra.qstride = static_cast<uint32_t>(ix.padded);
@@ -359,7 +536,19 @@ hipError_t LaunchRoute(hipStream_t st, const float* rotq, const float* consts, u
     ra.cstride = static_cast<uint32_t>(ix.padded);
     ra.nq = nq;
     ra.dist = dist;
-    return LaunchRouteDist(st, ra);
+    return ra;
+}
+
+hipError_t LaunchRoute(hipStream_t st, const float* rotq, const float* consts, uint64_t nq, const IndexView& ix,
+                       float* dist) {
+    return LaunchRouteDist(st, E8qRouteArgs(rotq, consts, nq, ix, dist));
+}
+
+hipError_t LaunchRouteProbes(hipStream_t st, const float* rotq, const float* consts, uint64_t nq, const IndexView& ix,
+                             float* dist, const uint32_t* nprobe_q, uint32_t nprobe_all, uint32_t maxp,
+                             uint32_t* probe_ids, float* probe_d2) {
+    return LaunchRouteSelect(st, E8qRouteArgs(rotq, consts, nq, ix, dist), nprobe_q, nprobe_all, maxp, probe_ids,
+                             probe_d2);
 }
 
 hipError_t LaunchRouteDist(hipStream_t st, const RouteArgs& ra) {
@@ -387,6 +576,38 @@ hipError_t LaunchRouteDist(hipStream_t st, const RouteArgs& ra) {
     // 16 queries per tile; 4 above D = 2560 (the query tile lives in LDS).
     if (8ull * ra.dim * sizeof(float) <= 160 * 1024) return go(RouteDistKernel<8>, 8);
     return go(RouteDistKernel<4>, 4);
+}
+
+hipError_t LaunchRouteSelect(hipStream_t st, const RouteArgs& ra, const uint32_t* nprobe_q, uint32_t nprobe_all,
+                             uint32_t maxp, uint32_t* probe_ids, float* probe_d2) {
+    if (ra.nq == 0) return hipSuccess;
+    auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+    const bool two = ra.dim > 0 && ra.dim % 8 == 0 && ra.qstride % 4 == 0 && ra.cstride % 4 == 0 && aligned(ra.rotq) &&
+                     aligned(ra.cent) && ra.nlist >= 256 && ra.nlist < (1ull << 31) &&
+                     static_cast<uint64_t>(maxp) * 8 <= ra.nlist && ra.nq <= 65535ull * 32;
+    if (!two) {
+        const hipError_t e = LaunchRouteDist(st, ra);
+        if (e != hipSuccess) return e;
+        return LaunchSelectProbes(st, ra.dist, ra.nq, static_cast<uint32_t>(ra.nlist), nprobe_q, nprobe_all, maxp,
+                                  probe_ids, probe_d2);
+    }
+    hipLaunchKernelGGL(RouteApproxKernel, dim3(static_cast<uint32_t>((ra.nlist + 127) / 128),
+                                               static_cast<uint32_t>((ra.nq + 31) / 32)),
+                       dim3(256), 0, st, ra);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // four times the bound on |a - d|: (D/2 + D/32 + 13) 2^-24 (qn2 + cn2)
+    const float errf = static_cast<float>(4.0 * (ra.dim / 2.0 + ra.dim / 32.0 + 13.0) * 5.9604644775390625e-8);
+    const uint32_t grid = static_cast<uint32_t>(ra.nq < 65536 ? ra.nq : 65536);
+    const size_t lds = SelectProbesLds(static_cast<uint32_t>(ra.nlist), maxp);
+    if (lds > 64 * 1024) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&SelectProbesTwoStageKernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(SelectProbesTwoStageKernel, dim3(grid), dim3(kSelThreads), lds, st, ra, nprobe_q, nprobe_all,
+                       maxp, probe_ids, probe_d2, errf);
+    return hipGetLastError();
 }
 
 size_t SelectProbesLds(uint32_t nlist, uint32_t maxp) {

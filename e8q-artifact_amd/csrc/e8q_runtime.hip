@@ -747,9 +747,7 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(dstats), 8, 0), "fill");
             Check(LaunchRotate(st, qsrc, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
             Check(LaunchQueryPrep(st, rotq, nq, v, lut, consts), "query prep");
-            Check(LaunchRoute(st, rotq, consts, nq, v, dist), "route");
-            Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq, nprobe_all, maxp, pids, pd2),
-                  "select probes");
+            Check(LaunchRouteProbes(st, rotq, consts, nq, v, dist, npq, nprobe_all, maxp, pids, pd2), "route");
             if (eager && !mfma) Check(hipEventRecord(ctx->ev[1], st), "event");
             if (!listed) {
                 Check(LaunchPlainSearch(st, v, pa, dparams, upload), "plain search");

@@ -254,9 +254,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     ra.cstride = static_cast<uint32_t>(P);
     ra.nq = nq;
     ra.dist = dist;
-    Check(LaunchRouteDist(st, ra), "route");
-    Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), npq, nprobe_all, maxp, probe, pd),
-          "select probes");
+    Check(LaunchRouteSelect(st, ra, npq, nprobe_all, maxp, probe, pd), "route");
     Check(LaunchM8RunDist(st, v, rotq, rc, probe, npq, nprobe_all, maxp, nq, sqd2), "run distances");
     uint32_t* out_ids = a->outputs_on_device ? a->out_ids : oid;
     float* out_d = a->outputs_on_device ? a->out_dists : od;
@@ -898,8 +896,7 @@ e8q_status e8q_m8_debug_query(e8q_ctx* ctx, const e8q_m8index* x, const float* q
         ra.cstride = static_cast<uint32_t>(P);
         ra.nq = nq;
         ra.dist = dist;
-        Check(LaunchRouteDist(st, ra), "route");
-        Check(LaunchSelectProbes(st, dist, nq, static_cast<uint32_t>(v.nlist), nullptr, maxp, maxp, pr, pd), "select");
+        Check(LaunchRouteSelect(st, ra, nullptr, maxp, maxp, pr, pd), "route");
         std::vector<uint8_t> hl(nq * PC * 32);
         std::vector<float> hq(nq * 4);
         if (rot_q) Check(hipMemcpyAsync(rot_q, rotq, nq * P * 4, hipMemcpyDeviceToHost, st), "d");
