@@ -343,37 +343,64 @@ __global__ void SelectProbesKernel(const float* __restrict__ dist, uint64_t nq, 
 // the exact top-m is selected from the keys as before.
 
 typedef float f32x16r __attribute__((ext_vector_type(16)));
+constexpr int kApproxPf = 4;   // float4 steps of each operand loaded ahead
 
 // Workgroup: 32 queries x 128 centroids, one 32 x 32 tile per wave; lane l
 // supplies row l % 32 and dimensions h * D/2 + s (h = l / 32) of the K = 2
 // operand, so each lane streams its own row as float4 loads.
 __global__ __launch_bounds__(256) void RouteApproxKernel(RouteArgs ra) {
+    __shared__ float s_qn2[32];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 31, h = lane >> 5;
     const uint32_t half = ra.dim / 2, n4 = half / 4;
     const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * 32;
+    if (threadIdx.x < 32) s_qn2[threadIdx.x] = ra.qn2[min<uint64_t>(q0 + threadIdx.x, ra.nq - 1) * ra.qn2_stride];
+    __syncthreads();
     const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * 128 + wave * 32;
     const uint64_t qa = min<uint64_t>(q0 + r, ra.nq - 1), ca = min<uint64_t>(c0 + r, ra.nlist - 1);
     const float4* qp = reinterpret_cast<const float4*>(ra.rotq + qa * ra.qstride + h * half);
     const float4* cp = reinterpret_cast<const float4*>(ra.cent + ca * ra.cstride + h * half);
     f32x16r acc = {};
-    float4 qv = qp[0], cv = cp[0];
-    for (uint32_t s4 = 0; s4 < n4; ++s4) {
-        const uint32_t nx = s4 + 1 < n4 ? s4 + 1 : s4;   // the next step's operands (the last one reloads)
-        const float4 qn = qp[nx], cn = cp[nx];
+    auto step = [&](const float4& qv, const float4& cv) {
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv.x, cv.x, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv.y, cv.y, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv.z, cv.z, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv.w, cv.w, acc, 0, 0, 0);
-        qv = qn;
-        cv = cn;
+    };
+    // Two register sets of kApproxPf steps each: a set's loads are issued a
+    // whole set ahead of its MFMAs (no copies between iterations).
+    uint32_t s4 = 0;
+    if (n4 >= 2 * kApproxPf) {
+        float4 qa[kApproxPf], ca[kApproxPf], qb[kApproxPf], cb[kApproxPf];
+        auto load = [&](float4* qd, float4* cd, uint32_t at) {
+#pragma unroll
+            for (int j = 0; j < kApproxPf; ++j) {
+                qd[j] = qp[at + j];
+                cd[j] = cp[at + j];
+            }
+        };
+        auto run = [&](const float4* qd, const float4* cd) {
+#pragma unroll
+            for (int j = 0; j < kApproxPf; ++j) step(qd[j], cd[j]);
+        };
+        load(qa, ca, 0);
+        for (; s4 + 3 * kApproxPf <= n4; s4 += 2 * kApproxPf) {   // sets at s4 (a) and s4 + kApproxPf (b)
+            load(qb, cb, s4 + kApproxPf);
+            run(qa, ca);
+            load(qa, ca, s4 + 2 * kApproxPf);
+            run(qb, cb);
+        }
+        run(qa, ca);   // the set loaded at s4
+        s4 += kApproxPf;
     }
+    for (; s4 < n4; ++s4) step(qp[s4], cp[s4]);
     const uint64_t c = c0 + r;   // accumulator register i: query row 8 (i / 4) + 4 h + i % 4, centroid c
     if (c >= ra.nlist) return;
     const float cn2 = ra.cn2[c];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-        const uint64_t q = q0 + 8 * (i / 4) + 4 * h + (i % 4);
-        if (q < ra.nq) ra.dist[q * ra.nlist + c] = fmaxf(0.0F, fmaf(-2.0F, acc[i], cn2 + ra.qn2[q * ra.qn2_stride]));
+        const uint32_t row = 8 * (i / 4) + 4 * h + (i % 4);
+        const uint64_t q = q0 + row;
+        if (q < ra.nq) ra.dist[q * ra.nlist + c] = fmaxf(0.0F, fmaf(-2.0F, acc[i], cn2 + s_qn2[row]));
     }
 }
 
