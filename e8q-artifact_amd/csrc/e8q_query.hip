@@ -482,7 +482,8 @@ __global__ void SelectProbesTwoStageKernel(RouteArgs ra, const uint32_t* __restr
                 for (uint32_t j = grp; j < nc; j += ngrp) {
                     const uint32_t cc = s_cand[j];
                     const float* crow = ra.cent + static_cast<uint64_t>(cc) * ra.cstride;
-                    float acc = 0.0F;
+                    float acc = 0.0F;   // the chain in order; its loads issue eight steps ahead
+#pragma unroll 8
                     for (uint32_t d = lane; d < D; d += 32) acc = fmaf(qrow[d], crow[d], acc);
                     const float v = RouteReduce32(acc);
                     if (lane == 0) {
