@@ -594,55 +594,75 @@ hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid,
 }
 
 // ---------------------------------------------------------------------------
-// Placement: one wave per (item, pair) takes the pair's records of the item,
-// 64 at a time, keeps those of the set it places (set A: tau0 bin <=
-// abin[q]; B: the rest, with a lower bound below min(tau0, tau1) -- the
-// compaction's test), takes their slots with one atomic and writes the pool
-// entries there.
+// Placement: 16 lanes per (item, pair) take the pair's records of the item,
+// keep those of the set they place (set A: tau0 bin <= abin[q]; B: the
+// rest, with a lower bound below min(tau0, tau1) -- the compaction's test),
+// take their slots with one atomic and write the pool entries there.
 
 __global__ __launch_bounds__(256) void M8PlaceKernel(M8BatchArgs a, int set_b) {
-    const uint32_t lane = threadIdx.x & 63;
+    // Each 16-lane quarter of a wave takes its own unit, so a wave has four
+    // units' load chains in flight; a quarter counts its unit's kept records
+    // (ballots), takes their slots with one atomic, then writes them.
+    const uint32_t lane = threadIdx.x & 63, g = lane >> 4, gl = lane & 15;
     const uint64_t nitems = min<uint64_t>(*G(a.counters), a.items_cap);
     const uint64_t units = nitems * kM8BatchQTile;
-    const uint64_t below = (1ull << lane) - 1ull;
-    const uint64_t wstride = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+    const uint64_t below = (1ull << gl) - 1ull;
+    const uint64_t wstride = ((static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6) * 4;
     uint32_t* fill = set_b ? a.fill_b : a.fill_a;
-    for (uint64_t u = Uniform64((static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6); u < units;
-         u += wstride) {
-        const uint32_t it = static_cast<uint32_t>(u / kM8BatchQTile), m = static_cast<uint32_t>(u % kM8BatchQTile);
-        const E8Q_GLOBAL BatchItem* item = G(a.items) + it;
-        if (m >= Uniform(item->np)) continue;
-        const E8Q_GLOBAL uint2* ep = G(a.itab) + u;
-        const uint32_t start = Uniform(ep->x), cnt = Uniform(ep->y) & 0xFFFFu, minbin = Uniform(ep->y) >> 16;
-        if (cnt == 0) continue;
-        const uint64_t pr = G(a.pairs)[Uniform(item->p0) + m];
-        const uint32_t q = static_cast<uint32_t>(pr >> 32), p = static_cast<uint32_t>(pr);
-        const uint32_t abin = G(a.abin)[q], na = G(a.na)[q];
-        if (!set_b && minbin > abin) continue;   // none of its records is in set A
-        const uint32_t room = set_b ? G(a.ncand)[q] - na : na;
-        const uint64_t base = G(a.coff)[q] + (set_b ? na : 0u);
+    for (uint64_t u0 = Uniform64(((static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6) * 4); u0 < units;
+         u0 += wstride) {
+        const uint64_t u = u0 + g;
+        uint32_t start = 0, cnt = 0, q = 0, p = 0, abin = 0, room = 0;
+        uint64_t base = 0;
         float lim = 0.0F;
-        if (set_b) {
-            const float tau0 = G(a.tau)[q], t1 = G(a.first_d)[static_cast<uint64_t>(q) * a.k + a.k - 1];
-            const float tau = t1 < tau0 ? t1 : tau0;
-            lim = tau + 2.4414062e-4F * fabsf(tau);
-        }
-        for (uint32_t j = 0; j < cnt; j += 64) {
-            uint2 r = make_uint2(0u, 0u);
-            bool keep = false;
-            if (j + lane < cnt) {
-                const E8Q_GLOBAL uint2* rp = G(a.rec) + static_cast<uint64_t>(start) + j + lane;
-                r = make_uint2(rp->x, rp->y);
-                const bool in_a = (r.y >> 26) <= abin;
-                keep = set_b ? !in_a && !(M8RecBound(r.y) >= lim) : in_a;
+        if (u < units) {
+            const uint32_t it = static_cast<uint32_t>(u / kM8BatchQTile), m = static_cast<uint32_t>(u % kM8BatchQTile);
+            const E8Q_GLOBAL BatchItem* item = G(a.items) + it;
+            const E8Q_GLOBAL uint2* ep = G(a.itab) + u;
+            const uint32_t np = item->np, ey = ep->y;
+            start = ep->x;
+            cnt = m < np ? (ey & 0xFFFFu) : 0u;
+            if (cnt > 0) {
+                const uint64_t pr = G(a.pairs)[item->p0 + m];
+                q = static_cast<uint32_t>(pr >> 32);
+                p = static_cast<uint32_t>(pr);
+                abin = G(a.abin)[q];
+                const uint32_t na = G(a.na)[q];
+                if (!set_b && (ey >> 16) > abin) cnt = 0;   // none of its records is in set A
+                room = set_b ? G(a.ncand)[q] - na : na;
+                base = G(a.coff)[q] + (set_b ? na : 0u);
+                if (set_b) {
+                    const float tau0 = G(a.tau)[q], t1 = G(a.first_d)[static_cast<uint64_t>(q) * a.k + a.k - 1];
+                    const float tau = t1 < tau0 ? t1 : tau0;
+                    lim = tau + 2.4414062e-4F * fabsf(tau);
+                }
             }
-            const uint64_t mk = __ballot(keep);
-            if (mk == 0) continue;
-            uint32_t s0 = 0;
-            if (lane == 0) s0 = atomicAdd(&fill[q], static_cast<uint32_t>(__popcll(mk)));
-            s0 = static_cast<uint32_t>(__shfl(static_cast<int>(s0), 0));
+        }
+        uint32_t most = cnt;   // the quarters' largest count
+        for (int o = 16; o < 64; o <<= 1) most = max(most, static_cast<uint32_t>(__shfl_xor(static_cast<int>(most), o)));
+        auto keep_of = [&](uint32_t j, uint2& r) -> bool {
+            if (j + gl >= cnt) return false;
+            const E8Q_GLOBAL uint2* rp = G(a.rec) + static_cast<uint64_t>(start) + j + gl;
+            r = make_uint2(rp->x, rp->y);
+            const bool in_a = (r.y >> 26) <= abin;
+            return set_b ? !in_a && !(M8RecBound(r.y) >= lim) : in_a;
+        };
+        uint32_t kept = 0;
+        for (uint32_t j = 0; j < most; j += 16) {
+            uint2 r;
+            kept += static_cast<uint32_t>(__popcll((__ballot(keep_of(j, r)) >> (16 * g)) & 0xFFFFull));
+        }
+        uint32_t s0 = 0;
+        if (gl == 0 && kept != 0) s0 = atomicAdd(&fill[q], kept);
+        s0 = static_cast<uint32_t>(__shfl(static_cast<int>(s0), static_cast<int>(16 * g)));
+        if (kept == 0) continue;
+        for (uint32_t j = 0; j < most; j += 16) {
+            uint2 r = make_uint2(0u, 0u);
+            const bool keep = keep_of(j, r);
+            const uint64_t mk = (__ballot(keep) >> (16 * g)) & 0xFFFFull;
             const uint32_t slot = s0 + static_cast<uint32_t>(__popcll(mk & below));
             if (keep && slot < room) Gw(a.pool)[base + slot] = (static_cast<uint64_t>(p) << 32) | r.x;
+            s0 += static_cast<uint32_t>(__popcll(mk));
         }
     }
 }
