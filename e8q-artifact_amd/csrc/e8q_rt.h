@@ -96,6 +96,7 @@ struct e8q_ctx {
     DBuf ws;   // per-call workspace, grown on demand
     DBuf pool; // candidate pool of the batched ME8Q form, sized per call after its count pass
     DBuf recs; // records of the batched ME8Q count pass
+    DBuf tpool; // the batched ME8Q threshold pass's candidates and task counts
     uint64_t rec_want = 0;   // records the last batched ME8Q call needed beyond `recs`
 
     // Replay of a search's device sequence as a HIP graph, per call shape.

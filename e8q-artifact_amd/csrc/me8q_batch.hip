@@ -674,6 +674,55 @@ hipError_t LaunchM8Place(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bo
 }
 
 // ---------------------------------------------------------------------------
+// Threshold pass: each query's first `budget` 32-vector batches in probe
+// order (empty lists skipped, the last list possibly cut at a batch), as pool
+// entries at q * budget * 32, their count, and where the pass stopped
+// (cover[q] = probe << 32 | batch: from that probe's batch on, and in every
+// later probe, the vectors are the contraction's).  The refine's whole cascade
+// without the death test against tau = +inf then gives their exact distances,
+// and the select their top-k: the starting list and tau0.  One wave per query.
+
+__global__ __launch_bounds__(64) void M8ThresholdPoolKernel(M8BatchArgs a, const uint32_t* __restrict__ nprobe_q,
+                                                            uint32_t nprobe_all, uint32_t budget) {
+    const uint32_t lane = threadIdx.x;
+    const uint64_t cap = static_cast<uint64_t>(budget) * 32;
+    for (uint64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
+        const uint64_t np = min<uint64_t>(nprobe_q ? nprobe_q[q] : nprobe_all, a.v.nlist);
+        uint64_t used = 0, p_stop = np, b_stop = 0, n = 0;
+        const uint64_t base = q * cap;
+        for (uint64_t p = 0; p < np; ++p) {
+            const uint32_t c = a.probe[q * a.maxp + p];
+            const uint64_t cntc = a.v.list_begin[c + 1] - a.v.list_begin[c];
+            if (cntc == 0) continue;
+            const uint64_t nbl = (cntc + 31) >> 5, nb = min<uint64_t>(nbl, budget - used);
+            const uint64_t npos = min<uint64_t>(nb * 32, cntc);
+            for (uint64_t i = lane; i < npos; i += 64) a.pool[base + n + i] = (p << 32) | i;
+            n += npos;
+            used += nb;
+            if (used >= budget) {
+                p_stop = nb == nbl ? p + 1 : p;
+                b_stop = nb == nbl ? 0 : nb;
+                break;
+            }
+        }
+        if (lane == 0) {
+            a.cover[q] = (p_stop << 32) | b_stop;
+            a.nfill[q] = static_cast<uint32_t>(n);
+            a.coff[q] = base;
+        }
+    }
+}
+
+hipError_t LaunchM8ThresholdPool(hipStream_t st, const M8BatchArgs& a, const uint32_t* nprobe_q, uint32_t nprobe_all,
+                                 uint32_t budget) {
+    if (a.nq == 0) return hipSuccess;
+    if (budget == 0) return hipErrorInvalidValue;
+    const uint32_t grid = static_cast<uint32_t>(a.nq < 65536 ? a.nq : 65536);
+    hipLaunchKernelGGL(M8ThresholdPoolKernel, dim3(grid), dim3(64), 0, st, a, nprobe_q, nprobe_all, budget);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Pool offsets (coff, from the count pass's ncand) and refine tasks (toff,
 // ceil(count / kM8RefineChunk) per query); one workgroup of 1024 threads,
 // each owning a contiguous run of queries.

@@ -102,7 +102,7 @@ struct M8BatchArgs {
     int32_t cascade;          // 1: the reference's cascade bound against tau_q in the refine
     const uint32_t* first_ids;   // [nq][k] threshold pass (ascending, UINT32_MAX / +inf padded)
     const float* first_d;
-    const uint64_t* cover;       // [nq] (p << 32) | b: first (probe, batch) the threshold pass left
+    uint64_t* cover;             // [nq] (p << 32) | b: first (probe, batch) the threshold pass left
     const BatchItem* items;
     const uint64_t* pairs;
     uint64_t items_cap;
@@ -171,5 +171,10 @@ size_t M8RefineLds(const M8View& v, uint32_t lev_floats);
 // false: the whole cascade, survivors written over the task's slots.
 hipError_t LaunchM8Refine(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool first);
 hipError_t LaunchM8Select(hipStream_t st, const M8BatchArgs& a);
+// The threshold pass's pool: each query's first `budget` batches in probe
+// order at q * budget * 32 (a.pool), their count (a.nfill), offsets (a.coff)
+// and cover (a.cover).
+hipError_t LaunchM8ThresholdPool(hipStream_t st, const M8BatchArgs& a, const uint32_t* nprobe_q, uint32_t nprobe_all,
+                                 uint32_t budget);
 
 }  // namespace e8qdev

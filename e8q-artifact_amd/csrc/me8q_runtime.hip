@@ -287,14 +287,8 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         Check(hipEventRecord(ctx->ev[2], st), "event");
         Check(hipEventRecord(ctx->ev[3], st), "event");
     } else {
-        // 1. threshold pass: the plain scan over each query's first `budget` batches
-        M8ScanArgs fa = sa;
-        fa.budget = a->batched_budget ? a->batched_budget : kM8DefaultBudget;
-        fa.cover = b.cover;
-        fa.out_ids = b.fid;
-        fa.out_d = b.fd;
-        Check(LaunchM8Scan(st, fa), "threshold pass");
-        // 2. the (query, list) work list: every probe, grouped by list
+        const uint32_t budget = a->batched_budget ? a->batched_budget : kM8DefaultBudget;
+        // the (query, list) work list: every probe, grouped by list
         Check(LaunchFill32(st, b.ccount, v.nlist, 0), "clear");
         Check(LaunchFill32(st, b.ncand, nq, 0), "clear");
         Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.wstats), 8, 0), "clear");
@@ -381,6 +375,31 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         }
         ba.set_a = a->batched_set_a ? a->batched_set_a : kM8SetA;
         const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
+        {   // the threshold pass: exact distances of each query's first `budget`
+            // batches (whole cascade, no death test, tau = +inf) and their top-k
+            M8BatchArgs ta = ba;
+            const uint64_t cap = static_cast<uint64_t>(budget) * 32;
+            const uint64_t pool_b = (nq * cap * 8 + 255) & ~uint64_t{255};
+            const uint64_t need = pool_b + (nq * ((cap + kM8RefineChunk - 1) / kM8RefineChunk) + 1) * 4 + 256;
+            if (ctx->tpool.bytes < need) ctx->tpool.alloc(need);
+            ta.pool = ctx->tpool.as<uint64_t>();
+            ta.tsurv = reinterpret_cast<uint32_t*>(ctx->tpool.as<char>() + pool_b);
+            ta.coff = b.coff_b;
+            ta.nfill = b.nfill;
+            ta.toff = b.toff;
+            ta.tau = b.tau;
+            ta.cascade = 0;
+            ta.first_ids = ta.out_ids = b.fid;
+            ta.first_d = ta.out_d = b.fd;
+            Check(LaunchFill32(st, b.fid, nq * a->k, 0xFFFFFFFFu), "clear");
+            Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.fd), nq * a->k, 0x7F800000u), "clear");
+            Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.tau), nq, 0x7F800000u), "clear");
+            Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(qst), nq * 16, 0), "clear");
+            Check(LaunchM8ThresholdPool(st, ta, npq, nprobe_all, budget), "threshold pool");
+            Check(LaunchM8RefineTasks(st, ta, b.nfill, false), "threshold tasks");
+            Check(LaunchM8Refine(st, ta, 8 * cus, false), "threshold refine");
+            Check(LaunchM8Select(st, ta), "threshold select");
+        }
         // records of the count pass, when the reserve holds any
         const uint64_t rcap = M8RecordCap(ctx, w.items_cap);
         if (rcap > 0 && ctx->recs.bytes < rcap * sizeof(uint2)) ctx->recs.alloc(rcap * sizeof(uint2));
