@@ -151,6 +151,27 @@ def test_larger_index_against_oracle(ctx):
     np.testing.assert_array_equal(codes, f["codes"])
 
 
+def test_two_stage_probe_selection_against_oracle(ctx):
+    # 512 lists and nprobe <= 64 take the two-stage selection (approximate
+    # MFMA route distances, the reference's chain order for the candidates
+    # their error bound leaves): probe lists and distances must be the
+    # oracle's bit for bit, and so must the search.
+    data, cent, lab, q, oix = _oracle_case(7, 12000, 128, 512, 4.0)
+    f = me8qx.parse(oix.image())
+    ix = capi_m8.M8Index.from_file(ctx, f)
+    for nprobe in (1, 16, 64):
+        print(f"me8q two-stage route nprobe={nprobe}", flush=True)
+        _rq, _lut, _md, probe, pd = ix.query(q, nprobe)
+        for i in range(q.shape[0]):
+            _r, _l, _m, opr, opd = oix.query(q[i], nprobe)
+            np.testing.assert_array_equal(probe[i], np.asarray(opr).reshape(-1), err_msg=f"query {i}")
+            np.testing.assert_array_equal(_bits(pd[i]), _bits(np.asarray(opd, np.float32).reshape(-1)))
+    ids, d, st, _ = ix.search(q, 10, 16, _z(10, 0.0))
+    oids, od, ost = oix.search(q, 10, 16)
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(_bits(d), _bits(od))
+
+
 def test_uneven_nprobe_and_skewed_lists(ctx):
     sizes = np.array([3000, 5, 0, 700, 1, 64, 33, 2500, 31, 900], np.int64)
     data, cent, lab, q, oix = _oracle_case(77, 0, 256, 10, 4.0, spec="64:6,128:4,64:2", sizes=sizes)
