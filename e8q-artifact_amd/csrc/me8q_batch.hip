@@ -10,9 +10,9 @@
 // 856-979).  The live threshold makes each query sequential, and under Zipf
 // list sizes one query's head list holds a large share of the index.  Here:
 //
-//   1. threshold pass: the plain kernel over each query's first `budget`
-//      32-vector batches in probe order; its top-k is the query's starting
-//      list and tau_q, its k-th distance, the threshold;
+//   1. threshold pass: the exact distances of each query's first `budget`
+//      32-vector batches in probe order (no pruning); their top-k is the
+//      query's starting list and tau_q, its k-th distance, the threshold;
 //   2. the (query, list) pairs are grouped by list and cut into items of 32
 //      pairs x 8 batches; a persistent kernel takes items from an atomic
 //      counter and computes the coarse sums of all 32 x 256 pairs of an item
