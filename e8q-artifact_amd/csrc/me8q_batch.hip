@@ -1041,19 +1041,44 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
             const double* sqd2 = a.sqd2 + (static_cast<uint64_t>(q) * maxp + p) * R;
             // integer coarse sums of each run (AccumulateCoarseHi over its columns)
             int32_t part[kM8BatchRuns] = {0, 0, 0, 0};
-            for (uint32_t r = 0; r < R; ++r) {
+            auto word_sum = [&](uint64_t w, uint32_t col) -> int32_t {
                 int32_t s = 0;
-                const uint32_t b0 = v.runs[r].blk0, b1 = b0 + v.runs[r].nblk;
-                for (uint32_t col = b0; col < b1; col += 8) {
-                    const uint64_t w = *reinterpret_cast<const uint64_t*>(prow + col);
 #pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        const uint32_t byte = static_cast<uint32_t>(w >> (8 * b)) & 0xFFu;
-                        const uint8_t* lc = slut + (col + b) * 32;
-                        s += lc[byte & 15u] + lc[16 + (byte >> 4)];
+                for (int b = 0; b < 8; ++b) {
+                    const uint32_t byte = static_cast<uint32_t>(w >> (8 * b)) & 0xFFu;
+                    const uint8_t* lc = slut + (col + b) * 32;
+                    s += lc[byte & 15u] + lc[16 + (byte >> 4)];
+                }
+                return s;
+            };
+            if (PC <= 8 * kM8RefRowWords) {
+                // the row's plane words in one burst of independent loads (the
+                // runs cover the row in whole words, in order from column 0),
+                // then each word's sum into its run
+                uint64_t wv[kM8RefRowWords];
+#pragma unroll
+                for (uint32_t j = 0; j < kM8RefRowWords; ++j)
+                    wv[j] = 8 * j < PC ? *reinterpret_cast<const E8Q_GLOBAL uint64_t*>(G(prow) + 8 * j) : 0ull;
+                const uint32_t e0 = v.runs[0].blk0 + v.runs[0].nblk;
+                const uint32_t e1 = R > 1 ? v.runs[1].blk0 + v.runs[1].nblk : PC;
+                const uint32_t e2 = R > 2 ? v.runs[2].blk0 + v.runs[2].nblk : PC;
+#pragma unroll
+                for (uint32_t j = 0; j < kM8RefRowWords; ++j) {
+                    if (8 * j < PC) {
+                        const int32_t sw = word_sum(wv[j], 8 * j);
+                        const uint32_t r = (8 * j >= e0 ? 1u : 0u) + (8 * j >= e1 ? 1u : 0u) + (8 * j >= e2 ? 1u : 0u);
+                        part[0] += r == 0 ? sw : 0;
+                        part[1] += r == 1 ? sw : 0;
+                        part[2] += r == 2 ? sw : 0;
+                        part[3] += r == 3 ? sw : 0;
                     }
                 }
-                part[r] = s;
+            } else {
+                for (uint32_t r = 0; r < R; ++r) {
+                    const uint32_t b0 = v.runs[r].blk0, b1 = b0 + v.runs[r].nblk;
+                    for (uint32_t col = b0; col < b1; col += 8)
+                        part[r] += word_sum(*reinterpret_cast<const uint64_t*>(prow + col), col);
+                }
             }
             // the coarse terms in the reference's arithmetic, against tau
             double ssc[kM8BatchRuns], str[kM8BatchRuns], pool = 0.0, qadd = 0.0;

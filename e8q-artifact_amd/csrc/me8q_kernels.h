@@ -79,6 +79,7 @@ constexpr uint32_t kM8BatchQTile = 32;   // pairs per work item (the MFMA's M)
 constexpr uint32_t kM8BatchBTile = 8;    // batches per work item (4 waves x 2 tiles)
 constexpr uint32_t kM8RefineChunk = 64;  // candidates per refine task
 constexpr uint32_t kM8RefineGrab = 8;    // refine tasks a wave takes at a time
+constexpr uint32_t kM8RefRowWords = 16;  // plane words (8 columns each) the refine loads in one burst
 constexpr uint32_t kM8RefineWaves = 4;   // waves of a refine workgroup (one LDS copy of the level tables)
 constexpr uint32_t kM8RefLevMax = 12288; // level-table floats (48 KiB) the refine stages in LDS
 constexpr uint32_t kM8LevGlobal = 0xFFFFFFFFu;   // a run whose level table the refine reads from global memory
