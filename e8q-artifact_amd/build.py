@@ -30,9 +30,18 @@ def run(cmd, cwd=None):
 
 
 def build_gpu():
+    """One object per .hip source, compiled in parallel, then one shared link."""
+    from concurrent.futures import ThreadPoolExecutor
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     out = os.path.join(PKG, "libe8q_gpu.so")
-    run([os.path.join(ROCM, "bin", "hipcc")] + HIP_FLAGS + ["-shared"] + srcs + ["-o", out + ".tmp"])
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    objs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: run([hipcc] + HIP_FLAGS + ["-c", so[0], "-o", so[1]]), zip(srcs, objs)))
+    run([hipcc] + HIP_FLAGS + ["-shared"] + objs + ["-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
 
 
