@@ -33,6 +33,10 @@ inline void Check(hipError_t e, const char* what) {
 
 template <class F>
 e8q_status Guard(F&& f) {
+    // The launchers report launch failures through hipGetLastError, which
+    // also returns the last failure of any earlier HIP call on this thread,
+    // including calls made by other libraries; drop that before the call.
+    (void)hipGetLastError();
     try {
         f();
         return E8Q_OK;

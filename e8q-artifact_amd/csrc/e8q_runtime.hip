@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -924,6 +925,152 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             Check(hipEventElapsedTime(&ms_all, ctx->ev[0], ctx->ev[2]), "event time");
             stats->candidates = hs[0];
             stats->survivors = hs[1] + extra_survivors;
+            stats->batches = hs[2];
+            stats->scanned = hs[3];
+            stats->scan_ms = ms_scan;
+            stats->total_ms = ms_all;
+        }
+    });
+}
+
+// ---- the allocation-free plain search --------------------------------------
+// Every per-call buffer of the plain form (staged queries, rotated queries,
+// LUTs, LUT constants, route distances, probe lists, staged outputs, the
+// counters and the kernel parameter block) is carved from `scratch`, one
+// fixed array per GPU, instead of the context workspace.  In host code the
+// array's name is only a stand-in for the symbol: kernels and copies get the
+// device address hipGetSymbolAddress returns, resolved once per device.
+static __device__ __align__(256) unsigned char scratch[E8Q_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kStaticDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_dev[kStaticDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+char* StaticScratch(int device) {
+    if (device < 0 || device >= kStaticDevices) Throw(E8Q_EINVAL, "device id out of range for the static scratch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* ptr = nullptr;
+        Check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) Throw(E8Q_ERUNTIME, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_dev[device] = static_cast<char*>(ptr);
+    }
+    return g_scratch_dev[device];
+}
+
+}  // namespace
+
+e8q_status e8q_search_static(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* a, e8q_search_stats* stats) {
+    return Guard([&] {
+        if (!ctx || !x || !a) Throw(E8Q_EINVAL, "null argument");
+        if (a->k == 0 || a->k > static_cast<uint64_t>(kMaxK))
+            Throw(E8Q_EINVAL, "k must be in [1, " + std::to_string(kMaxK) + "]");
+        if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
+        if (a->mode != E8Q_MODE_PLAIN) Throw(E8Q_EINVAL, "the allocation-free search runs the plain form only");
+        const int flush = a->flush_every > 0 ? a->flush_every : 32;
+        if (flush > 256) Throw(E8Q_EINVAL, "flush_every must be <= 256");
+        const IndexView& v = x->view;
+        const uint64_t nq = a->nq;
+        if (stats) *stats = e8q_search_stats{0, 0, 0, 0.0, 0.0, 0};
+        if (nq == 0) return;
+        uint32_t maxp = 0;
+        if (a->nprobe_per_query) {
+            for (uint64_t i = 0; i < nq; ++i) {
+                if (a->nprobe_per_query[i] == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+                maxp = std::max<uint32_t>(maxp, a->nprobe_per_query[i]);
+            }
+        } else {
+            if (a->nprobe == 0) Throw(E8Q_EINVAL, "nprobe must be positive");
+            maxp = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
+        }
+        maxp = static_cast<uint32_t>(std::min<uint64_t>(maxp, v.nlist));
+        if (SelectProbesLds(static_cast<uint32_t>(v.nlist), maxp) > 160 * 1024)
+            Throw(E8Q_EINVAL, "nlist too large for the probe selector");
+        const uint64_t P = v.padded, E = static_cast<uint64_t>(v.blocks) * 32;
+        float *dq = nullptr, *rotq = nullptr, *consts = nullptr, *dist = nullptr, *pd2 = nullptr, *odist = nullptr;
+        uint8_t* lut = nullptr;
+        uint32_t *pids = nullptr, *npq = nullptr, *oids = nullptr;
+        unsigned long long* dstats = nullptr;
+        PlainParams* dparams = nullptr;
+        e8q_ctx::Carve cv;   // the same buffers e8q_search carves for the plain form
+        if (!a->queries_on_device) cv.add(&dq, nq * v.dim);
+        cv.add(&rotq, nq * P);
+        cv.add(&lut, nq * E);
+        cv.add(&consts, nq * 8);
+        cv.add(&dist, nq * v.nlist);
+        cv.add(&pids, nq * maxp);
+        cv.add(&pd2, nq * maxp);
+        if (a->nprobe_per_query) cv.add(&npq, nq);
+        if (!a->outputs_on_device) {
+            cv.add(&oids, nq * a->k);
+            cv.add(&odist, nq * a->k);
+        }
+        cv.add(&dstats, 4);
+        cv.add(&dparams, 1);
+        const uint64_t need = cv.bytes();
+        if (need > E8Q_STATIC_SCRATCH_BYTES)
+            Throw(E8Q_EINVAL, "this search needs " + std::to_string(need) + " bytes of scratch; the allocation-free "
+                              "entry point holds " + std::to_string(static_cast<uint64_t>(E8Q_STATIC_SCRATCH_BYTES)) +
+                              " (search fewer queries per call)");
+        ctx->Bind();
+        char* base = StaticScratch(ctx->device);
+        size_t off = 0;
+        for (auto& w : cv.want) {
+            *w.first = base + off;
+            off += (w.second + 255) & ~size_t{255};
+        }
+        hipStream_t st = ctx->stream;
+        const float* qsrc = a->queries_on_device ? a->queries : dq;
+        if (a->outputs_on_device) {
+            oids = a->out_ids;
+            odist = a->out_dists;
+        }
+        PlainArgs pa{};
+        pa.nq = nq;
+        pa.rotq = rotq;
+        pa.lut = lut;
+        pa.consts = consts;
+        pa.probe_ids = pids;
+        pa.probe_d2 = pd2;
+        pa.maxp = maxp;
+        pa.nprobe_q = npq;
+        pa.nprobe_all = maxp;
+        pa.k = a->k;
+        pa.z = a->z;
+        pa.disable_prune = a->disable_prune ? 1 : 0;
+        pa.flush_every = flush;
+        pa.out_ids = oids;
+        pa.out_dists = odist;
+        pa.stats = dstats;
+        if (!a->queries_on_device)
+            Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+        if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
+        Check(hipEventRecord(ctx->ev[0], st), "event");
+        // The counters start at zero on every call: a fill kernel, since the
+        // array keeps the previous call's contents.
+        Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(dstats), 8, 0), "fill");
+        Check(LaunchRotate(st, qsrc, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
+        Check(LaunchQueryPrep(st, rotq, nq, v, lut, consts), "query prep");
+        Check(LaunchRouteProbes(st, rotq, consts, nq, v, dist, npq, maxp, maxp, pids, pd2), "route");
+        Check(hipEventRecord(ctx->ev[1], st), "event");
+        Check(LaunchPlainSearch(st, v, pa, dparams, true), "plain search");
+        Check(hipEventRecord(ctx->ev[2], st), "event");
+        if (!a->outputs_on_device) {
+            Check(hipMemcpyAsync(a->out_ids, oids, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download ids");
+            Check(hipMemcpyAsync(a->out_dists, odist, nq * a->k * 4, hipMemcpyDeviceToHost, st), "download dists");
+        }
+        unsigned long long hs[4] = {0, 0, 0, 0};
+        Check(hipMemcpyAsync(hs, dstats, 32, hipMemcpyDeviceToHost, st), "download stats");
+        Check(hipStreamSynchronize(st), "search");
+        if (stats) {
+            float ms_scan = 0.0F, ms_all = 0.0F;
+            Check(hipEventElapsedTime(&ms_scan, ctx->ev[1], ctx->ev[2]), "event time");
+            Check(hipEventElapsedTime(&ms_all, ctx->ev[0], ctx->ev[2]), "event time");
+            stats->candidates = hs[0];
+            stats->survivors = hs[1];
             stats->batches = hs[2];
             stats->scanned = hs[3];
             stats->scan_ms = ms_scan;

@@ -173,6 +173,26 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* idx, const e8q_search_args
  * context workspace); the serving loop admits work by it. */
 uint64_t e8q_search_workspace_bytes(const e8q_dindex* idx, uint64_t nq, uint64_t k, uint64_t max_nprobe);
 
+/* Size of the fixed per-GPU scratch array of e8q_search_static. */
+#define E8Q_STATIC_SCRATCH_BYTES (32u << 20)
+
+/* The plain form of e8q_search (mode must be E8Q_MODE_PLAIN) with no device
+ * allocation: no hipMalloc, no hipFree, no growth of the context workspace.
+ * No counterpart in the reference (its search allocates nothing on a
+ * device); it is for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Every
+ * per-call buffer (staged queries and outputs, rotated queries, LUTs, route
+ * distances, probe lists, counters, the kernel parameter block) lives in one
+ * fixed array of E8Q_STATIC_SCRATCH_BYTES per GPU, held by each GPU the
+ * library runs on until the process ends.  A call whose buffers do not fit
+ * (e8q_search_workspace_bytes with host queries and outputs is an upper
+ * bound) fails with E8Q_EINVAL: search fewer queries per call.  Every call
+ * on one GPU shares that GPU's array, so calls that overlap on one GPU
+ * (several contexts or threads at once) race; serialize them.  The results,
+ * stats and error behaviour are e8q_search's. */
+e8q_status e8q_search_static(e8q_ctx* ctx, const e8q_dindex* idx, const e8q_search_args* args,
+                             e8q_search_stats* stats);
+
 /* ---- diagnostics used by tests ------------------------------------------ */
 /* Coarse integer accumulate of one 32-lane batch (AccumulateCoarseHi,
  * e8q.hpp:333-336): rowa = blocks*32 bytes per batch, lut = blocks*64 bytes

@@ -182,6 +182,37 @@ class DeviceIndex:
         check(lib().e8q_search(self.ctx.h, self.h, ctypes.byref(s), ctypes.byref(st)))
         return ids, dists, (st.candidates, st.survivors)
 
+    def search_full(self, queries, k, nprobe, z, disable_prune=False, nprobe_per_query=None, flush_every=32,
+                    static=False, device_ptrs=None):
+        """A plain search through e8q_search or (static=True) e8q_search_static;
+        returns ids, dists and the whole stats struct as a dict.  device_ptrs =
+        (queries, out_ids, out_dists) device addresses: queries and outputs
+        stay on the device (ids / dists are then None)."""
+        s = SearchArgs()
+        npq = None if nprobe_per_query is None else np.ascontiguousarray(nprobe_per_query, np.uint32)
+        if device_ptrs is None:
+            q = np.ascontiguousarray(queries, np.float32)
+            nq = q.shape[0]
+            ids = np.empty((nq, k), np.uint32)
+            dists = np.empty((nq, k), np.float32)
+            s.queries, s.out_ids, s.out_dists = p(q), p(ids), p(dists)
+            s.queries_on_device, s.outputs_on_device = 0, 0
+        else:
+            nq = int(queries)
+            ids = dists = None
+            s.queries, s.out_ids, s.out_dists = device_ptrs
+            s.queries_on_device, s.outputs_on_device = 1, 1
+        s.nq, s.k, s.nprobe = nq, k, nprobe
+        s.nprobe_per_query = p(npq)
+        s.z, s.disable_prune, s.flush_every, s.mode = z, int(bool(disable_prune)), flush_every, MODE_PLAIN
+        st = SearchStats()
+        fn = lib().e8q_search_static if static else lib().e8q_search
+        check(fn(self.ctx.h, self.h, ctypes.byref(s), ctypes.byref(st)))
+        return ids, dists, {f: getattr(st, f) for f, _ in SearchStats._fields_}
+
+    def workspace_bytes(self, nq, k, max_nprobe):
+        return int(lib().e8q_search_workspace_bytes(self.h, nq, k, max_nprobe))
+
     def query_prep(self, queries, nprobe):
         q = np.ascontiguousarray(queries, np.float32)
         nq = q.shape[0]
