@@ -97,6 +97,8 @@ struct e8q_ctx {
     int cus = 0;      // compute units (the persistent batched scan's grid)
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // search timing
+    static constexpr int kKernelEvents = 16;
+    hipEvent_t kev[kKernelEvents] = {};   // per-launch timing of one kernel's launches in a search
     DBuf ws;   // per-call workspace, grown on demand
     DBuf pool; // candidate pool of the batched ME8Q form, sized per call after its count pass
     DBuf recs; // records of the batched ME8Q count pass

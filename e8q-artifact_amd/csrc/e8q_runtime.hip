@@ -200,6 +200,7 @@ e8q_status e8q_ctx_create(int device, e8q_ctx** out) {
             Check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
             Check(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
             for (auto& e : c->ev) Check(hipEventCreate(&e), "hipEventCreate");
+            for (auto& e : c->kev) Check(hipEventCreate(&e), "hipEventCreate");
         } catch (...) {
             delete c;
             throw;
@@ -216,6 +217,8 @@ e8q_status e8q_ctx_destroy(e8q_ctx* ctx) {
         ctx->ClearGraphs();
         ctx->ws.reset();
         for (auto& e : ctx->ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : ctx->kev)
             if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -497,7 +500,8 @@ e8q_status e8q_search(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search_args* 
             Throw(E8Q_EINVAL, "k must be in [1, " + std::to_string(kMaxK) + "]");
         if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
         const int flush = a->flush_every > 0 ? a->flush_every : 32;
-        if (flush > 256) Throw(E8Q_EINVAL, "flush_every must be <= 256");
+        if (flush > kMaxFlush)
+            Throw(E8Q_EINVAL, "flush_every must be <= 1024 (the survivor staging capacity, as in the reference)");
         if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED && a->mode != E8Q_MODE_MFMA)
             Throw(E8Q_EINVAL, "unknown search mode");
         const IndexView& v = x->view;
@@ -971,7 +975,8 @@ e8q_status e8q_search_static(e8q_ctx* ctx, const e8q_dindex* x, const e8q_search
         if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
         if (a->mode != E8Q_MODE_PLAIN) Throw(E8Q_EINVAL, "the allocation-free search runs the plain form only");
         const int flush = a->flush_every > 0 ? a->flush_every : 32;
-        if (flush > 256) Throw(E8Q_EINVAL, "flush_every must be <= 256");
+        if (flush > kMaxFlush)
+            Throw(E8Q_EINVAL, "flush_every must be <= 1024 (the survivor staging capacity, as in the reference)");
         const IndexView& v = x->view;
         const uint64_t nq = a->nq;
         if (stats) *stats = e8q_search_stats{0, 0, 0, 0.0, 0.0, 0};

@@ -174,6 +174,12 @@ class PyMe8qIndex {
         s["overflowed"] = st.overflowed;
         s["pairs_scored"] = st.pairs_scored;
         s["items"] = st.items;
+        s["refine_first_ms"] = st.refine_ms[0];
+        s["refine_cascade_ms"] = st.refine_ms[1];
+        s["refine_first_launches"] = st.refine_launches[0];
+        s["refine_cascade_launches"] = st.refine_launches[1];
+        s["refine_first_bytes"] = st.refine_bytes[0];
+        s["refine_cascade_bytes"] = st.refine_bytes[1];
         return py::make_tuple(ids, dists, s);
     }
 
@@ -213,6 +219,12 @@ class PyMe8qIndex {
         s["overflowed"] = st.overflowed;
         s["pairs_scored"] = st.pairs_scored;
         s["items"] = st.items;
+        s["refine_first_ms"] = st.refine_ms[0];
+        s["refine_cascade_ms"] = st.refine_ms[1];
+        s["refine_first_launches"] = st.refine_launches[0];
+        s["refine_cascade_launches"] = st.refine_launches[1];
+        s["refine_first_bytes"] = st.refine_bytes[0];
+        s["refine_cascade_bytes"] = st.refine_bytes[1];
         return s;
     }
 

@@ -526,6 +526,11 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
         stats->overflowed += st.overflowed;
         stats->pairs_scored += st.pairs_scored;
         stats->items += st.items;
+        for (int i = 0; i < 2; ++i) {
+            stats->refine_ms[i] += st.refine_ms[i];
+            stats->refine_launches[i] += st.refine_launches[i];
+            stats->refine_bytes[i] += st.refine_bytes[i];
+        }
     }
 }
 

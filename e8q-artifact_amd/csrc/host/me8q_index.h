@@ -82,6 +82,11 @@ struct SearchStats {
     size_t overflowed = 0;     // batched form: queries searched again by the plain form
     size_t pairs_scored = 0;   // batched form: (query, vector) pairs of the MFMA contraction
     size_t items = 0;          // batched form: its work items
+    // batched form, the exact refine kernel per phase (0: first runs, 1:
+    // cascades and the threshold pass): launch time, launches, algorithmic bytes
+    double refine_ms[2] = {0.0, 0.0};
+    size_t refine_launches[2] = {0, 0};
+    size_t refine_bytes[2] = {0, 0};
 };
 
 class Me8qIndex {

@@ -976,7 +976,7 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
     const double dz = a.z;
     const uint32_t ntask = Uniform(a.toff[a.nq]);
     uint32_t qcur = 0xFFFFFFFFu;
-    unsigned long long n_ref = 0, n_surv = 0, n_bits = 0;
+    unsigned long long n_ref = 0, n_surv = 0, n_bits = 0, n_bytes = 0;
     // Tasks are taken kM8RefineGrab at a time (consecutive tasks mostly share
     // a query: one search for it, its LDS data reused).
     uint32_t task = 0, tend = 0, q = 0, qend = 0;
@@ -1037,6 +1037,10 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
             const uint8_t* rec = v.plane + (v.batch_begin[c] + bj) * v.rec;
             const uint8_t* prow = rec + static_cast<uint64_t>(l) * PC;
             const uint8_t* banks = rec + 32ull * PC;
+            // algorithmic bytes: the pool entry, the plane row, per run its
+            // 16-byte header, its W and Omega bank bytes and 1/32 of the
+            // batch's scales
+            n_bytes += 8 + PC + 19 * R;
             const double d2 = static_cast<double>(a.pdist[static_cast<uint64_t>(q) * maxp + p]);
             const double* sqd2 = a.sqd2 + (static_cast<uint64_t>(q) * maxp + p) * R;
             // integer coarse sums of each run (AccumulateCoarseHi over its columns)
@@ -1174,6 +1178,7 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
                 const double t = str[r];
                 rem2 = fma(-t, t, rem2);
                 if (kFirst || oi > 0) bits += static_cast<uint64_t>(ru.rate) * ru.len;
+                n_bytes += static_cast<uint64_t>(ru.rate - 1) * ru.len / 8;   // the run's fields
                 if (a.cascade && fma(-dz, sqrt(fmax(rem2, 0.0)), dist) - qadd >= taud) live = false;
             }
             if (kFirst) ++n_ref;
@@ -1189,7 +1194,10 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
             uint32_t base = 0;
             if (lane == 0 && m != 0) base = atomicAdd(&a.ncont[q], static_cast<uint32_t>(__popcll(m)));
             base = Uniform(static_cast<uint32_t>(__shfl(static_cast<int>(base), 0)));
-            if (live) a.cont[a.coff[q] + base + __popcll(m & ((uint64_t{1} << lane) - 1))] = ent;
+            if (live) {
+                a.cont[a.coff[q] + base + __popcll(m & ((uint64_t{1} << lane) - 1))] = ent;
+                n_bytes += 8;
+            }
             if (lane == 0) a.tsurv[task] = 0u;
             ++task;
             continue;
@@ -1199,6 +1207,7 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
         if (live) {
             slots[__popcll(m & ((uint64_t{1} << lane) - 1))] = (static_cast<uint64_t>(M8Key(dsurv)) << 32) | idsurv;
             ++n_surv;
+            n_bytes += 12;   // its id read, its entry written
         }
         if (lane == 0) a.tsurv[task] = static_cast<uint32_t>(__popcll(m));
         ++task;
@@ -1208,11 +1217,13 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
         n_ref += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n_ref), o));
         n_surv += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n_surv), o));
         n_bits += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n_bits), o));
+        n_bytes += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n_bytes), o));
     }
-    if (lane == 0 && a.stats && (n_ref | n_surv)) {
+    if (lane == 0 && a.stats && (n_ref | n_surv | n_bytes)) {
         atomicAdd(&a.stats[2], n_ref);
         atomicAdd(&a.stats[3], n_surv);
         atomicAdd(&a.stats[4], n_bits);
+        atomicAdd(&a.stats[kFirst ? 6 : 7], n_bytes);
     }
 }
 

@@ -203,7 +203,10 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "mode must be plain or batched");
     const M8View& v = x->view;
     const uint64_t nq = a->nq;
-    if (stats) *stats = e8q_m8_stats{0, 0, 0, 0, 0, 0.0, 0.0, 0, 0, 0, E8Q_MODE_PLAIN, 0, 0, 0};
+    if (stats) {
+        *stats = e8q_m8_stats{};
+        stats->form = E8Q_MODE_PLAIN;
+    }
     if (nq == 0) return;
     uint32_t maxp = 0;
     uint64_t pairs = 0;
@@ -281,6 +284,18 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         if (r.rate <= kM8LdsLevRate) sa.lev_floats += 2u << r.rate;
     std::vector<unsigned long long> bst(8, 0), wst(4, 0);
     uint64_t novf = 0;
+    // The refine's launches, each between two of the context's kernel events:
+    // kind 0 = first-run phase, 1 = cascades.
+    int nkl = 0, kind[e8q_ctx::kKernelEvents / 2] = {};
+    auto refine_launch = [&](const M8BatchArgs& r, bool first, const char* what) {
+        const bool timed = stats && nkl < e8q_ctx::kKernelEvents / 2;
+        if (timed) Check(hipEventRecord(ctx->kev[2 * nkl], st), "event");
+        Check(LaunchM8Refine(st, r, 8 * static_cast<uint32_t>(std::max(ctx->cus, 1)), first), what);
+        if (timed) {
+            Check(hipEventRecord(ctx->kev[2 * nkl + 1], st), "event");
+            kind[nkl++] = first ? 0 : 1;
+        }
+    };
     if (!batched) {
         Check(hipEventRecord(ctx->ev[1], st), "event");
         Check(LaunchM8Scan(st, sa), "scan");
@@ -397,7 +412,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(qst), nq * 16, 0), "clear");
             Check(LaunchM8ThresholdPool(st, ta, npq, nprobe_all, budget), "threshold pool");
             Check(LaunchM8RefineTasks(st, ta, b.nfill, false), "threshold tasks");
-            Check(LaunchM8Refine(st, ta, 8 * cus, false), "threshold refine");
+            refine_launch(ta, false, "threshold refine");
             Check(LaunchM8Select(st, ta), "threshold select");
         }
         // records of the count pass, when the reserve holds any
@@ -462,14 +477,14 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
                 r.toff = b.toff;
                 Check(LaunchFill32(st, b.ncont, nq, 0), "clear");
                 Check(LaunchM8RefineTasks(st, r, r.nfill, false), "refine tasks");
-                Check(LaunchM8Refine(st, r, 8 * cus, true), "refine, first runs");
+                refine_launch(r, true, "refine, first runs");
                 if (v.nrun > 1) {
                     r.pool = r.cont;
                     r.nfill = b.ncont;
                     r.toff = b.toff2;
                     r.tsurv = tsurv2;
                     Check(LaunchM8RefineTasks(st, r, b.ncont, false), "refine tasks");
-                    Check(LaunchM8Refine(st, r, 8 * cus, false), "refine, cascades");
+                    refine_launch(r, false, "refine, cascades");
                 }
                 r.out_ids = sel_ids;
                 r.out_d = sel_d;
@@ -533,6 +548,14 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             stats->overflowed = novf;
             stats->pairs_scored = bst[0];
             stats->items = bst[5];
+            stats->refine_bytes[0] = bst[6];
+            stats->refine_bytes[1] = bst[7];
+            for (int i = 0; i < nkl; ++i) {
+                float ms = 0.0F;
+                Check(hipEventElapsedTime(&ms, ctx->kev[2 * i], ctx->kev[2 * i + 1]), "event time");
+                stats->refine_ms[kind[i]] += ms;
+                ++stats->refine_launches[kind[i]];
+            }
         }
         float ms = 0.0F;
         Check(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "event time");

@@ -84,6 +84,16 @@ typedef struct {
                               by the plain form */
     uint64_t pairs_scored; /* batched form: (query, vector) pairs of the MFMA contraction */
     uint64_t items;        /* batched form: its work items (32 pairs x 8 batches at most) */
+    /* batched form, the exact refine kernel per phase ([0]: every candidate's
+     * first run; [1]: the cascades of those still alive and the threshold
+     * pass): summed time of its launches (HIP events around each launch on the
+     * context stream), number of launches, and algorithmic bytes (per
+     * candidate: its 8-byte pool entry, its plane row, its 16-byte run
+     * headers, its two u8 bank entries (W, Omega) and 1/32 of the 32-byte
+     * batch scales per run, the fields of the runs it scores, and what it writes) */
+    double refine_ms[2];
+    uint64_t refine_launches[2];
+    uint64_t refine_bytes[2];
 } e8q_m8_stats;
 
 /* Allocates the device index and its static tables (the allocation half of

@@ -42,8 +42,10 @@ int main(int argc, char** argv) {
     const bool me8q = std::memcmp(magic, "ME8QIDXX", 8) == 0;
     e8qlib::E8qIndex idx;
     e8qlib::me8q::Me8qIndex midx;
+    const auto tl = std::chrono::steady_clock::now();
     if (me8q) midx.Load(argv[1]);
     else idx.Load(argv[1]);
+    const double load_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
     const size_t nq = std::strtoull(argv[3], nullptr, 10);
     const size_t k = std::strtoull(argv[4], nullptr, 10);
     const size_t nprobe = std::strtoull(argv[5], nullptr, 10);
@@ -90,7 +92,8 @@ int main(int argc, char** argv) {
         std::fclose(f);
     }
     std::printf("{\"qps_median\": %.3f, \"qps_p25\": %.3f, \"qps_p75\": %.3f, \"rounds\": %zu, \"seconds\": %.3f, "
-                "\"threads\": %d, \"nq\": %zu}\n",
-                qps[qps.size() / 2], qps[qps.size() / 4], qps[(3 * qps.size()) / 4], qps.size(), total, threads, nq);
+                "\"threads\": %d, \"nq\": %zu, \"load_seconds\": %.3f}\n",
+                qps[qps.size() / 2], qps[qps.size() / 4], qps[(3 * qps.size()) / 4], qps.size(), total, threads, nq,
+                load_s);
     return 0;
 }

@@ -33,7 +33,8 @@ class M8Stats(ctypes.Structure):
     _fields_ = [("candidates", u64), ("survivors", u64), ("bits_scored", u64), ("bits_total", u64),
                 ("pruned", u64), ("scan_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("batches", u64),
                 ("refined", u64), ("groups", u64), ("form", ctypes.c_int32), ("overflowed", u64),
-                ("pairs_scored", u64), ("items", u64)]
+                ("pairs_scored", u64), ("items", u64), ("refine_ms", ctypes.c_double * 2),
+                ("refine_launches", u64 * 2), ("refine_bytes", u64 * 2)]
 
 PLAIN, BATCHED = 0, 1
 

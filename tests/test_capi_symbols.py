@@ -28,4 +28,4 @@ def test_me8q_struct_sizes():
     # ... outputs_on_device, mode, batched_budget, batched_set_a
     assert ctypes.sizeof(capi_m8.M8SearchArgs) == 5 * 8 + 4 + 4 + 8 + 8 + 4 + 4 + 4 + 4
     # ... groups, form (+ padding), overflowed, pairs_scored, items
-    assert ctypes.sizeof(capi_m8.M8Stats) == 5 * 8 + 2 * 8 + 3 * 8 + 8 + 8 + 8 + 8
+    assert ctypes.sizeof(capi_m8.M8Stats) == 5 * 8 + 2 * 8 + 3 * 8 + 8 + 8 + 8 + 8 + 3 * 2 * 8
