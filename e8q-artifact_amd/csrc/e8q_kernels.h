@@ -8,6 +8,10 @@
 
 namespace e8qdev {
 
+// Largest survivor flush size: the reference's staging capacity
+// (kSurvCapMax, e8q_internal.hpp:33).
+constexpr int kMaxFlush = 1024;
+
 struct PlainArgs {
     uint64_t nq;
     const float* rotq;        // [nq][padded] rotated, centered queries

@@ -23,6 +23,13 @@ namespace e8qdev {
 
 // The query LUT first, at LDS address 0, so lookups fold the block offset
 // into the instruction.
+// LDS bytes of the refine's row buffer (kGroup rows of rowA bytes and
+// fields) of the plain kernel, 16-byte aligned; + 16: the refine reads each
+// field through an 8-byte window.
+__host__ __device__ inline uint32_t PlainRowBufBytes(uint32_t blocks, int bits) {
+    return (kGroup * blocks * static_cast<uint32_t>(bits) + 16 + 15) & ~15u;
+}
+
 template <int NQW>
 struct PlainLds {
     uint8_t lut[NQW > 0 ? NQW * 256 : 4];
@@ -194,6 +201,8 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
     uint8_t* lutc = NQW > 0 ? sh.lut : dyn;
     float* qv = reinterpret_cast<float*>(dyn + (NQW > 0 ? 0 : static_cast<size_t>(ix.blocks) * 32));  // padded
     uint8_t* rowbuf = reinterpret_cast<uint8_t*>(qv + ix.padded);                                     // kGroup rows
+    // staged rows (vector offsets inside the cluster), flush_every of them
+    uint32_t* stg = reinterpret_cast<uint32_t*>(rowbuf + PlainRowBufBytes(ix.blocks, ix.bits));
     const int lane = threadIdx.x;
     if (ix.bits <= 8)
         for (int i = lane; i < 512; i += kWave) S.tab[i] = G(ix.refine_tab)[i];
@@ -250,7 +259,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
         auto cluster_ref = [&]() {
             const uint32_t l = cp - ch.base;
             return ClusterRows{arena_c, ix.ref_arena + ReadLane64(ch.ro_lo, ch.ro_hi, l),
-                               ReadLane64(ch.vb_lo, ch.vb_hi, l), d2, S.staged};
+                               ReadLane64(ch.vb_lo, ch.vb_hi, l), d2, stg};
         };
         Rec<NQW> rn;
         cp = Uniform(cp);
@@ -360,7 +369,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                 if (pass) {
                     const uint32_t m = j ? pm1 : pm0;
                     const int rank = staged + (j ? __popc(pm0) : 0) + __popc(m & ((1u << t) - 1u));
-                    S.staged[rank] = bj * 32 + t;
+                    stg[rank] = bj * 32 + t;
                 }
                 staged += total;
                 n_surv += total;
@@ -395,7 +404,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                         const int tk = pc <= room ? pc : room;
                         if (in32 && ((ps >> lane) & 1u)) {
                             const int rank = __popc(ps & ((1u << lane) - 1u));
-                            if (rank < tk) S.staged[staged + rank] = off + lane;
+                            if (rank < tk) stg[staged + rank] = off + lane;
                         }
                         staged += tk;
                         n_surv += tk;
@@ -407,7 +416,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                         const int pl = __ffs(rest) - 1;
                         FlushRows<GEN>(S, staged, cnt, limit, &prm->ix, k, qv, cr, rowbuf);
                         Refresh(ch, limit, sk);
-                        if (LaneHere() == 0) S.staged[0] = off + static_cast<uint32_t>(pl);
+                        if (LaneHere() == 0) stg[0] = off + static_cast<uint32_t>(pl);
                         staged = 1;
                         n_surv += 1;
                         mask &= (pl >= 31) ? 0u : ~((2u << pl) - 1u);
@@ -482,12 +491,11 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
     }
 }
 
-size_t PlainSearchLds(const IndexView& ix, int /*flush_every*/) {
+size_t PlainSearchLds(const IndexView& ix, int flush_every) {
     const size_t nqw = static_cast<size_t>(ix.blocks) / 8;
     const bool lut_static = (nqw == 1 || nqw == 2 || nqw == 12 || nqw == 16) && ix.bits >= 2 && ix.bits <= 5;
-    const size_t rstride = static_cast<size_t>(ix.blocks) * static_cast<size_t>(ix.bits);
-    // + 16: the refine reads each field through an 8-byte window.
-    return (lut_static ? 0 : static_cast<size_t>(ix.blocks) * 32) + ix.padded * sizeof(float) + kGroup * rstride + 16;
+    return (lut_static ? 0 : static_cast<size_t>(ix.blocks) * 32) + ix.padded * sizeof(float) +
+           PlainRowBufBytes(ix.blocks, ix.bits) + static_cast<size_t>(flush_every) * sizeof(uint32_t);
 }
 
 template <int NQW, bool GEN>

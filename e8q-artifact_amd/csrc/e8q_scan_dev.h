@@ -10,7 +10,6 @@
 namespace e8qdev {
 
 constexpr int kWave = 64;       // one wave per query
-constexpr int kMaxFlush = 256;
 constexpr int kGroup = 32;      // staged rows refined and merged together
 
 __device__ __forceinline__ bool Better(float d1, uint32_t i1, float d2, uint32_t i2) {
@@ -137,7 +136,6 @@ struct PlainShared {
     const uint8_t* pf[kGroup];   // rows' refine fields
     const float* pg[kGroup];     // rows' g1 (h3 follows 32 floats later)
     const uint32_t* pid[kGroup]; // rows' ids
-    uint32_t staged[kMaxFlush];  // staged rows: vector offset inside the cluster
 };
 
 // Eight blocks of coarse sum: bytes of x.x are blocks 0..3, of x.y 4..7; the

@@ -96,7 +96,7 @@ typedef struct {
     float z;                       /* union-bound quantile lambda (host computed,
                                       e8q_internal.hpp:102-105)            */
     int32_t disable_prune;         /* E8qIndex::SetDisablePrune            */
-    int32_t flush_every;           /* survivor flush size (E8Q_FLUSH_EVERY, default 32) */
+    int32_t flush_every;           /* survivor flush size 1..1024 (E8Q_FLUSH_EVERY, default 32; 0: 32) */
     int32_t mode;                  /* E8Q_MODE_*                           */
     int32_t queries_on_device;     /* 1: queries is a device pointer       */
     uint32_t* out_ids;             /* nq*k (host, or device if outputs_on_device) */

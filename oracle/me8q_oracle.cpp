@@ -722,8 +722,12 @@ void Route(const Index& ix, const float* q, size_t nprobe, std::vector<float>& p
     }
 }
 
+// lambda_override: the union-bound quantile to use instead of the index's
+// (OrthoQuantile of its prune_alpha); NaN keeps the index's.  +inf prunes
+// nothing: every probed vector is scored and the result is the exact
+// estimator top-k over the probed lists.
 void Search(const Index& ix, const float* query, size_t k, size_t nprobe, uint32_t* out_ids, float* out_d,
-            Stats* st) {
+            Stats* st, float lambda_override = std::numeric_limits<float>::quiet_NaN()) {
     const float inf = std::numeric_limits<float>::infinity();
     for (size_t i = 0; i < k; ++i) {
         out_ids[i] = UINT32_MAX;
@@ -735,7 +739,9 @@ void Search(const Index& ix, const float* query, size_t k, size_t nprobe, uint32
     std::vector<float> pd;
     std::vector<uint32_t> probe;
     Route(ix, p.q.data(), nprobe, pd, probe);
-    const float lambda = OrthoQuantile(static_cast<double>(ix.ortho_alpha), std::max<size_t>(4 * k, 32));
+    const float lambda = std::isnan(lambda_override)
+                             ? OrthoQuantile(static_cast<double>(ix.ortho_alpha), std::max<size_t>(4 * k, 32))
+                             : lambda_override;
     const double dl = lambda;
     std::vector<std::pair<float, uint32_t>> heap;
     Stats s;
@@ -1096,14 +1102,15 @@ void orc_m8_query(void* h, const float* query, size_t nprobe, float* rot_q, uint
 }
 
 // stats5 per query: candidates, survivors, bits_scored, bits_total, pruned.
-int orc_m8_search(void* h, const float* queries, size_t nq, size_t k, size_t nprobe, int threads, uint32_t* ids,
-                  float* dists, uint64_t* stats5) {
+// lambda: as Search's lambda_override (NaN: the index's own).
+int orc_m8_search_z(void* h, const float* queries, size_t nq, size_t k, size_t nprobe, int threads, float lambda,
+                    uint32_t* ids, float* dists, uint64_t* stats5) {
     const auto* ix = static_cast<orc::m8::Index*>(h);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
     for (long long i = 0; i < static_cast<long long>(nq); ++i) {
         orc::m8::Stats st;
         orc::m8::Search(*ix, queries + static_cast<size_t>(i) * ix->dim, k, nprobe, ids + static_cast<size_t>(i) * k,
-                        dists + static_cast<size_t>(i) * k, &st);
+                        dists + static_cast<size_t>(i) * k, &st, lambda);
         if (stats5) {
             uint64_t* o = stats5 + static_cast<size_t>(i) * 5;
             o[0] = st.candidates;
@@ -1114,6 +1121,13 @@ int orc_m8_search(void* h, const float* queries, size_t nq, size_t k, size_t npr
         }
     }
     return 0;
+}
+
+
+int orc_m8_search(void* h, const float* queries, size_t nq, size_t k, size_t nprobe, int threads, uint32_t* ids,
+                  float* dists, uint64_t* stats5) {
+    return orc_m8_search_z(h, queries, nq, k, nprobe, threads, std::numeric_limits<float>::quiet_NaN(), ids, dists,
+                           stats5);
 }
 
 }  // extern "C"

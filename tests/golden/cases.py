@@ -114,3 +114,79 @@ def digest(*arrays):
     for a in arrays:
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()
+
+
+# ---------------------------------------------------------------------------
+# Configuration-shaped fixtures (make_golden.py --only flush,c1,c3): the index
+# image is kept as SHA-256 digests, the search results in full.
+
+# Survivor flush sizes (E8Q_FLUSH_EVERY) run against these fixture indexes:
+# name -> [(k, nprobe, disable_prune)]
+FLUSH_SIZES = [1, 7, 256, 1024]
+
+
+def flush_grid(name):
+    nlist = INDEX_CASES[name][2]
+    return [(10, nlist, 0), (100, 8 if nlist > 8 else nlist, 0)]
+
+
+FLUSH_INDEXES = ["d128b4", "d768b4"]
+
+# The CPU-reference E8Q configuration at full size: N = 100k, D = 128 i.i.d.
+# N(0, 1) (numpy PCG64 seed 42), then 1000 queries from the same stream,
+# 256 centroids = distinct random base rows from the same stream,
+# cluster_ids = nearest centroid (fp64, kept in the fixture); b = 4,
+# rotator seed 42, k = 10, nprobe = 16.
+C1 = dict(n=100_000, dim=128, nlist=256, nq=1000, bits=4, seed=42, k=10, nprobe=16)
+
+
+def c1_iid():
+    rng = np.random.default_rng(C1["seed"])
+    base = rng.standard_normal((C1["n"], C1["dim"]), dtype=np.float32)
+    queries = rng.standard_normal((C1["nq"], C1["dim"]), dtype=np.float32)
+    pick = rng.choice(C1["n"], C1["nlist"], replace=False)
+    return base, np.ascontiguousarray(base[pick]), queries
+
+
+def nearest_centroid(x, cent, chunk=4096):
+    """argmin_c ||x - c||^2 in fp64 (the fixture keeps the result)."""
+    c64 = cent.astype(np.float64)
+    cn = (c64 * c64).sum(1)
+    out = np.empty(x.shape[0], np.int64)
+    for s in range(0, x.shape[0], chunk):
+        xb = x[s:s + chunk].astype(np.float64)
+        out[s:s + chunk] = np.argmin(cn[None, :] - 2.0 * (xb @ c64.T), axis=1)
+    return out
+
+
+# The mixed-rate ME8Q configuration's shape: D = 960 with the power-law
+# spectrum lambda_i = i^-1.2 (the PCA domain's descending variance, passed as
+# `variance`), 256 lists = seeded random base rows, cluster_ids = nearest
+# (kept in the fixture), the SAQ DP plan at avg_bits 4, k = 100 / 10,
+# nprobe 32 / 64.
+C3 = dict(n=20_000, dim=960, nlist=256, nq=32, avg_bits=4.0, seed=0, components=512)
+
+
+def c3_shape():
+    rng = np.random.default_rng(960)
+    dim = C3["dim"]
+    lam = (np.arange(1, dim + 1, dtype=np.float64) ** -1.2).astype(np.float32)
+    sd = np.sqrt(lam).astype(np.float32)
+    comp = (rng.standard_normal((C3["components"], dim)).astype(np.float32) * (2.0 * sd)).astype(np.float32)
+    lab = rng.integers(0, C3["components"], size=C3["n"])
+    data = (comp[lab] + rng.standard_normal((C3["n"], dim)).astype(np.float32) * sd).astype(np.float32)
+    qlab = rng.integers(0, C3["components"], size=C3["nq"])
+    queries = (comp[qlab] + rng.standard_normal((C3["nq"], dim)).astype(np.float32) * sd).astype(np.float32)
+    pick = rng.choice(C3["n"], C3["nlist"], replace=False)
+    return data, np.ascontiguousarray(data[pick]), queries, lam
+
+
+def c3_grid():
+    return [(100, 32), (100, 64), (10, 32)]
+
+
+def e8ax_digests(ix):
+    """SHA-256 of each array section of an E8AX image (tests/e8ax.py read())."""
+    return {key: digest(np.asarray(ix[key])) for key in ("ids", "signs", "mean_rot", "centroid_rot", "centroid_n2",
+                                                          "cluster_begin", "arena_begin", "arena", "ref_arena_begin",
+                                                          "ref_arena")}

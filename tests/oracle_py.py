@@ -211,6 +211,8 @@ def _m8_lib():
         L.orc_m8_query.argtypes = [vp, vp, sz, vp, vp, vp, vp, vp]
         L.orc_m8_search.restype = i32
         L.orc_m8_search.argtypes = [vp, vp, sz, sz, sz, i32, vp, vp, vp]
+        L.orc_m8_search_z.restype = i32
+        L.orc_m8_search_z.argtypes = [vp, vp, sz, sz, sz, i32, f32, vp, vp, vp]
         L._m8_ready = True
     return L
 
@@ -278,11 +280,17 @@ class OracleMe8q:
         _m8_lib().orc_m8_query(self.h, _p(q), nprobe, _p(rq), _p(lut), _p(md), _p(probe), _p(pd))
         return rq, lut, md, probe, pd
 
-    def search(self, queries, k, nprobe, threads=8):
+    def search(self, queries, k, nprobe, threads=8, z=None):
+        """The reference's search; z overrides the union-bound quantile (the
+        index's own when None; +inf prunes nothing, giving the exact
+        estimator top-k over the probed lists)."""
         q = c32(queries)
         nq = q.shape[0]
         ids = np.empty((nq, k), np.uint32)
         d = np.empty((nq, k), np.float32)
         st = np.empty((nq, 5), np.uint64)
-        _m8_lib().orc_m8_search(self.h, _p(q), nq, k, nprobe, threads, _p(ids), _p(d), _p(st))
+        if z is None:
+            _m8_lib().orc_m8_search(self.h, _p(q), nq, k, nprobe, threads, _p(ids), _p(d), _p(st))
+        else:
+            _m8_lib().orc_m8_search_z(self.h, _p(q), nq, k, nprobe, threads, float(z), _p(ids), _p(d), _p(st))
         return ids, d, st

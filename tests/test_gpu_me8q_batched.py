@@ -250,3 +250,100 @@ def test_python_api_batched():
     with pytest.raises(ValueError):
         idx.search(q, 10, 4, mode="mfma")
     assert math.isfinite(float(bd[0, 0]))
+
+
+# ---------------------------------------------------------------------------
+# The 10M x 1024 served configuration's plan (64:11, 128:6, 384:4, 448:2:
+# a plane row of exactly 128 columns, the rate-11 level table staged in LDS)
+# on Zipf-sized lists with a head list and empty lists, and a D = 1152 plan
+# with 144 plane columns (the refine's word-loop row sums) and a rate-13 run
+# (its level table read from global memory), against the oracle.
+
+def _zipf_sizes(n, nlist, s=1.1, empty=(5, 17, 40)):
+    w = np.arange(1, nlist + 1, dtype=np.float64) ** -s
+    sizes = np.floor(w / w.sum() * n).astype(np.int64)
+    for e in empty:
+        if e < nlist:
+            sizes[e] = 0
+    return sizes
+
+
+SHAPES = [
+    # seed, n, dim, nlist, spec
+    (101, 30000, 1024, 64, "64:11,128:6,384:4,448:2"),
+    (102, 12000, 1152, 48, "64:13,192:5,448:3,448:2"),
+]
+
+
+@pytest.fixture(scope="module", params=range(len(SHAPES)), ids=["d1024_c5plan", "d1152_rate13"])
+def shape_case(request, ctx):
+    seed, n, dim, nlist, spec = SHAPES[request.param]
+    data, cent, lab, q, oix = _oracle_case(seed, 0, dim, nlist, 4.0, spec=spec, sizes=_zipf_sizes(n, nlist))
+    f = me8qx.parse(oix.image())
+    ix = capi_m8.M8Index.from_file(ctx, f)
+    assert [tuple(s) for s in f["plan"]] == [tuple(int(t) for t in s.split(":")) for s in spec.split(",")]
+    return ix, oix, q, f, (data, cent, lab)
+
+
+@pytest.mark.parametrize("k,nprobe", [(100, 8), (10, 24)])
+def test_shape_unpruned_against_oracle(shape_case, k, nprobe):
+    # z = +inf: nothing is pruned, the result is the exact estimator top-k
+    # over the probed lists.  The plain form equals the oracle bit for bit;
+    # the batched form has the same ids, and its distances are within one
+    # ulp (its cascade adds the fp64 run terms in a group-of-one order).
+    ix, oix, q, f, _ = shape_case
+    print(f"me8q shape {f['dim']} unpruned k={k} nprobe={nprobe}", flush=True)
+    oi, od, ost = oix.search(q, k, nprobe, z=INF)
+    pi, pd, _, pst = ix.search(q, k, nprobe, INF)
+    np.testing.assert_array_equal(pi, oi)
+    np.testing.assert_array_equal(pd.view(np.uint32), od.view(np.uint32))
+    assert pst.candidates == int(ost[:, 0].sum())
+    bi, bd, _, bst = ix.search(q, k, nprobe, INF, mode=BATCHED)
+    assert bst.form == BATCHED and bst.overflowed == 0
+    assert bst.candidates == int(ost[:, 0].sum())
+    oi, od = _sorted_rows(oi, od)
+    bi, bd = _sorted_rows(bi, bd)
+    np.testing.assert_array_equal(bi, oi)
+    assert int(np.max(_ulps(bd, od))) <= 1
+
+
+@pytest.mark.parametrize("k,nprobe", [(100, 8), (10, 24)])
+def test_shape_pruned_against_oracle(shape_case, k, nprobe):
+    # The reference's z: the plain form is the oracle bit for bit (ids,
+    # distances, per-query counters); every distance the batched form
+    # returns is that vector's exact estimator distance (the oracle's
+    # unpruned search), and its k-th distance is never above the plain form's.
+    ix, oix, q, f, _ = shape_case
+    z = _z(k, 0.0)
+    print(f"me8q shape {f['dim']} pruned k={k} nprobe={nprobe}", flush=True)
+    oi, od, ost = oix.search(q, k, nprobe)
+    pi, pd, pst, _ = ix.search(q, k, nprobe, z)
+    np.testing.assert_array_equal(pi, oi)
+    np.testing.assert_array_equal(pd.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(pst, ost)
+    bi, bd, _, bst = ix.search(q, k, nprobe, z, mode=BATCHED)
+    assert bst.form == BATCHED
+    ei, ed, _ = oix.search(q, 1024, nprobe, z=INF)
+    found = 0
+    for i in range(q.shape[0]):
+        ex = {int(a): float(b) for a, b in zip(ei[i], ed[i]) if a != 0xFFFFFFFF}
+        for a, b in zip(bi[i], bd[i]):
+            if a == 0xFFFFFFFF:
+                continue
+            assert int(a) in ex, f"query {i}: id {a} is not among the nearest probed vectors"
+            assert _ulps(b, ex[int(a)]) <= 1, f"query {i}: id {a} distance {b} vs exact {ex[int(a)]}"
+        kb, kp = bd[i][bd[i] < INF], pd[i][pd[i] < INF]
+        if kb.size == kp.size and kb.size > 0:
+            assert kb[-1] <= kp[-1] + 1e-6 * max(1.0, abs(float(kp[-1])))
+        found += len(set(map(int, bi[i])) & set(map(int, oi[i])))
+    rec = found / float(q.shape[0] * k)
+    print(f"  overlap with the oracle's pruned search: {rec:.4f}", flush=True)
+    assert rec >= 0.97
+
+
+def test_shape_coarse_sums_checksum(shape_case):
+    ix, oix, q, f, _ = shape_case
+    npq = np.random.default_rng(3).integers(1, f["nlist"] + 1, q.shape[0]).astype(np.uint32)
+    print(f"me8q shape {f['dim']} checksum", flush=True)
+    got = ix.batch_checksum(q, 10, f["nlist"], _z(10, 0.0), nprobe_per_query=npq)
+    np.testing.assert_array_equal(got, _coarse_checksum(ix, f, q, npq))

@@ -24,9 +24,9 @@ def _mixed_requests(base_q, n_req, seed, ks, nprobes):
     return out
 
 
-def _serve_and_compare(idx, reqs, workers):
+def _serve_and_compare(idx, reqs, workers, mode="plain"):
     import e8qlib
-    srv = e8qlib.Server(idx, workers=workers, max_batch=256)
+    srv = e8qlib.Server(idx, workers=workers, max_batch=256, mode=mode)
     tickets = [None] * len(reqs)
 
     def client(part):
@@ -43,7 +43,7 @@ def _serve_and_compare(idx, reqs, workers):
     st = srv.stats()
     srv.close()
     for (q, k, npb), (ids, d, info) in zip(reqs, results):
-        ai, ad = idx.search(q, k, npb)
+        ai, ad = idx.search(q, k, npb) if mode == "plain" else idx.search(q, k, npb, mode=mode)
         np.testing.assert_array_equal(ids, ai)
         np.testing.assert_array_equal(d.view(np.uint32), ad.view(np.uint32))
     assert st["batches"] < len(reqs), st
@@ -65,6 +65,27 @@ def test_me8q_served_equals_alone():
     base_q = np.load(os.path.join(G, "m400a4_me8q.npz"))["queries"]
     print("serve me8q", flush=True)
     _serve_and_compare(idx, _mixed_requests(base_q, 60, 2, (10, 100), (1, 2, 5, 9)), workers=2)
+
+
+def test_me8q_batched_served_equals_alone():
+    # The batched ME8Q form (the served 10M x 1024 configuration's form) at
+    # that configuration's plan on Zipf-sized lists: each request's ids and
+    # distance bits are the same alone and in the mix (every threshold and
+    # candidate split of the form is per query).
+    import e8qlib
+    from test_gpu_me8q import _oracle_case
+    from test_gpu_me8q_batched import _zipf_sizes
+    data, cent, lab, q, _oix = _oracle_case(101, 0, 1024, 64, 4.0, spec="64:11,128:6,384:4,448:2",
+                                            sizes=_zipf_sizes(30000, 64))
+    idx = e8qlib.Me8qIndex(1024, 4.0, 0)
+    idx.build(data, cent, lab, plan_spec="64:11,128:6,384:4,448:2")
+    print("serve me8q batched", flush=True)
+    reqs = _mixed_requests(q, 48, 3, (10, 100), (1, 3, 8, 17))
+    st = _serve_and_compare(idx, reqs, workers=2, mode="batched")
+    print(f"  {st}", flush=True)
+    # the batched form ran: its stats name it
+    _i, _d, s = idx.search(reqs[0][0], reqs[0][1], reqs[0][2], return_stats=True, mode="batched")
+    assert s["form"] == "batched"
 
 
 def test_admission_rejects_oversized_request():
