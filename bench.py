@@ -131,12 +131,8 @@ def make_data(torch, n, dim, nlist, nq, seed=42, center_sd=2.0):
     base = centers[lab] + torch.randn(n, dim, generator=g, device=dev)
     qlab = torch.randint(0, nlist, (nq,), generator=g, device=dev)
     queries = centers[qlab] + torch.randn(nq, dim, generator=g, device=dev)
-    cn = (centers * centers).sum(1)
-    cid = torch.empty(n, dtype=torch.int64, device=dev)
-    for s in range(0, n, 262144):
-        x = base[s:s + 262144]
-        d = cn[None, :] - 2.0 * (x @ centers.T)
-        cid[s:s + 262144] = d.argmin(1)
+    from e8qlib import prep
+    cid = prep.assign_ivf(base, centers).to(torch.int64)
     return base.contiguous(), centers.contiguous(), cid, queries.contiguous()
 
 
@@ -247,28 +243,14 @@ def make_me8q_data(torch, n, dim, nlist, nq, seed=42):
     sd = lam.sqrt().float()
     base = (torch.randn(n, dim, generator=g, device=dev) * sd) @ rot.T
     queries = (torch.randn(nq, dim, generator=g, device=dev) * sd) @ rot.T
-    mean = base.double().mean(0)
-    cov = torch.zeros(dim, dim, dtype=torch.float64, device=dev)
-    for s0 in range(0, n, 131072):
-        x = base[s0:s0 + 131072].double() - mean
-        cov += x.T @ x
-    cov /= n
-    ev, vec = np.linalg.eigh(cov.cpu().numpy())
-    order = np.argsort(ev)[::-1]
-    var = ev[order].astype(np.float32)
-    V = torch.from_numpy(vec[:, order].copy()).to(dev)
-    meanf = mean.float()
-    Vf = V.float()
-    for s0 in range(0, n, 131072):
-        base[s0:s0 + 131072] = (base[s0:s0 + 131072] - meanf) @ Vf
-    queries = (queries - meanf) @ Vf
+    from e8qlib import prep
+    pca = prep.pca_fit(base)
+    prep.pca_transform(pca, base, out=base)
+    queries = prep.pca_transform(pca, queries)
+    var = pca.variance.astype(np.float32)
     pick = torch.randperm(n, generator=g, device=dev)[:nlist]
     centers = base[pick].clone()
-    cn = (centers * centers).sum(1)
-    cid = torch.empty(n, dtype=torch.int64, device=dev)
-    for s0 in range(0, n, 65536):
-        x = base[s0:s0 + 65536]
-        cid[s0:s0 + 65536] = (cn[None, :] - 2.0 * (x @ centers.T)).argmin(1)
+    cid = prep.assign_ivf(base, centers).to(torch.int64)
     return base.contiguous(), centers.contiguous(), cid, queries.contiguous(), var
 
 
@@ -667,7 +649,15 @@ def make_zipf_data(torch, n, dim, nlist, nq, seed=7, s=1.1, spectrum=1.0):
         x = centers[lab[st:e]] + torch.randn(e - st, dim, generator=g, device=dev) * sd
         base[st:e] = x / x.norm(dim=1, keepdim=True)
     cent = torch.zeros(nlist, dim, device=dev, dtype=torch.float64)
-    for st in range(0, n, 1 << 20):
+    # lists are contiguous row ranges, largest first: the large ones summed
+    # directly (an index_add_ into one row serializes on its atomics), the
+    # rest with index_add_
+    off = torch.cat([torch.zeros(1, dtype=torch.int64), sizes.cumsum(0)]).tolist()
+    big = int((sizes >= 65536).sum())
+    for c in range(big):
+        for st in range(off[c], off[c + 1], 1 << 20):
+            cent[c] += base[st:min(off[c + 1], st + (1 << 20))].double().sum(0)
+    for st in range(off[big], n, 1 << 20):
         e = min(n, st + (1 << 20))
         cent.index_add_(0, lab[st:e], base[st:e].double())
     cent = (cent / sizes.to(dev).clamp(min=1)[:, None].double()).float()
@@ -801,29 +791,16 @@ def run_zipf(args):
     print(json.dumps(line), flush=True)
 
 
-def pca_domain(torch, base, queries, cent, chunk=131072):
-    """The reference runner's PCA step (fp64 covariance, eigh, descending),
-    applied in place to base and to queries / centroids; returns the
-    eigenvalues (the per-dimension variance)."""
+def pca_domain(torch, base, queries, cent):
+    """The reference's PCA step (e8qlib.prep: fp64 covariance, eigh,
+    descending, fp64 transform) applied in place to base and to queries /
+    centroids; returns the eigenvalues (the per-dimension variance)."""
     import numpy as np
-    n, dim = base.shape
-    dev = base.device
-    mean = torch.zeros(dim, dtype=torch.float64, device=dev)
-    for s0 in range(0, n, chunk):
-        mean += base[s0:s0 + chunk].double().sum(0)
-    mean /= n
-    cov = torch.zeros(dim, dim, dtype=torch.float64, device=dev)
-    for s0 in range(0, n, chunk):
-        x = base[s0:s0 + chunk].double() - mean
-        cov += x.T @ x
-    cov /= n
-    ev, vec = np.linalg.eigh(cov.cpu().numpy())
-    order = np.argsort(ev)[::-1]
-    V = torch.from_numpy(vec[:, order].copy()).to(dev).float()
-    meanf = mean.float()
-    for s0 in range(0, n, chunk):
-        base[s0:s0 + chunk] = (base[s0:s0 + chunk] - meanf) @ V
-    return ((queries - meanf) @ V).contiguous(), ((cent - meanf) @ V).contiguous(), ev[order].astype(np.float32)
+    from e8qlib import prep
+    pca = prep.pca_fit(base)
+    prep.pca_transform(pca, base, out=base)
+    return (prep.pca_transform(pca, queries).contiguous(), prep.pca_transform(pca, cent).contiguous(),
+            pca.variance.astype(np.float32))
 
 
 def serve_stream(rng, qpool, count, nprobes):
