@@ -1034,9 +1034,13 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
             const uint32_t bj = pos >> 5, l = pos & 31u;
             const uint32_t cntb = static_cast<uint32_t>(min<uint64_t>(32, cnt - static_cast<uint64_t>(bj) * 32));
             const uint32_t covered = cntb >= 8 ? (cntb & ~7u) : 0u;
-            const uint8_t* rec = v.plane + (v.batch_begin[c] + bj) * v.rec;
-            const uint8_t* prow = rec + static_cast<uint64_t>(l) * PC;
-            const uint8_t* banks = rec + 32ull * PC;
+            const uint64_t gb = v.batch_begin[c] + bj;
+            const uint8_t* prow = v.plane + gb * v.rec + static_cast<uint64_t>(l) * PC;
+            // the candidate's refine record: run headers, its W / Omega bank
+            // bytes, the run fields; the batch's scales per run
+            const uint8_t* rr = v.rrec + (lb + pos) * v.rrec_bytes;
+            const uint8_t* rbank = rr + 16 * R;
+            const float4* bsc = v.bscale + gb * R;
             // algorithmic bytes: the pool entry, the plane row, per run its
             // 16-byte header, its W and Omega bank bytes and 1/32 of the
             // batch's scales
@@ -1078,91 +1082,125 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
                     }
                 }
             } else {
-                for (uint32_t r = 0; r < R; ++r) {
-                    const uint32_t b0 = v.runs[r].blk0, b1 = b0 + v.runs[r].nblk;
-                    for (uint32_t col = b0; col < b1; col += 8)
-                        part[r] += word_sum(*reinterpret_cast<const uint64_t*>(prow + col), col);
+#pragma unroll
+                for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                    if (r < R) {
+                        const uint32_t b0 = v.runs[r].blk0, b1 = b0 + v.runs[r].nblk;
+                        for (uint32_t col = b0; col < b1; col += 8)
+                            part[r] += word_sum(*reinterpret_cast<const uint64_t*>(prow + col), col);
+                    }
                 }
             }
-            // the coarse terms in the reference's arithmetic, against tau
-            double ssc[kM8BatchRuns], str[kM8BatchRuns], pool = 0.0, qadd = 0.0;
+            // The coarse terms in the reference's arithmetic, against tau, run by
+            // run in run order.  Every private array here is indexed by
+            // compile-time constants only (guarded, unrolled run loops; Pick for
+            // a run-time position) so that it stays in registers.
+            double ssc[kM8BatchRuns] = {0.0, 0.0, 0.0, 0.0}, str[kM8BatchRuns] = {0.0, 0.0, 0.0, 0.0};
+            double pool = 0.0, qadd = 0.0;
             if (l < covered) {   // the AVX2 filter's fp32 values
                 float pl = 0.0F, qa = 0.0F, wpad = 0.0F;
-                for (uint32_t r = 0; r < R; ++r) {
-                    const uint8_t* bp = banks + r * kM8BankBytes;
-                    const float* sc = reinterpret_cast<const float*>(bp + 128);
-                    const float s_c = fmaf(static_cast<float>(part[r]), lut_delta, static_cast<float>(rcq[r * 4 + 1]));
-                    const float wq = fmaf(static_cast<float>(bp[32 + l]), sc[3], sc[2]);
-                    const float pq = fmaf(static_cast<float>(bp[64 + l]), sc[5], sc[4]);
-                    const float ra = static_cast<float>(sqd2[r]);
-                    const float rh = sqk < ra ? sqk : ra;
-                    const float t = pq * rh;
-                    ssc[r] = s_c;
-                    str[r] = t;
-                    pl = fmaf(t, t, pl);
-                    qa = fmaf(fabsf(wq), static_cast<float>(rcq[r * 4 + 2]), qa);
-                    wpad = wpad + static_cast<float>(0.5 * static_cast<double>(sc[3]) * (rcq[r * 4 + 3] + rcq[r * 4 + 2]));
+#pragma unroll
+                for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                    if (r < R) {
+                        const float4 s4 = bsc[r];
+                        const float s_c = fmaf(static_cast<float>(part[r]), lut_delta, static_cast<float>(rcq[r * 4 + 1]));
+                        const float wq = fmaf(static_cast<float>(rbank[2 * r]), s4.y, s4.x);
+                        const float pq = fmaf(static_cast<float>(rbank[2 * r + 1]), s4.w, s4.z);
+                        const float ra = static_cast<float>(sqd2[r]);
+                        const float rh = sqk < ra ? sqk : ra;
+                        const float t = pq * rh;
+                        ssc[r] = s_c;
+                        str[r] = t;
+                        pl = fmaf(t, t, pl);
+                        qa = fmaf(fabsf(wq), static_cast<float>(rcq[r * 4 + 2]), qa);
+                        wpad = wpad + static_cast<float>(0.5 * static_cast<double>(s4.y) * (rcq[r * 4 + 3] + rcq[r * 4 + 2]));
+                    }
                 }
                 pool = pl;
                 qadd = qa + wpad;
             } else {   // the fp64 twin for the lanes the filter does not cover
-                for (uint32_t r = 0; r < R; ++r) {
-                    const uint8_t* bp = banks + r * kM8BankBytes;
-                    const float* scl = reinterpret_cast<const float*>(bp + 128);
-                    const double scr = fma(static_cast<double>(lut_delta), static_cast<double>(part[r]), rcq[r * 4 + 1]);
-                    const double wq = fma(static_cast<double>(bp[32 + l]), static_cast<double>(scl[3]), static_cast<double>(scl[2]));
-                    const double pq = fma(static_cast<double>(bp[64 + l]), static_cast<double>(scl[5]), static_cast<double>(scl[4]));
-                    const double sa = sqd2[r];
-                    const double rh = sqdk < sa ? sqdk : sa;
-                    const double tr = pq * rh;
-                    ssc[r] = scr;
-                    str[r] = tr;
-                    pool = fma(tr, tr, pool);
-                    qadd = fma(fabs(wq), rcq[r * 4 + 2], qadd);
-                    qadd = fma(0.5 * static_cast<double>(scl[3]), rcq[r * 4 + 3] + rcq[r * 4 + 2], qadd);
-                }
-            }
-            // the cascade's run order for a group of one: decreasing t^2 / nblk, stable
-            uint32_t ord[kM8BatchRuns] = {0, 1, 2, 3};
-            if (R > 1) {
-                double mean[kM8BatchRuns];
-                for (uint32_t r = 0; r < R; ++r)
-                    mean[r] = fma(str[r], str[r], 0.0) / static_cast<double>(v.runs[r].nblk > 1 ? v.runs[r].nblk : 1);
-                for (uint32_t i = 1; i < R; ++i) {
-                    const uint32_t x = ord[i];
-                    uint32_t m = i;
-                    while (m > 0 && mean[x] > mean[ord[m - 1]]) {
-                        ord[m] = ord[m - 1];
-                        --m;
+#pragma unroll
+                for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                    if (r < R) {
+                        const float4 s4 = bsc[r];
+                        const double scr = fma(static_cast<double>(lut_delta), static_cast<double>(part[r]), rcq[r * 4 + 1]);
+                        const double wq = fma(static_cast<double>(rbank[2 * r]), static_cast<double>(s4.y), static_cast<double>(s4.x));
+                        const double pq = fma(static_cast<double>(rbank[2 * r + 1]), static_cast<double>(s4.w), static_cast<double>(s4.z));
+                        const double sa = sqd2[r];
+                        const double rh = sqdk < sa ? sqdk : sa;
+                        const double tr = pq * rh;
+                        ssc[r] = scr;
+                        str[r] = tr;
+                        pool = fma(tr, tr, pool);
+                        qadd = fma(fabs(wq), rcq[r * 4 + 2], qadd);
+                        qadd = fma(0.5 * static_cast<double>(s4.y), rcq[r * 4 + 3] + rcq[r * 4 + 2], qadd);
                     }
-                    ord[m] = x;
                 }
             }
-            const uint8_t* row = v.refine + (lb + pos) * v.rstride;
-            const float* f4 = reinterpret_cast<const float*>(row);
+            const float* f4 = reinterpret_cast<const float*>(rr);
             double dist;
             if (R == 1) {
                 dist = d2 + static_cast<double>(f4[1]);
             } else {
                 double dcq = d2;
-                for (uint32_t r = 0; r < R; ++r) {
-                    const double coef = f4[r * 4], h2 = f4[r * 4 + 1];
-                    const double gn = f4[r * 4 + 2], vp = f4[r * 4 + 3];
-                    dcq += fma(-(2.0 * coef), fma(gn, ssc[r], vp), h2);
+#pragma unroll
+                for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                    if (r < R) {
+                        const double coef = f4[r * 4], h2 = f4[r * 4 + 1];
+                        const double gn = f4[r * 4 + 2], vp = f4[r * 4 + 3];
+                        dcq += fma(-(2.0 * coef), fma(gn, ssc[r], vp), h2);
+                    }
                 }
                 dist = dcq;
             }
+            // The cascade's run order for a group of one: decreasing t^2 / nblk,
+            // stable (GCC's insertion sort below 17 elements): a sorting network
+            // on the total order (mean descending, run ascending) -- the same
+            // order -- carrying each run's coarse score and its t; runs past R
+            // sort last.
+            uint32_t ord[kM8BatchRuns] = {0, 1, 2, 3};
+            if (R > 1) {
+                double mean[kM8BatchRuns];
+#pragma unroll
+                for (uint32_t r = 0; r < kM8BatchRuns; ++r)
+                    mean[r] = r < R ? fma(str[r], str[r], 0.0) /
+                                          static_cast<double>(v.runs[r].nblk > 1 ? v.runs[r].nblk : 1)
+                                    : -1.0;
+                auto cswap = [&](uint32_t i, uint32_t j) {   // i < j: j first if it sorts before i
+                    const bool sw = mean[j] > mean[i] || (mean[j] == mean[i] && ord[j] < ord[i]);
+                    const double m0 = mean[i], s0 = ssc[i], t0 = str[i];
+                    const uint32_t o0 = ord[i];
+                    mean[i] = sw ? mean[j] : m0;
+                    ssc[i] = sw ? ssc[j] : s0;
+                    str[i] = sw ? str[j] : t0;
+                    ord[i] = sw ? ord[j] : o0;
+                    mean[j] = sw ? m0 : mean[j];
+                    ssc[j] = sw ? s0 : ssc[j];
+                    str[j] = sw ? t0 : str[j];
+                    ord[j] = sw ? o0 : ord[j];
+                };
+                cswap(0, 1);
+                cswap(2, 3);
+                cswap(0, 2);
+                cswap(1, 3);
+                cswap(1, 2);
+            }
+            auto pick = [](const auto (&x)[kM8BatchRuns], uint32_t i) {
+                return i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : x[3];
+            };
+            const uint32_t lo0 = a.ref_lev_off[0], lo1 = a.ref_lev_off[1], lo2 = a.ref_lev_off[2],
+                           lo3 = a.ref_lev_off[3];
             double rem2 = pool;
             live = true;
             uint64_t bits = 0;
             const E8Q_GLOBAL float* qrot = G(a.rotq) + static_cast<uint64_t>(q) * P;
             const uint32_t nsteps = kFirst ? 1u : R;
             for (uint32_t oi = 0; oi < nsteps && live; ++oi) {
-                const uint32_t r = ord[oi];
+                const uint32_t r = pick(ord, oi);
                 const M8Run& ru = v.runs[r];
                 const E8Q_GLOBAL uint8_t* pb = G(prow) + ru.blk0;
-                const E8Q_GLOBAL uint8_t* fr = G(row) + ru.ref_off;
-                const uint32_t lo = a.ref_lev_off[r];
+                const E8Q_GLOBAL uint8_t* fr = G(rr) + ru.ref_off + v.rrec_shift;
+                const uint32_t lo = r == 0 ? lo0 : r == 1 ? lo1 : r == 2 ? lo2 : lo3;
                 const double se =
                     lo != kM8LevGlobal
                         ? M8LaneScore(ru.rate, pb, fr, (const E8Q_LDS float*)(reinterpret_cast<const float*>(smem)) + lo,
@@ -1173,9 +1211,9 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
                     dist = fma(-(2.0 * coef), se, dist);
                 } else {
                     const double gn = f4[r * 4 + 2], vp = f4[r * 4 + 3];
-                    dist = fma(2.0 * coef, fma(gn, ssc[r], vp) - se, dist);
+                    dist = fma(2.0 * coef, fma(gn, pick(ssc, oi), vp) - se, dist);
                 }
-                const double t = str[r];
+                const double t = pick(str, oi);
                 rem2 = fma(-t, t, rem2);
                 if (kFirst || oi > 0) bits += static_cast<uint64_t>(ru.rate) * ru.len;
                 n_bytes += static_cast<uint64_t>(ru.rate - 1) * ru.len / 8;   // the run's fields

@@ -298,8 +298,63 @@ __global__ __launch_bounds__(256) void M8BanksKernel(M8View v, const float* __re
     scl[7] = 0.0F;
 }
 
+// The batched form's refine records (rrec) and batch scales (bscale) from the
+// refine rows and the plane records' banks: one thread per 8-byte word of a
+// record.  A record is the refine row's float4 header, the vector's W and
+// Omega bank bytes per run (zero padded to rrec_shift bytes), the refine
+// row's fields, zeros to the end of its last 128-byte line.
+__global__ __launch_bounds__(256) void M8RefineRecKernel(M8View v) {
+    const uint64_t words = v.rrec_bytes / 8;
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= v.n * words) return;
+    const uint64_t gi = t / words;
+    const uint32_t o = static_cast<uint32_t>(t % words) * 8;
+    const uint32_t R = v.nrun, hdr = 16 * R;
+    const uint32_t fields = v.rstride_fields;   // bytes of the fields after the header
+    const uint8_t* row = v.refine + gi * v.rstride;
+    uint64_t w = 0;
+    if (o < hdr) {
+        w = *reinterpret_cast<const uint64_t*>(row + o);
+    } else if (o < hdr + v.rrec_shift) {
+        const uint32_t c = M8ListOf(v.list_begin, v.nlist, gi);
+        const uint64_t i = gi - v.list_begin[c];
+        const uint8_t* banks = v.plane + (v.batch_begin[c] + i / 32) * v.rec + 32u * v.plane_cols;
+        const uint32_t l = static_cast<uint32_t>(i % 32);
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t j = o - hdr + k;   // byte j of the bank area: run j / 2, W (even) or Omega (odd)
+            if (j < 2 * R) w |= static_cast<uint64_t>(banks[(j / 2) * kM8BankBytes + 32 * (1 + (j & 1)) + l]) << (8 * k);
+        }
+    } else if (o < hdr + v.rrec_shift + fields) {
+        w = *reinterpret_cast<const uint64_t*>(row + o - v.rrec_shift);
+    }
+    *reinterpret_cast<uint64_t*>(v.rrec + gi * v.rrec_bytes + o) = w;
+}
+
+__global__ __launch_bounds__(256) void M8BatchScaleKernel(M8View v, uint64_t nbatch) {
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nbatch * v.nrun) return;
+    const uint64_t gb = t / v.nrun;
+    const uint32_t r = static_cast<uint32_t>(t % v.nrun);
+    const float* scl = reinterpret_cast<const float*>(v.plane + gb * v.rec + 32u * v.plane_cols + r * kM8BankBytes + 128);
+    v.bscale[t] = make_float4(scl[2], scl[3], scl[4], scl[5]);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
+
+hipError_t LaunchM8RefineRecords(hipStream_t st, const M8View& v, uint64_t nbatch) {
+    const uint64_t t1 = v.n * (v.rrec_bytes / 8);
+    if (t1 > 0) {
+        hipLaunchKernelGGL(M8RefineRecKernel, dim3(static_cast<uint32_t>((t1 + 255) / 256)), dim3(256), 0, st, v);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const uint64_t t2 = nbatch * v.nrun;
+    if (t2 > 0)
+        hipLaunchKernelGGL(M8BatchScaleKernel, dim3(static_cast<uint32_t>((t2 + 255) / 256)), dim3(256), 0, st, v,
+                           nbatch);
+    return hipGetLastError();
+}
 
 hipError_t LaunchM8Centroids(hipStream_t st, const M8View& v) {
     if (v.nlist == 0) return hipSuccess;

@@ -14,6 +14,14 @@
 //   refine   per vector, `rstride` bytes: nrun x float4 {coef, h2, gain, vpc}
 //            (the reference keeps gain/vpc in separate per-list arrays), then
 //            each run's (rate-1)-bit refine fields (me8q_fast.cpp:97-118).
+//   rrec     (indexes of up to four coded runs: the batched form's refine)
+//            per vector, `rrec_bytes` (whole 128-byte lines): the refine
+//            row's nrun float4 header, then the vector's W and Omega bank
+//            bytes per run (padded to `rrec_shift` bytes), then the refine
+//            row's fields -- everything the exact cascade reads for one
+//            candidate besides its plane row, on as few lines as it fits.
+//   bscale   per 32-vector batch, nrun x float4 {W_min, W_step, Omega_min,
+//            Omega_step}: the bank scales of the batch record.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -68,6 +76,11 @@ struct M8View {
     double* rc_n2;        // [nlist][nrun] per-run centroid energy
     uint8_t* plane;
     uint8_t* refine;
+    uint8_t* rrec;        // null when the index has more than four coded runs
+    float4* bscale;
+    uint32_t rrec_bytes, rrec_shift;
+    uint32_t rstride_fields;   // bytes of a refine row's fields (after its 16 x nrun header)
+    uint32_t pad_;
 };
 
 // `n` (<= 25) bits at bit `pos` of a little-endian byte string.

@@ -19,6 +19,8 @@ hipError_t LaunchM8Encode(hipStream_t st, const M8View& v, const float* base, co
                           const uint32_t* rank, uint64_t n);
 // perp: n*nrun float scratch; the plane buffer must be zero beforehand.
 hipError_t LaunchM8Fast(hipStream_t st, const M8View& v, float* perp, uint64_t nbatch);
+// The batched form's per-vector refine records and per-batch bank scales (rrec, bscale).
+hipError_t LaunchM8RefineRecords(hipStream_t st, const M8View& v, uint64_t nbatch);
 
 // ---- search (me8q_search.hip)
 // Per-query prep: padded, per-run rotated query (rotq [nq][padded]), the

@@ -30,12 +30,12 @@ struct e8q_m8index {
     std::vector<uint32_t> ref_run_off;
     uint32_t ref_stride = 0;
     DBuf d_runs, d_segs, d_list_begin, d_batch_begin, ids, codes, centroids, cent_rot, cn2, rc_n2, plane, refine,
-        signs, lev;
+        signs, lev, rrec, bscale;
     std::vector<DBuf> enc;   // encoder tables, 5 per distinct rate
     uint64_t device_bytes() const {
         uint64_t b = d_runs.bytes + d_segs.bytes + d_list_begin.bytes + d_batch_begin.bytes + ids.bytes + codes.bytes +
                      centroids.bytes + cent_rot.bytes + cn2.bytes + rc_n2.bytes + plane.bytes + refine.bytes +
-                     signs.bytes + lev.bytes;
+                     signs.bytes + lev.bytes + rrec.bytes + bscale.bytes;
         for (const auto& e : enc) b += e.bytes;
         return b;
     }
@@ -57,6 +57,8 @@ void RefreshM8View(e8q_m8index* x) {
     v.rc_n2 = x->rc_n2.as<double>();
     v.plane = x->plane.as<uint8_t>();
     v.refine = x->refine.as<uint8_t>();
+    v.rrec = x->rrec.as<uint8_t>();
+    v.bscale = x->bscale.as<float4>();
 }
 
 // Rotated centroids + FastScan layout from codes / centroids already on the device.
@@ -69,6 +71,7 @@ void BuildDerived(e8q_ctx* ctx, e8q_m8index* x) {
         DBuf perp;
         perp.alloc(v.n * v.nrun * sizeof(float));
         Check(LaunchM8Fast(st, v, perp.as<float>(), x->nbatch), "fast-layout kernels");
+        if (v.rrec) Check(LaunchM8RefineRecords(st, v, x->nbatch), "refine records");
         Check(hipStreamSynchronize(st), "fast layout");
     } else {
         Check(hipStreamSynchronize(st), "centroids");
@@ -651,6 +654,9 @@ e8q_status e8q_m8_create(e8q_ctx* ctx, const e8q_m8_desc* d, e8q_m8index** out) 
             }
             v.stride = code;
             v.rstride = (ref + 15) & ~15u;
+            v.rstride_fields = ref - nrun * 16;
+            v.rrec_shift = (2 * nrun + 7) & ~7u;
+            v.rrec_bytes = (nrun * 16 + v.rrec_shift + v.rstride_fields + 127) & ~127u;
             x->ref_stride = refr;
             v.rec = 32u * v.plane_cols + kM8BankBytes * nrun;
             // list / batch offsets
@@ -731,6 +737,11 @@ e8q_status e8q_m8_create(e8q_ctx* ctx, const e8q_m8_desc* d, e8q_m8index** out) 
             x->rc_n2.alloc(v.nlist * std::max<uint32_t>(nrun, 1) * sizeof(double));
             x->plane.alloc(x->nbatch * v.rec + 16);
             x->refine.alloc(v.n * v.rstride + 16);
+            // the batched form's refine records, for indexes it runs on
+            if (nrun >= 1 && nrun <= kM8BatchRuns && v.n > 0) {
+                x->rrec.alloc(v.n * v.rrec_bytes);
+                x->bscale.alloc(x->nbatch * nrun * sizeof(float4));
+            }
             Check(hipMemsetAsync(x->codes.p, 0, x->codes.bytes, st), "clear");
             Check(hipMemsetAsync(x->refine.p, 0, x->refine.bytes, st), "clear");
             Check(hipStreamSynchronize(st), "m8 create");
