@@ -85,9 +85,9 @@ __device__ inline void RotateInLds(float* buf, uint32_t P, uint32_t T, float fac
             if ((sg[i >> 3] >> (i & 7)) & 1) buf[i] = -buf[i];
         __syncthreads();
         float* win = buf + ((!kac || (r & 1) == 0) ? 0 : P - T);
-        for (uint32_t h = 1; h < T; h <<= 1) {
+        for (uint32_t h = 1, lg = 0; h < T; h <<= 1, ++lg) {   // h = 2^lg: pair j, j + h of butterfly t
             for (uint32_t t = threadIdx.x; t < T / 2; t += blockDim.x) {
-                const uint32_t j = (t / h) * 2 * h + (t % h);
+                const uint32_t j = ((t >> lg) << (lg + 1)) | (t & (h - 1));
                 const float a = win[j], b = win[j + h];
                 win[j] = a + b;
                 win[j + h] = a - b;

@@ -6,7 +6,8 @@
 // branches, packed bytes, g1/h3 and the u8 banks come out bit-identical.
 //
 // Kernels:
-//   MeanKernel        fp64 column sums in row order (one chain per dim)
+//   MeanKernel        fp64 column sums in row order (one lane per dim; four
+//                     waves stage rows through an LDS ring)
 //   MeanFinishKernel  mean -> float -> rotated mean
 //   CentroidKernel    rotate, center, fp64 ||c||^2
 //   EncodeRowsKernel  one 64-lane workgroup per row: rotate, residual norm,
@@ -21,27 +22,80 @@ namespace e8qdev {
 
 // ---------------------------------------------------------------------------
 // Mean: fp64 sum per dimension in row order (e8q_build.cpp:190-204).  A
-// workgroup owns 64 dims; all 256 threads stage 64-row tiles through LDS so
-// the loads stay coalesced, and 64 threads run the dependent adds.
+// workgroup owns 64 consecutive dimensions.  Its four waves load 64-row
+// stages (16 rows each, one coalesced 256-byte access per row) four stages
+// ahead into registers and copy each stage into an LDS ring; wave 0 then runs
+// the 64 dependent fp64 chains (one lane per dimension) over the stage in
+// row order.
+constexpr int kMeanRows = 16;   // rows per wave per stage
+constexpr int kMeanStage = 4 * kMeanRows;
+
+// Rows of stage `stage` for this wave.  Row addresses are clamped into the
+// array (wave-uniform, so they stay in scalar registers) so every load is
+// unconditional and nothing uses a loaded value before it is copied to LDS
+// four stages later: rows past the array are never added (the last stage
+// adds only its rows < n) and lanes past `dim` never store their sum.
+__device__ __forceinline__ void MeanLoad(float (&g)[kMeanRows], const float* __restrict__ base, uint64_t stage,
+                                         uint32_t wave, uint64_t n, uint32_t dim, uint32_t dc) {
+    const uint64_t r0 = stage * kMeanStage + wave * kMeanRows;
+#pragma unroll
+    for (int i = 0; i < kMeanRows; ++i) {
+        const uint64_t r = r0 + i;
+        g[i] = base[(r < n ? r : n - 1) * dim + dc];
+    }
+}
+
 __global__ __launch_bounds__(256) void MeanKernel(const float* __restrict__ base, uint64_t n, uint32_t dim,
                                                   double* __restrict__ mu) {
-    __shared__ float tile[64][65];
-    const uint32_t d0 = blockIdx.x * 64;
+    __shared__ float ring[2][kMeanStage][64];
+    const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t d = blockIdx.x * 64 + lane;
+    const bool live = d < dim;
+    const uint32_t dc = live ? d : dim - 1;
+    const uint64_t nfull = n / kMeanStage, nst = (n + kMeanStage - 1) / kMeanStage;
+    float g0[kMeanRows], g1[kMeanRows], g2[kMeanRows], g3[kMeanRows];
+    MeanLoad(g0, base, 0, wave, n, dim, dc);
+    MeanLoad(g1, base, 1, wave, n, dim, dc);
+    MeanLoad(g2, base, 2, wave, n, dim, dc);
+    MeanLoad(g3, base, 3, wave, n, dim, dc);
+    // Every wave runs the chains (only wave 0's result is stored), so the
+    // steady state has no branch.
     double acc = 0.0;
-    for (uint64_t r0 = 0; r0 < n; r0 += 64) {
-        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-            const int rr = e >> 6, dd = e & 63;
-            const uint64_t r = r0 + rr;
-            tile[rr][dd] = (r < n && d0 + dd < dim) ? base[r * dim + d0 + dd] : 0.0F;
-        }
+    auto publish = [&](float (&g)[kMeanRows], uint64_t s) {
+        float(*slot)[64] = ring[s & 1];
+#pragma unroll
+        for (int i = 0; i < kMeanRows; ++i) slot[wave * kMeanRows + i][lane] = g[i];
+        MeanLoad(g, base, s + 4, wave, n, dim, dc);
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const uint64_t rn = min<uint64_t>(64, n - r0);
-            for (uint64_t rr = 0; rr < rn; ++rr) acc += static_cast<double>(tile[rr][threadIdx.x]);
+        return slot;
+    };
+    auto add_full = [&](float (*slot)[64]) {
+#pragma unroll
+        for (int h = 0; h < kMeanStage; h += 16) {
+            float x[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) x[i] = slot[h + i][lane];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc += static_cast<double>(x[i]);
         }
-        __syncthreads();
+    };
+    auto add_rows = [&](float (*slot)[64], uint64_t s) {   // a stage that may be partial
+        const int rows = static_cast<int>(min<uint64_t>(kMeanStage, n - s * kMeanStage));
+        for (int i = 0; i < rows; ++i) acc += static_cast<double>(slot[i][lane]);
+    };
+    uint64_t s = 0;
+    for (; s + 4 <= nfull; s += 4) {
+        add_full(publish(g0, s));
+        add_full(publish(g1, s + 1));
+        add_full(publish(g2, s + 2));
+        add_full(publish(g3, s + 3));
     }
-    if (threadIdx.x < 64 && d0 + threadIdx.x < dim) mu[d0 + threadIdx.x] = acc;
+    // the last (at most four) stages, the final one possibly partial
+    if (s < nst) add_rows(publish(g0, s), s);
+    if (s + 1 < nst) add_rows(publish(g1, s + 1), s + 1);
+    if (s + 2 < nst) add_rows(publish(g2, s + 2), s + 2);
+    if (s + 3 < nst) add_rows(publish(g3, s + 3), s + 3);
+    if (wave == 0 && live) mu[d] = acc;
 }
 
 __global__ void MeanFinishKernel(const double* __restrict__ mu, uint64_t n, IndexView ix, float* __restrict__ mean_rot) {
@@ -83,7 +137,15 @@ __global__ void CentroidKernel(const float* __restrict__ cent, IndexView ix, flo
 // ---------------------------------------------------------------------------
 // Per-row encode.  Workgroup = one wave; rows in a grid-stride loop.
 constexpr int kEncThreads = 64;
+constexpr uint32_t kEncLdsLevels = 2048;   // merged tables up to this length (b <= 10) are copied to LDS
 
+// LDS of EncodeRowsKernel: rotated row / unit, rotated centroid, digits,
+// branches, then (kLdsTab) the merged level table and its digits.
+__host__ __device__ inline uint32_t EncodeRowsTabOffset(uint32_t P, uint32_t blocks) {
+    return (2 * P * 4 + P * 2 + blocks + 15) & ~15u;
+}
+
+template <bool kLdsTab>
 __global__ __launch_bounds__(kEncThreads) void EncodeRowsKernel(IndexView ix, EncodeView ev, EncodeArgs a) {
     extern __shared__ float sm[];
     const uint32_t P = static_cast<uint32_t>(ix.padded);
@@ -91,6 +153,24 @@ __global__ __launch_bounds__(kEncThreads) void EncodeRowsKernel(IndexView ix, En
     float* rc = sm + P;                                        // rotated centroid
     uint16_t* dg = reinterpret_cast<uint16_t*>(sm + 2 * P);    // digits
     uint8_t* brs = reinterpret_cast<uint8_t*>(dg + P);         // branches (blocks)
+    // kLdsTab: the merged table (mlen levels + digits), the per-branch level
+    // table (2 x 2^bits = mlen levels) and the 16 scan class means in LDS
+    double* tlev = reinterpret_cast<double*>(reinterpret_cast<uint8_t*>(sm) + EncodeRowsTabOffset(P, ix.blocks));
+    double* tbl = tlev + (kLdsTab ? ev.mlen : 0);
+    double* tgm = tbl + (kLdsTab ? ev.mlen : 0);
+    uint16_t* tdig = reinterpret_cast<uint16_t*>(tgm + (kLdsTab ? 16 : 0));
+    if (kLdsTab) {
+        for (uint32_t i = threadIdx.x; i < ev.mlen; i += kEncThreads) {
+            tlev[i] = ev.mlev[i];
+            tdig[i] = ev.mdig[i];
+            tbl[i] = ev.levels[i];
+        }
+        if (threadIdx.x < 16) tgm[threadIdx.x] = ix.gate_mid[threadIdx.x];
+    }
+    const double* mlev = kLdsTab ? tlev : ev.mlev;
+    const uint16_t* mdig = kLdsTab ? tdig : ev.mdig;
+    const double* blev = kLdsTab ? tbl : ev.levels;
+    const double* gmid = kLdsTab ? tgm : ix.gate_mid;
     __shared__ double s_fit[8];
     __shared__ double s_rn2;
     const int tid = threadIdx.x;
@@ -114,11 +194,14 @@ __global__ __launch_bounds__(kEncThreads) void EncodeRowsKernel(IndexView ix, En
             rc[i] = ix.centroid_rot[static_cast<uint64_t>(c) * P + i];
         }
         __syncthreads();
-        if (tid == 0) {
+        if (tid == 0) {   // P is a multiple of 64: eight coordinates' reads ahead of their FMAs
             double rn2 = 0.0;
-            for (uint32_t d = 0; d < P; ++d) {
-                const double rv = static_cast<double>(xb[d]) - static_cast<double>(rc[d]);
-                rn2 = fma(rv, rv, rn2);
+            for (uint32_t d0 = 0; d0 < P; d0 += 8) {
+                double rv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) rv[j] = static_cast<double>(xb[d0 + j]) - static_cast<double>(rc[d0 + j]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) rn2 = fma(rv[j], rv[j], rn2);
             }
             s_rn2 = rn2;
             a.ids[slot] = static_cast<uint32_t>(r);
@@ -137,7 +220,7 @@ __global__ __launch_bounds__(kEncThreads) void EncodeRowsKernel(IndexView ix, En
         for (int blk = tid; blk < ix.blocks; blk += kEncThreads) {
             uint16_t d8[8];
             int br;
-            EncodeBlock(xb + blk * 8, sr, ev, d8, &br);
+            EncodeBlock(xb + blk * 8, sr, ev, mlev, mdig, d8, &br);
             for (int i = 0; i < 8; ++i) dg[blk * 8 + i] = d8[i];
             brs[blk] = static_cast<uint8_t>(br);
             // rowA byte (e8q_scan.cpp:193-205)
@@ -167,35 +250,45 @@ __global__ __launch_bounds__(kEncThreads) void EncodeRowsKernel(IndexView ix, En
             }
         }
         __syncthreads();
-        // Seven fit chains, one lane each, in coordinate order (e8q_build.cpp:339-376):
+        // Seven fit chains in coordinate order (e8q_build.cpp:339-376), all on
+        // lane 0 and interleaved (each chain keeps its own order; one load of
+        // the operands per coordinate feeds the seven steps):
         // 0 <u,vE> 1 <c,vE> 2 ||vE||^2 3 <vE,w> 4 <u,w> 5 ||w||^2 6 <c,w>.
-        if (tid < 7) {
-            double acc = 0.0;
+        if (tid == 0) {
+            double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0, f5 = 0.0, f6 = 0.0;
             for (int blk = 0; blk < ix.blocks; ++blk) {
                 const uint32_t br = brs[blk];
+                double vev[8], wcv[8], uv[8], ccv[8];
+#pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
                     const uint32_t i = blk * 8 + kk;
                     const uint32_t digit = dg[i];
-                    const double ve = ev.levels[br * Q + digit] * inv_ls;
-                    const double wc = ix.gate_mid[2 * kk + (digit >> (bits - 1))];
-                    const double u = static_cast<double>(xb[i]);
-                    const double cc = static_cast<double>(rc[i]);
-                    double x, y;
-                    switch (tid) {
-                        case 0: x = u; y = ve; break;
-                        case 1: x = ve; y = cc; break;
-                        case 2: x = ve; y = ve; break;
-                        case 3: x = wc; y = ve; break;
-                        case 4: x = wc; y = u; break;
-                        case 5: x = wc; y = wc; break;
-                        default: x = wc; y = cc; break;
-                    }
+                    vev[kk] = blev[br * Q + digit] * inv_ls;
+                    wcv[kk] = gmid[2 * kk + (digit >> (bits - 1))];
+                    uv[kk] = static_cast<double>(xb[i]);
+                    ccv[kk] = static_cast<double>(rc[i]);
+                }
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    const double ve = vev[kk], wc = wcv[kk], u = uv[kk], cc = ccv[kk];
+                    f0 = fma(u, ve, f0);
+                    f1 = fma(ve, cc, f1);
+                    f2 = fma(ve, ve, f2);
+                    f3 = fma(wc, ve, f3);
+                    f4 = fma(wc, u, f4);
                     // ||w||^2 at the 8th coordinate of a block: separately rounded
                     // square (the reference build hoists it out of the FMA).
-                    acc = (tid == 5 && kk == 7) ? acc + x * y : fma(x, y, acc);
+                    f5 = kk == 7 ? f5 + wc * wc : fma(wc, wc, f5);
+                    f6 = fma(wc, cc, f6);
                 }
             }
-            s_fit[tid] = acc;
+            s_fit[0] = f0;
+            s_fit[1] = f1;
+            s_fit[2] = f2;
+            s_fit[3] = f3;
+            s_fit[4] = f4;
+            s_fit[5] = f5;
+            s_fit[6] = f6;
         }
         __syncthreads();
         if (tid == 0) {
@@ -392,9 +485,13 @@ hipError_t LaunchCentroids(hipStream_t st, const float* centroids, const IndexVi
 
 hipError_t LaunchEncodeRows(hipStream_t st, const IndexView& ix, const EncodeView& ev, const EncodeArgs& a) {
     if (a.n == 0) return hipSuccess;
-    const size_t lds = 2 * ix.padded * sizeof(float) + ix.padded * sizeof(uint16_t) + ix.blocks + 16;
-    const uint32_t grid = static_cast<uint32_t>(a.n < 32768 ? a.n : 32768);
-    hipLaunchKernelGGL(EncodeRowsKernel, dim3(grid), dim3(kEncThreads), lds, st, ix, ev, a);
+    const uint32_t P = static_cast<uint32_t>(ix.padded);
+    const bool tab = ev.merged && ev.mlen <= kEncLdsLevels;
+    const size_t lds =
+        EncodeRowsTabOffset(P, ix.blocks) + (tab ? ev.mlen * (2 * sizeof(double) + sizeof(uint16_t)) + 16 * sizeof(double) : 0) + 16;
+    const uint32_t grid = static_cast<uint32_t>(a.n < 131072 ? a.n : 131072);
+    if (tab) hipLaunchKernelGGL(EncodeRowsKernel<true>, dim3(grid), dim3(kEncThreads), lds, st, ix, ev, a);
+    else hipLaunchKernelGGL(EncodeRowsKernel<false>, dim3(grid), dim3(kEncThreads), lds, st, ix, ev, a);
     return hipGetLastError();
 }
 

@@ -107,9 +107,15 @@ __global__ __launch_bounds__(kM8EncThreads) void M8EncodeKernel(M8View v, const 
         for (uint32_t r = 0; r < v.nrun; ++r) {
             const M8Run& ru = v.runs[r];
             const float* x = res + ru.dim_off;
-            if (tid == 0) {
+            if (tid == 0) {   // run lengths are multiples of 64: eight reads ahead of their FMAs
                 double n2 = 0.0;
-                for (uint32_t j = 0; j < ru.len; ++j) n2 = fma(static_cast<double>(x[j]), static_cast<double>(x[j]), n2);
+                for (uint32_t j0 = 0; j0 < ru.len; j0 += 8) {
+                    double xv[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) xv[i] = static_cast<double>(x[j0 + i]);
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) n2 = fma(xv[i], xv[i], n2);
+                }
                 s_n2 = n2;
             }
             __syncthreads();
@@ -136,16 +142,27 @@ __global__ __launch_bounds__(kM8EncThreads) void M8EncodeKernel(M8View v, const 
                 M8Pack(d32, static_cast<uint32_t>(br), ru.rate, code + ru.code_off + b * ru.rate);
             }
             __syncthreads();
-            if (tid < 2) {   // the two fits, each one chain in coordinate order
-                const float* other = tid == 0 ? x : v.cent_rot + static_cast<uint64_t>(c) * P + ru.dim_off;
+            if (tid == 0) {   // the two fits, each one chain in coordinate order, interleaved on one lane
+                const float* cr = v.cent_rot + static_cast<uint64_t>(c) * P + ru.dim_off;
                 const uint32_t q = 1u << ru.rate;
-                double a = 0.0;
+                double a0 = 0.0, a1 = 0.0;
                 for (uint32_t b = 0; b < ru.nblk; ++b) {
                     const float* lev = ru.lev + brs[b] * q;
-                    for (int i = 0; i < 8; ++i)
-                        a = fma(static_cast<double>(other[b * 8 + i]), static_cast<double>(lev[dg[b * 8 + i]]), a);
+                    double lv[8], xv[8], cv[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        lv[i] = static_cast<double>(lev[dg[b * 8 + i]]);
+                        xv[i] = static_cast<double>(x[b * 8 + i]);
+                        cv[i] = static_cast<double>(cr[b * 8 + i]);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        a0 = fma(xv[i], lv[i], a0);
+                        a1 = fma(cv[i], lv[i], a1);
+                    }
                 }
-                s_fit[tid] = a;
+                s_fit[0] = a0;
+                s_fit[1] = a1;
             }
             __syncthreads();
             if (tid == 0) {
