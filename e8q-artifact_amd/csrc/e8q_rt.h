@@ -99,11 +99,9 @@ struct e8q_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // search timing
     static constexpr int kKernelEvents = 16;
     hipEvent_t kev[kKernelEvents] = {};   // per-launch timing of one kernel's launches in a search
-    DBuf ws;   // per-call workspace, grown on demand
-    DBuf pool; // candidate pool of the batched ME8Q form, sized per call after its count pass
-    DBuf recs; // records of the batched ME8Q count pass
-    DBuf tpool; // the batched ME8Q threshold pass's candidates and task counts
-    uint64_t rec_want = 0;   // records the last batched ME8Q call needed beyond `recs`
+    DBuf ws;   // per-call workspace, grown on demand (never by a call that fits it)
+    uint64_t reserved = 0;       // bytes of the largest e8q_ctx_reserve (0: none)
+    uint64_t m8_pool_per_q = 0;  // batched ME8Q pool entries per query a past call needed (unreserved contexts)
 
     // Replay of a search's device sequence as a HIP graph, per call shape.
     // A shape's first call runs eagerly and registers the shape; its second
@@ -152,6 +150,19 @@ struct e8q_ctx {
     std::vector<Tuned> tuned;
     uint32_t last_qtile = 0, last_btile = 0, last_grid = 0;
     uint64_t tunings = 0;
+
+    // Grids of the batched ME8Q form's persistent kernels (the MFMA
+    // contraction, the exact refine): fixed by the caller (nonzero), else
+    // chosen by timing candidates on the first batched search per (index
+    // shape, query bucket, k and nprobe buckets).
+    uint32_t m8_fix_mfma = 0, m8_fix_refine = 0;
+    struct M8Tuned {
+        uint64_t shape[6];
+        uint32_t mfma_grid, refine_grid;
+    };
+    std::vector<M8Tuned> m8_tuned;
+    uint32_t m8_last_mfma = 0, m8_last_refine = 0, m8_last_budget = 0, m8_last_set_a = 0;
+    uint64_t m8_tunings = 0;
 
     // Bump allocator over ws for one call.
     struct Carve {

@@ -231,6 +231,7 @@ e8q_status e8q_ctx_reserve(e8q_ctx* ctx, uint64_t bytes) {
         ctx->Bind();
         Check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         if (ctx->ws.bytes < bytes) ctx->ws.alloc(bytes);
+        ctx->reserved = std::max(ctx->reserved, bytes);
     });
 }
 

@@ -276,7 +276,12 @@ PYBIND11_MODULE(_e8qlib, m) {
         d["tunings"] = i.tunings;
         d["captures"] = i.captures;
         d["replays"] = i.replays;
+        d["me8q_mfma_grid"] = i.m8_mfma_grid;
+        d["me8q_refine_grid"] = i.m8_refine_grid;
+        d["me8q_tunings"] = i.m8_tunings;
         return d;
     });
+    m.def("set_me8q_grids", &e8qlib::SetMe8qGrids, py::arg("mfma") = 0, py::arg("refine") = 0,
+          "Fix the batched ME8Q form's MFMA-contraction and refine grids; zeros: autotuned.");
     RegisterServe(m);
 }

@@ -16,6 +16,7 @@
 #include <string>
 
 #include "e8q_gpu.h"
+#include "me8q_gpu.h"
 #include "e8q_host_math.h"
 
 namespace e8qlib {
@@ -76,7 +77,17 @@ ContextInfo GetContextInfo() {
     o.tunings = i.tunings;
     o.captures = i.captures;
     o.replays = i.replays;
+    e8q_m8_ctx_info m{};
+    Check(e8q_ctx_get_m8_info(DefaultContext(), &m));
+    o.m8_mfma_grid = m.mfma_grid;
+    o.m8_refine_grid = m.refine_grid;
+    o.m8_tunings = m.tunings;
     return o;
+}
+
+void SetMe8qGrids(uint32_t mfma_grid, uint32_t refine_grid) {
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    Check(e8q_ctx_set_m8_grids(DefaultContext(), mfma_grid, refine_grid));
 }
 
 E8qIndex::E8qIndex() = default;

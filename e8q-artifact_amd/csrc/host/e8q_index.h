@@ -129,7 +129,13 @@ struct ContextInfo {
     uint32_t qtile = 0, btile = 0, grid = 0;   // shape of the last batched search
     bool graphs = false;
     uint64_t tunings = 0, captures = 0, replays = 0;
+    // the batched ME8Q form: grids of its last search, autotuner runs
+    uint32_t m8_mfma_grid = 0, m8_refine_grid = 0;
+    uint64_t m8_tunings = 0;
 };
 ContextInfo GetContextInfo();
+// The batched ME8Q form's persistent grids (MFMA contraction, exact refine;
+// zero: the context's autotuner).
+void SetMe8qGrids(uint32_t mfma_grid, uint32_t refine_grid);
 
 }  // namespace e8qlib

@@ -121,6 +121,12 @@ static_assert(kM8BatchRuns == 4, "the per-pair run tables are float4");
 // end of the item with v_mfma_f32_32x32x2_f32 (two runs per instruction, lane
 // l supplying run 2j + l / 32 of pair l % 32 and of vector l % 32), whose
 // 32 x 32 result has the int8 MFMA's accumulator layout.
+// Whether the count pass's records all fit their reserve: the candidates are
+// then placed from the records, else written by a second contraction.
+__device__ __forceinline__ bool M8RecordsPlaced(const M8BatchArgs& a) {
+    return a.rec != nullptr && *G(a.rcount) <= a.rec_cap;
+}
+
 template <bool kChk, bool kWrite>
 __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     const M8View& v = a.v;
@@ -149,6 +155,10 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
     // The query operand, eight plane columns per stage, a ring of three
     // stages: [stage][column][half][pair][16 bytes] of the int8 LUT.
     __shared__ __align__(16) uint8_t s_a[3 * kM8StageBytes];
+    // The write pass runs only when the count pass's records did not fit
+    // their reserve (else they are placed instead); the choice is made here,
+    // on the device.
+    if (kWrite && M8RecordsPlaced(a)) return;
     const uint32_t tid = threadIdx.x, wave = Uniform(tid >> 6), lane = tid & 63;
     const uint32_t h = lane >> 5, n = lane & 31;
     if (tid < 16)
@@ -602,7 +612,9 @@ hipError_t LaunchM8MfmaScan(hipStream_t st, const M8BatchArgs& a, uint32_t grid,
 __global__ __launch_bounds__(256) void M8PlaceKernel(M8BatchArgs a, int set_b) {
     // Each 16-lane quarter of a wave takes its own unit, so a wave has four
     // units' load chains in flight; a quarter counts its unit's kept records
-    // (ballots), takes their slots with one atomic, then writes them.
+    // (ballots), takes their slots with one atomic, then writes them.  Runs
+    // only when the records fit their reserve (else the write pass places).
+    if (!M8RecordsPlaced(a)) return;
     const uint32_t lane = threadIdx.x & 63, g = lane >> 4, gl = lane & 15;
     const uint64_t nitems = min<uint64_t>(*G(a.counters), a.items_cap);
     const uint64_t units = nitems * kM8BatchQTile;
@@ -668,7 +680,7 @@ __global__ __launch_bounds__(256) void M8PlaceKernel(M8BatchArgs a, int set_b) {
 }
 
 hipError_t LaunchM8Place(hipStream_t st, const M8BatchArgs& a, uint32_t grid, bool set_b) {
-    if (a.rec == nullptr || a.itab == nullptr) return hipErrorInvalidValue;
+    if (a.rec == nullptr || a.itab == nullptr) return hipSuccess;   // no records: the write pass places
     hipLaunchKernelGGL(M8PlaceKernel, dim3(grid), dim3(256), 0, st, a, set_b ? 1 : 0);
     return hipGetLastError();
 }
@@ -802,8 +814,9 @@ hipError_t LaunchM8Split(hipStream_t st, const M8BatchArgs& a) {
 // a read) to the candidates whose lower bound at tau0 is below tau: the
 // bound only rises as the threshold falls, so no vector the bound keeps at
 // tau is dropped.
-__global__ __launch_bounds__(64) void M8CompactKernel(M8BatchArgs a, int placed) {
+__global__ __launch_bounds__(64) void M8CompactKernel(M8BatchArgs a) {
     const uint32_t lane = threadIdx.x;
+    const bool placed = M8RecordsPlaced(a);
     for (uint64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
         const float tau0 = a.tau[q], t1 = a.first_d[q * a.k + a.k - 1];
         const float tau = t1 < tau0 ? t1 : tau0;
@@ -831,10 +844,33 @@ __global__ __launch_bounds__(64) void M8CompactKernel(M8BatchArgs a, int placed)
     }
 }
 
-hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a, bool placed) {
+hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a) {
     if (a.nq == 0) return hipSuccess;
     const uint32_t grid = static_cast<uint32_t>(a.nq < 65536 ? a.nq : 65536);
-    hipLaunchKernelGGL(M8CompactKernel, dim3(grid), dim3(64), 0, st, a, placed ? 1 : 0);
+    hipLaunchKernelGGL(M8CompactKernel, dim3(grid), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+// The candidate pool holds pool_cap entries: a query whose slots (from the
+// count pass's offsets) pass its end keeps no candidates (ncand = |A| = 0,
+// so nothing is placed, written or refined for it) and goes on the overflow
+// list, whose queries the plain form searches at the end.  Offsets grow with
+// the query index, so the overflowed queries are the last ones.
+__global__ __launch_bounds__(256) void M8FitKernel(M8BatchArgs a, uint64_t pool_cap, uint32_t* __restrict__ ovf,
+                                                   uint32_t* __restrict__ novf) {
+    const uint64_t q = blockIdx.x * 256ull + threadIdx.x;
+    if (q >= a.nq) return;
+    if (a.coff[q] + a.ncand[q] > pool_cap) {
+        a.ncand[q] = 0;
+        a.na[q] = 0;
+        ovf[atomicAdd(novf, 1u)] = static_cast<uint32_t>(q);
+    }
+}
+
+hipError_t LaunchM8Fit(hipStream_t st, const M8BatchArgs& a, uint64_t pool_cap, uint32_t* ovf, uint32_t* novf) {
+    if (a.nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(M8FitKernel, dim3(static_cast<uint32_t>((a.nq + 255) / 256)), dim3(256), 0, st, a, pool_cap,
+                       ovf, novf);
     return hipGetLastError();
 }
 

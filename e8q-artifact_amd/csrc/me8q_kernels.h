@@ -61,6 +61,7 @@ struct M8ScanArgs {
     uint64_t* cover;
     const uint32_t* qsel;   // optional [nsel]: the queries to search (else 0..nq-1)
     uint64_t nsel;
+    const uint32_t* nsel_dev;   // optional: with qsel, the number of queries to search, on the device (<= nsel)
 };
 size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats);
 hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a);
@@ -167,7 +168,9 @@ hipError_t LaunchM8Split(hipStream_t st, const M8BatchArgs& a);
 // distance), coff_b = coff + na, and each query's B compacted in place to the
 // candidates whose lower bound is below tau (count in nfill); placed = 1: B
 // was placed from the records with that filter, nfill = fill_b.
-hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a, bool placed);
+hipError_t LaunchM8Compact(hipStream_t st, const M8BatchArgs& a);
+// Queries whose pool slots pass pool_cap: no candidates, listed in ovf (count novf).
+hipError_t LaunchM8Fit(hipStream_t st, const M8BatchArgs& a, uint64_t pool_cap, uint32_t* ovf, uint32_t* novf);
 size_t M8RefineLds(const M8View& v, uint32_t lev_floats);
 // first = true (indexes of two or more runs): each candidate of a.pool gets
 // the first run of its cascade; those still alive go to a.cont.  first =

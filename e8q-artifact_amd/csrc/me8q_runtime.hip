@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,7 @@ struct e8q_m8index {
     std::vector<M8Run> runs;
     std::vector<M8Seg> segs;
     std::vector<uint64_t> list_begin, batch_begin;
+    std::vector<uint64_t> top_lists;   // [nlist+1]: vectors in the c largest lists
     uint64_t nbatch = 0;
     // refine-row layout of the reference (for the diagnostics)
     std::vector<uint32_t> ref_run_off;
@@ -92,22 +94,36 @@ struct M8Bufs {
     float *fd, *tau;
     uint64_t *cover, *pairs, *poff;
     uint8_t* lut8;
+    uint64_t *pool, *cont, *tpool;
+    float* pdlo;
+    uint32_t *tsurv, *ttsurv, *ovf;
+    uint2* rec;
     BatchItem* items;
     unsigned long long *wstats, *bstats, *rcount;
     uint2* itab;
     BatchedParams* wparams;
 };
 
-constexpr uint64_t kM8PoolMax = uint64_t{1} << 31;   // candidate pool entries (more: the plain form)
-// Records of the count pass (8 B each): the first call of a context reserves
-// up to kM8RecFirst of them, a call that needed more raises the next call's
-// reserve to 1.25 x its count, up to kM8RecMax (2^31 = 16 GiB); a call whose
-// records do not fit places its candidates with the write pass.  The
-// environment variable E8Q_M8_RECORD_CAP lowers the limit (0: always the
-// write pass).
-constexpr uint64_t kM8RecFirst = uint64_t{1} << 28;
-constexpr uint64_t kM8RecMax = uint64_t{1} << 31;
+// Capacities of the batched form's candidate pool and count-pass records,
+// fixed before its first launch, so a search never waits on the host
+// between its kernels.  The pool holds
+//   min(2^31, the vectors the probed lists can hold, nq x per-query entries),
+// per-query entries = 65536 (E8Q_M8_POOL_PER_QUERY overrides it), raised
+// after a call that needed more (to 1.25 x its need) unless the context's
+// workspace was reserved (e8q_ctx_reserve): a reserved context caps the pool
+// to what fits the reservation and never grows.  The records hold as many as
+// the pool (E8Q_M8_RECORD_CAP lowers it; 0 = always the write pass).  Queries
+// whose candidates pass the pool's end are searched by the plain form.
+constexpr uint64_t kM8PoolMax = uint64_t{1} << 31;
+constexpr uint64_t kM8PoolPerQuery = 65536;
 constexpr uint32_t kM8DefaultBudget = 16;
+constexpr uint32_t kM8MaxBudget = 1024;    // threshold-pass batches per query at most
+constexpr uint64_t kM8TuneMinQueries = 64;   // smaller calls run the default grids untuned
+
+uint64_t EnvU64(const char* name, uint64_t dflt) {
+    const char* e = std::getenv(name);
+    return e ? std::strtoull(e, nullptr, 10) : dflt;
+}
 
 // Whether the batched form runs for this index and k.
 bool M8BatchedFits(const e8q_m8index* x, uint64_t k) {
@@ -132,8 +148,35 @@ uint64_t M8ItemsCap(const e8q_m8index* x, uint64_t nq, uint64_t pairs) {
     return std::min((nq + qt - 1) / qt * sum_nbt, (pairs + qt - 1) / qt * max_nbt + sum_nbt);
 }
 
+struct M8Caps {
+    uint64_t pool = 0;     // candidate pool entries
+    uint64_t rec = 0;      // count-pass records
+    uint64_t tasks = 0;    // refine tasks of one launch at most
+    uint64_t tpool = 0;    // threshold-pass entries (budget x 32 per query)
+    uint64_t ttasks = 0;   // its tasks
+};
+
+// Pool entries a call of nq queries probing up to maxp lists asks for (the
+// default, or what the context learned), before any reservation cap.
+uint64_t M8PoolWant(const e8q_m8index* x, uint64_t nq, uint32_t maxp, uint64_t per_q_learned) {
+    const uint64_t per_q = std::min(kM8PoolMax, std::max(EnvU64("E8Q_M8_POOL_PER_QUERY", kM8PoolPerQuery),
+                                                         per_q_learned));
+    const uint64_t reach = nq * x->top_lists[std::min<uint64_t>(maxp, x->view.nlist)];
+    return std::min({kM8PoolMax, reach, nq * per_q});
+}
+
+M8Caps M8CapsFor(uint64_t nq, uint64_t pool, uint32_t budget) {
+    M8Caps c;
+    c.pool = pool;
+    c.rec = std::min(pool, EnvU64("E8Q_M8_RECORD_CAP", pool));
+    c.tasks = pool / kM8RefineChunk + nq + 1;
+    c.tpool = nq * budget * 32ull;
+    c.ttasks = nq * ((budget * 32ull + kM8RefineChunk - 1) / kM8RefineChunk) + 1;
+    return c;
+}
+
 void M8CarveBatched(e8q_ctx::Carve& cv, M8Bufs& b, const e8q_m8index* x, uint64_t nq, uint64_t k, uint64_t pairs,
-                    bool chk) {
+                    bool chk, const M8Caps& caps) {
     const M8View& v = x->view;
     cv.add(&b.fid, nq * k);
     cv.add(&b.lut8, nq * v.plane_cols * 32);
@@ -164,16 +207,15 @@ void M8CarveBatched(e8q_ctx::Carve& cv, M8Bufs& b, const e8q_m8index* x, uint64_
     cv.add(&b.rcount, 1);
     cv.add(&b.itab, std::max<uint64_t>(M8ItemsCap(x, nq, pairs), 1) * kM8BatchQTile);
     cv.add(&b.wparams, 1);
+    cv.add(&b.ovf, nq);
+    cv.add(&b.pool, std::max<uint64_t>(caps.pool, 1));
+    cv.add(&b.cont, std::max<uint64_t>(caps.pool, 1));
+    cv.add(&b.pdlo, std::max<uint64_t>(caps.pool, 1));
+    cv.add(&b.tsurv, 4 * caps.tasks);
+    if (caps.rec > 0) cv.add(&b.rec, caps.rec);
+    cv.add(&b.tpool, std::max<uint64_t>(caps.tpool, 1));
+    cv.add(&b.ttsurv, caps.ttasks);
     if (chk) cv.add(&b.chk, nq);
-}
-
-// Records the count pass of this call may write.
-uint64_t M8RecordCap(e8q_ctx* ctx, uint64_t items_cap) {
-    uint64_t lim = kM8RecMax;
-    if (const char* e = std::getenv("E8Q_M8_RECORD_CAP")) lim = std::min<uint64_t>(lim, std::strtoull(e, nullptr, 10));
-    const uint64_t bound = items_cap * kM8BatchQTile * kM8BatchBTile * 32;   // every pair of every item
-    const uint64_t want = std::max(kM8RecFirst, ctx->rec_want);
-    return std::min({lim, bound, want});
 }
 
 void M8Carve(e8q_ctx::Carve& cv, M8Bufs& b, const M8View& v, uint64_t nq, uint64_t k, uint64_t maxp, bool q_host,
@@ -196,14 +238,40 @@ void M8Carve(e8q_ctx::Carve& cv, M8Bufs& b, const M8View& v, uint64_t nq, uint64
     cv.add(&b.qst, nq * 8);
 }
 
+// Graph-replay key words of an M8View.
+static_assert(sizeof(M8View) % 8 == 0, "M8View is compared as 64-bit words");
+
+uint64_t Bucket(uint64_t n) {
+    uint64_t b = 0;
+    while ((uint64_t{1} << b) < n) ++b;
+    return b;
+}
+
+uint64_t F32Bits(float f) {
+    uint32_t u = 0;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
 // One search: plain (the reference's per-query schedule) or batched.  chk
 // (host [nq], batched only): the coarse-sum checksum of the diagnostic.
+//
+// The batched form is one device sequence with no host decision inside:
+// its buffers are carved before the first launch, a query whose candidates
+// would pass the pool's end is listed on the device and searched by the
+// plain form at the end of the same sequence, and whether the count pass's
+// records fit (placement from them, else a second contraction) is read by
+// the kernels themselves.  So the sequence can be captured as a HIP graph,
+// and its persistent grids timed and chosen (results do not depend on them:
+// every per-query threshold is fixed before the kernel that applies it, and
+// each top-k is sorted by (distance, id)).
 void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e8q_m8_stats* stats,
               uint64_t* per_query_stats, uint32_t* chk) {
     if (!ctx || !x || !a) Throw(E8Q_EINVAL, "null argument");
     if (a->k == 0 || a->k > kM8MaxK) Throw(E8Q_EINVAL, "k must be in [1, 1024]");
     if (!a->queries || !a->out_ids || !a->out_dists) Throw(E8Q_EINVAL, "null buffer");
     if (a->mode != E8Q_MODE_PLAIN && a->mode != E8Q_MODE_BATCHED) Throw(E8Q_EINVAL, "mode must be plain or batched");
+    if (a->batched_budget > kM8MaxBudget) Throw(E8Q_EINVAL, "batched_budget must be <= 1024");
     const M8View& v = x->view;
     const uint64_t nq = a->nq;
     if (stats) {
@@ -229,25 +297,45 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     ctx->Bind();
     hipStream_t st = ctx->stream;
     const uint64_t P = v.padded;
+    const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
+    const uint32_t budget = a->batched_budget ? a->batched_budget : kM8DefaultBudget;
+    const uint32_t set_a = a->batched_set_a ? a->batched_set_a : kM8SetA;
     M8Bufs b{};
-    e8q_ctx::Carve cv;
-    M8Carve(cv, b, v, nq, a->k, maxp, !a->queries_on_device, !a->outputs_on_device, a->nprobe_per_query != nullptr);
-    if (batched) M8CarveBatched(cv, b, x, nq, a->k, pairs, chk != nullptr);
-    ctx->Take(cv);
+    M8Caps caps;
+    {
+        e8q_ctx::Carve cv;
+        M8Carve(cv, b, v, nq, a->k, maxp, !a->queries_on_device, !a->outputs_on_device, a->nprobe_per_query != nullptr);
+        if (batched) {
+            uint64_t pool = M8PoolWant(x, nq, maxp, ctx->reserved ? 0 : ctx->m8_pool_per_q);
+            auto bytes_at = [&](uint64_t pl) {
+                e8q_ctx::Carve c2 = cv;
+                M8Bufs b2{};
+                M8CarveBatched(c2, b2, x, nq, a->k, pairs, chk != nullptr, M8CapsFor(nq, pl, budget));
+                return static_cast<uint64_t>(c2.bytes());
+            };
+            // a reserved workspace is never grown for the pool: the largest
+            // pool that fits it (down to none: then the plain form answers)
+            if (ctx->reserved && bytes_at(pool) > ctx->ws.bytes && bytes_at(0) <= ctx->ws.bytes) {
+                uint64_t lo = 0, hi = pool;
+                while (lo < hi) {
+                    const uint64_t mid = lo + (hi - lo + 1) / 2;
+                    if (bytes_at(mid) <= ctx->ws.bytes) lo = mid;
+                    else hi = mid - 1;
+                }
+                pool = lo;
+            }
+            caps = M8CapsFor(nq, pool, budget);
+            M8CarveBatched(cv, b, x, nq, a->k, pairs, chk != nullptr, caps);
+        }
+        ctx->Take(cv);
+    }
     float *dq = b.dq, *rotq = b.rotq, *qc = b.qc, *dist = b.dist, *pd = b.pd, *od = b.od;
     double *rc = b.rc, *sqd2 = b.sqd2;
     uint8_t* lut = b.lut;
     uint32_t *probe = b.probe, *npq = b.npq, *oid = b.oid;
     uint64_t* qst = b.qst;
-    const float* q = a->queries;
-    if (!a->queries_on_device) {
-        Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
-        q = dq;
-    }
-    if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
+    const float* q = a->queries_on_device ? a->queries : dq;
     const uint32_t nprobe_all = static_cast<uint32_t>(std::min<uint64_t>(a->nprobe, v.nlist));
-    Check(hipEventRecord(ctx->ev[0], st), "event");
-    Check(LaunchM8QueryPrep(st, v, q, nq, rotq, lut, qc, rc), "query prep");
     RouteArgs ra{};
     ra.rotq = rotq;
     ra.qstride = static_cast<uint32_t>(P);
@@ -260,8 +348,6 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     ra.cstride = static_cast<uint32_t>(P);
     ra.nq = nq;
     ra.dist = dist;
-    Check(LaunchRouteSelect(st, ra, npq, nprobe_all, maxp, probe, pd), "route");
-    Check(LaunchM8RunDist(st, v, rotq, rc, probe, npq, nprobe_all, maxp, nq, sqd2), "run distances");
     uint32_t* out_ids = a->outputs_on_device ? a->out_ids : oid;
     float* out_d = a->outputs_on_device ? a->out_dists : od;
     M8ScanArgs sa{};
@@ -285,37 +371,14 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     sa.lev_floats = 0;
     for (const auto& r : x->runs)
         if (r.rate <= kM8LdsLevRate) sa.lev_floats += 2u << r.rate;
-    std::vector<unsigned long long> bst(8, 0), wst(4, 0);
-    uint64_t novf = 0;
-    // The refine's launches, each between two of the context's kernel events:
-    // kind 0 = first-run phase, 1 = cascades.
-    int nkl = 0, kind[e8q_ctx::kKernelEvents / 2] = {};
-    auto refine_launch = [&](const M8BatchArgs& r, bool first, const char* what) {
-        const bool timed = stats && nkl < e8q_ctx::kKernelEvents / 2;
-        if (timed) Check(hipEventRecord(ctx->kev[2 * nkl], st), "event");
-        Check(LaunchM8Refine(st, r, 8 * static_cast<uint32_t>(std::max(ctx->cus, 1)), first), what);
-        if (timed) {
-            Check(hipEventRecord(ctx->kev[2 * nkl + 1], st), "event");
-            kind[nkl++] = first ? 0 : 1;
-        }
-    };
-    if (!batched) {
-        Check(hipEventRecord(ctx->ev[1], st), "event");
-        Check(LaunchM8Scan(st, sa), "scan");
-        Check(hipEventRecord(ctx->ev[2], st), "event");
-        Check(hipEventRecord(ctx->ev[3], st), "event");
-    } else {
-        const uint32_t budget = a->batched_budget ? a->batched_budget : kM8DefaultBudget;
-        // the (query, list) work list: every probe, grouped by list
-        Check(LaunchFill32(st, b.ccount, v.nlist, 0), "clear");
-        Check(LaunchFill32(st, b.ncand, nq, 0), "clear");
-        Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.wstats), 8, 0), "clear");
-        Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.bstats), 16, 0), "clear");
-        if (chk) Check(LaunchFill32(st, b.chk, nq, 0), "clear");
-        IndexView lv{};   // the work-list kernels read the list count and offsets only
+
+    // Arguments of the batched kernels (pointers into the carve).
+    IndexView lv{};   // the work-list kernels read the list count and offsets only
+    BatchedArgs w{};
+    M8BatchArgs ba{};
+    if (batched) {
         lv.nlist = v.nlist;
         lv.cluster_begin = v.list_begin;
-        BatchedArgs w{};
         w.nq = nq;
         w.maxp = maxp;
         w.nprobe_q = npq;
@@ -334,14 +397,6 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         w.btile = kM8BatchBTile;
         w.stats = b.wstats;
         w.p_lo = 0;
-        Check(LaunchBatchedWorkList(st, lv, w, b.wparams, true), "work list");
-        // 3. the MFMA contraction and the coarse bound: a count pass, the pool
-        // offsets, a write pass
-        Check(LaunchFill32(st, b.nfill, nq, 0), "clear");
-        Check(LaunchFill32(st, b.hist, nq * kM8TauBins, 0), "clear");
-        Check(LaunchFill32(st, b.fill_a, nq, 0), "clear");
-        Check(LaunchFill32(st, b.fill_b, nq, 0), "clear");
-        M8BatchArgs ba{};
         ba.v = v;
         ba.lut = lut;
         ba.lut8 = b.lut8;
@@ -391,17 +446,79 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             }
             ba.ref_lev_floats = o;
         }
-        ba.set_a = a->batched_set_a ? a->batched_set_a : kM8SetA;
-        const uint32_t cus = static_cast<uint32_t>(std::max(ctx->cus, 1));
+        ba.set_a = set_a;
+        ba.rec = caps.rec > 0 ? b.rec : nullptr;
+        ba.rec_cap = caps.rec;
+        ba.rcount = b.rcount;
+        ba.itab = b.itab;
+        ba.pool = b.pool;
+        ba.cont = b.cont;
+        ba.pdlo = b.pdlo;
+        ba.ncont = b.ncont;
+    }
+
+    // Grids of the persistent batched kernels: fixed, tuned before, or tuned
+    // in this call (cands non-empty).
+    struct Grids {
+        uint32_t mfma, refine;
+    };
+    Grids grids{ctx->m8_fix_mfma ? ctx->m8_fix_mfma : 2 * cus, ctx->m8_fix_refine ? ctx->m8_fix_refine : 8 * cus};
+    const uint64_t tkey[6] = {v.n, v.plane_cols, v.nlist, Bucket(nq), Bucket(maxp), Bucket(a->k)};
+    bool tune = false;
+    if (batched && (ctx->m8_fix_mfma == 0 || ctx->m8_fix_refine == 0)) {
+        bool found = false;
+        for (const auto& t : ctx->m8_tuned)
+            if (std::equal(t.shape, t.shape + 6, tkey)) {
+                if (!ctx->m8_fix_mfma) grids.mfma = t.mfma_grid;
+                if (!ctx->m8_fix_refine) grids.refine = t.refine_grid;
+                found = true;
+            }
+        tune = !found && nq >= kM8TuneMinQueries && chk == nullptr;
+    }
+
+    // The refine's launches, each between two of the context's kernel events
+    // (eager, timed runs only): kind 0 = first-run phase, 1 = cascades.
+    int nkl = 0, kind[e8q_ctx::kKernelEvents / 2] = {};
+    // The device sequence from the query rotation to the results.  eager:
+    // events around the scan and the refine launches, and the work list's
+    // parameter block uploaded on the stream (a capture records kernels
+    // only: the block is uploaded before the graph launch).
+    auto device_seq = [&](const Grids& g, bool eager, bool timed) {
+        if (eager) Check(hipEventRecord(ctx->ev[0], st), "event");
+        Check(LaunchM8QueryPrep(st, v, q, nq, rotq, lut, qc, rc), "query prep");
+        Check(LaunchRouteSelect(st, ra, npq, nprobe_all, maxp, probe, pd), "route");
+        Check(LaunchM8RunDist(st, v, rotq, rc, probe, npq, nprobe_all, maxp, nq, sqd2), "run distances");
+        if (!batched) {
+            if (eager) Check(hipEventRecord(ctx->ev[1], st), "event");
+            Check(LaunchM8Scan(st, sa), "scan");
+            if (eager) Check(hipEventRecord(ctx->ev[2], st), "event");
+            return;
+        }
+        auto refine_launch = [&](const M8BatchArgs& r, bool first, const char* what) {
+            const bool t = timed && nkl < e8q_ctx::kKernelEvents / 2;
+            if (t) Check(hipEventRecord(ctx->kev[2 * nkl], st), "event");
+            Check(LaunchM8Refine(st, r, g.refine, first), what);
+            if (t) {
+                Check(hipEventRecord(ctx->kev[2 * nkl + 1], st), "event");
+                kind[nkl++] = first ? 0 : 1;
+            }
+        };
+        // the (query, list) work list: every probe, grouped by list
+        Check(LaunchFill32(st, b.ccount, v.nlist, 0), "clear");
+        Check(LaunchFill32(st, b.ncand, nq, 0), "clear");
+        Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.wstats), 8, 0), "clear");
+        Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.bstats), 16, 0), "clear");
+        if (chk) Check(LaunchFill32(st, b.chk, nq, 0), "clear");
+        Check(LaunchBatchedWorkList(st, lv, w, b.wparams, eager), "work list");
+        Check(LaunchFill32(st, b.nfill, nq, 0), "clear");
+        Check(LaunchFill32(st, b.hist, nq * kM8TauBins, 0), "clear");
+        Check(LaunchFill32(st, b.fill_a, nq, 0), "clear");
+        Check(LaunchFill32(st, b.fill_b, nq, 0), "clear");
         {   // the threshold pass: exact distances of each query's first `budget`
             // batches (whole cascade, no death test, tau = +inf) and their top-k
             M8BatchArgs ta = ba;
-            const uint64_t cap = static_cast<uint64_t>(budget) * 32;
-            const uint64_t pool_b = (nq * cap * 8 + 255) & ~uint64_t{255};
-            const uint64_t need = pool_b + (nq * ((cap + kM8RefineChunk - 1) / kM8RefineChunk) + 1) * 4 + 256;
-            if (ctx->tpool.bytes < need) ctx->tpool.alloc(need);
-            ta.pool = ctx->tpool.as<uint64_t>();
-            ta.tsurv = reinterpret_cast<uint32_t*>(ctx->tpool.as<char>() + pool_b);
+            ta.pool = b.tpool;
+            ta.tsurv = b.ttsurv;
             ta.coff = b.coff_b;
             ta.nfill = b.nfill;
             ta.toff = b.toff;
@@ -418,93 +535,183 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             refine_launch(ta, false, "threshold refine");
             Check(LaunchM8Select(st, ta), "threshold select");
         }
-        // records of the count pass, when the reserve holds any
-        const uint64_t rcap = M8RecordCap(ctx, w.items_cap);
-        if (rcap > 0 && ctx->recs.bytes < rcap * sizeof(uint2)) ctx->recs.alloc(rcap * sizeof(uint2));
-        ba.rec = rcap > 0 ? ctx->recs.as<uint2>() : nullptr;
-        ba.rec_cap = rcap;
-        ba.rcount = b.rcount;
-        ba.itab = b.itab;
+        // 3. the MFMA contraction and the coarse bound: a count pass (with
+        // its records when they fit), the pool offsets, the pool's fit
         Check(LaunchFill32(st, reinterpret_cast<uint32_t*>(b.rcount), 2, 0), "clear");
         Check(LaunchM8LutInt8(st, lut, b.lut8, nq * v.plane_cols * 32), "int8 LUT");
-        Check(hipEventRecord(ctx->ev[1], st), "event");
-        Check(LaunchM8MfmaScan(st, ba, 2 * cus, false), "mfma count pass");
-        Check(hipEventRecord(ctx->ev[2], st), "event");
+        if (eager) Check(hipEventRecord(ctx->ev[1], st), "event");
+        Check(LaunchM8MfmaScan(st, ba, g.mfma, false), "mfma count pass");
+        if (eager) Check(hipEventRecord(ctx->ev[2], st), "event");
         Check(LaunchM8Split(st, ba), "candidate split");
         Check(LaunchM8RefineTasks(st, ba, b.ncand, true), "pool offsets");
-        uint64_t total = 0, nrec = 0;
-        uint32_t ntask = 0, cnt[4] = {0, 0, 0, 0};
-        Check(hipMemcpyAsync(&total, b.poff + nq, 8, hipMemcpyDeviceToHost, st), "download pool size");
-        Check(hipMemcpyAsync(&ntask, b.toff + nq, 4, hipMemcpyDeviceToHost, st), "download task count");
-        Check(hipMemcpyAsync(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost, st), "download counters");
-        Check(hipMemcpyAsync(&nrec, b.rcount, 8, hipMemcpyDeviceToHost, st), "download record count");
-        Check(hipStreamSynchronize(st), "batched count pass");
-        if (cnt[0] > w.items_cap) Throw(E8Q_ERUNTIME, "batched search: work items exceed their bound");
-        const bool placed = ba.rec != nullptr && nrec <= rcap;
-        if (ba.rec != nullptr && nrec != total && nrec <= rcap)
-            Throw(E8Q_ERUNTIME, "batched search: records and candidate counts differ");
-        if (!placed && nrec > rcap && nrec <= kM8RecMax) ctx->rec_want = nrec + nrec / 4;
-        if (total > kM8PoolMax) {   // too many candidates for one pool: the plain form
-            novf = nq;
-            Check(hipEventRecord(ctx->ev[1], st), "event");
-            Check(LaunchM8Scan(st, sa), "plain search");
-            Check(hipEventRecord(ctx->ev[2], st), "event");
-        } else {
-            // pool: the candidates (A then B per query), their lower bounds,
-            // the candidates alive after their first run, then the per-task
-            // survivor counts of the four refine launches
-            const uint64_t pool_b = (total * 8 + 255) & ~uint64_t{255};
-            const uint64_t dlo_b = (total * 4 + 255) & ~uint64_t{255};
-            const uint64_t task_b = (static_cast<uint64_t>(ntask) * 4 + 255) & ~uint64_t{255};
-            const uint64_t need = 2 * pool_b + dlo_b + 4 * task_b + 256;
-            if (ctx->pool.bytes < need) ctx->pool.alloc(need + need / 4);
-            char* pb = ctx->pool.as<char>();
-            ba.pool = reinterpret_cast<uint64_t*>(pb);
-            ba.cont = reinterpret_cast<uint64_t*>(pb + pool_b);
-            ba.pdlo = reinterpret_cast<float*>(pb + 2 * pool_b);
-            ba.ncont = b.ncont;
-            uint32_t* ts = reinterpret_cast<uint32_t*>(pb + 2 * pool_b + dlo_b);
-            const uint64_t tsn = task_b / 4;
-            if (placed) {
-                Check(LaunchM8Place(st, ba, 8 * cus, false), "placement of set A");
-            } else {
-                Check(LaunchFill32(st, b.counters + 1, 1, 0), "clear");
-                Check(LaunchM8MfmaScan(st, ba, 2 * cus, true), "mfma write pass");
+        Check(LaunchM8Fit(st, ba, caps.pool, b.ovf, b.counters + 2), "pool fit");
+        // set A placed from the records, or (when they did not fit) every
+        // candidate written by a second contraction; each kernel reads which
+        Check(LaunchM8Place(st, ba, 8 * cus, false), "placement of set A");
+        Check(LaunchFill32(st, b.counters + 1, 1, 0), "clear");
+        Check(LaunchM8MfmaScan(st, ba, g.mfma, true), "mfma write pass");
+        // 4. exact cascades (every candidate's first run, then the whole
+        // cascade of those still alive) and a top-k merge, first of each
+        // query's set A (its exact top-k gives tau1), then of the rest whose
+        // lower bound is below tau1
+        const uint64_t tsn = caps.tasks;
+        auto refine = [&](M8BatchArgs r, uint32_t* tsurv1, uint32_t* tsurv2, uint32_t* sel_ids, float* sel_d) {
+            r.tsurv = tsurv1;
+            r.toff = b.toff;
+            Check(LaunchFill32(st, b.ncont, nq, 0), "clear");
+            Check(LaunchM8RefineTasks(st, r, r.nfill, false), "refine tasks");
+            refine_launch(r, true, "refine, first runs");
+            if (v.nrun > 1) {
+                r.pool = r.cont;
+                r.nfill = b.ncont;
+                r.toff = b.toff2;
+                r.tsurv = tsurv2;
+                Check(LaunchM8RefineTasks(st, r, b.ncont, false), "refine tasks");
+                refine_launch(r, false, "refine, cascades");
             }
-            // 4. exact cascades (every candidate's first run, then the whole
-            // cascade of those still alive) and a top-k merge, first of each
-            // query's set A (its exact top-k gives tau1), then of the rest
-            // whose lower bound is below tau1
-            auto refine = [&](M8BatchArgs r, uint32_t* tsurv1, uint32_t* tsurv2, uint32_t* sel_ids, float* sel_d) {
-                r.tsurv = tsurv1;
-                r.toff = b.toff;
-                Check(LaunchFill32(st, b.ncont, nq, 0), "clear");
-                Check(LaunchM8RefineTasks(st, r, r.nfill, false), "refine tasks");
-                refine_launch(r, true, "refine, first runs");
-                if (v.nrun > 1) {
-                    r.pool = r.cont;
-                    r.nfill = b.ncont;
-                    r.toff = b.toff2;
-                    r.tsurv = tsurv2;
-                    Check(LaunchM8RefineTasks(st, r, b.ncont, false), "refine tasks");
-                    refine_launch(r, false, "refine, cascades");
-                }
-                r.out_ids = sel_ids;
-                r.out_d = sel_d;
-                Check(LaunchM8Select(st, r), "select");
-            };
-            M8BatchArgs ra = ba;   // set A
-            ra.nfill = b.na;
-            refine(ra, ts, ts + tsn, b.fid, b.fd);
-            if (placed) Check(LaunchM8Place(st, ba, 8 * cus, true), "placement of the rest");
-            Check(LaunchM8Compact(st, ba, placed), "compaction");
-            M8BatchArgs rb = ba;   // the rest
-            rb.coff = b.coff_b;
-            refine(rb, ts + 2 * tsn, ts + 3 * tsn, out_ids, out_d);
+            r.out_ids = sel_ids;
+            r.out_d = sel_d;
+            Check(LaunchM8Select(st, r), "select");
+        };
+        M8BatchArgs rA = ba;   // set A
+        rA.nfill = b.na;
+        refine(rA, b.tsurv, b.tsurv + tsn, b.fid, b.fd);
+        Check(LaunchM8Place(st, ba, 8 * cus, true), "placement of the rest");
+        Check(LaunchM8Compact(st, ba), "compaction");
+        M8BatchArgs rB = ba;   // the rest
+        rB.coff = b.coff_b;
+        refine(rB, b.tsurv + 2 * tsn, b.tsurv + 3 * tsn, out_ids, out_d);
+        // queries whose candidates passed the pool's end: the plain form
+        // (their candidates and batches are counted already)
+        M8ScanArgs oa = sa;
+        oa.qsel = b.ovf;
+        oa.nsel = nq;
+        oa.nsel_dev = b.counters + 2;
+        oa.qstats = nullptr;
+        Check(LaunchM8Scan(st, oa), "plain search of overflowed queries");
+    };
+
+    // Graph replay: the key is everything the captured launches bake in.
+    e8q_ctx::GraphEntry* ge = nullptr;
+    if (ctx->graphs && !tune && chk == nullptr) {
+        std::vector<uint64_t> key{0x4D38u,   // the ME8Q search
+                                  reinterpret_cast<uint64_t>(x),
+                                  static_cast<uint64_t>(batched ? 1 : 0),
+                                  nq,
+                                  a->k,
+                                  maxp,
+                                  nprobe_all,
+                                  npq != nullptr ? 1u : 0u,
+                                  F32Bits(a->z),
+                                  budget,
+                                  set_a,
+                                  grids.mfma,
+                                  grids.refine,
+                                  reinterpret_cast<uint64_t>(q),
+                                  reinterpret_cast<uint64_t>(out_ids),
+                                  reinterpret_cast<uint64_t>(out_d),
+                                  reinterpret_cast<uint64_t>(ctx->ws.p),
+                                  ctx->ws.bytes,
+                                  caps.pool,
+                                  caps.rec,
+                                  pairs};
+        const size_t w0 = key.size();
+        key.resize(w0 + sizeof(M8View) / 8);
+        std::memcpy(key.data() + w0, &v, sizeof(M8View));
+        ge = ctx->FindGraph(key);
+        if (!ge) ctx->AddGraph(key);   // first call of this shape runs eagerly
+    }
+    if (!a->queries_on_device)
+        Check(hipMemcpyAsync(dq, a->queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+    if (npq) Check(hipMemcpyAsync(npq, a->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st), "upload nprobe");
+    bool replayed = false;
+    if (ge) {
+        if (batched) {
+            const BatchedParams hw{lv, w};
+            Check(hipMemcpyAsync(b.wparams, &hw, sizeof(hw), hipMemcpyHostToDevice, st), "upload parameters");
         }
-        Check(hipEventRecord(ctx->ev[3], st), "event");
+        if (!ge->exec) {
+            Check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+            try {
+                device_seq(grids, false, false);
+            } catch (...) {
+                hipGraph_t gr = nullptr;
+                (void)hipStreamEndCapture(st, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            hipGraph_t gr = nullptr;
+            Check(hipStreamEndCapture(st, &gr), "end capture");
+            hipGraphExec_t ex = nullptr;
+            const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            Check(e, "graph instantiate");
+            ge->exec = ex;
+            ++ctx->captures;
+        }
+        Check(hipEventRecord(ctx->ev[0], st), "event");
+        Check(hipEventRecord(ctx->ev[1], st), "event");
+        Check(hipGraphLaunch(ge->exec, st), "graph launch");
+        Check(hipEventRecord(ctx->ev[2], st), "event");
+        ++ctx->replays;
+        replayed = true;
+    } else {
+        if (tune) {
+            // Times the whole sequence at each candidate grid (the MFMA
+            // contraction's at 1, 2 and 4 per CU, then the refine's at 4, 8
+            // and 16 per CU at the best of those); each run writes the same
+            // results.
+            auto time_grids = [&](const Grids& g) {
+                Check(hipEventRecord(ctx->kev[0], st), "event");
+                device_seq(g, true, false);
+                Check(hipEventRecord(ctx->kev[1], st), "event");
+                Check(hipEventSynchronize(ctx->kev[1]), "tuning");
+                float ms = 0.0F;
+                Check(hipEventElapsedTime(&ms, ctx->kev[0], ctx->kev[1]), "event time");
+                return ms;
+            };
+            Grids best = grids;
+            float best_ms = time_grids(best);
+            if (!ctx->m8_fix_mfma)
+                for (uint32_t m : {cus, 4 * cus}) {
+                    const Grids c{m, best.refine};
+                    const float ms = time_grids(c);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
+                    }
+                }
+            if (!ctx->m8_fix_refine)
+                for (uint32_t r : {4 * cus, 16 * cus}) {
+                    const Grids c{best.mfma, r};
+                    const float ms = time_grids(c);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
+                    }
+                }
+            ctx->m8_tuned.push_back({{tkey[0], tkey[1], tkey[2], tkey[3], tkey[4], tkey[5]}, best.mfma, best.refine});
+            ++ctx->m8_tunings;
+            grids = best;
+        }
+        device_seq(grids, true, stats != nullptr);
+    }
+    Check(hipEventRecord(ctx->ev[3], st), "event");
+    if (batched) {
+        ctx->m8_last_mfma = grids.mfma;
+        ctx->m8_last_refine = grids.refine;
+        ctx->m8_last_budget = budget;
+        ctx->m8_last_set_a = set_a;
+    }
+    std::vector<unsigned long long> bst(8, 0), wst(4, 0);
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    unsigned long long nrec = 0, total = 0;
+    if (batched) {
         Check(hipMemcpyAsync(bst.data(), b.bstats, 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
         Check(hipMemcpyAsync(wst.data(), b.wstats, 4 * 8, hipMemcpyDeviceToHost, st), "download stats");
+        Check(hipMemcpyAsync(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost, st), "download counters");
+        Check(hipMemcpyAsync(&nrec, b.rcount, 8, hipMemcpyDeviceToHost, st), "download record count");
+        Check(hipMemcpyAsync(&total, b.poff + nq, 8, hipMemcpyDeviceToHost, st), "download pool size");
         if (chk) Check(hipMemcpyAsync(chk, b.chk, nq * 4, hipMemcpyDeviceToHost, st), "download checksums");
     }
     if (!a->outputs_on_device) {
@@ -514,9 +721,20 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     std::vector<uint64_t> hs(nq * 8);
     Check(hipMemcpyAsync(hs.data(), qst, nq * 8 * 8, hipMemcpyDeviceToHost, st), "download stats");
     Check(hipStreamSynchronize(st), "search");
-    const bool ran_batched = batched && novf == 0;   // else the plain form answered every query
+    uint64_t novf = 0;
+    if (batched) {
+        if (cnt[0] > w.items_cap) Throw(E8Q_ERUNTIME, "batched search: work items exceed their bound");
+        if (cnt[2] > nq) Throw(E8Q_ERUNTIME, "batched search: overflow count exceeds the query count");
+        if (ba.rec != nullptr && nrec <= caps.rec && nrec != total)
+            Throw(E8Q_ERUNTIME, "batched search: records and candidate counts differ");
+        novf = cnt[2];
+        // a call whose candidates passed the pool's end raises the next
+        // call's pool (a reserved context keeps its size)
+        if (total > caps.pool && !ctx->reserved)
+            ctx->m8_pool_per_q = std::max<uint64_t>(ctx->m8_pool_per_q, (total + total / 4) / nq + 1);
+    }
     if (per_query_stats) {
-        if (ran_batched) std::fill(per_query_stats, per_query_stats + nq * 5, 0ull);
+        if (batched) std::fill(per_query_stats, per_query_stats + nq * 5, 0ull);
         else
             for (uint64_t i = 0; i < nq; ++i) std::memcpy(per_query_stats + i * 5, hs.data() + i * 8, 5 * 8);
     }
@@ -524,8 +742,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         uint64_t s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // threshold pass (batched) or the whole search (plain)
         for (uint64_t i = 0; i < nq; ++i)
             for (int f = 0; f < 8; ++f) s8[f] += hs[i * 8 + f];
-        if (!ran_batched) {
-            stats->overflowed = novf;
+        if (!batched) {
             stats->candidates = s8[0];
             stats->survivors = s8[1];
             stats->bits_scored = s8[2];
@@ -536,7 +753,8 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             stats->groups = s8[7];
         } else {
             // candidates: the vectors of every probed list; the threshold
-            // pass's refined / survivors / bits plus the batched ones
+            // pass's refined / survivors / bits plus the batched ones (the
+            // plain search of overflowed queries is not counted again)
             uint64_t bits_row = 0;
             for (const auto& r : x->runs) bits_row += static_cast<uint64_t>(r.rate) * r.len;
             stats->form = E8Q_MODE_BATCHED;
@@ -565,6 +783,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         stats->scan_ms = ms;
         Check(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]), "event time");
         stats->total_ms = ms;
+        (void)replayed;
     }
 }
 
@@ -579,7 +798,11 @@ uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* x, uint64_t nq, uint64
     e8q_ctx::Carve cv;
     const uint64_t maxp = std::min<uint64_t>(max_nprobe, x->view.nlist);
     M8Carve(cv, b, x->view, nq, k, maxp, true, true, true);
-    if (mode == E8Q_MODE_BATCHED && M8BatchedFits(x, k)) M8CarveBatched(cv, b, x, nq, k, nq * maxp, false);
+    if (mode == E8Q_MODE_BATCHED && M8BatchedFits(x, k) && nq > 0) {
+        const uint32_t mp = static_cast<uint32_t>(maxp);
+        M8CarveBatched(cv, b, x, nq, k, nq * maxp, false,
+                       M8CapsFor(nq, M8PoolWant(x, nq, mp, 0), kM8DefaultBudget));
+    }
     return cv.bytes();
 }
 
@@ -668,6 +891,13 @@ e8q_status e8q_m8_create(e8q_ctx* ctx, const e8q_m8_desc* d, e8q_m8index** out) 
                 x->batch_begin[c + 1] = x->batch_begin[c] + (x->list_begin[c + 1] - x->list_begin[c] + 31) / 32;
             }
             x->nbatch = x->batch_begin[d->nlist];
+            {   // the largest lists first: what c probed lists can hold at most
+                std::vector<uint64_t> len(d->nlist);
+                for (uint64_t c = 0; c < d->nlist; ++c) len[c] = x->list_begin[c + 1] - x->list_begin[c];
+                std::sort(len.begin(), len.end(), std::greater<uint64_t>());
+                x->top_lists.assign(d->nlist + 1, 0);
+                for (uint64_t c = 0; c < d->nlist; ++c) x->top_lists[c + 1] = x->top_lists[c] + len[c];
+            }
             // tables: signs, float levels per run, encoder tables per distinct rate
             Upload(x->signs, d->signs, total_signs, st);
             size_t lev_total = 0;
@@ -834,6 +1064,23 @@ e8q_status e8q_m8_encode(e8q_ctx* ctx, e8q_m8index* x, const float* base, const 
 e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e8q_m8_stats* stats,
                          uint64_t* per_query_stats) {
     return Guard([&] { M8Search(ctx, x, a, stats, per_query_stats, nullptr); });
+}
+
+e8q_status e8q_ctx_set_m8_grids(e8q_ctx* ctx, uint32_t mfma_grid, uint32_t refine_grid) {
+    return Guard([&] {
+        if (!ctx) Throw(E8Q_EINVAL, "ctx is null");
+        if (mfma_grid > 65536 || refine_grid > 65536) Throw(E8Q_EINVAL, "grids must be <= 65536");
+        ctx->m8_fix_mfma = mfma_grid;
+        ctx->m8_fix_refine = refine_grid;
+    });
+}
+
+e8q_status e8q_ctx_get_m8_info(const e8q_ctx* ctx, e8q_m8_ctx_info* out) {
+    return Guard([&] {
+        if (!ctx || !out) Throw(E8Q_EINVAL, "null argument");
+        *out = e8q_m8_ctx_info{ctx->m8_last_mfma, ctx->m8_last_refine, ctx->m8_last_budget, ctx->m8_last_set_a,
+                               ctx->m8_tunings, ctx->m8_pool_per_q};
+    });
 }
 
 e8q_status e8q_m8_debug_batch_checksum(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a,

@@ -292,7 +292,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
     const float z = a.z;
     const double dz = z;
 
-    const uint64_t nrun_q = a.qsel ? a.nsel : a.nq;
+    const uint64_t nrun_q = a.qsel ? (a.nsel_dev ? min<uint64_t>(a.nsel, *a.nsel_dev) : a.nsel) : a.nq;
     for (uint64_t qj = blockIdx.x; qj < nrun_q; qj += gridDim.x) {
         const uint64_t qi = a.qsel ? a.qsel[qj] : qj;
         for (uint32_t d = lane; d < P; d += 64) q[d] = a.rotq[qi * P + d];
@@ -683,7 +683,10 @@ hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a) {
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
     }
-    const uint32_t grid = static_cast<uint32_t>(nrun < 1048576 ? nrun : 1048576);
+    // with the count on the device, most of nsel are usually not searched:
+    // a smaller grid strides over them
+    const uint64_t gmax = a.nsel_dev ? 4096 : 1048576;
+    const uint32_t grid = static_cast<uint32_t>(nrun < gmax ? nrun : gmax);
     hipLaunchKernelGGL(M8ScanKernel, dim3(grid), dim3(64), lds, st, a);
     return hipGetLastError();
 }

@@ -24,6 +24,7 @@ if not os.path.exists(os.path.join(_here, "libe8q_gpu.so")):
     raise ImportError("e8qlib: libe8q_gpu.so is not built (run __graft_entry__.build())")
 
 from ._e8qlib import (E8qIndex, Me8qIndex, Server, search_context_info, set_batched_shape,  # noqa: E402
-                      set_search_graphs)
+                      set_me8q_grids, set_search_graphs)
 
-__all__ = ["E8qIndex", "Me8qIndex", "Server", "set_search_graphs", "set_batched_shape", "search_context_info"]
+__all__ = ["E8qIndex", "Me8qIndex", "Server", "set_search_graphs", "set_batched_shape", "set_me8q_grids",
+           "search_context_info"]

@@ -126,9 +126,35 @@ e8q_status e8q_m8_encode(e8q_ctx* ctx, e8q_m8index* idx, const float* base, cons
 e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* idx, const e8q_m8_search_args* args,
                          e8q_m8_stats* stats, uint64_t* per_query_stats);
 /* Device scratch of one search of nq host queries with per-query nprobe <=
- * max_nprobe and host outputs in `mode`, carved from the context workspace. */
+ * max_nprobe and host outputs in `mode`, carved from the context workspace.
+ * In the batched form it counts the candidate pool and the count pass's
+ * records at their default capacity: min(2^31, what the max_nprobe largest
+ * lists hold, nq x 65536) entries of 28 bytes (pool, cascade list, lower
+ * bound, record), plus the threshold pass's at the default budget.  A
+ * context whose workspace was reserved (e8q_ctx_reserve) to at least this
+ * size runs such a call without allocating; on a reserved context a call
+ * never grows the pool past the reservation, and the queries whose
+ * candidates do not fit are searched by the plain form (stats.overflowed). */
 uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* idx, uint64_t nq, uint64_t k, uint64_t max_nprobe,
                                        int32_t mode);
+
+/* Grids of the batched form's two persistent kernels on this context (no
+ * reference counterpart: the reference makes no device launches): the int8
+ * MFMA contraction and the exact refine, in workgroups.  0 leaves each to
+ * the context's autotuner, which times the whole search at 1, 2 and 4
+ * workgroups per CU for the contraction, then 4, 8 and 16 per CU for the
+ * refine, on the first batched search of at least 64 queries per (index
+ * shape, power-of-two buckets of the query count, nprobe and k).  Results do
+ * not depend on the grids.  Graph replay (e8q_ctx_set_graphs) covers
+ * e8q_m8_search too: the batched form's whole sequence is one graph. */
+e8q_status e8q_ctx_set_m8_grids(e8q_ctx* ctx, uint32_t mfma_grid, uint32_t refine_grid);
+typedef struct {
+    uint32_t mfma_grid, refine_grid;   /* grids of the last batched search */
+    uint32_t budget, set_a;            /* its threshold budget and set-A size */
+    uint64_t tunings;                  /* autotuner runs of the batched ME8Q form */
+    uint64_t pool_per_query;           /* pool entries per query learned from past calls (0: default) */
+} e8q_m8_ctx_info;
+e8q_status e8q_ctx_get_m8_info(const e8q_ctx* ctx, e8q_m8_ctx_info* out);
 
 /* ---- diagnostics used by tests (reference layouts, host pointers) -------- */
 /* The batched form with a checksum of its integer coarse sums: chk[q] =

@@ -48,6 +48,11 @@ class CtxInfo(ctypes.Structure):
                 ("graphs", ctypes.c_uint32), ("tunings", u64), ("captures", u64), ("replays", u64)]
 
 
+class M8CtxInfo(ctypes.Structure):
+    _fields_ = [("mfma_grid", ctypes.c_uint32), ("refine_grid", ctypes.c_uint32), ("budget", ctypes.c_uint32),
+                ("set_a", ctypes.c_uint32), ("tunings", u64), ("pool_per_query", u64)]
+
+
 MODE_PLAIN, MODE_BATCHED = 0, 1
 
 _lib = None
@@ -105,6 +110,20 @@ class Ctx:
         i = CtxInfo()
         check(lib().e8q_ctx_get_info(self.h, ctypes.byref(i)))
         return {f: getattr(i, f) for f, _ in CtxInfo._fields_}
+
+    def set_m8_grids(self, mfma=0, refine=0):
+        check(lib().e8q_ctx_set_m8_grids(self.h, mfma, refine))
+
+    def m8_info(self):
+        i = M8CtxInfo()
+        check(lib().e8q_ctx_get_m8_info(self.h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in M8CtxInfo._fields_}
+
+    def reserve(self, nbytes):
+        check(lib().e8q_ctx_reserve(self.h, ctypes.c_uint64(int(nbytes))))
+
+    def workspace(self):
+        return int(lib().e8q_ctx_workspace_bytes(self.h))
 
     def close(self):
         if self.h:

@@ -126,22 +126,25 @@ class M8Index:
         check(lib().e8q_m8_debug_query(self.ctx.h, self.h, p(q), nq, nprobe, p(rq), p(lut), p(md), p(probe), p(pd)))
         return rq, lut, md, probe, pd
 
-    def _args(self, q, k, nprobe, z, npq, mode, budget, ids, d):
+    def _args(self, q, k, nprobe, z, npq, mode, budget, ids, d, set_a=0):
         s = M8SearchArgs()
         s.queries, s.nq, s.k, s.nprobe, s.nprobe_per_query = p(q), q.shape[0], k, nprobe, p(npq)
         s.z, s.queries_on_device, s.outputs_on_device = z, 0, 0
         s.out_ids, s.out_dists = p(ids), p(d)
-        s.mode, s.batched_budget = mode, budget
+        s.mode, s.batched_budget, s.batched_set_a = mode, budget, set_a
         return s
 
-    def search(self, queries, k, nprobe, z, nprobe_per_query=None, mode=PLAIN, budget=0):
+    def workspace_bytes(self, nq, k, max_nprobe, mode=PLAIN):
+        return int(lib().e8q_m8_search_workspace_bytes(self.h, u64(nq), u64(k), u64(max_nprobe), mode))
+
+    def search(self, queries, k, nprobe, z, nprobe_per_query=None, mode=PLAIN, budget=0, set_a=0):
         q = np.ascontiguousarray(queries, np.float32)
         nq = q.shape[0]
         ids = np.empty((nq, k), np.uint32)
         d = np.empty((nq, k), np.float32)
         st = np.empty((nq, 5), np.uint64)
         npq = None if nprobe_per_query is None else np.ascontiguousarray(nprobe_per_query, np.uint32)
-        s = self._args(q, k, nprobe, z, npq, mode, budget, ids, d)
+        s = self._args(q, k, nprobe, z, npq, mode, budget, ids, d, set_a)
         tot = M8Stats()
         check(lib().e8q_m8_search(self.ctx.h, self.h, ctypes.byref(s), ctypes.byref(tot), p(st)))
         return ids, d, st, tot
