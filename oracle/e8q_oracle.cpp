@@ -965,6 +965,215 @@ void Dist64(const Index& ix, const float* query, size_t nprobe, const uint32_t* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Diagnostics: ExhaustiveTopK, Estimate, ProbeBoundW (e8q_oracle.cpp:14-219).
+
+// ExactScore (e8q_search.cpp:87-126).  b = 1: the AVX2 form
+// (e8q_search_avx2.cpp:53-80), one FMA chain per component over the blocks
+// then the horizontal sum; b >= 2: one fp64 chain in dimension order (GCC
+// contracts `acc += q * tab`).
+float ExactScore(const Index& ix, const Row& r, const float* q) {
+    const size_t stride = ix.bits >= 9 ? (size_t{1} << ix.bits) : 256;
+    if (ix.bits == 1) {
+        float acc[8] = {};
+        for (int blk = 0; blk < ix.blocks; ++blk) {
+            int d[8];
+            int br;
+            DecodeBlock(ix, r, blk, d, &br);
+            for (int i = 0; i < 8; ++i)
+                acc[i] = std::fma(ix.refine_tab[static_cast<size_t>(br) * stride + d[i]], q[blk * 8 + i], acc[i]);
+        }
+        return Hsum8(acc);
+    }
+    double acc = 0.0;
+    for (int blk = 0; blk < ix.blocks; ++blk) {
+        int d[8];
+        int br;
+        DecodeBlock(ix, r, blk, d, &br);
+        const float* tab = ix.refine_tab.data() + static_cast<size_t>(br) * stride;
+        for (int i = 0; i < 8; ++i)
+            acc = std::fma(static_cast<double>(q[blk * 8 + i]), static_cast<double>(tab[d[i]]), acc);
+    }
+    return static_cast<float>(acc);
+}
+
+// (query, cluster) distance of ExhaustiveTopK / Estimate: |q|^2 and <q, c>
+// summed in dimension order, (qn2 + cn2) - 2 dot, clamped at 0.  GCC
+// vectorizes the two sums as in-order reductions: the products of each run
+// of 8 (then 4) dimensions are rounded on their own and added one by one;
+// only the last P % 4 dimensions are fused multiply-adds.
+float ClusterD2Seq(const Index& ix, const float* q, size_t c) {
+    const float* cc = &ix.centroid_rot[c * ix.padded];
+    const size_t P = ix.padded, body = P & ~size_t{3};
+    float qn2 = 0.0F, dot = 0.0F;
+    for (size_t d = 0; d < body; ++d) {
+        const float pq = q[d] * q[d], pc = q[d] * cc[d];
+        qn2 = qn2 + pq;
+        dot = dot + pc;
+    }
+    for (size_t d = body; d < P; ++d) {
+        qn2 = std::fma(q[d], q[d], qn2);
+        dot = std::fma(q[d], cc[d], dot);
+    }
+    float d2 = std::fma(-2.0F, dot, qn2 + ix.centroid_n2[c]);
+    return d2 < 0.0F ? 0.0F : d2;
+}
+
+// RowsAt (e8q.hpp:289-306): vector i of cluster c.
+static Row RowAt(const Index& ix, size_t c, size_t i) {
+    const uint8_t* bp = ix.arena.data() + ix.arena_begin[c] + (i / 32) * ix.batch_bytes;
+    const uint8_t* rr = ix.ref_arena.data() + ix.ref_arena_begin[c] + (i / 32) * ix.ref_batch_bytes;
+    const size_t t = i % 32;
+    return Row{ix.bits >= 2 ? rr + t * ix.RefineStride() : nullptr, bp + kHdr, static_cast<uint32_t>(t),
+               reinterpret_cast<const float*>(rr + ix.RefG1Off())[t],
+               reinterpret_cast<const float*>(rr + ix.RefH3Off())[t], ix.ids[ix.cluster_begin[c] + i]};
+}
+
+// ExhaustiveTopK (e8q_oracle.cpp:14-56): every vector, d2 + h3 - 2 g1 S
+// (fused), the k best under Better (distance, then id).
+void ExhaustiveTopK(const Index& ix, const float* query, size_t k, uint32_t* out_ids, float* out_dists) {
+    std::vector<float> q(ix.padded);
+    RotateQuery(ix, query, q.data());
+    std::vector<std::pair<float, uint32_t>> all;
+    for (size_t c = 0; c < ix.nlist; ++c) {
+        const float d2 = ClusterD2Seq(ix, q.data(), c);
+        const size_t cnt = ix.cluster_begin[c + 1] - ix.cluster_begin[c];
+        for (size_t i = 0; i < cnt; ++i) {
+            const Row r = RowAt(ix, c, i);
+            all.push_back({std::fma(-(2.0F * r.g1), ExactScore(ix, r, q.data()), d2 + r.h3), r.id});
+        }
+    }
+    const size_t take = std::min(k, all.size());
+    std::partial_sort(all.begin(), all.begin() + static_cast<long>(take), all.end(), TopK::Cmp);
+    for (size_t i = 0; i < k; ++i) {
+        out_ids[i] = i < take ? all[i].second : std::numeric_limits<uint32_t>::max();
+        out_dists[i] = i < take ? all[i].first : std::numeric_limits<float>::infinity();
+    }
+}
+
+// Position of global id `id`: the last stored vector carrying it (the
+// reference's inv_ids_ map, e8q_oracle.cpp:63-74); npos when none.
+static size_t PositionOf(const Index& ix, uint32_t id) {
+    size_t pos = static_cast<size_t>(-1);
+    if (id >= ix.n) return pos;
+    for (size_t i = 0; i < ix.n; ++i)
+        if (ix.ids[i] == id) pos = i;
+    return pos;
+}
+
+static size_t ClusterOf(const Index& ix, size_t pos) {
+    const auto it = std::upper_bound(ix.cluster_begin.begin() + 1, ix.cluster_begin.end(), pos);
+    return static_cast<size_t>(it - ix.cluster_begin.begin() - 1);
+}
+
+// Estimate (e8q_oracle.cpp:58-101); returns false for an unknown id.
+bool Estimate(const Index& ix, const float* query, uint32_t id, float* out) {
+    const size_t pos = PositionOf(ix, id);
+    if (pos == static_cast<size_t>(-1)) return false;
+    const size_t c = ClusterOf(ix, pos), local = pos - ix.cluster_begin[c];
+    std::vector<float> q(ix.padded);
+    RotateQuery(ix, query, q.data());
+    const float d2 = ClusterD2Seq(ix, q.data(), c);
+    const Row r = RowAt(ix, c, local);
+    *out = std::fma(-(2.0F * r.g1), ExactScore(ix, r, q.data()), d2 + r.h3);
+    return true;
+}
+
+// DecodeSplitBlockLane (e8q_scan.cpp:34-61) as ProbeBoundW calls it: the
+// 8-bit-digit form for every rate.  At b = 1 the (empty) field makes digit 7
+// the rowA byte's bit 0 and the branch 0; at b = 9 the digits keep their low
+// 8 bits.  (At b >= 10 the reference's 8-byte staging is overrun: undefined.)
+static void DecodeLiteral(int bits, uint8_t rb, const uint8_t* field, uint8_t d[8], int* br) {
+    const int c = bits - 1;
+    uint64_t x = 0;
+    for (int i = 0; i < c; ++i) x |= static_cast<uint64_t>(field[i]) << (8 * i);
+    const uint64_t m = (uint64_t{1} << c) - 1;
+    int par = 0;
+    for (int i = 0; i < 7; ++i) {
+        const uint64_t f = (x >> (c * i)) & m;
+        d[i] = static_cast<uint8_t>(f | (static_cast<uint64_t>((rb >> (7 - i)) & 1) << c));
+        par ^= static_cast<int>(f & 1u);
+    }
+    const uint64_t f7 = (x >> (c * 7)) & m;
+    d[7] = static_cast<uint8_t>(((f7 >> 1) << 1) | static_cast<uint64_t>(par) | (static_cast<uint64_t>(rb & 1) << c));
+    *br = static_cast<int>(f7 & 1u);
+}
+
+// ProbeBoundW (e8q_oracle.cpp:103-219): the normalized gap between the
+// coarse (class-mean) and the exact estimate of one stored vector from its
+// build row, (d_coarse - d_full) / (r_q Omega); NaN on a zero residual or a
+// degenerate code.  Returns false for an unknown id (or b >= 10).
+bool ProbeBoundW(const Index& ix, const float* query, const float* base_row, uint32_t id, double* out) {
+    const size_t pos = PositionOf(ix, id);
+    if (pos == static_cast<size_t>(-1) || ix.bits >= 10) return false;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const size_t c = ClusterOf(ix, pos), local = pos - ix.cluster_begin[c];
+    const size_t P = ix.padded;
+    const float* cc = &ix.centroid_rot[c * P];
+    std::vector<float> rq(P), rx(P);
+    RotateQuery(ix, query, rq.data());
+    RotateQuery(ix, base_row, rx.data());
+    double d2 = 0.0, rn2 = 0.0;
+    std::vector<double> unit(P);
+    for (size_t d = 0; d < P; ++d) {
+        const double qr = static_cast<double>(rq[d]) - static_cast<double>(cc[d]);
+        const double xr = static_cast<double>(rx[d]) - static_cast<double>(cc[d]);
+        d2 = std::fma(qr, qr, d2);
+        rn2 = std::fma(xr, xr, rn2);
+        unit[d] = xr;
+    }
+    const double rn = std::sqrt(rn2);
+    if (!(rn > 0.0)) {
+        *out = nan;
+        return true;
+    }
+    for (size_t d = 0; d < P; ++d) unit[d] /= rn;
+    const std::vector<double> lev[2] = {BranchLevels(ix.bits, 0), BranchLevels(ix.bits, 1)};
+    const double rd = std::sqrt(static_cast<double>(P));
+    const Row r = RowAt(ix, c, local);
+    double dot_ue = 0, n2_e = 0, dot_ce = 0, dot_ew = 0, dot_uw = 0, n2_w = 0, dot_cw = 0, s_exact = 0, s_coarse = 0;
+    const int cb = ix.bits - 1;
+    for (int blk = 0; blk < ix.blocks; ++blk) {
+        uint8_t dg[8];
+        int br;
+        DecodeLiteral(ix.bits, r.plane[static_cast<size_t>(blk) * 32 + r.lane],
+                      ix.bits >= 2 ? r.refine + static_cast<size_t>(blk) * cb : nullptr, dg, &br);
+        for (int k = 0; k < 8; ++k) {
+            const size_t i = static_cast<size_t>(blk) * 8 + static_cast<size_t>(k);
+            const double ve = lev[br][dg[k]] / rd;
+            const double wc = ix.gate_mid[2 * k + (dg[k] >> cb)];
+            dot_ue = std::fma(unit[i], ve, dot_ue);
+            n2_e = std::fma(ve, ve, n2_e);
+            dot_ce = std::fma(static_cast<double>(cc[i]), ve, dot_ce);
+            dot_ew = std::fma(ve, wc, dot_ew);
+            dot_uw = std::fma(unit[i], wc, dot_uw);
+            n2_w = std::fma(wc, wc, n2_w);
+            dot_cw = std::fma(static_cast<double>(cc[i]), wc, dot_cw);
+            s_exact = std::fma(static_cast<double>(rq[i]), ve, s_exact);
+            s_coarse = std::fma(static_cast<double>(rq[i]), wc, s_coarse);
+        }
+    }
+    if (!(std::abs(dot_ue) > 0.0) || !(std::abs(dot_uw) > 0.0)) {
+        *out = nan;
+        return true;
+    }
+    const double g1 = rn / dot_ue;
+    const double h3 = std::fma(2.0 * g1, dot_ce, rn2);
+    const double d_full = std::fma(-(2.0 * g1), s_exact, d2 + h3);
+    const double w_x = 2.0 * rn / dot_uw;
+    const double d_coarse = std::fma(-w_x, s_coarse, std::fma(w_x, dot_cw, d2 + rn2));
+    const double rx2 = std::fma(-((4.0 * g1) * w_x), dot_ew, std::fma((4.0 * g1) * g1, n2_e, (w_x * w_x) * n2_w));
+    const double sqrt_dm1 = std::sqrt(static_cast<double>(std::max<size_t>(ix.dim - 1, 1)));
+    const double omega = std::sqrt(std::max(0.0, rx2)) / sqrt_dm1;
+    const double r_q = std::sqrt(std::max(0.0, d2));
+    if (!(omega > 0.0) || !(r_q > 0.0)) {
+        *out = nan;
+        return true;
+    }
+    *out = (d_coarse - d_full) / (r_q * omega);
+    return true;
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -1138,6 +1347,32 @@ void orc_scan_all(void* h, const float* queries, size_t nq, size_t k, size_t npr
     for (long long i = 0; i < static_cast<long long>(nq); ++i)
         orc::ScanAllTopK(*ix, queries + static_cast<size_t>(i) * ix->dim, k, nprobe, ids + static_cast<size_t>(i) * k,
                          dists + static_cast<size_t>(i) * k);
+}
+
+
+// Diagnostics over (query, id) pairs; status 1 = an unknown id (or b >= 10
+// for ProbeBoundW).
+int orc_estimate(void* h, const float* queries, const uint32_t* qidx, const uint32_t* ids, size_t npair,
+                 float* out) {
+    const auto& ix = *static_cast<orc::Index*>(h);
+    for (size_t i = 0; i < npair; ++i)
+        if (!orc::Estimate(ix, queries + static_cast<size_t>(qidx[i]) * ix.dim, ids[i], out + i)) return 1;
+    return 0;
+}
+
+int orc_probe_bound_w(void* h, const float* queries, const float* base_rows, const uint32_t* qidx,
+                      const uint32_t* ids, size_t npair, double* out) {
+    const auto& ix = *static_cast<orc::Index*>(h);
+    for (size_t i = 0; i < npair; ++i)
+        if (!orc::ProbeBoundW(ix, queries + static_cast<size_t>(qidx[i]) * ix.dim, base_rows + i * ix.dim, ids[i],
+                              out + i))
+            return 1;
+    return 0;
+}
+
+void orc_exhaustive_topk(void* h, const float* queries, size_t nq, size_t k, uint32_t* ids, float* dists) {
+    const auto& ix = *static_cast<orc::Index*>(h);
+    for (size_t i = 0; i < nq; ++i) orc::ExhaustiveTopK(ix, queries + i * ix.dim, k, ids + i * k, dists + i * k);
 }
 
 }  // extern "C"

@@ -1014,6 +1014,138 @@ std::vector<uint8_t> Image(const Index& ix) {
     return out;
 }
 
+// ---------------------------------------------------------------------------
+// Diagnostics: Estimate (me8q_measure.cpp:41-67), ExhaustiveTopK
+// (me8q_search.cpp:238-266), SearchIVF (me8q_search.cpp:136-236).
+
+// ScoreFast (me8q_score_avx2.cpp:104-129) over a run's bit-tight code:
+// component i of even blocks in acc[i], of odd blocks in acc2[i] (one chain
+// at rate 1), FMA chains, then the AVX horizontal sum of acc + acc2.
+static float ScoreFast(const uint8_t* code, int len, int bits, const float* q) {
+    const Codec& cd = GetCodec();
+    float acc[8] = {}, acc2[8] = {};
+    for (int b = 0; b < len / 8; ++b) {
+        uint16_t d[8];
+        uint8_t br;
+        UnpackBlock(code + static_cast<size_t>(b) * bits, bits, d, &br);
+        const float* lev = cd.tab[br][bits].data();
+        const float* qb = q + static_cast<size_t>(b) * 8;
+        for (int i = 0; i < 8; ++i) {
+            if (bits >= 2 && (b & 1)) acc2[i] = std::fma(lev[d[i]], qb[i], acc2[i]);
+            else acc[i] = std::fma(lev[d[i]], qb[i], acc[i]);
+        }
+    }
+    if (bits >= 2)
+        for (int i = 0; i < 8; ++i) acc[i] = acc[i] + acc2[i];
+    return Hsum8(acc);
+}
+
+// EstimateRow (me8q_search.cpp:77-99): d2 + sum over the coded runs of
+// h2 - 2 coef S (the product fused into the subtraction; a run with coef 0
+// adds h2).
+static double EstimateRow(const Index& ix, const uint8_t* row, double d2, const float* q) {
+    double acc = d2;
+    for (size_t r = 0; r < ix.nrun; ++r) {
+        float fac[2];
+        std::memcpy(fac, row + r * 8, 8);
+        const double coef = fac[0];
+        if (coef == 0.0) {
+            acc += fac[1];
+            continue;
+        }
+        const double sc = ScoreFast(row + ix.run_code_off[r], static_cast<int>(ix.run_len[r]),
+                                    static_cast<int>(ix.run_rate[r]), q + ix.run_dim_off[r]);
+        acc += std::fma(-(2.0 * coef), sc, static_cast<double>(fac[1]));
+    }
+    return acc;
+}
+
+// The padded query, its per-run rotation (zero tail) and the fp64 squared
+// distance to list c's unrotated centroid (dimension order, contracted).
+static void PadRotate(const Index& ix, const float* query, std::vector<float>& pad, std::vector<float>& rot) {
+    pad.assign(ix.padded, 0.0F);
+    std::memcpy(pad.data(), query, ix.dim * 4);
+    rot.assign(ix.padded, 0.0F);
+    for (size_t r = 0; r < ix.nrun; ++r) ix.rot[r].Rotate(&pad[ix.run_dim_off[r]], &rot[ix.run_dim_off[r]]);
+}
+
+static double CentroidD2(const Index& ix, const float* pad, size_t c) {
+    double d2 = 0.0;
+    for (size_t j = 0; j < ix.padded; ++j) {
+        const double d = static_cast<double>(pad[j]) - static_cast<double>(ix.centroids[c * ix.padded + j]);
+        d2 = std::fma(d, d, d2);
+    }
+    return d2;
+}
+
+static const uint8_t* RowOf(const Index& ix, size_t c, size_t i) {
+    return ix.codes.data() + ix.byte_begin[c] + (i - ix.list_begin[c]) * ix.stride;
+}
+
+// Estimate; false for an unknown id (the last stored vector carrying it
+// answers, as EnsureIdMap's overwrite order leaves it).
+bool Estimate(const Index& ix, const float* query, uint32_t id, float* out) {
+    size_t pos = static_cast<size_t>(-1);
+    if (id < ix.n)
+        for (size_t i = 0; i < ix.n; ++i)
+            if (ix.ids[i] == id) pos = i;
+    if (pos == static_cast<size_t>(-1)) return false;
+    const auto it = std::upper_bound(ix.list_begin.begin() + 1, ix.list_begin.end(), pos);
+    const size_t c = static_cast<size_t>(it - ix.list_begin.begin() - 1);
+    std::vector<float> pad, rot;
+    PadRotate(ix, query, pad, rot);
+    *out = static_cast<float>(EstimateRow(ix, RowOf(ix, c, pos), CentroidD2(ix, pad.data(), c), rot.data()));
+    return true;
+}
+
+void ExhaustiveTopK(const Index& ix, const float* query, size_t k, std::vector<std::pair<float, uint32_t>>& out) {
+    out.clear();
+    std::vector<float> pad, rot;
+    PadRotate(ix, query, pad, rot);
+    for (size_t c = 0; c < ix.nlist; ++c) {
+        const double d2 = CentroidD2(ix, pad.data(), c);
+        for (size_t i = ix.list_begin[c]; i < ix.list_begin[c + 1]; ++i)
+            Push(out, k, static_cast<float>(EstimateRow(ix, RowOf(ix, c, i), d2, rot.data())), ix.ids[i]);
+    }
+}
+
+// SearchIVF: the route's probe distances, every row of every probed list in
+// probe order.  stats6: candidates, clusters, bits_scored, bits_total,
+// survivors, pruned (0).
+void SearchIVF(const Index& ix, const float* query, size_t k, size_t nprobe, uint32_t* out_ids, float* out_d,
+               uint64_t* stats6) {
+    if (k == 0 || nprobe == 0 || ix.n == 0) return;
+    std::vector<float> pad, rot;
+    PadRotate(ix, query, pad, rot);
+    std::vector<float> pd;
+    std::vector<uint32_t> probe;
+    Route(ix, rot.data(), nprobe, pd, probe);
+    uint64_t bits_row = 0;
+    for (size_t r = 0; r < ix.nrun; ++r) bits_row += static_cast<uint64_t>(ix.run_rate[r]) * ix.run_len[r];
+    std::vector<std::pair<float, uint32_t>> h;
+    uint64_t cand = 0;
+    for (size_t p = 0; p < probe.size(); ++p) {
+        const size_t c = probe[p];
+        for (size_t i = ix.list_begin[c]; i < ix.list_begin[c + 1]; ++i) {
+            Push(h, k, static_cast<float>(EstimateRow(ix, RowOf(ix, c, i), static_cast<double>(pd[p]), rot.data())),
+                 ix.ids[i]);
+            ++cand;
+        }
+    }
+    for (size_t i = 0; i < k; ++i) {
+        out_ids[i] = i < h.size() ? h[i].second : std::numeric_limits<uint32_t>::max();
+        out_d[i] = i < h.size() ? h[i].first : std::numeric_limits<float>::infinity();
+    }
+    if (stats6) {
+        stats6[0] = cand;
+        stats6[1] = probe.size();
+        stats6[2] = cand * bits_row;
+        stats6[3] = cand * bits_row;
+        stats6[4] = cand;
+        stats6[5] = 0;
+    }
+}
+
 }  // namespace orc::m8
 
 // ---------------------------------------------------------------------------
@@ -1128,6 +1260,36 @@ int orc_m8_search(void* h, const float* queries, size_t nq, size_t k, size_t npr
                   float* dists, uint64_t* stats5) {
     return orc_m8_search_z(h, queries, nq, k, nprobe, threads, std::numeric_limits<float>::quiet_NaN(), ids, dists,
                            stats5);
+}
+
+// Diagnostics.  orc_m8_estimate: status 1 = an unknown id.
+int orc_m8_estimate(void* h, const float* queries, const uint32_t* qidx, const uint32_t* ids, size_t npair,
+                    float* out) {
+    const auto& ix = *static_cast<orc::m8::Index*>(h);
+    for (size_t i = 0; i < npair; ++i)
+        if (!orc::m8::Estimate(ix, queries + static_cast<size_t>(qidx[i]) * ix.dim, ids[i], out + i)) return 1;
+    return 0;
+}
+
+// k results per query, UINT32_MAX / +inf padded.
+void orc_m8_exhaustive_topk(void* h, const float* queries, size_t nq, size_t k, uint32_t* ids, float* dists) {
+    const auto& ix = *static_cast<orc::m8::Index*>(h);
+    for (size_t i = 0; i < nq; ++i) {
+        std::vector<std::pair<float, uint32_t>> out;
+        orc::m8::ExhaustiveTopK(ix, queries + i * ix.dim, k, out);
+        for (size_t j = 0; j < k; ++j) {
+            ids[i * k + j] = j < out.size() ? out[j].second : std::numeric_limits<uint32_t>::max();
+            dists[i * k + j] = j < out.size() ? out[j].first : std::numeric_limits<float>::infinity();
+        }
+    }
+}
+
+void orc_m8_search_ivf(void* h, const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* ids,
+                       float* dists, uint64_t* stats6) {
+    const auto& ix = *static_cast<orc::m8::Index*>(h);
+    for (size_t i = 0; i < nq; ++i)
+        orc::m8::SearchIVF(ix, queries + i * ix.dim, k, nprobe, ids + i * k, dists + i * k,
+                           stats6 ? stats6 + i * 6 : nullptr);
 }
 
 }  // extern "C"

@@ -190,3 +190,32 @@ def e8ax_digests(ix):
     return {key: digest(np.asarray(ix[key])) for key in ("ids", "signs", "mean_rot", "centroid_rot", "centroid_n2",
                                                           "cluster_begin", "arena_begin", "arena", "ref_arena_begin",
                                                           "ref_arena")}
+
+
+# Diagnostics (Estimate, ProbeBoundW, ExhaustiveTopK, SearchIVF) over the
+# fixture indexes: (query, id) pairs, six random ids per query plus the row
+# that sits exactly on its centroid (row 5 of an E8Q mixture, row 3 of an
+# ME8Q one) for the first and the last query.
+def diag_pairs(name):
+    if name in INDEX_CASES:
+        n, nq, seed, zero = INDEX_CASES[name][0], INDEX_CASES[name][7], INDEX_CASES[name][4], 5
+    else:
+        n, nq, seed, zero = ME8Q_CASES[name][0], ME8Q_CASES[name][10], ME8Q_CASES[name][4], 3
+    rng = np.random.default_rng(7000 + seed)
+    pq = np.repeat(np.arange(nq), 6)
+    pid = rng.integers(0, n, size=pq.size)
+    pq = np.concatenate([pq, [0, nq - 1]])
+    pid = np.concatenate([pid, [zero, zero]])
+    return pq.astype(np.uint32), pid.astype(np.uint32)
+
+
+def diag_k(name):
+    """k of E8Q ExhaustiveTopK (ME8Q: the k of diag_ivf_grid)."""
+    return [10, 64]
+
+
+def diag_ivf_grid(name):
+    """(k, nprobe) of ME8Q SearchIVF."""
+    nlist = ME8Q_CASES[name][2]
+    grid = me8q_search_grid(name)
+    return grid + [(400, nlist)] if name == "m64drop" else grid

@@ -15,6 +15,8 @@
 //   mbuild   DATA.npy CENT.npy IDS.npy AVG_BITS SEED ALPHA PLAN_SPEC|- VAR.npy|- DROP_ENERGY
 //            OUT.me8q OUT_PREFIX
 //   msearch  INDEX.me8q QUERIES.npy K NPROBE OUT_PREFIX
+//   diag     INDEX.e8ax QUERIES.npy BASE.npy PAIR_Q.npy PAIR_ID.npy K OUT_PREFIX
+//   mdiag    INDEX.me8q QUERIES.npy PAIR_Q.npy PAIR_ID.npy K NPROBE OUT_PREFIX
 //
 // The ME8Q commands read the index's fast layout (private members) to dump
 // it: the class's `private` is mapped to `public` around its header only.
@@ -383,6 +385,91 @@ int CmdMSearch(char** av) {
     return 0;
 }
 
+// E8Q diagnostics: Estimate and ProbeBoundW for (query, id) pairs (the base
+// row of id is row id of BASE), ExhaustiveTopK of every query at k.
+int CmdDiag(char** av) {
+    e8qlib::E8qIndex idx;
+    idx.Load(av[0]);
+    const Npy q = ReadNpy(av[1]);
+    const Npy base = ReadNpy(av[2]);
+    const Npy pq = ReadNpy(av[3]);
+    const Npy pid = ReadNpy(av[4]);
+    const size_t k = std::strtoull(av[5], nullptr, 10);
+    const std::string pre = av[6];
+    const size_t nq = q.shape[0], dim = q.shape[1], np_ = pq.shape[0];
+    const float* qp = As<float>(q, "<f4");
+    const float* bp = As<float>(base, "<f4");
+    const uint32_t* pqp = As<uint32_t>(pq, "<u4");
+    const uint32_t* pip = As<uint32_t>(pid, "<u4");
+    std::vector<float> est(np_);
+    std::vector<double> pbw(np_);
+    for (size_t i = 0; i < np_; ++i) {
+        est[i] = idx.Estimate(qp + pqp[i] * dim, pip[i]);
+        pbw[i] = idx.ProbeBoundW(qp + pqp[i] * dim, bp + static_cast<size_t>(pip[i]) * dim, pip[i]);
+    }
+    std::vector<uint32_t> ids(nq * k, std::numeric_limits<uint32_t>::max());
+    std::vector<float> d(nq * k, std::numeric_limits<float>::infinity());
+    for (size_t i = 0; i < nq; ++i) {
+        std::vector<std::pair<float, uint32_t>> out;
+        idx.ExhaustiveTopK(qp + i * dim, k, out);
+        for (size_t j = 0; j < out.size(); ++j) {
+            d[i * k + j] = out[j].first;
+            ids[i * k + j] = out[j].second;
+        }
+    }
+    WriteNpy(pre + "_est.npy", "<f4", {np_}, est.data(), est.size() * 4);
+    WriteNpy(pre + "_pbw.npy", "<f8", {np_}, pbw.data(), pbw.size() * 8);
+    WriteNpy(pre + "_exids.npy", "<u4", {nq, k}, ids.data(), ids.size() * 4);
+    WriteNpy(pre + "_exd.npy", "<f4", {nq, k}, d.data(), d.size() * 4);
+    return 0;
+}
+
+// ME8Q diagnostics: Estimate for (query, id) pairs, ExhaustiveTopK at k and
+// SearchIVF at (k, nprobe) of every query; the SearchIVF stats per query are
+// candidates, clusters, bits_scored, bits_total, survivors, pruned.
+int CmdMDiag(char** av) {
+    e8qlib::me8q::Me8qIndex idx;
+    idx.Load(av[0]);
+    const Npy q = ReadNpy(av[1]);
+    const Npy pq = ReadNpy(av[2]);
+    const Npy pid = ReadNpy(av[3]);
+    const size_t k = std::strtoull(av[4], nullptr, 10);
+    const size_t nprobe = std::strtoull(av[5], nullptr, 10);
+    const std::string pre = av[6];
+    const size_t nq = q.shape[0], dim = q.shape[1], np_ = pq.shape[0];
+    const float* qp = As<float>(q, "<f4");
+    const uint32_t* pqp = As<uint32_t>(pq, "<u4");
+    const uint32_t* pip = As<uint32_t>(pid, "<u4");
+    std::vector<float> est(np_);
+    for (size_t i = 0; i < np_; ++i) est[i] = idx.Estimate(qp + pqp[i] * dim, pip[i]);
+    std::vector<uint32_t> ex(nq * k, std::numeric_limits<uint32_t>::max()), iv(nq * k);
+    std::vector<float> exd(nq * k, std::numeric_limits<float>::infinity()), ivd(nq * k);
+    std::vector<uint64_t> st(nq * 6);
+    for (size_t i = 0; i < nq; ++i) {
+        std::vector<std::pair<float, uint32_t>> out;
+        idx.ExhaustiveTopK(qp + i * dim, k, &out);
+        for (size_t j = 0; j < out.size(); ++j) {
+            exd[i * k + j] = out[j].first;
+            ex[i * k + j] = out[j].second;
+        }
+        e8qlib::me8q::SearchStats s;
+        idx.SearchIVF(qp + i * dim, k, nprobe, iv.data() + i * k, &s, ivd.data() + i * k);
+        st[i * 6 + 0] = s.candidates;
+        st[i * 6 + 1] = s.clusters;
+        st[i * 6 + 2] = s.bits_scored;
+        st[i * 6 + 3] = s.bits_total;
+        st[i * 6 + 4] = s.survivors;
+        st[i * 6 + 5] = s.pruned;
+    }
+    WriteNpy(pre + "_est.npy", "<f4", {np_}, est.data(), est.size() * 4);
+    WriteNpy(pre + "_exids.npy", "<u4", {nq, k}, ex.data(), ex.size() * 4);
+    WriteNpy(pre + "_exd.npy", "<f4", {nq, k}, exd.data(), exd.size() * 4);
+    WriteNpy(pre + "_ivfids.npy", "<u4", {nq, k}, iv.data(), iv.size() * 4);
+    WriteNpy(pre + "_ivfd.npy", "<f4", {nq, k}, ivd.data(), ivd.size() * 4);
+    WriteNpy(pre + "_ivfstats.npy", "<u8", {nq, 6}, st.data(), st.size() * 8);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -401,6 +488,8 @@ int main(int argc, char** argv) {
         if (cmd == "search" && argc == 8) return CmdSearch(argv + 2);
         if (cmd == "mbuild" && argc == 13) return CmdMBuild(argv + 2);
         if (cmd == "msearch" && argc == 7) return CmdMSearch(argv + 2);
+        if (cmd == "diag" && argc == 9) return CmdDiag(argv + 2);
+        if (cmd == "mdiag" && argc == 9) return CmdMDiag(argv + 2);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "gen_golden: %s\n", e.what());
         return 1;

@@ -7,7 +7,7 @@ build made by `make -f oracle/ref.mk`):
 
 It drives oracle/_ref/gen_golden (tests/golden/gen_golden.cpp linked against
 the reference e8qlib) on the seeded inputs of cases.py and keeps its outputs
-as .npy/.e8ax fixtures plus manifest.json.  `--only base,flush,c1,c3` writes
+as .npy/.e8ax fixtures plus manifest.json.  `--only base,flush,c1,c3,diag` writes
 only the named sections (default: all).
 """
 import json
@@ -170,7 +170,42 @@ def c3_fixtures(out, manifest, t):
                       "image_sha256": cases.digest(np.fromfile(path, np.uint8))}
 
 
-SECTIONS = {"base": base_fixtures, "flush": flush_fixtures, "c1": c1_fixtures, "c3": c3_fixtures}
+def diag_fixtures(out, manifest, t):
+    """The diagnostics over the fixture indexes: E8Q Estimate, ProbeBoundW
+    and ExhaustiveTopK; ME8Q Estimate, ExhaustiveTopK and SearchIVF."""
+    res = {}
+    for name in cases.INDEX_CASES:
+        data, _c, _l, queries = cases.mixture(name)
+        pq, pid = cases.diag_pairs(name)
+        for fn, arr in (("q", queries), ("base", data), ("pq", pq), ("pid", pid)):
+            np.save(t(fn + ".npy"), arr)
+        for k in cases.diag_k(name):
+            pre = t("g")
+            run("diag", os.path.join(out, f"{name}.e8ax"), t("q.npy"), t("base.npy"), t("pq.npy"), t("pid.npy"), k,
+                pre)
+            res[f"{name}_est"] = np.load(f"{pre}_est.npy")
+            res[f"{name}_pbw"] = np.load(f"{pre}_pbw.npy")
+            res[f"{name}_k{k}_exids"] = np.load(f"{pre}_exids.npy")
+            res[f"{name}_k{k}_exd"] = np.load(f"{pre}_exd.npy")
+    for name in cases.ME8Q_CASES:
+        _d, _c, _l, queries, _lam = cases.me8q_mixture(name)
+        pq, pid = cases.diag_pairs(name)
+        for fn, arr in (("q", queries), ("pq", pq), ("pid", pid)):
+            np.save(t(fn + ".npy"), arr)
+        for (k, nprobe) in cases.diag_ivf_grid(name):
+            pre = t("g")
+            run("mdiag", os.path.join(out, f"{name}.me8q"), t("q.npy"), t("pq.npy"), t("pid.npy"), k, nprobe, pre)
+            res[f"{name}_est"] = np.load(f"{pre}_est.npy")
+            res[f"{name}_k{k}_exids"] = np.load(f"{pre}_exids.npy")
+            res[f"{name}_k{k}_exd"] = np.load(f"{pre}_exd.npy")
+            for part in ("ivfids", "ivfd", "ivfstats"):
+                res[f"{name}_k{k}_p{nprobe}_{part}"] = np.load(f"{pre}_{part}.npy")
+    np.savez_compressed(os.path.join(out, "diag.npz"), **res)
+    manifest["diag"] = {"e8q": list(cases.INDEX_CASES), "me8q": list(cases.ME8Q_CASES)}
+
+
+SECTIONS = {"base": base_fixtures, "flush": flush_fixtures, "c1": c1_fixtures, "c3": c3_fixtures,
+            "diag": diag_fixtures}
 
 
 def main():

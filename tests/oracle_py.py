@@ -43,6 +43,11 @@ def lib():
         L.orc_query_prep.argtypes = [vp, vp, sz, sz, vp, vp, vp, vp, vp]
         L.orc_scan_all.argtypes = [vp, vp, sz, sz, sz, vp, vp]
         L.orc_dist64.argtypes = [vp, vp, sz, sz, vp, sz, vp]
+        L.orc_estimate.restype = i32
+        L.orc_estimate.argtypes = [vp, vp, vp, vp, sz, vp]
+        L.orc_probe_bound_w.restype = i32
+        L.orc_probe_bound_w.argtypes = [vp, vp, vp, vp, vp, sz, vp]
+        L.orc_exhaustive_topk.argtypes = [vp, vp, sz, sz, vp, vp]
         _lib = L
     return _lib
 
@@ -183,6 +188,31 @@ class OracleIndex:
         lib().orc_scan_all(self.h, _p(q), nq, k, nprobe, _p(ids), _p(d))
         return ids, d
 
+    def estimate(self, queries, qidx, ids):
+        """Estimate of vector ids[i] for queries[qidx[i]] (KeyError: unknown id)."""
+        q = c32(queries)
+        qi, ii = c32(qidx, np.uint32), c32(ids, np.uint32)
+        out = np.empty(ii.size, np.float32)
+        if lib().orc_estimate(self.h, _p(q), _p(qi), _p(ii), ii.size, _p(out)) != 0:
+            raise KeyError("unknown id")
+        return out
+
+    def probe_bound_w(self, queries, base_rows, qidx, ids):
+        q, b = c32(queries), c32(base_rows)
+        qi, ii = c32(qidx, np.uint32), c32(ids, np.uint32)
+        out = np.empty(ii.size, np.float64)
+        if lib().orc_probe_bound_w(self.h, _p(q), _p(b), _p(qi), _p(ii), ii.size, _p(out)) != 0:
+            raise KeyError("unknown id")
+        return out
+
+    def exhaustive_topk(self, queries, k):
+        q = c32(queries)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        lib().orc_exhaustive_topk(self.h, _p(q), nq, k, _p(ids), _p(d))
+        return ids, d
+
     def dist64(self, queries, nprobe, ids):
         """fp64 estimator distances of the given ids (NaN: not in the probed lists)."""
         q = c32(queries)
@@ -213,6 +243,10 @@ def _m8_lib():
         L.orc_m8_search.argtypes = [vp, vp, sz, sz, sz, i32, vp, vp, vp]
         L.orc_m8_search_z.restype = i32
         L.orc_m8_search_z.argtypes = [vp, vp, sz, sz, sz, i32, f32, vp, vp, vp]
+        L.orc_m8_estimate.restype = i32
+        L.orc_m8_estimate.argtypes = [vp, vp, vp, vp, sz, vp]
+        L.orc_m8_exhaustive_topk.argtypes = [vp, vp, sz, sz, vp, vp]
+        L.orc_m8_search_ivf.argtypes = [vp, vp, sz, sz, sz, vp, vp, vp]
         L._m8_ready = True
     return L
 
@@ -293,4 +327,32 @@ class OracleMe8q:
             _m8_lib().orc_m8_search(self.h, _p(q), nq, k, nprobe, threads, _p(ids), _p(d), _p(st))
         else:
             _m8_lib().orc_m8_search_z(self.h, _p(q), nq, k, nprobe, threads, float(z), _p(ids), _p(d), _p(st))
+        return ids, d, st
+
+    def estimate(self, queries, qidx, ids):
+        """Estimate of vector ids[i] for queries[qidx[i]] (KeyError: unknown id)."""
+        q = c32(queries)
+        qi, ii = c32(qidx, np.uint32), c32(ids, np.uint32)
+        out = np.empty(ii.size, np.float32)
+        if _m8_lib().orc_m8_estimate(self.h, _p(q), _p(qi), _p(ii), ii.size, _p(out)) != 0:
+            raise KeyError("unknown id")
+        return out
+
+    def exhaustive_topk(self, queries, k):
+        q = c32(queries)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        _m8_lib().orc_m8_exhaustive_topk(self.h, _p(q), nq, k, _p(ids), _p(d))
+        return ids, d
+
+    def search_ivf(self, queries, k, nprobe):
+        """SearchIVF: ids, distances and per-query (candidates, clusters,
+        bits_scored, bits_total, survivors, pruned)."""
+        q = c32(queries)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        st = np.empty((nq, 6), np.uint64)
+        _m8_lib().orc_m8_search_ivf(self.h, _p(q), nq, k, nprobe, _p(ids), _p(d), _p(st))
         return ids, d, st
