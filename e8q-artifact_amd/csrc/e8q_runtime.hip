@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/e8q_gpu.h"
+#include "diag_kernels.h"
 #include "e8q_device.h"
 #include "e8q_kernels.h"
 
@@ -1181,6 +1182,141 @@ e8q_status e8q_debug_encode(e8q_ctx* ctx, const e8q_dindex* x, const float* unit
         Check(hipMemcpyAsync(digits, dd.p, nrows * v.padded * 2, hipMemcpyDeviceToHost, st), "dl");
         Check(hipMemcpyAsync(branch, db.p, nrows * v.blocks, hipMemcpyDeviceToHost, st), "dl");
         Check(hipStreamSynchronize(st), "encode");
+    });
+}
+
+// ---- diagnostics (e8q_oracle.cpp:14-219) ------------------------------
+
+namespace {
+
+uint64_t DiagChunk(uint64_t nq, uint64_t per_query) {
+    const uint64_t lim = uint64_t{1} << 28;   // candidates held at once (1 GiB of distances)
+    const uint64_t c = std::min<uint64_t>({nq, lim / std::max<uint64_t>(per_query, 1), 65535});   // grid y
+    return std::max<uint64_t>(1, c);
+}
+
+}  // namespace
+
+e8q_status e8q_estimate(e8q_ctx* ctx, const e8q_dindex* x, const float* queries, uint64_t nq, const uint32_t* qidx,
+                        const uint32_t* ids, uint64_t npair, float* out) {
+    return Guard([&] {
+        if (!ctx || !x) Throw(E8Q_EINVAL, "null argument");
+        if (npair && (!queries || !qidx || !ids || !out)) Throw(E8Q_EINVAL, "null buffer");
+        const IndexView& v = x->view;
+        if (v.n == 0) Throw(E8Q_ERUNTIME, "Estimate: empty index");
+        for (uint64_t i = 0; i < npair; ++i)
+            if (qidx[i] >= nq) Throw(E8Q_EINVAL, "Estimate: query index out of range");
+        if (npair == 0) return;
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        float *dq = nullptr, *rotq = nullptr, *dout = nullptr;
+        uint32_t *dqi = nullptr, *dids = nullptr, *inv = nullptr, *bad = nullptr;
+        e8q_ctx::Carve cv;
+        cv.add(&dq, nq * v.dim);
+        cv.add(&rotq, nq * v.padded);
+        cv.add(&dqi, npair);
+        cv.add(&dids, npair);
+        cv.add(&inv, v.n);
+        cv.add(&dout, npair);
+        cv.add(&bad, 1);
+        ctx->Take(cv);
+        Check(hipMemcpyAsync(dq, queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+        Check(hipMemcpyAsync(dqi, qidx, npair * 4, hipMemcpyHostToDevice, st), "upload pairs");
+        Check(hipMemcpyAsync(dids, ids, npair * 4, hipMemcpyHostToDevice, st), "upload pairs");
+        Check(LaunchFill32(st, inv, v.n, 0), "clear");
+        Check(LaunchFill32(st, bad, 1, 0), "clear");
+        Check(LaunchDiagInverseIds(st, v.ids, v.n, inv), "id map");
+        Check(LaunchRotate(st, dq, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
+        Check(LaunchE8Estimate(st, v, rotq, dqi, dids, npair, inv, dout, bad), "estimate");
+        uint32_t hbad = 0;
+        Check(hipMemcpyAsync(out, dout, npair * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipStreamSynchronize(st), "estimate");
+        if (hbad) Throw(E8Q_EINVAL, "Estimate: id out of range");
+    });
+}
+
+e8q_status e8q_probe_bound_w(e8q_ctx* ctx, const e8q_dindex* x, const float* queries, uint64_t nq,
+                             const float* base_rows, const uint32_t* qidx, const uint32_t* ids, uint64_t npair,
+                             double* out) {
+    return Guard([&] {
+        if (!ctx || !x) Throw(E8Q_EINVAL, "null argument");
+        if (npair && (!queries || !base_rows || !qidx || !ids || !out)) Throw(E8Q_EINVAL, "null buffer");
+        const IndexView& v = x->view;
+        if (v.n == 0) Throw(E8Q_ERUNTIME, "ProbeBoundW: empty index");
+        if (v.bits >= 10)
+            Throw(E8Q_EINVAL, "ProbeBoundW: b >= 10 is undefined in the reference (its 8-byte digit staging overflows)");
+        if (npair == 0) return;
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const uint64_t nqa = std::max<uint64_t>(nq, 1);
+        float *dq = nullptr, *rotq = nullptr, *db = nullptr, *rotx = nullptr;
+        double* dout = nullptr;
+        uint32_t *dqi = nullptr, *dids = nullptr, *inv = nullptr, *bad = nullptr;
+        e8q_ctx::Carve cv;
+        cv.add(&dq, nqa * v.dim);
+        cv.add(&rotq, nqa * v.padded);
+        cv.add(&db, npair * v.dim);
+        cv.add(&rotx, npair * v.padded);
+        cv.add(&dqi, npair);
+        cv.add(&dids, npair);
+        cv.add(&inv, v.n);
+        cv.add(&dout, npair);
+        cv.add(&bad, 1);
+        ctx->Take(cv);
+        if (nq) Check(hipMemcpyAsync(dq, queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+        Check(hipMemcpyAsync(db, base_rows, npair * v.dim * 4, hipMemcpyHostToDevice, st), "upload base rows");
+        Check(hipMemcpyAsync(dqi, qidx, npair * 4, hipMemcpyHostToDevice, st), "upload pairs");
+        Check(hipMemcpyAsync(dids, ids, npair * 4, hipMemcpyHostToDevice, st), "upload pairs");
+        Check(LaunchFill32(st, inv, v.n, 0), "clear");
+        Check(LaunchFill32(st, bad, 1, 0), "clear");
+        Check(LaunchDiagInverseIds(st, v.ids, v.n, inv), "id map");
+        Check(LaunchRotate(st, dq, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
+        Check(LaunchRotate(st, db, npair, static_cast<uint32_t>(v.dim), v, rotx, true), "rotate");
+        Check(LaunchE8ProbeBoundW(st, v, x->levels.as<double>(), rotq, rotx, dqi, dids, nq, npair, inv, dout, bad),
+              "probe bound");
+        uint32_t hbad = 0;
+        Check(hipMemcpyAsync(out, dout, npair * 8, hipMemcpyDeviceToHost, st), "download");
+        Check(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipStreamSynchronize(st), "probe bound");
+        if (hbad) Throw(E8Q_EINVAL, "ProbeBoundW: id out of range");
+    });
+}
+
+e8q_status e8q_exhaustive_topk(e8q_ctx* ctx, const e8q_dindex* x, const float* queries, uint64_t nq, uint64_t k,
+                               uint32_t* out_ids, float* out_dists) {
+    return Guard([&] {
+        if (!ctx || !x) Throw(E8Q_EINVAL, "null argument");
+        if (k == 0 || k > kDiagMaxK) Throw(E8Q_EINVAL, "k must be in [1, 1024]");
+        if (nq && (!queries || !out_ids || !out_dists)) Throw(E8Q_EINVAL, "null buffer");
+        if (nq == 0) return;
+        const IndexView& v = x->view;
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const uint64_t chunk = DiagChunk(nq, v.n);
+        float *dq = nullptr, *rotq = nullptr, *d2c = nullptr, *dist = nullptr, *od = nullptr;
+        uint32_t* oid = nullptr;
+        e8q_ctx::Carve cv;
+        cv.add(&dq, nq * v.dim);
+        cv.add(&rotq, nq * v.padded);
+        cv.add(&d2c, nq * v.nlist);
+        cv.add(&dist, std::max<uint64_t>(chunk * v.n, 1));
+        cv.add(&oid, nq * k);
+        cv.add(&od, nq * k);
+        ctx->Take(cv);
+        Check(hipMemcpyAsync(dq, queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+        Check(LaunchRotate(st, dq, nq, static_cast<uint32_t>(v.dim), v, rotq, true), "rotate");
+        Check(LaunchE8ListD2(st, v, rotq, nq, d2c), "list distances");
+        for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+            const uint64_t nqc = std::min(chunk, nq - q0);
+            Check(LaunchE8Exhaustive(st, v, rotq, d2c, q0, nqc, dist), "exhaustive distances");
+            Check(LaunchDiagSelect(st, false, dist, v.ids, 0, nullptr, v.n, v.n, q0, nqc, static_cast<uint32_t>(k),
+                                   oid, od),
+                  "top-k");
+        }
+        Check(hipMemcpyAsync(out_ids, oid, nq * k * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipMemcpyAsync(out_dists, od, nq * k * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipStreamSynchronize(st), "exhaustive top-k");
     });
 }
 

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/me8q_gpu.h"
+#include "diag_kernels.h"
 #include "e8q_kernels.h"
 #include "e8q_rt.h"
 #include "me8q_device.h"
@@ -1219,6 +1220,183 @@ e8q_status e8q_m8_debug_query(e8q_ctx* ctx, const e8q_m8index* x, const float* q
                         o[32 + e] = o[48 + e] = s[16 + e];
                     }
                 }
+        }
+    });
+}
+
+// ---- diagnostics (me8q_measure.cpp:41-67, me8q_search.cpp:136-266) --------
+
+namespace {
+
+uint64_t M8DiagChunk(uint64_t nq, uint64_t per_query) {
+    const uint64_t lim = uint64_t{1} << 28;   // candidates held at once
+    const uint64_t c = std::min<uint64_t>({nq, lim / std::max<uint64_t>(per_query, 1), 65535});   // grid y
+    return std::max<uint64_t>(1, c);
+}
+
+// Queries on the device and their per-run rotations (the query prep's other
+// outputs land in scratch).
+struct M8DiagQueries {
+    float *dq = nullptr, *rotq = nullptr, *qc = nullptr;
+    uint8_t* lut = nullptr;
+    double* rc = nullptr;
+    void Carve(e8q_ctx::Carve& cv, const M8View& v, uint64_t nq) {
+        cv.add(&dq, nq * v.dim);
+        cv.add(&rotq, nq * v.padded);
+        cv.add(&lut, nq * v.plane_cols * 32 + 16);
+        cv.add(&qc, nq * 4);
+        cv.add(&rc, nq * std::max<uint64_t>(v.nrun, 1) * 4);
+    }
+    void Prepare(hipStream_t st, const M8View& v, const float* queries, uint64_t nq) {
+        Check(hipMemcpyAsync(dq, queries, nq * v.dim * 4, hipMemcpyHostToDevice, st), "upload queries");
+        Check(LaunchM8QueryPrep(st, v, dq, nq, rotq, lut, qc, rc), "query prep");
+    }
+};
+
+}  // namespace
+
+e8q_status e8q_m8_estimate(e8q_ctx* ctx, const e8q_m8index* x, const float* queries, uint64_t nq,
+                           const uint32_t* qidx, const uint32_t* ids, uint64_t npair, float* out) {
+    return Guard([&] {
+        if (!ctx || !x) Throw(E8Q_EINVAL, "null argument");
+        if (npair && (!queries || !qidx || !ids || !out)) Throw(E8Q_EINVAL, "null buffer");
+        const M8View& v = x->view;
+        for (uint64_t i = 0; i < npair; ++i)
+            if (qidx[i] >= nq) Throw(E8Q_EINVAL, "Me8qIndex::Estimate: query index out of range");
+        if (npair == 0) return;
+        if (v.n == 0) Throw(E8Q_EINVAL, "Me8qIndex::Estimate: unknown id");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        M8DiagQueries dqs;
+        float* dout = nullptr;
+        uint32_t *dqi = nullptr, *dids = nullptr, *inv = nullptr, *bad = nullptr;
+        e8q_ctx::Carve cv;
+        dqs.Carve(cv, v, nq);
+        cv.add(&dqi, npair);
+        cv.add(&dids, npair);
+        cv.add(&inv, v.n);
+        cv.add(&dout, npair);
+        cv.add(&bad, 1);
+        ctx->Take(cv);
+        dqs.Prepare(st, v, queries, nq);
+        Check(hipMemcpyAsync(dqi, qidx, npair * 4, hipMemcpyHostToDevice, st), "upload pairs");
+        Check(hipMemcpyAsync(dids, ids, npair * 4, hipMemcpyHostToDevice, st), "upload pairs");
+        Check(LaunchFill32(st, inv, v.n, 0), "clear");
+        Check(LaunchFill32(st, bad, 1, 0), "clear");
+        Check(LaunchDiagInverseIds(st, v.ids, v.n, inv), "id map");
+        Check(LaunchM8Estimate(st, v, dqs.dq, dqs.rotq, dqi, dids, npair, inv, dout, bad), "estimate");
+        uint32_t hbad = 0;
+        Check(hipMemcpyAsync(out, dout, npair * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipStreamSynchronize(st), "estimate");
+        if (hbad) Throw(E8Q_EINVAL, "Me8qIndex::Estimate: unknown id");
+    });
+}
+
+e8q_status e8q_m8_exhaustive_topk(e8q_ctx* ctx, const e8q_m8index* x, const float* queries, uint64_t nq, uint64_t k,
+                                  uint32_t* out_ids, float* out_dists) {
+    return Guard([&] {
+        if (!ctx || !x) Throw(E8Q_EINVAL, "null argument");
+        if (k == 0 || k > kDiagMaxK) Throw(E8Q_EINVAL, "k must be in [1, 1024]");
+        if (nq && (!queries || !out_ids || !out_dists)) Throw(E8Q_EINVAL, "null buffer");
+        if (nq == 0) return;
+        const M8View& v = x->view;
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const uint64_t chunk = M8DiagChunk(nq, v.n);
+        M8DiagQueries dqs;
+        double* d2c = nullptr;
+        float *dist = nullptr, *od = nullptr;
+        uint32_t* oid = nullptr;
+        e8q_ctx::Carve cv;
+        dqs.Carve(cv, v, nq);
+        cv.add(&d2c, nq * v.nlist);
+        cv.add(&dist, std::max<uint64_t>(chunk * v.n, 1));
+        cv.add(&oid, nq * k);
+        cv.add(&od, nq * k);
+        ctx->Take(cv);
+        dqs.Prepare(st, v, queries, nq);
+        Check(LaunchM8ListD2(st, v, dqs.dq, nq, d2c), "list distances");
+        for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+            const uint64_t nqc = std::min(chunk, nq - q0);
+            Check(LaunchM8Exhaustive(st, v, dqs.rotq, d2c, q0, nqc, dist), "exhaustive distances");
+            Check(LaunchDiagSelect(st, true, dist, v.ids, 0, nullptr, v.n, v.n, q0, nqc, static_cast<uint32_t>(k), oid,
+                                   od),
+                  "top-k");
+        }
+        Check(hipMemcpyAsync(out_ids, oid, nq * k * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipMemcpyAsync(out_dists, od, nq * k * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipStreamSynchronize(st), "exhaustive top-k");
+    });
+}
+
+e8q_status e8q_m8_search_ivf(e8q_ctx* ctx, const e8q_m8index* x, const float* queries, uint64_t nq, uint64_t k,
+                             uint64_t nprobe, uint32_t* out_ids, float* out_dists, uint64_t* stats6) {
+    return Guard([&] {
+        if (!ctx || !x) Throw(E8Q_EINVAL, "null argument");
+        if (k > kDiagMaxK) Throw(E8Q_EINVAL, "k must be <= 1024");
+        const M8View& v = x->view;
+        if (nq == 0 || k == 0 || nprobe == 0 || v.n == 0) return;   // the reference returns untouched
+        if (!queries || !out_ids || !out_dists) Throw(E8Q_EINVAL, "null buffer");
+        ctx->Bind();
+        hipStream_t st = ctx->stream;
+        const uint32_t take = static_cast<uint32_t>(std::min<uint64_t>(nprobe, v.nlist));
+        const uint64_t cap = std::max<uint64_t>(x->top_lists[take], 1);   // candidates of one query at most
+        const uint64_t chunk = M8DiagChunk(nq, cap);
+        M8DiagQueries dqs;
+        float *dist = nullptr, *pd = nullptr, *cd = nullptr, *od = nullptr;
+        uint32_t *probe = nullptr, *cid = nullptr, *oid = nullptr;
+        uint64_t *off = nullptr, *count = nullptr;
+        e8q_ctx::Carve cv;
+        dqs.Carve(cv, v, nq);
+        cv.add(&dist, nq * v.nlist);
+        cv.add(&probe, nq * take);
+        cv.add(&pd, nq * take);
+        cv.add(&off, nq * (take + 1));
+        cv.add(&count, nq);
+        cv.add(&cd, chunk * cap);
+        cv.add(&cid, chunk * cap);
+        cv.add(&oid, nq * k);
+        cv.add(&od, nq * k);
+        ctx->Take(cv);
+        dqs.Prepare(st, v, queries, nq);
+        RouteArgs ra{};
+        ra.rotq = dqs.rotq;
+        ra.qstride = static_cast<uint32_t>(v.padded);
+        ra.qn2 = dqs.qc + 2;
+        ra.qn2_stride = 4;
+        ra.cent = v.cent_rot;
+        ra.cn2 = v.cn2;
+        ra.nlist = v.nlist;
+        ra.dim = v.route_dim;
+        ra.cstride = static_cast<uint32_t>(v.padded);
+        ra.nq = nq;
+        ra.dist = dist;
+        Check(LaunchRouteSelect(st, ra, nullptr, take, take, probe, pd), "route");
+        Check(LaunchM8IvfOffsets(st, v, probe, take, take, nq, off, count), "candidate offsets");
+        for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+            const uint64_t nqc = std::min(chunk, nq - q0);
+            Check(LaunchM8Ivf(st, v, dqs.rotq, probe, pd, take, take, off, q0, nqc, cap, cd, cid), "distances");
+            Check(LaunchDiagSelect(st, true, cd, cid, cap, count, 0, cap, q0, nqc, static_cast<uint32_t>(k), oid, od),
+                  "top-k");
+        }
+        std::vector<uint64_t> hc(nq);
+        Check(hipMemcpyAsync(out_ids, oid, nq * k * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipMemcpyAsync(out_dists, od, nq * k * 4, hipMemcpyDeviceToHost, st), "download");
+        Check(hipMemcpyAsync(hc.data(), count, nq * 8, hipMemcpyDeviceToHost, st), "download");
+        Check(hipStreamSynchronize(st), "search ivf");
+        if (stats6) {
+            uint64_t bits_row = 0;
+            for (const auto& r : x->runs) bits_row += static_cast<uint64_t>(r.rate) * r.len;
+            for (uint64_t i = 0; i < nq; ++i) {
+                uint64_t* o = stats6 + i * 6;
+                o[0] = hc[i];
+                o[1] = take;
+                o[2] = hc[i] * bits_row;
+                o[3] = hc[i] * bits_row;
+                o[4] = hc[i];
+                o[5] = 0;
+            }
         }
     });
 }

@@ -193,6 +193,34 @@ uint64_t e8q_search_workspace_bytes(const e8q_dindex* idx, uint64_t nq, uint64_t
 e8q_status e8q_search_static(e8q_ctx* ctx, const e8q_dindex* idx, const e8q_search_args* args,
                              e8q_search_stats* stats);
 
+/* ---- diagnostics (e8q_oracle.cpp:14-219) --------------------------------
+ * Host pointers.  Each value is the reference's bit for bit. */
+/* E8qIndex::Estimate (e8q_oracle.cpp:58-101; binding `estimate`,
+ * e8q_bindings.cpp:171-180) for npair (query, id) pairs: out[i] = the
+ * estimator distance of the stored vector with global id ids[i] from query
+ * queries[qidx[i]] (nq x dim).  An id no stored vector carries fails the call
+ * with E8Q_EINVAL ("Estimate: id out of range"), as does qidx[i] >= nq; an
+ * empty index fails with E8Q_ERUNTIME. */
+e8q_status e8q_estimate(e8q_ctx* ctx, const e8q_dindex* idx, const float* queries, uint64_t nq,
+                        const uint32_t* qidx, const uint32_t* ids, uint64_t npair, float* out);
+/* E8qIndex::ProbeBoundW (e8q_oracle.cpp:103-219) in the binding's batch form
+ * `probe_bound_w(queries, base_rows, qidx, ids)` (e8q_bindings.cpp:181-234):
+ * base_rows[i] (dim floats) is the build row of ids[i]; out[i] = (d_coarse -
+ * d_full) / (r_q Omega) in fp64, NaN for a pair with qidx[i] >= nq or
+ * ids[i] >= n and where the reference returns NaN (zero residual, degenerate
+ * code).  An id below n that no stored vector carries fails the call with
+ * E8Q_EINVAL; so does an index at b >= 10, where the reference's 8-byte digit
+ * staging overflows (undefined behaviour). */
+e8q_status e8q_probe_bound_w(e8q_ctx* ctx, const e8q_dindex* idx, const float* queries, uint64_t nq,
+                             const float* base_rows, const uint32_t* qidx, const uint32_t* ids, uint64_t npair,
+                             double* out);
+/* E8qIndex::ExhaustiveTopK (e8q_oracle.cpp:14-56) for each of nq queries:
+ * the k (1..1024) best (distance, id) over every stored vector, ordered by
+ * (distance, id) as its partial_sort under Better; rows [nq*k], UINT32_MAX /
+ * +inf padded. */
+e8q_status e8q_exhaustive_topk(e8q_ctx* ctx, const e8q_dindex* idx, const float* queries, uint64_t nq, uint64_t k,
+                               uint32_t* out_ids, float* out_dists);
+
 /* ---- diagnostics used by tests ------------------------------------------ */
 /* Coarse integer accumulate of one 32-lane batch (AccumulateCoarseHi,
  * e8q.hpp:333-336): rowa = blocks*32 bytes per batch, lut = blocks*64 bytes

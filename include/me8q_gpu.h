@@ -156,6 +156,30 @@ typedef struct {
 } e8q_m8_ctx_info;
 e8q_status e8q_ctx_get_m8_info(const e8q_ctx* ctx, e8q_m8_ctx_info* out);
 
+/* ---- diagnostics (me8q_measure.cpp:41-67, me8q_search.cpp:136-266) -----
+ * Host pointers.  Each value is the reference's bit for bit. */
+/* Me8qIndex::Estimate (me8q_measure.cpp:41-67; binding `estimate`,
+ * me8q_bindings.cpp:209-218) for npair (query, id) pairs: out[i] = the
+ * EstimateRow distance of the vector with id ids[i] from queries[qidx[i]]
+ * (nq x dim, the PCA domain).  An unknown id fails the call with E8Q_EINVAL
+ * ("Me8qIndex::Estimate: unknown id"), as does qidx[i] >= nq. */
+e8q_status e8q_m8_estimate(e8q_ctx* ctx, const e8q_m8index* idx, const float* queries, uint64_t nq,
+                           const uint32_t* qidx, const uint32_t* ids, uint64_t npair, float* out);
+/* Me8qIndex::ExhaustiveTopK (me8q_search.cpp:238-266; me8q.hpp:122-126) for
+ * each of nq queries: the list PushCandidate leaves after every stored
+ * vector in list order, k (1..1024) wide -- ascending distance, a later
+ * vector first among equal distances; rows [nq*k], UINT32_MAX / +inf padded. */
+e8q_status e8q_m8_exhaustive_topk(e8q_ctx* ctx, const e8q_m8index* idx, const float* queries, uint64_t nq,
+                                  uint64_t k, uint32_t* out_ids, float* out_dists);
+/* Me8qIndex::SearchIVF (me8q_search.cpp:136-236; me8q.hpp:85-96), the full
+ * estimate of every vector of the nprobe nearest lists, for each query:
+ * ids / dists [nq*k] as ExhaustiveTopK's order; stats6 (optional, [nq*6]) =
+ * candidates, clusters, bits_scored, bits_total, survivors, pruned (0).  With
+ * k == 0, nprobe == 0 or an empty index the outputs are left untouched, as
+ * in the reference; k <= 1024. */
+e8q_status e8q_m8_search_ivf(e8q_ctx* ctx, const e8q_m8index* idx, const float* queries, uint64_t nq, uint64_t k,
+                             uint64_t nprobe, uint32_t* out_ids, float* out_dists, uint64_t* stats6);
+
 /* ---- diagnostics used by tests (reference layouts, host pointers) -------- */
 /* The batched form with a checksum of its integer coarse sums: chk[q] =
  * sum over the (q, vector) pairs of q's probed lists of (pos + 1) *

@@ -232,6 +232,30 @@ class DeviceIndex:
     def workspace_bytes(self, nq, k, max_nprobe):
         return int(lib().e8q_search_workspace_bytes(self.h, nq, k, max_nprobe))
 
+    def estimate(self, queries, qidx, ids):
+        q = np.ascontiguousarray(queries, np.float32)
+        qi, ii = np.ascontiguousarray(qidx, np.uint32), np.ascontiguousarray(ids, np.uint32)
+        out = np.empty(ii.size, np.float32)
+        check(lib().e8q_estimate(self.ctx.h, self.h, p(q), u64(q.shape[0]), p(qi), p(ii), u64(ii.size), p(out)))
+        return out
+
+    def probe_bound_w(self, queries, base_rows, qidx, ids):
+        q = np.ascontiguousarray(queries, np.float32)
+        b = np.ascontiguousarray(base_rows, np.float32)
+        qi, ii = np.ascontiguousarray(qidx, np.uint32), np.ascontiguousarray(ids, np.uint32)
+        out = np.empty(ii.size, np.float64)
+        check(lib().e8q_probe_bound_w(self.ctx.h, self.h, p(q), u64(q.shape[0]), p(b), p(qi), p(ii), u64(ii.size),
+                                      p(out)))
+        return out
+
+    def exhaustive_topk(self, queries, k):
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        check(lib().e8q_exhaustive_topk(self.ctx.h, self.h, p(q), u64(nq), u64(k), p(ids), p(d)))
+        return ids, d
+
     def query_prep(self, queries, nprobe):
         q = np.ascontiguousarray(queries, np.float32)
         nq = q.shape[0]

@@ -149,6 +149,30 @@ class M8Index:
         check(lib().e8q_m8_search(self.ctx.h, self.h, ctypes.byref(s), ctypes.byref(tot), p(st)))
         return ids, d, st, tot
 
+    def estimate(self, queries, qidx, ids):
+        q = np.ascontiguousarray(queries, np.float32)
+        qi, ii = np.ascontiguousarray(qidx, np.uint32), np.ascontiguousarray(ids, np.uint32)
+        out = np.empty(ii.size, np.float32)
+        check(lib().e8q_m8_estimate(self.ctx.h, self.h, p(q), u64(q.shape[0]), p(qi), p(ii), u64(ii.size), p(out)))
+        return out
+
+    def exhaustive_topk(self, queries, k):
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        check(lib().e8q_m8_exhaustive_topk(self.ctx.h, self.h, p(q), u64(nq), u64(k), p(ids), p(d)))
+        return ids, d
+
+    def search_ivf(self, queries, k, nprobe):
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        st = np.empty((nq, 6), np.uint64)
+        check(lib().e8q_m8_search_ivf(self.ctx.h, self.h, p(q), u64(nq), u64(k), u64(nprobe), p(ids), p(d), p(st)))
+        return ids, d, st
+
     def batch_checksum(self, queries, k, nprobe, z, nprobe_per_query=None, budget=0):
         """Per-query checksum of the batched form's integer coarse sums."""
         q = np.ascontiguousarray(queries, np.float32)
