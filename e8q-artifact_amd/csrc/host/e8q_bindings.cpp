@@ -182,6 +182,41 @@ class PyE8qIndex {
         index_->SetDisablePrune(on);
     }
 
+    // The reference binding's estimate (e8q_bindings.cpp:171-180).
+    float estimate(py::handle query, uint32_t id) const {
+        if (!index_) throw std::runtime_error("E8qIndex must be built or loaded before estimate");
+        auto q = Ensure<float>(query, "query", 1);
+        if (static_cast<size_t>(q.shape(0)) != dim_) throw std::invalid_argument("query dimension does not match index dim");
+        py::gil_scoped_release nogil;
+        return index_->Estimate(q.data(), id);
+    }
+
+    // The reference binding's batched ZPR bound probe (e8q_bindings.cpp:
+    // 181-234): pair i probes base_rows[i] (the build row of ids[i]) with
+    // queries[qidx[i]]; NaN for a pair out of range.  One device call for all
+    // pairs (num_threads is accepted for the reference's signature).
+    py::array_t<double> probe_bound_w(py::handle queries, py::handle base_rows, py::handle qidx, py::handle ids,
+                                      size_t num_threads) const {
+        (void)num_threads;
+        if (!index_) throw std::runtime_error("E8qIndex must be built or loaded before probe_bound_w");
+        auto q = Ensure<float>(queries, "queries", 2);
+        auto b = Ensure<float>(base_rows, "base_rows", 2);
+        auto qi = Ensure<uint32_t>(qidx, "qidx", 1);
+        auto ii = Ensure<uint32_t>(ids, "ids", 1);
+        if (static_cast<size_t>(q.shape(1)) != dim_) throw std::invalid_argument("queries dimension does not match index dim");
+        if (static_cast<size_t>(b.shape(1)) != dim_)
+            throw std::invalid_argument("base_rows dimension does not match index dim");
+        const size_t m = static_cast<size_t>(ii.shape(0));
+        if (static_cast<size_t>(qi.shape(0)) != m || static_cast<size_t>(b.shape(0)) != m)
+            throw std::invalid_argument("qidx, ids and base_rows must have the same length");
+        py::array_t<double> out(static_cast<py::ssize_t>(m));
+        if (m == 0) return out;
+        py::gil_scoped_release nogil;
+        index_->ProbeBoundWBatch(q.data(), static_cast<size_t>(q.shape(0)), b.data(), qi.data(), ii.data(), m,
+                                 out.mutable_data());
+        return out;
+    }
+
     py::list exhaustive_topk(py::handle query, size_t k) const {
         if (!index_) throw std::runtime_error("E8qIndex must be built or loaded before search");
         auto q = Ensure<float>(query, "query", 1);
@@ -247,6 +282,9 @@ PYBIND11_MODULE(_e8qlib, m) {
              py::arg("nprobe"), py::arg("out_ids_ptr"), py::arg("out_dists_ptr"), py::arg("mode") = "plain",
              py::arg("nprobe_per_query") = py::none())
         .def("exhaustive_topk", &PyE8qIndex::exhaustive_topk, py::arg("query"), py::arg("k"))
+        .def("estimate", &PyE8qIndex::estimate, py::arg("query"), py::arg("id"))
+        .def("probe_bound_w", &PyE8qIndex::probe_bound_w, py::arg("queries"), py::arg("base"), py::arg("qidx"),
+             py::arg("ids"), py::arg("num_threads") = 1)
         .def("set_disable_prune", &PyE8qIndex::set_disable_prune, py::arg("on"))
         .def("set_prune_alpha", &PyE8qIndex::set_prune_alpha, py::arg("alpha"))
         .def("save", &PyE8qIndex::save, py::arg("path"))

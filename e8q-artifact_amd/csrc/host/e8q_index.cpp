@@ -430,26 +430,44 @@ void E8qIndex::Search(const float* query, size_t k, size_t nprobe, uint32_t* out
 
 void E8qIndex::ExhaustiveTopK(const float* query, size_t k, std::vector<std::pair<float, uint32_t>>& out) const {
     if (dev_ == nullptr) throw std::runtime_error("E8qIndex must be built or loaded before search");
-    const size_t kk = std::min(k, kMaxK);
+    out.clear();
+    const size_t kk = std::min(k, n_);
+    if (kk == 0) return;
+    if (kk > 1024) throw std::invalid_argument("ExhaustiveTopK: k above the device limit of 1024");
     std::vector<uint32_t> ids(kk);
     std::vector<float> d(kk);
-    e8q_search_args a{};
-    a.queries = query;
-    a.nq = 1;
-    a.k = kk;
-    a.nprobe = nlist_;
-    a.z = 1.0F;
-    a.disable_prune = 1;   // every lane refined: the estimator over all vectors
-    a.flush_every = 32;
-    a.mode = E8Q_MODE_PLAIN;
-    a.out_ids = ids.data();
-    a.out_dists = d.data();
     {
         std::lock_guard<std::mutex> lk(DeviceMutex());
-        Check(e8q_search(ctx_, dev_, &a, nullptr));
+        Check(e8q_exhaustive_topk(ctx_, dev_, query, 1, kk, ids.data(), d.data()));
     }
-    out.clear();
     for (size_t i = 0; i < kk && ids[i] != std::numeric_limits<uint32_t>::max(); ++i) out.emplace_back(d[i], ids[i]);
+}
+
+float E8qIndex::Estimate(const float* query, uint32_t id) const {
+    if (dev_ == nullptr || n_ == 0) throw std::runtime_error("Estimate: empty index");
+    float out = 0.0F;
+    const uint32_t q0 = 0;
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    const e8q_status st = e8q_estimate(ctx_, dev_, query, 1, &q0, &id, 1, &out);
+    if (st == E8Q_EINVAL) throw std::invalid_argument(e8q_last_error());
+    Check(st);
+    return out;
+}
+
+double E8qIndex::ProbeBoundW(const float* query, const float* base_row, uint32_t id) const {
+    double out = 0.0;
+    const uint32_t q0 = 0;
+    ProbeBoundWBatch(query, 1, base_row, &q0, &id, 1, &out);
+    return out;
+}
+
+void E8qIndex::ProbeBoundWBatch(const float* queries, size_t nq, const float* base_rows, const uint32_t* qidx,
+                                const uint32_t* ids, size_t npair, double* out) const {
+    if (dev_ == nullptr || n_ == 0) throw std::runtime_error("ProbeBoundW: empty index");
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    const e8q_status st = e8q_probe_bound_w(ctx_, dev_, queries, nq, base_rows, qidx, ids, npair, out);
+    if (st == E8Q_EINVAL) throw std::invalid_argument(e8q_last_error());
+    Check(st);
 }
 
 }  // namespace e8qlib

@@ -85,8 +85,19 @@ class E8qIndex {
                          uint32_t* out_ids, float* out_dists) const;
     // Device scratch such a search carves from its context's workspace.
     [[nodiscard]] uint64_t SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const;
-    // Top-k of the refine estimator over every stored vector (no pruning).
+    // Top-k of the refine estimator over every stored vector (no pruning),
+    // ordered by (distance, id) (e8q_oracle.cpp:14-56).
     void ExhaustiveTopK(const float* query, size_t k, std::vector<std::pair<float, uint32_t>>& out) const;
+    // The estimator distance of stored vector `id` (e8q_oracle.cpp:58-101);
+    // std::invalid_argument for an id no vector carries.
+    [[nodiscard]] float Estimate(const float* query, uint32_t id) const;
+    // (d_coarse - d_full) / (r_q Omega) of stored vector `id` from its build
+    // row (e8q_oracle.cpp:103-219).
+    [[nodiscard]] double ProbeBoundW(const float* query, const float* base_row, uint32_t id) const;
+    // The binding's batch form: pair i = (queries[qidx[i]], base_rows[i],
+    // ids[i]); NaN for qidx >= nq or id >= n (e8q_bindings.cpp:181-234).
+    void ProbeBoundWBatch(const float* queries, size_t nq, const float* base_rows, const uint32_t* qidx,
+                          const uint32_t* ids, size_t npair, double* out) const;
 
     void SetDisablePrune(bool on) { disable_prune_ = on; }
     [[nodiscard]] bool disable_prune() const { return disable_prune_; }

@@ -228,6 +228,63 @@ class PyMe8qIndex {
         return s;
     }
 
+    // The reference binding's estimate (me8q_bindings.cpp:209-218).
+    float estimate(py::handle query, uint32_t id) const {
+        if (!built_ || !index_) throw std::runtime_error("Me8qIndex must be built or loaded before estimate");
+        auto q = EnsureArr<float>(query, "query", 1);
+        if (static_cast<size_t>(q.shape(0)) != dim_)
+            throw std::invalid_argument("query dimension does not match index dim");
+        py::gil_scoped_release nogil;
+        return index_->Estimate(q.data(), id);
+    }
+
+    // Me8qIndex::ExhaustiveTopK (me8q_search.cpp:238-266) of each query row,
+    // k wide, UINT32_MAX / +inf padded.
+    py::tuple exhaustive_topk(py::handle queries, size_t k) const {
+        if (!built_ || !index_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
+        auto q = EnsureArr<float>(queries, "queries", 2);
+        if (static_cast<size_t>(q.shape(1)) != dim_)
+            throw std::invalid_argument("queries dimension does not match index dim");
+        const size_t nq = static_cast<size_t>(q.shape(0));
+        py::array_t<uint32_t> ids({nq, k});
+        py::array_t<float> dists({nq, k});
+        if (nq > 0 && k > 0) {
+            py::gil_scoped_release nogil;
+            index_->ExhaustiveTopKBatch(q.data(), nq, k, ids.mutable_data(), dists.mutable_data());
+        }
+        return py::make_tuple(ids, dists);
+    }
+
+    // Me8qIndex::SearchIVF (me8q_search.cpp:136-236) of each query row.
+    py::tuple search_ivf(py::handle queries, size_t k, size_t nprobe, bool return_stats) const {
+        if (!built_ || !index_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
+        auto q = EnsureArr<float>(queries, "queries", 2);
+        if (static_cast<size_t>(q.shape(1)) != dim_)
+            throw std::invalid_argument("queries dimension does not match index dim");
+        const size_t nq = static_cast<size_t>(q.shape(0));
+        py::array_t<uint32_t> ids({nq, k});
+        py::array_t<float> dists({nq, k});
+        std::fill(ids.mutable_data(), ids.mutable_data() + nq * k, std::numeric_limits<uint32_t>::max());
+        std::fill(dists.mutable_data(), dists.mutable_data() + nq * k, std::numeric_limits<float>::infinity());
+        std::vector<uint64_t> st(nq * 6, 0);
+        {
+            py::gil_scoped_release nogil;
+            index_->SearchIVFBatch(q.data(), nq, k, nprobe, ids.mutable_data(), dists.mutable_data(), st.data());
+        }
+        if (!return_stats) return py::make_tuple(ids, dists);
+        py::dict s;
+        uint64_t tot[6] = {0, 0, 0, 0, 0, 0};
+        for (size_t i = 0; i < nq; ++i)
+            for (int f = 0; f < 6; ++f) tot[f] += st[i * 6 + f];
+        s["candidates"] = tot[0];
+        s["clusters"] = tot[1];
+        s["bits_scored"] = tot[2];
+        s["bits_total"] = tot[3];
+        s["survivors"] = tot[4];
+        s["pruned"] = tot[5];
+        return py::make_tuple(ids, dists, s);
+    }
+
     void save(const std::string& path) const {
         if (!built_ || !index_) throw std::runtime_error("Me8qIndex must be built or loaded before save");
         index_->Save(path.c_str());
@@ -287,6 +344,10 @@ void RegisterMe8q(py::module_& m) {
         .def("search_device", &PyMe8qIndex::search_device, py::arg("queries_ptr"), py::arg("nq"), py::arg("k"),
              py::arg("nprobe"), py::arg("out_ids_ptr"), py::arg("out_dists_ptr"),
              py::arg("nprobe_per_query") = py::none(), py::arg("mode") = "plain")
+        .def("estimate", &PyMe8qIndex::estimate, py::arg("query"), py::arg("id"))
+        .def("exhaustive_topk", &PyMe8qIndex::exhaustive_topk, py::arg("queries"), py::arg("k"))
+        .def("search_ivf", &PyMe8qIndex::search_ivf, py::arg("queries"), py::arg("k"), py::arg("nprobe"),
+             py::arg("return_stats") = false)
         .def("save", &PyMe8qIndex::save, py::arg("path"))
         .def_static("load", &PyMe8qIndex::load, py::arg("path"))
         .def_property_readonly("dim", &PyMe8qIndex::dim)

@@ -558,6 +558,63 @@ uint64_t Me8qIndex::SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe,
     return dev_ ? e8q_m8_search_workspace_bytes(dev_, nq, k, max_nprobe, mode) : 0;
 }
 
+void Me8qIndex::SearchIVFBatch(const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* out_ids,
+                               float* out_dists, uint64_t* stats6) const {
+    if (!dev_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    Check(e8q_m8_search_ivf(ctx_, dev_, queries, nq, k, nprobe, out_ids, out_dists, stats6));
+}
+
+void Me8qIndex::SearchIVF(const float* query, size_t k, size_t nprobe, uint32_t* out_ids, SearchStats* stats,
+                          float* out_dists) const {
+    if (k == 0 || nprobe == 0 || n_ == 0) return;
+    std::vector<float> d(k);
+    uint64_t st[6] = {0, 0, 0, 0, 0, 0};
+    SearchIVFBatch(query, 1, k, nprobe, out_ids, d.data(), st);
+    if (out_dists) std::copy(d.begin(), d.end(), out_dists);
+    if (stats) {
+        stats->candidates = st[0];
+        stats->clusters = st[1];
+        stats->bits_scored = st[2];
+        stats->bits_total = st[3];
+        stats->survivors = st[4];
+        stats->pruned = st[5];
+    }
+}
+
+void Me8qIndex::ExhaustiveTopKBatch(const float* queries, size_t nq, size_t k, uint32_t* out_ids,
+                                    float* out_dists) const {
+    if (!dev_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    Check(e8q_m8_exhaustive_topk(ctx_, dev_, queries, nq, k, out_ids, out_dists));
+}
+
+void Me8qIndex::ExhaustiveTopK(const float* query, size_t k, std::vector<std::pair<float, uint32_t>>* out) const {
+    out->clear();
+    const size_t kk = std::min(k, n_);
+    if (kk == 0) return;
+    std::vector<uint32_t> ids(kk);
+    std::vector<float> d(kk);
+    ExhaustiveTopKBatch(query, 1, kk, ids.data(), d.data());
+    for (size_t i = 0; i < kk && ids[i] != std::numeric_limits<uint32_t>::max(); ++i) out->emplace_back(d[i], ids[i]);
+}
+
+void Me8qIndex::EstimateBatch(const float* queries, size_t nq, const uint32_t* qidx, const uint32_t* ids,
+                              size_t npair, float* out) const {
+    if (!dev_) throw std::runtime_error("Me8qIndex must be built or loaded before estimate");
+    std::lock_guard<std::mutex> lk(DeviceMutex());
+    const e8q_status st = e8q_m8_estimate(ctx_, dev_, queries, nq, qidx, ids, npair, out);
+    if (st == E8Q_EINVAL) throw std::out_of_range(e8q_last_error());
+    Check(st);
+}
+
+float Me8qIndex::Estimate(const float* query, uint32_t id) const {
+    float out = 0.0F;
+    const uint32_t q0 = 0;
+    EstimateBatch(query, 1, &q0, &id, 1, &out);
+    return out;
+}
+
 void Me8qIndex::SearchIVFFastBatch(const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* out_ids,
                                    SearchStats* stats, float* out_dists, int /*num_threads*/) const {
     if (nq == 0) return;

@@ -128,6 +128,23 @@ class Me8qIndex {
                          uint32_t* out_ids, float* out_dists, int mode = 0) const;
     [[nodiscard]] uint64_t SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe, int mode = 0) const;
 
+    // The full-estimate paths (me8q.hpp:85-139): SearchIVF scores every vector
+    // of the nprobe nearest lists (me8q_search.cpp:136-236), ExhaustiveTopK
+    // every stored vector (:238-266), both keeping PushCandidate's list;
+    // Estimate one stored vector (me8q_measure.cpp:41-67; std::out_of_range
+    // for an unknown id).  k <= 1024 on the device.
+    void SearchIVF(const float* query, size_t k, size_t nprobe, uint32_t* out_ids, SearchStats* stats = nullptr,
+                   float* out_dists = nullptr) const;
+    void ExhaustiveTopK(const float* query, size_t k, std::vector<std::pair<float, uint32_t>>* out) const;
+    [[nodiscard]] float Estimate(const float* query, uint32_t id) const;
+    // Batch forms of the three (nq queries in one device call; stats6 per
+    // query: candidates, clusters, bits_scored, bits_total, survivors, pruned).
+    void SearchIVFBatch(const float* queries, size_t nq, size_t k, size_t nprobe, uint32_t* out_ids,
+                        float* out_dists, uint64_t* stats6) const;
+    void ExhaustiveTopKBatch(const float* queries, size_t nq, size_t k, uint32_t* out_ids, float* out_dists) const;
+    void EstimateBatch(const float* queries, size_t nq, const uint32_t* qidx, const uint32_t* ids, size_t npair,
+                       float* out) const;
+
     [[nodiscard]] size_t n() const { return n_; }
     [[nodiscard]] size_t dim() const { return dim_; }
     [[nodiscard]] size_t padded_dim() const { return padded_; }

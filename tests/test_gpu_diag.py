@@ -168,3 +168,40 @@ def test_me8q_diagnostics_against_oracle_at_size(ctx):
         np.testing.assert_array_equal(_bits(d), _bits(od))
         np.testing.assert_array_equal(st, ost)
     ix.close()
+
+
+def test_module_diagnostics_against_reference(gold):
+    # e8qlib's E8qIndex.estimate / probe_bound_w / exhaustive_topk and
+    # Me8qIndex.estimate / exhaustive_topk / search_ivf on the reference's
+    # own files: the drop-in Python surface (e8q_bindings.cpp:323-332,
+    # me8q_bindings.cpp:304)
+    import e8qlib
+    name = "d128b2"
+    data, _c, _l, queries = cases.mixture(name)
+    idx = e8qlib.E8qIndex.load(os.path.join(G, f"{name}.e8ax"))
+    pq, pid = cases.diag_pairs(name)
+    est = np.array([idx.estimate(queries[q], int(i)) for q, i in zip(pq, pid)], np.float32)
+    np.testing.assert_array_equal(_bits(est), _bits(gold[f"{name}_est"]))
+    _same_f(idx.probe_bound_w(queries, data[pid], pq, pid, num_threads=4), gold[f"{name}_pbw"])
+    ex = idx.exhaustive_topk(queries[3], 10)
+    assert [e[1] for e in ex] == gold[f"{name}_k10_exids"][3].tolist()
+    assert [np.float32(e[0]) for e in ex] == gold[f"{name}_k10_exd"][3].tolist()
+    with pytest.raises(ValueError):
+        idx.estimate(queries[0], 99999)
+    name = "m256sp"
+    _d, _c, _l, mq, _lam = cases.me8q_mixture(name)
+    m = e8qlib.Me8qIndex.load(os.path.join(G, f"{name}.me8q"))
+    pq, pid = cases.diag_pairs(name)
+    est = np.array([m.estimate(mq[q], int(i)) for q, i in zip(pq, pid)], np.float32)
+    np.testing.assert_array_equal(_bits(est), _bits(gold[f"{name}_est"]))
+    ids, d = m.exhaustive_topk(mq, 100)
+    np.testing.assert_array_equal(ids, gold[f"{name}_k100_exids"])
+    np.testing.assert_array_equal(_bits(d), _bits(gold[f"{name}_k100_exd"]))
+    nlist = cases.ME8Q_CASES[name][2]
+    ids, d, st = m.search_ivf(mq, 10, 3, return_stats=True)
+    np.testing.assert_array_equal(ids, gold[f"{name}_k10_p3_ivfids"])
+    np.testing.assert_array_equal(_bits(d), _bits(gold[f"{name}_k10_p3_ivfd"]))
+    assert st["candidates"] == int(gold[f"{name}_k10_p3_ivfstats"][:, 0].sum())
+    with pytest.raises(IndexError):
+        m.estimate(mq[0], 5000)
+    assert nlist > 3
