@@ -484,7 +484,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     // events around the scan and the refine launches, and the work list's
     // parameter block uploaded on the stream (a capture records kernels
     // only: the block is uploaded before the graph launch).
-    auto device_seq = [&](const Grids& g, bool eager, bool timed) {
+    auto device_seq = [&](const Grids& g, bool eager, bool timed, bool fallback) {
         if (eager) Check(hipEventRecord(ctx->ev[0], st), "event");
         Check(LaunchM8QueryPrep(st, v, q, nq, rotq, lut, qc, rc), "query prep");
         Check(LaunchRouteSelect(st, ra, npq, nprobe_all, maxp, probe, pd), "route");
@@ -584,12 +584,22 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         refine(rB, b.tsurv + 2 * tsn, b.tsurv + 3 * tsn, out_ids, out_d);
         // queries whose candidates passed the pool's end: the plain form
         // (their candidates and batches are counted already)
+        if (!fallback) return;
         M8ScanArgs oa = sa;
         oa.qsel = b.ovf;
         oa.nsel = nq;
         oa.nsel_dev = b.counters + 2;
         oa.qstats = nullptr;
         Check(LaunchM8Scan(st, oa), "plain search of overflowed queries");
+    };
+    // An unreserved context whose pool can still grow answers an overflow
+    // by learning the need and running the whole call again (outside a
+    // graph, where the host decides after the sequence); the plain
+    // fallback is then left out of the first run.
+    const uint64_t pool_ceiling = std::min(kM8PoolMax, nq * x->top_lists[std::min<uint64_t>(maxp, v.nlist)]);
+    const bool can_grow = batched && !ctx->reserved && chk == nullptr && caps.pool < pool_ceiling;
+    auto learn = [&](uint64_t total) {
+        ctx->m8_pool_per_q = std::max<uint64_t>(ctx->m8_pool_per_q, (total + total / 4) / nq + 1);
     };
 
     // Graph replay: the key is everything the captured launches bake in.
@@ -634,7 +644,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         if (!ge->exec) {
             Check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
             try {
-                device_seq(grids, false, false);
+                device_seq(grids, false, false, true);
             } catch (...) {
                 hipGraph_t gr = nullptr;
                 (void)hipStreamEndCapture(st, &gr);
@@ -657,6 +667,17 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         ++ctx->replays;
         replayed = true;
     } else {
+        if (tune && can_grow) {   // size the pool before timing anything
+            device_seq(grids, true, false, false);
+            unsigned long long t0 = 0;
+            Check(hipMemcpyAsync(&t0, b.poff + nq, 8, hipMemcpyDeviceToHost, st), "download pool size");
+            Check(hipStreamSynchronize(st), "sizing run");
+            if (t0 > caps.pool) {
+                learn(t0);
+                M8Search(ctx, x, a, stats, per_query_stats, chk);
+                return;
+            }
+        }
         if (tune) {
             // Times the whole sequence at each candidate grid (the MFMA
             // contraction's at 1, 2 and 4 per CU, then the refine's at 4, 8
@@ -664,7 +685,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             // results.
             auto time_grids = [&](const Grids& g) {
                 Check(hipEventRecord(ctx->kev[0], st), "event");
-                device_seq(g, true, false);
+                device_seq(g, true, false, !can_grow);
                 Check(hipEventRecord(ctx->kev[1], st), "event");
                 Check(hipEventSynchronize(ctx->kev[1]), "tuning");
                 float ms = 0.0F;
@@ -695,7 +716,7 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             ++ctx->m8_tunings;
             grids = best;
         }
-        device_seq(grids, true, stats != nullptr);
+        device_seq(grids, true, stats != nullptr, !can_grow);
     }
     Check(hipEventRecord(ctx->ev[3], st), "event");
     if (batched) {
@@ -730,9 +751,13 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
             Throw(E8Q_ERUNTIME, "batched search: records and candidate counts differ");
         novf = cnt[2];
         // a call whose candidates passed the pool's end raises the next
-        // call's pool (a reserved context keeps its size)
-        if (total > caps.pool && !ctx->reserved)
-            ctx->m8_pool_per_q = std::max<uint64_t>(ctx->m8_pool_per_q, (total + total / 4) / nq + 1);
+        // call's pool (a reserved context keeps its size); run without the
+        // fallback, it runs again at that size
+        if (total > caps.pool && !ctx->reserved) learn(total);
+        if (total > caps.pool && can_grow && !replayed) {
+            M8Search(ctx, x, a, stats, per_query_stats, chk);
+            return;
+        }
     }
     if (per_query_stats) {
         if (batched) std::fill(per_query_stats, per_query_stats + nq * 5, 0ull);

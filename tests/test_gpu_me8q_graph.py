@@ -177,7 +177,8 @@ def unpruned(ctx, case):
 def half_pool(case):
     """Pool entries per query that hold about half of the case's candidates
     at z = +inf: a fresh context with a one-entry-per-query pool learns 1.25 x
-    the candidates per query (plus one) from its first call."""
+    the candidates per query (plus one) from its first call (which then runs
+    again at that size: nothing overflows in what it returns)."""
     _ix, f, qs, _ref = case
     mp = pytest.MonkeyPatch()
     mp.setenv("E8Q_M8_POOL_PER_QUERY", "1")
@@ -185,7 +186,7 @@ def half_pool(case):
     try:
         ix = capi_m8.M8Index.from_file(c, f)
         r = ix.search(qs, K, NPROBE, ZO, mode=BATCHED)
-        assert r[3].overflowed > 0
+        assert r[3].overflowed == 0
         learned = c.m8_info()["pool_per_query"]
         ix.close()
     finally:
@@ -201,14 +202,15 @@ def test_pool_overflow_falls_back_to_plain(ctx, case, unpruned, half_pool, monke
     _ix, f, qs, _ref = case
     ref, plain = unpruned
     monkeypatch.setenv("E8Q_M8_POOL_PER_QUERY", str(half_pool))
-    c = capi.Ctx(0)   # a context that has learned nothing yet
+    c = capi.Ctx(0)   # an unreserved context that has learned nothing yet
     try:
         ix2 = capi_m8.M8Index.from_file(c, f)
-        print(f"me8q batched pool of {half_pool} per query", flush=True)
+        print(f"me8q batched pool of {half_pool} per query, unreserved", flush=True)
+        # the call overflows, learns the need and runs again at that size:
+        # the batched results for every query
         r = ix2.search(qs, K, NPROBE, ZO, mode=BATCHED)
-        novf = _check_overflow(r, ref, plain)
-        print(f"  {novf} of {qs.shape[0]} queries overflowed", flush=True)
-        # the context raises its pool for the next call: nothing overflows
+        assert r[3].overflowed == 0
+        _same(r, ref)
         assert c.m8_info()["pool_per_query"] > half_pool
         r2 = ix2.search(qs, K, NPROBE, ZO, mode=BATCHED)
         assert r2[3].overflowed == 0
@@ -216,6 +218,20 @@ def test_pool_overflow_falls_back_to_plain(ctx, case, unpruned, half_pool, monke
         ix2.close()
     finally:
         c.close()
+    # a reserved context keeps its pool: the queries past it are answered by
+    # the plain form inside the same sequence, eager and in a replayed graph
+    # (it learns nothing)
+    c4 = capi.Ctx(0)
+    try:
+        ix4 = capi_m8.M8Index.from_file(c4, f)
+        c4.reserve(ix4.workspace_bytes(qs.shape[0], K, NPROBE, BATCHED))
+        print("me8q batched overflow on a reserved context, eager", flush=True)
+        novf = _check_overflow(ix4.search(qs, K, NPROBE, ZO, mode=BATCHED), ref, plain)
+        print(f"  {novf} of {qs.shape[0]} queries overflowed", flush=True)
+        assert c4.m8_info()["pool_per_query"] == 0
+        ix4.close()
+    finally:
+        c4.close()
     # the same overflow inside a replayed graph, on a reserved context (which
     # learns nothing)
     c3 = capi.Ctx(0)
