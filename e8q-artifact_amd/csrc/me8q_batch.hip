@@ -527,7 +527,9 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                         if (!((mk >> n) & 1u)) continue;
                         const unsigned long long at =
                             rb + s_rs[row] + s_off[wave][t][row][1] + __popc(mk & ((1u << n) - 1u));
-                        if (at < a.rec_cap) a.rec[at] = make_uint2(bj * 32 + n, s_rw[kWrite ? 0 : wave][t][i][lane]);
+                        if (at < a.rec_cap)
+                            a.rec[at] = make_uint2(static_cast<uint32_t>((bat0 + bj) * 32 + n),
+                                                   s_rw[kWrite ? 0 : wave][t][i][lane]);
                     }
                 }
             }
@@ -557,7 +559,7 @@ __global__ __launch_bounds__(256, 2) void M8MfmaScanKernel(M8BatchArgs a) {
                                                          __popc(mk & ~ma & below);
                         if (in_a ? slot < s_na[row] : slot < G(a.ncand)[q]) {   // the count pass saw the same vectors pass
                             const uint64_t at = G(a.coff)[q] + slot;
-                            Gw(a.pool)[at] = (static_cast<uint64_t>(s_p[row]) << 32) | pos;
+                            Gw(a.pool)[at] = (static_cast<uint64_t>(s_p[row]) << 32) | ((bat0 + bj) * 32 + n);
                             Gw(a.pdlo)[at] = dm;
                         }
                     }
@@ -708,7 +710,8 @@ __global__ __launch_bounds__(64) void M8ThresholdPoolKernel(M8BatchArgs a, const
             if (cntc == 0) continue;
             const uint64_t nbl = (cntc + 31) >> 5, nb = min<uint64_t>(nbl, budget - used);
             const uint64_t npos = min<uint64_t>(nb * 32, cntc);
-            for (uint64_t i = lane; i < npos; i += 64) a.pool[base + n + i] = (p << 32) | i;
+            const uint64_t s0 = a.v.batch_begin[c] * 32;   // the list's first batch slot
+            for (uint64_t i = lane; i < npos; i += 64) a.pool[base + n + i] = (p << 32) | (s0 + i);
             n += npos;
             used += nb;
             if (used >= budget) {
@@ -1064,17 +1067,17 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
         float dsurv = 0.0F;
         uint32_t idsurv = 0;
         if (has) {
-            const uint32_t p = static_cast<uint32_t>(ent >> 32), pos = static_cast<uint32_t>(ent);
-            const uint32_t c = a.probe[static_cast<uint64_t>(q) * maxp + p];
-            const uint64_t lb = v.list_begin[c], cnt = v.list_begin[c + 1] - lb;
-            const uint32_t bj = pos >> 5, l = pos & 31u;
-            const uint32_t cntb = static_cast<uint32_t>(min<uint64_t>(32, cnt - static_cast<uint64_t>(bj) * 32));
+            // the entry names the probe and the vector's batch slot: every
+            // load below depends on the entry alone
+            const uint32_t p = static_cast<uint32_t>(ent >> 32), slot = static_cast<uint32_t>(ent);
+            const uint64_t gb = slot >> 5;
+            const uint32_t l = slot & 31u;
+            const uint32_t cntb = v.bcnt[gb];
             const uint32_t covered = cntb >= 8 ? (cntb & ~7u) : 0u;
-            const uint64_t gb = v.batch_begin[c] + bj;
             const uint8_t* prow = v.plane + gb * v.rec + static_cast<uint64_t>(l) * PC;
             // the candidate's refine record: run headers, its W / Omega bank
             // bytes, the run fields; the batch's scales per run
-            const uint8_t* rr = v.rrec + (lb + pos) * v.rrec_bytes;
+            const uint8_t* rr = v.rrec + static_cast<uint64_t>(slot) * v.rrec_bytes;
             const uint8_t* rbank = rr + 16 * R;
             const float4* bsc = v.bscale + gb * R;
             // algorithmic bytes: the pool entry, the plane row, per run its
@@ -1260,7 +1263,7 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
             if (!kFirst || R == 1) {
                 dsurv = static_cast<float>(dist);
                 live = live && (dsurv < tau || !(tau < finf));
-                if (live) idsurv = v.ids[lb + pos];
+                if (live) idsurv = v.pids[slot];
             }
         }
         if (kFirst && R > 1) {   // the cascade continues in the second phase

@@ -320,31 +320,51 @@ __global__ __launch_bounds__(256) void M8BanksKernel(M8View v, const float* __re
 // record.  A record is the refine row's float4 header, the vector's W and
 // Omega bank bytes per run (zero padded to rrec_shift bytes), the refine
 // row's fields, zeros to the end of its last 128-byte line.
-__global__ __launch_bounds__(256) void M8RefineRecKernel(M8View v) {
+// One 8-byte word of the refine record of batch slot s = gb * 32 + l (zero
+// past the list's vectors).
+__global__ __launch_bounds__(256) void M8RefineRecKernel(M8View v, uint64_t nbatch) {
     const uint64_t words = v.rrec_bytes / 8;
     const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= v.n * words) return;
-    const uint64_t gi = t / words;
+    if (t >= nbatch * 32 * words) return;
+    const uint64_t s = t / words, gb = s >> 5;
+    const uint32_t l = static_cast<uint32_t>(s & 31);
     const uint32_t o = static_cast<uint32_t>(t % words) * 8;
     const uint32_t R = v.nrun, hdr = 16 * R;
     const uint32_t fields = v.rstride_fields;   // bytes of the fields after the header
-    const uint8_t* row = v.refine + gi * v.rstride;
+    const uint32_t c = M8ListOf(v.batch_begin, v.nlist, gb);
+    const uint64_t i = (gb - v.batch_begin[c]) * 32 + l;   // position in list c
     uint64_t w = 0;
-    if (o < hdr) {
-        w = *reinterpret_cast<const uint64_t*>(row + o);
-    } else if (o < hdr + v.rrec_shift) {
-        const uint32_t c = M8ListOf(v.list_begin, v.nlist, gi);
-        const uint64_t i = gi - v.list_begin[c];
-        const uint8_t* banks = v.plane + (v.batch_begin[c] + i / 32) * v.rec + 32u * v.plane_cols;
-        const uint32_t l = static_cast<uint32_t>(i % 32);
-        for (uint32_t k = 0; k < 8; ++k) {
-            const uint32_t j = o - hdr + k;   // byte j of the bank area: run j / 2, W (even) or Omega (odd)
-            if (j < 2 * R) w |= static_cast<uint64_t>(banks[(j / 2) * kM8BankBytes + 32 * (1 + (j & 1)) + l]) << (8 * k);
+    if (i < v.list_begin[c + 1] - v.list_begin[c]) {
+        const uint64_t gi = v.list_begin[c] + i;
+        const uint8_t* row = v.refine + gi * v.rstride;
+        if (o < hdr) {
+            w = *reinterpret_cast<const uint64_t*>(row + o);
+        } else if (o < hdr + v.rrec_shift) {
+            const uint8_t* banks = v.plane + gb * v.rec + 32u * v.plane_cols;
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t j = o - hdr + k;   // byte j of the bank area: run j / 2, W (even) or Omega (odd)
+                if (j < 2 * R)
+                    w |= static_cast<uint64_t>(banks[(j / 2) * kM8BankBytes + 32 * (1 + (j & 1)) + l]) << (8 * k);
+            }
+        } else if (o < hdr + v.rrec_shift + fields) {
+            w = *reinterpret_cast<const uint64_t*>(row + o - v.rrec_shift);
         }
-    } else if (o < hdr + v.rrec_shift + fields) {
-        w = *reinterpret_cast<const uint64_t*>(row + o - v.rrec_shift);
     }
-    *reinterpret_cast<uint64_t*>(v.rrec + gi * v.rrec_bytes + o) = w;
+    *reinterpret_cast<uint64_t*>(v.rrec + s * v.rrec_bytes + o) = w;
+}
+
+// Each batch's vector count and each batch slot's id.
+__global__ __launch_bounds__(256) void M8SlotKernel(M8View v, uint64_t nbatch) {
+    const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= nbatch * 32) return;
+    const uint64_t gb = s >> 5;
+    const uint32_t c = M8ListOf(v.batch_begin, v.nlist, gb);
+    const uint64_t cnt = v.list_begin[c + 1] - v.list_begin[c], i = (gb - v.batch_begin[c]) * 32 + (s & 31);
+    v.pids[s] = i < cnt ? v.ids[v.list_begin[c] + i] : 0xFFFFFFFFu;
+    if ((s & 31) == 0) {
+        const uint64_t left = cnt - (i < cnt ? i : cnt);
+        v.bcnt[gb] = static_cast<uint8_t>(left < 32 ? left : 32);
+    }
 }
 
 __global__ __launch_bounds__(256) void M8BatchScaleKernel(M8View v, uint64_t nbatch) {
@@ -360,10 +380,15 @@ __global__ __launch_bounds__(256) void M8BatchScaleKernel(M8View v, uint64_t nba
 // Launchers
 
 hipError_t LaunchM8RefineRecords(hipStream_t st, const M8View& v, uint64_t nbatch) {
-    const uint64_t t1 = v.n * (v.rrec_bytes / 8);
+    const uint64_t t1 = nbatch * 32 * (v.rrec_bytes / 8);
     if (t1 > 0) {
-        hipLaunchKernelGGL(M8RefineRecKernel, dim3(static_cast<uint32_t>((t1 + 255) / 256)), dim3(256), 0, st, v);
-        const hipError_t e = hipGetLastError();
+        hipLaunchKernelGGL(M8RefineRecKernel, dim3(static_cast<uint32_t>((t1 + 255) / 256)), dim3(256), 0, st, v,
+                           nbatch);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(M8SlotKernel, dim3(static_cast<uint32_t>((nbatch * 32 + 255) / 256)), dim3(256), 0, st, v,
+                           nbatch);
+        e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     const uint64_t t2 = nbatch * v.nrun;

@@ -15,13 +15,18 @@
 //            (the reference keeps gain/vpc in separate per-list arrays), then
 //            each run's (rate-1)-bit refine fields (me8q_fast.cpp:97-118).
 //   rrec     (indexes of up to four coded runs: the batched form's refine)
-//            per vector, `rrec_bytes` (whole 128-byte lines): the refine
+//            per batch slot (32 per 32-vector batch, the batch tail's slots
+//            zero), `rrec_bytes` (whole 128-byte lines): the refine
 //            row's nrun float4 header, then the vector's W and Omega bank
 //            bytes per run (padded to `rrec_shift` bytes), then the refine
 //            row's fields -- everything the exact cascade reads for one
 //            candidate besides its plane row, on as few lines as it fits.
 //   bscale   per 32-vector batch, nrun x float4 {W_min, W_step, Omega_min,
 //            Omega_step}: the bank scales of the batch record.
+//   bcnt     per batch, its vector count; pids per batch slot, the id.
+//   The batched form's pool entries name a vector by its batch slot
+//   (global batch * 32 + lane), so its plane row, refine record, batch
+//   scales and count are addressed without the list's offsets.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -78,6 +83,8 @@ struct M8View {
     uint8_t* refine;
     uint8_t* rrec;        // null when the index has more than four coded runs
     float4* bscale;
+    uint8_t* bcnt;        // [nbatch] vectors in each batch (with rrec)
+    uint32_t* pids;       // [nbatch * 32] the id at each batch slot, UINT32_MAX past the list (with rrec)
     uint32_t rrec_bytes, rrec_shift;
     uint32_t rstride_fields;   // bytes of a refine row's fields (after its 16 x nrun header)
     uint32_t pad_;

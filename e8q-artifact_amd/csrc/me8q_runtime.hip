@@ -33,12 +33,12 @@ struct e8q_m8index {
     std::vector<uint32_t> ref_run_off;
     uint32_t ref_stride = 0;
     DBuf d_runs, d_segs, d_list_begin, d_batch_begin, ids, codes, centroids, cent_rot, cn2, rc_n2, plane, refine,
-        signs, lev, rrec, bscale;
+        signs, lev, rrec, bscale, bcnt, pids;
     std::vector<DBuf> enc;   // encoder tables, 5 per distinct rate
     uint64_t device_bytes() const {
         uint64_t b = d_runs.bytes + d_segs.bytes + d_list_begin.bytes + d_batch_begin.bytes + ids.bytes + codes.bytes +
                      centroids.bytes + cent_rot.bytes + cn2.bytes + rc_n2.bytes + plane.bytes + refine.bytes +
-                     signs.bytes + lev.bytes + rrec.bytes + bscale.bytes;
+                     signs.bytes + lev.bytes + rrec.bytes + bscale.bytes + bcnt.bytes + pids.bytes;
         for (const auto& e : enc) b += e.bytes;
         return b;
     }
@@ -62,6 +62,8 @@ void RefreshM8View(e8q_m8index* x) {
     v.refine = x->refine.as<uint8_t>();
     v.rrec = x->rrec.as<uint8_t>();
     v.bscale = x->bscale.as<float4>();
+    v.bcnt = x->bcnt.as<uint8_t>();
+    v.pids = x->pids.as<uint32_t>();
 }
 
 // Rotated centroids + FastScan layout from codes / centroids already on the device.
@@ -130,6 +132,7 @@ uint64_t EnvU64(const char* name, uint64_t dflt) {
 bool M8BatchedFits(const e8q_m8index* x, uint64_t k) {
     const M8View& v = x->view;
     if (v.nrun < 1 || v.nrun > kM8BatchRuns || k > kM8SelMaxK || (v.plane_cols & 7u) != 0) return false;
+    if (v.n > 0 && v.rrec == nullptr) return false;   // more batch slots than 32-bit words name
     uint32_t cols = 0;
     for (const auto& r : x->runs) cols += r.nblk;
     return cols == v.plane_cols;
@@ -993,10 +996,13 @@ e8q_status e8q_m8_create(e8q_ctx* ctx, const e8q_m8_desc* d, e8q_m8index** out) 
             x->rc_n2.alloc(v.nlist * std::max<uint32_t>(nrun, 1) * sizeof(double));
             x->plane.alloc(x->nbatch * v.rec + 16);
             x->refine.alloc(v.n * v.rstride + 16);
-            // the batched form's refine records, for indexes it runs on
-            if (nrun >= 1 && nrun <= kM8BatchRuns && v.n > 0) {
-                x->rrec.alloc(v.n * v.rrec_bytes);
+            // the batched form's refine records, per batch slot, for
+            // indexes it runs on (batch slots are named by 32-bit words)
+            if (nrun >= 1 && nrun <= kM8BatchRuns && v.n > 0 && x->nbatch * 32 < (uint64_t{1} << 32)) {
+                x->rrec.alloc(x->nbatch * 32 * v.rrec_bytes);
                 x->bscale.alloc(x->nbatch * nrun * sizeof(float4));
+                x->bcnt.alloc(x->nbatch);
+                x->pids.alloc(x->nbatch * 32 * sizeof(uint32_t));
             }
             Check(hipMemsetAsync(x->codes.p, 0, x->codes.bytes, st), "clear");
             Check(hipMemsetAsync(x->refine.p, 0, x->refine.bytes, st), "clear");
