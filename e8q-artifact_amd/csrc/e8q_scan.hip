@@ -207,6 +207,9 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
     if (ix.bits <= 8)
         for (int i = lane; i < 512; i += kWave) S.tab[i] = G(ix.refine_tab)[i];
     const uint32_t j = lane >> 5, t = lane & 31;
+    // All ones in the upper half-wave (kept in a vector register).
+    uint32_t jmask = 0u - j;
+    asm volatile("" : "+v"(jmask));
     const int k = static_cast<int>(a.k);
     const int fe = a.flush_every;
     const float inf = __builtin_inff();
@@ -261,7 +264,10 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             return ClusterRows{arena_c, ix.ref_arena + ReadLane64(ch.ro_lo, ch.ro_hi, l),
                                ReadLane64(ch.vb_lo, ch.vb_hi, l), d2, stg};
         };
-        Rec<NQW> rn;
+        // Two record sets: a step scores one while the next step's records
+        // load into the other (the loop body runs twice per iteration with
+        // the roles swapped, so no set is ever copied).
+        Rec<NQW> ra, rb;
         cp = Uniform(cp);
         ch.base = Uniform(ch.base);
         ch.active = Uniform64(ch.active);
@@ -269,14 +275,16 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
         if (cp < np) {
             open(cp);
             horizon();
-            if (j < min(2u, cnb)) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(j) * BB, t, blocks);
+            LoadRec<NQW>(ra, arena_c + static_cast<uint64_t>(min(j, cnb - 1)) * BB, t, blocks);
         }
-        for (;;) {
+        // One step on rc (its records loaded), prefetching into rn; false
+        // when the query's probes are done.
+        auto step = [&](const Rec<NQW>& rc, Rec<NQW>& rn) -> bool {
             // The scan state is wave-uniform; saying so keeps it in scalar
             // registers and the loop on scalar branches (the compiler's
             // uniformity analysis gives up after the per-lane merge loops).
             cp = Uniform(cp);
-            if (cp >= np) break;
+            if (cp >= np) return false;
             cb0 = Uniform(cb0);
             cnb = Uniform(cnb);
             ccnt = Uniform(ccnt);
@@ -293,25 +301,36 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             n_surv = Uniform(n_surv);
             n_scan = Uniform(n_scan);
             // The next step; its records are loaded now when its probe is in
-            // the loaded chunk (otherwise after this step, see below).
+            // the loaded chunk (otherwise after this step, see below).  Every
+            // lane loads, so that the number of loads in flight is the same
+            // on every path and a step waits only for its own records: a
+            // lane past the cluster's batches loads the last one, and when
+            // the next records come after the step this step's are loaded
+            // in their place.
             uint32_t xp = cp, xb0 = cb0 + 2;
             if (xb0 >= cnb) {
                 xp = NextInChunk(ch, cp + 1, np);
                 xb0 = 0;
             }
             const bool x_in = xp < np && xp - ch.base < 64;
-            const Rec<NQW> rc = rn;
-            if (x_in) {
+            {
                 const uint8_t* xa = arena_c;
-                uint32_t xnb = cnb;
-                if (xp != cp) {
-                    const uint32_t l = xp - ch.base;
-                    xa = arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
-                    xnb = (ReadLane(ch.cnt, l) + 31) / 32;
+                uint32_t xnb = cnb, xb = cb0;
+                if (x_in) {
+                    xb = xb0;
+                    if (xp != cp) {
+                        const uint32_t l = xp - ch.base;
+                        xa = arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
+                        xnb = (ReadLane(ch.cnt, l) + 31) / 32;
+                    }
                 }
-                if (j < min(2u, xnb - xb0)) LoadRec<NQW>(rn, xa + static_cast<uint64_t>(xb0 + j) * BB, t, blocks);
+                LoadRec<NQW>(rn, xa + static_cast<uint64_t>(min(xb + j, xnb - 1)) * BB, t, blocks);
             }
 
+            // The record's unused header word (sc[7]) stays live until here,
+            // so that no temporary takes its register while the next
+            // records' loads are in flight (that would wait for them).
+            asm volatile("" ::"v"(rc.s1.w));
             const uint32_t jn = min(2u, cnb - cb0);
             n_scan += jn;
             const uint32_t bj = cb0 + j;
@@ -354,21 +373,23 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             // reference's batch bound input (non-negative floats order as
             // their bit patterns).
             const uint32_t hm = HalfMax(sabs);
-            const float sa = __uint_as_float(j ? ReadLane(hm, 63) : ReadLane(hm, 31));
+            const uint32_t h31 = ReadLane(hm, 31);
+            const float sa = __uint_as_float(h31 ^ ((h31 ^ ReadLane(hm, 63)) & jmask));
             if (act) lbp = fmaf(-rc.s1.z, sa, lbp);
             bool skip = false;
             if (prune && L < inf) skip = fmaf(-pam, zrh, lbp) >= L;
-            const uint32_t n_logic = act ? min(32u, ccnt - bj * 32) : 0u;
             const float dlo = fmaf(-zrh, pp, t1);
-            const bool pass = act && !skip && t < n_logic && (!prune || dlo < L);
+            // (a batch's lanes past the cluster's vectors hold no vector)
+            const bool pass = act & !skip & (bj * 32 + t < ccnt) & (!prune | (dlo < L));
             const uint64_t bal = __ballot(pass);
-            const uint32_t pm0 = static_cast<uint32_t>(bal), pm1 = static_cast<uint32_t>(bal >> 32);
-            const int total = __popc(pm0) + __popc(pm1);
+            const int total = __popcll(bal);
             if (staged + total <= fe) {
-                // No flush inside the step: apply the speculative decisions.
+                // No flush inside the step: apply the speculative decisions
+                // (staged in lane order: batch by batch, lane by lane).
                 if (pass) {
-                    const uint32_t m = j ? pm1 : pm0;
-                    const int rank = staged + (j ? __popc(pm0) : 0) + __popc(m & ((1u << t) - 1u));
+                    const int rank = staged + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                                  static_cast<uint32_t>(bal >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u)));
                     stg[rank] = bj * 32 + t;
                 }
                 staged += total;
@@ -446,16 +467,16 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                 xp = NextInChunk(ch, xp + 1, np);
                 reload = true;
             }
-            if (xp >= np) break;
+            if (xp >= np) return false;
             if (xp - ch.base >= 64) {
                 xp = Settle(ch, xp, ix, a, qi, np, limit, sk, lane_cand, lane_nb);
-                if (xp >= np) break;
+                if (xp >= np) return false;
             }
             if (xp != cp) {
                 open(xp);
                 horizon();
             }
-            if (reload && j < min(2u, cnb)) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(j) * BB, t, blocks);
+            if (reload) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(min(j, cnb - 1)) * BB, t, blocks);
             cp = Uniform(xp);
             cb0 = Uniform(xb0);
             cnb = Uniform(cnb);
@@ -472,6 +493,9 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             ch.nonempty = Uniform64(ch.nonempty);
             n_surv = Uniform(n_surv);
             n_scan = Uniform(n_scan);
+            return true;
+        };
+        while (step(ra, rb) && step(rb, ra)) {
         }
         for (int i = lane; i < k; i += kWave) {
             Gw(a.out_ids)[qi * a.k + i] = i < cnt ? S.ti[i] : 0xFFFFFFFFu;
