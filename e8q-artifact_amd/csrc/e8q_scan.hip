@@ -17,6 +17,8 @@
 // waiting and at the end of each cluster; the top-k is a sorted (dist, id)
 // list in LDS whose k-th entry is the reference heap's root, the prune
 // threshold.
+#include <cstdlib>
+
 #include "e8q_scan_dev.h"
 
 namespace e8qdev {
@@ -142,22 +144,46 @@ struct Rec {
     f32x4 s1;                    // sc[4..7]
 };
 
+// This lane's byte offsets inside a step's records: its rowA bytes, its
+// bank bytes and its batch header's scales (lane half j reads batch j of
+// the step).  The loads add small constants as their scalar offsets.
+struct LaneOffs {
+    uint32_t row, hdr, hs96;
+};
+
+// The step's records: `nrec` records of `bb` bytes from `rec_p`, the first
+// batch's record (wave-uniform), read through a buffer descriptor with these
+// per-lane offsets, so a step spends no vector instructions on addresses.
+// The second batch may lie past the cluster (then nrec is 1): such loads
+// return zeros, and those lanes do not use them.
 template <int NQW>
-__device__ __forceinline__ void LoadRec(Rec<NQW>& r, const uint8_t* rec_p, uint32_t t, int blocks) {
-    const E8Q_GLOBAL uint8_t* rec = G(rec_p);
-    const uint32_t row = kHdr + t * static_cast<uint32_t>(blocks);
+__device__ __forceinline__ void LoadRec(Rec<NQW>& r, const uint8_t* rec_p, uint32_t nrec, uint32_t bb,
+                                        const LaneOffs& o) {
+    const uint32_t nb = __umulhi(nrec, bb) != 0 ? 0xFFFFFFFFu : nrec * bb;   // byte count, saturated
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(UniformPtr(rec_p)), 0,
+                                                                        static_cast<int>(nb), kBufferDword3);
     if constexpr (NQW > 0) {
-        const E8Q_GLOBAL u32x2* rw = reinterpret_cast<const E8Q_GLOBAL u32x2*>(rec + row);
+        if constexpr (NQW % 2 == 0) {
 #pragma unroll
-        for (int w = 0; w < NQW; ++w) r.w[w] = rw[w];
+            for (int w = 0; w < NQW; w += 2) {
+                const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, o.row, 8 * w, 0);
+                r.w[w] = u32x2{v.x, v.y};
+                r.w[w + 1] = u32x2{v.z, v.w};
+            }
+        } else {
+#pragma unroll
+            for (int w = 0; w < NQW; ++w) r.w[w] = __builtin_amdgcn_raw_buffer_load_b64(rs, o.row, 8 * w, 0);
+        }
     } else {
-        r.rp = rec_p + row;
+        r.rp = rec_p + o.row;
     }
-    r.hb = rec[t];
-    r.hw = rec[32 + t];
-    r.hp = rec[64 + t];
-    r.s0 = *reinterpret_cast<const E8Q_GLOBAL f32x4*>(rec + 96);
-    r.s1 = *reinterpret_cast<const E8Q_GLOBAL f32x4*>(rec + 112);
+    r.hb = __builtin_amdgcn_raw_buffer_load_b8(rs, o.hdr, 0, 0);
+    r.hw = __builtin_amdgcn_raw_buffer_load_b8(rs, o.hdr, 32, 0);
+    r.hp = __builtin_amdgcn_raw_buffer_load_b8(rs, o.hdr, 64, 0);
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, o.hs96, 0, 0);
+    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, o.hs96, 16, 0);
+    r.s0 = f32x4{__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w)};
+    r.s1 = f32x4{__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w)};
 }
 
 template <int NQW>
@@ -210,6 +236,8 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
     // All ones in the upper half-wave (kept in a vector register).
     uint32_t jmask = 0u - j;
     asm volatile("" : "+v"(jmask));
+    const LaneOffs lo{j * Uniform(ix.batch_bytes) + kHdr + t * static_cast<uint32_t>(ix.blocks),
+                      j * Uniform(ix.batch_bytes) + t, j * Uniform(ix.batch_bytes) + 96};
     const int k = static_cast<int>(a.k);
     const int fe = a.flush_every;
     const float inf = __builtin_inff();
@@ -275,7 +303,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
         if (cp < np) {
             open(cp);
             horizon();
-            LoadRec<NQW>(ra, arena_c + static_cast<uint64_t>(min(j, cnb - 1)) * BB, t, blocks);
+            LoadRec<NQW>(ra, arena_c, cnb, BB, lo);
         }
         // One step on rc (its records loaded), prefetching into rn; false
         // when the query's probes are done.
@@ -303,8 +331,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             // The next step; its records are loaded now when its probe is in
             // the loaded chunk (otherwise after this step, see below).  Every
             // lane loads, so that the number of loads in flight is the same
-            // on every path and a step waits only for its own records: a
-            // lane past the cluster's batches loads the last one, and when
+            // on every path and a step waits only for its own records: when
             // the next records come after the step this step's are loaded
             // in their place.
             uint32_t xp = cp, xb0 = cb0 + 2;
@@ -324,7 +351,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                         xnb = (ReadLane(ch.cnt, l) + 31) / 32;
                     }
                 }
-                LoadRec<NQW>(rn, xa + static_cast<uint64_t>(min(xb + j, xnb - 1)) * BB, t, blocks);
+                LoadRec<NQW>(rn, xa + static_cast<uint64_t>(xb) * BB, xnb - xb, BB, lo);
             }
 
             // The record's unused header word (sc[7]) stays live until here,
@@ -476,7 +503,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                 open(xp);
                 horizon();
             }
-            if (reload) LoadRec<NQW>(rn, arena_c + static_cast<uint64_t>(min(j, cnb - 1)) * BB, t, blocks);
+            if (reload) LoadRec<NQW>(rn, arena_c, cnb, BB, lo);
             cp = Uniform(xp);
             cb0 = Uniform(xb0);
             cnb = Uniform(cnb);
@@ -545,7 +572,8 @@ hipError_t LaunchPlainSearch(hipStream_t st, const IndexView& ix, const PlainArg
         if (e != hipSuccess) return e;
     }
     const uint32_t grid = static_cast<uint32_t>(nrun < 262144 ? nrun : 262144);
-    const size_t dyn = PlainSearchLds(ix, a.flush_every);
+    size_t dyn = PlainSearchLds(ix, a.flush_every);
+    if (const char* e = std::getenv("E8Q_PLAIN_LDS_PAD")) dyn += std::strtoul(e, nullptr, 10);
     // Rates 2..5 (refine fields of 1..4 bytes per block) take the specialized
     // kernels; other rates and row sizes the general one.
     if (ix.bits < 2 || ix.bits > 5) return LaunchPlainT<0, true>(st, dparams, grid, dyn);

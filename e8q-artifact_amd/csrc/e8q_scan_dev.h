@@ -24,7 +24,12 @@ __device__ __forceinline__ bool Better(float d1, uint32_t i1, float d2, uint32_t
 #define E8Q_LDS __attribute__((address_space(3)))
 // Plain clang vectors (no constructors) for loads through such pointers.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Word 3 of a raw buffer descriptor on gfx9 parts (32-bit data format, no
+// swizzle, no per-lane index): byte-addressed loads range-checked against
+// the descriptor's byte count.
+constexpr int kBufferDword3 = 0x00020000;
 template <class T>
 __device__ __forceinline__ const E8Q_GLOBAL T* G(const T* p) {
     return (const E8Q_GLOBAL T*)(p);
