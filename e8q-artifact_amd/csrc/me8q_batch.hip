@@ -1086,6 +1086,102 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
             n_bytes += 8 + PC + 19 * R;
             const double d2 = static_cast<double>(a.pdist[static_cast<uint64_t>(q) * maxp + p]);
             const double* sqd2 = a.sqd2 + (static_cast<uint64_t>(q) * maxp + p) * R;
+            // The coarse terms in the reference's arithmetic, against tau, run by
+            // run in run order (fp32 for the lanes the AVX2 filter covers, the
+            // fp64 twin for the rest).  Every private array here is indexed by
+            // compile-time constants only (guarded, unrolled run loops; Pick for
+            // a run-time position) so that it stays in registers.  The terms
+            // that do not involve the coarse sums come first: they fix the
+            // cascade's run order, so the first run is scored before the plane
+            // row is read again for the sums, while its line is still cached.
+            double str[kM8BatchRuns] = {0.0, 0.0, 0.0, 0.0};
+            double pool = 0.0, qadd = 0.0;
+            const bool f32 = l < covered;
+            if (f32) {   // the AVX2 filter's fp32 values
+                float pl = 0.0F, qa = 0.0F, wpad = 0.0F;
+#pragma unroll
+                for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                    if (r < R) {
+                        const float4 s4 = bsc[r];
+                        const float wq = fmaf(static_cast<float>(rbank[2 * r]), s4.y, s4.x);
+                        const float pq = fmaf(static_cast<float>(rbank[2 * r + 1]), s4.w, s4.z);
+                        const float ra = static_cast<float>(sqd2[r]);
+                        const float rh = sqk < ra ? sqk : ra;
+                        const float t = pq * rh;
+                        str[r] = t;
+                        pl = fmaf(t, t, pl);
+                        qa = fmaf(fabsf(wq), static_cast<float>(rcq[r * 4 + 2]), qa);
+                        wpad = wpad + static_cast<float>(0.5 * static_cast<double>(s4.y) * (rcq[r * 4 + 3] + rcq[r * 4 + 2]));
+                    }
+                }
+                pool = pl;
+                qadd = qa + wpad;
+            } else {   // the fp64 twin for the lanes the filter does not cover
+#pragma unroll
+                for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                    if (r < R) {
+                        const float4 s4 = bsc[r];
+                        const double wq = fma(static_cast<double>(rbank[2 * r]), static_cast<double>(s4.y), static_cast<double>(s4.x));
+                        const double pq = fma(static_cast<double>(rbank[2 * r + 1]), static_cast<double>(s4.w), static_cast<double>(s4.z));
+                        const double sa = sqd2[r];
+                        const double rh = sqdk < sa ? sqdk : sa;
+                        const double tr = pq * rh;
+                        str[r] = tr;
+                        pool = fma(tr, tr, pool);
+                        qadd = fma(fabs(wq), rcq[r * 4 + 2], qadd);
+                        qadd = fma(0.5 * static_cast<double>(s4.y), rcq[r * 4 + 3] + rcq[r * 4 + 2], qadd);
+                    }
+                }
+            }
+            // The cascade's run order for a group of one candidate: decreasing
+            // t^2 / nblk, stable (GCC's insertion sort below 17 elements): a
+            // sorting network on the total order (mean descending, run
+            // ascending) -- the same order -- carrying each run's t; runs past R
+            // sort last.
+            uint32_t ord[kM8BatchRuns] = {0, 1, 2, 3};
+            double sts[kM8BatchRuns] = {str[0], str[1], str[2], str[3]};   // t in cascade order
+            if (R > 1) {
+                double mean[kM8BatchRuns];
+#pragma unroll
+                for (uint32_t r = 0; r < kM8BatchRuns; ++r)
+                    mean[r] = r < R ? fma(str[r], str[r], 0.0) /
+                                          static_cast<double>(v.runs[r].nblk > 1 ? v.runs[r].nblk : 1)
+                                    : -1.0;
+                auto cswap = [&](uint32_t i, uint32_t j) {   // i < j: j first if it sorts before i
+                    const bool sw = mean[j] > mean[i] || (mean[j] == mean[i] && ord[j] < ord[i]);
+                    const double m0 = mean[i], t0 = sts[i];
+                    const uint32_t o0 = ord[i];
+                    mean[i] = sw ? mean[j] : m0;
+                    sts[i] = sw ? sts[j] : t0;
+                    ord[i] = sw ? ord[j] : o0;
+                    mean[j] = sw ? m0 : mean[j];
+                    sts[j] = sw ? t0 : sts[j];
+                    ord[j] = sw ? o0 : ord[j];
+                };
+                cswap(0, 1);
+                cswap(2, 3);
+                cswap(0, 2);
+                cswap(1, 3);
+                cswap(1, 2);
+            }
+            auto pick = [](const auto (&x)[kM8BatchRuns], uint32_t i) {
+                return i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : x[3];
+            };
+            const uint32_t lo0 = a.ref_lev_off[0], lo1 = a.ref_lev_off[1], lo2 = a.ref_lev_off[2],
+                           lo3 = a.ref_lev_off[3];
+            const E8Q_GLOBAL float* qrot = G(a.rotq) + static_cast<uint64_t>(q) * P;
+            // The exact score of run r (its plane bytes and fields).
+            auto run_score = [&](uint32_t r) -> double {
+                const M8Run& ru = v.runs[r];
+                const E8Q_GLOBAL uint8_t* pb = G(prow) + ru.blk0;
+                const E8Q_GLOBAL uint8_t* fr = G(rr) + ru.ref_off + v.rrec_shift;
+                const uint32_t lo = r == 0 ? lo0 : r == 1 ? lo1 : r == 2 ? lo2 : lo3;
+                return lo != kM8LevGlobal
+                           ? M8LaneScore(ru.rate, pb, fr, (const E8Q_LDS float*)(reinterpret_cast<const float*>(smem)) + lo,
+                                         qrot + ru.dim_off, ru.nblk)
+                           : M8LaneScore(ru.rate, pb, fr, G(ru.lev), qrot + ru.dim_off, ru.nblk);
+            };
+            const double se0 = run_score(ord[0]);
             // integer coarse sums of each run (AccumulateCoarseHi over its columns)
             int32_t part[kM8BatchRuns] = {0, 0, 0, 0};
             auto word_sum = [&](uint64_t w, uint32_t col) -> int32_t {
@@ -1130,51 +1226,14 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
                     }
                 }
             }
-            // The coarse terms in the reference's arithmetic, against tau, run by
-            // run in run order.  Every private array here is indexed by
-            // compile-time constants only (guarded, unrolled run loops; Pick for
-            // a run-time position) so that it stays in registers.
-            double ssc[kM8BatchRuns] = {0.0, 0.0, 0.0, 0.0}, str[kM8BatchRuns] = {0.0, 0.0, 0.0, 0.0};
-            double pool = 0.0, qadd = 0.0;
-            if (l < covered) {   // the AVX2 filter's fp32 values
-                float pl = 0.0F, qa = 0.0F, wpad = 0.0F;
+            // each run's coarse score
+            double ssc[kM8BatchRuns] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
-                    if (r < R) {
-                        const float4 s4 = bsc[r];
-                        const float s_c = fmaf(static_cast<float>(part[r]), lut_delta, static_cast<float>(rcq[r * 4 + 1]));
-                        const float wq = fmaf(static_cast<float>(rbank[2 * r]), s4.y, s4.x);
-                        const float pq = fmaf(static_cast<float>(rbank[2 * r + 1]), s4.w, s4.z);
-                        const float ra = static_cast<float>(sqd2[r]);
-                        const float rh = sqk < ra ? sqk : ra;
-                        const float t = pq * rh;
-                        ssc[r] = s_c;
-                        str[r] = t;
-                        pl = fmaf(t, t, pl);
-                        qa = fmaf(fabsf(wq), static_cast<float>(rcq[r * 4 + 2]), qa);
-                        wpad = wpad + static_cast<float>(0.5 * static_cast<double>(s4.y) * (rcq[r * 4 + 3] + rcq[r * 4 + 2]));
-                    }
-                }
-                pool = pl;
-                qadd = qa + wpad;
-            } else {   // the fp64 twin for the lanes the filter does not cover
-#pragma unroll
-                for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
-                    if (r < R) {
-                        const float4 s4 = bsc[r];
-                        const double scr = fma(static_cast<double>(lut_delta), static_cast<double>(part[r]), rcq[r * 4 + 1]);
-                        const double wq = fma(static_cast<double>(rbank[2 * r]), static_cast<double>(s4.y), static_cast<double>(s4.x));
-                        const double pq = fma(static_cast<double>(rbank[2 * r + 1]), static_cast<double>(s4.w), static_cast<double>(s4.z));
-                        const double sa = sqd2[r];
-                        const double rh = sqdk < sa ? sqdk : sa;
-                        const double tr = pq * rh;
-                        ssc[r] = scr;
-                        str[r] = tr;
-                        pool = fma(tr, tr, pool);
-                        qadd = fma(fabs(wq), rcq[r * 4 + 2], qadd);
-                        qadd = fma(0.5 * static_cast<double>(s4.y), rcq[r * 4 + 3] + rcq[r * 4 + 2], qadd);
-                    }
-                }
+            for (uint32_t r = 0; r < kM8BatchRuns; ++r) {
+                if (r < R)
+                    ssc[r] = f32 ? static_cast<double>(fmaf(static_cast<float>(part[r]), lut_delta,
+                                                            static_cast<float>(rcq[r * 4 + 1])))
+                                 : fma(static_cast<double>(lut_delta), static_cast<double>(part[r]), rcq[r * 4 + 1]);
             }
             const float* f4 = reinterpret_cast<const float*>(rr);
             double dist;
@@ -1192,67 +1251,22 @@ __global__ __launch_bounds__(64 * kM8RefineWaves, 3) void M8RefineKernel(M8Batch
                 }
                 dist = dcq;
             }
-            // The cascade's run order for a group of one: decreasing t^2 / nblk,
-            // stable (GCC's insertion sort below 17 elements): a sorting network
-            // on the total order (mean descending, run ascending) -- the same
-            // order -- carrying each run's coarse score and its t; runs past R
-            // sort last.
-            uint32_t ord[kM8BatchRuns] = {0, 1, 2, 3};
-            if (R > 1) {
-                double mean[kM8BatchRuns];
-#pragma unroll
-                for (uint32_t r = 0; r < kM8BatchRuns; ++r)
-                    mean[r] = r < R ? fma(str[r], str[r], 0.0) /
-                                          static_cast<double>(v.runs[r].nblk > 1 ? v.runs[r].nblk : 1)
-                                    : -1.0;
-                auto cswap = [&](uint32_t i, uint32_t j) {   // i < j: j first if it sorts before i
-                    const bool sw = mean[j] > mean[i] || (mean[j] == mean[i] && ord[j] < ord[i]);
-                    const double m0 = mean[i], s0 = ssc[i], t0 = str[i];
-                    const uint32_t o0 = ord[i];
-                    mean[i] = sw ? mean[j] : m0;
-                    ssc[i] = sw ? ssc[j] : s0;
-                    str[i] = sw ? str[j] : t0;
-                    ord[i] = sw ? ord[j] : o0;
-                    mean[j] = sw ? m0 : mean[j];
-                    ssc[j] = sw ? s0 : ssc[j];
-                    str[j] = sw ? t0 : str[j];
-                    ord[j] = sw ? o0 : ord[j];
-                };
-                cswap(0, 1);
-                cswap(2, 3);
-                cswap(0, 2);
-                cswap(1, 3);
-                cswap(1, 2);
-            }
-            auto pick = [](const auto (&x)[kM8BatchRuns], uint32_t i) {
-                return i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : x[3];
-            };
-            const uint32_t lo0 = a.ref_lev_off[0], lo1 = a.ref_lev_off[1], lo2 = a.ref_lev_off[2],
-                           lo3 = a.ref_lev_off[3];
             double rem2 = pool;
             live = true;
             uint64_t bits = 0;
-            const E8Q_GLOBAL float* qrot = G(a.rotq) + static_cast<uint64_t>(q) * P;
             const uint32_t nsteps = kFirst ? 1u : R;
             for (uint32_t oi = 0; oi < nsteps && live; ++oi) {
                 const uint32_t r = pick(ord, oi);
                 const M8Run& ru = v.runs[r];
-                const E8Q_GLOBAL uint8_t* pb = G(prow) + ru.blk0;
-                const E8Q_GLOBAL uint8_t* fr = G(rr) + ru.ref_off + v.rrec_shift;
-                const uint32_t lo = r == 0 ? lo0 : r == 1 ? lo1 : r == 2 ? lo2 : lo3;
-                const double se =
-                    lo != kM8LevGlobal
-                        ? M8LaneScore(ru.rate, pb, fr, (const E8Q_LDS float*)(reinterpret_cast<const float*>(smem)) + lo,
-                                      qrot + ru.dim_off, ru.nblk)
-                        : M8LaneScore(ru.rate, pb, fr, G(ru.lev), qrot + ru.dim_off, ru.nblk);
+                const double se = oi == 0 ? se0 : run_score(r);
                 const double coef = f4[r * 4];
                 if (R == 1) {
                     dist = fma(-(2.0 * coef), se, dist);
                 } else {
                     const double gn = f4[r * 4 + 2], vp = f4[r * 4 + 3];
-                    dist = fma(2.0 * coef, fma(gn, pick(ssc, oi), vp) - se, dist);
+                    dist = fma(2.0 * coef, fma(gn, pick(ssc, r), vp) - se, dist);
                 }
-                const double t = pick(str, oi);
+                const double t = pick(sts, oi);
                 rem2 = fma(-t, t, rem2);
                 if (kFirst || oi > 0) bits += static_cast<uint64_t>(ru.rate) * ru.len;
                 n_bytes += static_cast<uint64_t>(ru.rate - 1) * ru.len / 8;   // the run's fields
