@@ -148,9 +148,8 @@ __device__ __forceinline__ uint32_t M8ListOf(const uint64_t* begin, uint64_t nli
 // digit of (block, ci) = its refine field bits (T = rate-1 per digit, digit
 // 7's LSB the parity of digits 0..6's, the branch in the field's top bit)
 // under the plane's top bit; at rate 1 the plane byte is the whole code.
-template <int T>
-__device__ __forceinline__ float M8HalfChainT(const uint8_t* __restrict__ pb, const uint8_t* __restrict__ fr,
-                                              const float* __restrict__ lev, const float* __restrict__ qb,
+template <int T, class BP = const uint8_t*>
+__device__ __forceinline__ float M8HalfChainT(BP pb, BP fr, const float* __restrict__ lev, const float* __restrict__ qb,
                                               uint32_t nblk, uint32_t half, uint32_t ci) {
     constexpr uint32_t rate = T + 1;
     uint64_t pml = 0;
@@ -170,7 +169,7 @@ __device__ __forceinline__ float M8HalfChainT(const uint8_t* __restrict__ pb, co
             dgt = ci < 7 ? top : (__popc(pbyte >> 1) & 1u);
             br = pbyte & 1u;
         } else {
-            const uint8_t* fb = fr + blk * T;
+            const auto fb = fr + blk * T;
             uint64_t flo = 0;
             uint32_t fhi = 0;
 #pragma unroll
@@ -200,22 +199,23 @@ __device__ __forceinline__ float M8HalfChainT(const uint8_t* __restrict__ pb, co
     return acc;
 }
 
-__device__ inline float M8HalfChain(uint32_t rate, const uint8_t* pb, const uint8_t* fr, const float* lev, const float* qb,
-                             uint32_t nblk, uint32_t half, uint32_t ci) {
+template <class BP = const uint8_t*>
+__device__ inline float M8HalfChain(uint32_t rate, BP pb, BP fr, const float* lev, const float* qb, uint32_t nblk,
+                                    uint32_t half, uint32_t ci) {
     switch (rate) {
-        case 1: return M8HalfChainT<0>(pb, fr, lev, qb, nblk, half, ci);
-        case 2: return M8HalfChainT<1>(pb, fr, lev, qb, nblk, half, ci);
-        case 3: return M8HalfChainT<2>(pb, fr, lev, qb, nblk, half, ci);
-        case 4: return M8HalfChainT<3>(pb, fr, lev, qb, nblk, half, ci);
-        case 5: return M8HalfChainT<4>(pb, fr, lev, qb, nblk, half, ci);
-        case 6: return M8HalfChainT<5>(pb, fr, lev, qb, nblk, half, ci);
-        case 7: return M8HalfChainT<6>(pb, fr, lev, qb, nblk, half, ci);
-        case 8: return M8HalfChainT<7>(pb, fr, lev, qb, nblk, half, ci);
-        case 9: return M8HalfChainT<8>(pb, fr, lev, qb, nblk, half, ci);
-        case 10: return M8HalfChainT<9>(pb, fr, lev, qb, nblk, half, ci);
-        case 11: return M8HalfChainT<10>(pb, fr, lev, qb, nblk, half, ci);
-        case 12: return M8HalfChainT<11>(pb, fr, lev, qb, nblk, half, ci);
-        default: return M8HalfChainT<12>(pb, fr, lev, qb, nblk, half, ci);
+        case 1: return M8HalfChainT<0, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 2: return M8HalfChainT<1, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 3: return M8HalfChainT<2, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 4: return M8HalfChainT<3, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 5: return M8HalfChainT<4, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 6: return M8HalfChainT<5, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 7: return M8HalfChainT<6, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 8: return M8HalfChainT<7, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 9: return M8HalfChainT<8, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 10: return M8HalfChainT<9, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 11: return M8HalfChainT<10, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 12: return M8HalfChainT<11, BP>(pb, fr, lev, qb, nblk, half, ci);
+        default: return M8HalfChainT<12, BP>(pb, fr, lev, qb, nblk, half, ci);
     }
 }
 

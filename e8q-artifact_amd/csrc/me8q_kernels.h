@@ -54,6 +54,7 @@ struct M8ScanArgs {
     uint64_t* qstats;   // [nq][8] candidates, survivors, bits_scored, bits_total, pruned, batches read,
                         //         candidates refined, refine groups
     uint32_t lev_floats;   // total size of the runs' level tables
+    uint32_t run_slot;     // bytes of one run's plane bytes + fields, largest run (a multiple of 16)
     // Threshold pass of the batched form (budget > 0): lists in probe order
     // until `budget` 32-vector batches are scanned (the last list possibly in
     // part); cover[q] = (p << 32) | b, the first (probe, batch) not scanned.
@@ -63,7 +64,7 @@ struct M8ScanArgs {
     uint64_t nsel;
     const uint32_t* nsel_dev;   // optional: with qsel, the number of queries to search, on the device (<= nsel)
 };
-size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats);
+size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats, uint32_t run_slot);
 hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a);
 
 // ---- batched form (me8q_batch.hip) ----------------------------------------

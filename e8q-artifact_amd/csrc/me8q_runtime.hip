@@ -373,8 +373,11 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     sa.out_d = out_d;
     sa.qstats = qst;
     sa.lev_floats = 0;
-    for (const auto& r : x->runs)
+    sa.run_slot = 16;
+    for (const auto& r : x->runs) {
         if (r.rate <= kM8LdsLevRate) sa.lev_floats += 2u << r.rate;
+        sa.run_slot = std::max(sa.run_slot, (r.rate * r.nblk + 15) & ~15u);
+    }
 
     // Arguments of the batched kernels (pointers into the carve).
     IndexView lv{};   // the work-list kernels read the list count and offsets only

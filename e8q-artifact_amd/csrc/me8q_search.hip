@@ -159,8 +159,9 @@ __global__ __launch_bounds__(256) void M8RunDistKernel(M8View v, const float* __
 // Scan: LDS layout of one wave (one query).
 struct M8ScanLayout {
     uint32_t q, lut, td, ti, acc, fdc, fpool, fqa, fsc, ftr, tsc, ttr, ssc, str, sord, smean, rq, rqf, ssq,
-        stage, srend, slev, sS32, plist, total;
-    __host__ __device__ M8ScanLayout(uint32_t P, uint32_t PC, uint32_t R, uint32_t K, uint32_t RS, uint32_t LV) {
+        stage, srend, slev, sS32, plist, runbuf, hdr, idl, total;
+    __host__ __device__ M8ScanLayout(uint32_t P, uint32_t PC, uint32_t R, uint32_t K, uint32_t RS, uint32_t LV,
+                                     uint32_t SB) {
         uint32_t o = 0;
         auto take = [&](uint32_t bytes, uint32_t align) {
             o = (o + align - 1) & ~(align - 1);
@@ -193,6 +194,9 @@ struct M8ScanLayout {
         sS32 = take(32 * R1 * 4, 16);   // exact run scores of a batch's passing lanes
         plist = take(32, 16);           // those lanes, in order
         stage = take(4 * 64, 16);       // staged candidates: lane,row,pool,qadd,dist,died,bits
+        runbuf = take(4 * SB, 16);      // per scored (lane, run) pair of a pass: its plane bytes, then its fields
+        hdr = take(32 * R1 * 16, 16);   // a batch's passing lanes' refine row headers (R float4)
+        idl = take(32 * 4, 16);         // ... and their ids
         total = (o + 15) & ~15u;
     }
 };
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     const M8View& v = a.v;
     const uint32_t P = static_cast<uint32_t>(v.padded), PC = v.plane_cols, R = v.nrun, K = a.k;
-    const M8ScanLayout L(P, PC, R, K, v.rstride, a.lev_floats);
+    const M8ScanLayout L(P, PC, R, K, v.rstride, a.lev_floats, a.run_slot);
     float* q = reinterpret_cast<float*>(smem + L.q);
     uint8_t* lut = smem + L.lut;
     float* td = reinterpret_cast<float*>(smem + L.td);
@@ -272,6 +276,9 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
     float* slev = reinterpret_cast<float*>(smem + L.slev);
     float* sS32 = reinterpret_cast<float*>(smem + L.sS32);
     uint8_t* plist = smem + L.plist;
+    uint8_t* runbuf = smem + L.runbuf;
+    float* hdrl = reinterpret_cast<float*>(smem + L.hdr);
+    uint32_t* idl = reinterpret_cast<uint32_t*>(smem + L.idl);
     // level tables of every run, copied once per workgroup; run r's table
     // starts at slev + lev_off(r)
     // Level tables of the runs at rates <= kM8LdsLevRate (at most 256
@@ -337,35 +344,46 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                     const uint64_t b = b0 + (lane >> 5);
                     const uint32_t l = lane & 31;
                     if (b < nb) {
-                        // runs occupy consecutive plane columns, 8 per word pair: four
-                        // word pairs are loaded before any is summed
                         const uint8_t* rowp = v.plane + (bat0 + b) * v.rec + l * PC;
                         uint32_t r = 0, rend = srend[0];
                         int32_t s = 0;
-                        for (uint32_t j0 = 0; j0 < PC; j0 += 32) {
-                            uint2 w[4];
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                const uint32_t col = j0 + 8 * u;
-                                w[u] = col < PC ? *reinterpret_cast<const uint2*>(rowp + col) : uint2{0u, 0u};
+                        auto sum8 = [&](uint2 w, uint32_t col) {   // eight columns' lookups into the current run
+                            if (col == rend) {
+                                sacc[r * 64 + lane] = s;
+                                s = 0;
+                                ++r;
+                                rend = srend[r];
                             }
+                            const uint8_t* lc = lut + col * 32;
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                const uint32_t col = j0 + 8 * u;
-                                if (col >= PC) break;
-                                if (col == rend) {
-                                    sacc[r * 64 + lane] = s;
-                                    s = 0;
-                                    ++r;
-                                    rend = srend[r];
-                                }
-                                const uint8_t* lc = lut + col * 32;
+                            for (int k2 = 0; k2 < 4; ++k2) {
+                                const uint32_t x0 = (w.x >> (8 * k2)) & 0xFFu, x1 = (w.y >> (8 * k2)) & 0xFFu;
+                                s += lc[k2 * 32 + (x0 & 15)] + lc[k2 * 32 + 16 + (x0 >> 4)];
+                                s += lc[(4 + k2) * 32 + (x1 & 15)] + lc[(4 + k2) * 32 + 16 + (x1 >> 4)];
+                            }
+                        };
+                        if (PC <= 128) {
+                            // the whole row in one burst of independent loads
+                            uint2 w[16];
 #pragma unroll
-                                for (int k2 = 0; k2 < 4; ++k2) {
-                                    const uint32_t x0 = (w[u].x >> (8 * k2)) & 0xFFu, x1 = (w[u].y >> (8 * k2)) & 0xFFu;
-                                    s += lc[k2 * 32 + (x0 & 15)] + lc[k2 * 32 + 16 + (x0 >> 4)];
-                                    s += lc[(4 + k2) * 32 + (x1 & 15)] + lc[(4 + k2) * 32 + 16 + (x1 >> 4)];
+                            for (uint32_t u = 0; u < 16; ++u)
+                                w[u] = 8 * u < PC ? *reinterpret_cast<const uint2*>(rowp + 8 * u) : uint2{0u, 0u};
+#pragma unroll
+                            for (uint32_t u = 0; u < 16; ++u)
+                                if (8 * u < PC) sum8(w[u], 8 * u);
+                        } else {
+                            // runs occupy consecutive plane columns, 8 per word pair: four
+                            // word pairs are loaded before any is summed
+                            for (uint32_t j0 = 0; j0 < PC; j0 += 32) {
+                                uint2 w[4];
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) {
+                                    const uint32_t col = j0 + 8 * u;
+                                    w[u] = col < PC ? *reinterpret_cast<const uint2*>(rowp + col) : uint2{0u, 0u};
                                 }
+#pragma unroll
+                                for (int u = 0; u < 4; ++u)
+                                    if (j0 + 8 * u < PC) sum8(w[u], j0 + 8 * u);
                             }
                         }
                         sacc[r * 64 + lane] = s;
@@ -471,32 +489,56 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                             pass0 = lane_test(cnt < K ? dinf : static_cast<double>(td[K - 1]), pl, qd);
                         }
                         p0 = static_cast<uint32_t>(__ballot(pass0) >> (32 * hh));
-                        if (pass0) plist[__popc(p0 & ((1u << l) - 1u))] = static_cast<uint8_t>(l);
+                        if (pass0) {
+                            plist[__popc(p0 & ((1u << l) - 1u))] = static_cast<uint8_t>(l);
+                            // the lane's refine row header and id, read by its cascade and
+                            // push if it is staged (every staged lane passes now)
+                            const float4* h =
+                                reinterpret_cast<const float4*>(v.refine + (lb + bb * 32 + l) * v.rstride);
+                            for (uint32_t r = 0; r < R; ++r) reinterpret_cast<float4*>(hdrl + l * R * 4)[r] = h[r];
+                            idl[l] = v.ids[lb + bb * 32 + l];
+                        }
                         __syncthreads();
                         const uint32_t np0 = __popc(p0);
                         const uint32_t npairs = R * np0;
                         const uint32_t half = (lane >> 3) & 1u, ci = lane & 7u;
+                        // Per pass four (lane, run) pairs, 16 lanes each: the pair's plane
+                        // bytes and fields are copied to its LDS slot in one round of
+                        // 8-byte loads (runs start 8-byte aligned in both rows), then its
+                        // two FMA chains read them there.
                         for (uint32_t t0 = 0; t0 < npairs; t0 += 4) {
-                            const uint32_t pair = t0 + (static_cast<uint32_t>(lane) >> 4);
+                            const uint32_t g = static_cast<uint32_t>(lane) >> 4, pair = t0 + g;
+                            uint8_t* slot = runbuf + g * a.run_slot;
                             float acc = 0.0F;
                             uint32_t lj = 0, r = 0;
                             if (pair < npairs) {
                                 r = pair / np0;
                                 lj = plist[pair - r * np0];
                                 const M8Run& ru = v.runs[r];
+                                const uint32_t nwp = ru.nblk / 8, nwf = ru.nblk * (ru.rate - 1) / 8;
+                                const uint64_t* sp = reinterpret_cast<const uint64_t*>(rec + lj * PC + ru.blk0);
+                                const uint64_t* sf = reinterpret_cast<const uint64_t*>(
+                                    v.refine + (lb + bb * 32 + lj) * v.rstride + ru.ref_off);
+                                for (uint32_t w = lane & 15u; w < nwp + nwf; w += 16)
+                                    reinterpret_cast<uint64_t*>(slot)[w] = w < nwp ? sp[w] : sf[w - nwp];
+                            }
+                            __syncthreads();
+                            if (pair < npairs) {
+                                const M8Run& ru = v.runs[r];
                                 uint32_t lo_off = 0;
                                 for (uint32_t r2 = 0; r2 < r; ++r2)
                                     if (v.runs[r2].rate <= kM8LdsLevRate) lo_off += 2u << v.runs[r2].rate;
                                 const float* lev = ru.rate <= kM8LdsLevRate ? slev + lo_off : ru.lev;
-                                const uint8_t* pb = rec + lj * PC + ru.blk0;
-                                const uint8_t* fr = v.refine + (lb + bb * 32 + lj) * v.rstride + ru.ref_off;
-                                acc = M8HalfChain(ru.rate, pb, fr, lev, q + ru.dim_off + ci, ru.nblk, half, ci);
+                                typedef const __attribute__((address_space(3))) uint8_t* LdsBytes;
+                                const LdsBytes pb = (LdsBytes)(slot);
+                                acc = M8HalfChain<LdsBytes>(ru.rate, pb, pb + ru.nblk, lev, q + ru.dim_off + ci, ru.nblk,
+                                                            half, ci);
                             }
                             const float other = __shfl_xor(acc, 8);
                             const float sc8 = Hsum8Lanes(acc + other);
                             if (ci == 0 && half == 0 && pair < npairs) sS32[lj * R + r] = sc8;
+                            __syncthreads();   // the slots are refilled by the next pass
                         }
-                        __syncthreads();
                     }
                     uint32_t nst = 0, pos = 0;
                     for (;;) {
@@ -560,7 +602,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                             if (static_cast<uint32_t>(lane) < n) {   // the cascade, one lane per candidate
                                 const uint32_t j = lane;
                                 M8Stage& s = stg[j];
-                                const float* f4 = reinterpret_cast<const float*>(v.refine + (lb + s.row) * v.rstride);
+                                const float* f4 = hdrl + s.lane * R * 4;
                                 double dist;
                                 if (R == 1) {
                                     dist = d2 + static_cast<double>(f4[1]);
@@ -601,7 +643,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                                 n_bits += s.bits;
                                 if (s.died) continue;
                                 ++n_surv;
-                                M8Push(td, ti, cnt, K, static_cast<float>(s.dist), v.ids[lb + s.row]);
+                                M8Push(td, ti, cnt, K, static_cast<float>(s.dist), idl[s.lane]);
                             }
                             __syncthreads();
                             pos = full ? 32 - __clz(sel) : 32;
@@ -670,14 +712,15 @@ hipError_t LaunchM8RunDist(hipStream_t st, const M8View& v, const float* rotq, c
     return hipGetLastError();
 }
 
-size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats) {
-    return M8ScanLayout(static_cast<uint32_t>(v.padded), v.plane_cols, v.nrun, k, v.rstride, lev_floats).total;
+size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats, uint32_t run_slot) {
+    return M8ScanLayout(static_cast<uint32_t>(v.padded), v.plane_cols, v.nrun, k, v.rstride, lev_floats, run_slot)
+        .total;
 }
 
 hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a) {
     const uint64_t nrun = a.qsel ? a.nsel : a.nq;
     if (nrun == 0) return hipSuccess;
-    const size_t lds = M8ScanLds(a.v, a.k, a.lev_floats);
+    const size_t lds = M8ScanLds(a.v, a.k, a.lev_floats, a.run_slot);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&M8ScanKernel),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
