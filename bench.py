@@ -924,7 +924,15 @@ def run_me8q_zipf(args):
     plain_qps = sum(r[0].shape[0] for r in sample) / plain_s
     plain_lat.sort()
     log(f"plain: {plain_qps:.0f} QPS ({plain_rps:.2f} requests/s), service p50 {plain_lat[len(plain_lat) // 2]:.0f} us")
-    srv = e8qlib.Server(idx, workers=args.serve_workers, max_batch=4096, mode="batched")
+    # each worker reserves an equal share of half the free HBM: the batched
+    # form's candidate pool takes what its batch leaves of the reservation,
+    # and a head-list query at nprobe 256 needs far more than the per-query
+    # default the minimal reservation assumes (queries past the pool would
+    # be answered by the plain form; server_stats.overflowed counts them)
+    free_b, _total_b = torch.cuda.mem_get_info()
+    ws = int(0.5 * free_b / args.serve_workers)
+    srv = e8qlib.Server(idx, workers=args.serve_workers, max_batch=4096, mode="batched", workspace_bytes=ws)
+    log(f"server: {args.serve_workers} workers, {ws / 1e9:.1f} GB workspace each")
     warm = [srv.submit(q, kk, npb) for q, kk, npb in reqs[:20]]
     for t in warm:
         srv.wait(t)

@@ -398,7 +398,8 @@ void E8qIndex::SearchBatch(const float* queries, size_t nq, size_t k, size_t npr
 }
 
 void E8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k,
-                               const uint32_t* nprobe_per_query, uint32_t* out_ids, float* out_dists) const {
+                               const uint32_t* nprobe_per_query, uint32_t* out_ids, float* out_dists,
+                               e8q_search_stats* stats) const {
     if (dev_ == nullptr) throw std::runtime_error("E8qIndex must be built or loaded before search");
     if (k == 0 || k > kMaxK) throw std::invalid_argument("k must be in [1, " + std::to_string(kMaxK) + "]");
     e8q_search_args a{};
@@ -413,7 +414,7 @@ void E8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, si
     a.mode = E8Q_MODE_PLAIN;
     a.out_ids = out_ids;
     a.out_dists = out_dists;
-    Check(e8q_search(ctx, dev_, &a, nullptr));
+    Check(e8q_search(ctx, dev_, &a, stats));
 }
 
 uint64_t E8qIndex::SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const {

@@ -16,6 +16,8 @@
 #include <utility>
 #include <vector>
 
+#include "e8q_gpu.h"
+
 struct e8q_ctx;
 struct e8q_dindex;
 
@@ -82,7 +84,7 @@ class E8qIndex {
     // worker's): no process-wide lock, so calls on different contexts run
     // concurrently.  Host queries and outputs, per-query nprobe.
     void SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k, const uint32_t* nprobe_per_query,
-                         uint32_t* out_ids, float* out_dists) const;
+                         uint32_t* out_ids, float* out_dists, e8q_search_stats* stats = nullptr) const;
     // Device scratch such a search carves from its context's workspace.
     [[nodiscard]] uint64_t SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const;
     // Top-k of the refine estimator over every stored vector (no pruning),

@@ -536,7 +536,7 @@ void Me8qIndex::SearchDevice(const float* queries, size_t nq, size_t k, size_t n
 
 void Me8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k,
                                 const uint32_t* nprobe_per_query, uint32_t* out_ids, float* out_dists,
-                                int mode) const {
+                                int mode, e8q_m8_stats* stats) const {
     if (!dev_) throw std::runtime_error("Me8qIndex must be built or loaded before search");
     if (k == 0 || k > kMaxDeviceK) throw std::invalid_argument("k must be in [1, 1024] on the device");
     e8q_m8_search_args a{};
@@ -551,7 +551,7 @@ void Me8qIndex::SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, s
     a.mode = mode;
     a.batched_budget = BatchedBudget();
     a.batched_set_a = BatchedSetA();
-    Check(e8q_m8_search(ctx, dev_, &a, nullptr, nullptr));
+    Check(e8q_m8_search(ctx, dev_, &a, stats, nullptr));
 }
 
 uint64_t Me8qIndex::SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe, int mode) const {

@@ -15,6 +15,8 @@
 #include <utility>
 #include <vector>
 
+#include "me8q_gpu.h"
+
 struct e8q_ctx;
 struct e8q_m8index;
 
@@ -125,7 +127,7 @@ class Me8qIndex {
     // Search on a caller-owned device context (a serving worker's): no
     // process-wide lock.  Host queries and outputs, per-query nprobe.
     void SearchOnContext(e8q_ctx* ctx, const float* queries, size_t nq, size_t k, const uint32_t* nprobe_per_query,
-                         uint32_t* out_ids, float* out_dists, int mode = 0) const;
+                         uint32_t* out_ids, float* out_dists, int mode = 0, e8q_m8_stats* stats = nullptr) const;
     [[nodiscard]] uint64_t SearchWorkspaceBytes(size_t nq, size_t k, size_t max_nprobe, int mode = 0) const;
 
     // The full-estimate paths (me8q.hpp:85-139): SearchIVF scores every vector

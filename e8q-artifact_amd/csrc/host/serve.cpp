@@ -117,6 +117,7 @@ void Server::WorkerLoop(int w) {
             off += r.nq;
         }
         std::exception_ptr err;
+        BatchReport rep;
         try {
             BatchView bv;
             bv.queries = q.data();
@@ -126,13 +127,16 @@ void Server::WorkerLoop(int w) {
             bv.max_nprobe = static_cast<uint32_t>(maxp);
             bv.ids = ids.data();
             bv.dists = dists.data();
-            be.Search(bv);
+            rep = be.Search(bv);
         } catch (...) {
             err = std::current_exception();
         }
         const double t_end = NowUs();
         {
             std::lock_guard<std::mutex> lk(mu_);
+            stats_.overflowed += rep.overflowed;
+            stats_.device_ms += rep.device_ms;
+            stats_.batch_ms += (t_end - t_start) / 1e3;
             off = 0;
             for (const Request& r : batch) {
                 Slot& s = slots_[r.ticket];
@@ -226,10 +230,19 @@ class DeviceBackend final : public Backend {
         else return ix_.SearchWorkspaceBytes(nq, k, max_nprobe);
     }
     void Reserve(uint64_t bytes) override { CheckSt(e8q_ctx_reserve(ctx_, bytes)); }
-    void Search(const BatchView& b) override {
-        if constexpr (std::is_same_v<Index, me8q::Me8qIndex>)
-            ix_.SearchOnContext(ctx_, b.queries, b.nq, b.k, b.nprobe, b.ids, b.dists, mode_);
-        else ix_.SearchOnContext(ctx_, b.queries, b.nq, b.k, b.nprobe, b.ids, b.dists);
+    BatchReport Search(const BatchView& b) override {
+        BatchReport r;
+        if constexpr (std::is_same_v<Index, me8q::Me8qIndex>) {
+            e8q_m8_stats st{};
+            ix_.SearchOnContext(ctx_, b.queries, b.nq, b.k, b.nprobe, b.ids, b.dists, mode_, &st);
+            r.overflowed = st.overflowed;
+            r.device_ms = st.total_ms;
+        } else {
+            e8q_search_stats st{};
+            ix_.SearchOnContext(ctx_, b.queries, b.nq, b.k, b.nprobe, b.ids, b.dists, &st);
+            r.device_ms = st.total_ms;
+        }
+        return r;
     }
 
    private:
@@ -264,7 +277,7 @@ class FakeBackend final : public Backend {
         return nq * (bpq_ + 8 * k + 4 * max_nprobe);
     }
     void Reserve(uint64_t) override {}
-    void Search(const BatchView& b) override {
+    BatchReport Search(const BatchView& b) override {
         if (delay_us_ > 0) std::this_thread::sleep_for(std::chrono::microseconds(static_cast<int64_t>(delay_us_)));
         for (size_t i = 0; i < b.nq; ++i) {
             const float x = b.queries[i * dim_];
@@ -275,6 +288,7 @@ class FakeBackend final : public Backend {
                 b.dists[i * b.k + j] = x + static_cast<float>(j);
             }
         }
+        return BatchReport{};
     }
 
    private:

@@ -38,6 +38,12 @@ struct BatchView {
     float* dists = nullptr;
 };
 
+// What one device search reports back to the server.
+struct BatchReport {
+    uint64_t overflowed = 0;   // queries the batched ME8Q form handed to the plain form
+    double device_ms = 0.0;    // device time of the whole search (HIP events)
+};
+
 // What a worker runs on: its own device context over a shared index.
 class Backend {
    public:
@@ -47,7 +53,7 @@ class Backend {
     virtual uint64_t WorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const = 0;
     // Grows the worker's workspace to `bytes` once, before serving.
     virtual void Reserve(uint64_t bytes) = 0;
-    virtual void Search(const BatchView& b) = 0;   // blocking
+    virtual BatchReport Search(const BatchView& b) = 0;   // blocking
 };
 
 struct Options {
@@ -80,6 +86,9 @@ struct Stats {
     uint64_t max_batch_queries_seen = 0;
     uint64_t max_workspace_used = 0;   // largest WorkspaceBytes of a formed batch
     uint64_t workspace_bytes = 0;      // per worker
+    uint64_t overflowed = 0;           // queries answered by the plain form after a pool overflow
+    double device_ms = 0.0;            // device time of every batch, summed
+    double batch_ms = 0.0;             // wall time of every batch's search call, summed
 };
 
 class Server {

@@ -100,6 +100,9 @@ class PyServer {
         d["max_batch_queries"] = st.max_batch_queries_seen;
         d["max_workspace_used"] = st.max_workspace_used;
         d["workspace_bytes"] = st.workspace_bytes;
+        d["overflowed"] = st.overflowed;
+        d["device_ms"] = st.device_ms;
+        d["batch_ms"] = st.batch_ms;
         return d;
     }
 

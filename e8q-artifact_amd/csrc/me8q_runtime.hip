@@ -29,6 +29,7 @@ struct e8q_m8index {
     std::vector<uint64_t> list_begin, batch_begin;
     std::vector<uint64_t> top_lists;   // [nlist+1]: vectors in the c largest lists
     uint64_t nbatch = 0;
+    uint64_t sum_nbt = 0, max_nbt = 0;   // batch tiles of the batched form, summed and largest over the lists
     // refine-row layout of the reference (for the diagnostics)
     std::vector<uint32_t> ref_run_off;
     uint32_t ref_stride = 0;
@@ -113,8 +114,9 @@ struct M8Bufs {
 //   min(2^31, the vectors the probed lists can hold, nq x per-query entries),
 // per-query entries = 65536 (E8Q_M8_POOL_PER_QUERY overrides it), raised
 // after a call that needed more (to 1.25 x its need) unless the context's
-// workspace was reserved (e8q_ctx_reserve): a reserved context caps the pool
-// to what fits the reservation and never grows.  The records hold as many as
+// workspace was reserved (e8q_ctx_reserve): a reserved context gives the
+// pool all of the reservation the rest of the call leaves (up to what the
+// probed lists can hold) and never grows.  The records hold as many as
 // the pool (E8Q_M8_RECORD_CAP lowers it; 0 = always the write pass).  Queries
 // whose candidates pass the pool's end are searched by the plain form.
 constexpr uint64_t kM8PoolMax = uint64_t{1} << 31;
@@ -141,15 +143,8 @@ bool M8BatchedFits(const e8q_m8index* x, uint64_t k) {
 // Work items of the batched form: an upper bound of their count (the list
 // count of each pair tile times the batch tiles of its list).
 uint64_t M8ItemsCap(const e8q_m8index* x, uint64_t nq, uint64_t pairs) {
-    uint64_t sum_nbt = 0, max_nbt = 0;
-    for (uint64_t c = 0; c < x->view.nlist; ++c) {
-        const uint64_t nb = (x->list_begin[c + 1] - x->list_begin[c] + 31) / 32;
-        const uint64_t nbt = (nb + kM8BatchBTile - 1) / kM8BatchBTile;
-        sum_nbt += nbt;
-        max_nbt = std::max(max_nbt, nbt);
-    }
     const uint64_t qt = kM8BatchQTile;
-    return std::min((nq + qt - 1) / qt * sum_nbt, (pairs + qt - 1) / qt * max_nbt + sum_nbt);
+    return std::min((nq + qt - 1) / qt * x->sum_nbt, (pairs + qt - 1) / qt * x->max_nbt + x->sum_nbt);
 }
 
 struct M8Caps {
@@ -310,7 +305,13 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         e8q_ctx::Carve cv;
         M8Carve(cv, b, v, nq, a->k, maxp, !a->queries_on_device, !a->outputs_on_device, a->nprobe_per_query != nullptr);
         if (batched) {
-            uint64_t pool = M8PoolWant(x, nq, maxp, ctx->reserved ? 0 : ctx->m8_pool_per_q);
+            // A reserved context offers its whole reservation to the pool (up
+            // to what the probed lists can hold): a small call in a
+            // workspace sized for a large one may have a query whose
+            // candidates alone pass the per-query default.
+            uint64_t pool = ctx->reserved
+                                ? std::min(kM8PoolMax, nq * x->top_lists[std::min<uint64_t>(maxp, v.nlist)])
+                                : M8PoolWant(x, nq, maxp, ctx->m8_pool_per_q);
             auto bytes_at = [&](uint64_t pl) {
                 e8q_ctx::Carve c2 = cv;
                 M8Bufs b2{};
@@ -923,6 +924,11 @@ e8q_status e8q_m8_create(e8q_ctx* ctx, const e8q_m8_desc* d, e8q_m8index** out) 
                 x->batch_begin[c + 1] = x->batch_begin[c] + (x->list_begin[c + 1] - x->list_begin[c] + 31) / 32;
             }
             x->nbatch = x->batch_begin[d->nlist];
+            for (uint64_t c = 0; c < d->nlist; ++c) {
+                const uint64_t nbt = (x->batch_begin[c + 1] - x->batch_begin[c] + kM8BatchBTile - 1) / kM8BatchBTile;
+                x->sum_nbt += nbt;
+                x->max_nbt = std::max(x->max_nbt, nbt);
+            }
             {   // the largest lists first: what c probed lists can hold at most
                 std::vector<uint64_t> len(d->nlist);
                 for (uint64_t c = 0; c < d->nlist; ++c) len[c] = x->list_begin[c + 1] - x->list_begin[c];

@@ -132,8 +132,9 @@ e8q_status e8q_m8_search(e8q_ctx* ctx, const e8q_m8index* idx, const e8q_m8_sear
  * lists hold, nq x 65536) entries of 28 bytes (pool, cascade list, lower
  * bound, record), plus the threshold pass's at the default budget.  A
  * context whose workspace was reserved (e8q_ctx_reserve) to at least this
- * size runs such a call without allocating; on a reserved context a call
- * never grows the pool past the reservation, and the queries whose
+ * size runs such a call without allocating; on a reserved context a call's
+ * pool is what the reservation leaves after the call's other buffers (up to
+ * what the probed lists can hold), never more, and the queries whose
  * candidates do not fit are searched by the plain form (stats.overflowed). */
 uint64_t e8q_m8_search_workspace_bytes(const e8q_m8index* idx, uint64_t nq, uint64_t k, uint64_t max_nprobe,
                                        int32_t mode);

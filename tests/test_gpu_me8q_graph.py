@@ -292,6 +292,30 @@ def test_reserved_workspace_is_never_grown(case, unpruned, half_pool, monkeypatc
         c2.close()
 
 
+def test_reserved_small_call_takes_the_whole_reservation(case, unpruned, half_pool, monkeypatch):
+    """A reservation sized for every query at a per-query default that holds
+    half of each one's candidates: a call of a few queries gets all of it for
+    its pool, so nothing overflows (a serving worker's small batch with a
+    heavy query is answered by the batched form, not the plain one)."""
+    _ix, f, qs, _ref = case
+    ref, _plain = unpruned
+    monkeypatch.setenv("E8Q_M8_POOL_PER_QUERY", str(half_pool))
+    c = capi.Ctx(0)
+    try:
+        ix = capi_m8.M8Index.from_file(c, f)
+        c.reserve(ix.workspace_bytes(qs.shape[0], K, NPROBE, BATCHED))
+        ws = c.workspace()
+        for n in (1, 4, 9):
+            print(f"me8q batched: {n} queries in a reservation for {qs.shape[0]}", flush=True)
+            r = ix.search(qs[:n], K, NPROBE, ZO, mode=BATCHED)
+            assert r[3].overflowed == 0 and c.workspace() == ws
+            _same(r, ref, slice(0, n))
+        assert c.m8_info()["pool_per_query"] == 0
+        ix.close()
+    finally:
+        c.close()
+
+
 def test_grid_validation(ctx):
     with pytest.raises(RuntimeError):
         ctx.set_m8_grids(70000, 0)
