@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -154,13 +156,19 @@ struct e8q_ctx {
     // Grids of the batched ME8Q form's persistent kernels (the MFMA
     // contraction, the exact refine): fixed by the caller (nonzero), else
     // chosen by timing candidates on the first batched search per (index
-    // shape, query bucket, k and nprobe buckets).
+    // shape, query bucket, k and nprobe buckets).  The table of tuned grids
+    // is the context's own unless it shares another's (e8q_ctx_share_tuning:
+    // a server's workers tune each bucket once between them).
     uint32_t m8_fix_mfma = 0, m8_fix_refine = 0;
     struct M8Tuned {
         uint64_t shape[6];
         uint32_t mfma_grid, refine_grid;
     };
-    std::vector<M8Tuned> m8_tuned;
+    struct M8TuneTable {
+        std::mutex mu;
+        std::vector<M8Tuned> v;
+    };
+    std::shared_ptr<M8TuneTable> m8_table = std::make_shared<M8TuneTable>();
     uint32_t m8_last_mfma = 0, m8_last_refine = 0, m8_last_budget = 0, m8_last_set_a = 0;
     uint64_t m8_tunings = 0;
 

@@ -144,46 +144,66 @@ struct Rec {
     f32x4 s1;                    // sc[4..7]
 };
 
-// This lane's byte offsets inside a step's records: its rowA bytes, its
-// bank bytes and its batch header's scales (lane half j reads batch j of
-// the step).  The loads add small constants as their scalar offsets.
+// This lane's byte offsets inside its half's batch record: its rowA bytes
+// and its bank bytes (the batch header's scales sit at fixed offsets).
 struct LaneOffs {
-    uint32_t row, hdr, hs96;
+    uint32_t row, hdr;
 };
 
-// The step's records: `nrec` records of `bb` bytes from `rec_p`, the first
-// batch's record (wave-uniform), read through a buffer descriptor with these
-// per-lane offsets, so a step spends no vector instructions on addresses.
-// The second batch may lie past the cluster (then nrec is 1): such loads
-// return zeros, and those lanes do not use them.
+// The step's records: batch record `rec_p` (wave-uniform) for lanes 0..31
+// and the one `dj` bytes further for lanes 32..63 (dj: this lane's half's
+// displacement, zero in the lower half), `nb` bytes from rec_p in all, read
+// through a buffer descriptor.  A step of one batch points the upper half
+// past nb: its loads return zeros, and those lanes do not use them.
 template <int NQW>
-__device__ __forceinline__ void LoadRec(Rec<NQW>& r, const uint8_t* rec_p, uint32_t nrec, uint32_t bb,
+__device__ __forceinline__ void LoadRec(Rec<NQW>& r, const uint8_t* rec_p, uint32_t nb, uint32_t dj,
                                         const LaneOffs& o) {
-    const uint32_t nb = __umulhi(nrec, bb) != 0 ? 0xFFFFFFFFu : nrec * bb;   // byte count, saturated
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(UniformPtr(rec_p)), 0,
                                                                         static_cast<int>(nb), kBufferDword3);
+    const uint32_t row = o.row + dj, hdr = o.hdr + dj;
     if constexpr (NQW > 0) {
         if constexpr (NQW % 2 == 0) {
 #pragma unroll
             for (int w = 0; w < NQW; w += 2) {
-                const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, o.row, 8 * w, 0);
+                const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, row, 8 * w, 0);
                 r.w[w] = u32x2{v.x, v.y};
                 r.w[w + 1] = u32x2{v.z, v.w};
             }
         } else {
 #pragma unroll
-            for (int w = 0; w < NQW; ++w) r.w[w] = __builtin_amdgcn_raw_buffer_load_b64(rs, o.row, 8 * w, 0);
+            for (int w = 0; w < NQW; ++w) r.w[w] = __builtin_amdgcn_raw_buffer_load_b64(rs, row, 8 * w, 0);
         }
     } else {
-        r.rp = rec_p + o.row;
+        r.rp = rec_p + row;
     }
-    r.hb = __builtin_amdgcn_raw_buffer_load_b8(rs, o.hdr, 0, 0);
-    r.hw = __builtin_amdgcn_raw_buffer_load_b8(rs, o.hdr, 32, 0);
-    r.hp = __builtin_amdgcn_raw_buffer_load_b8(rs, o.hdr, 64, 0);
-    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, o.hs96, 0, 0);
-    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, o.hs96, 16, 0);
+    r.hb = __builtin_amdgcn_raw_buffer_load_b8(rs, hdr, 0, 0);
+    r.hw = __builtin_amdgcn_raw_buffer_load_b8(rs, hdr, 32, 0);
+    r.hp = __builtin_amdgcn_raw_buffer_load_b8(rs, hdr, 64, 0);
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, dj, 96, 0);
+    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, dj, 112, 0);
     r.s0 = f32x4{__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w)};
     r.s1 = f32x4{__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w)};
+}
+
+// The batch header scales the U test reads (sc[0], sc[3..6]) of the batch
+// this lane stands for in a 64-batch window.
+struct WinScales {
+    float sc0;
+    f32x4 s36;   // sc[3..6]
+};
+
+// Loads the scales of batches w0 + lane of a cluster's records at `rec_c`
+// with `nbat` batches (lanes past them read zeros through the descriptor's
+// range check); `woff` is lane x batch bytes.
+__device__ __forceinline__ void LoadWin(WinScales& s, const uint8_t* rec_c, uint32_t w0, uint32_t nbat, uint32_t bb,
+                                        uint32_t woff) {
+    const uint32_t n = min(nbat - w0, 64u);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t*>(UniformPtr(rec_c + static_cast<uint64_t>(w0) * bb)), 0, static_cast<int>(n * bb),
+        kBufferDword3);
+    s.sc0 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, woff, 96, 0));
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, woff, 108, 0);
+    s.s36 = f32x4{__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w)};
 }
 
 template <int NQW>
@@ -236,8 +256,9 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
     // All ones in the upper half-wave (kept in a vector register).
     uint32_t jmask = 0u - j;
     asm volatile("" : "+v"(jmask));
-    const LaneOffs lo{j * Uniform(ix.batch_bytes) + kHdr + t * static_cast<uint32_t>(ix.blocks),
-                      j * Uniform(ix.batch_bytes) + t, j * Uniform(ix.batch_bytes) + 96};
+    const LaneOffs lo{kHdr + t * static_cast<uint32_t>(ix.blocks), t};
+    // byte offset of batch `lane` of a window of records
+    const uint32_t woff = static_cast<uint32_t>(lane) * Uniform(ix.batch_bytes);
     const int k = static_cast<int>(a.k);
     const int fe = a.flush_every;
     const float inf = __builtin_inff();
@@ -272,9 +293,8 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
         uint32_t n_surv = 0, n_scan = 0, lane_cand = 0, lane_nb = 0;
         ProbeChunk ch;
         LoadChunk(ch, 0, ix, a, qi, np, limit, sk, lane_cand, lane_nb);
-        uint32_t cp = Settle(ch, NextInChunk(ch, 0, np), ix, a, qi, np, limit, sk, lane_cand, lane_nb);
-        // The open cluster.
-        uint32_t ccnt = 0, cnb = 0, cb0 = 0;
+        // The open cluster (the one whose batches are being scored).
+        uint32_t cp = np, ccnt = 0, cnb = 0;
         float d2 = 0.0F, sqrt_d2 = 0.0F, zrh = 0.0F;
         const uint8_t* arena_c = nullptr;
         auto open = [&](uint32_t p) {
@@ -292,18 +312,142 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             return ClusterRows{arena_c, ix.ref_arena + ReadLane64(ch.ro_lo, ch.ro_hi, l),
                                ReadLane64(ch.vb_lo, ch.vb_hi, l), d2, stg};
         };
+        // The batch cursor walks the active probes' batches one 64-batch
+        // window at a time.  A window's mask keeps the batches that the U
+        // test (the step's first test, on the batch header alone) does not
+        // rule out at the threshold of the moment: a batch it rules out is
+        // skipped by the reference at that threshold and at every lower one,
+        // so its records are never read.  Steps take the mask's batches two
+        // at a time, in order; the scales of the window after the cursor's
+        // are loaded a window ahead.
+        uint32_t kp = np, kw0 = 0, kcnb = 0;   // the cursor's window: probe, first batch, the probe's batches
+        uint64_t kwm = 0;                      // its batches not handed out yet
+        uint32_t pfp = np, pfw0 = 0;           // the prefetched window (pfp == np: none)
+        WinScales pf;
+        auto nbat = [&](uint32_t p) { return (ReadLane(ch.cnt, p - ch.base) + 31) / 32; };
+        auto rec_of = [&](uint32_t p) { return arena + ReadLane64(ch.ao_lo, ch.ao_hi, p - ch.base); };
+        auto window_mask = [&](const WinScales& s, uint32_t p, uint32_t w0, uint32_t nb) -> uint64_t {
+            bool need = LaneHere() < nb - w0;
+            if (prune && limit < inf) {
+                const float d2w = __uint_as_float(ReadLane(__float_as_uint(ch.d2), p - ch.base));
+                const float zr = z * (limit < d2w ? sqrtf(fmaxf(0.0F, limit)) : sqrtf(fmaxf(0.0F, d2w)));
+                const float sc3 = s.s36.x, sc4 = s.s36.y, sc5 = s.s36.z, sc6 = s.s36.w;
+                const float pad_w_round = (0.5F * sc3) * sml;
+                const float pad_add = fmaf(lut_err, sc6, pad_w_round);
+                const float lbp = (s.sc0 + d2w) - pad_add;
+                const float pam = fmaf(sc5, 255.0F, sc4);
+                need = need && !(fmaf(-pam, zr, fmaf(-sc6, sk.U, lbp)) >= limit);
+            }
+            return __ballot(need);
+        };
+        // The window after (p, w0): the probe's next 64 batches, else the
+        // first batches of the next active probe (np: none; past the chunk:
+        // base + 64).
+        auto after = [&](uint32_t p, uint32_t w0, uint32_t nb, uint32_t& ap, uint32_t& aw0) {
+            if (w0 + 64 < nb) {
+                ap = p;
+                aw0 = w0 + 64;
+            } else {
+                ap = NextInChunk(ch, p + 1, np);
+                aw0 = 0;
+            }
+        };
+        // The cursor enters window (p, w0) of the loaded chunk: its mask
+        // (from the prefetched scales when they are this window's), then the
+        // next window's scales are loaded.
+        auto enter = [&](uint32_t p, uint32_t w0) {
+            kp = Uniform(p);
+            kw0 = Uniform(w0);
+            kcnb = Uniform(nbat(kp));
+            if (kp == pfp && kw0 == pfw0) {
+                kwm = window_mask(pf, kp, kw0, kcnb);
+            } else {
+                WinScales s;
+                LoadWin(s, rec_of(kp), kw0, kcnb, BB, woff);
+                kwm = window_mask(s, kp, kw0, kcnb);
+            }
+            kwm = Uniform64(kwm);
+            uint32_t bp, bw0;
+            after(kp, kw0, kcnb, bp, bw0);
+            pfp = np;
+            if (bp < np && bp - ch.base < 64) {
+                LoadWin(pf, rec_of(bp), bw0, nbat(bp), BB, woff);
+                pfp = bp;
+                pfw0 = bw0;
+            }
+        };
+        // The next step: probe xp, batches xa and xb (xjn of them).  0: a
+        // step; 1: no batch is left; 2: the next window lies past the
+        // loaded chunk (the cursor waits until the open cluster is done).
+        auto cursor = [&](uint32_t& xp, uint32_t& xa, uint32_t& xb, uint32_t& xjn) -> int {
+            while (kwm == 0) {
+                if (kp >= np) return 1;
+                uint32_t ap, aw0;
+                after(kp, kw0, kcnb, ap, aw0);
+                if (ap >= np) return 1;
+                if (ap - ch.base >= 64) return 2;
+                enter(ap, aw0);
+            }
+            xp = kp;
+            xa = kw0 + static_cast<uint32_t>(__builtin_ctzll(kwm));
+            kwm &= kwm - 1;
+            if (kwm != 0) {
+                xb = kw0 + static_cast<uint32_t>(__builtin_ctzll(kwm));
+                kwm &= kwm - 1;
+                xjn = 2;
+            } else {
+                xb = xa;
+                xjn = 1;
+            }
+            return 0;
+        };
+        // The same, loading chunks when the cursor has passed the loaded one
+        // (only when no cluster of it is open).
+        auto cursor_settled = [&](uint32_t& xp, uint32_t& xa, uint32_t& xb, uint32_t& xjn) -> bool {
+            for (;;) {
+                const int r = cursor(xp, xa, xb, xjn);
+                if (r == 0) return true;
+                if (r == 1) return false;
+                const uint32_t p = Settle(ch, ch.base + 64, ix, a, qi, np, limit, sk, lane_cand, lane_nb);
+                ch.base = Uniform(ch.base);
+                ch.active = Uniform64(ch.active);
+                ch.nonempty = Uniform64(ch.nonempty);
+                if (p >= np) {
+                    kp = np;
+                    kwm = 0;
+                    return false;
+                }
+                pfp = np;
+                enter(p, 0);
+            }
+        };
+        // Records of step (xp, xa, xb, xjn) into r.
+        auto load_step = [&](Rec<NQW>& r, uint32_t xp, uint32_t xa, uint32_t xb, uint32_t xjn) {
+            const uint32_t span = xjn == 2 ? xb - xa : 1u;
+            LoadRec<NQW>(r, rec_of(xp) + static_cast<uint64_t>(xa) * BB, (xjn == 2 ? span + 1 : 1u) * BB,
+                         jmask & (span * BB), lo);
+        };
+        // The step being scored: probe cp, batches sa and sb (sjn of them).
+        uint32_t sa = 0, sb = 0, sjn = 0;
         // Two record sets: a step scores one while the next step's records
         // load into the other (the loop body runs twice per iteration with
         // the roles swapped, so no set is ever copied).
         Rec<NQW> ra, rb;
-        cp = Uniform(cp);
-        ch.base = Uniform(ch.base);
-        ch.active = Uniform64(ch.active);
-        ch.nonempty = Uniform64(ch.nonempty);
-        if (cp < np) {
-            open(cp);
-            horizon();
-            LoadRec<NQW>(ra, arena_c, cnb, BB, lo);
+        {
+            const uint32_t p0 = Settle(ch, NextInChunk(ch, 0, np), ix, a, qi, np, limit, sk, lane_cand, lane_nb);
+            ch.base = Uniform(ch.base);
+            ch.active = Uniform64(ch.active);
+            ch.nonempty = Uniform64(ch.nonempty);
+            if (p0 < np) {
+                enter(p0, 0);
+                uint32_t xp = np;
+                if (cursor_settled(xp, sa, sb, sjn)) {
+                    cp = xp;
+                    open(cp);
+                    horizon();
+                    load_step(ra, cp, sa, sb, sjn);
+                }
+            }
         }
         // One step on rc (its records loaded), prefetching into rn; false
         // when the query's probes are done.
@@ -313,7 +457,9 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             // uniformity analysis gives up after the per-lane merge loops).
             cp = Uniform(cp);
             if (cp >= np) return false;
-            cb0 = Uniform(cb0);
+            sa = Uniform(sa);
+            sb = Uniform(sb);
+            sjn = Uniform(sjn);
             cnb = Uniform(cnb);
             ccnt = Uniform(ccnt);
             staged = static_cast<int>(Uniform(static_cast<uint32_t>(staged)));
@@ -326,41 +472,32 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             ch.base = Uniform(ch.base);
             ch.active = Uniform64(ch.active);
             ch.nonempty = Uniform64(ch.nonempty);
+            kp = Uniform(kp);
+            kw0 = Uniform(kw0);
+            kcnb = Uniform(kcnb);
+            kwm = Uniform64(kwm);
+            pfp = Uniform(pfp);
+            pfw0 = Uniform(pfw0);
             n_surv = Uniform(n_surv);
             n_scan = Uniform(n_scan);
-            // The next step; its records are loaded now when its probe is in
-            // the loaded chunk (otherwise after this step, see below).  Every
-            // lane loads, so that the number of loads in flight is the same
-            // on every path and a step waits only for its own records: when
-            // the next records come after the step this step's are loaded
-            // in their place.
-            uint32_t xp = cp, xb0 = cb0 + 2;
-            if (xb0 >= cnb) {
-                xp = NextInChunk(ch, cp + 1, np);
-                xb0 = 0;
-            }
-            const bool x_in = xp < np && xp - ch.base < 64;
-            {
-                const uint8_t* xa = arena_c;
-                uint32_t xnb = cnb, xb = cb0;
-                if (x_in) {
-                    xb = xb0;
-                    if (xp != cp) {
-                        const uint32_t l = xp - ch.base;
-                        xa = arena + ReadLane64(ch.ao_lo, ch.ao_hi, l);
-                        xnb = (ReadLane(ch.cnt, l) + 31) / 32;
-                    }
-                }
-                LoadRec<NQW>(rn, xa + static_cast<uint64_t>(xb) * BB, xnb - xb, BB, lo);
-            }
+            // The next step; its records are loaded now when the cursor has
+            // it in the loaded chunk (otherwise after this step, see below).
+            // Every lane loads, so that the number of loads in flight is the
+            // same on every path and a step waits only for its own records:
+            // when the next records come after the step this step's are
+            // loaded in their place.
+            uint32_t xp = np, xa = 0, xb = 0, xjn = 0;
+            const bool x_in = cursor(xp, xa, xb, xjn) == 0;
+            if (x_in) load_step(rn, xp, xa, xb, xjn);
+            else load_step(rn, cp, sa, sb, sjn);
 
             // The record's unused header word (sc[7]) stays live until here,
             // so that no temporary takes its register while the next
             // records' loads are in flight (that would wait for them).
             asm volatile("" ::"v"(rc.s1.w));
-            const uint32_t jn = min(2u, cnb - cb0);
+            const uint32_t jn = sjn;
             n_scan += jn;
-            const uint32_t bj = cb0 + j;
+            const uint32_t bj = sa + (jmask & (sb - sa));
             const bool act = j < jn;
             const float L = limit;
             // Header terms of the ZPR bound.  Batch bound:
@@ -401,8 +538,8 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             // their bit patterns).
             const uint32_t hm = HalfMax(sabs);
             const uint32_t h31 = ReadLane(hm, 31);
-            const float sa = __uint_as_float(h31 ^ ((h31 ^ ReadLane(hm, 63)) & jmask));
-            if (act) lbp = fmaf(-rc.s1.z, sa, lbp);
+            const float sa_ = __uint_as_float(h31 ^ ((h31 ^ ReadLane(hm, 63)) & jmask));
+            if (act) lbp = fmaf(-rc.s1.z, sa_, lbp);
             bool skip = false;
             if (prune && L < inf) skip = fmaf(-pam, zrh, lbp) >= L;
             const float dlo = fmaf(-zrh, pp, t1);
@@ -432,7 +569,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                 __syncthreads();
                 const ClusterRows cr = cluster_ref();
                 for (uint32_t jj = 0; jj < jn; ++jj) {
-                    const uint32_t off = (cb0 + jj) * 32;
+                    const uint32_t off = (jj == 0 ? sa : sb) * 32;
                     const uint32_t nl = min(32u, ccnt - off);
                     const float limit0 = limit;
                     const float zrh0 = z * (limit0 < d2 ? sqrtf(fmaxf(0.0F, limit0)) : sqrt_d2);
@@ -479,33 +616,43 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
                 horizon();
             }
             }
-            if (cb0 + 2 >= cnb && staged > 0) {   // end of the cluster
+            // The open cluster ends with this step when the next one lies in
+            // another cluster (or there is none): its staged rows are refined.
+            if ((!x_in || xp != cp) && staged > 0) {
                 __syncthreads();
                 FlushRows<GEN>(S, staged, cnt, limit, &prm->ix, k, qv, cluster_ref(), rowbuf);
                 Refresh(ch, limit, sk);
                 staged = 0;
                 horizon();
             }
-            // The records of the next step are reloaded here when its probe
-            // lies beyond the loaded chunk, or when a flush has just made its
-            // whole cluster skippable (then the next active probe is taken).
+            // The next step's records are loaded here when the cursor had to
+            // wait for the next chunk, or when a flush has just made the next
+            // step's whole cluster skippable (the cursor then passes over the
+            // rest of that cluster).
             bool reload = !x_in;
             if (x_in && xp != cp && !((ch.active >> (xp - ch.base)) & 1u)) {
-                xp = NextInChunk(ch, xp + 1, np);
+                kp = xp;
+                kcnb = nbat(xp);
+                kw0 = kcnb;   // no further window of this probe
+                kwm = 0;
                 reload = true;
             }
-            if (xp >= np) return false;
-            if (xp - ch.base >= 64) {
-                xp = Settle(ch, xp, ix, a, qi, np, limit, sk, lane_cand, lane_nb);
-                if (xp >= np) return false;
+            if (reload) {
+                if (!cursor_settled(xp, xa, xb, xjn)) {
+                    cp = np;
+                    return false;
+                }
             }
             if (xp != cp) {
-                open(xp);
+                cp = xp;
+                open(cp);
                 horizon();
             }
-            if (reload) LoadRec<NQW>(rn, arena_c, cnb, BB, lo);
-            cp = Uniform(xp);
-            cb0 = Uniform(xb0);
+            if (reload) load_step(rn, cp, xa, xb, xjn);
+            cp = Uniform(cp);
+            sa = Uniform(xa);
+            sb = Uniform(xb);
+            sjn = Uniform(xjn);
             cnb = Uniform(cnb);
             ccnt = Uniform(ccnt);
             staged = static_cast<int>(Uniform(static_cast<uint32_t>(staged)));
@@ -518,6 +665,12 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             ch.base = Uniform(ch.base);
             ch.active = Uniform64(ch.active);
             ch.nonempty = Uniform64(ch.nonempty);
+            kp = Uniform(kp);
+            kw0 = Uniform(kw0);
+            kcnb = Uniform(kcnb);
+            kwm = Uniform64(kwm);
+            pfp = Uniform(pfp);
+            pfw0 = Uniform(pfw0);
             n_surv = Uniform(n_surv);
             n_scan = Uniform(n_scan);
             return true;

@@ -28,6 +28,7 @@ Server::Server(std::vector<std::unique_ptr<Backend>> workers, const Options& opt
                     ? opt_.workspace_bytes
                     : workers_[0]->WorkspaceBytes(opt_.max_batch_queries, opt_.max_k, opt_.max_nprobe);
     for (auto& w : workers_) w->Reserve(ws_bytes_);
+    for (size_t i = 1; i < workers_.size(); ++i) workers_[i]->ShareTuning(*workers_[0]);
     stats_.workspace_bytes = ws_bytes_;
     for (size_t i = 0; i < workers_.size(); ++i) threads_.emplace_back([this, i] { WorkerLoop(static_cast<int>(i)); });
 }
@@ -230,6 +231,9 @@ class DeviceBackend final : public Backend {
         else return ix_.SearchWorkspaceBytes(nq, k, max_nprobe);
     }
     void Reserve(uint64_t bytes) override { CheckSt(e8q_ctx_reserve(ctx_, bytes)); }
+    void ShareTuning(Backend& first) override {
+        CheckSt(e8q_ctx_share_tuning(ctx_, static_cast<DeviceBackend&>(first).ctx_));
+    }
     BatchReport Search(const BatchView& b) override {
         BatchReport r;
         if constexpr (std::is_same_v<Index, me8q::Me8qIndex>) {

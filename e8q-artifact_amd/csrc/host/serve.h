@@ -53,6 +53,9 @@ class Backend {
     virtual uint64_t WorkspaceBytes(size_t nq, size_t k, size_t max_nprobe) const = 0;
     // Grows the worker's workspace to `bytes` once, before serving.
     virtual void Reserve(uint64_t bytes) = 0;
+    // From now on tuned launch shapes are shared with `first` (a worker of
+    // the same server).
+    virtual void ShareTuning(Backend& first) { (void)first; }
     virtual BatchReport Search(const BatchView& b) = 0;   // blocking
 };
 

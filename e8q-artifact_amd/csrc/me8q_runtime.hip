@@ -475,12 +475,15 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
     bool tune = false;
     if (batched && (ctx->m8_fix_mfma == 0 || ctx->m8_fix_refine == 0)) {
         bool found = false;
-        for (const auto& t : ctx->m8_tuned)
-            if (std::equal(t.shape, t.shape + 6, tkey)) {
-                if (!ctx->m8_fix_mfma) grids.mfma = t.mfma_grid;
-                if (!ctx->m8_fix_refine) grids.refine = t.refine_grid;
-                found = true;
-            }
+        {
+            std::lock_guard<std::mutex> lk(ctx->m8_table->mu);
+            for (const auto& t : ctx->m8_table->v)
+                if (std::equal(t.shape, t.shape + 6, tkey)) {
+                    if (!ctx->m8_fix_mfma) grids.mfma = t.mfma_grid;
+                    if (!ctx->m8_fix_refine) grids.refine = t.refine_grid;
+                    found = true;
+                }
+        }
         tune = !found && nq >= kM8TuneMinQueries && chk == nullptr;
     }
 
@@ -719,7 +722,11 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
                         best = c;
                     }
                 }
-            ctx->m8_tuned.push_back({{tkey[0], tkey[1], tkey[2], tkey[3], tkey[4], tkey[5]}, best.mfma, best.refine});
+            {
+                std::lock_guard<std::mutex> lk(ctx->m8_table->mu);
+                ctx->m8_table->v.push_back(
+                    {{tkey[0], tkey[1], tkey[2], tkey[3], tkey[4], tkey[5]}, best.mfma, best.refine});
+            }
             ++ctx->m8_tunings;
             grids = best;
         }
@@ -1113,6 +1120,14 @@ e8q_status e8q_ctx_set_m8_grids(e8q_ctx* ctx, uint32_t mfma_grid, uint32_t refin
         if (mfma_grid > 65536 || refine_grid > 65536) Throw(E8Q_EINVAL, "grids must be <= 65536");
         ctx->m8_fix_mfma = mfma_grid;
         ctx->m8_fix_refine = refine_grid;
+    });
+}
+
+e8q_status e8q_ctx_share_tuning(e8q_ctx* ctx, const e8q_ctx* from) {
+    return Guard([&] {
+        if (!ctx || !from) Throw(E8Q_EINVAL, "null argument");
+        if (ctx->device != from->device) Throw(E8Q_EINVAL, "contexts on different devices");
+        ctx->m8_table = from->m8_table;
     });
 }
 

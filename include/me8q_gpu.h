@@ -156,6 +156,11 @@ typedef struct {
     uint64_t pool_per_query;           /* pool entries per query learned from past calls (0: default) */
 } e8q_m8_ctx_info;
 e8q_status e8q_ctx_get_m8_info(const e8q_ctx* ctx, e8q_m8_ctx_info* out);
+/* Makes ctx use the autotuned-grid table of `from` (a context on the same
+ * device) from now on, so that a bucket either one tunes serves both.  No
+ * reference counterpart: for the worker contexts of a server, which would
+ * otherwise each time the same buckets. */
+e8q_status e8q_ctx_share_tuning(e8q_ctx* ctx, const e8q_ctx* from);
 
 /* ---- diagnostics (me8q_measure.cpp:41-67, me8q_search.cpp:136-266) -----
  * Host pointers.  Each value is the reference's bit for bit. */

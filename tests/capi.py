@@ -122,6 +122,9 @@ class Ctx:
     def reserve(self, nbytes):
         check(lib().e8q_ctx_reserve(self.h, ctypes.c_uint64(int(nbytes))))
 
+    def share_tuning(self, other):
+        check(lib().e8q_ctx_share_tuning(self.h, other.h))
+
     def workspace(self):
         return int(lib().e8q_ctx_workspace_bytes(self.h))
 

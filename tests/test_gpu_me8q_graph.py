@@ -106,6 +106,33 @@ def test_autotuner_once_per_bucket(case):
         c.close()
 
 
+def test_shared_tuning_table(case):
+    """Two contexts sharing one table of tuned grids: a bucket the first
+    tunes is not tuned again by the second, and both give the same results."""
+    _ix, f, qs, ref = case
+    c1, c2 = capi.Ctx(0), capi.Ctx(0)
+    try:
+        c2.share_tuning(c1)
+        ix1 = capi_m8.M8Index.from_file(c1, f)
+        ix2 = capi_m8.M8Index.from_file(c2, f)
+        print("me8q shared tuning: first context tunes", flush=True)
+        _same(ix1.search(qs, K, NPROBE, _z(K, 0.0), mode=BATCHED), ref)
+        assert c1.m8_info()["tunings"] == 1
+        print("me8q shared tuning: second context reuses", flush=True)
+        _same(ix2.search(qs, K, NPROBE, _z(K, 0.0), mode=BATCHED), ref)
+        assert c2.m8_info()["tunings"] == 0
+        assert (c2.m8_info()["mfma_grid"], c2.m8_info()["refine_grid"]) == \
+            (c1.m8_info()["mfma_grid"], c1.m8_info()["refine_grid"])
+        r = ix2.search(qs[:100], K, NPROBE, _z(K, 0.0), mode=BATCHED)   # a bucket neither has seen
+        assert c2.m8_info()["tunings"] == 1
+        _same(r, ref, slice(0, 100))
+        ix1.close()
+        ix2.close()
+    finally:
+        c1.close()
+        c2.close()
+
+
 def test_graph_replay_matches_eager(ctx, case):
     ix, _f, qs, _ref = case
     rng = np.random.default_rng(5)
