@@ -109,6 +109,8 @@ __device__ __forceinline__ void LoadChunk(ProbeChunk& c, uint32_t base, const In
         lane_nb += (c.cnt + 31) / 32;
     }
     c.nonempty = __ballot(c.cnt > 0);
+    asm volatile("" ::"v"(c.vb_lo), "v"(c.vb_hi), "v"(c.ao_lo), "v"(c.ao_hi), "v"(c.ro_lo), "v"(c.ro_hi), "v"(c.d2),
+                 "v"(c.sqd2), "v"(c.lbpc), "v"(c.pamx));
     Refresh(c, L, k);
 }
 
@@ -488,8 +490,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
             // loaded in their place.
             uint32_t xp = np, xa = 0, xb = 0, xjn = 0;
             const bool x_in = cursor(xp, xa, xb, xjn) == 0;
-            if (x_in) load_step(rn, xp, xa, xb, xjn);
-            else load_step(rn, cp, sa, sb, sjn);
+            load_step(rn, x_in ? xp : cp, x_in ? xa : sa, x_in ? xb : sb, x_in ? xjn : sjn);
 
             // The record's unused header word (sc[7]) stays live until here,
             // so that no temporary takes its register while the next
