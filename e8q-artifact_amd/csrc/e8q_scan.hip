@@ -32,10 +32,11 @@ __host__ __device__ inline uint32_t PlainRowBufBytes(uint32_t blocks, int bits) 
     return (kGroup * blocks * static_cast<uint32_t>(bits) + 16 + 15) & ~15u;
 }
 
-template <int NQW>
+// The specialized kernels (rates 2..5) hold only the levels they read.
+template <int NQW, bool GEN>
 struct PlainLds {
     uint8_t lut[NQW > 0 ? NQW * 256 : 4];
-    PlainShared s;
+    PlainSharedT<GEN ? 512 : 64> s;
 };
 
 // Per-query inputs of the whole-cluster skip test.
@@ -243,8 +244,8 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
     const PlainParams* __restrict__ prm) {
     const IndexView& ix = prm->ix;
     const PlainArgs& a = prm->a;
-    __shared__ PlainLds<NQW> sh;
-    PlainShared& S = sh.s;
+    __shared__ PlainLds<NQW, GEN> sh;
+    auto& S = sh.s;
     extern __shared__ __align__(16) uint8_t dyn[];
     uint8_t* lutc = NQW > 0 ? sh.lut : dyn;
     float* qv = reinterpret_cast<float*>(dyn + (NQW > 0 ? 0 : static_cast<size_t>(ix.blocks) * 32));  // padded
@@ -252,8 +253,7 @@ __global__ __launch_bounds__(kWave, (PlainWaves<NQW, GEN>())) void PlainSearchKe
     // staged rows (vector offsets inside the cluster), flush_every of them
     uint32_t* stg = reinterpret_cast<uint32_t*>(rowbuf + PlainRowBufBytes(ix.blocks, ix.bits));
     const int lane = threadIdx.x;
-    if (ix.bits <= 8)
-        for (int i = lane; i < 512; i += kWave) S.tab[i] = G(ix.refine_tab)[i];
+    LoadRefineTab(S, ix, lane, kWave);
     const uint32_t j = lane >> 5, t = lane & 31;
     // All ones in the upper half-wave (kept in a vector register).
     uint32_t jmask = 0u - j;

@@ -123,9 +123,13 @@ __device__ __forceinline__ void DecodeDigit(int bits, uint8_t rb, const uint8_t*
     }
 }
 
-// Fixed-size LDS state of one query.
-struct PlainShared {
-    float tab[512];              // refine level table (bits <= 8)
+// Fixed-size LDS state of one query.  TAB: floats of the refine level
+// table held in LDS, both branches: 512 (bits <= 8, the device table's
+// 256-entry branch stride), or 64 (bits <= 5: each branch's 2^bits levels,
+// branch stride 2^bits).
+template <int TAB>
+struct PlainSharedT {
+    float tab[TAB];              // refine level table
     float t1[kWave];             // exact replay: ZPR bound minus the slack term
     float pp[kWave];             //               perp (Omega) per lane
     float lbp[2], pam[2];        //               batch bound parts
@@ -142,6 +146,19 @@ struct PlainShared {
     const float* pg[kGroup];     // rows' g1 (h3 follows 32 floats later)
     const uint32_t* pid[kGroup]; // rows' ids
 };
+using PlainShared = PlainSharedT<512>;
+
+// Copies the refine level table into S.tab (the whole workgroup).
+template <int TAB>
+__device__ __forceinline__ void LoadRefineTab(PlainSharedT<TAB>& S, const IndexView& ix, int tid, int nthreads) {
+    if (ix.bits > 8) return;
+    if constexpr (TAB == 512) {
+        for (int i = tid; i < 512; i += nthreads) S.tab[i] = G(ix.refine_tab)[i];
+    } else {
+        const int d = 1 << ix.bits;   // bits <= 5
+        for (int i = tid; i < 2 * d; i += nthreads) S.tab[i] = G(ix.refine_tab)[(i / d) * ix.tab_stride + i % d];
+    }
+}
 
 // Eight blocks of coarse sum: bytes of x.x are blocks 0..3, of x.y 4..7; the
 // LUT holds per block 16 low-nibble then 16 high-nibble entries.
@@ -184,11 +201,13 @@ struct ClusterRows {
 // then 8 rows x 8 components per pass decode and accumulate in the AVX2
 // order (even blocks in acc, odd in acc2, then the 8-lane horizontal sum).
 // Only rows that beat the current k-th entry take part in the merge.
-template <bool GEN, class Rows>
-__device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, const IndexView* ixp, int k,
+template <bool GEN, int TAB, class Rows>
+__device__ void FlushRows(PlainSharedT<TAB>& S, int ns, int& cnt, float& limit, const IndexView* ixp, int k,
                           const float* qv, const Rows& rows, uint8_t* rowbuf) {
     const IndexView& ix = *Here(ixp);
     const bool tab_lds = ix.bits <= 8;
+    // branch stride of the LDS table
+    const uint32_t tstride = TAB == 512 ? ix.tab_stride : (1u << ix.bits);
     const int lane = static_cast<int>(LaneHere());
     const int B = ix.blocks;
     const int c = ix.bits - 1;
@@ -246,35 +265,43 @@ __device__ void FlushRows(PlainShared& S, int ns, int& cnt, float& limit, const 
                 const uint8_t* plane = rowbuf + r * rstride;
                 const uint8_t* field = plane + B;
                 float acc = 0.0F, acc2 = 0.0F;
-                if constexpr (!GEN) {   // bits 2..5: the field fits 32 bits
+                if constexpr (!GEN) {   // bits 2..5: the field fits 32 bits, the levels sit in LDS
                     // The block's field lies in the 8 bytes from its 4-byte
-                    // aligned start (the field area of a row is 8-aligned).
-                    const uint32_t fmask = c >= 4 ? ~0u : (1u << (8 * c)) - 1u;
+                    // aligned start (the field area of a row is 8-aligned);
+                    // the 32 bits from its first byte hold it.  Two blocks per
+                    // iteration: the even one into acc, the odd one into acc2.
                     const uint32_t cm = (1u << c) - 1u;
                     uint32_t pm = 0;                                // LSB positions of digits 0..6
                     for (int q = 0; q < 7; ++q) pm |= 1u << (c * q);
-                    for (int blk = 0; blk < B; ++blk) {
-                        const uint32_t bo = static_cast<uint32_t>(blk * c);
-                        const uint32_t* wp = reinterpret_cast<const uint32_t*>(field + (bo & ~3u));
-                        const uint64_t win = (static_cast<uint64_t>(wp[1]) << 32) | wp[0];
-                        const uint32_t f = static_cast<uint32_t>(win >> (8 * (bo & 3u))) & fmask;
-                        const uint32_t rbv = plane[blk];
-                        const uint32_t f7 = (f >> (7 * c)) & cm;
-                        const uint32_t br = f7 & 1u;
-                        const uint32_t lo = i < 7 ? ((f >> (c * i)) & cm) : ((f7 & ~1u) | (__popc(f & pm) & 1u));
-                        const uint32_t digit = lo | (((rbv >> (7 - i)) & 1u) << c);
-                        const uint32_t ti = br * ix.tab_stride + digit;
-                        const float lv = tab_lds ? S.tab[ti] : G(ix.refine_tab)[ti];
-                        if (blk & 1) acc2 = fmaf(lv, qv[blk * 8 + i], acc2);
-                        else acc = fmaf(lv, qv[blk * 8 + i], acc);
+                    const uint32_t sh = static_cast<uint32_t>(c * i), rsh = static_cast<uint32_t>(7 - i);
+                    // all ones in the lanes of component 7 (its digit is the parity form)
+                    uint32_t lastm = i == 7 ? ~0u : 0u;
+                    asm volatile("" : "+v"(lastm));
+                    uint32_t bo = 0;
+                    for (int blk = 0; blk < B; blk += 2) {
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            const uint32_t* wp = reinterpret_cast<const uint32_t*>(field + (bo & ~3u));
+                            const uint32_t f = __builtin_amdgcn_alignbit(wp[1], wp[0], 8 * (bo & 3u));
+                            const uint32_t f7 = (f >> (7 * c)) & cm;
+                            const uint32_t par = static_cast<uint32_t>(__popc(f & pm)) & 1u;
+                            const uint32_t lo = (((f7 & ~1u) | par) & lastm) | (((f >> sh) & cm) & ~lastm);
+                            const uint32_t rbv = plane[blk + u];
+                            const uint32_t digit = lo | (((rbv >> rsh) & 1u) << c);
+                            const float lv = S.tab[(f7 & 1u) * tstride + digit];
+                            const float qq = qv[(blk + u) * 8 + i];
+                            if (u == 0) acc = fmaf(lv, qq, acc);
+                            else acc2 = fmaf(lv, qq, acc2);
+                            bo += static_cast<uint32_t>(c);
+                        }
                     }
                 } else {
                     for (int blk = 0; blk < B; ++blk) {
                         uint32_t digit;
                         int br;
                         DecodeDigit(ix.bits, plane[blk], field + blk * c, i, &digit, &br);
-                        const uint32_t ti = static_cast<uint32_t>(br) * ix.tab_stride + digit;
-                        const float lv = tab_lds ? S.tab[ti] : G(ix.refine_tab)[ti];
+                        const float lv = tab_lds ? S.tab[static_cast<uint32_t>(br) * tstride + digit]
+                                                 : G(ix.refine_tab)[static_cast<uint32_t>(br) * ix.tab_stride + digit];
                         if (blk & 1) acc2 = fmaf(lv, qv[blk * 8 + i], acc2);
                         else acc = fmaf(lv, qv[blk * 8 + i], acc);
                     }
