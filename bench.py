@@ -1114,6 +1114,10 @@ def main():
         opp = next((r for r in prow if r["recall"] >= args.target), prow[-1])
         rop = max(prow, key=lambda r: r["nprobe"])
     achieved = rop["scan_achieved_gbs"]
+    # HBM bytes per launch of the plain kernel at that nprobe, from the
+    # committed rocprofv3 summary of this workload (copy-calibrated
+    # FETCH_SIZE + WRITE_SIZE), when one covers it
+    traffic = profile_traffic("PlainSearchKernel", f"e8q_{args.n // 1_000_000}Mx{args.dim}", rop["nprobe"])
     line = {
         "metric": "QPS at recall@10>=0.95 (plain vs batched/served, 1 GPU); p50/p99 latency",
         "value": round(value, 1) if value == value else None,
@@ -1144,7 +1148,8 @@ def main():
         "build_seconds": round(build_s, 2),
         "roofline": {"bound": "hbm", "kernel": "PlainSearchKernel", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "basis": f"plain form at nprobe={rop['nprobe']}, per launch: {rop['batches']} scan records x "
                               f"{batch_bytes} B + {rop['survivors']} survivors x {survivor_bytes} B, / mean kernel "
                               f"time {rop['scan_kernel_ms']:.3f} ms",
