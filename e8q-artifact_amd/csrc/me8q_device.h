@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "e8q_device.h"
 
@@ -152,6 +153,39 @@ template <int T, class BP = const uint8_t*>
 __device__ __forceinline__ float M8HalfChainT(BP pb, BP fr, const float* __restrict__ lev, const float* __restrict__ qb,
                                               uint32_t nblk, uint32_t half, uint32_t ci) {
     constexpr uint32_t rate = T + 1;
+    typedef const __attribute__((address_space(3))) uint8_t* LdsB;
+    if constexpr (T >= 1 && T <= 4 && std::is_same_v<BP, LdsB>) {
+        // Fields in LDS at a 4-byte aligned start, T <= 4: a block's 8T field
+        // bits fit 32, so one funnel-shifted window of two LDS words holds
+        // all this lane needs of a block -- its digit's T bits (component
+        // ci < 7: the window starts at them), or, for component 7, the
+        // block's whole field (the parity form and f7), and the branch bit.
+        typedef const __attribute__((address_space(3))) uint32_t* LdsW;
+        const LdsW fw = reinterpret_cast<LdsW>(fr);
+        const uint32_t boff = ci < 7 ? T * ci : 0u;   // the window's start in the block's field
+        const uint32_t sbr = 8 * T - 1 - boff;       // the branch bit in the window
+        uint32_t pm = 0;                             // LSBs of digits 0..6
+#pragma unroll
+        for (uint32_t jj = 0; jj < 7; ++jj) pm |= 1u << (jj * T);
+        // all ones in the lanes of component 7
+        uint32_t lastm = ci == 7 ? ~0u : 0u;
+        asm volatile("" : "+v"(lastm));
+        const uint32_t rsh = 7 - ci;
+        float acc = 0.0F;
+#pragma unroll 2
+        for (uint32_t blk = half; blk < nblk; blk += 2) {
+            const uint32_t p = blk * 8 * T + boff;
+            const uint32_t di = p >> 5;
+            const uint32_t win = __builtin_amdgcn_alignbit(fw[di + 1], fw[di], p & 31u);
+            const uint32_t f7 = T > 1 ? (win >> (7 * T)) & ((1u << (T - 1)) - 1u) : 0u;
+            const uint32_t par = static_cast<uint32_t>(__popc(win & pm)) & 1u;
+            const uint32_t lo = (((f7 << 1) | par) & lastm) | ((win & ((1u << T) - 1u)) & ~lastm);
+            const uint32_t br = (win >> sbr) & 1u;
+            const uint32_t top = (static_cast<uint32_t>(pb[blk]) >> rsh) & 1u;
+            acc = fmaf(lev[(br << rate) + (lo | (top << T))], qb[blk * 8], acc);
+        }
+        return acc;
+    }
     uint64_t pml = 0;
     uint32_t pmh = 0;
 #pragma unroll
