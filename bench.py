@@ -402,10 +402,14 @@ def me8q_line(metric, workload, data, args, results, plan, bits_per_dim, k, nq, 
         pr = results["plain"]["rows"]
         rop = max(pr, key=lambda r: r["nprobe"])
         achieved = rop["scan_achieved_gbs"]
+        traffic = profile_traffic("M8ScanKernel", workload_tag, rop["nprobe"])
         roof = {"bound": "hbm", "kernel": "M8ScanKernel", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "basis": f"per launch: {rop['batches']} scan records x {rec_bytes} B + {rop['survivors']} "
                          f"survivors x {refine_stride + 4} B, / mean kernel time {rop['scan_kernel_ms']:.3f} ms"}
+        if traffic:
+            roof["traffic_source"] = traffic["source"]
     line = {
         "metric": metric,
         "value": round(value, 1) if value is not None and value == value else None,
@@ -489,7 +493,8 @@ def run_me8q(args):
                      f"ME8Q mixed-rate scan: N={args.n}, D={args.dim} PCA-domain power law, {args.nlist} lists, "
                      f"avg_bits={args.avg_bits}, {nq} queries, k={k}, nprobe {args.grid}",
                      "synthetic power-law spectrum (lambda_i ~ i^-1.2), seeded rotation, PCA domain, generated on the GPU",
-                     args, results, plan, idx.bits_per_dim, k, nq, build_s, lat, cpu, {})
+                     args, results, plan, idx.bits_per_dim, k, nq, build_s, lat, cpu, {},
+                     workload_tag=f"me8q_{args.n // 1_000_000}Mx{args.dim}")
     print(json.dumps(line), flush=True)
 
 

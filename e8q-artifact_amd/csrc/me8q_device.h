@@ -149,9 +149,8 @@ __device__ __forceinline__ uint32_t M8ListOf(const uint64_t* begin, uint64_t nli
 // digit of (block, ci) = its refine field bits (T = rate-1 per digit, digit
 // 7's LSB the parity of digits 0..6's, the branch in the field's top bit)
 // under the plane's top bit; at rate 1 the plane byte is the whole code.
-template <int T, class BP = const uint8_t*>
-__device__ __forceinline__ float M8HalfChainT(BP pb, BP fr, const float* __restrict__ lev, const float* __restrict__ qb,
-                                              uint32_t nblk, uint32_t half, uint32_t ci) {
+template <int T, class BP = const uint8_t*, class LP = const float*, class QP = const float*>
+__device__ __forceinline__ float M8HalfChainT(BP pb, BP fr, LP lev, QP qb, uint32_t nblk, uint32_t half, uint32_t ci) {
     constexpr uint32_t rate = T + 1;
     typedef const __attribute__((address_space(3))) uint8_t* LdsB;
     if constexpr (T >= 1 && T <= 4 && std::is_same_v<BP, LdsB>) {
@@ -172,7 +171,7 @@ __device__ __forceinline__ float M8HalfChainT(BP pb, BP fr, const float* __restr
         asm volatile("" : "+v"(lastm));
         const uint32_t rsh = 7 - ci;
         float acc = 0.0F;
-#pragma unroll 2
+#pragma unroll 4
         for (uint32_t blk = half; blk < nblk; blk += 2) {
             const uint32_t p = blk * 8 * T + boff;
             const uint32_t di = p >> 5;
@@ -233,23 +232,28 @@ __device__ __forceinline__ float M8HalfChainT(BP pb, BP fr, const float* __restr
     return acc;
 }
 
-template <class BP = const uint8_t*>
-__device__ inline float M8HalfChain(uint32_t rate, BP pb, BP fr, const float* lev, const float* qb, uint32_t nblk,
+// The chain at a run-time rate.  Level tables up to kM8LdsLevRate come from
+// LDS (`lev_lds`), wider ones from global memory (`lev_glob`); with every
+// operand's address space known, the chain's LDS reads wait on the LDS
+// counter alone and never on global loads in flight.
+template <class BP, class LL, class LG, class QP>
+__device__ inline float M8HalfChain(uint32_t rate, BP pb, BP fr, LL lev_lds, LG lev_glob, QP qb, uint32_t nblk,
                                     uint32_t half, uint32_t ci) {
+    static_assert(kM8LdsLevRate == 7, "rates 1..7 read the LDS tables");
     switch (rate) {
-        case 1: return M8HalfChainT<0, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 2: return M8HalfChainT<1, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 3: return M8HalfChainT<2, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 4: return M8HalfChainT<3, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 5: return M8HalfChainT<4, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 6: return M8HalfChainT<5, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 7: return M8HalfChainT<6, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 8: return M8HalfChainT<7, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 9: return M8HalfChainT<8, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 10: return M8HalfChainT<9, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 11: return M8HalfChainT<10, BP>(pb, fr, lev, qb, nblk, half, ci);
-        case 12: return M8HalfChainT<11, BP>(pb, fr, lev, qb, nblk, half, ci);
-        default: return M8HalfChainT<12, BP>(pb, fr, lev, qb, nblk, half, ci);
+        case 1: return M8HalfChainT<0>(pb, fr, lev_lds, qb, nblk, half, ci);
+        case 2: return M8HalfChainT<1>(pb, fr, lev_lds, qb, nblk, half, ci);
+        case 3: return M8HalfChainT<2>(pb, fr, lev_lds, qb, nblk, half, ci);
+        case 4: return M8HalfChainT<3>(pb, fr, lev_lds, qb, nblk, half, ci);
+        case 5: return M8HalfChainT<4>(pb, fr, lev_lds, qb, nblk, half, ci);
+        case 6: return M8HalfChainT<5>(pb, fr, lev_lds, qb, nblk, half, ci);
+        case 7: return M8HalfChainT<6>(pb, fr, lev_lds, qb, nblk, half, ci);
+        case 8: return M8HalfChainT<7>(pb, fr, lev_glob, qb, nblk, half, ci);
+        case 9: return M8HalfChainT<8>(pb, fr, lev_glob, qb, nblk, half, ci);
+        case 10: return M8HalfChainT<9>(pb, fr, lev_glob, qb, nblk, half, ci);
+        case 11: return M8HalfChainT<10>(pb, fr, lev_glob, qb, nblk, half, ci);
+        case 12: return M8HalfChainT<11>(pb, fr, lev_glob, qb, nblk, half, ci);
+        default: return M8HalfChainT<12>(pb, fr, lev_glob, qb, nblk, half, ci);
     }
 }
 

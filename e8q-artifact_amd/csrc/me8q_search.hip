@@ -30,6 +30,7 @@
 namespace e8qdev {
 
 constexpr int kM8PrepThreads = 256;
+constexpr uint32_t kM8PassWords = 4;   // 8-byte words per lane of a scoring pass held in registers
 
 __global__ __launch_bounds__(kM8PrepThreads) void M8QueryPrepKernel(M8View v, const float* __restrict__ queries,
                                                                     uint64_t nq, float* __restrict__ rotq,
@@ -156,10 +157,21 @@ __global__ __launch_bounds__(256) void M8RunDistKernel(M8View v, const float* __
 }
 
 // ---------------------------------------------------------------------------
+// A run's scoring constants, copied to LDS once per workgroup: the scoring
+// passes read them per lane, for a different run in each 16-lane group.
+struct M8RunTab {
+    uint32_t rate, nblk, blk0, ref_off;
+    uint32_t dim_off, lo_off;   // first dim; its level table's offset in the LDS tables
+    uint32_t nwp, nw;           // 8-byte words of its plane bytes, and of plane bytes + fields
+    const float* lev;           // its level table in global memory
+    uint64_t pad;
+};
+static_assert(sizeof(M8RunTab) == 48, "LDS run table entry");
+
 // Scan: LDS layout of one wave (one query).
 struct M8ScanLayout {
     uint32_t q, lut, td, ti, acc, fdc, fpool, fqa, fsc, ftr, tsc, ttr, ssc, str, sord, smean, rq, rqf, ssq,
-        stage, srend, slev, sS32, plist, runbuf, hdr, idl, total;
+        stage, srend, slev, sS32, plist, runbuf, hdr, idl, rtab, total;
     __host__ __device__ M8ScanLayout(uint32_t P, uint32_t PC, uint32_t R, uint32_t K, uint32_t RS, uint32_t LV,
                                      uint32_t SB) {
         uint32_t o = 0;
@@ -197,6 +209,7 @@ struct M8ScanLayout {
         runbuf = take(4 * SB, 16);      // per scored (lane, run) pair of a pass: its plane bytes, then its fields
         hdr = take(32 * R1 * 16, 16);   // a batch's passing lanes' refine row headers (R float4)
         idl = take(32 * 4, 16);         // ... and their ids
+        rtab = take(R1 * sizeof(M8RunTab), 16);   // the runs' scoring constants
         total = (o + 15) & ~15u;
     }
 };
@@ -292,6 +305,25 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
             for (uint32_t e = threadIdx.x; e < nq2; e += 64) slev[o + e] = v.runs[r].lev[e];
             o += nq2;
         }
+    }
+    M8RunTab* rtab = reinterpret_cast<M8RunTab*>(smem + L.rtab);
+    for (uint32_t r = threadIdx.x; r < R; r += 64) {
+        const M8Run& ru = v.runs[r];
+        uint32_t lo = 0;
+        for (uint32_t r2 = 0; r2 < r; ++r2)
+            if (v.runs[r2].rate <= kM8LdsLevRate) lo += 2u << v.runs[r2].rate;
+        M8RunTab t;
+        t.rate = ru.rate;
+        t.nblk = ru.nblk;
+        t.blk0 = ru.blk0;
+        t.ref_off = ru.ref_off;
+        t.dim_off = ru.dim_off;
+        t.lo_off = lo;
+        t.nwp = ru.nblk / 8;
+        t.nw = t.nwp + ru.nblk * (ru.rate - 1) / 8;
+        t.lev = ru.lev;
+        t.pad = 0;
+        rtab[r] = t;
     }
     const int lane = threadIdx.x;
     const float finf = __builtin_inff();
@@ -502,41 +534,76 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                         const uint32_t np0 = __popc(p0);
                         const uint32_t npairs = R * np0;
                         const uint32_t half = (lane >> 3) & 1u, ci = lane & 7u;
+                        const uint32_t g = static_cast<uint32_t>(lane) >> 4, wl = lane & 15u;
+                        typedef const __attribute__((address_space(3))) uint8_t* LdsBytes;
+                        typedef const __attribute__((address_space(3))) float* LdsFloats;
+                        typedef const __attribute__((address_space(3))) M8RunTab* LdsRun;
+                        const LdsRun rt = (LdsRun)(rtab);
+                        uint8_t* slot = runbuf + g * a.run_slot;
                         // Per pass four (lane, run) pairs, 16 lanes each: the pair's plane
-                        // bytes and fields are copied to its LDS slot in one round of
-                        // 8-byte loads (runs start 8-byte aligned in both rows), then its
-                        // two FMA chains read them there.
+                        // bytes and fields go to its LDS slot (8-byte words; runs start
+                        // 8-byte aligned in both rows), then its two FMA chains read them
+                        // there.  Slots of at most 16 x kM8PassWords words: each lane holds
+                        // its words of the next pass in registers, loaded before this
+                        // pass's chains run, so their latency hides behind them.
+                        const bool pf = a.run_slot <= 16 * 8 * kM8PassWords;
+                        auto pair_of = [&](uint32_t t0, uint32_t& r, uint32_t& lj) -> bool {
+                            const uint32_t pair = t0 + g;
+                            if (pair >= npairs) return false;
+                            r = pair / np0;
+                            lj = plist[pair - r * np0];
+                            return true;
+                        };
+                        auto words_of = [&](uint32_t r, uint32_t lj, const uint64_t*& sp, const uint64_t*& sf) {
+                            sp = reinterpret_cast<const uint64_t*>(rec + lj * PC + rt[r].blk0);
+                            sf = reinterpret_cast<const uint64_t*>(v.refine + (lb + bb * 32 + lj) * v.rstride +
+                                                                   rt[r].ref_off);
+                        };
+                        uint64_t pw[kM8PassWords];
+                        auto fetch = [&](uint32_t t0) {
+                            uint32_t r = 0, lj = 0, nwp = 0, nw = 0;
+                            const uint64_t *sp = nullptr, *sf = nullptr;
+                            if (pair_of(t0, r, lj)) {
+                                nwp = rt[r].nwp;
+                                nw = rt[r].nw;
+                                words_of(r, lj, sp, sf);
+                            }
+#pragma unroll
+                            for (uint32_t i2 = 0; i2 < kM8PassWords; ++i2) {
+                                const uint32_t w = wl + 16 * i2;
+                                pw[i2] = w < nw ? (w < nwp ? sp[w] : sf[w - nwp]) : 0ull;
+                            }
+                        };
+                        if (pf) fetch(0);
                         for (uint32_t t0 = 0; t0 < npairs; t0 += 4) {
-                            const uint32_t g = static_cast<uint32_t>(lane) >> 4, pair = t0 + g;
-                            uint8_t* slot = runbuf + g * a.run_slot;
                             float acc = 0.0F;
                             uint32_t lj = 0, r = 0;
-                            if (pair < npairs) {
-                                r = pair / np0;
-                                lj = plist[pair - r * np0];
-                                const M8Run& ru = v.runs[r];
-                                const uint32_t nwp = ru.nblk / 8, nwf = ru.nblk * (ru.rate - 1) / 8;
-                                const uint64_t* sp = reinterpret_cast<const uint64_t*>(rec + lj * PC + ru.blk0);
-                                const uint64_t* sf = reinterpret_cast<const uint64_t*>(
-                                    v.refine + (lb + bb * 32 + lj) * v.rstride + ru.ref_off);
-                                for (uint32_t w = lane & 15u; w < nwp + nwf; w += 16)
-                                    reinterpret_cast<uint64_t*>(slot)[w] = w < nwp ? sp[w] : sf[w - nwp];
+                            const bool ok = pair_of(t0, r, lj);
+                            if (ok) {
+                                uint64_t* sw = reinterpret_cast<uint64_t*>(slot);
+                                const uint32_t nw = rt[r].nw;
+                                if (pf) {
+#pragma unroll
+                                    for (uint32_t i2 = 0; i2 < kM8PassWords; ++i2)
+                                        if (wl + 16 * i2 < nw) sw[wl + 16 * i2] = pw[i2];
+                                } else {
+                                    const uint32_t nwp = rt[r].nwp;
+                                    const uint64_t *sp, *sf;
+                                    words_of(r, lj, sp, sf);
+                                    for (uint32_t w = wl; w < nw; w += 16) sw[w] = w < nwp ? sp[w] : sf[w - nwp];
+                                }
                             }
                             __syncthreads();
-                            if (pair < npairs) {
-                                const M8Run& ru = v.runs[r];
-                                uint32_t lo_off = 0;
-                                for (uint32_t r2 = 0; r2 < r; ++r2)
-                                    if (v.runs[r2].rate <= kM8LdsLevRate) lo_off += 2u << v.runs[r2].rate;
-                                const float* lev = ru.rate <= kM8LdsLevRate ? slev + lo_off : ru.lev;
-                                typedef const __attribute__((address_space(3))) uint8_t* LdsBytes;
+                            if (pf && t0 + 4 < npairs) fetch(t0 + 4);
+                            if (ok) {
                                 const LdsBytes pb = (LdsBytes)(slot);
-                                acc = M8HalfChain<LdsBytes>(ru.rate, pb, pb + ru.nblk, lev, q + ru.dim_off + ci, ru.nblk,
-                                                            half, ci);
+                                const uint32_t nblk = rt[r].nblk;
+                                acc = M8HalfChain(rt[r].rate, pb, pb + nblk, (LdsFloats)(slev) + rt[r].lo_off, rt[r].lev,
+                                                  (LdsFloats)(q) + rt[r].dim_off + ci, nblk, half, ci);
                             }
                             const float other = __shfl_xor(acc, 8);
                             const float sc8 = Hsum8Lanes(acc + other);
-                            if (ci == 0 && half == 0 && pair < npairs) sS32[lj * R + r] = sc8;
+                            if (ci == 0 && half == 0 && ok) sS32[lj * R + r] = sc8;
                             __syncthreads();   // the slots are refilled by the next pass
                         }
                     }
