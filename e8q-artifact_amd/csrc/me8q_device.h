@@ -171,7 +171,7 @@ __device__ __forceinline__ float M8HalfChainT(BP pb, BP fr, LP lev, QP qb, uint3
         asm volatile("" : "+v"(lastm));
         const uint32_t rsh = 7 - ci;
         float acc = 0.0F;
-#pragma unroll 4
+#pragma unroll 8
         for (uint32_t blk = half; blk < nblk; blk += 2) {
             const uint32_t p = blk * 8 * T + boff;
             const uint32_t di = p >> 5;

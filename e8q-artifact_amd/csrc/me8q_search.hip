@@ -164,9 +164,10 @@ struct M8RunTab {
     uint32_t dim_off, lo_off;   // first dim; its level table's offset in the LDS tables
     uint32_t nwp, nw;           // 8-byte words of its plane bytes, and of plane bytes + fields
     const float* lev;           // its level table in global memory
-    uint64_t pad;
+    uint64_t bits;              // rate x len: the bits a cascade step scores
 };
 static_assert(sizeof(M8RunTab) == 48, "LDS run table entry");
+typedef const __attribute__((address_space(3))) M8RunTab* LdsRun;
 
 // Scan: LDS layout of one wave (one query).
 struct M8ScanLayout {
@@ -322,9 +323,10 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
         t.nwp = ru.nblk / 8;
         t.nw = t.nwp + ru.nblk * (ru.rate - 1) / 8;
         t.lev = ru.lev;
-        t.pad = 0;
+        t.bits = static_cast<uint64_t>(ru.rate) * ru.len;
         rtab[r] = t;
     }
+    const LdsRun rtl = (LdsRun)(rtab);
     const int lane = threadIdx.x;
     const float finf = __builtin_inff();
     const double dinf = __builtin_inf();
@@ -537,8 +539,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                         const uint32_t g = static_cast<uint32_t>(lane) >> 4, wl = lane & 15u;
                         typedef const __attribute__((address_space(3))) uint8_t* LdsBytes;
                         typedef const __attribute__((address_space(3))) float* LdsFloats;
-                        typedef const __attribute__((address_space(3))) M8RunTab* LdsRun;
-                        const LdsRun rt = (LdsRun)(rtab);
+                        const LdsRun rt = rtl;
                         uint8_t* slot = runbuf + g * a.run_slot;
                         // Per pass four (lane, run) pairs, 16 lanes each: the pair's plane
                         // bytes and fields go to its LDS slot (8-byte words; runs start
@@ -643,16 +644,33 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                             const uint32_t n = nst;
                             n_staged += n;
                             ++n_flush;
-                            if (lane == 0) {
-                                for (uint32_t r = 0; r < R; ++r) sord[r] = r;
-                                if (R > 1) {
-                                    for (uint32_t r = 0; r < R; ++r) smean[r] = 0.0;
-                                    for (uint32_t j = 0; j < n; ++j)
-                                        for (uint32_t r = 0; r < R; ++r)
-                                            smean[r] = fma(str[j * R + r], str[j * R + r], smean[r]);
-                                    for (uint32_t r = 0; r < R; ++r)
-                                        smean[r] /= static_cast<double>(v.runs[r].nblk > 1 ? v.runs[r].nblk : 1);
-                                    // insertion sort, descending, stable (GCC's std::sort below 17 elements)
+                            // The cascade's run order: mean t^2 / nblk of each run over the
+                            // group (lane r sums run r's in candidate order), then an
+                            // insertion sort, descending, stable (GCC's std::sort below 17
+                            // elements).  With no NaN mean that order is each run's rank:
+                            // the runs with a larger mean, or an equal one and a lower index.
+                            if (R > 1) {
+                                double m = 0.0;
+                                const bool mr = static_cast<uint32_t>(lane) < R;
+                                if (mr) {
+                                    for (uint32_t j = 0; j < n; ++j) m = fma(str[j * R + lane], str[j * R + lane], m);
+                                    const uint32_t nbk = rtl[lane].nblk;
+                                    m /= static_cast<double>(nbk > 1 ? nbk : 1);
+                                    smean[lane] = m;
+                                }
+                                const bool nan_any = __ballot(mr && m != m) != 0;
+                                __syncthreads();
+                                if (!nan_any) {
+                                    if (mr) {
+                                        uint32_t rank = 0;
+                                        for (uint32_t r2 = 0; r2 < R; ++r2) {
+                                            const double m2 = smean[r2];
+                                            rank += (m2 > m || (m2 == m && r2 < static_cast<uint32_t>(lane))) ? 1u : 0u;
+                                        }
+                                        sord[rank] = lane;
+                                    }
+                                } else if (lane == 0) {
+                                    for (uint32_t r = 0; r < R; ++r) sord[r] = r;
                                     for (uint32_t i = 1; i < R; ++i) {
                                         const uint32_t x = sord[i];
                                         uint32_t j = i;
@@ -663,6 +681,8 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                                         sord[j] = x;
                                     }
                                 }
+                            } else if (lane == 0) {
+                                sord[0] = 0;
                             }
                             __syncthreads();
                             const double distk_c = cnt < K ? dinf : static_cast<double>(td[K - 1]);
@@ -697,7 +717,7 @@ __global__ __launch_bounds__(64) void M8ScanKernel(M8ScanArgs a) {
                                     }
                                     const double t = str[j * R + r];
                                     rem2 = fma(-t, t, rem2);
-                                    bs += static_cast<uint64_t>(v.runs[r].rate) * v.runs[r].len;
+                                    bs += rtl[r].bits;
                                     if (fma(-dz, sqrt(fmax(rem2, 0.0)), dist) - s.qadd >= distk_c) live = false;
                                 }
                                 s.dist = dist;
