@@ -185,6 +185,52 @@ __device__ __forceinline__ float M8HalfChainT(BP pb, BP fr, LP lev, QP qb, uint3
         }
         return acc;
     }
+    if constexpr (T >= 5 && T <= 12 && std::is_same_v<BP, LdsB>) {
+        // Fields in LDS at a 4-byte aligned start, 5 <= T <= 12: a block's 8T
+        // field bits fit 96, three funnel-shifted words from the block's
+        // start (f0, f1, f2).  Lane ci < 7 takes its digit at T ci, lane 7
+        // the T-1 bits of f7 at 7T (the same position formula) and the parity
+        // of digits 0..6's LSBs; every lane the branch at 8T-1.  The words
+        // past the last block's field that the window may read lie in the
+        // next slot or LDS region; their bits are never used.
+        typedef const __attribute__((address_space(3))) uint32_t* LdsW;
+        const LdsW fw = reinterpret_cast<LdsW>(fr);
+        constexpr uint32_t bpos = 8 * T - 1, bk = bpos >> 5, bsh = bpos & 31;
+        uint32_t pm0 = 0, pm1 = 0, pm2 = 0;   // LSBs of digits 0..6 in f0, f1, f2
+#pragma unroll
+        for (uint32_t jj = 0; jj < 7; ++jj) {
+            const uint32_t b = jj * T;
+            if (b < 32) pm0 |= 1u << b;
+            else if (b < 64) pm1 |= 1u << (b - 32);
+            else pm2 |= 1u << (b - 64);
+        }
+        const uint32_t dpos = T * ci, dk = dpos >> 5, dsh = dpos & 31;
+        const uint32_t dmask = ci < 7 ? (1u << T) - 1u : (1u << (T - 1)) - 1u;
+        uint32_t lastm = ci == 7 ? ~0u : 0u;   // all ones in the lanes of component 7
+        asm volatile("" : "+v"(lastm));
+        const uint32_t rsh = 7 - ci;
+        float acc = 0.0F;
+#pragma unroll 4
+        for (uint32_t blk = half; blk < nblk; blk += 2) {
+            const uint32_t p = blk * 8 * T;
+            const uint32_t di = p >> 5, sh = p & 31u;
+            const uint32_t w0 = fw[di], w1 = fw[di + 1], w2 = fw[di + 2], w3 = fw[di + 3];
+            const uint32_t f0 = __builtin_amdgcn_alignbit(w1, w0, sh);
+            const uint32_t f1 = __builtin_amdgcn_alignbit(w2, w1, sh);
+            const uint32_t f2 = __builtin_amdgcn_alignbit(w3, w2, sh);
+            const uint32_t fa = dk == 0 ? f0 : dk == 1 ? f1 : f2;
+            const uint32_t fb = dk == 0 ? f1 : dk == 1 ? f2 : 0u;
+            const uint32_t d = __builtin_amdgcn_alignbit(fb, fa, dsh) & dmask;
+            const uint32_t par =
+                static_cast<uint32_t>(__popc(f0 & pm0) + __popc(f1 & pm1) + __popc(f2 & pm2)) & 1u;
+            const uint32_t lo = (((d << 1) | par) & lastm) | (d & ~lastm);
+            const uint32_t fbk = bk == 0 ? f0 : bk == 1 ? f1 : f2;
+            const uint32_t br = (fbk >> bsh) & 1u;
+            const uint32_t top = (static_cast<uint32_t>(pb[blk]) >> rsh) & 1u;
+            acc = fmaf(lev[(br << rate) + (lo | (top << T))], qb[blk * 8], acc);
+        }
+        return acc;
+    }
     uint64_t pml = 0;
     uint32_t pmh = 0;
 #pragma unroll
