@@ -91,6 +91,7 @@ struct M8Bufs {
     uint8_t* lut;
     uint32_t *probe, *npq, *oid;
     uint64_t* qst;
+    uint32_t *qord, *qkey, *qhist;   // the plain scan's query order and its scratch
     // batched form
     uint32_t *fid, *ccount, *cfill, *coff, *ioff, *counters, *ncand, *nfill, *ncont, *toff, *toff2, *chk, *hist;
     uint32_t *abin, *na, *fill_a, *fill_b;
@@ -235,6 +236,9 @@ void M8Carve(e8q_ctx::Carve& cv, M8Bufs& b, const M8View& v, uint64_t nq, uint64
         cv.add(&b.od, nq * k);
     }
     cv.add(&b.qst, nq * 8);
+    cv.add(&b.qord, nq);
+    cv.add(&b.qkey, nq);
+    cv.add(&b.qhist, 2 * kM8OrderBuckets);
 }
 
 // Graph-replay key words of an M8View.
@@ -500,8 +504,12 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         Check(LaunchRouteSelect(st, ra, npq, nprobe_all, maxp, probe, pd), "route");
         Check(LaunchM8RunDist(st, v, rotq, rc, probe, npq, nprobe_all, maxp, nq, sqd2), "run distances");
         if (!batched) {
+            Check(LaunchM8QueryOrder(st, v, probe, npq, nprobe_all, maxp, nq, b.qkey, b.qhist, b.qord), "query order");
+            M8ScanArgs so = sa;   // the queries heaviest first
+            so.qsel = b.qord;
+            so.nsel = nq;
             if (eager) Check(hipEventRecord(ctx->ev[1], st), "event");
-            Check(LaunchM8Scan(st, sa), "scan");
+            Check(LaunchM8Scan(st, so), "scan");
             if (eager) Check(hipEventRecord(ctx->ev[2], st), "event");
             return;
         }
