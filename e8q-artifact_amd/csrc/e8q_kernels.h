@@ -201,6 +201,13 @@ hipError_t LaunchMfmaSelect(hipStream_t st, const MfmaParams* dp, uint64_t nrun,
 
 // Fill a device buffer with a 32-bit value (graph-safe, no memset nodes).
 hipError_t LaunchFill32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
+// The queries by estimated scan work (the sizes begin[c + 1] - begin[c] of
+// their probed lists), heaviest first, into order[nq]; key[nq] and
+// hist[2 * kOrderBuckets] are scratch.
+constexpr uint32_t kOrderBuckets = 132;
+hipError_t LaunchQueryOrder(hipStream_t st, const uint64_t* begin, uint64_t nlist, const uint32_t* probe,
+                            const uint32_t* nprobe_q, uint32_t nprobe_all, uint32_t maxp, uint64_t nq, uint32_t* key,
+                            uint32_t* hist, uint32_t* order);
 
 // Build (encode) path.
 struct EncodeArgs {

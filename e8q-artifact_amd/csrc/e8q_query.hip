@@ -659,4 +659,73 @@ hipError_t LaunchSelectProbes(hipStream_t st, const float* dist, uint64_t nq, ui
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Heaviest-first query order for the plain scans.  One wave scans one query,
+// so a query with a long probed list (a Zipf head list) runs for most of the
+// kernel; started last, it sets the kernel's end.  Each query's work is
+// estimated by the sizes of its probed lists (begin[c + 1] - begin[c], in
+// any unit), bucketed on a log scale (four buckets per octave), and the
+// queries are listed by bucket, heaviest first, so the longest ones start
+// with the grid.  Each query is searched alone, so the order changes no
+// result.
+
+__device__ __forceinline__ uint32_t WorkBucket(uint64_t w) {
+    if (w == 0) return 0;
+    const uint32_t e = 63 - static_cast<uint32_t>(__clzll(w));
+    const uint32_t m = static_cast<uint32_t>(e >= 2 ? (w >> (e - 2)) & 3u : (w << (2 - e)) & 3u);
+    return min(1 + 4 * e + m, kOrderBuckets - 1);
+}
+
+__global__ __launch_bounds__(256) void QueryWorkKernel(const uint64_t* __restrict__ begin, uint64_t nlist,
+                                                       const uint32_t* __restrict__ probe,
+                                                       const uint32_t* __restrict__ nprobe_q, uint32_t nprobe_all,
+                                                       uint32_t maxp, uint64_t nq, uint32_t* __restrict__ key,
+                                                       uint32_t* __restrict__ hist) {
+    const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    const uint64_t np = min<uint64_t>(min<uint64_t>(nprobe_q ? nprobe_q[qi] : nprobe_all, maxp), nlist);
+    uint64_t w = 0;
+    for (uint64_t p = 0; p < np; ++p) {
+        const uint32_t c = probe[qi * maxp + p];
+        if (c < nlist) w += begin[c + 1] - begin[c];
+    }
+    const uint32_t b = WorkBucket(w);
+    key[qi] = b;
+    atomicAdd(&hist[b], 1u);
+}
+
+// hist[kOrderBuckets] the bucket sizes, cur[kOrderBuckets] zeroed.
+__global__ __launch_bounds__(256) void QueryOrderKernel(uint64_t nq, const uint32_t* __restrict__ key,
+                                                        const uint32_t* __restrict__ hist, uint32_t* __restrict__ cur,
+                                                        uint32_t* __restrict__ order) {
+    __shared__ uint32_t start[kOrderBuckets];
+    if (threadIdx.x == 0) {   // bucket b starts after every heavier bucket
+        uint32_t s = 0;
+        for (int32_t b = kOrderBuckets - 1; b >= 0; --b) {
+            start[b] = s;
+            s += hist[b];
+        }
+    }
+    __syncthreads();
+    const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    const uint32_t b = key[qi];
+    order[start[b] + atomicAdd(&cur[b], 1u)] = static_cast<uint32_t>(qi);
+}
+
+hipError_t LaunchQueryOrder(hipStream_t st, const uint64_t* begin, uint64_t nlist, const uint32_t* probe,
+                            const uint32_t* nprobe_q, uint32_t nprobe_all, uint32_t maxp, uint64_t nq, uint32_t* key,
+                            uint32_t* hist, uint32_t* order) {
+    if (nq == 0) return hipSuccess;
+    hipError_t e = LaunchFill32(st, hist, 2 * kOrderBuckets, 0);
+    if (e != hipSuccess) return e;
+    const uint32_t grid = static_cast<uint32_t>((nq + 255) / 256);
+    hipLaunchKernelGGL(QueryWorkKernel, dim3(grid), dim3(256), 0, st, begin, nlist, probe, nprobe_q, nprobe_all, maxp,
+                       nq, key, hist);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(QueryOrderKernel, dim3(grid), dim3(256), 0, st, nq, key, hist, hist + kOrderBuckets, order);
+    return hipGetLastError();
+}
+
 }  // namespace e8qdev

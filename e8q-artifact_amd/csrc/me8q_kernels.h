@@ -66,12 +66,7 @@ struct M8ScanArgs {
 };
 size_t M8ScanLds(const M8View& v, uint32_t k, uint32_t lev_floats, uint32_t run_slot);
 hipError_t LaunchM8Scan(hipStream_t st, const M8ScanArgs& a);
-// The queries by estimated scan work, heaviest first, into order[nq]; key[nq]
-// and hist[2 * kM8OrderBuckets] are scratch.
-constexpr uint32_t kM8OrderBuckets = 132;
-hipError_t LaunchM8QueryOrder(hipStream_t st, const M8View& v, const uint32_t* probe, const uint32_t* nprobe_q,
-                              uint32_t nprobe_all, uint32_t maxp, uint64_t nq, uint32_t* key, uint32_t* hist,
-                              uint32_t* order);
+
 
 // ---- batched form (me8q_batch.hip) ----------------------------------------
 // Every query of a call as one (query, list) work list.  A threshold pass

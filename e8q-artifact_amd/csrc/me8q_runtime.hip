@@ -238,7 +238,7 @@ void M8Carve(e8q_ctx::Carve& cv, M8Bufs& b, const M8View& v, uint64_t nq, uint64
     cv.add(&b.qst, nq * 8);
     cv.add(&b.qord, nq);
     cv.add(&b.qkey, nq);
-    cv.add(&b.qhist, 2 * kM8OrderBuckets);
+    cv.add(&b.qhist, 2 * kOrderBuckets);
 }
 
 // Graph-replay key words of an M8View.
@@ -504,7 +504,8 @@ void M8Search(e8q_ctx* ctx, const e8q_m8index* x, const e8q_m8_search_args* a, e
         Check(LaunchRouteSelect(st, ra, npq, nprobe_all, maxp, probe, pd), "route");
         Check(LaunchM8RunDist(st, v, rotq, rc, probe, npq, nprobe_all, maxp, nq, sqd2), "run distances");
         if (!batched) {
-            Check(LaunchM8QueryOrder(st, v, probe, npq, nprobe_all, maxp, nq, b.qkey, b.qhist, b.qord), "query order");
+            Check(LaunchQueryOrder(st, v.batch_begin, v.nlist, probe, npq, nprobe_all, maxp, nq, b.qkey, b.qhist, b.qord),
+                  "query order");
             M8ScanArgs so = sa;   // the queries heaviest first
             so.qsel = b.qord;
             so.nsel = nq;
