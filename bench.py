@@ -938,9 +938,23 @@ def run_me8q_zipf(args):
     ws = int(0.5 * free_b / args.serve_workers)
     srv = e8qlib.Server(idx, workers=args.serve_workers, max_batch=4096, mode="batched", workspace_bytes=ws)
     log(f"server: {args.serve_workers} workers, {ws / 1e9:.1f} GB workspace each")
-    warm = [srv.submit(q, kk, npb) for q, kk, npb in reqs[:20]]
-    for t in warm:
+    # warm-up (untimed): the batched form tunes its grids once per batch
+    # shape (query count, nprobe and k buckets) and the workers share the
+    # table, so the stream is replayed in groups that form batches of about
+    # 64 .. 8192 queries, then all at once
+    pos = 0
+    for target in (64, 128, 256, 512, 1024, 2048, 4096, 8192):
+        group, got = [], 0
+        while got < target:
+            q, kk, npb = reqs[pos % len(reqs)]
+            group.append(srv.submit(q, kk, npb))
+            got += q.shape[0]
+            pos += 1
+        for t in group:
+            srv.wait(t)
+    for t in [srv.submit(q, kk, npb) for q, kk, npb in reqs]:
         srv.wait(t)
+    log(f"server warmed up ({srv.stats()['batches']} batches)")
     # capacity first (every request submitted at once), then Poisson streams
     # offered at fractions of it, each bounded to ~20 s of offered traffic
     sat = serve_run(srv, reqs, None, rng)
